@@ -1,0 +1,390 @@
+"""cellgpu_amd — MI355X-native Voronoi / vertex-model time step.
+
+Python front end of libcellgpu_amd.so. The classes keep the reference's
+(abaso/cellGPU) plugin API names — Simulation, VoronoiQuadraticEnergy,
+selfPropelledParticleDynamics, brownianParticleDynamics, EnergyMinimizerFIRE,
+VertexQuadraticEnergy — on top of device-resident batches (VoronoiBatch,
+VertexBatch). Everything runs on the GPU; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CellGPUError, check, lib, require_device, VoronoiBatchView  # noqa: F401
+
+__all__ = [
+    "CellGPUError", "VoronoiBatch", "VoronoiQuadraticEnergy", "Simulation", "selfPropelledParticleDynamics",
+    "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
+]
+
+RING_CAP = 24
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.shape != (n,):
+        a = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+    return a
+
+
+class VoronoiBatch:
+    """A device-resident batch of independent periodic Voronoi monolayers.
+
+    n: cells per system; L: box sides (default sqrt(n), unit density);
+    seeds: reference mt19937 seeds of the initial condition (used when x, y are
+    None); precision: "fp64" (plain) or "fp32" (FP32 bulk, FP64 predicates).
+    """
+
+    def __init__(self, n, L=None, seeds=None, x=None, y=None, theta=None, precision="fp32", device=0):
+        require_device()
+        n = np.atleast_1d(np.asarray(n, dtype=np.int32))
+        self.S = len(n)
+        self.n = n
+        self.offsets = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+        self.N = int(self.offsets[-1])
+        Ls = None if L is None else _f64(np.broadcast_to(np.asarray(L, np.float64), (self.S,)))
+        self.L = np.sqrt(n.astype(np.float64)) if Ls is None else Ls
+        sd = None if seeds is None else np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, np.uint64), (self.S,)))
+        xs = None if x is None else _f64(x)
+        ys = None if y is None else _f64(y)
+        ts = None if theta is None else _f64(theta)
+        self.precision = precision
+        prec = 1 if precision == "fp32" else 0
+        self._keep = (n, Ls, sd, xs, ys, ts)
+        self.h = lib().cgpu_vor_create(self.S, _p(n), _p(Ls), _p(sd), _p(xs), _p(ys), _p(ts), prec, device)
+        if not self.h:
+            raise CellGPUError("cgpu_vor_create failed: " + lib().cgpu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cgpu_vor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---------------------------------------------------------------- setters
+    def set_params(self, KA=1.0, KP=1.0, dt=0.01, mu=1.0, T=0.0, system=-1):
+        check(lib().cgpu_vor_set_params(self.h, system, KA, KP, dt, mu, T), "set_params")
+
+    def set_prefs(self, A0, P0):
+        a, p = _f64(A0, self.N), _f64(P0, self.N)
+        check(lib().cgpu_vor_set_prefs(self.h, _p(a), _p(p)), "set_prefs")
+
+    def set_motility(self, v0, Dr):
+        a, b = _f64(v0, self.N), _f64(Dr, self.N)
+        check(lib().cgpu_vor_set_motility(self.h, _p(a), _p(b)), "set_motility")
+
+    def set_exclusions(self, ex):
+        e = np.ascontiguousarray(ex, dtype=np.int32)
+        check(lib().cgpu_vor_set_exclusions(self.h, _p(e)), "set_exclusions")
+
+    def set_positions(self, x, y, retriangulate=True):
+        a, b = _f64(x, self.N), _f64(y, self.N)
+        check(lib().cgpu_vor_set_positions(self.h, _p(a), _p(b), 1 if retriangulate else 0), "set_positions")
+
+    def set_noise_buffer(self, on=True):
+        check(lib().cgpu_vor_set_noise_mode(self.h, 1 if on else 0), "set_noise_mode")
+
+    def upload_noise(self, eta, eta2=None):
+        buf = np.zeros(2 * self.N)
+        buf[: self.N] = eta
+        if eta2 is not None:
+            buf[self.N:] = eta2
+        check(lib().cgpu_vor_upload_noise(self.h, _p(buf)), "upload_noise")
+
+    def set_active(self, active=None, steps_left=None):
+        a = None if active is None else np.ascontiguousarray(active, np.int32)
+        s = None if steps_left is None else np.ascontiguousarray(steps_left, np.int32)
+        check(lib().cgpu_vor_set_active(self.h, _p(a), _p(s)), "set_active")
+
+    # ---------------------------------------------------------------- actions
+    def compute_forces(self):
+        check(lib().cgpu_vor_compute_forces(self.h), "compute_forces")
+
+    def spv_steps(self, steps, use_graph=True):
+        check(lib().cgpu_vor_spv_steps(self.h, int(steps), 1 if use_graph else 0), "spv_steps")
+
+    def brownian_steps(self, steps):
+        check(lib().cgpu_vor_brownian_steps(self.h, int(steps)), "brownian_steps")
+
+    def sync(self):
+        check(lib().cgpu_vor_sync(self.h), "sync")
+
+    def status(self):
+        return lib().cgpu_vor_status(self.h)
+
+    def check_status(self):
+        check(self.status(), "batch status")
+
+    # ---------------------------------------------------------------- getters
+    def get(self):
+        N = self.N
+        a = {k: np.zeros(N) for k in ("x", "y", "theta", "vx", "vy", "Fx", "Fy", "A", "P")}
+        check(lib().cgpu_vor_get(self.h, *[_p(a[k]) for k in ("x", "y", "theta", "vx", "vy", "Fx", "Fy", "A", "P")]),
+              "get")
+        return a
+
+    def topology(self, cap=RING_CAP):
+        nn = np.zeros(self.N, np.int32)
+        nbr = np.full(self.N * cap, -1, np.int32)
+        nm = np.zeros(self.S, np.int32)
+        check(lib().cgpu_vor_get_topology(self.h, _p(nn), _p(nbr), cap, _p(nm)), "get_topology")
+        return nn, nbr.reshape(self.N, cap), nm
+
+    def counters(self, s=0):
+        out = np.zeros(8, np.int64)
+        check(lib().cgpu_vor_get_counters(self.h, s, _p(out)), "get_counters")
+        keys = ["global_fixes", "local_repairs", "repair_events", "step", "last_mode", "last_nf", "neigh_max", "error"]
+        return dict(zip(keys, out.tolist()))
+
+    def energy(self, s=0):
+        return lib().cgpu_vor_energy(self.h, s)
+
+    def view(self):
+        v = VoronoiBatchView()
+        lib().cgpu_vor_view(self.h, C.byref(v))
+        return v
+
+    def stream(self):
+        return lib().cgpu_vor_stream(self.h)
+
+    def system_slice(self, s):
+        return slice(int(self.offsets[s]), int(self.offsets[s + 1]))
+
+
+# ----------------------------------------------------------------------------
+# Reference-named single-system API (abaso/cellGPU inc/models, inc/updaters,
+# inc/Simulation.h). A model owns a one-system VoronoiBatch.
+# ----------------------------------------------------------------------------
+class VoronoiQuadraticEnergy:
+    """reference inc/models/voronoiQuadraticEnergy.h:19-64 (GPU branch only)."""
+
+    def __init__(self, n, A0=1.0, P0=3.8, reprod=True, precision="fp64", x=None, y=None, theta=None, seed=13377,
+                 device=0):
+        self.Ncells = int(n)
+        self.batch = VoronoiBatch([n], seeds=[seed if reprod else np.random.randint(1, 2**31)], x=x, y=y,
+                                  theta=theta, precision=precision, device=device)
+        self.KA = self.KP = 1.0
+        self.deltaT = 0.01
+        self.setCellPreferencesUniform(A0, P0)
+        self.setv0Dr(0.05, 1.0)
+
+    def setCellPreferencesUniform(self, A0, P0):
+        self.batch.set_prefs(np.full(self.Ncells, A0), np.full(self.Ncells, P0))
+
+    def setCellPreferences(self, AP):
+        AP = np.asarray(AP, np.float64)
+        self.batch.set_prefs(AP[:, 0], AP[:, 1])
+
+    def setv0Dr(self, v0, Dr):
+        self.batch.set_motility(np.full(self.Ncells, v0), np.full(self.Ncells, Dr))
+
+    def setModuliUniform(self, KA, KP):
+        self.KA, self.KP = KA, KP
+        self.batch.set_params(KA=KA, KP=KP, dt=self.deltaT)
+
+    def setDeltaT(self, dt):
+        self.deltaT = dt
+        self.batch.set_params(KA=self.KA, KP=self.KP, dt=dt)
+
+    def setExclusions(self, ex):
+        self.batch.set_exclusions(ex)
+
+    def setCPU(self, globalOnly=True):
+        raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+
+    def computeForces(self):
+        self.batch.compute_forces()
+
+    def computeEnergy(self):
+        return self.batch.energy(0)
+
+    def returnPositions(self):
+        g = self.batch.get()
+        return np.stack([g["x"], g["y"]], 1)
+
+    def returnForces(self):
+        g = self.batch.get()
+        return np.stack([g["Fx"], g["Fy"]], 1)
+
+    def returnAreaPeri(self):
+        g = self.batch.get()
+        return np.stack([g["A"], g["P"]], 1)
+
+    def reportq(self):
+        g = self.batch.get()
+        return float(np.mean(g["P"] / np.sqrt(g["A"])))
+
+    def reportMeanCellForce(self, verbose=False):
+        g = self.batch.get()
+        return float(np.mean(g["Fx"])), float(np.mean(g["Fy"]))
+
+
+class _Updater:
+    def __init__(self, n=None):
+        self.Period = 1
+        self.Phase = 0
+        self.model = None
+        self.deltaT = 0.01
+
+    def set2DModel(self, m):
+        self.model = m
+
+    def setDeltaT(self, dt):
+        self.deltaT = dt
+
+    def setGPU(self):
+        pass
+
+    def setCPU(self):
+        raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+
+    def setReproducible(self, r):
+        pass
+
+
+class selfPropelledParticleDynamics(_Updater):
+    """reference inc/updaters/selfPropelledParticleDynamics.h:13-44 (GPU semantics)."""
+
+    def performUpdate(self):
+        self.model.batch.spv_steps(1, use_graph=False)
+
+    def advance(self, steps):
+        self.model.batch.spv_steps(steps, use_graph=True)
+
+
+class brownianParticleDynamics(_Updater):
+    """reference inc/updaters/brownianParticleDynamics.h:14-51."""
+
+    def __init__(self, n=None, T=0.0, mu=1.0):
+        super().__init__(n)
+        self.T = T
+        self.mu = mu
+
+    def setT(self, T):
+        self.T = T
+
+    def set2DModel(self, m):
+        self.model = m
+        m.batch.set_params(KA=m.KA, KP=m.KP, dt=m.deltaT, mu=self.mu, T=self.T)
+
+    def performUpdate(self):
+        m = self.model
+        m.batch.set_params(KA=m.KA, KP=m.KP, dt=m.deltaT, mu=self.mu, T=self.T)
+        m.batch.brownian_steps(1)
+
+
+class Simulation:
+    """reference inc/Simulation.h:18-87."""
+
+    def __init__(self):
+        self.updaters = []
+        self.config = None
+        self.integerTimestep = 0
+        self.Time = 0.0
+        self.integrationTimestep = 0.01
+
+    def setConfiguration(self, c):
+        self.config = c
+
+    def addUpdater(self, upd, config):
+        upd.set2DModel(config)
+        self.updaters.append(upd)
+
+    def setIntegrationTimestep(self, dt):
+        self.integrationTimestep = dt
+        self.config.setDeltaT(dt)
+        for u in self.updaters:
+            u.setDeltaT(dt)
+
+    def setCPUOperation(self, setcpu):
+        if setcpu:
+            raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+
+    def setReproducible(self, r):
+        for u in self.updaters:
+            u.setReproducible(r)
+
+    def performTimestep(self):
+        self.integerTimestep += 1
+        self.Time += self.integrationTimestep
+        for u in self.updaters:
+            u.performUpdate()
+
+    def performTimesteps(self, k):
+        """k steps at once (graph-captured); same result as k performTimestep calls."""
+        if len(self.updaters) == 1 and isinstance(self.updaters[0], selfPropelledParticleDynamics):
+            self.updaters[0].advance(k)
+            self.integerTimestep += k
+            self.Time += k * self.integrationTimestep
+        else:
+            for _ in range(k):
+                self.performTimestep()
+
+
+class EnergyMinimizerFIRE:
+    """reference inc/updaters/EnergyMinimizerFIRE2D.h:20-132 (GPU branch)."""
+
+    def __init__(self, model):
+        self.model = model
+        batch = model.batch if hasattr(model, "batch") else model
+        self.batch = batch
+        self.h = lib().cgpu_fire_create(batch.h)
+        if not self.h:
+            raise CellGPUError("cgpu_fire_create failed: " + lib().cgpu_last_error().decode())
+        self.params = dict(max_it=1000, force_cutoff=1e-7, alpha_start=0.99, dt=0.01, dt_max=0.1, dt_inc=1.05,
+                           dt_dec=0.95, alpha_dec=0.9, nmin=5)
+        self._push()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cgpu_fire_destroy(self.h)
+            self.h = None
+
+    def _push(self):
+        p = self.params
+        check(lib().cgpu_fire_set(self.h, p["max_it"], p["force_cutoff"], p["alpha_start"], p["dt"], p["dt_max"],
+                                  p["dt_inc"], p["dt_dec"], p["alpha_dec"], p["nmin"]), "fire_set")
+
+    def setMaximumIterations(self, m):
+        self.params["max_it"] = int(m); self._push()
+
+    def setForceCutoff(self, f):
+        self.params["force_cutoff"] = f; self._push()
+
+    def setAlphaStart(self, a):
+        self.params["alpha_start"] = a; self._push()
+
+    def setDeltaT(self, dt):
+        self.params["dt"] = dt; self._push()
+
+    def setDeltaTMax(self, t):
+        self.params["dt_max"] = t; self._push()
+
+    def setDeltaTInc(self, t):
+        self.params["dt_inc"] = t; self._push()
+
+    def setDeltaTDec(self, t):
+        self.params["dt_dec"] = t; self._push()
+
+    def setAlphaDec(self, a):
+        self.params["alpha_dec"] = a; self._push()
+
+    def setNMin(self, n):
+        self.params["nmin"] = int(n); self._push()
+
+    def minimize(self):
+        check(lib().cgpu_fire_minimize(self.h), "fire_minimize")
+
+    def state(self, s=0):
+        out = np.zeros(6)
+        check(lib().cgpu_fire_get(self.h, s, _p(out)), "fire_get")
+        return dict(force_max=out[0], power=out[1], alpha=out[2], dt=out[3], max_f2=out[4], iterations=int(out[5]))
+
+    def getMaxForce(self, s=0):
+        return self.state(s)["force_max"]

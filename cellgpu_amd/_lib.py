@@ -1,0 +1,120 @@
+"""Loader of the native library libcellgpu_amd.so (HIP kernels + C++ host layer).
+
+There is no fallback: if the library is missing or no GPU is visible, calls
+raise. The C ABI is declared in include/cellgpu_amd.h.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcellgpu_amd.so")
+_LIB = None
+
+
+class CellGPUError(RuntimeError):
+    pass
+
+
+STATUS = {
+    0: "ok", -1: "bad arguments", -2: "HIP error", -3: "Delaunay degree exceeds ring capacity",
+    -4: "cell-list bin overflow", -5: "topology inconsistent", -6: "non-finite position", -7: "no device",
+}
+
+
+class SysParams(C.Structure):
+    _fields_ = [("off", C.c_int), ("n", C.c_int), ("bin_off", C.c_int), ("xsize", C.c_int),
+                ("L", C.c_double), ("boxsize", C.c_double), ("KA", C.c_double), ("KP", C.c_double),
+                ("dt", C.c_double), ("mu", C.c_double), ("T", C.c_double), ("seed", C.c_uint64),
+                ("ncells", C.c_int), ("pad0", C.c_int)]
+
+
+class VoronoiBatchView(C.Structure):
+    _fields_ = [("num_systems", C.c_int), ("num_cells", C.c_int), ("total_bins", C.c_int), ("precision", C.c_int)] + [
+        (name, C.c_void_p) for name in (
+            "sys", "st", "cell_sys", "pos", "pos_next", "nn", "nbr", "oth", "AP", "dAP", "APpref", "mot", "vel",
+            "theta", "force", "excl", "ext_force", "bin_count", "bin_idx", "repair", "mark", "flagged", "nfq", "gq",
+            "counters", "noise")]
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise CellGPUError("libcellgpu_amd.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    P = C.POINTER
+    sig = {
+        "cgpu_device_count": (i, []),
+        "cgpu_last_error": (C.c_char_p, []),
+        "cgpu_set_block_size": (i, [i]),
+        "cgpu_vor_create": (vp, [i, vp, vp, vp, vp, vp, vp, i, i]),
+        "cgpu_vor_destroy": (None, [vp]),
+        "cgpu_vor_set_params": (i, [vp, i, d, d, d, d, d]),
+        "cgpu_vor_set_prefs": (i, [vp, vp, vp]),
+        "cgpu_vor_set_motility": (i, [vp, vp, vp]),
+        "cgpu_vor_set_exclusions": (i, [vp, vp]),
+        "cgpu_vor_set_positions": (i, [vp, vp, vp, i]),
+        "cgpu_vor_set_noise_mode": (i, [vp, i]),
+        "cgpu_vor_upload_noise": (i, [vp, vp]),
+        "cgpu_vor_compute_forces": (i, [vp]),
+        "cgpu_vor_spv_steps": (i, [vp, i, i]),
+        "cgpu_vor_brownian_steps": (i, [vp, i]),
+        "cgpu_vor_sync": (i, [vp]),
+        "cgpu_vor_status": (i, [vp]),
+        "cgpu_vor_get": (i, [vp] + [vp] * 9),
+        "cgpu_vor_get_topology": (i, [vp, vp, vp, i, vp]),
+        "cgpu_vor_get_counters": (i, [vp, i, vp]),
+        "cgpu_vor_num_cells": (i, [vp]),
+        "cgpu_vor_energy": (d, [vp, i]),
+        "cgpu_vor_set_active": (i, [vp, vp, vp]),
+        "cgpu_vor_autotune": (i, [vp]),
+        "cgpu_vor_view": (i, [vp, P(VoronoiBatchView)]),
+        "cgpu_vor_stream": (vp, [vp]),
+        "cgpu_compute_cell_list": (i, [P(VoronoiBatchView), i, vp]),
+        "cgpu_test_and_geometry": (i, [P(VoronoiBatchView), i, vp]),
+        "cgpu_repair_topology": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_global_triangulation": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_voronoi_forces": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_spv_force_eom_move": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_spv_step": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_brownian_step": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_brownian_force_eom_move": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_fill_normal": (i, [P(VoronoiBatchView), i, i, vp, vp]),
+    }
+    optional = {
+        "cgpu_fire_create": (vp, [vp]),
+        "cgpu_fire_destroy": (None, [vp]),
+        "cgpu_fire_set": (i, [vp, i, d, d, d, d, d, d, d, i]),
+        "cgpu_fire_minimize": (i, [vp]),
+        "cgpu_fire_get": (i, [vp, i, vp]),
+        "cgpu_vor_kernel_times": (i, [vp, i, vp, vp, i]),
+        "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, vp, vp, vp, i, i]),
+    }
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        try:
+            f = getattr(L, name)
+        except AttributeError:
+            if name in optional:
+                continue
+            raise
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(status, what="call"):
+    if status != 0:
+        msg = STATUS.get(status, "status %d" % status)
+        extra = lib().cgpu_last_error()
+        raise CellGPUError("%s failed: %s %s" % (what, msg, extra.decode() if extra else ""))
+    return status
+
+
+def require_device():
+    n = lib().cgpu_device_count()
+    if n < 1:
+        raise CellGPUError("no HIP device visible: cellgpu_amd runs on the GPU only")
+    return n

@@ -1,0 +1,577 @@
+// VoronoiBatch: allocation, initial triangulation, step loop and host access of
+// a device-resident batch of Voronoi systems, plus the object-layer C ABI.
+#include "voronoi_batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+
+namespace cgpu_host {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+void reference_random_init(int n, uint32_t seed, double L, double *x, double *y, double *theta) {
+    std::mt19937 gen(seed);
+    for (int i = 0; i < n; ++i) {
+        std::uniform_real_distribution<double> u(0.0, L);
+        x[i] = u(gen);
+        y[i] = u(gen);
+    }
+    for (int i = 0; i < n; ++i) {
+        std::uniform_real_distribution<double> u(0.0, 2.0 * 3.14159265358979323846);
+        theta[i] = u(gen);
+    }
+}
+
+VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const uint64_t *seed, const double *x,
+                           const double *y, const double *theta, int precision, int device)
+    : S_(num_systems), prec_(precision ? 1 : 0), device_(device) {
+    if (S_ <= 0) throw std::invalid_argument("num_systems must be positive");
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    rsz_ = prec_ ? sizeof(float) : sizeof(double);
+    hp_.resize(S_);
+    int off = 0, boff = 0;
+    for (int s = 0; s < S_; ++s) {
+        if (n[s] < 4) throw std::invalid_argument("each system needs at least 4 cells");
+        cgpu_sys_params &p = hp_[s];
+        std::memset(&p, 0, sizeof(p));
+        p.off = off;
+        p.n = n[s];
+        p.ncells = n[s];
+        p.L = L ? L[s] : std::sqrt((double)n[s]);
+        // reference cellListGPU::setGridSize(1.25): even number of bins per side
+        int xs = (int)std::floor(p.L / 1.25);
+        if (xs % 2 == 1) xs += 1;
+        if (xs < 2) xs = 2;
+        p.xsize = xs;
+        p.boxsize = p.L / xs;
+        p.bin_off = boff;
+        p.KA = 1.0;
+        p.KP = 1.0;
+        p.dt = 0.01;
+        p.mu = 1.0;
+        p.T = 0.0;
+        p.seed = seed ? seed[s] : 13377ull;
+        off += n[s];
+        boff += xs * xs;
+    }
+    N_ = off;
+    total_bins_ = boff;
+    cell_sys_h_.resize(N_);
+    for (int s = 0; s < S_; ++s)
+        for (int i = 0; i < hp_[s].n; ++i) cell_sys_h_[hp_[s].off + i] = s;
+
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    sys_.alloc(S_);
+    st_.alloc(S_);
+    cell_sys_.alloc(N_);
+    nn_.alloc(N_);
+    nbr_.alloc((size_t)CGPU_RING_CAP * N_);
+    oth_.alloc((size_t)CGPU_RING_CAP * N_);
+    repair_.alloc(N_);
+    mark_.alloc(N_);
+    flagged_.alloc(N_);
+    nfq_.alloc(N_);
+    gq_.alloc(N_);
+    counters_.alloc(8);
+    bin_count_.alloc(total_bins_);
+    bin_idx_.alloc((size_t)total_bins_ * CGPU_BIN_CAP);
+    pos0_.alloc(2 * (size_t)N_);
+    pos1_.alloc(2 * (size_t)N_);
+    AP_.alloc(2 * N_ * rsz_);
+    dAP_.alloc(2 * N_ * rsz_);
+    APpref_.alloc(2 * N_ * rsz_);
+    mot_.alloc(2 * N_ * rsz_);
+    vel_.alloc(2 * N_ * rsz_);
+    theta_.alloc(N_ * rsz_);
+    force_.alloc(2 * N_ * rsz_);
+    cell_sys_.upload(cell_sys_h_.data(), N_);
+    upload_params();
+    std::vector<cgpu_sys_state> st(S_);
+    std::memset(st.data(), 0, sizeof(cgpu_sys_state) * S_);
+    for (auto &q : st) {
+        q.active = 1;
+        q.steps_left = -1;
+    }
+    st_.upload(st.data(), S_);
+
+    // initial condition
+    std::vector<double> hx(N_), hy(N_), ht(N_);
+    for (int s = 0; s < S_; ++s) {
+        const cgpu_sys_params &p = hp_[s];
+        if (x && y) {
+            for (int i = 0; i < p.n; ++i) {
+                hx[p.off + i] = x[p.off + i];
+                hy[p.off + i] = y[p.off + i];
+                ht[p.off + i] = theta ? theta[p.off + i] : 0.0;
+            }
+        } else {
+            reference_random_init(p.n, (uint32_t)p.seed, p.L, &hx[p.off], &hy[p.off], &ht[p.off]);
+        }
+    }
+    std::vector<double> xy(2 * (size_t)N_);
+    for (int i = 0; i < N_; ++i) {
+        if (!std::isfinite(hx[i]) || !std::isfinite(hy[i]) || !std::isfinite(ht[i]))
+            throw std::invalid_argument("non-finite initial position or director");
+        xy[2 * i] = hx[i];
+        xy[2 * i + 1] = hy[i];
+    }
+    pos0_.upload(xy.data(), 2 * (size_t)N_);
+    real_upload(theta_.p, ht.data(), N_);
+    std::vector<double> a0(N_, 1.0), p0(N_, 3.8), v0(N_, 0.05), dr(N_, 1.0);
+    set_prefs(a0.data(), p0.data());
+    // reference VoronoiQuadraticEnergy(n, A0, P0, reprod) -> setv0Dr(0.05, 1.0)
+    {
+        std::vector<double> mv(2 * (size_t)N_), vv(2 * (size_t)N_);
+        for (int i = 0; i < N_; ++i) {
+            mv[2 * i] = 0.05;
+            mv[2 * i + 1] = 1.0;
+            vv[2 * i] = 0.05 * std::cos(ht[i]);
+            vv[2 * i + 1] = 0.05 * std::sin(ht[i]);
+        }
+        real_upload(mot_.p, mv.data(), 2 * (size_t)N_);
+        real_upload(vel_.p, vv.data(), 2 * (size_t)N_);
+    }
+    int r = global_triangulation();
+    if (r != CGPU_OK) throw std::runtime_error("initial triangulation failed, status " + std::to_string(r));
+}
+
+VoronoiBatch::~VoronoiBatch() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void VoronoiBatch::upload_params() { sys_.upload(hp_.data(), S_); }
+
+void VoronoiBatch::real_upload(void *dst, const double *src, size_t count) {
+    if (prec_) {
+        std::vector<float> f(count);
+        for (size_t i = 0; i < count; ++i) f[i] = (float)src[i];
+        hip_check(hipMemcpy(dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice), "upload");
+    } else {
+        hip_check(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyHostToDevice), "upload");
+    }
+}
+void VoronoiBatch::real_download(const void *src, double *dst, size_t count) {
+    if (prec_) {
+        std::vector<float> f(count);
+        hip_check(hipMemcpy(f.data(), src, count * sizeof(float), hipMemcpyDeviceToHost), "download");
+        for (size_t i = 0; i < count; ++i) dst[i] = f[i];
+    } else {
+        hip_check(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost), "download");
+    }
+}
+
+cgpu_voronoi_batch VoronoiBatch::view() const {
+    cgpu_voronoi_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.num_systems = S_;
+    b.num_cells = N_;
+    b.total_bins = total_bins_;
+    b.precision = prec_;
+    b.sys = sys_.p;
+    b.st = st_.p;
+    b.cell_sys = cell_sys_.p;
+    b.pos = parity_ ? pos1_.p : pos0_.p;
+    b.pos_next = parity_ ? pos0_.p : pos1_.p;
+    b.nn = nn_.p;
+    b.nbr = nbr_.p;
+    b.oth = oth_.p;
+    b.AP = AP_.p;
+    b.dAP = dAP_.p;
+    b.APpref = APpref_.p;
+    b.mot = mot_.p;
+    b.vel = vel_.p;
+    b.theta = theta_.p;
+    b.force = force_.p;
+    b.excl = excl_.p;
+    b.ext_force = ext_.p;
+    b.bin_count = bin_count_.p;
+    b.bin_idx = bin_idx_.p;
+    b.repair = repair_.p;
+    b.mark = mark_.p;
+    b.flagged = flagged_.p;
+    b.nfq = nfq_.p;
+    b.gq = gq_.p;
+    b.counters = counters_.p;
+    b.noise = noise_buffer_ ? noise_.p : nullptr;
+    return b;
+}
+
+void VoronoiBatch::set_params(int s, double KA, double KP, double dt, double mu, double T) {
+    int lo = s < 0 ? 0 : s, hi = s < 0 ? S_ : s + 1;
+    for (int q = lo; q < hi; ++q) {
+        hp_[q].KA = KA;
+        hp_[q].KP = KP;
+        hp_[q].dt = dt;
+        hp_[q].mu = mu;
+        hp_[q].T = T;
+    }
+    upload_params();
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    // dAP depends on KA, KP
+    cgpu_voronoi_batch b = view();
+    cgpu_test_and_geometry(&b, 0, stream_);
+    sync();
+}
+
+void VoronoiBatch::set_prefs(const double *A0, const double *P0) {
+    std::vector<double> v(2 * (size_t)N_);
+    for (int i = 0; i < N_; ++i) {
+        v[2 * i] = A0[i];
+        v[2 * i + 1] = P0[i];
+    }
+    real_upload(APpref_.p, v.data(), v.size());
+    if (nn_.p && hp_.size()) {
+        cgpu_voronoi_batch b = view();
+        // geometry only valid once a triangulation exists (nn > 0)
+        std::vector<int> probe(1, 0);
+        nn_.download(probe.data(), 1);
+        if (probe[0] > 0) {
+            cgpu_test_and_geometry(&b, 0, stream_);
+            sync();
+        }
+    }
+}
+
+// reference Simple2DActiveCell::setv0Dr (.cpp:76-92) resets v = v0 (cos, sin)(theta)
+void VoronoiBatch::set_motility(const double *v0, const double *Dr) {
+    std::vector<double> th(N_);
+    real_download(theta_.p, th.data(), N_);
+    std::vector<double> mv(2 * (size_t)N_), vv(2 * (size_t)N_);
+    for (int i = 0; i < N_; ++i) {
+        mv[2 * i] = v0[i];
+        mv[2 * i + 1] = Dr[i];
+        vv[2 * i] = v0[i] * std::cos(th[i]);
+        vv[2 * i + 1] = v0[i] * std::sin(th[i]);
+    }
+    real_upload(mot_.p, mv.data(), mv.size());
+    real_upload(vel_.p, vv.data(), vv.size());
+}
+
+// reference voronoiModelBase::setExclusions (.cpp:89-107): excluded cells get
+// v0 = Dr = 0 and their force is reported as an external force
+void VoronoiBatch::set_exclusions(const int *ex) {
+    if (!excl_.p) {
+        excl_.alloc(N_);
+        ext_.alloc(2 * N_ * rsz_);
+    }
+    excl_.upload(ex, N_);
+    std::vector<double> mv(2 * (size_t)N_);
+    real_download(mot_.p, mv.data(), mv.size());
+    for (int i = 0; i < N_; ++i)
+        if (ex[i]) mv[2 * i] = mv[2 * i + 1] = 0.0;
+    real_upload(mot_.p, mv.data(), mv.size());
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
+    std::vector<double> xy(2 * (size_t)N_);
+    for (int i = 0; i < N_; ++i) {
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i])) throw std::invalid_argument("non-finite position");
+        xy[2 * i] = x[i];
+        xy[2 * i + 1] = y[i];
+    }
+    double *p = parity_ ? pos1_.p : pos0_.p;
+    hip_check(hipMemcpy(p, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice), "set_positions");
+    if (retri) global_triangulation();
+}
+
+void VoronoiBatch::set_noise_mode(bool use_buffer) {
+    noise_buffer_ = use_buffer;
+    if (use_buffer && !noise_.p) noise_.alloc(2 * (size_t)N_);
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+void VoronoiBatch::upload_noise(const double *eta, int count) {
+    if (!noise_.p) noise_.alloc(2 * (size_t)N_);
+    noise_.upload(eta, std::min<size_t>(count, 2 * (size_t)N_));
+}
+
+void VoronoiBatch::set_active(const int *active, const int *steps_left) {
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    for (int s = 0; s < S_; ++s) {
+        if (active) st[s].active = active[s];
+        if (steps_left) st[s].steps_left = steps_left[s];
+    }
+    st_.upload(st.data(), S_);
+}
+
+int VoronoiBatch::global_triangulation() {
+    cgpu_voronoi_batch b = view();
+    int r = cgpu_global_triangulation(&b, stream_);
+    if (r) return r;
+    return status();
+}
+
+int VoronoiBatch::compute_forces() {
+    cgpu_voronoi_batch b = view();
+    int r = cgpu_voronoi_forces(&b, stream_);
+    if (r) return r;
+    return sync();
+}
+
+int VoronoiBatch::step_launch(int kind) {
+    cgpu_voronoi_batch b = view();
+    int r = kind == 0 ? cgpu_spv_step(&b, stream_) : cgpu_brownian_step(&b, stream_);
+    parity_ ^= 1;
+    return r;
+}
+
+int VoronoiBatch::spv_steps(int steps, bool use_graph) {
+    if (steps <= 0) return CGPU_OK;
+    if (!use_graph || noise_buffer_) {
+        for (int k = 0; k < steps; ++k) {
+            int r = step_launch(0);
+            if (r) return r;
+        }
+        return CGPU_OK;
+    }
+    // a graph of two steps returns to the same position buffer parity
+    if (graph_exec_ && graph_parity_ != parity_) {
+        int r = step_launch(0);
+        if (r) return r;
+        --steps;
+    }
+    if (!graph_exec_ && steps >= 2) {
+        // capture only records the launches; the two parity flips cancel
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+        graph_parity_ = parity_;
+        step_launch(0);
+        step_launch(0);
+        hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+        hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate");
+    }
+    while (steps >= 2) {
+        hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+        steps -= 2;
+    }
+    if (steps == 1) return step_launch(0);
+    return CGPU_OK;
+}
+
+int VoronoiBatch::brownian_steps(int steps) {
+    for (int k = 0; k < steps; ++k) {
+        int r = step_launch(1);
+        if (r) return r;
+    }
+    return CGPU_OK;
+}
+
+int VoronoiBatch::sync() {
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return CGPU_OK;
+}
+
+int VoronoiBatch::status() {
+    sync();
+    std::vector<int> c(8);
+    counters_.download(c.data(), 8);
+    if (c[3]) return CGPU_ERR_BIN_CAPACITY;
+    if (c[5]) return CGPU_ERR_NONFINITE;
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    for (auto &q : st)
+        if (q.error) return q.error;
+    if (c[4]) return CGPU_ERR_TOPOLOGY;
+    return CGPU_OK;
+}
+
+void VoronoiBatch::get(double *x, double *y, double *theta, double *vx, double *vy, double *Fx, double *Fy,
+                       double *A, double *P) {
+    sync();
+    std::vector<double> t(2 * (size_t)N_);
+    if (x || y) {
+        (parity_ ? pos1_ : pos0_).download(t.data(), t.size());
+        for (int i = 0; i < N_; ++i) {
+            if (x) x[i] = t[2 * i];
+            if (y) y[i] = t[2 * i + 1];
+        }
+    }
+    if (theta) real_download(theta_.p, theta, N_);
+    if (vx || vy) {
+        real_download(vel_.p, t.data(), t.size());
+        for (int i = 0; i < N_; ++i) {
+            if (vx) vx[i] = t[2 * i];
+            if (vy) vy[i] = t[2 * i + 1];
+        }
+    }
+    if (Fx || Fy) {
+        real_download(force_.p, t.data(), t.size());
+        for (int i = 0; i < N_; ++i) {
+            if (Fx) Fx[i] = t[2 * i];
+            if (Fy) Fy[i] = t[2 * i + 1];
+        }
+    }
+    if (A || P) {
+        real_download(AP_.p, t.data(), t.size());
+        for (int i = 0; i < N_; ++i) {
+            if (A) A[i] = t[2 * i];
+            if (P) P[i] = t[2 * i + 1];
+        }
+    }
+}
+
+void VoronoiBatch::get_topology(int *nn, int *nbr, int cap, int *neigh_max) {
+    sync();
+    std::vector<int> hn(N_), hr((size_t)CGPU_RING_CAP * N_);
+    nn_.download(hn.data(), N_);
+    nbr_.download(hr.data(), hr.size());
+    for (int i = 0; i < N_; ++i) {
+        nn[i] = hn[i];
+        for (int k = 0; k < hn[i] && k < cap; ++k) nbr[(size_t)i * cap + k] = hr[(size_t)k * N_ + i];
+    }
+    if (neigh_max) {
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s) neigh_max[s] = st[s].neigh_max;
+    }
+}
+
+void VoronoiBatch::get_counters(int s, long long *out) {
+    sync();
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    const cgpu_sys_state &q = st[s];
+    out[0] = q.global_fixes;
+    out[1] = q.local_repairs;
+    out[2] = q.repair_events;
+    out[3] = q.step;
+    out[4] = q.mode;
+    out[5] = q.nf_count;
+    out[6] = q.neigh_max;
+    out[7] = q.error;
+}
+
+// reference VoronoiQuadraticEnergy::computeEnergy (.cpp:384-397), FP64 sum on the host
+double VoronoiBatch::energy(int s) {
+    const cgpu_sys_params &p = hp_[s];
+    std::vector<double> A(N_), P(N_);
+    get(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, A.data(), P.data());
+    std::vector<double> pref(2 * (size_t)N_);
+    real_download(APpref_.p, pref.data(), pref.size());
+    double E = 0.0;
+    for (int i = p.off; i < p.off + p.n; ++i) {
+        E += p.KA * (A[i] - pref[2 * i]) * (A[i] - pref[2 * i]);
+        E += p.KP * (P[i] - pref[2 * i + 1]) * (P[i] - pref[2 * i + 1]);
+    }
+    return E;
+}
+
+int VoronoiBatch::autotune() { return CGPU_OK; }
+
+}  // namespace cgpu_host
+
+// ================================================================ object-layer C ABI
+using cgpu_host::VoronoiBatch;
+
+namespace {
+thread_local std::string g_last_error;
+template <typename F> int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return CGPU_ERR_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char *cgpu_last_error(void) { return g_last_error.c_str(); }
+
+int cgpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void *cgpu_vor_create(int num_systems, const int *n, const double *L, const uint64_t *seed, const double *x,
+                      const double *y, const double *theta, int precision, int device) {
+    try {
+        return new VoronoiBatch(num_systems, n, L, seed, x, y, theta, precision, device);
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+void cgpu_vor_destroy(void *h) { delete (VoronoiBatch *)h; }
+int cgpu_vor_set_params(void *h, int s, double KA, double KP, double dt, double mu, double T) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_params(s, KA, KP, dt, mu, T); return 0; });
+}
+int cgpu_vor_set_prefs(void *h, const double *A0, const double *P0) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_prefs(A0, P0); return 0; });
+}
+int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_motility(v0, Dr); return 0; });
+}
+int cgpu_vor_set_exclusions(void *h, const int *ex) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_exclusions(ex); return 0; });
+}
+int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retri) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_positions(x, y, retri != 0); return ((VoronoiBatch *)h)->status(); });
+}
+int cgpu_vor_set_noise_mode(void *h, int use_buffer) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_noise_mode(use_buffer != 0); return 0; });
+}
+int cgpu_vor_upload_noise(void *h, const double *eta) {
+    return guarded([&] {
+        VoronoiBatch *b = (VoronoiBatch *)h;
+        b->upload_noise(eta, 2 * b->num_cells());
+        return 0;
+    });
+}
+int cgpu_vor_compute_forces(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->compute_forces(); }); }
+int cgpu_vor_spv_steps(void *h, int steps, int use_graph) {
+    return guarded([&] { return ((VoronoiBatch *)h)->spv_steps(steps, use_graph != 0); });
+}
+int cgpu_vor_brownian_steps(void *h, int steps) {
+    return guarded([&] { return ((VoronoiBatch *)h)->brownian_steps(steps); });
+}
+int cgpu_vor_sync(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->sync(); }); }
+int cgpu_vor_status(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->status(); }); }
+int cgpu_vor_get(void *h, double *x, double *y, double *theta, double *vx, double *vy, double *Fx, double *Fy,
+                 double *A, double *P) {
+    return guarded([&] { ((VoronoiBatch *)h)->get(x, y, theta, vx, vy, Fx, Fy, A, P); return 0; });
+}
+int cgpu_vor_get_topology(void *h, int *nn, int *nbr, int cap, int *neigh_max) {
+    return guarded([&] { ((VoronoiBatch *)h)->get_topology(nn, nbr, cap, neigh_max); return 0; });
+}
+int cgpu_vor_get_counters(void *h, int s, long long *out) {
+    return guarded([&] { ((VoronoiBatch *)h)->get_counters(s, out); return 0; });
+}
+int cgpu_vor_num_cells(void *h) { return ((VoronoiBatch *)h)->num_cells(); }
+double cgpu_vor_energy(void *h, int s) {
+    try {
+        return ((VoronoiBatch *)h)->energy(s);
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return NAN;
+    }
+}
+int cgpu_vor_set_active(void *h, const int *active, const int *steps_left) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_active(active, steps_left); return 0; });
+}
+int cgpu_vor_autotune(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->autotune(); }); }
+int cgpu_vor_view(void *h, cgpu_voronoi_batch *out) {
+    *out = ((VoronoiBatch *)h)->view();
+    return 0;
+}
+void *cgpu_vor_stream(void *h) { return (void *)((VoronoiBatch *)h)->stream(); }
+
+}  // extern "C"

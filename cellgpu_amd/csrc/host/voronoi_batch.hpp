@@ -1,0 +1,117 @@
+// Host-side owner of a device-resident batch of Voronoi systems.
+//
+// One VoronoiBatch holds S independent periodic monolayers back to back in
+// structure-of-arrays device memory (a single reference VoronoiQuadraticEnergy
+// is the S = 1 case). It allocates once, runs the step as a fixed kernel
+// sequence on its own stream (optionally captured in a hipGraph) and copies
+// state to the host only when asked.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/cellgpu_amd.h"
+
+namespace cgpu_host {
+
+void hip_check(hipError_t e, const char *what);
+
+template <typename T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) hip_check(hipMalloc((void **)&p, count * sizeof(T)), "hipMalloc");
+        if (count) hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, size_t offset = 0) {
+        hip_check(hipMemcpy(p + offset, h, count * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+    void download(T *h, size_t count, size_t offset = 0) const {
+        hip_check(hipMemcpy(h, p + offset, count * sizeof(T), hipMemcpyDeviceToHost), "download");
+    }
+};
+
+// reference initial condition (Simple2DCell.cpp:85-100, Simple2DActiveCell.cpp:55-70):
+// std::mt19937(seed), x then y per cell, then the directors, uniform in [0,L) / [0,2pi)
+void reference_random_init(int n, uint32_t seed, double L, double *x, double *y, double *theta);
+
+class VoronoiBatch {
+   public:
+    VoronoiBatch(int num_systems, const int *n, const double *L, const uint64_t *seed, const double *x,
+                 const double *y, const double *theta, int precision, int device);
+    ~VoronoiBatch();
+
+    int num_systems() const { return S_; }
+    int num_cells() const { return N_; }
+    int precision() const { return prec_; }
+    hipStream_t stream() const { return stream_; }
+
+    void set_params(int s, double KA, double KP, double dt, double mu, double T);
+    void set_prefs(const double *A0, const double *P0);
+    void set_motility(const double *v0, const double *Dr);
+    void set_exclusions(const int *ex);
+    void set_positions(const double *x, const double *y, bool retriangulate);
+    void set_noise_mode(bool use_buffer);
+    void upload_noise(const double *eta, int count);
+    void set_active(const int *active, const int *steps_left);
+
+    int global_triangulation();
+    int compute_forces();
+    int spv_steps(int steps, bool use_graph);
+    int brownian_steps(int steps);
+    int sync();
+    int status();
+
+    void get(double *x, double *y, double *theta, double *vx, double *vy, double *Fx, double *Fy, double *A,
+             double *P);
+    void get_topology(int *nn, int *nbr, int cap, int *neigh_max);
+    void get_counters(int s, long long *out);
+    double energy(int s);
+    cgpu_voronoi_batch view() const;  // current parity
+    const std::vector<cgpu_sys_params> &params() const { return hp_; }
+
+    // raw device access for other engines (FIRE, vertex model initialisation)
+    double *pos_dev() { return parity_ ? pos1_.p : pos0_.p; }
+    void *force_dev() { return force_.p; }
+    void swap_parity() { parity_ ^= 1; }
+    int autotune();
+
+   private:
+    void upload_params();
+    void real_upload(void *dst, const double *src2, size_t count);
+    void real_download(const void *src, double *dst, size_t count);
+    int step_launch(int kind);
+
+    int S_ = 0, N_ = 0, prec_ = 1, device_ = 0, total_bins_ = 0;
+    size_t rsz_ = 4;
+    int parity_ = 0;
+    bool noise_buffer_ = false;
+    hipStream_t stream_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    int graph_parity_ = -1;
+    std::vector<cgpu_sys_params> hp_;
+    std::vector<int> cell_sys_h_;
+    DevBuf<cgpu_sys_params> sys_;
+    DevBuf<cgpu_sys_state> st_;
+    DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_, bin_idx_;
+    DevBuf<unsigned int> bin_count_;
+    DevBuf<double> pos0_, pos1_, noise_;
+    DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
+};
+
+}  // namespace cgpu_host

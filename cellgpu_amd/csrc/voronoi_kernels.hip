@@ -1,0 +1,840 @@
+// HIP kernels of the Voronoi-model time step for gfx950 (MI355X), and the
+// kernel-caller half of the C ABI declared in include/cellgpu_amd.h.
+//
+// Layout: structure-of-arrays over all cells of all systems of a batch; rings
+// are slot-major (nbr[k*N + i]) so that a wavefront reading slot k of 64
+// consecutive cells issues one coalesced load. Positions stay FP64 in every
+// mode; precision 1 runs geometry, forces and the integrator in FP32 while the
+// Delaunay test, the star construction and the cell list stay FP64.
+//
+// One step = force/EOM/move -> bin -> test+geometry -> repair chain. The repair
+// chain is a fixed sequence of kernels that exit immediately when no system
+// needs them, so a whole step is a fixed launch sequence that can be captured
+// in a hipGraph with no host round trip.
+#include <hip/hip_runtime.h>
+
+#include "../../include/cellgpu_amd.h"
+#include "cgpu_device.h"
+
+using namespace cgpu;
+
+namespace {
+
+constexpr int CAP = CGPU_RING_CAP;
+constexpr int BCAP = CGPU_BIN_CAP;
+
+__device__ inline dvec2 ldpos(const double *p, int i) {
+    double2 v = reinterpret_cast<const double2 *>(p)[i];
+    return dvec2{v.x, v.y};
+}
+__device__ inline void stpos(double *p, int i, dvec2 v) {
+    reinterpret_cast<double2 *>(p)[i] = make_double2(v.x, v.y);
+}
+template <typename R> __device__ inline vec2<R> ld2(const void *a, int i) {
+    const R *p = reinterpret_cast<const R *>(a);
+    return vec2<R>{p[2 * i], p[2 * i + 1]};
+}
+template <> __device__ inline vec2<float> ld2<float>(const void *a, int i) {
+    float2 v = reinterpret_cast<const float2 *>(a)[i];
+    return vec2<float>{v.x, v.y};
+}
+template <> __device__ inline vec2<double> ld2<double>(const void *a, int i) {
+    double2 v = reinterpret_cast<const double2 *>(a)[i];
+    return vec2<double>{v.x, v.y};
+}
+template <typename R> __device__ inline void st2(void *a, int i, vec2<R> v) {
+    R *p = reinterpret_cast<R *>(a);
+    p[2 * i] = v.x;
+    p[2 * i + 1] = v.y;
+}
+template <> __device__ inline void st2<float>(void *a, int i, vec2<float> v) {
+    reinterpret_cast<float2 *>(a)[i] = make_float2(v.x, v.y);
+}
+template <> __device__ inline void st2<double>(void *a, int i, vec2<double> v) {
+    reinterpret_cast<double2 *>(a)[i] = make_double2(v.x, v.y);
+}
+template <typename R> __device__ inline vec2<R> cvt(dvec2 d) { return vec2<R>{(R)d.x, (R)d.y}; }
+
+__device__ inline int ring_at(const cgpu_voronoi_batch &b, int i, int k) { return b.nbr[k * b.num_cells + i]; }
+
+// wave-aggregated append: every lane with pred gets a distinct slot of list
+__device__ inline void wave_append(bool pred, int value, int *list, int *counter) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    if (pred) {
+        int rank = __popcll(m & ((1ull << lane) - 1ull));
+        list[base + rank] = value;
+    }
+}
+
+// ------------------------------------------------------------------ geometry
+// Voronoi area and perimeter of cell i in the reference GPU order
+// (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
+template <typename R>
+__device__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
+    const Box box = make_box(sp.L);
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    if (z < 3) return;
+    dvec2 pi = ldpos(pos, i);
+    vec2<R> rij = cvt<R>(min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi));
+    vec2<R> rik = cvt<R>(min_dist(box, ldpos(pos, b.nbr[i]), pi));
+    vec2<R> vfirst = circ0<R>(rij, rik), vlast = vfirst;
+    R A = R(0), P = R(0);
+    for (int k = 1; k < z; ++k) {
+        rij = rik;
+        rik = cvt<R>(min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi));
+        vec2<R> vnext = circ0<R>(rij, rik);
+        A += tri_area<R>(vlast, vnext);
+        R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
+        P += sqrt(dx * dx + dy * dy);
+        vlast = vnext;
+    }
+    A += tri_area<R>(vlast, vfirst);
+    R dx = vlast.x - vfirst.x, dy = vlast.y - vfirst.y;
+    P += sqrt(dx * dx + dy * dy);
+    st2<R>(b.AP, i, vec2<R>{A, P});
+    vec2<R> pref = ld2<R>(b.APpref, i);
+    st2<R>(b.dAP, i, vec2<R>{(R)sp.KA * (A - pref.x), (R)sp.KP * (P - pref.y)});
+}
+
+// -------------------------------------------------------------------- forces
+// Force on cell i: sum over ring slots k of the reference force set
+// (voronoiQuadraticEnergy.cu:99-234): slot k is the Voronoi vertex of
+// (i, ring k-2, ring k-1); its neighbours in the three polygons are vertex k-1,
+// vertex k+1 and the circumcenter of (ring k-2, ring k-1, delOther k).
+template <typename R>
+__device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
+    const Box box = make_box(sp.L);
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    if (z < 3) return vec2<R>{R(0), R(0)};
+    const dvec2 pi = ldpos(pos, i);
+    auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
+    auto dd = [&](int id) { return cvt<R>(min_dist(box, ldpos(pos, id), pi)); };
+    int jB = rk(-2), jC = rk(-1);
+    vec2<R> dA = dd(rk(-3)), dB = dd(jB), dC = dd(jC);
+    vec2<R> qB = ld2<R>(b.dAP, jB), qC = ld2<R>(b.dAP, jC);
+    vec2<R> vL = circ0<R>(dA, dB), vM = circ0<R>(dB, dC);
+    const vec2<R> qI = ld2<R>(b.dAP, i);
+    vec2<R> sum{R(0), R(0)};
+    for (int k = 0; k < z; ++k) {
+        int jD = b.nbr[k * N + i];
+        vec2<R> dD = dd(jD);
+        vec2<R> vN = circ0<R>(dC, dD);
+        vec2<R> dO = dd(b.oth[k * N + i]);
+        // --- reference force set for slot k
+        const vec2<R> dnext = dB, dcl = dC;
+        R c = dnext.y * dcl.x - dnext.x * dcl.y;
+        R den = R(1.0) / (c * c);
+        R rik2 = dcl.x * dcl.x + dcl.y * dcl.y;
+        R rij2 = dnext.x * dnext.x + dnext.y * dnext.y;
+        R m11 = R(0.5) * den * (dcl.y - dnext.y) * (dcl.y * rij2 - dnext.y * rik2);
+        R m12 = R(0.5) * den * (dnext.y - dcl.y) * (dcl.x * rij2 - dnext.x * rik2);
+        R m21 = R(0.5) * den * (dnext.x - dcl.x) * (dcl.y * rij2 - dnext.y * rik2);
+        R m22 = R(0.5) * den * (dcl.x - dnext.x) * (dcl.x * rij2 - dnext.x * rik2);
+        const vec2<R> vcur = vM, vlast = vL, vnext = vN;
+        vec2<R> vother = circ3<R>(dnext, dcl, dO);
+        vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
+        vec2<R> dl{vlast.x - vcur.x, vlast.y - vcur.y};
+        R dlnorm = sqrt(dl.x * dl.x + dl.y * dl.y);
+        vec2<R> dn{vcur.x - vnext.x, vcur.y - vnext.y};
+        R dnnorm = sqrt(dn.x * dn.x + dn.y * dn.y);
+        const vec2<R> dclv = dl, dncv = dn;
+        const R dclnorm = dlnorm, dncnorm = dnnorm;
+        vec2<R> dPdv{dl.x / dlnorm - dn.x / dnnorm, dl.y / dlnorm - dn.y / dnnorm};
+        vec2<R> dEdv{qI.x * dAdv.x + qI.y * dPdv.x, qI.x * dAdv.y + qI.y * dPdv.y};
+        // cell ring k-1
+        dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
+        dn = vec2<R>{vcur.x - vother.x, vcur.y - vother.y};
+        dnnorm = sqrt(dn.x * dn.x + dn.y * dn.y);
+        dPdv = vec2<R>{-dncv.x / dncnorm - dn.x / dnnorm, -dncv.y / dncnorm - dn.y / dnnorm};
+        dEdv.x += qC.x * dAdv.x + qC.y * dPdv.x;
+        dEdv.y += qC.x * dAdv.y + qC.y * dPdv.y;
+        // cell ring k-2
+        dAdv = vec2<R>{R(0.5) * (vother.y - vlast.y), R(0.5) * (vlast.x - vother.x)};
+        dPdv = vec2<R>{-dn.x / dnnorm + dclv.x / dclnorm, -dn.y / dnnorm + dclv.y / dclnorm};
+        dEdv.x += qB.x * dAdv.x + qB.y * dPdv.x;
+        dEdv.y += qB.x * dAdv.y + qB.y * dPdv.y;
+        dEdv.x *= R(2.0);
+        dEdv.y *= R(2.0);
+        sum.x += dEdv.x * m11 + dEdv.y * m21;
+        sum.y += dEdv.x * m12 + dEdv.y * m22;
+        // --- advance the window
+        dB = dC;
+        dC = dD;
+        qB = qC;
+        qC = ld2<R>(b.dAP, jD);
+        vL = vM;
+        vM = vN;
+    }
+    return sum;
+}
+
+// ------------------------------------------------------- Delaunay star (repair)
+// Counter-clockwise Delaunay 1-ring of cell i by gift wrapping over cell-list
+// candidates with exact predicates. Each triangle is certified empty by
+// requiring its circumcircle to lie inside the scanned bin block; otherwise the
+// block grows. Rotation is canonical (smallest neighbour first). Returns the
+// degree, -1 when no certified star exists, -2 when the degree exceeds CAP.
+struct Cand {
+    int id;
+    dvec2 d;
+};
+template <typename F>
+__device__ inline void scan_block(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp, int ib, int jb, int w,
+                                  F &&f) {
+    const int xs = sp.xsize;
+    const int nb = min(2 * w + 1, xs);
+    for (int jj = 0; jj < nb; ++jj) {
+        int cy = jb - w + jj;
+        cy = ((cy % xs) + xs) % xs;
+        for (int ii = 0; ii < nb; ++ii) {
+            int cx = ib - w + ii;
+            cx = ((cx % xs) + xs) % xs;
+            int bin = sp.bin_off + cy * xs + cx;
+            int cnt = min((int)b.bin_count[bin], BCAP);
+            for (int q = 0; q < cnt; ++q) f(b.bin_idx[bin * BCAP + q]);
+        }
+    }
+}
+
+__device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp,
+                             int *ring) {
+    const Box box = make_box(sp.L);
+    const dvec2 pi = ldpos(pos, i);
+    const double bs = sp.boxsize;
+    const int xs = sp.xsize;
+    int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
+    int jb = max(0, min(xs - 1, (int)floor(pi.y / bs)));
+    const int wmax = max(1, xs / 2);
+    for (int w = 1; w <= wmax; ++w) {
+        const bool full = (2 * w + 1 >= xs);
+        double rcov = fmin(fmin(pi.x - (ib - w) * bs, (ib + w + 1) * bs - pi.x),
+                           fmin(pi.y - (jb - w) * bs, (jb + w + 1) * bs - pi.y));
+        // nearest candidate
+        Cand first{-1, {0, 0}};
+        int count = 0;
+        scan_block(b, sp, ib, jb, w, [&](int m) {
+            if (m == i) return;
+            ++count;
+            dvec2 d = min_dist(box, ldpos(pos, m), pi);
+            if (first.id < 0) { first = Cand{m, d}; return; }
+            int c = cmp_norm2(d.x, d.y, first.d.x, first.d.y);
+            if (c < 0 || (c == 0 && m < first.id)) first = Cand{m, d};
+        });
+        if (count < 3) continue;
+        Cand cur = first;
+        int z = 0;
+        bool ok = true;
+        for (int guard = 0; guard <= CAP; ++guard) {
+            if (z >= CAP) return -2;
+            ring[z++] = cur.id;
+            Cand best{-1, {0, 0}};
+            scan_block(b, sp, ib, jb, w, [&](int m) {
+                if (m == i || m == cur.id) return;
+                dvec2 d = min_dist(box, ldpos(pos, m), pi);
+                if (orient0(cur.d.x, cur.d.y, d.x, d.y) <= 0) return;
+                if (best.id < 0) { best = Cand{m, d}; return; }
+                int ic = incircle0(cur.d.x, cur.d.y, best.d.x, best.d.y, d.x, d.y);
+                if (ic > 0 || (ic == 0 && m < best.id)) best = Cand{m, d};
+            });
+            if (best.id < 0) { ok = false; break; }
+            if (!full) {
+                dvec2 c = circ0<double>(cur.d, best.d);
+                double rr = sqrt(c.x * c.x + c.y * c.y);
+                if (!(2.0 * rr <= rcov * (1.0 - 1e-9))) { ok = false; break; }
+            }
+            cur = best;
+            if (cur.id == first.id) break;
+        }
+        if (!ok || cur.id != first.id) continue;
+        // canonical rotation
+        int k0 = 0;
+        for (int k = 1; k < z; ++k)
+            if (ring[k] < ring[k0]) k0 = k;
+        if (k0 != 0) {
+            int tmp[CAP];
+            for (int k = 0; k < z; ++k) tmp[k] = ring[(k + k0) % z];
+            for (int k = 0; k < z; ++k) ring[k] = tmp[k];
+        }
+        return z;
+    }
+    return -1;
+}
+
+// reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
+// A ring that does not contain nm1 leaves that slot's delOther unchanged.
+__device__ bool del_sets(const cgpu_voronoi_batch &b, int i) {
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    if (z < 3) return false;
+    int nm1 = b.nbr[(z - 2) * N + i], n1 = b.nbr[(z - 1) * N + i];
+    for (int k = 0; k < z; ++k) {
+        int n2 = b.nbr[k * N + i];
+        int zn = min(b.nn[n1], CAP);
+        for (int q = 0; q < zn; ++q) {
+            if (b.nbr[q * N + n1] == nm1) {
+                b.oth[k * N + i] = b.nbr[((q + 1) % zn) * N + n1];
+                break;
+            }
+        }
+        int o = b.oth[k * N + i];
+        if (nm1 == o || n1 == o || i == o) return false;
+        nm1 = n1;
+        n1 = n2;
+    }
+    return true;
+}
+
+__device__ inline int owned_triangles(const cgpu_voronoi_batch &b, int i) {
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    int c = 0;
+    for (int k = 0; k < z; ++k) {
+        int a = b.nbr[k * N + i], q = b.nbr[((k + 1) % z) * N + i];
+        if (i < a && i < q) ++c;
+    }
+    return c;
+}
+
+// ===================================================================== kernels
+
+// per-step reset of per-system topology state and queue counters
+__global__ void k_reset(cgpu_voronoi_batch b) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        b.counters[0] = 0;
+        b.counters[1] = 0;
+        b.counters[2] = 0;
+    }
+    if (s < b.num_systems) {
+        cgpu_sys_state &st = b.st[s];
+        st.any_fail = 0;
+        st.nf_count = 0;
+        st.mode = 0;
+        st.need_global = 0;
+        st.nn_delta = 0;
+        st.tri_delta = 0;
+        st.max_deg = 0;
+    }
+}
+
+__global__ void k_zero_bins(cgpu_voronoi_batch b) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < b.total_bins; q += gridDim.x * blockDim.x)
+        b.bin_count[q] = 0;
+}
+
+// reference cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)), atomic slot
+__global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
+    dvec2 p = ldpos(pos, i);
+    int ib = max(0, min(sp.xsize - 1, (int)floor(p.x / sp.boxsize)));
+    int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
+    int bin = sp.bin_off + jb * sp.xsize + ib;
+    unsigned int slot = atomicAdd(&b.bin_count[bin], 1u);
+    if (slot < (unsigned)BCAP)
+        b.bin_idx[bin * BCAP + slot] = i;
+    else
+        b.counters[3] = 1;
+}
+
+__device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
+    if (atomicExch(&b.repair[m], 1) == 0) {
+        int q = atomicAdd(&b.counters[0], 1);
+        b.flagged[q] = m;
+    }
+}
+
+// FP64 empty-circumcircle test of the triangles owned by cell i (i smallest
+// index), reference voronoiModelBase.cu:24-109 incl. its search window, then
+// the geometry of cell i with its current ring.
+template <typename R>
+__global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    if (!b.st[s].active) return;
+    if (test) {
+        const Box box = make_box(sp.L);
+        const int N = b.num_cells;
+        const int z = b.nn[i];
+        const int xs = sp.xsize;
+        const dvec2 v = ldpos(pos, i);
+        int jn = b.nbr[i];
+        for (int k = 0; k < z; ++k) {
+            int j = jn;
+            jn = b.nbr[((k + 1) % z) * N + i];
+            int l = jn;
+            if (!(i < j && i < l)) continue;
+            int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+            dvec2 pt1 = min_dist(box, ldpos(pos, j), v), pt2 = min_dist(box, ldpos(pos, l), v);
+            dvec2 Q = circ0<double>(pt1, pt2);
+            double dx = pt1.x - Q.x, dy = pt1.y - Q.y;
+            double rad = sqrt(dx * dx + dy * dy);
+            bool bad = false;
+            // reference: wcheck = ceil(rad/boxsize) capped at xsize/2 (a non-finite
+            // radius takes the cap)
+            double wc = ceil(rad / sp.boxsize);
+            int wcheck = (wc < (double)(xs / 2)) ? (int)wc : xs / 2;
+            rad = rad * rad;
+            for (int ii = ib - wcheck; ii <= ib + wcheck; ++ii) {
+                int cx = ii;
+                if (cx < 0) cx += xs;
+                if (cx >= xs) cx -= xs;
+                for (int jj = jb - wcheck; jj <= jb + wcheck; ++jj) {
+                    int cy = jj;
+                    if (cy < 0) cy += xs;
+                    if (cy >= xs) cy -= xs;
+                    int bin = sp.bin_off + cy * xs + cx;
+                    int cnt = min((int)b.bin_count[bin], BCAP);
+                    for (int q = 0; q < cnt; ++q) {
+                        int m = b.bin_idx[bin * BCAP + q];
+                        dvec2 a = min_dist(box, ldpos(pos, m), v);
+                        dvec2 c = min_dist(box, a, Q);
+                        if (c.x * c.x + c.y * c.y < rad) {
+                            if (m != i && m != j && m != l) {
+                                bad = true;
+                                flag_cell(b, m);
+                            }
+                        }
+                    }
+                }
+            }
+            if (bad) {
+                b.st[s].any_fail = 1;
+                flag_cell(b, i);
+                flag_cell(b, j);
+                flag_cell(b, l);
+            }
+        }
+    }
+    cell_geometry<R>(b, pos, i, sp);
+}
+
+// NeedsFixing = flagged cells and their current neighbours (reference
+// voronoiModelBase.cpp:617-646), built as a deduplicated device queue.
+__global__ void k_mark_nf(cgpu_voronoi_batch b) {
+    const int nflag = b.counters[0];
+    const int N = b.num_cells;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
+        int i = b.flagged[q];
+        b.repair[i] = 0;
+        int z = b.nn[i];
+        for (int k = -1; k < z; ++k) {
+            int x = (k < 0) ? i : b.nbr[k * N + i];
+            if (atomicExch(&b.mark[x], 1) == 0) {
+                int slot = atomicAdd(&b.counters[1], 1);
+                b.nfq[slot] = x;
+                atomicAdd(&b.st[b.cell_sys[x]].nf_count, 1);
+            }
+        }
+    }
+}
+
+// enqueue every cell of system s into the global queue
+__device__ void enqueue_system(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp) {
+    __shared__ int base;
+    if (threadIdx.x == 0) base = atomicAdd(&b.counters[2], sp.n);
+    __syncthreads();
+    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) b.gq[base + t] = sp.off + t;
+}
+
+// one block per system: |NeedsFixing| > N/6 -> global retriangulation
+// (reference voronoiModelBase.cpp:650-653); all=1 forces a global pass
+__global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
+    const int s = blockIdx.x;
+    const cgpu_sys_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    int mode;
+    if (all) {
+        mode = 2;
+    } else {
+        if (!st.any_fail) return;
+        mode = (st.nf_count > sp.n / 6) ? 2 : 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st.mode = mode;
+        if (!all) {
+            st.repair_events += 1;
+            if (mode == 1) st.local_repairs += st.nf_count;
+        }
+    }
+    if (mode == 2) enqueue_system(b, sp);
+}
+
+// local repair: new Delaunay star of each NeedsFixing cell (reference
+// repairTriangulation, voronoiModelBase.cpp:429-505), with the neighMax rule and
+// the changes of the 6N / 2N invariants accumulated per system
+__global__ void k_star_local(cgpu_voronoi_batch b, const double *pos) {
+    const int nq = b.counters[1];
+    const int N = b.num_cells;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        int i = b.nfq[q];
+        int s = b.cell_sys[i];
+        cgpu_sys_state &st = b.st[s];
+        if (st.mode != 1) continue;
+        const cgpu_sys_params &sp = b.sys[s];
+        int zold = b.nn[i];
+        int told = owned_triangles(b, i);
+        int ring[CAP];
+        int z = delaunay_star(b, pos, i, sp, ring);
+        if (z < 0 || z > st.neigh_max) {
+            st.need_global = 1;
+            continue;
+        }
+        b.nn[i] = z;
+        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+        int tnew = owned_triangles(b, i);
+        atomicAdd(&st.nn_delta, z - zold);
+        atomicAdd(&st.tri_delta, tnew - told);
+    }
+}
+
+// delOther of repaired cells (reference enforceTopology, voronoiModelBase.cpp:
+// 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
+__global__ void k_delsets_local(cgpu_voronoi_batch b) {
+    const int nq = b.counters[1];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        int i = b.nfq[q];
+        cgpu_sys_state &st = b.st[b.cell_sys[i]];
+        if (st.mode != 1 || st.need_global) continue;
+        if (st.nn_delta != 0 || st.tri_delta != 0) {
+            st.need_global = 1;
+            continue;
+        }
+        if (!del_sets(b, i)) st.need_global = 1;
+    }
+}
+
+// one block per system: a failed local repair becomes a global one
+__global__ void k_global_fallback(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x;
+    cgpu_sys_state &st = b.st[s];
+    if (!(st.mode == 1 && st.need_global)) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st.mode = 3;
+        st.nn_delta = 0;
+    }
+    enqueue_system(b, b.sys[s]);
+}
+
+// global retriangulation of queued systems (globalTriangulationCGAL)
+__global__ void k_star_global(cgpu_voronoi_batch b, const double *pos) {
+    const int nq = b.counters[2];
+    const int N = b.num_cells;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        int i = b.gq[q];
+        int s = b.cell_sys[i];
+        cgpu_sys_state &st = b.st[s];
+        int ring[CAP];
+        int z = delaunay_star(b, pos, i, b.sys[s], ring);
+        if (z < 0) {
+            st.error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
+            b.counters[4] = 1;
+            continue;
+        }
+        b.nn[i] = z;
+        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+        atomicMax(&st.max_deg, z);
+        atomicAdd(&st.nn_delta, z);
+    }
+}
+
+// delOther + geometry of retriangulated systems; geometry of locally repaired
+// cells; neighMax rule (voronoiModelBase.cpp:309-312); clears NeedsFixing marks
+template <typename R>
+__global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
+    const int ng = b.counters[2];
+    const int nl = b.counters[1];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
+        if (q < ng) {
+            int i = b.gq[q];
+            int s = b.cell_sys[i];
+            const cgpu_sys_params &sp = b.sys[s];
+            cgpu_sys_state &st = b.st[s];
+            del_sets(b, i);
+            cell_geometry<R>(b, pos, i, sp);
+            if (i == sp.off) {
+                int nmax = st.max_deg;
+                st.neigh_max = (nmax % 2 == 0) ? nmax + 2 : nmax + 1;
+                st.global_fixes += 1;
+                if (st.nn_delta != 6 * sp.n) st.error = CGPU_ERR_TOPOLOGY;
+            }
+        } else {
+            int i = b.nfq[q - ng];
+            b.mark[i] = 0;
+            if (b.st[b.cell_sys[i]].mode == 1) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
+        }
+    }
+}
+
+// force of every active cell, optional exclusions (reference
+// gpu_sum_forces_with_exclusions_kernel, voronoiQuadraticEnergy.cu:59-93)
+template <typename R>
+__device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, const double *pos, int i,
+                                                const cgpu_sys_params &sp) {
+    vec2<R> F = cell_force<R>(b, pos, i, sp);
+    if (b.excl) {
+        if (b.excl[i] != 0) {
+            if (b.ext_force) st2<R>(b.ext_force, i, vec2<R>{-F.x, -F.y});
+            F = vec2<R>{R(0), R(0)};
+        } else if (b.ext_force) {
+            st2<R>(b.ext_force, i, vec2<R>{R(0), R(0)});
+        }
+    }
+    return F;
+}
+
+template <typename R>
+__global__ void k_forces(cgpu_voronoi_batch b, const double *pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
+    st2<R>(b.force, i, force_with_exclusions<R>(b, pos, i, sp));
+}
+
+// reference noise draw for cell i: buffer or in-kernel counter-based deviate
+__device__ inline double eta_of(const cgpu_voronoi_batch &b, int i, const cgpu_sys_params &sp, int step, int stream,
+                                int which) {
+    if (b.noise) return b.noise[(size_t)which * b.num_cells + i];
+    double z0, z1;
+    normal_pair(sp.seed, (uint32_t)(i - sp.off), (uint32_t)step, (uint32_t)stream, z0, z1);
+    return which ? z1 : z0;
+}
+
+template <typename R> __device__ inline R cos_r(R x);
+template <> __device__ inline float cos_r<float>(float x) { return cosf(x); }
+template <> __device__ inline double cos_r<double>(double x) { return cos(x); }
+template <typename R> __device__ inline R sin_r(R x);
+template <> __device__ inline float sin_r<float>(float x) { return sinf(x); }
+template <> __device__ inline double sin_r<double>(double x) { return sin(x); }
+template <typename R> __device__ inline R atan2_r(R y, R x);
+template <> __device__ inline float atan2_r<float>(float y, float x) { return atan2f(y, x); }
+template <> __device__ inline double atan2_r<double>(double y, double x) { return atan2(y, x); }
+
+// fused force -> self-propelled EOM (reference GPU semantics,
+// selfPropelledParticleDynamics.cu:35-53) -> move (Simple2DCell.cu:21-35).
+// kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
+template <typename R, int KIND>
+__global__ void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double *pos_next) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    dvec2 p = ldpos(pos, i);
+    if (!st.active) {
+        stpos(pos_next, i, p);
+        return;
+    }
+    const int step = st.step + 1;  // Timestep is incremented before integrating
+    vec2<R> F = force_with_exclusions<R>(b, pos, i, sp);
+    st2<R>(b.force, i, F);
+    const R dt = (R)sp.dt, mu = (R)sp.mu;
+    vec2<R> disp;
+    if (KIND == 0) {
+        vec2<R> v = ld2<R>(b.vel, i);
+        disp.x = dt * (v.x + mu * F.x);
+        disp.y = dt * (v.y + mu * F.y);
+        vec2<R> m = ld2<R>(b.mot, i);  // v0, Dr
+        R th = reinterpret_cast<R *>(b.theta)[i];
+        R angleDiff = (R)eta_of(b, i, sp, step, 0, 0) * sqrt(R(2.0) * dt * m.y);
+        R cur = th;
+        v.x = m.x * cos_r<R>(th);
+        v.y = m.x * sin_r<R>(th);
+        if (v.y != R(0) && v.x != R(0)) cur = atan2_r<R>(v.y, v.x);
+        reinterpret_cast<R *>(b.theta)[i] = cur + angleDiff;
+        st2<R>(b.vel, i, v);
+    } else {
+        R sc = sqrt(dt * (R)sp.T * mu * R(2.0));
+        disp.x = (R)eta_of(b, i, sp, step, 1, 0) * sc + dt * mu * F.x;
+        disp.y = (R)eta_of(b, i, sp, step, 1, 1) * sc + dt * mu * F.y;
+    }
+    p.x += (double)disp.x;
+    p.y += (double)disp.y;
+    // a non-finite or runaway displacement (more than a box length) is an error
+    if (!(isfinite(p.x) && isfinite(p.y)) || fabs((double)disp.x) > sp.L || fabs((double)disp.y) > sp.L) {
+        b.counters[5] = 1;
+        st.error = CGPU_ERR_NONFINITE;
+        p = ldpos(pos, i);
+    }
+    put_in_box(make_box(sp.L), p);
+    stpos(pos_next, i, p);
+}
+
+__global__ void k_advance_step(cgpu_voronoi_batch b) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    cgpu_sys_state &st = b.st[s];
+    if (!st.active) return;
+    st.step += 1;
+    if (st.steps_left > 0) {
+        st.steps_left -= 1;
+        if (st.steps_left == 0) st.active = 0;
+    }
+}
+
+__global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
+    double z0, z1;
+    normal_pair(sp.seed, (uint32_t)(i - sp.off), (uint32_t)step, (uint32_t)stream, z0, z1);
+    out[i] = z0;
+    out[(size_t)b.num_cells + i] = z1;
+}
+
+inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
+
+int g_block = 256;
+constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
+constexpr int REPAIR_THREADS = 64;
+
+int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
+    hipLaunchKernelGGL(k_star_local, dim3(REPAIR_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos);
+    hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_star_global, dim3(REPAIR_BLOCKS * 4), dim3(REPAIR_THREADS), 0, st, b, pos);
+    if (b.precision)
+        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
+    else
+        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
+    return check();
+}
+
+}  // namespace
+
+// ============================================================== C ABI (kernel layer)
+extern "C" {
+
+int cgpu_set_block_size(int bs) {
+    if (bs < 64 || bs > 1024 || bs % 64) return CGPU_ERR_ARGS;
+    g_block = bs;
+    return CGPU_OK;
+}
+
+int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_zero_bins, dim3(min(nblocks(b->total_bins, 256), 4096)), dim3(256), 0, S(stream), *b);
+    hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b,
+                       use_next ? b->pos_next : b->pos);
+    return check();
+}
+
+int cgpu_test_and_geometry(const cgpu_voronoi_batch *b, int test, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, S(stream), *b, b->pos, test);
+    else
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, S(stream), *b, b->pos, test);
+    return check();
+}
+
+int cgpu_repair_topology(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    return repair_chain(*b, b->pos, S(stream));
+}
+
+int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
+    int r = cgpu_compute_cell_list(b, 0, stream);
+    if (r) return r;
+    hipLaunchKernelGGL(k_global_by_count, dim3(b->num_systems), dim3(256), 0, st, *b, 1);
+    hipLaunchKernelGGL(k_star_global, dim3(REPAIR_BLOCKS * 4), dim3(REPAIR_THREADS), 0, st, *b, b->pos);
+    if (b->precision)
+        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, *b, b->pos);
+    else
+        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, *b, b->pos);
+    return check();
+}
+
+int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_forces<float>, g, t, 0, S(stream), *b, b->pos);
+    else
+        hipLaunchKernelGGL(k_forces<double>, g, t, 0, S(stream), *b, b->pos);
+    return check();
+}
+
+static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream) {
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    hipStream_t st = S(stream);
+    if (b->precision) {
+        if (kind == 0)
+            hipLaunchKernelGGL((k_force_eom_move<float, 0>), g, t, 0, st, *b, b->pos, b->pos_next);
+        else
+            hipLaunchKernelGGL((k_force_eom_move<float, 1>), g, t, 0, st, *b, b->pos, b->pos_next);
+    } else {
+        if (kind == 0)
+            hipLaunchKernelGGL((k_force_eom_move<double, 0>), g, t, 0, st, *b, b->pos, b->pos_next);
+        else
+            hipLaunchKernelGGL((k_force_eom_move<double, 1>), g, t, 0, st, *b, b->pos, b->pos_next);
+    }
+    return check();
+}
+
+int cgpu_spv_force_eom_move(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    return force_eom_move(b, 0, stream);
+}
+int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    return force_eom_move(b, 1, stream);
+}
+
+// One step; after it, b->pos_next holds the new positions (the caller swaps).
+static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
+    int r = force_eom_move(b, kind, stream);
+    if (r) return r;
+    r = cgpu_compute_cell_list(b, 1, stream);
+    if (r) return r;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, *b, b->pos_next, 1);
+    else
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, *b, b->pos_next, 1);
+    r = repair_chain(*b, b->pos_next, st);
+    if (r) return r;
+    hipLaunchKernelGGL(k_advance_step, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
+    return check();
+}
+
+int cgpu_spv_step(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    return step_common(b, 0, stream);
+}
+int cgpu_brownian_step(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    return step_common(b, 1, stream);
+}
+
+int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, double *out, void *stream) {
+    if (!b || !out) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_fill_normal, dim3(nblocks(b->num_cells, 256)), dim3(256), 0, S(stream), *b, step, stream_id,
+                       out);
+    return check();
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+/*
+ * cellgpu_amd — C ABI of the MI355X (gfx950) Voronoi / vertex-model time step.
+ *
+ * Two layers, both plain C (pointers, sizes, status codes):
+ *
+ *  1. Kernel-caller layer (cgpu_*). Each entry point replaces one family of the
+ *     reference's kernel callers (abaso/cellGPU, declared in the inc/ .cuh headers) and is
+ *     cited below. Arguments are device pointers gathered in a batch view struct
+ *     (structure-of-arrays, one or many independent systems laid out back to back
+ *     with per-system offset tables) plus a hipStream_t passed as void*. Launches
+ *     are asynchronous, never allocate, never synchronize the host, and can be
+ *     captured into a hipGraph.
+ *
+ *  2. Object layer (cgpu_vor_*, cgpu_vm_*, cgpu_fire_*, cgpu_server_*). Handles
+ *     owned by the C++ classes that keep the reference's Simulation /
+ *     voronoiModelBase / vertexModelBase / updater API; used by drivers and by
+ *     foreign-language bindings.
+ *
+ * Status codes: 0 = ok; >0 recoverable / informational; <0 fatal.
+ */
+#ifndef CELLGPU_AMD_H
+#define CELLGPU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cgpu_status {
+    CGPU_OK = 0,
+    CGPU_ERR_ARGS = -1,
+    CGPU_ERR_HIP = -2,
+    CGPU_ERR_RING_CAPACITY = -3,   /* a Delaunay degree exceeded the ring capacity  */
+    CGPU_ERR_BIN_CAPACITY = -4,    /* a cell-list bin overflowed                    */
+    CGPU_ERR_TOPOLOGY = -5,        /* global triangulation inconsistent / failed    */
+    CGPU_ERR_NONFINITE = -6,       /* a position became NaN / inf                   */
+    CGPU_ERR_NO_DEVICE = -7
+};
+
+/* Maximum Delaunay degree stored per cell (ring capacity, slot-major storage). */
+#define CGPU_RING_CAP 24
+/* Cell-list bin capacity (the reference grows Nmax on the host; here fixed). */
+#define CGPU_BIN_CAP 24
+
+/* Per-system constants of a batch (host fills, device reads). */
+typedef struct cgpu_sys_params {
+    int off;            /* first cell (or vertex) of the system            */
+    int n;              /* number of cells (degrees of freedom)            */
+    int bin_off;        /* first cell-list bin of the system               */
+    int xsize;          /* bins per side (reference cellListGPU::xsize)    */
+    double L;           /* square box side                                 */
+    double boxsize;     /* bin side (L / xsize)                            */
+    double KA, KP;      /* quadratic area / perimeter moduli               */
+    double dt, mu;      /* integration step, mobility                      */
+    double T;           /* Brownian temperature                            */
+    uint64_t seed;      /* counter-based noise key                         */
+    int ncells;         /* vertex model: number of cells (else = n)        */
+    int pad0;
+} cgpu_sys_params;
+
+/* Device-mutable per-system state of the topology machinery. */
+typedef struct cgpu_sys_state {
+    int any_fail;       /* a circumcircle test failed this step            */
+    int nf_count;       /* |NeedsFixing|                                   */
+    int mode;           /* 0 none, 1 local repair, 2 global by count, 3 global fallback */
+    int need_global;    /* local repair must be replaced by a global one   */
+    int nn_delta;       /* change of sum of degrees by local repairs       */
+    int tri_delta;      /* change of owned-triangle count by local repairs */
+    int max_deg;        /* max degree seen by the last global pass         */
+    int neigh_max;      /* reference neighMax (even rule)                  */
+    int step;           /* integrator step counter (noise counter)         */
+    int active;         /* 1 = system advances this step                   */
+    int steps_left;     /* serving: remaining steps (-1 = unlimited)       */
+    int error;          /* sticky cgpu_status for this system              */
+    long long global_fixes, repair_events, local_repairs, t1_events;
+} cgpu_sys_state;
+
+/*
+ * Batch view of a Voronoi-model batch. All pointers are device pointers.
+ * Per-cell arrays have num_cells entries; ring arrays are slot-major:
+ * nbr[k * num_cells + i] is the k-th counter-clockwise Delaunay neighbour of cell i.
+ * 'real' arrays are float when precision == 1 (FP32 bulk) and double when 0.
+ */
+typedef struct cgpu_voronoi_batch {
+    int num_systems, num_cells, total_bins, precision;
+    const cgpu_sys_params *sys;  /* [S] */
+    cgpu_sys_state *st;          /* [S] */
+    const int *cell_sys;         /* [N] system of each cell */
+    double *pos;                 /* [2N] x,y interleaved (current)  */
+    double *pos_next;            /* [2N] x,y interleaved (next)     */
+    int *nn;                     /* [N] Delaunay degree             */
+    int *nbr;                    /* [CAP*N] rings                   */
+    int *oth;                    /* [CAP*N] reference delOther      */
+    void *AP;                    /* [2N] real: area, perimeter      */
+    void *dAP;                   /* [2N] real: KA(A-A0), KP(P-P0)   */
+    const void *APpref;          /* [2N] real: A0, P0               */
+    const void *mot;             /* [2N] real: v0, Dr               */
+    void *vel;                   /* [2N] real: velocity             */
+    void *theta;                 /* [N]  real: director angle       */
+    void *force;                 /* [2N] real: force                */
+    const int *excl;             /* [N] or NULL: exclusions         */
+    void *ext_force;             /* [2N] real or NULL               */
+    unsigned int *bin_count;     /* [total_bins]                    */
+    int *bin_idx;                /* [total_bins*BIN_CAP]            */
+    int *repair;                 /* [N] flag (cleared by consumer)  */
+    int *mark;                   /* [N] NeedsFixing mark            */
+    int *flagged;                /* [N] list of flagged cells       */
+    int *nfq;                    /* [N] NeedsFixing queue           */
+    int *gq;                     /* [N] global-retriangulation queue */
+    int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow, 5 nonfinite */
+    const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
+} cgpu_voronoi_batch;
+
+/* ---------------------------------------------------------------- kernel layer */
+
+/* Replaces gpu_compute_cell_list + gpu_zero_array (reference
+ * inc/utility/cellListGPU.cuh:18-38; src/utility/cellListGPU.cu:20-88).
+ * Bins cur (0) or next (1) positions. */
+int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stream);
+
+/* Replaces gpu_test_circumcenters + gpu_compute_voronoi_geometry (reference
+ * inc/models/voronoiModelBase.cuh:20-48; voronoiModelBase.cu:24-201): FP64
+ * empty-circumcircle test of every Delaunay triangle with the reference's search
+ * window, fused with the Voronoi area/perimeter pass. test=0 only computes geometry. */
+int cgpu_test_and_geometry(const cgpu_voronoi_batch *b, int test, void *stream);
+
+/* Replaces voronoiModelBase::testAndRepairTriangulation / repairTriangulation /
+ * getDelSets / allDelSets and the CGAL calls behind them (reference
+ * voronoiModelBase.cpp:429-505, 600-663, 686-739, 1236-1286; DelaunayLoc.cpp:381-403):
+ * device-side NeedsFixing, N/6 global rule, exact-predicate Delaunay stars,
+ * neighMax and 6N/2N checks, delOther maintenance, geometry of repaired cells. */
+int cgpu_repair_topology(const cgpu_voronoi_batch *b, void *stream);
+
+/* Replaces voronoiModelBase::globalTriangulationCGAL (reference
+ * voronoiModelBase.cpp:281-351; DelaunayCGAL.cpp:59-65,164-229) for every system
+ * of the batch: bins current positions, computes all Delaunay stars, delOther and
+ * geometry. */
+int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream);
+
+/* Replaces gpu_force_sets + gpu_sum_force_sets(_with_exclusions) (reference
+ * inc/models/voronoiQuadraticEnergy.cuh:20-54; voronoiQuadraticEnergy.cu:30-234):
+ * per-cell force of the quadratic area-perimeter energy, fused over the ring. */
+int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream);
+
+/* Replaces gpu_spp_eom_integration + gpu_move_degrees_of_freedom (reference
+ * inc/updaters/selfPropelledParticleDynamics.cuh:17-27, selfPropelledParticleDynamics.cu:20-57;
+ * inc/models/Simple2DCell.cuh:22-40, Simple2DCell.cu:21-58), fused with the
+ * force pass: F, then disp = dt(v + mu F), v <- v0 e(theta),
+ * theta <- atan2(v) + sqrt(2 dt Dr) eta, x_next = wrap(x + disp). */
+int cgpu_spv_force_eom_move(const cgpu_voronoi_batch *b, void *stream);
+
+/* One complete self-propelled Voronoi step of every active system
+ * (Simulation::performTimestep, reference src/Simulation.cpp:141-166). */
+int cgpu_spv_step(const cgpu_voronoi_batch *b, void *stream);
+
+/* Brownian step of every active system (brownianParticleDynamics GPU path,
+ * reference brownianParticleDynamics.cpp:68-87). */
+int cgpu_brownian_step(const cgpu_voronoi_batch *b, void *stream);
+
+/* Default work-group size of the per-cell kernels (multiple of 64, <= 1024). */
+int cgpu_set_block_size(int block);
+
+/* Replaces initialize_RNG_array_kernel / curand draws (reference
+ * noiseSource.cu:21-29): fills out[i] with the N(0,1) deviate of cell i
+ * at (seed of its system, step, stream). */
+int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, double *out, void *stream);
+
+/* Brownian update of cells (reference brownianParticleDynamics.cu:20-39),
+ * fused with the force pass and the move. */
+int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream);
+
+/* FIRE pieces (reference EnergyMinimizerFIRE2D.cu:16-68, utilities.cu:28-35, 134-249).
+ * fire_state is a device array of cgpu_fire_state[S]. */
+typedef struct cgpu_fire_state {
+    double alpha, alpha_start, alpha_dec, dt, dt_max, dt_inc, dt_dec, force_cutoff;
+    double power, force_norm, vel_norm, force_max, scaling, max_f2;
+    int nmin, nsince, iterations, max_iterations;
+    int converged, zero_v, pad0, pad1;
+} cgpu_fire_state;
+int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel_half, void *stream);
+int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, double *partials, void *stream);
+
+/* ---------------------------------------------------------------- object layer */
+
+/* A batch of independent Voronoi systems (one system = the reference's
+ * VoronoiQuadraticEnergy; many = an ensemble launched together).
+ * n[s], L[s]: cells and box side; x,y,theta: concatenated host arrays (may be
+ * NULL: reference initial condition from mt19937(seed[s])). precision 0 = FP64,
+ * 1 = FP32 bulk with FP64 predicates. */
+void *cgpu_vor_create(int num_systems, const int *n, const double *L, const uint64_t *seed,
+                      const double *x, const double *y, const double *theta, int precision, int device);
+void cgpu_vor_destroy(void *h);
+int cgpu_vor_set_params(void *h, int s, double KA, double KP, double dt, double mu, double T);
+int cgpu_vor_set_prefs(void *h, const double *A0, const double *P0);       /* per cell, all systems */
+int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr);    /* per cell; resets v   */
+int cgpu_vor_set_exclusions(void *h, const int *ex);
+int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retriangulate);
+int cgpu_vor_set_noise_mode(void *h, int use_buffer);
+int cgpu_vor_upload_noise(void *h, const double *eta);                      /* N doubles, this step */
+int cgpu_vor_compute_forces(void *h);
+int cgpu_vor_spv_steps(void *h, int steps, int use_graph);
+int cgpu_vor_brownian_steps(void *h, int steps);
+int cgpu_vor_sync(void *h);
+int cgpu_vor_get(void *h, double *x, double *y, double *theta, double *vx, double *vy, double *Fx,
+                 double *Fy, double *A, double *P);
+int cgpu_vor_get_topology(void *h, int *nn, int *nbr, int cap, int *neigh_max);
+int cgpu_vor_get_counters(void *h, int s, long long *out);
+int cgpu_vor_num_cells(void *h);
+double cgpu_vor_energy(void *h, int s);
+int cgpu_vor_status(void *h);
+int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
+int cgpu_vor_autotune(void *h);
+double cgpu_vor_last_kernel_ms(void *h, const char *name);
+int cgpu_vor_view(void *h, cgpu_voronoi_batch *out);
+void *cgpu_vor_stream(void *h);
+
+/* FIRE minimization of every system of a Voronoi batch (EnergyMinimizerFIRE,
+ * reference EnergyMinimizerFIRE2D.cpp:269-284, GPU branch). */
+void *cgpu_fire_create(void *vor);
+void cgpu_fire_destroy(void *h);
+int cgpu_fire_set(void *h, int max_it, double force_cutoff, double alpha_start, double dt, double dt_max,
+                  double dt_inc, double dt_dec, double alpha_dec, int nmin);
+int cgpu_fire_minimize(void *h);
+int cgpu_fire_get(void *h, int s, double *out /* force_max, power, alpha, dt, max_f2, iterations */);
+
+int cgpu_device_count(void);
+const char *cgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CELLGPU_AMD_H */

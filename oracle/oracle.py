@@ -1,0 +1,216 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of ``liboracle.so``, the CPU FP64 restatement of the reference
+(abaso/cellGPU) Voronoi/vertex time step. Only ``tests/``, the smoke check and
+``bench.py``'s CPU-baseline leg import this module; the product never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` or __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        sig = {
+            "orc_orient0": (C.c_int, [C.c_double] * 4),
+            "orc_incircle0": (C.c_int, [C.c_double] * 6),
+            "orc_circumcenter0": (None, [C.c_double] * 4 + [_dp]),
+            "orc_min_dist": (None, [C.c_double] * 5 + [_dp]),
+            "orc_put_in_box": (None, [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "orc_random_init": (None, [C.c_int, C.c_uint, C.c_double, _dp, _dp, _dp]),
+            "orc_triangulate": (C.c_int, [C.c_int, C.c_double, _dp, _dp, C.c_int, _ip, _ip]),
+            "orc_vsys_create": (vp, [C.c_int, C.c_double, _dp, _dp, _dp]),
+            "orc_vsys_destroy": (None, [vp]),
+            "orc_vsys_set_params": (None, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "orc_vsys_set_prefs": (None, [vp, _dp, _dp]),
+            "orc_vsys_set_motility": (None, [vp, _dp, _dp]),
+            "orc_vsys_set_exclusions": (None, [vp, _ip]),
+            "orc_vsys_set_state": (None, [vp, _dp, _dp, vp, vp, vp]),
+            "orc_vsys_set_topology": (None, [vp, _ip, _ip, C.c_int, C.c_int]),
+            "orc_vsys_get": (C.c_int, [vp] + [vp] * 9),
+            "orc_vsys_get_topology": (C.c_int, [vp, _ip, _ip, C.c_int]),
+            "orc_vsys_get_counters": (None, [vp, np.ctypeslib.ndpointer(dtype=np.int64)]),
+            "orc_vsys_compute_forces": (C.c_double, [vp, C.c_int]),
+            "orc_vsys_test_flags": (C.c_int, [vp, _ip]),
+            "orc_vsys_enforce_topology": (C.c_int, [vp, C.c_int]),
+            "orc_vsys_spp_step": (C.c_int, [vp, _dp, C.c_int]),
+            "orc_vsys_brownian_step": (C.c_int, [vp, _dp, _dp, C.c_double]),
+            "orc_fire_create": (vp, [vp]),
+            "orc_fire_destroy": (None, [vp]),
+            "orc_fire_set": (None, [vp, C.c_int] + [C.c_double] * 7 + [C.c_int]),
+            "orc_fire_minimize": (C.c_int, [vp, vp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def orient0(a, b):
+    return lib().orc_orient0(a[0], a[1], b[0], b[1])
+
+
+def incircle0(b, c, d):
+    return lib().orc_incircle0(b[0], b[1], c[0], c[1], d[0], d[1])
+
+
+def circumcenter0(x1, x2):
+    out = np.zeros(2)
+    lib().orc_circumcenter0(x1[0], x1[1], x2[0], x2[1], out)
+    return out
+
+
+def min_dist(L, p1, p2):
+    out = np.zeros(2)
+    lib().orc_min_dist(L, p1[0], p1[1], p2[0], p2[1], out)
+    return out
+
+
+def put_in_box(L, x, y):
+    cx, cy = C.c_double(x), C.c_double(y)
+    lib().orc_put_in_box(L, C.byref(cx), C.byref(cy))
+    return cx.value, cy.value
+
+
+def random_init(N, seed=13377, L=None):
+    """Reference initial condition: mt19937(seed), x,y per cell then directors."""
+    if L is None:
+        L = float(np.sqrt(N))
+    x, y, th = np.zeros(N), np.zeros(N), np.zeros(N)
+    lib().orc_random_init(N, seed, L, x, y, th)
+    return x, y, th, L
+
+
+def triangulate(x, y, L, cap=32):
+    N = len(x)
+    nn = np.zeros(N, np.int32)
+    nbr = np.zeros(N * cap, np.int32)
+    nm = lib().orc_triangulate(N, L, np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64), cap, nn, nbr)
+    if nm < 0:
+        raise RuntimeError("oracle triangulation failed (%d)" % nm)
+    return nn, nbr.reshape(N, cap), nm
+
+
+class VoronoiSystem:
+    """FP64 oracle of one periodic Voronoi monolayer (reference voronoiModelBase +
+    VoronoiQuadraticEnergy + selfPropelledParticleDynamics semantics)."""
+
+    def __init__(self, x, y, theta, L, A0=1.0, P0=3.8, KA=1.0, KP=1.0, dt=0.01, mu=1.0, v0=0.05, Dr=1.0):
+        self.N = len(x)
+        self.L = float(L)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        th = np.ascontiguousarray(theta, np.float64)
+        self.h = lib().orc_vsys_create(self.N, self.L, x, y, th)
+        if not self.h:
+            raise RuntimeError("oracle: initial triangulation failed")
+        lib().orc_vsys_set_params(self.h, KA, KP, dt, mu)
+        self.set_prefs(A0, P0)
+        self.set_motility(v0, Dr)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_vsys_destroy(self.h)
+            self.h = None
+
+    def _full(self, v):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.N,)))
+
+    def set_prefs(self, A0, P0):
+        lib().orc_vsys_set_prefs(self.h, self._full(A0), self._full(P0))
+
+    def set_motility(self, v0, Dr):
+        lib().orc_vsys_set_motility(self.h, self._full(v0), self._full(Dr))
+
+    def set_exclusions(self, ex):
+        lib().orc_vsys_set_exclusions(self.h, np.ascontiguousarray(ex, np.int32))
+
+    def set_state(self, x, y, theta=None, vx=None, vy=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        keep = [np.ascontiguousarray(a, np.float64) if a is not None else None for a in (theta, vx, vy)]
+        lib().orc_vsys_set_state(self.h, x, y, _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]))
+
+    def set_topology(self, nn, nbr, neigh_max):
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        cap = nbr.shape[1]
+        lib().orc_vsys_set_topology(self.h, np.ascontiguousarray(nn, np.int32), nbr.reshape(-1), cap, int(neigh_max))
+
+    def get(self):
+        N = self.N
+        a = {k: np.zeros(N) for k in ("x", "y", "theta", "vx", "vy", "Fx", "Fy", "A", "P")}
+        lib().orc_vsys_get(self.h, *[_ptr(a[k]) for k in ("x", "y", "theta", "vx", "vy", "Fx", "Fy", "A", "P")])
+        return a
+
+    def topology(self, cap=32):
+        nn = np.zeros(self.N, np.int32)
+        nbr = np.zeros(self.N * cap, np.int32)
+        nm = lib().orc_vsys_get_topology(self.h, nn, nbr, cap)
+        return nn, nbr.reshape(self.N, cap), nm
+
+    def counters(self):
+        out = np.zeros(6, np.int64)
+        lib().orc_vsys_get_counters(self.h, out)
+        return dict(zip(["global_fixes", "local_repairs", "repair_events", "skipped", "last_mode", "last_nf"], out.tolist()))
+
+    def compute_forces(self, gpu_style=True):
+        return lib().orc_vsys_compute_forces(self.h, 0 if gpu_style else 1)
+
+    def test_flags(self):
+        r = np.zeros(self.N, np.int32)
+        any_ = lib().orc_vsys_test_flags(self.h, r)
+        return bool(any_), r
+
+    def enforce_topology(self, gpu_path=True):
+        if lib().orc_vsys_enforce_topology(self.h, 1 if gpu_path else 0) != 0:
+            raise RuntimeError("oracle: topology repair failed")
+
+    def spp_step(self, eta, gpu_path=True):
+        if lib().orc_vsys_spp_step(self.h, np.ascontiguousarray(eta, np.float64), 1 if gpu_path else 0) != 0:
+            raise RuntimeError("oracle: topology repair failed")
+
+    def brownian_step(self, etax, etay, T):
+        r = lib().orc_vsys_brownian_step(self.h, np.ascontiguousarray(etax, np.float64), np.ascontiguousarray(etay, np.float64), T)
+        if r != 0:
+            raise RuntimeError("oracle: topology repair failed")
+
+
+class FireMinimizer:
+    """Oracle of EnergyMinimizerFIRE (GPU branch, survey note N2)."""
+
+    def __init__(self, sys, max_it=1000, force_cutoff=1e-7, alpha_start=0.99, dt=0.01, dt_max=0.1,
+                 dt_inc=1.05, dt_dec=0.95, alpha_dec=0.9, n_min=5):
+        self.sys = sys
+        self.h = lib().orc_fire_create(sys.h)
+        lib().orc_fire_set(self.h, max_it, force_cutoff, alpha_start, dt, dt_max, dt_inc, dt_dec, alpha_dec, n_min)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_fire_destroy(self.h)
+            self.h = None
+
+    def minimize(self):
+        out = np.zeros(5)
+        it = lib().orc_fire_minimize(self.h, self.sys.h, out)
+        if it < 0:
+            raise RuntimeError("oracle: topology repair failed during FIRE")
+        return it, dict(force_max=out[0], power=out[1], alpha=out[2], dt=out[3], max_f2=out[4])

@@ -1,0 +1,128 @@
+"""GPU parity of the Voronoi time step against the FP64 oracle.
+
+Bars: Delaunay rings bit-exact (canonical rotation), FP64 forces / positions
+within 1e-12 relative, FP32-bulk forces within 1e-5 of FP64 (max-norm), and
+step-by-step topology parity (same positions -> same repaired rings).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CAP = 24
+
+
+def _cg():
+    import cellgpu_amd as cg
+    cg.require_device()
+    return cg
+
+
+def _rings(nn, nbr):
+    return [tuple(nbr[i, : nn[i]]) for i in range(len(nn))]
+
+
+def _oracle_from_gpu(O, b, s=0, P0=3.8, v0=0.05, Dr=1.0, dt=0.01):
+    g = b.get()
+    sl = b.system_slice(s)
+    nn, nbr, nm = b.topology(CAP)
+    off = sl.start
+    o = O.VoronoiSystem(g["x"][sl], g["y"][sl], g["theta"][sl], b.L[s], P0=P0, dt=dt, v0=v0, Dr=Dr)
+    o.set_state(g["x"][sl], g["y"][sl], g["theta"][sl], g["vx"][sl], g["vy"][sl])
+    nbr_local = np.where(nbr[sl] >= 0, nbr[sl] - off, -1).astype(np.int32)
+    o.set_topology(nn[sl], nbr_local, int(nm[s]))
+    return o
+
+
+@pytest.mark.parametrize("N", [64, 300, 1000])
+def test_initial_triangulation_bit_exact(oracle, N):
+    cg = _cg()
+    b = cg.VoronoiBatch([N], seeds=[13377], precision="fp64")
+    x, y, th, L = oracle.random_init(N, 13377)
+    g = b.get()
+    assert np.array_equal(g["x"], x) and np.array_equal(g["y"], y) and np.array_equal(g["theta"], th)
+    nn, nbr, nm = b.topology(CAP)
+    onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert int(nm[0]) == onm
+    assert nn.sum() == 6 * N
+
+
+@pytest.mark.parametrize("N", [100, 1000])
+def test_fp64_forces_match_oracle(oracle, N):
+    cg = _cg()
+    b = cg.VoronoiBatch([N], seeds=[7], precision="fp64")
+    b.set_prefs(1.0, 3.75)
+    b.compute_forces()
+    g = b.get()
+    o = _oracle_from_gpu(oracle, b, P0=3.75)
+    E = o.compute_forces(gpu_style=True)
+    r = o.get()
+    np.testing.assert_allclose(g["A"], r["A"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["P"], r["P"], rtol=1e-12, atol=0)
+    scale = np.abs(np.concatenate([r["Fx"], r["Fy"]])).max()
+    assert np.abs(g["Fx"] - r["Fx"]).max() <= 1e-12 * scale
+    assert np.abs(g["Fy"] - r["Fy"]).max() <= 1e-12 * scale
+    assert abs(b.energy(0) - E) <= 1e-12 * abs(E)
+
+
+def test_fp32_forces_within_1e5(oracle):
+    cg = _cg()
+    N = 2000
+    b64 = cg.VoronoiBatch([N], seeds=[11], precision="fp64")
+    b32 = cg.VoronoiBatch([N], seeds=[11], precision="fp32")
+    for b in (b64, b32):
+        b.set_prefs(1.0, 3.8)
+        b.compute_forces()
+    g64, g32 = b64.get(), b32.get()
+    F64 = np.concatenate([g64["Fx"], g64["Fy"]])
+    F32 = np.concatenate([g32["Fx"], g32["Fy"]])
+    err = np.abs(F32 - F64).max() / np.abs(F64).max()
+    assert err <= 1e-5, err
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_spv_steps_match_oracle_stepwise(oracle, precision):
+    """Every step: oracle restarted from the GPU state, same noise, then compare."""
+    cg = _cg()
+    N = 400
+    b = cg.VoronoiBatch([N], seeds=[3], precision=precision)
+    b.set_prefs(1.0, 3.8)
+    b.set_motility(0.1, 1.0)
+    b.set_params(dt=0.01)
+    b.set_noise_buffer(True)
+    rng = np.random.default_rng(5)
+    repairs = 0
+    for step in range(60):
+        o = _oracle_from_gpu(oracle, b, P0=3.8, v0=0.1, Dr=1.0, dt=0.01)
+        eta = rng.standard_normal(N)
+        b.upload_noise(eta)
+        b.spv_steps(1, use_graph=False)
+        b.check_status()
+        o.spp_step(eta, gpu_path=True)
+        g, r = b.get(), o.get()
+        np.testing.assert_allclose(g["x"], r["x"], rtol=0, atol=1e-12 * b.L[0])
+        np.testing.assert_allclose(g["y"], r["y"], rtol=0, atol=1e-12 * b.L[0])
+        nn, nbr, nm = b.topology(CAP)
+        onn, onbr, onm = o.topology(CAP)
+        assert _rings(nn, nbr) == _rings(onn, onbr), "step %d" % step
+        repairs += o.counters()["repair_events"]
+    assert repairs > 0
+
+
+def test_batch_equals_single_systems():
+    """A system gives the same result alone and inside a batch (FP64)."""
+    cg = _cg()
+    ns = [64, 150, 333]
+    seeds = [1, 2, 3]
+    batch = cg.VoronoiBatch(ns, seeds=seeds, precision="fp64")
+    batch.set_motility(0.1, 1.0)
+    batch.spv_steps(20)
+    gb = batch.get()
+    for s, (n, sd) in enumerate(zip(ns, seeds)):
+        one = cg.VoronoiBatch([n], seeds=[sd], precision="fp64")
+        one.set_motility(0.1, 1.0)
+        one.spv_steps(20)
+        g1 = one.get()
+        sl = batch.system_slice(s)
+        assert np.array_equal(gb["x"][sl], g1["x"]) and np.array_equal(gb["y"][sl], g1["y"])
