@@ -1,0 +1,210 @@
+"""Benchmark: batched self-propelled Voronoi ensemble on one MI355X.
+
+Workload (the ensemble the headline metric is quoted on): 512 independent
+periodic monolayers, N log-uniform in [64, 8192] at unit density, P0 ~ U[3.5, 4],
+v0 ~ U[0.01, 0.2], Dr ~ U[0.1, 10], dt = 0.01, A0 = 1, KA = KP = 1; FP32 bulk
+arithmetic with FP64 Delaunay predicates; one hipGraph-captured batched step
+advances every system. A step = one time step of all 512 systems.
+
+Prints ONE JSON line. Besides value (aggregate cell-steps/s) it reports the
+plain one-system-at-a-time FP64 GPU path on a sample of the same systems, the
+FP32-vs-FP64 force error, the roofline of the dominant kernel (HIP events on the
+batch stream) and the oracle's CPU path timed on the host (bounded sample).
+
+usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s batched vs plain FP64 on same GPU at 1e-5 force err; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic (compulsory) bytes per cell for one launch, FP32 bulk, FP64 positions,
+# mean Delaunay degree 6 (Euler): neighbour gathers are served on chip.
+BYTES_FORCE_EOM_MOVE = 16 + 4 + 24 + 24 + 8 + 8 + 8 + 4 + 4 + 16 + 8 + 8 + 4  # = 136
+BYTES_TEST_GEOMETRY = 16 + 4 + 24 + 4 + 8 + 8 + 8  # = 72
+BYTES_CELL_LIST = 16 + 4 + 4 + 8  # = 32
+
+
+def ensemble(S=512, master_seed=20240607):
+    rng = np.random.default_rng(master_seed)
+    n = np.rint(np.exp(rng.uniform(np.log(64), np.log(8192), S))).astype(np.int32)
+    P0 = rng.uniform(3.5, 4.0, S)
+    v0 = rng.uniform(0.01, 0.2, S)
+    Dr = rng.uniform(0.1, 10.0, S)
+    seeds = rng.integers(1, 2**31 - 1, S).astype(np.uint64)
+    return n, P0, v0, Dr, seeds
+
+
+def per_cell(n, v):
+    return np.repeat(np.asarray(v, np.float64), n)
+
+
+def make_batch(cg, n, P0, v0, Dr, seeds, precision):
+    b = cg.VoronoiBatch(n, seeds=seeds, precision=precision)
+    b.set_params(KA=1.0, KP=1.0, dt=0.01, mu=1.0)
+    b.set_prefs(1.0, per_cell(n, P0))
+    b.set_motility(per_cell(n, v0), per_cell(n, Dr))
+    return b
+
+
+def force_error(cg, n, P0, v0, Dr, seeds, b32):
+    b64 = make_batch(cg, n, P0, v0, Dr, seeds, "fp64")
+    b64.compute_forces()
+    b32.compute_forces()
+    g64, g32 = b64.get(), b32.get()
+    b64.close()
+    F64 = np.concatenate([g64["Fx"], g64["Fy"]])
+    F32 = np.concatenate([g32["Fx"], g32["Fy"]])
+    return float(np.abs(F32 - F64).max() / np.abs(F64).max())
+
+
+def plain_fp64(cg, n, P0, v0, Dr, seeds, sample_every=8, steps=20):
+    """Reference-structured plain path: one FP64 system at a time, host sync per step."""
+    cells = 0
+    t = 0.0
+    for s in range(0, len(n), sample_every):
+        b = make_batch(cg, n[s:s + 1], P0[s:s + 1], v0[s:s + 1], Dr[s:s + 1], seeds[s:s + 1], "fp64")
+        b.spv_steps(2, use_graph=False)
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.spv_steps(1, use_graph=False)
+            b.status()  # the reference reads the "any circumcircle failed" flag back every step
+        t += time.perf_counter() - t0
+        cells += int(n[s]) * steps
+        b.close()
+    return cells / t, "every %dth system (%d systems), %d steps each, per-step host sync" % (
+        sample_every, len(range(0, len(n), sample_every)), steps)
+
+
+def cpu_baseline(n, P0, v0, Dr, seeds, budget_s=12.0, steps=10):
+    """Oracle restatement of the reference CPU path (local test-and-repair), 1 thread."""
+    from oracle import oracle as O
+    cells = 0
+    t = 0.0
+    used = 0
+    rng = np.random.default_rng(7)
+    for s in range(len(n)):
+        if t > budget_s:
+            break
+        x, y, th, L = O.random_init(int(n[s]), int(seeds[s]))
+        o = O.VoronoiSystem(x, y, th, L, P0=P0[s], v0=v0[s], Dr=Dr[s], dt=0.01)
+        etas = rng.standard_normal((steps, int(n[s])))
+        t0 = time.perf_counter()
+        for k in range(steps):
+            o.spp_step(etas[k], gpu_path=False)
+        t += time.perf_counter() - t0
+        cells += int(n[s]) * steps
+        used += 1
+    return dict(value=cells / t, unit="cell-steps/s", cores=1, kind="port",
+                sample="first %d systems of the ensemble, %d steps each, oracle CPU path (local repair), 1 thread"
+                       % (used, steps))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--timed-stage-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every system of the ensemble runs on a single MI355X "
+              "(multi-GPU sharding of an ensemble is by system; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cellgpu_amd as cg
+    cg.require_device()
+    n, P0, v0, Dr, seeds = ensemble()
+    N = int(n.sum())
+    b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
+    ferr = force_error(cg, n, P0, v0, Dr, seeds, b)
+    print("[bench] ensemble: %d systems, %d cells; fp32-vs-fp64 force error %.2e" % (len(n), N, ferr), flush=True)
+
+    b.spv_steps(args.warmup, use_graph=True)
+    b.sync()
+    b.check_status()
+    cg.lib().cgpu_device_synchronize()
+    t0 = time.perf_counter()
+    b.spv_steps(args.steps, use_graph=True)
+    b.sync()
+    cg.lib().cgpu_device_synchronize()
+    t1 = time.perf_counter()
+    b.check_status()
+    elapsed = t1 - t0
+    value = N * args.steps / elapsed
+    print("[bench] timed %d steps in %.3f s" % (args.steps, elapsed), flush=True)
+
+    # per-stage durations with HIP events on the batch stream (no graph)
+    ks = max(1, min(args.timed_stage_steps, args.steps))
+    ms = np.zeros(4)
+    import ctypes as C
+    r = cg.lib().cgpu_vor_spv_steps_timed(b.h, ks, ms.ctypes.data_as(C.c_void_p))
+    cg.check(r, "timed steps")
+    b.check_status()
+    avg_ms = ms / ks
+    stage = {"force_eom_move": avg_ms[0], "cell_list": avg_ms[1], "test_geometry": avg_ms[2], "repair_chain": avg_ms[3]}
+    bytes_map = {"force_eom_move": BYTES_FORCE_EOM_MOVE, "test_geometry": BYTES_TEST_GEOMETRY}
+    dom = max(bytes_map, key=lambda k: stage[k])
+    achieved = bytes_map[dom] * N / (stage[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": {"force_eom_move": "k_force_eom_move<float,0>",
+                                           "test_geometry": "k_test_geometry<float>"}[dom],
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_cell": bytes_map[dom], "cells_per_launch": N,
+                "avg_launch_ms": round(stage[dom], 5)}
+    others = {k: round(bytes_map[k] * N / (stage[k] * 1e-3) / 1e9, 1) for k in bytes_map}
+
+    plain_val, plain_desc = (None, None)
+    if not args.no_plain:
+        plain_val, plain_desc = plain_fp64(cg, n, P0, v0, Dr, seeds)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, P0, v0, Dr, seeds)
+
+    per_step_ms = 1e3 * elapsed / args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(per_step_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "dtype_detail": "f32 geometry/forces/integrator, f64 positions, f64 Delaunay predicates and cell list",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched Voronoi SPV ensemble: 512 systems, N log-uniform [64,8192], unit density, "
+                        "P0~U[3.5,4], v0~U[0.01,0.2], Dr~U[0.1,10], dt=0.01",
+            "systems": int(len(n)), "cells": N, "precision": "fp32-bulk/fp64-predicates",
+            "graph": True,
+        },
+        "plain_fp64_value": None if plain_val is None else round(plain_val, 1),
+        "plain_fp64_sample": plain_desc,
+        "ratio_vs_plain_fp64": None if plain_val is None else round(value / plain_val, 2),
+        "force_rel_err_vs_fp64": ferr,
+        "stage_ms": {k: round(v, 5) for k, v in stage.items()},
+        "stage_achieved_gbs": others,
+        "latency_ms": {"batch_completion_p50": round(1e3 * elapsed, 2), "batch_completion_p99": round(1e3 * elapsed, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,13 @@ def lib():
         "cgpu_brownian_step": (i, [P(VoronoiBatchView), vp]),
         "cgpu_brownian_force_eom_move": (i, [P(VoronoiBatchView), vp]),
         "cgpu_fill_normal": (i, [P(VoronoiBatchView), i, i, vp, vp]),
+        "cgpu_step_begin": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_step_end": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_device_synchronize": (i, []),
+        "cgpu_vor_spv_steps_timed": (i, [vp, i, vp]),
+        "cgpu_philox4x32_10": (None, [vp, vp, vp]),
+        "cgpu_normal_pair": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp]),
+        "cgpu_put_in_box": (None, [d, P(d), P(d)]),
     }
     optional = {
         "cgpu_fire_create": (vp, [vp]),

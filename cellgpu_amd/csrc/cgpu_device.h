@@ -34,12 +34,21 @@ CGPU_HD dvec2 min_dist(const Box &b, dvec2 p1, dvec2 p2) {
     for (int t = 0; t < WRAP_MAX && a.y > b.half; ++t) a.y -= b.L;
     return a;
 }
+// Fold a unit-square coordinate into [0,1) with the same rounding as the
+// reference loops "while(v<0) v+=1; while(v>=1) v-=1" (gpubox.h:139-152), in
+// closed form: every +-1 step but the last one that crosses zero is exact
+// (|v| < 2^52), so the loops reduce to one exact integer shift plus that step.
+CGPU_HD double fold_unit(double v) {
+    if (v < 0) {
+        double k = ceil(-v) - 1.0;  // exact steps that keep v negative
+        v += k;
+        if (v < 0) v += 1.0;  // the rounded step
+    }
+    if (v >= 1.0) v -= floor(v);
+    return v;
+}
 CGPU_HD void put_in_box(const Box &b, dvec2 &p) {
-    double vx = b.inv * p.x, vy = b.inv * p.y;
-    for (int t = 0; t < WRAP_MAX && vx < 0; ++t) vx += 1.0;
-    for (int t = 0; t < WRAP_MAX && vy < 0; ++t) vy += 1.0;
-    for (int t = 0; t < WRAP_MAX && vx >= 1.0; ++t) vx -= 1.0;
-    for (int t = 0; t < WRAP_MAX && vy >= 1.0; ++t) vy -= 1.0;
+    double vx = fold_unit(b.inv * p.x), vy = fold_unit(b.inv * p.y);
     p.x = b.L * vx;
     p.y = b.L * vy;
 }
@@ -132,13 +141,13 @@ CGPU_HD int cross(double ax, double ay, double bx, double by, double *e) {
     n = grow(n, e, -r);
     return n;
 }
-__host__ __device__ __noinline__ int orient_exact(double ax, double ay, double bx, double by) {
+static __host__ __device__ __noinline__ int orient_exact(double ax, double ay, double bx, double by) {
     double e[8];
     int n = cross(ax, ay, bx, by, e);
     return sign_of(n, e);
 }
 // sign of  |b|^2 (c x d) + |c|^2 (d x b) + |d|^2 (b x c)
-__host__ __device__ __noinline__ int incircle_exact(double bx, double by, double cx, double cy, double dx,
+static __host__ __device__ __noinline__ int incircle_exact(double bx, double by, double cx, double cy, double dx,
                                                      double dy) {
     double acc[200];
     int n = 0;
@@ -156,7 +165,7 @@ __host__ __device__ __noinline__ int incircle_exact(double bx, double by, double
     for (int j = 0; j < nc; ++j) n = add_scaled(n, acc, nrm, nn, crs[j]);
     return sign_of(n, acc);
 }
-__host__ __device__ __noinline__ int cmp_norm2_exact(double ax, double ay, double bx, double by) {
+static __host__ __device__ __noinline__ int cmp_norm2_exact(double ax, double ay, double bx, double by) {
     double acc[16];
     int n = norm2(ax, ay, acc);
     double p, q;

@@ -364,6 +364,40 @@ int VoronoiBatch::spv_steps(int steps, bool use_graph) {
     return CGPU_OK;
 }
 
+int VoronoiBatch::spv_steps_timed(int steps, double *ms_out) {
+    hipEvent_t ev[5];
+    for (auto &e : ev) hip_check(hipEventCreate(&e), "event");
+    double acc[4] = {0, 0, 0, 0};
+    int r = CGPU_OK;
+    for (int k = 0; k < steps && r == CGPU_OK; ++k) {
+        cgpu_voronoi_batch b = view();
+        cgpu_voronoi_batch bn = b;  // stages after the move work on the new positions
+        bn.pos = b.pos_next;
+        bn.pos_next = b.pos;
+        r = cgpu_step_begin(&b, stream_);
+        hipEventRecord(ev[0], stream_);
+        if (!r) r = cgpu_spv_force_eom_move(&b, stream_);
+        hipEventRecord(ev[1], stream_);
+        if (!r) r = cgpu_compute_cell_list(&b, 1, stream_);
+        hipEventRecord(ev[2], stream_);
+        if (!r) r = cgpu_test_and_geometry(&bn, 1, stream_);
+        hipEventRecord(ev[3], stream_);
+        if (!r) r = cgpu_repair_topology(&bn, stream_);
+        hipEventRecord(ev[4], stream_);
+        if (!r) r = cgpu_step_end(&b, stream_);
+        parity_ ^= 1;
+        hip_check(hipEventSynchronize(ev[4]), "event sync");
+        for (int q = 0; q < 4; ++q) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+            acc[q] += ms;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int q = 0; q < 4; ++q) ms_out[q] = acc[q];
+    return r;
+}
+
 int VoronoiBatch::brownian_steps(int steps) {
     for (int k = 0; k < steps; ++k) {
         int r = step_launch(1);
@@ -495,6 +529,11 @@ extern "C" {
 
 const char *cgpu_last_error(void) { return g_last_error.c_str(); }
 
+int cgpu_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
+int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out) {
+    return guarded([&] { return ((VoronoiBatch *)h)->spv_steps_timed(steps, ms_out); });
+}
+
 int cgpu_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -575,3 +614,21 @@ int cgpu_vor_view(void *h, cgpu_voronoi_batch *out) {
 void *cgpu_vor_stream(void *h) { return (void *)((VoronoiBatch *)h)->stream(); }
 
 }  // extern "C"
+
+// host copies of the counter-based generator (same code as the device path)
+#include "../cgpu_device.h"
+extern "C" void cgpu_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    cgpu::philox4x32_10(c, key[0], key[1]);
+    for (int q = 0; q < 4; ++q) out[q] = c[q];
+}
+extern "C" void cgpu_normal_pair(uint64_t seed, uint32_t index, uint32_t step, uint32_t stream, double *out) {
+    cgpu::normal_pair(seed, index, step, stream, out[0], out[1]);
+}
+extern "C" void cgpu_put_in_box(double L, double *x, double *y) {
+    cgpu::Box b = cgpu::make_box(L);
+    cgpu::dvec2 p{*x, *y};
+    cgpu::put_in_box(b, p);
+    *x = p.x;
+    *y = p.y;
+}

@@ -72,6 +72,7 @@ class VoronoiBatch {
     int global_triangulation();
     int compute_forces();
     int spv_steps(int steps, bool use_graph);
+    int spv_steps_timed(int steps, double *ms_out);
     int brownian_steps(int steps);
     int sync();
     int status();

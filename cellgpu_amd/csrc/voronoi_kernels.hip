@@ -72,6 +72,64 @@ __device__ inline void wave_append(bool pred, int value, int *list, int *counter
     }
 }
 
+// ------------------------------------------------- precision-dispatched pieces
+// FP64 path: the reference expressions, evaluated exactly as written.
+// FP32 path: the same expressions in float, except that a circumcenter (or the
+// Voronoi-vertex Jacobian) of an ill-conditioned triangle — cross product below
+// 1e-2 of |a|_1 |b|_1, i.e. an angle under about half a degree — is evaluated in
+// FP64 from the FP64 displacements and rounded once.
+template <typename R> struct Prec;
+template <> struct Prec<double> {
+    static __device__ inline dvec2 c0(dvec2 a, dvec2 b) { return circ0<double>(a, b); }
+    static __device__ inline dvec2 c3(dvec2 a, dvec2 b, dvec2 c) { return circ3<double>(a, b, c); }
+};
+template <> struct Prec<float> {
+    static __device__ inline bool well(float cr, vec2<float> a, vec2<float> b) {
+        float sc = (fabsf(a.x) + fabsf(a.y)) * (fabsf(b.x) + fabsf(b.y));
+        return fabsf(cr) > 1e-2f * sc;
+    }
+    static __device__ inline vec2<float> c0(dvec2 A, dvec2 B) {
+        vec2<float> a = cvt<float>(A), b = cvt<float>(B);
+        if (well(a.x * b.y - a.y * b.x, a, b)) return circ0<float>(a, b);
+        return cvt<float>(circ0<double>(A, B));
+    }
+    static __device__ inline vec2<float> c3(dvec2 A, dvec2 B, dvec2 C) {
+        vec2<float> a = cvt<float>(A), b = cvt<float>(B), c = cvt<float>(C);
+        vec2<float> u{a.x - c.x, a.y - c.y}, v{b.x - c.x, b.y - c.y};
+        if (well(u.x * v.y - u.y * v.x, u, v)) return circ3<float>(a, b, c);
+        return cvt<float>(circ3<double>(A, B, C));
+    }
+};
+
+// reference getdhdr (functions.h:234-242): d(Voronoi vertex)/d(r_i) for the
+// vertex of (i, j, k) with rij, rik relative to i; m = (x11, x12, x21, x22)
+template <typename R> __device__ inline void dhdr_eval(vec2<R> rij, vec2<R> rik, R m[4]) {
+    R c = rij.y * rik.x - rij.x * rik.y;
+    R den = R(1.0) / (c * c);
+    R rik2 = rik.x * rik.x + rik.y * rik.y;
+    R rij2 = rij.x * rij.x + rij.y * rij.y;
+    m[0] = R(0.5) * den * (rik.y - rij.y) * (rik.y * rij2 - rij.y * rik2);
+    m[1] = R(0.5) * den * (rij.y - rik.y) * (rik.x * rij2 - rij.x * rik2);
+    m[2] = R(0.5) * den * (rij.x - rik.x) * (rik.y * rij2 - rij.y * rik2);
+    m[3] = R(0.5) * den * (rik.x - rij.x) * (rik.x * rij2 - rij.x * rik2);
+}
+template <typename R> __device__ inline void dhdr(dvec2 rij, dvec2 rik, R m[4]);
+template <> __device__ inline void dhdr<double>(dvec2 rij, dvec2 rik, double m[4]) { dhdr_eval<double>(rij, rik, m); }
+template <> __device__ inline void dhdr<float>(dvec2 rij, dvec2 rik, float m[4]) {
+    vec2<float> a = cvt<float>(rij), b = cvt<float>(rik);
+    if (Prec<float>::well(a.y * b.x - a.x * b.y, a, b)) {
+        dhdr_eval<float>(a, b, m);
+    } else {
+        double md[4];
+        dhdr_eval<double>(rij, rik, md);
+        for (int q = 0; q < 4; ++q) m[q] = (float)md[q];
+    }
+}
+// reference float builds clamp edge lengths at THRESHOLD = 1e-18
+// (voronoiQuadraticEnergy.cu:172-177, 201-204); double builds do not
+template <typename R> __device__ inline R clamp_len(R x) { return x; }
+template <> __device__ inline float clamp_len<float>(float x) { return x < 1e-18f ? 1e-18f : x; }
+
 // ------------------------------------------------------------------ geometry
 // Voronoi area and perimeter of cell i in the reference GPU order
 // (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
@@ -82,14 +140,14 @@ __device__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, in
     const int z = b.nn[i];
     if (z < 3) return;
     dvec2 pi = ldpos(pos, i);
-    vec2<R> rij = cvt<R>(min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi));
-    vec2<R> rik = cvt<R>(min_dist(box, ldpos(pos, b.nbr[i]), pi));
-    vec2<R> vfirst = circ0<R>(rij, rik), vlast = vfirst;
+    dvec2 rij = min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi);
+    dvec2 rik = min_dist(box, ldpos(pos, b.nbr[i]), pi);
+    vec2<R> vfirst = Prec<R>::c0(rij, rik), vlast = vfirst;
     R A = R(0), P = R(0);
     for (int k = 1; k < z; ++k) {
         rij = rik;
-        rik = cvt<R>(min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi));
-        vec2<R> vnext = circ0<R>(rij, rik);
+        rik = min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi);
+        vec2<R> vnext = Prec<R>::c0(rij, rik);
         A += tri_area<R>(vlast, vnext);
         R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
         P += sqrt(dx * dx + dy * dy);
@@ -116,35 +174,28 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, in
     if (z < 3) return vec2<R>{R(0), R(0)};
     const dvec2 pi = ldpos(pos, i);
     auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
-    auto dd = [&](int id) { return cvt<R>(min_dist(box, ldpos(pos, id), pi)); };
+    auto dd = [&](int id) { return min_dist(box, ldpos(pos, id), pi); };
     int jB = rk(-2), jC = rk(-1);
-    vec2<R> dA = dd(rk(-3)), dB = dd(jB), dC = dd(jC);
+    dvec2 dA = dd(rk(-3)), dB = dd(jB), dC = dd(jC);
     vec2<R> qB = ld2<R>(b.dAP, jB), qC = ld2<R>(b.dAP, jC);
-    vec2<R> vL = circ0<R>(dA, dB), vM = circ0<R>(dB, dC);
+    vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
     const vec2<R> qI = ld2<R>(b.dAP, i);
     vec2<R> sum{R(0), R(0)};
     for (int k = 0; k < z; ++k) {
         int jD = b.nbr[k * N + i];
-        vec2<R> dD = dd(jD);
-        vec2<R> vN = circ0<R>(dC, dD);
-        vec2<R> dO = dd(b.oth[k * N + i]);
-        // --- reference force set for slot k
-        const vec2<R> dnext = dB, dcl = dC;
-        R c = dnext.y * dcl.x - dnext.x * dcl.y;
-        R den = R(1.0) / (c * c);
-        R rik2 = dcl.x * dcl.x + dcl.y * dcl.y;
-        R rij2 = dnext.x * dnext.x + dnext.y * dnext.y;
-        R m11 = R(0.5) * den * (dcl.y - dnext.y) * (dcl.y * rij2 - dnext.y * rik2);
-        R m12 = R(0.5) * den * (dnext.y - dcl.y) * (dcl.x * rij2 - dnext.x * rik2);
-        R m21 = R(0.5) * den * (dnext.x - dcl.x) * (dcl.y * rij2 - dnext.y * rik2);
-        R m22 = R(0.5) * den * (dcl.x - dnext.x) * (dcl.x * rij2 - dnext.x * rik2);
+        dvec2 dD = dd(jD);
+        vec2<R> vN = Prec<R>::c0(dC, dD);
+        dvec2 dO = dd(b.oth[k * N + i]);
+        // --- reference force set for slot k: (nm1, n1) = (ring k-2, ring k-1)
+        R m[4];
+        dhdr<R>(dB, dC, m);
         const vec2<R> vcur = vM, vlast = vL, vnext = vN;
-        vec2<R> vother = circ3<R>(dnext, dcl, dO);
+        vec2<R> vother = Prec<R>::c3(dB, dC, dO);
         vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
         vec2<R> dl{vlast.x - vcur.x, vlast.y - vcur.y};
-        R dlnorm = sqrt(dl.x * dl.x + dl.y * dl.y);
+        R dlnorm = clamp_len<R>(sqrt(dl.x * dl.x + dl.y * dl.y));
         vec2<R> dn{vcur.x - vnext.x, vcur.y - vnext.y};
-        R dnnorm = sqrt(dn.x * dn.x + dn.y * dn.y);
+        R dnnorm = clamp_len<R>(sqrt(dn.x * dn.x + dn.y * dn.y));
         const vec2<R> dclv = dl, dncv = dn;
         const R dclnorm = dlnorm, dncnorm = dnnorm;
         vec2<R> dPdv{dl.x / dlnorm - dn.x / dnnorm, dl.y / dlnorm - dn.y / dnnorm};
@@ -152,7 +203,7 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, in
         // cell ring k-1
         dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
         dn = vec2<R>{vcur.x - vother.x, vcur.y - vother.y};
-        dnnorm = sqrt(dn.x * dn.x + dn.y * dn.y);
+        dnnorm = clamp_len<R>(sqrt(dn.x * dn.x + dn.y * dn.y));
         dPdv = vec2<R>{-dncv.x / dncnorm - dn.x / dnnorm, -dncv.y / dncnorm - dn.y / dnnorm};
         dEdv.x += qC.x * dAdv.x + qC.y * dPdv.x;
         dEdv.y += qC.x * dAdv.y + qC.y * dPdv.y;
@@ -163,8 +214,8 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, in
         dEdv.y += qB.x * dAdv.y + qB.y * dPdv.y;
         dEdv.x *= R(2.0);
         dEdv.y *= R(2.0);
-        sum.x += dEdv.x * m11 + dEdv.y * m21;
-        sum.y += dEdv.x * m12 + dEdv.y * m22;
+        sum.x += dEdv.x * m[0] + dEdv.y * m[2];
+        sum.y += dEdv.x * m[1] + dEdv.y * m[3];
         // --- advance the window
         dB = dC;
         dC = dD;
@@ -386,6 +437,9 @@ __global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int tes
             double wc = ceil(rad / sp.boxsize);
             int wcheck = (wc < (double)(xs / 2)) ? (int)wc : xs / 2;
             rad = rad * rad;
+            // wrapped distances are at most L/sqrt(2): a wider circle contains
+            // every candidate, exactly as the reference's wrap loops decide
+            const bool all_inside = rad > 0.5 * sp.L * sp.L * (1.0 + 1e-9);
             for (int ii = ib - wcheck; ii <= ib + wcheck; ++ii) {
                 int cx = ii;
                 if (cx < 0) cx += xs;
@@ -398,9 +452,13 @@ __global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int tes
                     int cnt = min((int)b.bin_count[bin], BCAP);
                     for (int q = 0; q < cnt; ++q) {
                         int m = b.bin_idx[bin * BCAP + q];
-                        dvec2 a = min_dist(box, ldpos(pos, m), v);
-                        dvec2 c = min_dist(box, a, Q);
-                        if (c.x * c.x + c.y * c.y < rad) {
+                        bool inside = all_inside;
+                        if (!inside) {
+                            dvec2 a = min_dist(box, ldpos(pos, m), v);
+                            dvec2 c = min_dist(box, a, Q);
+                            inside = c.x * c.x + c.y * c.y < rad;
+                        }
+                        if (inside) {
                             if (m != i && m != j && m != l) {
                                 bad = true;
                                 flag_cell(b, m);
@@ -663,8 +721,9 @@ __global__ void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double
     }
     p.x += (double)disp.x;
     p.y += (double)disp.y;
-    // a non-finite or runaway displacement (more than a box length) is an error
-    if (!(isfinite(p.x) && isfinite(p.y)) || fabs((double)disp.x) > sp.L || fabs((double)disp.y) > sp.L) {
+    // a non-finite (or absurdly large) position is an error; any other finite
+    // displacement is wrapped back into the box as in the reference
+    if (!(fabs(p.x) < 1e12 * sp.L && fabs(p.y) < 1e12 * sp.L)) {
         b.counters[5] = 1;
         st.error = CGPU_ERR_NONFINITE;
         p = ldpos(pos, i);
@@ -693,6 +752,154 @@ __global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double
     normal_pair(sp.seed, (uint32_t)(i - sp.off), (uint32_t)step, (uint32_t)stream, z0, z1);
     out[i] = z0;
     out[(size_t)b.num_cells + i] = z1;
+}
+
+
+// ------------------------------------------------------------------------ FIRE
+// reference EnergyMinimizerFIRE GPU branch (EnergyMinimizerFIRE2D.cpp:93-114,
+// 156-207; kernels EnergyMinimizerFIRE2D.cu:16-68). One iteration:
+//   [previous mixing] disp = dt v + dt^2/2 F, v += dt/2 F, move, enforce topology,
+//   F(new), v += dt/2 F, P = F.v, |F|^2, |v|^2 (FP64), v = (1-a) v + a sqrt(vv/ff) F,
+//   dt / alpha update or v = 0.
+template <typename R> __device__ inline void fire_mix(vec2<R> &v, vec2<R> F, const cgpu_fire_state &fs) {
+    if (!fs.pending) return;
+    if (fs.zero_v) {
+        v = vec2<R>{R(0), R(0)};
+        return;
+    }
+    R a = (R)fs.mix_alpha, sc = (R)fs.scaling;
+    v.x = (R(1) - a) * v.x + a * sc * F.x;
+    v.y = (R(1) - a) * v.y + a * sc * F.y;
+}
+
+template <typename R>
+__global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
+                               double *pos_next) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    const cgpu_fire_state &f = fs[s];
+    dvec2 p = ldpos(pos, i);
+    vec2<R> v = ld2<R>(vel, i), F = ld2<R>(b.force, i);
+    fire_mix<R>(v, F, f);
+    if (f.converged || !b.st[s].active) {
+        st2<R>(vel, i, v);
+        stpos(pos_next, i, p);
+        return;
+    }
+    const R dt = (R)f.dt;
+    vec2<R> d{dt * v.x + R(0.5) * dt * dt * F.x, dt * v.y + R(0.5) * dt * dt * F.y};
+    v.x += R(0.5) * dt * F.x;
+    v.y += R(0.5) * dt * F.y;
+    st2<R>(vel, i, v);
+    p.x += (double)d.x;
+    p.y += (double)d.y;
+    if (!(fabs(p.x) < 1e12 * sp.L && fabs(p.y) < 1e12 * sp.L)) {
+        b.counters[5] = 1;
+        b.st[s].error = CGPU_ERR_NONFINITE;
+        p = ldpos(pos, i);
+    }
+    put_in_box(make_box(sp.L), p);
+    stpos(pos_next, i, p);
+}
+
+__global__ void k_fire_clear_pending(cgpu_fire_state *fs, int S) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) fs[s].pending = 0;
+}
+
+template <typename R>
+__global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
+                                 double *dots) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int s = b.cell_sys[i];
+    const cgpu_fire_state &f = fs[s];
+    if (f.converged || !b.st[s].active) return;
+    vec2<R> F = force_with_exclusions<R>(b, pos, i, b.sys[s]);
+    st2<R>(b.force, i, F);
+    vec2<R> v = ld2<R>(vel, i);
+    const R dt = (R)f.dt;
+    v.x += R(0.5) * dt * F.x;
+    v.y += R(0.5) * dt * F.y;
+    st2<R>(vel, i, v);
+    double fx = F.x, fy = F.y, vx = v.x, vy = v.y;
+    const int N = b.num_cells;
+    dots[i] = fx * fx + fy * fy;
+    dots[N + i] = fx * vx + fy * vy;
+    dots[2 * N + i] = vx * vx + vy * vy;
+}
+
+// one block per system: deterministic FP64 sums, then the FIRE scalar update
+__global__ void k_fire_reduce(cgpu_voronoi_batch b, cgpu_fire_state *fs, const double *dots) {
+    const int s = blockIdx.x;
+    cgpu_fire_state &f = fs[s];
+    if (f.converged || !b.st[s].active) return;
+    const cgpu_sys_params &sp = b.sys[s];
+    const int N = b.num_cells;
+    __shared__ double sh[4][256];
+    double a = 0, c = 0, d = 0, m = 0;
+    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) {
+        int i = sp.off + t;
+        double ff = dots[i];
+        a += ff;
+        c += dots[N + i];
+        d += dots[2 * N + i];
+        m = fmax(m, ff);
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = c;
+    sh[2][threadIdx.x] = d;
+    sh[3][threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+            sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double forceNorm = sh[0][0], power = sh[1][0], velNorm = sh[2][0];
+        f.iterations += 1;
+        f.power = power;
+        f.force_max = forceNorm / (double)sp.n;
+        f.max_f2 = sh[3][0];
+        f.scaling = forceNorm > 0. ? sqrt(velNorm / forceNorm) : 0.0;
+        f.mix_alpha = f.alpha;
+        f.pending = 1;
+        if (power > 0) {
+            f.zero_v = 0;
+            if (f.nsince > f.nmin) {
+                f.dt = fmin(f.dt * f.dt_inc, f.dt_max);
+                f.alpha = f.alpha * f.alpha_dec;
+            }
+            f.nsince += 1;
+        } else {
+            f.dt = f.dt * f.dt_dec;
+            f.alpha = f.alpha_start;
+            f.zero_v = 1;
+        }
+        if (!(f.iterations < f.max_iterations && sqrt(f.force_max) > f.force_cutoff)) f.converged = 1;
+    }
+}
+
+__global__ void k_fire_park(cgpu_voronoi_batch b, const cgpu_fire_state *fs) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < b.num_systems && fs[s].converged) b.st[s].active = 0;
+}
+
+template <typename R>
+__global__ void k_fire_apply_mix(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const cgpu_fire_state &f = fs[b.cell_sys[i]];
+    vec2<R> v = ld2<R>(vel, i), F = ld2<R>(b.force, i);
+    fire_mix<R>(v, F, f);
+    st2<R>(vel, i, v);
 }
 
 inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
@@ -821,6 +1028,17 @@ static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
     return check();
 }
 
+int cgpu_step_begin(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), *b);
+    return check();
+}
+int cgpu_step_end(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_advance_step, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), *b);
+    return check();
+}
+
 int cgpu_spv_step(const cgpu_voronoi_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     return step_common(b, 0, stream);
@@ -837,4 +1055,46 @@ int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, doubl
     return check();
 }
 
+int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
+    if (!b || !fs || !vel) return CGPU_ERR_ARGS;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next);
+    else
+        hipLaunchKernelGGL(k_fire_vv_move<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next);
+    hipLaunchKernelGGL(k_fire_clear_pending, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
+                       b->num_systems);
+    return check();
+}
+
+int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
+    if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+    else
+        hipLaunchKernelGGL(k_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+    hipLaunchKernelGGL(k_fire_reduce, dim3(b->num_systems), dim3(256), 0, S(stream), *b, fs, dots);
+    return check();
+}
+
+int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state *fs, void *stream) {
+    if (!b || !fs) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_fire_park, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), *b, fs);
+    return check();
+}
+
+int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
+    if (!b || !fs || !vel) return CGPU_ERR_ARGS;
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_fire_apply_mix<float>, g, t, 0, S(stream), *b, fs, vel);
+    else
+        hipLaunchKernelGGL(k_fire_apply_mix<double>, g, t, 0, S(stream), *b, fs, vel);
+    hipLaunchKernelGGL(k_fire_clear_pending, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
+                       b->num_systems);
+    return check();
+}
+
 }  // extern "C"
+

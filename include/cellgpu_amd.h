@@ -150,6 +150,13 @@ int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream);
  * theta <- atan2(v) + sqrt(2 dt Dr) eta, x_next = wrap(x + disp). */
 int cgpu_spv_force_eom_move(const cgpu_voronoi_batch *b, void *stream);
 
+/* Per-step bookkeeping kernels: reset of the per-system topology state and
+ * queues (begin) and the step counter / serving countdown (end). A step is
+ * begin, force_eom_move, cell list (next), test_and_geometry and
+ * repair_topology on the next positions, end. */
+int cgpu_step_begin(const cgpu_voronoi_batch *b, void *stream);
+int cgpu_step_end(const cgpu_voronoi_batch *b, void *stream);
+
 /* One complete self-propelled Voronoi step of every active system
  * (Simulation::performTimestep, reference src/Simulation.cpp:141-166). */
 int cgpu_spv_step(const cgpu_voronoi_batch *b, void *stream);
@@ -171,15 +178,25 @@ int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, doubl
 int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream);
 
 /* FIRE pieces (reference EnergyMinimizerFIRE2D.cu:16-68, utilities.cu:28-35, 134-249).
- * fire_state is a device array of cgpu_fire_state[S]. */
+ * fs is a device array cgpu_fire_state[S]; vel a real [2N] velocity array;
+ * dots a double [3N] scratch array (f.f, f.v, v.v per cell). */
 typedef struct cgpu_fire_state {
     double alpha, alpha_start, alpha_dec, dt, dt_max, dt_inc, dt_dec, force_cutoff;
-    double power, force_norm, vel_norm, force_max, scaling, max_f2;
+    double power, force_max, scaling, mix_alpha, max_f2, pad_d;
     int nmin, nsince, iterations, max_iterations;
-    int converged, zero_v, pad0, pad1;
+    int converged, zero_v, pending, pad_i;
 } cgpu_fire_state;
-int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel_half, void *stream);
-int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, double *partials, void *stream);
+/* velocity-Verlet first half + move (gpu_displacement_vv, gpu_update_velocity,
+ * moveDegreesOfFreedom), after applying the previous iteration's FIRE mixing */
+int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
+/* forces on the moved positions, second velocity half step and the three
+ * FP64 dot products, then per-system reduction and the FIRE update of alpha,
+ * dt and the stopping test (fireStepGPU) */
+int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
+/* mark systems whose FIRE loop has stopped as inactive for the topology kernels */
+int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state *fs, void *stream);
+/* apply a pending FIRE velocity mixing (end of minimize) */
+int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
 /* ---------------------------------------------------------------- object layer */
 
@@ -211,7 +228,6 @@ double cgpu_vor_energy(void *h, int s);
 int cgpu_vor_status(void *h);
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
 int cgpu_vor_autotune(void *h);
-double cgpu_vor_last_kernel_ms(void *h, const char *name);
 int cgpu_vor_view(void *h, cgpu_voronoi_batch *out);
 void *cgpu_vor_stream(void *h);
 
@@ -223,8 +239,19 @@ int cgpu_fire_set(void *h, int max_it, double force_cutoff, double alpha_start, 
                   double dt_inc, double dt_dec, double alpha_dec, int nmin);
 int cgpu_fire_minimize(void *h);
 int cgpu_fire_get(void *h, int s, double *out /* force_max, power, alpha, dt, max_f2, iterations */);
+/* position-only view of the vertex engine is separate; FIRE drives Voronoi batches */
 
 int cgpu_device_count(void);
+int cgpu_device_synchronize(void);
+/* Steps with HIP events around each stage (no graph); ms_out[0..3] receive the
+ * summed durations of force/EOM/move, cell list, test+geometry, repair chain. */
+int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out);
+/* Host-side Philox4x32-10 and the normal pair of (seed, index, step, stream),
+ * identical to the device generator; for known-answer tests. */
+void cgpu_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+void cgpu_normal_pair(uint64_t seed, uint32_t index, uint32_t step, uint32_t stream, double *out);
+/* Host copy of the device periodic wrap (reference gpubox::putInBoxReal). */
+void cgpu_put_in_box(double L, double *x, double *y);
 const char *cgpu_last_error(void);
 
 #ifdef __cplusplus

@@ -47,12 +47,15 @@ struct Box {
         for (int t = 0; t < 64 && a.y > half; ++t) a.y -= L;
         return a;
     }
+    // reference loops; iteration count bounded by |v| + 2 for finite v
+    static inline double wrapUnit(double v) {
+        long long guard = 0;
+        while (v < 0 && guard++ < (1LL << 40)) v += 1.0;
+        while (v >= 1.0 && guard++ < (1LL << 40)) v -= 1.0;
+        return v;
+    }
     inline void putInBoxReal(d2 &p) const {
-        double vx = inv * p.x, vy = inv * p.y;
-        for (int t = 0; t < 64 && vx < 0; ++t) vx += 1.0;
-        for (int t = 0; t < 64 && vy < 0; ++t) vy += 1.0;
-        for (int t = 0; t < 64 && vx >= 1.0; ++t) vx -= 1.0;
-        for (int t = 0; t < 64 && vy >= 1.; ++t) vy -= 1.;
+        double vx = wrapUnit(inv * p.x), vy = wrapUnit(inv * p.y);
         p.x = L * vx;
         p.y = L * vy;
     }
@@ -323,6 +326,8 @@ static bool testTriangulationGPUStyle(VSys &s, vector<int> &repair) {
             double wc = std::ceil(rad / cl.boxsize);
             int wcheck = (wc < (double)(xs / 2)) ? (int)wc : xs / 2;
             rad = rad * rad;
+            // a circle wider than the box: every wrapped distance is inside
+            bool all_inside = rad > 0.5 * s.box.L * s.box.L * (1.0 + 1e-9);
             for (int ii = ib - wcheck; ii <= ib + wcheck; ++ii)
                 for (int jj = jb - wcheck; jj <= jb + wcheck; ++jj) {
                     int cx = ii;
@@ -332,9 +337,13 @@ static bool testTriangulationGPUStyle(VSys &s, vector<int> &repair) {
                     if (cy < 0) cy += xs;
                     if (cy >= xs) cy -= xs;
                     for (int m : cl.bins[(size_t)cy * xs + cx]) {
-                        d2 a = s.box.minDist(s.P[m], v);
-                        d2 b = s.box.minDist(a, Q);
-                        if (b.x * b.x + b.y * b.y < rad) {
+                        bool inside = all_inside;
+                        if (!inside) {
+                            d2 a = s.box.minDist(s.P[m], v);
+                            d2 b = s.box.minDist(a, Q);
+                            inside = b.x * b.x + b.y * b.y < rad;
+                        }
+                        if (inside) {
                             if (m != i && m != j && m != l) {
                                 bad = true;
                                 repair[m] = 1;
@@ -736,7 +745,8 @@ int orc_triangulate(int N, double L, const double *x, const double *y, int cap, 
     s.P.resize(N);
     for (int i = 0; i < N; ++i) s.P[i] = d2{x[i], y[i]};
     if (!globalTriangulation(s)) return -1;
-    if (s.cap > cap) return -2;
+    for (int i = 0; i < N; ++i)
+        if (s.nn[i] > cap) return -2;
     for (int i = 0; i < N; ++i) {
         nn[i] = s.nn[i];
         for (int k = 0; k < s.nn[i]; ++k) nbr[(size_t)i * cap + k] = s.ring(i, k);

@@ -126,3 +126,48 @@ def test_batch_equals_single_systems():
         g1 = one.get()
         sl = batch.system_slice(s)
         assert np.array_equal(gb["x"][sl], g1["x"]) and np.array_equal(gb["y"][sl], g1["y"])
+
+
+def test_fire_matches_oracle_fp64(oracle):
+    """GPU FIRE (per-system scalars, FP64) against the oracle's GPU-branch FIRE."""
+    cg = _cg()
+    N = 300
+    b = cg.VoronoiBatch([N], seeds=[17], precision="fp64")
+    b.set_prefs(1.0, 3.7)
+    b.set_motility(0.0, 0.0)
+    b.spv_steps(40, use_graph=False)  # leave the random start: no near-degenerate triangles
+    o = _oracle_from_gpu(oracle, b, P0=3.7, v0=0.0, Dr=0.0)
+    f = cg.EnergyMinimizerFIRE(b)
+    f.setMaximumIterations(60)
+    f.setForceCutoff(1e-12)
+    f.setAlphaStart(0.1)
+    f.setDeltaT(0.01)
+    f.minimize()
+    st = f.state()
+    fo = oracle.FireMinimizer(o, max_it=60, force_cutoff=1e-12, alpha_start=0.1, dt=0.01)
+    it, so = fo.minimize()
+    assert st["iterations"] == it == 60
+    g, r = b.get(), o.get()
+    # reductions run in a different order (tree vs sequential): 1e-10 over 60 iterations
+    assert np.abs(g["x"] - r["x"]).max() < 1e-10 * b.L[0]
+    assert abs(st["force_max"] - so["force_max"]) <= 1e-8 * so["force_max"]
+    assert abs(st["dt"] - so["dt"]) <= 1e-12 and abs(st["alpha"] - so["alpha"]) <= 1e-12
+
+
+def test_fire_batch_converges_independently():
+    cg = _cg()
+    b = cg.VoronoiBatch([100, 400], seeds=[1, 2], precision="fp64")
+    b.set_prefs(1.0, np.concatenate([np.full(100, 3.6), np.full(400, 3.9)]))
+    b.set_motility(0.0, 0.0)
+    f = cg.EnergyMinimizerFIRE(b)
+    f.setMaximumIterations(5000)
+    f.setForceCutoff(1e-6)
+    f.setAlphaStart(0.1)
+    f.minimize()
+    for s in range(2):
+        st = f.state(s)
+        assert st["force_max"] ** 0.5 < 1e-6 and st["iterations"] < 5000
+    b.compute_forces()
+    g = b.get()
+    rms0 = np.sqrt(np.mean(g["Fx"][:100] ** 2 + g["Fy"][:100] ** 2))
+    assert rms0 < 1e-6
