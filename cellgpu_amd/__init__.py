@@ -13,7 +13,7 @@ import numpy as np
 from ._lib import CellGPUError, check, lib, require_device, VoronoiBatchView  # noqa: F401
 
 __all__ = [
-    "CellGPUError", "VoronoiBatch", "VoronoiQuadraticEnergy", "Simulation", "selfPropelledParticleDynamics",
+    "CellGPUError", "VoronoiBatch", "VertexBatch", "VertexQuadraticEnergy", "VoronoiQuadraticEnergy", "Simulation", "selfPropelledParticleDynamics",
     "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
 ]
 
@@ -388,3 +388,114 @@ class EnergyMinimizerFIRE:
 
     def getMaxForce(self, s=0):
         return self.state(s)["force_max"]
+
+
+class VertexBatch:
+    """A device-resident batch of vertex-model monolayers, each built from the
+    Voronoi tessellation of its cell positions (reference
+    vertexModelBase::setCellsVoronoiTesselation). Cells: ncells per system;
+    positions from x, y (concatenated) or the reference random start (seeds)."""
+
+    VCAP = 32
+
+    def __init__(self, ncells, L=None, seeds=None, x=None, y=None, precision="fp32", device=0):
+        require_device()
+        nc = np.atleast_1d(np.asarray(ncells, dtype=np.int32))
+        self.S = len(nc)
+        self.Nc = int(nc.sum())
+        Ls = None if L is None else _f64(np.broadcast_to(np.asarray(L, np.float64), (self.S,)))
+        self.L = np.sqrt(nc.astype(np.float64)) if Ls is None else Ls
+        sd = None if seeds is None else np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, np.uint64), (self.S,)))
+        xs = None if x is None else _f64(x)
+        ys = None if y is None else _f64(y)
+        self._keep = (nc, Ls, sd, xs, ys)
+        self.h = lib().cgpu_vm_create(self.S, _p(nc), _p(Ls), _p(sd), _p(xs), _p(ys), 1 if precision == "fp32" else 0,
+                                      device)
+        if not self.h:
+            raise CellGPUError("cgpu_vm_create failed")
+        self.Nv = lib().cgpu_vm_num_vertices(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cgpu_vm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, KA=1.0, KP=1.0, dt=0.005, mu=1.0, T=0.0, t1_threshold=0.04, system=-1):
+        check(lib().cgpu_vm_set_params(self.h, system, KA, KP, dt, mu, T, t1_threshold), "vm_set_params")
+
+    def set_prefs(self, A0, P0):
+        a, p = _f64(A0, self.Nc), _f64(P0, self.Nc)
+        check(lib().cgpu_vm_set_prefs(self.h, _p(a), _p(p)), "vm_set_prefs")
+
+    def set_positions(self, vx, vy):
+        a, b = _f64(vx, self.Nv), _f64(vy, self.Nv)
+        check(lib().cgpu_vm_set_positions(self.h, _p(a), _p(b)), "vm_set_positions")
+
+    def set_noise_buffer(self, on=True):
+        check(lib().cgpu_vm_set_noise_mode(self.h, 1 if on else 0), "vm_set_noise_mode")
+
+    def upload_noise(self, etax, etay):
+        buf = np.concatenate([_f64(etax, self.Nv), _f64(etay, self.Nv)])
+        check(lib().cgpu_vm_upload_noise(self.h, _p(buf)), "vm_upload_noise")
+
+    def compute_forces(self):
+        check(lib().cgpu_vm_compute_forces(self.h), "vm_compute_forces")
+
+    def brownian_steps(self, steps, use_graph=True):
+        check(lib().cgpu_vm_brownian_steps(self.h, int(steps), 1 if use_graph else 0), "vm_brownian_steps")
+
+    def check_status(self):
+        check(lib().cgpu_vm_status(self.h), "vertex batch status")
+
+    def get(self):
+        a = {k: np.zeros(self.Nv) for k in ("vx", "vy", "Fx", "Fy")}
+        a.update({k: np.zeros(self.Nc) for k in ("A", "P")})
+        check(lib().cgpu_vm_get(self.h, *[_p(a[k]) for k in ("vx", "vy", "Fx", "Fy", "A", "P")]), "vm_get")
+        return a
+
+    def topology(self):
+        vn = np.zeros(3 * self.Nv, np.int32)
+        vcn = np.zeros(3 * self.Nv, np.int32)
+        cvn = np.zeros(self.Nc, np.int32)
+        cv = np.full(self.Nc * self.VCAP, -1, np.int32)
+        check(lib().cgpu_vm_get_topology(self.h, _p(vn), _p(vcn), _p(cvn), _p(cv), self.VCAP), "vm_get_topology")
+        return vn, vcn, cvn, cv.reshape(self.Nc, self.VCAP)
+
+    def counters(self, s=0):
+        out = np.zeros(4, np.int64)
+        check(lib().cgpu_vm_get_counters(self.h, s, _p(out)), "vm_get_counters")
+        return dict(zip(["t1_events", "step", "error", "nv"], out.tolist()))
+
+
+class VertexQuadraticEnergy:
+    """reference inc/models/vertexQuadraticEnergy.h:16-36 (GPU branch), one system."""
+
+    def __init__(self, n, A0=1.0, P0=3.9, reprod=True, precision="fp64", seed=13377, device=0):
+        self.Ncells = int(n)
+        self.batch = VertexBatch([n], seeds=[seed], precision=precision, device=device)
+        self.Nvertices = self.batch.Nv
+        self.KA = self.KP = 1.0
+        self.deltaT = 0.005
+        self.T1Threshold = 0.01  # reference initializeVertexModelBase default
+        self.setCellPreferencesUniform(A0, P0)
+
+    def setCellPreferencesUniform(self, A0, P0):
+        self.batch.set_prefs(np.full(self.Ncells, A0), np.full(self.Ncells, P0))
+
+    def setT1Threshold(self, t):
+        self.T1Threshold = t
+
+    def computeForces(self):
+        self.batch.compute_forces()
+
+    def computeEnergy(self):
+        self.batch.compute_forces()
+        g = self.batch.get()
+        return float(np.sum(self.KA * (g["A"] - 1.0) ** 2 + self.KP * (g["P"] - 3.9) ** 2))
+
+    def returnForces(self):
+        g = self.batch.get()
+        return np.stack([g["Fx"], g["Fy"]], 1)

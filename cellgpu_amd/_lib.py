@@ -97,7 +97,20 @@ def lib():
         "cgpu_fire_minimize": (i, [vp]),
         "cgpu_fire_get": (i, [vp, i, vp]),
         "cgpu_vor_kernel_times": (i, [vp, i, vp, vp, i]),
-        "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, vp, vp, vp, i, i]),
+        "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, i, i]),
+        "cgpu_vm_destroy": (None, [vp]),
+        "cgpu_vm_num_vertices": (i, [vp]),
+        "cgpu_vm_set_params": (i, [vp, i, d, d, d, d, d, d]),
+        "cgpu_vm_set_prefs": (i, [vp, vp, vp]),
+        "cgpu_vm_set_positions": (i, [vp, vp, vp]),
+        "cgpu_vm_set_noise_mode": (i, [vp, i]),
+        "cgpu_vm_upload_noise": (i, [vp, vp]),
+        "cgpu_vm_compute_forces": (i, [vp]),
+        "cgpu_vm_brownian_steps": (i, [vp, i, i]),
+        "cgpu_vm_get": (i, [vp] + [vp] * 6),
+        "cgpu_vm_get_topology": (i, [vp, vp, vp, vp, vp, i]),
+        "cgpu_vm_get_counters": (i, [vp, i, vp]),
+        "cgpu_vm_status": (i, [vp]),
     }
     for name, (res, args) in list(sig.items()) + list(optional.items()):
         try:

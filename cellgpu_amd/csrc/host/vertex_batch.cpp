@@ -1,0 +1,412 @@
+// VertexBatch: device-resident batch of vertex-model systems (reference
+// vertexModelBase + VertexQuadraticEnergy + brownianParticleDynamics, GPU
+// branch), built from the Voronoi tessellation of cell positions.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../cgpu_device.h"
+#include "voronoi_batch.hpp"
+
+namespace cgpu_host {
+
+class VertexBatch {
+   public:
+    VertexBatch(int S, const int *ncells, const double *L, const uint64_t *seed, const double *x, const double *y,
+                int precision, int device)
+        : S_(S), prec_(precision ? 1 : 0) {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        rsz_ = prec_ ? sizeof(float) : sizeof(double);
+        // Delaunay rings of the cell positions, computed on the device in FP64
+        std::vector<double> Ls(S);
+        for (int s = 0; s < S; ++s) Ls[s] = L ? L[s] : std::sqrt((double)ncells[s]);
+        VoronoiBatch vor(S, ncells, Ls.data(), seed, x, y, nullptr, 0, device);
+        const int Nc = vor.num_cells();
+        std::vector<double> px(Nc), py(Nc);
+        vor.get(px.data(), py.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        const int cap = CGPU_RING_CAP;
+        std::vector<int> nn(Nc), nbr((size_t)Nc * cap);
+        vor.get_topology(nn.data(), nbr.data(), cap, nullptr);
+
+        hp_.resize(S);
+        std::vector<int> tri_of((size_t)Nc * cap, -1);
+        std::vector<int> tri_cells;  // 3 per vertex
+        int voff = 0, coff = 0;
+        for (int s = 0; s < S; ++s) {
+            cgpu_vm_params &p = hp_[s];
+            std::memset(&p, 0, sizeof(p));
+            p.coff = coff;
+            p.nc = ncells[s];
+            p.voff = voff;
+            p.L = Ls[s];
+            p.KA = p.KP = 1.0;
+            p.dt = 0.005;
+            p.mu = 1.0;
+            p.T = 0.0;
+            p.thr = 0.04;
+            p.seed = seed ? seed[s] : 13377ull;
+            // vertices: triangles owned by their smallest cell, (cell, slot) order
+            for (int i = coff; i < coff + p.nc; ++i)
+                for (int k = 0; k < nn[i]; ++k) {
+                    int j = nbr[(size_t)i * cap + k], l = nbr[(size_t)i * cap + (k + 1) % nn[i]];
+                    if (i < j && i < l) {
+                        tri_of[(size_t)i * cap + k] = (int)tri_cells.size() / 3;
+                        tri_cells.push_back(i);
+                        tri_cells.push_back(j);
+                        tri_cells.push_back(l);
+                    }
+                }
+            p.nv = (int)tri_cells.size() / 3 - voff;
+            voff += p.nv;
+            coff += p.nc;
+        }
+        Nv_ = voff;
+        Nc_ = coff;
+        auto find_tri = [&](int a, int b, int c) {  // counter-clockwise (a, b, c)
+            int o = std::min(a, std::min(b, c)), p, q;
+            if (o == a) { p = b; q = c; } else if (o == b) { p = c; q = a; } else { p = a; q = b; }
+            for (int k = 0; k < nn[o]; ++k)
+                if (nbr[(size_t)o * cap + k] == p && nbr[(size_t)o * cap + (k + 1) % nn[o]] == q)
+                    return tri_of[(size_t)o * cap + k];
+            return -1;
+        };
+        std::vector<int> vn(3 * (size_t)Nv_), vcn(3 * (size_t)Nv_), vsys(Nv_), csys(Nc_);
+        std::vector<double> vpos(2 * (size_t)Nv_);
+        for (int s = 0; s < S; ++s) {
+            const cgpu_vm_params &p = hp_[s];
+            cgpu::Box box = cgpu::make_box(p.L);
+            for (int t = p.voff; t < p.voff + p.nv; ++t) {
+                vsys[t] = s;
+                int c3[3] = {tri_cells[3 * t], tri_cells[3 * t + 1], tri_cells[3 * t + 2]};
+                cgpu::dvec2 pi{px[c3[0]], py[c3[0]]};
+                cgpu::dvec2 dj = cgpu::min_dist(box, cgpu::dvec2{px[c3[1]], py[c3[1]]}, pi);
+                cgpu::dvec2 dl = cgpu::min_dist(box, cgpu::dvec2{px[c3[2]], py[c3[2]]}, pi);
+                cgpu::dvec2 c = cgpu::circ0<double>(dj, dl);
+                cgpu::dvec2 q{pi.x + c.x, pi.y + c.y};
+                cgpu::put_in_box(box, q);
+                vpos[2 * t] = q.x;
+                vpos[2 * t + 1] = q.y;
+                for (int f = 0; f < 3; ++f) {
+                    vcn[3 * t + f] = c3[f];
+                    int a = c3[(f + 1) % 3], b = c3[(f + 2) % 3], X = -1;
+                    for (int k = 0; k < nn[b]; ++k)
+                        if (nbr[(size_t)b * cap + (k + 1) % nn[b]] == a) X = nbr[(size_t)b * cap + (k + 2) % nn[b]];
+                    vn[3 * t + f] = find_tri(b, a, X);
+                }
+            }
+            for (int c = p.coff; c < p.coff + p.nc; ++c) csys[c] = s;
+        }
+        std::vector<int> cvn(Nc_), cv((size_t)CGPU_VERTEX_CAP * Nc_, -1);
+        for (int i = 0; i < Nc_; ++i) {
+            if (nn[i] > CGPU_VERTEX_CAP) throw std::runtime_error("cell with more vertices than the capacity");
+            cvn[i] = nn[i];
+            for (int k = 0; k < nn[i]; ++k)
+                cv[(size_t)k * Nc_ + i] =
+                    find_tri(i, nbr[(size_t)i * cap + k], nbr[(size_t)i * cap + (k + 1) % nn[i]]);
+        }
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+        sys_.alloc(S);
+        sys_.upload(hp_.data(), S);
+        st_.alloc(S);
+        std::vector<cgpu_sys_state> st(S);
+        std::memset(st.data(), 0, sizeof(cgpu_sys_state) * S);
+        for (auto &q : st) {
+            q.active = 1;
+            q.steps_left = -1;
+        }
+        st_.upload(st.data(), S);
+        vsys_.alloc(Nv_);
+        vsys_.upload(vsys.data(), Nv_);
+        csys_.alloc(Nc_);
+        csys_.upload(csys.data(), Nc_);
+        pos0_.alloc(2 * (size_t)Nv_);
+        pos0_.upload(vpos.data(), 2 * (size_t)Nv_);
+        pos1_.alloc(2 * (size_t)Nv_);
+        vn_.alloc(3 * (size_t)Nv_);
+        vn_.upload(vn.data(), vn.size());
+        vcn_.alloc(3 * (size_t)Nv_);
+        vcn_.upload(vcn.data(), vcn.size());
+        cvn_.alloc(Nc_);
+        cvn_.upload(cvn.data(), Nc_);
+        cv_.alloc(cv.size());
+        cv_.upload(cv.data(), cv.size());
+        AP_.alloc(2 * Nc_ * rsz_);
+        dAP_.alloc(2 * Nc_ * rsz_);
+        APpref_.alloc(2 * Nc_ * rsz_);
+        force_.alloc(2 * Nv_ * rsz_);
+        flag_count_.alloc(S);
+        flag_list_.alloc(6 * (size_t)Nv_);
+        flag_cells_.alloc(12 * (size_t)Nv_);
+        flag_touched_.alloc(12 * (size_t)Nv_);
+        cell_min_.alloc(Nc_);
+        std::vector<unsigned long long> ones(Nc_, ~0ull);
+        cell_min_.upload(ones.data(), Nc_);
+        std::vector<double> a0(Nc_, 1.0), p0(Nc_, 3.9);
+        set_prefs(a0.data(), p0.data());
+    }
+    ~VertexBatch() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    int num_vertices() const { return Nv_; }
+    cgpu_vertex_batch view() const {
+        cgpu_vertex_batch b;
+        std::memset(&b, 0, sizeof(b));
+        b.num_systems = S_;
+        b.num_vertices = Nv_;
+        b.num_cells = Nc_;
+        b.precision = prec_;
+        b.sys = sys_.p;
+        b.st = st_.p;
+        b.vert_sys = vsys_.p;
+        b.cell_sys = csys_.p;
+        b.pos = parity_ ? pos1_.p : pos0_.p;
+        b.pos_next = parity_ ? pos0_.p : pos1_.p;
+        b.vn = vn_.p;
+        b.vcn = vcn_.p;
+        b.cvn = cvn_.p;
+        b.cv = cv_.p;
+        b.AP = AP_.p;
+        b.dAP = dAP_.p;
+        b.APpref = APpref_.p;
+        b.force = force_.p;
+        b.flag_count = flag_count_.p;
+        b.flag_list = flag_list_.p;
+        b.flag_cells = flag_cells_.p;
+        b.flag_touched = flag_touched_.p;
+        b.cell_min = cell_min_.p;
+        b.noise = noise_buffer_ ? noise_.p : nullptr;
+        return b;
+    }
+    void set_params(int s, double KA, double KP, double dt, double mu, double T, double thr) {
+        int lo = s < 0 ? 0 : s, hi = s < 0 ? S_ : s + 1;
+        for (int q = lo; q < hi; ++q) {
+            hp_[q].KA = KA;
+            hp_[q].KP = KP;
+            hp_[q].dt = dt;
+            hp_[q].mu = mu;
+            hp_[q].T = T;
+            hp_[q].thr = thr;
+        }
+        sys_.upload(hp_.data(), S_);
+        drop_graph();
+    }
+    void set_prefs(const double *A0, const double *P0) {
+        std::vector<double> v(2 * (size_t)Nc_);
+        for (int i = 0; i < Nc_; ++i) {
+            v[2 * i] = A0[i];
+            v[2 * i + 1] = P0[i];
+        }
+        real_upload(APpref_.p, v.data(), v.size());
+    }
+    void set_positions(const double *vx, const double *vy) {
+        std::vector<double> xy(2 * (size_t)Nv_);
+        for (int i = 0; i < Nv_; ++i) {
+            if (!std::isfinite(vx[i]) || !std::isfinite(vy[i])) throw std::invalid_argument("non-finite position");
+            xy[2 * i] = vx[i];
+            xy[2 * i + 1] = vy[i];
+        }
+        (parity_ ? pos1_ : pos0_).upload(xy.data(), xy.size());
+    }
+    void set_noise_mode(bool on) {
+        noise_buffer_ = on;
+        if (on && !noise_.p) noise_.alloc(2 * (size_t)Nv_);
+        drop_graph();
+    }
+    void upload_noise(const double *eta) {
+        if (!noise_.p) noise_.alloc(2 * (size_t)Nv_);
+        noise_.upload(eta, 2 * (size_t)Nv_);
+    }
+    int compute_forces() {
+        cgpu_vertex_batch b = view();
+        int r = cgpu_vm_geometry(&b, stream_);
+        if (!r) r = cgpu_vm_forces(&b, stream_);
+        sync();
+        return r;
+    }
+    int step_launch() {
+        cgpu_vertex_batch b = view();
+        int r = cgpu_vm_brownian_step(&b, stream_);
+        parity_ ^= 1;
+        return r;
+    }
+    int brownian_steps(int steps, bool use_graph) {
+        if (!use_graph || noise_buffer_) {
+            for (int k = 0; k < steps; ++k)
+                if (int r = step_launch()) return r;
+            return CGPU_OK;
+        }
+        if (graph_exec_ && graph_parity_ != parity_ && steps > 0) {
+            if (int r = step_launch()) return r;
+            --steps;
+        }
+        if (!graph_exec_ && steps >= 2) {
+            hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+            graph_parity_ = parity_;
+            step_launch();
+            step_launch();
+            hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+            hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate");
+        }
+        while (steps >= 2) {
+            hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+            steps -= 2;
+        }
+        if (steps == 1) return step_launch();
+        return CGPU_OK;
+    }
+    void sync() { hip_check(hipStreamSynchronize(stream_), "sync"); }
+    int status() {
+        sync();
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        for (auto &q : st)
+            if (q.error) return q.error;
+        return CGPU_OK;
+    }
+    void get(double *vx, double *vy, double *Fx, double *Fy, double *A, double *P) {
+        sync();
+        std::vector<double> t(2 * (size_t)std::max(Nv_, Nc_));
+        if (vx || vy) {
+            (parity_ ? pos1_ : pos0_).download(t.data(), 2 * (size_t)Nv_);
+            for (int i = 0; i < Nv_; ++i) {
+                if (vx) vx[i] = t[2 * i];
+                if (vy) vy[i] = t[2 * i + 1];
+            }
+        }
+        if (Fx || Fy) {
+            real_download(force_.p, t.data(), 2 * (size_t)Nv_);
+            for (int i = 0; i < Nv_; ++i) {
+                if (Fx) Fx[i] = t[2 * i];
+                if (Fy) Fy[i] = t[2 * i + 1];
+            }
+        }
+        if (A || P) {
+            real_download(AP_.p, t.data(), 2 * (size_t)Nc_);
+            for (int i = 0; i < Nc_; ++i) {
+                if (A) A[i] = t[2 * i];
+                if (P) P[i] = t[2 * i + 1];
+            }
+        }
+    }
+    void get_topology(int *vn, int *vcn, int *cvn, int *cv, int cap) {
+        sync();
+        vn_.download(vn, 3 * (size_t)Nv_);
+        vcn_.download(vcn, 3 * (size_t)Nv_);
+        std::vector<int> n(Nc_), c((size_t)CGPU_VERTEX_CAP * Nc_);
+        cvn_.download(n.data(), Nc_);
+        cv_.download(c.data(), c.size());
+        for (int i = 0; i < Nc_; ++i) {
+            cvn[i] = n[i];
+            for (int k = 0; k < n[i] && k < cap; ++k) cv[(size_t)i * cap + k] = c[(size_t)k * Nc_ + i];
+        }
+    }
+    void get_counters(int s, long long *out) {
+        sync();
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        out[0] = st[s].t1_events;
+        out[1] = st[s].step;
+        out[2] = st[s].error;
+        out[3] = hp_[s].nv;
+    }
+
+   private:
+    void drop_graph() {
+        if (graph_exec_) {
+            (void)hipGraphExecDestroy(graph_exec_);
+            graph_exec_ = nullptr;
+        }
+    }
+    void real_upload(void *dst, const double *src, size_t n) {
+        if (prec_) {
+            std::vector<float> f(src, src + n);
+            hip_check(hipMemcpy(dst, f.data(), n * 4, hipMemcpyHostToDevice), "upload");
+        } else {
+            hip_check(hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice), "upload");
+        }
+    }
+    void real_download(const void *src, double *dst, size_t n) {
+        if (prec_) {
+            std::vector<float> f(n);
+            hip_check(hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost), "download");
+            for (size_t i = 0; i < n; ++i) dst[i] = f[i];
+        } else {
+            hip_check(hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost), "download");
+        }
+    }
+    int S_, Nv_ = 0, Nc_ = 0, prec_;
+    size_t rsz_;
+    int parity_ = 0, graph_parity_ = -1;
+    bool noise_buffer_ = false;
+    hipStream_t stream_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    std::vector<cgpu_vm_params> hp_;
+    DevBuf<cgpu_vm_params> sys_;
+    DevBuf<cgpu_sys_state> st_;
+    DevBuf<int> vsys_, csys_, vn_, vcn_, cvn_, cv_, flag_count_, flag_list_, flag_cells_, flag_touched_;
+    DevBuf<unsigned long long> cell_min_;
+    DevBuf<double> pos0_, pos1_, noise_;
+    DevBuf<unsigned char> AP_, dAP_, APpref_, force_;
+};
+
+}  // namespace cgpu_host
+
+using cgpu_host::VertexBatch;
+
+namespace {
+template <typename F> int vguard(F &&f) {
+    try {
+        return f();
+    } catch (...) {
+        return CGPU_ERR_HIP;
+    }
+}
+}  // namespace
+
+extern "C" {
+void *cgpu_vm_create(int S, const int *ncells, const double *L, const uint64_t *seed, const double *x, const double *y,
+                     int precision, int device) {
+    try {
+        return new VertexBatch(S, ncells, L, seed, x, y, precision, device);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void cgpu_vm_destroy(void *h) { delete (VertexBatch *)h; }
+int cgpu_vm_num_vertices(void *h) { return ((VertexBatch *)h)->num_vertices(); }
+int cgpu_vm_set_params(void *h, int s, double KA, double KP, double dt, double mu, double T, double thr) {
+    return vguard([&] { ((VertexBatch *)h)->set_params(s, KA, KP, dt, mu, T, thr); return 0; });
+}
+int cgpu_vm_set_prefs(void *h, const double *A0, const double *P0) {
+    return vguard([&] { ((VertexBatch *)h)->set_prefs(A0, P0); return 0; });
+}
+int cgpu_vm_set_positions(void *h, const double *vx, const double *vy) {
+    return vguard([&] { ((VertexBatch *)h)->set_positions(vx, vy); return 0; });
+}
+int cgpu_vm_set_noise_mode(void *h, int on) {
+    return vguard([&] { ((VertexBatch *)h)->set_noise_mode(on != 0); return 0; });
+}
+int cgpu_vm_upload_noise(void *h, const double *eta) {
+    return vguard([&] { ((VertexBatch *)h)->upload_noise(eta); return 0; });
+}
+int cgpu_vm_compute_forces(void *h) { return vguard([&] { return ((VertexBatch *)h)->compute_forces(); }); }
+int cgpu_vm_brownian_steps(void *h, int steps, int use_graph) {
+    return vguard([&] { return ((VertexBatch *)h)->brownian_steps(steps, use_graph != 0); });
+}
+int cgpu_vm_get(void *h, double *vx, double *vy, double *Fx, double *Fy, double *A, double *P) {
+    return vguard([&] { ((VertexBatch *)h)->get(vx, vy, Fx, Fy, A, P); return 0; });
+}
+int cgpu_vm_get_topology(void *h, int *vn, int *vcn, int *cvn, int *cv, int cap) {
+    return vguard([&] { ((VertexBatch *)h)->get_topology(vn, vcn, cvn, cv, cap); return 0; });
+}
+int cgpu_vm_get_counters(void *h, int s, long long *out) {
+    return vguard([&] { ((VertexBatch *)h)->get_counters(s, out); return 0; });
+}
+int cgpu_vm_status(void *h) { return vguard([&] { return ((VertexBatch *)h)->status(); }); }
+}

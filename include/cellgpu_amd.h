@@ -112,6 +112,47 @@ typedef struct cgpu_voronoi_batch {
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
 } cgpu_voronoi_batch;
 
+/* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
+ * vertexModelBase.cpp:210, grown on demand there; fixed capacity here). */
+#define CGPU_VERTEX_CAP 32
+
+/* Per-system constants of a vertex-model batch. */
+typedef struct cgpu_vm_params {
+    int voff, nv;       /* first vertex, vertices (= 2 nc)                 */
+    int coff, nc;       /* first cell, cells                               */
+    double L;           /* square box side                                 */
+    double KA, KP;      /* moduli                                          */
+    double dt, mu, T;   /* Brownian step                                   */
+    double thr;         /* T1 edge-length threshold                        */
+    uint64_t seed;      /* counter-based noise key                         */
+} cgpu_vm_params;
+
+/*
+ * Batch view of a vertex-model batch (device pointers). Vertex arrays have
+ * num_vertices entries (vn, vcn: 3 per vertex, cells counter-clockwise around
+ * the vertex, vn[3v+f] across from vcn[3v+f]); cell vertex lists are slot-major
+ * cv[k*num_cells + c], counter-clockwise.
+ */
+typedef struct cgpu_vertex_batch {
+    int num_systems, num_vertices, num_cells, precision;
+    const cgpu_vm_params *sys;   /* [S] */
+    cgpu_sys_state *st;          /* [S] */
+    const int *vert_sys;         /* [Nv] */
+    const int *cell_sys;         /* [Nc] */
+    double *pos, *pos_next;      /* [2Nv] vertex positions */
+    int *vn, *vcn;               /* [3Nv] */
+    int *cvn, *cv;               /* [Nc], [VERTEX_CAP*Nc] */
+    void *AP, *dAP;              /* [2Nc] real */
+    const void *APpref;          /* [2Nc] real */
+    void *force;                 /* [2Nv] real */
+    int *flag_count;             /* [S] */
+    int *flag_list;              /* [6Nv] flagged edges (v1, v2) */
+    int *flag_cells;             /* [12Nv] */
+    int *flag_touched;           /* [12Nv] */
+    unsigned long long *cell_min;/* [Nc], all ones when idle */
+    const double *noise;         /* [2Nv] or NULL */
+} cgpu_vertex_batch;
+
 /* ---------------------------------------------------------------- kernel layer */
 
 /* Replaces gpu_compute_cell_list + gpu_zero_array (reference
@@ -198,6 +239,17 @@ int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state 
 /* apply a pending FIRE velocity mixing (end of minimize) */
 int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
+/* Vertex model, replacing gpu_vm_geometry (reference vertexModelBase.cuh:19-30,
+ * vertexModelBase.cu:20-77), gpu_avm_force_sets + gpu_avm_sum_force_sets
+ * (vertexQuadraticEnergy.cuh:20-33, .cu:21-70), gpu_brownian_eom_integration
+ * (brownianParticleDynamics.cuh:17-24), gpu_vm_test_edges_for_T1 +
+ * gpu_vm_parse_multiple_flips + gpu_vm_flip_edges (vertexModelBase.cuh:40-79,
+ * vertexModelBase.cu:118-544). */
+int cgpu_vm_geometry(const cgpu_vertex_batch *b, void *stream);
+int cgpu_vm_forces(const cgpu_vertex_batch *b, void *stream);
+int cgpu_vm_brownian_step(const cgpu_vertex_batch *b, void *stream);
+int cgpu_vm_t1(const cgpu_vertex_batch *b, void *stream);
+
 /* ---------------------------------------------------------------- object layer */
 
 /* A batch of independent Voronoi systems (one system = the reference's
@@ -230,6 +282,26 @@ int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
 int cgpu_vor_autotune(void *h);
 int cgpu_vor_view(void *h, cgpu_voronoi_batch *out);
 void *cgpu_vor_stream(void *h);
+
+/* A batch of vertex-model systems built from the Voronoi tessellation of the
+ * given cell positions (reference vertexModelBase::setCellsVoronoiTesselation,
+ * vertexModelBase.cpp:115-229). Vertex ids follow (smallest cell, ring slot) of
+ * the Delaunay triangles. x, y NULL: reference random points from seed[s]. */
+void *cgpu_vm_create(int num_systems, const int *ncells, const double *L, const uint64_t *seed, const double *x,
+                     const double *y, int precision, int device);
+void cgpu_vm_destroy(void *h);
+int cgpu_vm_num_vertices(void *h);
+int cgpu_vm_set_params(void *h, int s, double KA, double KP, double dt, double mu, double T, double t1_threshold);
+int cgpu_vm_set_prefs(void *h, const double *A0, const double *P0);       /* per cell */
+int cgpu_vm_set_positions(void *h, const double *vx, const double *vy);  /* per vertex */
+int cgpu_vm_set_noise_mode(void *h, int use_buffer);
+int cgpu_vm_upload_noise(void *h, const double *eta);                     /* 2 Nv: x block, y block */
+int cgpu_vm_compute_forces(void *h);
+int cgpu_vm_brownian_steps(void *h, int steps, int use_graph);
+int cgpu_vm_get(void *h, double *vx, double *vy, double *Fx, double *Fy, double *A, double *P);
+int cgpu_vm_get_topology(void *h, int *vn, int *vcn, int *cvn, int *cv, int cap);
+int cgpu_vm_get_counters(void *h, int s, long long *out);
+int cgpu_vm_status(void *h);
 
 /* FIRE minimization of every system of a Voronoi batch (EnergyMinimizerFIRE,
  * reference EnergyMinimizerFIRE2D.cpp:269-284, GPU branch). */

@@ -214,3 +214,99 @@ class FireMinimizer:
         if it < 0:
             raise RuntimeError("oracle: topology repair failed during FIRE")
         return it, dict(force_max=out[0], power=out[1], alpha=out[2], dt=out[3], max_f2=out[4])
+
+
+# ----------------------------------------------------------------------------- vertex model
+def _vlib():
+    L = lib()
+    if not getattr(L, "_vm_ready", False):
+        vp = C.c_void_p
+        sig = {
+            "vmo_create": (vp, [C.c_int, C.c_double, _dp, _dp, _ip, _ip, C.c_int]),
+            "vmo_destroy": (None, [vp]),
+            "vmo_nv": (C.c_int, [vp]),
+            "vmo_set_params": (None, [vp] + [C.c_double] * 6),
+            "vmo_set_prefs": (None, [vp, _dp, _dp]),
+            "vmo_set_positions": (None, [vp, _dp, _dp]),
+            "vmo_get_topology": (None, [vp, _ip, _ip, _ip, _ip, C.c_int]),
+            "vmo_set_topology": (None, [vp, _ip, _ip, _ip, _ip, C.c_int]),
+            "vmo_compute": (C.c_double, [vp] + [vp] * 6),
+            "vmo_brownian_step": (C.c_int, [vp, _dp, _dp]),
+            "vmo_t1": (C.c_int, [vp]),
+            "vmo_last_flagged": (C.c_int, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._vm_ready = True
+    return L
+
+
+class VertexSystem:
+    """FP64 oracle of the reference vertex model (GPU semantics, deterministic T1 rounds)."""
+
+    CAP = 32
+
+    def __init__(self, x, y, L, nn, nbr, KA=1.0, KP=1.0, dt=0.005, mu=1.0, T=0.0, thr=0.04, A0=1.0, P0=3.9):
+        self.Nc = len(x)
+        self.L = float(L)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        self.h = _vlib().vmo_create(self.Nc, self.L, np.ascontiguousarray(x, np.float64),
+                                    np.ascontiguousarray(y, np.float64), np.ascontiguousarray(nn, np.int32),
+                                    nbr.reshape(-1), nbr.shape[1])
+        self.Nv = _vlib().vmo_nv(self.h)
+        _vlib().vmo_set_params(self.h, KA, KP, dt, mu, T, thr)
+        self.set_prefs(A0, P0)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _vlib().vmo_destroy(self.h)
+            self.h = None
+
+    def set_params(self, KA=1.0, KP=1.0, dt=0.005, mu=1.0, T=0.0, thr=0.04):
+        _vlib().vmo_set_params(self.h, KA, KP, dt, mu, T, thr)
+
+    def set_prefs(self, A0, P0):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(A0, np.float64), (self.Nc,)))
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(P0, np.float64), (self.Nc,)))
+        _vlib().vmo_set_prefs(self.h, a, p)
+
+    def set_positions(self, vx, vy):
+        _vlib().vmo_set_positions(self.h, np.ascontiguousarray(vx, np.float64), np.ascontiguousarray(vy, np.float64))
+
+    def topology(self):
+        vn = np.zeros(3 * self.Nv, np.int32)
+        vcn = np.zeros(3 * self.Nv, np.int32)
+        cvn = np.zeros(self.Nc, np.int32)
+        cv = np.full(self.Nc * self.CAP, -1, np.int32)
+        _vlib().vmo_get_topology(self.h, vn, vcn, cvn, cv, self.CAP)
+        return vn, vcn, cvn, cv.reshape(self.Nc, self.CAP)
+
+    def set_topology(self, vn, vcn, cvn, cv):
+        cv = np.ascontiguousarray(cv, np.int32)
+        _vlib().vmo_set_topology(self.h, np.ascontiguousarray(vn, np.int32), np.ascontiguousarray(vcn, np.int32),
+                                 np.ascontiguousarray(cvn, np.int32), cv.reshape(-1), cv.shape[1])
+
+    def compute(self):
+        a = {k: np.zeros(self.Nv) for k in ("vx", "vy", "Fx", "Fy")}
+        a.update({k: np.zeros(self.Nc) for k in ("A", "P")})
+        E = _vlib().vmo_compute(self.h, *[_ptr(a[k]) for k in ("vx", "vy", "Fx", "Fy", "A", "P")])
+        a["E"] = E
+        return a
+
+    def brownian_step(self, etax, etay):
+        return _vlib().vmo_brownian_step(self.h, np.ascontiguousarray(etax, np.float64),
+                                         np.ascontiguousarray(etay, np.float64))
+
+    def t1(self):
+        return _vlib().vmo_t1(self.h)
+
+    def last_flagged(self):
+        return _vlib().vmo_last_flagged(self.h)
+
+
+def vertex_from_random(Nc, seed=13377, **kw):
+    x, y, th, L = random_init(Nc, seed)
+    nn, nbr, _ = triangulate(x, y, L, cap=32)
+    return VertexSystem(x, y, L, nn, nbr, **kw), (x, y, L, nn, nbr)

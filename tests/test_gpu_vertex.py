@@ -1,0 +1,86 @@
+"""GPU parity of the vertex-model step (geometry, forces, Brownian move, T1
+flips) against the FP64 oracle. Topology is compared as sets (vertex
+neighbours, vertex cells) and as cyclic counter-clockwise cell vertex lists."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cg():
+    import cellgpu_amd as cg
+    cg.require_device()
+    return cg
+
+
+def _canon(cv, cvn):
+    out = []
+    for c in range(len(cvn)):
+        l = list(cv[c, : cvn[c]])
+        k = int(np.argmin(l))
+        out.append(tuple(l[k:] + l[:k]))
+    return out
+
+
+def _sets(a):
+    return [frozenset(a[3 * v: 3 * v + 3]) for v in range(len(a) // 3)]
+
+
+def _oracle_like(oracle, Nc, seed, **kw):
+    o, (x, y, L, nn, nbr) = oracle.vertex_from_random(Nc, seed, **kw)
+    return o
+
+
+@pytest.mark.parametrize("Nc", [64, 500])
+def test_vertex_init_and_forces_match_oracle(oracle, Nc):
+    cg = _cg()
+    b = cg.VertexBatch([Nc], seeds=[5], precision="fp64")
+    o = _oracle_like(oracle, Nc, 5)
+    assert b.Nv == o.Nv == 2 * Nc
+    vn, vcn, cvn, cv = b.topology()
+    ovn, ovcn, ocvn, ocv = o.topology()
+    assert _sets(vn) == _sets(ovn) and _sets(vcn) == _sets(ovcn)
+    assert _canon(cv, cvn) == _canon(ocv, ocvn)
+    b.set_prefs(1.0, 3.9)
+    b.compute_forces()
+    g = b.get()
+    r = o.compute()
+    assert np.array_equal(g["vx"], r["vx"]) and np.array_equal(g["vy"], r["vy"])
+    np.testing.assert_allclose(g["A"], r["A"], rtol=1e-12)
+    np.testing.assert_allclose(g["P"], r["P"], rtol=1e-12)
+    scale = np.abs(np.concatenate([r["Fx"], r["Fy"]])).max()
+    assert np.abs(g["Fx"] - r["Fx"]).max() <= 1e-12 * scale
+    assert np.abs(g["Fy"] - r["Fy"]).max() <= 1e-12 * scale
+
+
+def test_vertex_brownian_t1_stepwise(oracle):
+    """Each step: oracle restarted from the GPU state, same noise; T1 flips included."""
+    cg = _cg()
+    Nc = 300
+    b = cg.VertexBatch([Nc], seeds=[9], precision="fp64")
+    b.set_params(dt=0.005, T=1e-3, t1_threshold=0.04)
+    b.set_prefs(1.0, 3.9)
+    b.set_noise_buffer(True)
+    o = _oracle_like(oracle, Nc, 9, dt=0.005, T=1e-3, thr=0.04, P0=3.9)
+    rng = np.random.default_rng(2)
+    t1_total = 0
+    for step in range(25):
+        g = b.get()
+        vn, vcn, cvn, cv = b.topology()
+        o.set_positions(g["vx"], g["vy"])
+        o.set_topology(vn, vcn, cvn, cv)
+        ex, ey = rng.standard_normal(b.Nv), rng.standard_normal(b.Nv)
+        b.upload_noise(ex, ey)
+        b.brownian_steps(1, use_graph=False)
+        b.check_status()
+        t1_total += o.brownian_step(ex, ey)
+        g, r = b.get(), o.compute()
+        np.testing.assert_allclose(g["vx"], r["vx"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(g["vy"], r["vy"], rtol=0, atol=1e-11)
+        vn, vcn, cvn, cv = b.topology()
+        ovn, ovcn, ocvn, ocv = o.topology()
+        assert _sets(vn) == _sets(ovn), step
+        assert _sets(vcn) == _sets(ovcn), step
+        assert _canon(cv, cvn) == _canon(ocv, ocvn), step
+    assert t1_total > 0
+    assert b.counters()["t1_events"] == t1_total
