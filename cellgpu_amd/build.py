@@ -59,6 +59,19 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, OUT)
+    # drivers (reference voronoi.cpp command line) linked against the library
+    ddir = os.path.join(CSRC, "drivers")
+    bdir = os.path.join(HERE, "bin")
+    os.makedirs(bdir, exist_ok=True)
+    for f in sorted(os.listdir(ddir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(bdir, f[:-4])
+        cmd = [HIPCC] + COMMON + ["-o", exe, os.path.join(ddir, f), "-L" + HERE, "-lcellgpu_amd",
+                                  "-Wl,-rpath," + HERE]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("driver build failed: %s\n%s" % (r.stdout, r.stderr))
     if verbose:
         print("built", OUT)
     return OUT

@@ -1,0 +1,315 @@
+// C++ drop-in layer over the cellgpu_amd C ABI, keeping the reference's
+// plugin API (abaso/cellGPU): Simulation (inc/Simulation.h:18-87), the model
+// hierarchy Simple2DModel -> Simple2DCell -> Simple2DActiveCell ->
+// voronoiModelBase / vertexModelBase -> VoronoiQuadraticEnergy /
+// VertexQuadraticEnergy (inc/models/*.h), and the updaters updater ->
+// simpleEquationOfMotion -> selfPropelledParticleDynamics /
+// brownianParticleDynamics / EnergyMinimizerFIRE (inc/updaters/*.h).
+//
+// Differences a caller sees: state lives on the GPU and host accessors return
+// copies (std::vector) instead of GPUArray handles; setCPU / setCPUOperation(true)
+// throw, since there is no CPU path; a model is a one-system batch (see
+// cgpu_vor_create for many systems in one launch).
+#pragma once
+#include <cmath>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cellgpu_amd.h"
+
+namespace cellgpu {
+
+using std::shared_ptr;
+using std::vector;
+
+struct double2v { double x, y; };
+
+inline void check(int status, const char *what) {
+    if (status != CGPU_OK)
+        throw std::runtime_error(std::string(what) + " failed, status " + std::to_string(status) + ": " +
+                                 cgpu_last_error());
+}
+
+//! reference inc/models/Simple2DModel.h:26-67
+class Simple2DModel {
+   public:
+    virtual ~Simple2DModel() = default;
+    virtual int getNumberOfDegreesOfFreedom() = 0;
+    virtual void computeForces() = 0;
+    virtual double computeEnergy() = 0;
+    virtual void setGPU() {}
+    virtual void setCPU(bool = true) { throw std::runtime_error("cellgpu_amd has no CPU path"); }
+    virtual void setDeltaT(double dt) = 0;
+    virtual vector<double2v> returnForces() = 0;
+    virtual vector<double2v> returnPositions() = 0;
+};
+
+//! reference voronoiModelBase + VoronoiQuadraticEnergy (GPU branch)
+class VoronoiQuadraticEnergy : public Simple2DModel {
+   public:
+    VoronoiQuadraticEnergy(int n, double A0, double P0, bool reprod, int precision = 0, unsigned seed = 13377)
+        : Ncells(n) {
+        uint64_t s = reprod ? seed : (uint64_t)std::random_device{}();
+        h_ = cgpu_vor_create(1, &n, nullptr, &s, nullptr, nullptr, nullptr, precision, 0);
+        if (!h_) throw std::runtime_error(std::string("cgpu_vor_create: ") + cgpu_last_error());
+        setCellPreferencesUniform(A0, P0);
+        setv0Dr(0.05, 1.0);
+    }
+    ~VoronoiQuadraticEnergy() override { cgpu_vor_destroy(h_); }
+    void setCellPreferencesUniform(double A0, double P0) {
+        vector<double> a(Ncells, A0), p(Ncells, P0);
+        check(cgpu_vor_set_prefs(h_, a.data(), p.data()), "setCellPreferencesUniform");
+    }
+    void setCellPreferences(const vector<double2v> &ap) {
+        vector<double> a(Ncells), p(Ncells);
+        for (int i = 0; i < Ncells; ++i) { a[i] = ap[i].x; p[i] = ap[i].y; }
+        check(cgpu_vor_set_prefs(h_, a.data(), p.data()), "setCellPreferences");
+    }
+    void setv0Dr(double v0, double Dr) {
+        vector<double> a(Ncells, v0), b(Ncells, Dr);
+        check(cgpu_vor_set_motility(h_, a.data(), b.data()), "setv0Dr");
+    }
+    void setModuliUniform(double ka, double kp) {
+        KA = ka;
+        KP = kp;
+        push();
+    }
+    void setExclusions(const vector<int> &ex) { check(cgpu_vor_set_exclusions(h_, ex.data()), "setExclusions"); }
+    void setDeltaT(double dt) override {
+        deltaT = dt;
+        push();
+    }
+    void setMobility(double m) {
+        mu = m;
+        push();
+    }
+    int getNumberOfDegreesOfFreedom() override { return Ncells; }
+    void computeForces() override { check(cgpu_vor_compute_forces(h_), "computeForces"); }
+    double computeEnergy() override { return cgpu_vor_energy(h_, 0); }
+    vector<double2v> returnForces() override { return fetch(2); }
+    vector<double2v> returnPositions() override { return fetch(0); }
+    vector<double2v> returnAreaPeri() { return fetch(3); }
+    double reportq() {
+        auto ap = returnAreaPeri();
+        double q = 0;
+        for (auto &v : ap) q += v.y / std::sqrt(v.x);
+        return q / Ncells;
+    }
+    double2v reportMeanCellForce() {
+        auto f = returnForces();
+        double2v m{0, 0};
+        for (auto &v : f) { m.x += v.x / Ncells; m.y += v.y / Ncells; }
+        return m;
+    }
+    void *handle() { return h_; }
+    int Ncells;
+    double KA = 1, KP = 1, deltaT = 0.01, mu = 1;
+
+   private:
+    void push() { check(cgpu_vor_set_params(h_, 0, KA, KP, deltaT, mu, 0.0), "set params"); }
+    vector<double2v> fetch(int what) {
+        vector<double> a(Ncells), b(Ncells);
+        int r;
+        if (what == 0) r = cgpu_vor_get(h_, a.data(), b.data(), 0, 0, 0, 0, 0, 0, 0);
+        else if (what == 2) r = cgpu_vor_get(h_, 0, 0, 0, 0, 0, a.data(), b.data(), 0, 0);
+        else r = cgpu_vor_get(h_, 0, 0, 0, 0, 0, 0, 0, a.data(), b.data());
+        check(r, "get");
+        vector<double2v> out(Ncells);
+        for (int i = 0; i < Ncells; ++i) out[i] = double2v{a[i], b[i]};
+        return out;
+    }
+    void *h_;
+};
+
+//! reference vertexModelBase + VertexQuadraticEnergy (GPU branch)
+class VertexQuadraticEnergy : public Simple2DModel {
+   public:
+    VertexQuadraticEnergy(int n, double A0, double P0, bool reprod, int precision = 0, unsigned seed = 13377)
+        : Ncells(n) {
+        uint64_t s = reprod ? seed : (uint64_t)std::random_device{}();
+        h_ = cgpu_vm_create(1, &n, nullptr, &s, nullptr, nullptr, precision, 0);
+        if (!h_) throw std::runtime_error("cgpu_vm_create failed");
+        Nvertices = cgpu_vm_num_vertices(h_);
+        setCellPreferencesUniform(A0, P0);
+    }
+    ~VertexQuadraticEnergy() override { cgpu_vm_destroy(h_); }
+    void setCellPreferencesUniform(double A0, double P0) {
+        vector<double> a(Ncells, A0), p(Ncells, P0);
+        check(cgpu_vm_set_prefs(h_, a.data(), p.data()), "setCellPreferencesUniform");
+    }
+    void setT1Threshold(double t) {
+        T1Threshold = t;
+        push();
+    }
+    void setDeltaT(double dt) override {
+        deltaT = dt;
+        push();
+    }
+    void setTemperature(double t) {
+        T = t;
+        push();
+    }
+    int getNumberOfDegreesOfFreedom() override { return Nvertices; }
+    void computeForces() override { check(cgpu_vm_compute_forces(h_), "computeForces"); }
+    double computeEnergy() override {
+        computeForces();
+        vector<double> A(Ncells), P(Ncells);
+        check(cgpu_vm_get(h_, 0, 0, 0, 0, A.data(), P.data()), "get");
+        double E = 0;
+        for (int c = 0; c < Ncells; ++c) E += KA * (A[c] - 1.0) * (A[c] - 1.0) + KP * (P[c] - P0_) * (P[c] - P0_);
+        return E;
+    }
+    vector<double2v> returnForces() override {
+        vector<double> a(Nvertices), b(Nvertices);
+        check(cgpu_vm_get(h_, 0, 0, a.data(), b.data(), 0, 0), "get");
+        vector<double2v> out(Nvertices);
+        for (int i = 0; i < Nvertices; ++i) out[i] = double2v{a[i], b[i]};
+        return out;
+    }
+    vector<double2v> returnPositions() override {
+        vector<double> a(Nvertices), b(Nvertices);
+        check(cgpu_vm_get(h_, a.data(), b.data(), 0, 0, 0, 0), "get");
+        vector<double2v> out(Nvertices);
+        for (int i = 0; i < Nvertices; ++i) out[i] = double2v{a[i], b[i]};
+        return out;
+    }
+    void *handle() { return h_; }
+    int Ncells, Nvertices;
+    double KA = 1, KP = 1, deltaT = 0.005, mu = 1, T = 0, T1Threshold = 0.01, P0_ = 3.9;
+
+   private:
+    void push() { check(cgpu_vm_set_params(h_, 0, KA, KP, deltaT, mu, T, T1Threshold), "set params"); }
+    void *h_;
+};
+
+//! reference inc/updaters/updater.h:17-103
+class updater {
+   public:
+    virtual ~updater() = default;
+    virtual void Update(int timestep) {
+        if (Period > 0 && (timestep + Phase) % Period == 0) performUpdate();
+    }
+    virtual void performUpdate() {}
+    virtual void set2DModel(shared_ptr<Simple2DModel> m) { model = m; }
+    virtual void setGPU() {}
+    virtual void setCPU() { throw std::runtime_error("cellgpu_amd has no CPU path"); }
+    virtual void setReproducible(bool) {}
+    virtual void setDeltaT(double dt) { deltaT = dt; }
+    void setPeriod(int p) { Period = p; }
+    void setPhase(int p) { Phase = p; }
+    shared_ptr<Simple2DModel> model;
+
+   protected:
+    int Period = 1, Phase = 0;
+    double deltaT = 0.01;
+};
+
+//! reference simpleEquationOfMotion.h:26-56
+class simpleEquationOfMotion : public updater {
+   public:
+    void performUpdate() override { integrateEquationsOfMotion(); }
+    virtual void integrateEquationsOfMotion() = 0;
+};
+
+//! reference selfPropelledParticleDynamics.h:13-44 (GPU semantics)
+class selfPropelledParticleDynamics : public simpleEquationOfMotion {
+   public:
+    explicit selfPropelledParticleDynamics(int n = 0) { (void)n; }
+    void integrateEquationsOfMotion() override {
+        auto v = std::dynamic_pointer_cast<VoronoiQuadraticEnergy>(model);
+        if (!v) throw std::runtime_error("selfPropelledParticleDynamics drives a Voronoi model");
+        check(cgpu_vor_spv_steps(v->handle(), 1, 0), "self-propelled step");
+    }
+};
+
+//! reference brownianParticleDynamics.h:14-51
+class brownianParticleDynamics : public simpleEquationOfMotion {
+   public:
+    explicit brownianParticleDynamics(int n = 0) { (void)n; }
+    void setT(double t) { Temperature = t; }
+    void integrateEquationsOfMotion() override {
+        if (auto vm = std::dynamic_pointer_cast<VertexQuadraticEnergy>(model)) {
+            vm->setTemperature(Temperature);
+            check(cgpu_vm_brownian_steps(vm->handle(), 1, 0), "vertex Brownian step");
+        } else if (auto v = std::dynamic_pointer_cast<VoronoiQuadraticEnergy>(model)) {
+            check(cgpu_vor_set_params(v->handle(), 0, v->KA, v->KP, v->deltaT, v->mu, Temperature), "params");
+            check(cgpu_vor_brownian_steps(v->handle(), 1), "Voronoi Brownian step");
+        }
+    }
+    double Temperature = 0.0;
+};
+
+//! reference EnergyMinimizerFIRE2D.h:20-132 (GPU branch), Voronoi models
+class EnergyMinimizerFIRE : public simpleEquationOfMotion {
+   public:
+    explicit EnergyMinimizerFIRE(shared_ptr<Simple2DModel> system) {
+        set2DModel(system);
+        auto v = std::dynamic_pointer_cast<VoronoiQuadraticEnergy>(system);
+        if (!v) throw std::runtime_error("EnergyMinimizerFIRE drives a Voronoi model");
+        h_ = cgpu_fire_create(v->handle());
+        push();
+    }
+    ~EnergyMinimizerFIRE() override { cgpu_fire_destroy(h_); }
+    void setMaximumIterations(int m) { maxIt = m; push(); }
+    void setForceCutoff(double f) { fc = f; push(); }
+    void setAlphaStart(double a) { a0 = a; push(); }
+    void setDeltaT(double d) override { dt = d; if (h_) push(); }
+    void setDeltaTMax(double d) { dtMax = d; push(); }
+    void setDeltaTInc(double d) { dtInc = d; push(); }
+    void setDeltaTDec(double d) { dtDec = d; push(); }
+    void setAlphaDec(double a) { aDec = a; push(); }
+    void setNMin(int n) { nMin = n; push(); }
+    void minimize() { check(cgpu_fire_minimize(h_), "minimize"); }
+    void integrateEquationsOfMotion() override { minimize(); }
+    double getMaxForce() {
+        double out[6];
+        check(cgpu_fire_get(h_, 0, out), "fire get");
+        return out[0];
+    }
+
+   private:
+    void push() { check(cgpu_fire_set(h_, maxIt, fc, a0, dt, dtMax, dtInc, dtDec, aDec, nMin), "fire set"); }
+    void *h_ = nullptr;
+    int maxIt = 1000, nMin = 5;
+    double fc = 1e-7, a0 = 0.99, dt = 0.01, dtMax = 0.1, dtInc = 1.05, dtDec = 0.95, aDec = 0.9;
+};
+
+//! reference inc/Simulation.h:18-87, src/Simulation.cpp:141-166
+class Simulation {
+   public:
+    void setConfiguration(shared_ptr<Simple2DModel> c) { config = c; }
+    void addUpdater(shared_ptr<updater> u, shared_ptr<Simple2DModel> c) {
+        u->set2DModel(c);
+        updaters.push_back(u);
+    }
+    void setIntegrationTimestep(double dt) {
+        integrationTimestep = dt;
+        config->setDeltaT(dt);
+        for (auto &u : updaters) u->setDeltaT(dt);
+    }
+    void setCPUOperation(bool setcpu) {
+        if (setcpu) throw std::runtime_error("cellgpu_amd has no CPU path");
+    }
+    void setReproducible(bool r) {
+        for (auto &u : updaters) u->setReproducible(r);
+    }
+    void performTimestep() {
+        integerTimestep += 1;
+        Time += integrationTimestep;
+        for (auto &u : updaters) u->Update(integerTimestep);
+    }
+    int integerTimestep = 0;
+    double Time = 0, integrationTimestep = 0.01;
+    shared_ptr<Simple2DModel> config;
+    vector<shared_ptr<updater>> updaters;
+};
+
+typedef shared_ptr<Simple2DModel> ForcePtr;
+typedef shared_ptr<simpleEquationOfMotion> EOMPtr;
+typedef shared_ptr<updater> UpdaterPtr;
+typedef shared_ptr<Simulation> SimulationPtr;
+
+}  // namespace cellgpu
