@@ -50,6 +50,10 @@ def per_cell(n, v):
 def make_batch(cg, n, P0, v0, Dr, seeds, precision):
     b = cg.VoronoiBatch(n, seeds=seeds, precision=precision)
     b.set_params(KA=1.0, KP=1.0, dt=0.01, mu=1.0)
+    # reference spatialSorting: relabel each system's cells along a Hilbert curve
+    # (same random positions, neighbour gathers become local); per-cell
+    # preferences are uniform within a system, so they are set after the sort
+    b.spatial_sort()
     b.set_prefs(1.0, per_cell(n, P0))
     b.set_motility(per_cell(n, v0), per_cell(n, Dr))
     return b

@@ -153,6 +153,13 @@ class VoronoiBatch:
     def stream(self):
         return lib().cgpu_vor_stream(self.h)
 
+    def spatial_sort(self):
+        """Relabel cells along a Hilbert curve (per system) and retriangulate.
+        Returns perm with new index -> old index."""
+        perm = np.zeros(self.N, np.int32)
+        check(lib().cgpu_vor_spatial_sort(self.h, _p(perm)), "spatial_sort")
+        return perm
+
     def system_slice(self, s):
         return slice(int(self.offsets[s]), int(self.offsets[s + 1]))
 

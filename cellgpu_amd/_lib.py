@@ -89,6 +89,7 @@ def lib():
         "cgpu_philox4x32_10": (None, [vp, vp, vp]),
         "cgpu_normal_pair": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp]),
         "cgpu_put_in_box": (None, [d, P(d), P(d)]),
+        "cgpu_vor_spatial_sort": (i, [vp, vp]),
     }
     optional = {
         "cgpu_fire_create": (vp, [vp]),

@@ -506,6 +506,92 @@ double VoronoiBatch::energy(int s) {
     return E;
 }
 
+// Hilbert index of cell (x, y) on a 2^k x 2^k grid (the classical rotate-and-
+// reflect construction of the curve, D. Hilbert 1891; iterative form)
+static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+        d += (uint64_t)s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+int VoronoiBatch::spatial_sort(int *perm_out) {
+    sync();
+    const size_t N = N_;
+    std::vector<double> xy(2 * N), th(N), vel(2 * N), pref(2 * N), mot(2 * N);
+    (parity_ ? pos1_ : pos0_).download(xy.data(), xy.size());
+    real_download(theta_.p, th.data(), N);
+    real_download(vel_.p, vel.data(), 2 * N);
+    real_download(APpref_.p, pref.data(), 2 * N);
+    real_download(mot_.p, mot.data(), 2 * N);
+    std::vector<int> ex;
+    if (excl_.p) {
+        ex.resize(N);
+        excl_.download(ex.data(), N);
+    }
+    std::vector<int> perm(N);
+    for (int s = 0; s < S_; ++s) {
+        const cgpu_sys_params &p = hp_[s];
+        uint32_t g = 1;
+        while (g * g < (uint32_t)p.n) g *= 2;  // about one cell per Hilbert pixel
+        std::vector<std::pair<uint64_t, int>> key(p.n);
+        for (int i = 0; i < p.n; ++i) {
+            int c = p.off + i;
+            uint32_t gx = std::min<uint32_t>(g - 1, (uint32_t)(xy[2 * c] / p.L * g));
+            uint32_t gy = std::min<uint32_t>(g - 1, (uint32_t)(xy[2 * c + 1] / p.L * g));
+            key[i] = {hilbert_index(g, gx, gy), c};
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
+        for (int i = 0; i < p.n; ++i) perm[p.off + i] = key[i].second;
+    }
+    auto apply2 = [&](std::vector<double> &a) {
+        std::vector<double> b(a.size());
+        for (size_t i = 0; i < N; ++i) {
+            b[2 * i] = a[2 * perm[i]];
+            b[2 * i + 1] = a[2 * perm[i] + 1];
+        }
+        a.swap(b);
+    };
+    apply2(xy);
+    apply2(vel);
+    apply2(pref);
+    apply2(mot);
+    {
+        std::vector<double> b(N);
+        for (size_t i = 0; i < N; ++i) b[i] = th[perm[i]];
+        th.swap(b);
+    }
+    (parity_ ? pos1_ : pos0_).upload(xy.data(), xy.size());
+    real_upload(theta_.p, th.data(), N);
+    real_upload(vel_.p, vel.data(), 2 * N);
+    real_upload(APpref_.p, pref.data(), 2 * N);
+    real_upload(mot_.p, mot.data(), 2 * N);
+    if (excl_.p) {
+        std::vector<int> b(N);
+        for (size_t i = 0; i < N; ++i) b[i] = ex[perm[i]];
+        excl_.upload(b.data(), N);
+    }
+    if (perm_out)
+        for (size_t i = 0; i < N; ++i) perm_out[i] = perm[i];
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    return global_triangulation();
+}
+
 int VoronoiBatch::autotune() { return CGPU_OK; }
 
 }  // namespace cgpu_host
@@ -607,6 +693,9 @@ int cgpu_vor_set_active(void *h, const int *active, const int *steps_left) {
     return guarded([&] { ((VoronoiBatch *)h)->set_active(active, steps_left); return 0; });
 }
 int cgpu_vor_autotune(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->autotune(); }); }
+int cgpu_vor_spatial_sort(void *h, int *perm) {
+    return guarded([&] { return ((VoronoiBatch *)h)->spatial_sort(perm); });
+}
 int cgpu_vor_view(void *h, cgpu_voronoi_batch *out) {
     *out = ((VoronoiBatch *)h)->view();
     return 0;

@@ -90,6 +90,10 @@ class VoronoiBatch {
     void *force_dev() { return force_.p; }
     void swap_parity() { parity_ ^= 1; }
     int autotune();
+    // reference spatialSorting (Simple2DCell.cpp:438-477, voronoiModelBase.cpp:669-680):
+    // relabel the cells of every system along a Hilbert curve of their
+    // positions, then retriangulate; perm[new] = old (global indices)
+    int spatial_sort(int *perm_out);
 
    private:
     void upload_params();

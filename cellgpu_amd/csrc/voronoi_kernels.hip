@@ -319,6 +319,65 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
     return -1;
 }
 
+// True when the current ring of cell i already is its exact Delaunay star:
+// consecutive neighbours strictly counter-clockwise, winding once, and no cell-list
+// candidate strictly inside any of its triangles' circumcircles (exact in-circle
+// test; every circle lies inside the scanned block). The Delaunay triangulation is
+// unique for points in general position, so keeping such a ring gives the same
+// result as rebuilding it, for a fraction of the work.
+__device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    if (z < 3 || z > CAP) return false;
+    const Box box = make_box(sp.L);
+    const dvec2 pi = ldpos(pos, i);
+    const double bs = sp.boxsize;
+    const int xs = sp.xsize;
+    // orientation, winding and the largest circumcircle
+    double turn = 0.0, need = 0.0;
+    dvec2 d0 = min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi);
+    for (int k = 0; k < z; ++k) {
+        dvec2 d1 = min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi);
+        if (orient0(d0.x, d0.y, d1.x, d1.y) <= 0) return false;
+        turn += atan2(d0.x * d1.y - d0.y * d1.x, d0.x * d1.x + d0.y * d1.y);
+        dvec2 c = circ0<double>(d0, d1);
+        need = fmax(need, 2.0 * sqrt(c.x * c.x + c.y * c.y));
+        d0 = d1;
+    }
+    if (!(turn < 9.42)) return false;  // 2 pi: winds once (3 pi separates 2 pi from 4 pi)
+    int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
+    int jb = max(0, min(xs - 1, (int)floor(pi.y / bs)));
+    const int wmax = max(1, xs / 2);
+    int w = 1;
+    bool full = false;
+    for (;; ++w) {
+        if (w > wmax) return false;
+        full = (2 * w + 1 >= xs);
+        double rcov = fmin(fmin(pi.x - (ib - w) * bs, (ib + w + 1) * bs - pi.x),
+                           fmin(pi.y - (jb - w) * bs, (jb + w + 1) * bs - pi.y));
+        if (full || need <= rcov * (1.0 - 1e-9)) break;
+    }
+    bool ok = true;
+    int ja = b.nbr[(z - 1) * N + i];
+    dvec2 da = min_dist(box, ldpos(pos, ja), pi);
+    for (int k = 0; k < z && ok; ++k) {
+        int jb2 = b.nbr[k * N + i];
+        dvec2 db = min_dist(box, ldpos(pos, jb2), pi);
+        dvec2 c = circ0<double>(da, db);
+        double r2 = c.x * c.x + c.y * c.y;
+        scan_block(b, sp, ib, jb, w, [&](int m) {
+            if (!ok || m == i || m == ja || m == jb2) return;
+            dvec2 dm = min_dist(box, ldpos(pos, m), pi);
+            double ex = dm.x - c.x, ey = dm.y - c.y;
+            if (ex * ex + ey * ey > r2 * (1.0 + 1e-8)) return;  // clearly outside
+            if (incircle0(da.x, da.y, db.x, db.y, dm.x, dm.y) > 0) ok = false;
+        });
+        ja = jb2;
+        da = db;
+    }
+    return ok;
+}
+
 // reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
 // A ring that does not contain nm1 leaves that slot's delOther unchanged.
 __device__ bool del_sets(const cgpu_voronoi_batch &b, int i) {
@@ -543,6 +602,7 @@ __global__ void k_star_local(cgpu_voronoi_batch b, const double *pos) {
         if (st.mode != 1) continue;
         const cgpu_sys_params &sp = b.sys[s];
         int zold = b.nn[i];
+        if (verify_ring(b, pos, i, sp)) continue;  // unchanged: no invariant change
         int told = owned_triangles(b, i);
         int ring[CAP];
         int z = delaunay_star(b, pos, i, sp, ring);
@@ -595,6 +655,12 @@ __global__ void k_star_global(cgpu_voronoi_batch b, const double *pos) {
         int i = b.gq[q];
         int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
+        if (verify_ring(b, pos, i, b.sys[s])) {
+            int z = b.nn[i];
+            atomicMax(&st.max_deg, z);
+            atomicAdd(&st.nn_delta, z);
+            continue;
+        }
         int ring[CAP];
         int z = delaunay_star(b, pos, i, b.sys[s], ring);
         if (z < 0) {

@@ -280,6 +280,11 @@ double cgpu_vor_energy(void *h, int s);
 int cgpu_vor_status(void *h);
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
 int cgpu_vor_autotune(void *h);
+/* Spatial sorting (reference Simulation::setSortPeriod -> spatialSorting,
+ * Simple2DCell.cpp:438-477, voronoiModelBase.cpp:669-680): relabels the cells of
+ * every system along a Hilbert curve of their positions (all per-cell state
+ * follows) and retriangulates. perm (may be NULL) receives new -> old. */
+int cgpu_vor_spatial_sort(void *h, int *perm);
 int cgpu_vor_view(void *h, cgpu_voronoi_batch *out);
 void *cgpu_vor_stream(void *h);
 

@@ -1,0 +1,123 @@
+"""CPU pins of the vertex-model oracle.
+
+* centred-rectangular lattice: every cell a hexagon of known area/perimeter, F = 0;
+* forces = -grad E by central differences on vertex positions;
+* Euler / bookkeeping invariants (Nv = 2 Nc, sum of cell vertex counts = 3 Nv);
+* single-T1 known answer: a shortened edge flips, the two cells on its sides
+  lose a vertex (6 -> 5), the two at its ends gain one (6 -> 7), the new edge is
+  perpendicular to the old one and twice as long.
+"""
+import numpy as np
+
+
+def _lattice(nx, ny, L):
+    ax, ay = L / nx, L / ny
+    pts = [((i + 0.5 * (j % 2) + 0.25) * ax, (j + 0.5) * ay) for j in range(ny) for i in range(nx)]
+    p = np.array(pts)
+    return p[:, 0].copy(), p[:, 1].copy(), ax, ay
+
+
+def _lattice_vm(oracle, nx=8, ny=8, L=8.0, **kw):
+    x, y, ax, ay = _lattice(nx, ny, L)
+    nn, nbr, _ = oracle.triangulate(x, y, L, cap=32)
+    return oracle.VertexSystem(x, y, L, nn, nbr, **kw), ax, ay
+
+
+def test_lattice_closed_form(oracle):
+    vm, ax, ay = _lattice_vm(oracle)
+    k = (ay * ay - ax * ax / 4) / (2 * ay)
+    h = (ay * ay + ax * ax / 4) / (2 * ay)
+    A_cf, P_cf = ax * ay, 4 * k + 4 * np.hypot(ax / 2, h - k)
+    vm.set_prefs(A_cf, P_cf)
+    r = vm.compute()
+    assert vm.Nv == 2 * vm.Nc
+    np.testing.assert_allclose(r["A"], A_cf, rtol=1e-12)
+    np.testing.assert_allclose(r["P"], P_cf, rtol=1e-12)
+    assert abs(r["E"]) < 1e-20
+    assert np.abs(r["Fx"]).max() < 1e-9 and np.abs(r["Fy"]).max() < 1e-9  # at the minimum
+    vm.set_prefs(1.0, 3.0)
+    r = vm.compute()
+    # identical cells under tension: forces balance over the whole tissue
+    assert abs(r["Fx"].sum()) < 1e-10 and abs(r["Fy"].sum()) < 1e-10
+    vn, vcn, cvn, cv = vm.topology()
+    assert (cvn == 6).all() and cvn.sum() == 3 * vm.Nv
+
+
+def test_forces_are_minus_energy_gradient(oracle):
+    vm, (x, y, L, nn, nbr) = oracle.vertex_from_random(120, 4, P0=3.7)
+    r0 = vm.compute()
+    vx, vy = r0["vx"].copy(), r0["vy"].copy()
+    h = 1e-6
+    rng = np.random.default_rng(1)
+    for v in rng.choice(vm.Nv, 10, replace=False):
+        for comp in ("x", "y"):
+            a, b = vx.copy(), vy.copy()
+            (a if comp == "x" else b)[v] += h
+            vm.set_positions(a, b)
+            Ep = vm.compute()["E"]
+            a, b = vx.copy(), vy.copy()
+            (a if comp == "x" else b)[v] -= h
+            vm.set_positions(a, b)
+            Em = vm.compute()["E"]
+            fd = -(Ep - Em) / (2 * h)
+            an = r0["F" + comp][v]
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (v, comp, fd, an)
+    vm.set_positions(vx, vy)
+    r = vm.compute()
+    assert abs(r["Fx"].sum()) < 1e-9 and abs(r["Fy"].sum()) < 1e-9
+    assert abs(r["A"].sum() - L * L) < 1e-9 * L * L
+
+
+def test_single_t1_known_answer(oracle):
+    vm, ax, ay = _lattice_vm(oracle, thr=0.04)
+    vn, vcn, cvn, cv = vm.topology()
+    r = vm.compute()
+    vx, vy = r["vx"].copy(), r["vy"].copy()
+    L = vm.L
+    # pick the vertex pair of the first vertical edge (x equal) and shrink it
+    v1 = 0
+    v2 = None
+    for f in range(3):
+        c = vn[3 * v1 + f]
+        d = vx[c] - vx[v1]
+        d -= L * np.round(d / L)
+        if abs(d) < 1e-9:
+            v2 = c
+    assert v2 is not None
+    mid = np.array([vx[v1], (vy[v1] + vy[v2]) / 2])
+    dy = vy[v2] - vy[v1]
+    dy -= L * np.round(dy / L)
+    mid[1] = vy[v1] + dy / 2
+    vx[v1], vy[v1] = mid[0], mid[1] - 0.01
+    vx[v2], vy[v2] = mid[0], mid[1] + 0.01
+    vx, vy = vx % L, vy % L
+    vm.set_positions(vx, vy)
+    cells_before = set(vcn[3 * v1: 3 * v1 + 3]) | set(vcn[3 * v2: 3 * v2 + 3])
+    flips = vm.t1()
+    assert flips == 1 and vm.last_flagged() == 1
+    vn2, vcn2, cvn2, cv2 = vm.topology()
+    assert cvn2.sum() == cvn.sum()
+    changed = {c: (cvn[c], cvn2[c]) for c in cells_before}
+    assert sorted(changed.values()) == [(6, 5), (6, 5), (6, 7), (6, 7)]
+    r2 = vm.compute()
+    e_old = np.array([0.0, 0.02])
+    e_new = np.array([r2["vx"][v1] - r2["vx"][v2], r2["vy"][v1] - r2["vy"][v2]])
+    e_new -= L * np.round(e_new / L)
+    assert abs(np.dot(e_new, e_old)) < 1e-12
+    assert abs(np.linalg.norm(e_new) - 2 * np.linalg.norm(e_old)) < 1e-12
+    # vertex bookkeeping stays consistent: every vertex is in the lists of its 3 cells
+    for v in range(vm.Nv):
+        for c in vcn2[3 * v: 3 * v + 3]:
+            assert v in cv2[c, : cvn2[c]]
+
+
+def test_brownian_zero_temperature_relaxes(oracle):
+    vm, _ = oracle.vertex_from_random(200, 6, P0=3.9, dt=0.005, T=0.0)
+    E0 = vm.compute()["E"]
+    z = np.zeros(vm.Nv)
+    for _ in range(50):
+        vm.brownian_step(z, z)
+    E1 = vm.compute()["E"]
+    assert E1 < E0
+    vn, vcn, cvn, cv = vm.topology()
+    assert cvn.sum() == 3 * vm.Nv
