@@ -320,11 +320,13 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
 }
 
 // True when the current ring of cell i already is its exact Delaunay star:
-// consecutive neighbours strictly counter-clockwise, winding once, and no cell-list
-// candidate strictly inside any of its triangles' circumcircles (exact in-circle
-// test; every circle lies inside the scanned block). The Delaunay triangulation is
-// unique for points in general position, so keeping such a ring gives the same
-// result as rebuilding it, for a fraction of the work.
+// consecutive neighbours strictly counter-clockwise, winding once, and no point
+// strictly inside any of its triangles' circumcircles (exact in-circle test on
+// the cell-list bins covering each circle's bounding box, widened by 1e-9 L for
+// rounding at bin edges). The Delaunay triangulation is unique for points in
+// general position, so keeping such a ring gives the same result as rebuilding it
+// for a fraction of the work. Circles wider than an eighth of the box are left to
+// the rebuild.
 __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
     const int N = b.num_cells;
     const int z = b.nn[i];
@@ -333,49 +335,41 @@ __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int 
     const dvec2 pi = ldpos(pos, i);
     const double bs = sp.boxsize;
     const int xs = sp.xsize;
-    // orientation, winding and the largest circumcircle
-    double turn = 0.0, need = 0.0;
-    dvec2 d0 = min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi);
-    for (int k = 0; k < z; ++k) {
-        dvec2 d1 = min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi);
-        if (orient0(d0.x, d0.y, d1.x, d1.y) <= 0) return false;
-        turn += atan2(d0.x * d1.y - d0.y * d1.x, d0.x * d1.x + d0.y * d1.y);
-        dvec2 c = circ0<double>(d0, d1);
-        need = fmax(need, 2.0 * sqrt(c.x * c.x + c.y * c.y));
-        d0 = d1;
-    }
-    if (!(turn < 9.42)) return false;  // 2 pi: winds once (3 pi separates 2 pi from 4 pi)
-    int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
-    int jb = max(0, min(xs - 1, (int)floor(pi.y / bs)));
-    const int wmax = max(1, xs / 2);
-    int w = 1;
-    bool full = false;
-    for (;; ++w) {
-        if (w > wmax) return false;
-        full = (2 * w + 1 >= xs);
-        double rcov = fmin(fmin(pi.x - (ib - w) * bs, (ib + w + 1) * bs - pi.x),
-                           fmin(pi.y - (jb - w) * bs, (jb + w + 1) * bs - pi.y));
-        if (full || need <= rcov * (1.0 - 1e-9)) break;
-    }
-    bool ok = true;
+    double turn = 0.0;
     int ja = b.nbr[(z - 1) * N + i];
     dvec2 da = min_dist(box, ldpos(pos, ja), pi);
-    for (int k = 0; k < z && ok; ++k) {
-        int jb2 = b.nbr[k * N + i];
-        dvec2 db = min_dist(box, ldpos(pos, jb2), pi);
-        dvec2 c = circ0<double>(da, db);
-        double r2 = c.x * c.x + c.y * c.y;
-        scan_block(b, sp, ib, jb, w, [&](int m) {
-            if (!ok || m == i || m == ja || m == jb2) return;
-            dvec2 dm = min_dist(box, ldpos(pos, m), pi);
-            double ex = dm.x - c.x, ey = dm.y - c.y;
-            if (ex * ex + ey * ey > r2 * (1.0 + 1e-8)) return;  // clearly outside
-            if (incircle0(da.x, da.y, db.x, db.y, dm.x, dm.y) > 0) ok = false;
-        });
+    for (int k = 0; k < z; ++k) {
+        const int jb2 = b.nbr[k * N + i];
+        const dvec2 db = min_dist(box, ldpos(pos, jb2), pi);
+        if (orient0(da.x, da.y, db.x, db.y) <= 0) return false;
+        turn += atan2(da.x * db.y - da.y * db.x, da.x * db.x + da.y * db.y);
+        const dvec2 c = circ0<double>(da, db);
+        const double r2 = c.x * c.x + c.y * c.y;
+        const double r = sqrt(r2);
+        if (!(r < 0.125 * sp.L)) return false;
+        const double m = r + 1e-9 * sp.L;
+        const int x0 = (int)floor((pi.x + c.x - m) / bs), x1 = (int)floor((pi.x + c.x + m) / bs);
+        const int y0 = (int)floor((pi.y + c.y - m) / bs), y1 = (int)floor((pi.y + c.y + m) / bs);
+        for (int gy = y0; gy <= y1; ++gy) {
+            const int cy = ((gy % xs) + xs) % xs;
+            for (int gx = x0; gx <= x1; ++gx) {
+                const int cx = ((gx % xs) + xs) % xs;
+                const int bin = sp.bin_off + cy * xs + cx;
+                const int cnt = min((int)b.bin_count[bin], BCAP);
+                for (int q = 0; q < cnt; ++q) {
+                    const int mm = b.bin_idx[bin * BCAP + q];
+                    if (mm == i || mm == ja || mm == jb2) continue;
+                    const dvec2 dm = min_dist(box, ldpos(pos, mm), pi);
+                    const double ex = dm.x - c.x, ey = dm.y - c.y;
+                    if (ex * ex + ey * ey > r2 * (1.0 + 1e-8)) continue;  // clearly outside
+                    if (incircle0(da.x, da.y, db.x, db.y, dm.x, dm.y) > 0) return false;
+                }
+            }
+        }
         ja = jb2;
         da = db;
     }
-    return ok;
+    return turn < 9.42;  // 2 pi: winds once (3 pi separates 2 pi from 4 pi)
 }
 
 // reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
@@ -499,14 +493,28 @@ __global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int tes
             // wrapped distances are at most L/sqrt(2): a wider circle contains
             // every candidate, exactly as the reference's wrap loops decide
             const bool all_inside = rad > 0.5 * sp.L * sp.L * (1.0 + 1e-9);
+            // A bin whose rectangle (seen from v, unwrapped) stays farther than the
+            // radius from Q (plus 1e-9 L for rounding and bin-edge assignment)
+            // holds no point the reference test can find inside: skip it. Valid
+            // while the window and circle stay well inside half a box, so every
+            // minimum image is the unwrapped one.
+            const double r1 = sqrt(rad) + 1e-9 * sp.L;
+            const bool can_skip = !all_inside && (wcheck + 2) * sp.boxsize + 2.0 * r1 < 0.5 * sp.L;
             for (int ii = ib - wcheck; ii <= ib + wcheck; ++ii) {
                 int cx = ii;
                 if (cx < 0) cx += xs;
                 if (cx >= xs) cx -= xs;
+                const double x0 = ii * sp.boxsize - v.x;
+                const double ddx = fmax(fmax(x0 - Q.x, Q.x - (x0 + sp.boxsize)), 0.0);
                 for (int jj = jb - wcheck; jj <= jb + wcheck; ++jj) {
                     int cy = jj;
                     if (cy < 0) cy += xs;
                     if (cy >= xs) cy -= xs;
+                    if (can_skip) {
+                        const double y0 = jj * sp.boxsize - v.y;
+                        const double ddy = fmax(fmax(y0 - Q.y, Q.y - (y0 + sp.boxsize)), 0.0);
+                        if (ddx * ddx + ddy * ddy > r1 * r1) continue;
+                    }
                     int bin = sp.bin_off + cy * xs + cx;
                     int cnt = min((int)b.bin_count[bin], BCAP);
                     for (int q = 0; q < cnt; ++q) {
