@@ -12,6 +12,7 @@
 // needs them, so a whole step is a fixed launch sequence that can be captured
 // in a hipGraph with no host round trip.
 #include <hip/hip_runtime.h>
+#include <climits>
 
 #include "../../include/cellgpu_amd.h"
 #include "cgpu_device.h"
@@ -56,6 +57,46 @@ template <> __device__ inline void st2<double>(void *a, int i, vec2<double> v) {
 template <typename R> __device__ inline vec2<R> cvt(dvec2 d) { return vec2<R>{(R)d.x, (R)d.y}; }
 
 __device__ inline int ring_at(const cgpu_voronoi_batch &b, int i, int k) { return b.nbr[k * b.num_cells + i]; }
+
+// wave-aggregated reservation: lanes with pred get consecutive slots of *counter
+// (one atomic per wave instead of one per lane); returns -1 for the others
+__device__ inline int wave_reserve(bool pred, int *counter) {
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+// Wave-aggregated add and maximum of per-lane values into *addr for the lanes
+// with part set. Every lane of the wave must make the call (converged wave):
+// the cross-lane tree reads all 64 lanes. One atomic per wave when all
+// participating lanes name the same address, else one per participating lane.
+__device__ inline void wave_add_max(bool part, int *sum_addr, int *max_addr, int val) {
+    const unsigned long long pm = __ballot(part);
+    if (pm == 0ull) return;
+    const int leader = __ffsll((long long)pm) - 1;
+    const long long lead = __shfl((long long)(uintptr_t)sum_addr, leader);
+    const bool same = __ballot(part && (long long)(uintptr_t)sum_addr != lead) == 0ull;
+    if (!same) {
+        if (part) {
+            atomicAdd(sum_addr, val);
+            atomicMax(max_addr, val);
+        }
+        return;
+    }
+    int sum = part ? val : 0, mx = part ? val : INT_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        mx = max(mx, __shfl_xor(mx, o));
+    }
+    if (__lane_id() == leader) {
+        atomicAdd(sum_addr, sum);
+        atomicMax(max_addr, mx);
+    }
+}
 
 // wave-aggregated append: every lane with pred gets a distinct slot of list
 __device__ inline void wave_append(bool pred, int value, int *list, int *counter) {
@@ -266,8 +307,10 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
     const int wmax = max(1, xs / 2);
     for (int w = 1; w <= wmax; ++w) {
         const bool full = (2 * w + 1 >= xs);
-        double rcov = fmin(fmin(pi.x - (ib - w) * bs, (ib + w + 1) * bs - pi.x),
-                           fmin(pi.y - (jb - w) * bs, (jb + w + 1) * bs - pi.y));
+        // scanned block relative to pi, shrunk by 1e-9 L for rounding at bin edges
+        const double mg = 1e-9 * sp.L;
+        const double xlo = (ib - w) * bs - pi.x + mg, xhi = (ib + w + 1) * bs - pi.x - mg;
+        const double ylo = (jb - w) * bs - pi.y + mg, yhi = (jb + w + 1) * bs - pi.y - mg;
         // nearest candidate
         Cand first{-1, {0, 0}};
         int count = 0;
@@ -297,9 +340,14 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
             });
             if (best.id < 0) { ok = false; break; }
             if (!full) {
+                // certified when the closed circumdisk lies inside the scanned
+                // block: no unscanned point can be inside it
                 dvec2 c = circ0<double>(cur.d, best.d);
                 double rr = sqrt(c.x * c.x + c.y * c.y);
-                if (!(2.0 * rr <= rcov * (1.0 - 1e-9))) { ok = false; break; }
+                if (!(c.x - rr >= xlo && c.x + rr <= xhi && c.y - rr >= ylo && c.y + rr <= yhi)) {
+                    ok = false;
+                    break;
+                }
             }
             cur = best;
             if (cur.id == first.id) break;
@@ -319,6 +367,15 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
     return -1;
 }
 
+// polar-angle quadrant of a nonzero vector: [0, pi/2) -> 0, [pi/2, pi) -> 1,
+// [pi, 3pi/2) -> 2, [3pi/2, 2pi) -> 3 (exact sign tests)
+__device__ inline int quadrant(dvec2 d) {
+    if (d.y >= 0.0 && d.x > 0.0) return 0;
+    if (d.x <= 0.0 && d.y > 0.0) return 1;
+    if (d.y <= 0.0 && d.x < 0.0) return 2;
+    return 3;
+}
+
 // True when the current ring of cell i already is its exact Delaunay star:
 // consecutive neighbours strictly counter-clockwise, winding once, and no point
 // strictly inside any of its triangles' circumcircles (exact in-circle test on
@@ -335,14 +392,16 @@ __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int 
     const dvec2 pi = ldpos(pos, i);
     const double bs = sp.boxsize;
     const int xs = sp.xsize;
-    double turn = 0.0;
+    int wraps = 0;
     int ja = b.nbr[(z - 1) * N + i];
     dvec2 da = min_dist(box, ldpos(pos, ja), pi);
     for (int k = 0; k < z; ++k) {
         const int jb2 = b.nbr[k * N + i];
         const dvec2 db = min_dist(box, ldpos(pos, jb2), pi);
         if (orient0(da.x, da.y, db.x, db.y) <= 0) return false;
-        turn += atan2(da.x * db.y - da.y * db.x, da.x * db.x + da.y * db.y);
+        // every step turns counter-clockwise by less than pi, so the polar angle
+        // decreases (quadrant index drops) exactly once per full turn
+        wraps += (quadrant(db) < quadrant(da)) ? 1 : 0;
         const dvec2 c = circ0<double>(da, db);
         const double r2 = c.x * c.x + c.y * c.y;
         const double r = sqrt(r2);
@@ -369,7 +428,7 @@ __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int 
         ja = jb2;
         da = db;
     }
-    return turn < 9.42;  // 2 pi: winds once (3 pi separates 2 pi from 4 pi)
+    return wraps == 1;
 }
 
 // reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
@@ -451,10 +510,9 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
 }
 
 __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
-    if (atomicExch(&b.repair[m], 1) == 0) {
-        int q = atomicAdd(&b.counters[0], 1);
-        b.flagged[q] = m;
-    }
+    const bool fresh = atomicExch(&b.repair[m], 1) == 0;
+    const int q = wave_reserve(fresh, &b.counters[0]);
+    if (fresh) b.flagged[q] = m;
 }
 
 // FP64 empty-circumcircle test of the triangles owned by cell i (i smallest
@@ -556,8 +614,9 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
         int z = b.nn[i];
         for (int k = -1; k < z; ++k) {
             int x = (k < 0) ? i : b.nbr[k * N + i];
-            if (atomicExch(&b.mark[x], 1) == 0) {
-                int slot = atomicAdd(&b.counters[1], 1);
+            const bool fresh = atomicExch(&b.mark[x], 1) == 0;
+            const int slot = wave_reserve(fresh, &b.counters[1]);
+            if (fresh) {
                 b.nfq[slot] = x;
                 atomicAdd(&b.st[b.cell_sys[x]].nf_count, 1);
             }
@@ -621,8 +680,8 @@ __global__ void k_star_local(cgpu_voronoi_batch b, const double *pos) {
         b.nn[i] = z;
         for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
         int tnew = owned_triangles(b, i);
-        atomicAdd(&st.nn_delta, z - zold);
-        atomicAdd(&st.tri_delta, tnew - told);
+        if (z != zold) atomicAdd(&st.nn_delta, z - zold);
+        if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
     }
 }
 
@@ -656,30 +715,30 @@ __global__ void k_global_fallback(cgpu_voronoi_batch b) {
 }
 
 // global retriangulation of queued systems (globalTriangulationCGAL)
+// (block size a multiple of 64: whole waves stay in the loop together)
 __global__ void k_star_global(cgpu_voronoi_batch b, const double *pos) {
     const int nq = b.counters[2];
     const int N = b.num_cells;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        int i = b.gq[q];
+    for (int q0 = blockIdx.x * blockDim.x; q0 < nq; q0 += gridDim.x * blockDim.x) {
+        const int q = q0 + threadIdx.x;
+        const bool valid = q < nq;
+        int i = valid ? b.gq[q] : 0;
         int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
-        if (verify_ring(b, pos, i, b.sys[s])) {
-            int z = b.nn[i];
-            atomicMax(&st.max_deg, z);
-            atomicAdd(&st.nn_delta, z);
-            continue;
+        int z = b.nn[i];
+        if (valid && !verify_ring(b, pos, i, b.sys[s])) {
+            int ring[CAP];
+            z = delaunay_star(b, pos, i, b.sys[s], ring);
+            if (z < 0) {
+                st.error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
+                b.counters[4] = 1;
+                z = 0;
+            } else {
+                b.nn[i] = z;
+                for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+            }
         }
-        int ring[CAP];
-        int z = delaunay_star(b, pos, i, b.sys[s], ring);
-        if (z < 0) {
-            st.error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
-            b.counters[4] = 1;
-            continue;
-        }
-        b.nn[i] = z;
-        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
-        atomicMax(&st.max_deg, z);
-        atomicAdd(&st.nn_delta, z);
+        wave_add_max(valid, &st.nn_delta, &st.max_deg, z);
     }
 }
 

@@ -159,9 +159,10 @@ int delaunayStar(int i, const vector<d2> &P, const Box &box, const CellList &cl,
                     cd.push_back(box.minDist(P[m], pi));
                 }
             }
-        // radius of the disk around pi covered by the scanned block
-        double rcov = std::min(std::min(pi.x - (ib - w) * cl.boxsize, (ib + w + 1) * cl.boxsize - pi.x),
-                               std::min(pi.y - (jb - w) * cl.boxsize, (jb + w + 1) * cl.boxsize - pi.y));
+        // scanned block relative to pi, shrunk by 1e-9 L for rounding at bin edges
+        const double mg = 1e-9 * box.L;
+        const double xlo = (ib - w) * cl.boxsize - pi.x + mg, xhi = (ib + w + 1) * cl.boxsize - pi.x - mg;
+        const double ylo = (jb - w) * cl.boxsize - pi.y + mg, yhi = (jb + w + 1) * cl.boxsize - pi.y - mg;
         bool fullbox = (2 * w + 1 >= xs);
         int nc = (int)cid.size();
         if (nc < 3) continue;
@@ -188,7 +189,11 @@ int delaunayStar(int i, const vector<d2> &P, const Box &box, const CellList &cl,
             if (!fullbox) {
                 d2 c = circumcenter0(cd[cur], cd[best]);
                 double rr = std::sqrt(c.x * c.x + c.y * c.y);
-                if (!(2.0 * rr <= rcov * (1.0 - 1e-9))) { ok = false; break; }
+                // certified: the closed circumdisk lies inside the scanned block
+                if (!(c.x - rr >= xlo && c.x + rr <= xhi && c.y - rr >= ylo && c.y + rr <= yhi)) {
+                    ok = false;
+                    break;
+                }
             }
             cur = best;
             if (cur == first) break;
