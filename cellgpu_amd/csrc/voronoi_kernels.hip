@@ -70,49 +70,6 @@ __device__ inline int wave_reserve(bool pred, int *counter) {
     base = __shfl(base, leader);
     return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
-// Wave-aggregated add and maximum of per-lane values into *addr for the lanes
-// with part set. Every lane of the wave must make the call (converged wave):
-// the cross-lane tree reads all 64 lanes. One atomic per wave when all
-// participating lanes name the same address, else one per participating lane.
-__device__ inline void wave_add_max(bool part, int *sum_addr, int *max_addr, int val) {
-    const unsigned long long pm = __ballot(part);
-    if (pm == 0ull) return;
-    const int leader = __ffsll((long long)pm) - 1;
-    const long long lead = __shfl((long long)(uintptr_t)sum_addr, leader);
-    const bool same = __ballot(part && (long long)(uintptr_t)sum_addr != lead) == 0ull;
-    if (!same) {
-        if (part) {
-            atomicAdd(sum_addr, val);
-            atomicMax(max_addr, val);
-        }
-        return;
-    }
-    int sum = part ? val : 0, mx = part ? val : INT_MIN;
-    for (int o = 32; o > 0; o >>= 1) {
-        sum += __shfl_xor(sum, o);
-        mx = max(mx, __shfl_xor(mx, o));
-    }
-    if (__lane_id() == leader) {
-        atomicAdd(sum_addr, sum);
-        atomicMax(max_addr, mx);
-    }
-}
-
-// wave-aggregated append: every lane with pred gets a distinct slot of list
-__device__ inline void wave_append(bool pred, int value, int *list, int *counter) {
-    unsigned long long m = __ballot(pred);
-    if (m == 0ull) return;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    if (pred) {
-        int rank = __popcll(m & ((1ull << lane) - 1ull));
-        list[base + rank] = value;
-    }
-}
-
 // ------------------------------------------------- precision-dispatched pieces
 // FP64 path: the reference expressions, evaluated exactly as written.
 // FP32 path: the same expressions in float, except that a circumcenter (or the
@@ -431,6 +388,214 @@ __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int 
     return wraps == 1;
 }
 
+// ------------------------------------------ wave-cooperative star (repair)
+// One wavefront per queued cell. The candidates of a (2w+1)^2 bin block sit one
+// per lane, the current ring one member per lane, so ring verification and the
+// gift-wrapping selections are wave-wide reductions under exact total orders
+// instead of serial scans with dependent loads. The result is the same ring as
+// verify_ring / delaunay_star give: a verified ring is the unique Delaunay star,
+// and each wrap step selects the minimum of a strict total order, which no
+// evaluation order changes.
+struct WCand {
+    int id;
+    double x, y;
+};
+__device__ inline WCand shfl_xor_cand(WCand c, int o) {
+    return WCand{__shfl_xor(c.id, o), __shfl_xor(c.x, o), __shfl_xor(c.y, o)};
+}
+// nearest-neighbour order: squared norm, then index (exact)
+__device__ inline bool nearer(const WCand &a, const WCand &b) {
+    if (a.id < 0) return false;
+    if (b.id < 0) return true;
+    const int c = cmp_norm2(a.x, a.y, b.x, b.y);
+    return c < 0 || (c == 0 && a.id < b.id);
+}
+// wrap order seen from cur (all points strictly left of 0 -> cur): a precedes b
+// when a lies strictly inside the circle through 0, cur and b, then by index
+__device__ inline bool wraps_before(double cx, double cy, const WCand &a, const WCand &b) {
+    if (a.id < 0) return false;
+    if (b.id < 0) return true;
+    const int ic = incircle0(cx, cy, b.x, b.y, a.x, a.y);
+    return ic > 0 || (ic == 0 && a.id < b.id);
+}
+
+// Candidates of the block of half-width w around bin (ib, jb), one per lane
+// (cell i itself excluded: id -1). Returns the number of points in the block
+// (i included), or -1 when they do not fit in one wave. Requires nb^2 <= 64.
+__device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double *pos, const Box &box, dvec2 pi,
+                                      int i, const cgpu_sys_params &sp, int ib, int jb, int w, WCand &c) {
+    const int lane = __lane_id();
+    const int xs = sp.xsize;
+    const int nb = min(2 * w + 1, xs);
+    const int B = nb * nb;
+    int cnt = 0, bin = 0;
+    if (lane < B) {
+        int cx = ib - w + lane % nb;
+        int cy = jb - w + lane / nb;
+        cx = ((cx % xs) + xs) % xs;
+        cy = ((cy % xs) + xs) % xs;
+        bin = sp.bin_off + cy * xs + cx;
+        cnt = min((int)b.bin_count[bin], BCAP);
+    }
+    int incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    c.id = -1;
+    c.x = c.y = 0.0;
+    if (total > 64) return -1;
+    const int excl = incl - cnt;
+    // bin of slot `lane`: the last bin whose exclusive offset is <= lane (an
+    // empty bin shares its offset with the next one)
+    int t = 0;
+    for (int u = 1; u < B; ++u)
+        if (__shfl(excl, u) <= lane) t = u;
+    const int tbin = __shfl(bin, t), toff = __shfl(excl, t);
+    if (lane < total) {
+        const int m = b.bin_idx[tbin * BCAP + (lane - toff)];
+        if (m != i) {
+            const dvec2 d = min_dist(box, ldpos(pos, m), pi);
+            c = WCand{m, d.x, d.y};
+        }
+    }
+    return total;
+}
+
+// Star of cell i by one wave. Returns the new degree (ring in lanes 0..z-1 of
+// *ring_out, canonical rotation), 0 when the current ring is verified as the
+// Delaunay star (nothing to write), -1 when no certified star exists within
+// the largest block, -2 when the degree exceeds CAP, -3 when the block's
+// candidates do not fit in one wave (the caller uses the serial path).
+__device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp,
+                         int &ring_out) {
+    const int lane = __lane_id();
+    const int N = b.num_cells;
+    const Box box = make_box(sp.L);
+    const dvec2 pi = ldpos(pos, i);
+    const double bs = sp.boxsize;
+    const int xs = sp.xsize;
+    const int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
+    const int jb = max(0, min(xs - 1, (int)floor(pi.y / bs)));
+    const int wmax = max(1, xs / 2);
+    const double mg = 1e-9 * sp.L;
+
+    // current ring: member k in lane k, with the triangle (k-1, k) and its circle
+    const int zold = b.nn[i];
+    int verify = (zold >= 3 && zold <= CAP) ? 1 : 0;  // 1 undecided, 0 rebuild, 2 verified
+    int rid = -1, pid = -1;
+    double rx = 0, ry = 0, px = 0, py = 0, ccx = 0, ccy = 0, cr2 = 0, cr = 0;
+    if (verify) {
+        if (lane < zold) {
+            rid = b.nbr[lane * N + i];
+            const dvec2 d = min_dist(box, ldpos(pos, rid), pi);
+            rx = d.x;
+            ry = d.y;
+        }
+        const int src = (lane + zold - 1) % zold;
+        pid = __shfl(rid, src);
+        px = __shfl(rx, src);
+        py = __shfl(ry, src);
+        bool bad = false, wrapped = false;
+        if (lane < zold) {
+            bad = orient0(px, py, rx, ry) <= 0;
+            wrapped = quadrant(dvec2{rx, ry}) < quadrant(dvec2{px, py});
+            const dvec2 cc = circ0<double>(dvec2{px, py}, dvec2{rx, ry});
+            ccx = cc.x;
+            ccy = cc.y;
+            cr2 = cc.x * cc.x + cc.y * cc.y;
+            cr = sqrt(cr2);
+        }
+        if (__ballot(bad) != 0ull || __popcll(__ballot(wrapped)) != 1) verify = 0;
+    }
+
+    for (int w = 1; w <= wmax; ++w) {
+        const int nb = min(2 * w + 1, xs);
+        if (nb * nb > 64) return -3;
+        WCand c;
+        const int total = wave_load_block(b, pos, box, pi, i, sp, ib, jb, w, c);
+        if (total < 0) return -3;
+        const bool full = (2 * w + 1 >= xs);
+        const double xlo = (ib - w) * bs - pi.x + mg, xhi = (ib + w + 1) * bs - pi.x - mg;
+        const double ylo = (jb - w) * bs - pi.y + mg, yhi = (jb + w + 1) * bs - pi.y - mg;
+
+        if (verify == 1) {
+            if (full) {
+                verify = 0;
+            } else {
+                const bool fits = lane >= zold ||
+                                  (ccx - cr >= xlo && ccx + cr <= xhi && ccy - cr >= ylo && ccy + cr <= yhi);
+                if (__ballot(!fits) != 0ull) continue;  // a circle leaves the block: widen
+                bool viol = false;
+                for (int k = 0; k < zold; ++k) {
+                    const double cx = __shfl(ccx, k), cy = __shfl(ccy, k), r2 = __shfl(cr2, k);
+                    const double ax = __shfl(px, k), ay = __shfl(py, k), bx = __shfl(rx, k), by = __shfl(ry, k);
+                    const int ia = __shfl(pid, k), ibb = __shfl(rid, k);
+                    if (c.id >= 0 && c.id != ia && c.id != ibb) {
+                        const double ex = c.x - cx, ey = c.y - cy;
+                        if (!(ex * ex + ey * ey > r2 * (1.0 + 1e-8)) && incircle0(ax, ay, bx, by, c.x, c.y) > 0)
+                            viol = true;
+                    }
+                }
+                verify = (__ballot(viol) == 0ull) ? 2 : 0;
+            }
+            if (verify == 2) return 0;
+        }
+
+        // gift wrapping over the block's candidates
+        if (__popcll(__ballot(c.id >= 0)) < 3) continue;
+        WCand first = c;
+        for (int o = 32; o > 0; o >>= 1) {
+            const WCand oth = shfl_xor_cand(first, o);
+            if (nearer(oth, first)) first = oth;
+        }
+        WCand cur = first;
+        int z = 0, myring = -1;
+        bool ok = true;
+        for (int guard = 0; guard <= CAP; ++guard) {
+            if (z >= CAP) return -2;
+            if (lane == z) myring = cur.id;
+            ++z;
+            WCand best = c;
+            if (!(c.id >= 0 && c.id != cur.id && orient0(cur.x, cur.y, c.x, c.y) > 0)) best.id = -1;
+            for (int o = 32; o > 0; o >>= 1) {
+                const WCand oth = shfl_xor_cand(best, o);
+                if (wraps_before(cur.x, cur.y, oth, best)) best = oth;
+            }
+            if (best.id < 0) {
+                ok = false;
+                break;
+            }
+            if (!full) {
+                const dvec2 cc = circ0<double>(dvec2{cur.x, cur.y}, dvec2{best.x, best.y});
+                const double rr = sqrt(cc.x * cc.x + cc.y * cc.y);
+                if (!(cc.x - rr >= xlo && cc.x + rr <= xhi && cc.y - rr >= ylo && cc.y + rr <= yhi)) {
+                    ok = false;
+                    break;
+                }
+            }
+            cur = best;
+            if (cur.id == first.id) break;
+        }
+        if (!ok || cur.id != first.id) continue;
+        // canonical rotation: smallest index first
+        int mn = (lane < z) ? myring : INT_MAX;
+        for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o));
+        const int k0 = __ffsll((long long)__ballot(lane < z && myring == mn)) - 1;
+        ring_out = __shfl(myring, (lane + k0) % z);
+        return z;
+    }
+    return -1;
+}
+
+// number of triangles of the ring held in lanes 0..z-1 owned by cell i
+__device__ inline int wave_owned(int i, int ring_lane, int z) {
+    const int lane = __lane_id();
+    const int nxt = __shfl(ring_lane, (lane + 1) % max(z, 1));
+    return __popcll(__ballot(lane < z && i < ring_lane && i < nxt));
+}
+
 // reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
 // A ring that does not contain nm1 leaves that slot's delOther unchanged.
 __device__ bool del_sets(const cgpu_voronoi_batch &b, int i) {
@@ -475,6 +640,7 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
         b.counters[0] = 0;
         b.counters[1] = 0;
         b.counters[2] = 0;
+        b.counters[6] = 0;
     }
     if (s < b.num_systems) {
         cgpu_sys_state &st = b.st[s];
@@ -658,12 +824,14 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
 
 // local repair: new Delaunay star of each NeedsFixing cell (reference
 // repairTriangulation, voronoiModelBase.cpp:429-505), with the neighMax rule and
-// the changes of the 6N / 2N invariants accumulated per system
-__global__ void k_star_local(cgpu_voronoi_batch b, const double *pos) {
-    const int nq = b.counters[1];
+// the changes of the 6N / 2N invariants accumulated per system. Serial form, one
+// thread per cell of queue[0 .. *count): used for the cells whose bin block
+// does not fit the wave-cooperative kernel.
+__global__ void k_star_local(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
+    const int nq = *count;
     const int N = b.num_cells;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        int i = b.nfq[q];
+        int i = queue[q];
         int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
         if (st.mode != 1) continue;
@@ -685,6 +853,42 @@ __global__ void k_star_local(cgpu_voronoi_batch b, const double *pos) {
     }
 }
 
+// the same, one wavefront per NeedsFixing cell; cells that need the serial
+// form are queued in b.flagged / counters[6]
+__global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
+    const int nq = b.counters[1];
+    const int N = b.num_cells;
+    const int lane = __lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+        const int i = b.nfq[q];
+        const int s = b.cell_sys[i];
+        cgpu_sys_state &st = b.st[s];
+        if (st.mode != 1) continue;
+        int ring = -1;
+        const int z = wave_star(b, pos, i, b.sys[s], ring);
+        if (z == 0) continue;  // verified: unchanged
+        if (z == -3) {
+            if (lane == 0) b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+            continue;
+        }
+        if (z < 0 || z > st.neigh_max) {
+            if (lane == 0) st.need_global = 1;
+            continue;
+        }
+        const int zold = b.nn[i];
+        const int rold = (lane < zold && lane < CAP) ? b.nbr[lane * N + i] : -1;
+        const int told = wave_owned(i, rold, min(zold, CAP));
+        const int tnew = wave_owned(i, ring, z);
+        if (lane < z) b.nbr[lane * N + i] = ring;
+        if (lane == 0) {
+            b.nn[i] = z;
+            if (z != zold) atomicAdd(&st.nn_delta, z - zold);
+            if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
+        }
+    }
+}
+
 // delOther of repaired cells (reference enforceTopology, voronoiModelBase.cpp:
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
@@ -701,9 +905,11 @@ __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     }
 }
 
-// one block per system: a failed local repair becomes a global one
+// one block per system: a failed local repair becomes a global one; block 0
+// also empties the serial-form queue for the global pass
 __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     const int s = blockIdx.x;
+    if (s == 0 && threadIdx.x == 0) b.counters[6] = 0;
     cgpu_sys_state &st = b.st[s];
     if (!(st.mode == 1 && st.need_global)) return;
     __syncthreads();
@@ -714,31 +920,90 @@ __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     enqueue_system(b, b.sys[s]);
 }
 
-// global retriangulation of queued systems (globalTriangulationCGAL)
-// (block size a multiple of 64: whole waves stay in the loop together)
-__global__ void k_star_global(cgpu_voronoi_batch b, const double *pos) {
+// global retriangulation of queued systems (globalTriangulationCGAL), serial
+// form over queue[0 .. *count)
+__global__ void k_star_global(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
+    const int nq = *count;
+    const int N = b.num_cells;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const int i = queue[q];
+        const int s = b.cell_sys[i];
+        if (verify_ring(b, pos, i, b.sys[s])) continue;
+        int ring[CAP];
+        const int z = delaunay_star(b, pos, i, b.sys[s], ring);
+        if (z < 0) {
+            b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
+            b.counters[4] = 1;
+            continue;
+        }
+        b.nn[i] = z;
+        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+    }
+}
+
+// the same, one wavefront per cell of the global queue
+__global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, const double *pos) {
     const int nq = b.counters[2];
     const int N = b.num_cells;
-    for (int q0 = blockIdx.x * blockDim.x; q0 < nq; q0 += gridDim.x * blockDim.x) {
-        const int q = q0 + threadIdx.x;
-        const bool valid = q < nq;
-        int i = valid ? b.gq[q] : 0;
-        int s = b.cell_sys[i];
-        cgpu_sys_state &st = b.st[s];
-        int z = b.nn[i];
-        if (valid && !verify_ring(b, pos, i, b.sys[s])) {
-            int ring[CAP];
-            z = delaunay_star(b, pos, i, b.sys[s], ring);
-            if (z < 0) {
-                st.error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
-                b.counters[4] = 1;
-                z = 0;
-            } else {
-                b.nn[i] = z;
-                for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
-            }
+    const int lane = __lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+        const int i = b.gq[q];
+        const int s = b.cell_sys[i];
+        int ring = -1;
+        const int z = wave_star(b, pos, i, b.sys[s], ring);
+        if (z == 0) continue;
+        if (z == -3) {
+            if (lane == 0) b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+            continue;
         }
-        wave_add_max(valid, &st.nn_delta, &st.max_deg, z);
+        if (z < 0) {
+            if (lane == 0) {
+                b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
+                b.counters[4] = 1;
+            }
+            continue;
+        }
+        if (lane < z) b.nbr[lane * N + i] = ring;
+        if (lane == 0) b.nn[i] = z;
+    }
+}
+
+// one block per retriangulated system: degree sum (6N invariant,
+// voronoiModelBase.cpp:263) and neighMax = max degree + 2 (even) or + 1 (odd)
+// (:309-312)
+__global__ void k_global_stats(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x;
+    const cgpu_sys_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    if (st.mode < 2) return;
+    int sum = 0, mx = 0;
+    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) {
+        const int z = b.nn[sp.off + t];
+        sum += z;
+        mx = max(mx, z);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        mx = max(mx, __shfl_xor(mx, o));
+    }
+    __shared__ int ssum[16], smax[16];
+    const int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        ssum[wv] = sum;
+        smax[wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            sum += ssum[k];
+            mx = max(mx, smax[k]);
+        }
+        st.max_deg = mx;
+        st.nn_delta = sum;
+        st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
+        st.global_fixes += 1;
+        if (sum != 6 * sp.n) st.error = CGPU_ERR_TOPOLOGY;
     }
 }
 
@@ -752,16 +1017,8 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
         if (q < ng) {
             int i = b.gq[q];
             int s = b.cell_sys[i];
-            const cgpu_sys_params &sp = b.sys[s];
-            cgpu_sys_state &st = b.st[s];
             del_sets(b, i);
-            cell_geometry<R>(b, pos, i, sp);
-            if (i == sp.off) {
-                int nmax = st.max_deg;
-                st.neigh_max = (nmax % 2 == 0) ? nmax + 2 : nmax + 1;
-                st.global_fixes += 1;
-                if (st.nn_delta != 6 * sp.n) st.error = CGPU_ERR_TOPOLOGY;
-            }
+            cell_geometry<R>(b, pos, i, b.sys[s]);
         } else {
             int i = b.nfq[q - ng];
             b.mark[i] = 0;
@@ -1042,18 +1299,31 @@ inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR
 int g_block = 256;
 constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
 constexpr int REPAIR_THREADS = 64;
+constexpr int STAR_WAVE_BLOCKS = 4096;  // 16384 wavefronts, one queued cell each at a time
+constexpr int SERIAL_BLOCKS = 64;       // serial-form fallback grid
 
-int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
-    hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
-    hipLaunchKernelGGL(k_star_local, dim3(REPAIR_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos);
-    hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_star_global, dim3(REPAIR_BLOCKS * 4), dim3(REPAIR_THREADS), 0, st, b, pos);
+// global pass over the queued systems: wave kernel, serial form for the cells
+// it hands back, per-system degree statistics, delOther and geometry
+void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
+    hipLaunchKernelGGL(k_star_global_wave, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_global, dim3(SERIAL_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos, b.flagged,
+                       b.counters + 6);
+    hipLaunchKernelGGL(k_global_stats, dim3(b.num_systems), dim3(256), 0, st, b);
     if (b.precision)
         hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
     else
         hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
+}
+
+int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
+    hipLaunchKernelGGL(k_star_local_wave, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_local, dim3(SERIAL_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos, b.flagged,
+                       b.counters + 6);
+    hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
+    global_pass(b, pos, st);
     return check();
 }
 
@@ -1098,11 +1368,7 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     int r = cgpu_compute_cell_list(b, 0, stream);
     if (r) return r;
     hipLaunchKernelGGL(k_global_by_count, dim3(b->num_systems), dim3(256), 0, st, *b, 1);
-    hipLaunchKernelGGL(k_star_global, dim3(REPAIR_BLOCKS * 4), dim3(REPAIR_THREADS), 0, st, *b, b->pos);
-    if (b->precision)
-        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, *b, b->pos);
-    else
-        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, *b, b->pos);
+    global_pass(*b, b->pos, st);
     return check();
 }
 

@@ -108,7 +108,7 @@ typedef struct cgpu_voronoi_batch {
     int *flagged;                /* [N] list of flagged cells       */
     int *nfq;                    /* [N] NeedsFixing queue           */
     int *gq;                     /* [N] global-retriangulation queue */
-    int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow, 5 nonfinite */
+    int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow, 5 nonfinite, 6 serial-star queue (in flagged) */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
 } cgpu_voronoi_batch;
 
