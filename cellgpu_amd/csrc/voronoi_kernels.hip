@@ -681,92 +681,263 @@ __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
     if (fresh) b.flagged[q] = m;
 }
 
-// FP64 empty-circumcircle test of the triangles owned by cell i (i smallest
-// index), reference voronoiModelBase.cu:24-109 incl. its search window, then
-// the geometry of cell i with its current ring.
-template <typename R>
-__global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.num_cells) return;
-    const int s = b.cell_sys[i];
-    const cgpu_sys_params &sp = b.sys[s];
-    if (!b.st[s].active) return;
-    if (test) {
-        const Box box = make_box(sp.L);
-        const int N = b.num_cells;
-        const int z = b.nn[i];
-        const int xs = sp.xsize;
-        const dvec2 v = ldpos(pos, i);
-        int jn = b.nbr[i];
-        for (int k = 0; k < z; ++k) {
-            int j = jn;
-            jn = b.nbr[((k + 1) % z) * N + i];
-            int l = jn;
-            if (!(i < j && i < l)) continue;
-            int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
-            dvec2 pt1 = min_dist(box, ldpos(pos, j), v), pt2 = min_dist(box, ldpos(pos, l), v);
-            dvec2 Q = circ0<double>(pt1, pt2);
-            double dx = pt1.x - Q.x, dy = pt1.y - Q.y;
-            double rad = sqrt(dx * dx + dy * dy);
-            bool bad = false;
-            // reference: wcheck = ceil(rad/boxsize) capped at xsize/2 (a non-finite
-            // radius takes the cap)
-            double wc = ceil(rad / sp.boxsize);
-            int wcheck = (wc < (double)(xs / 2)) ? (int)wc : xs / 2;
-            rad = rad * rad;
-            // wrapped distances are at most L/sqrt(2): a wider circle contains
-            // every candidate, exactly as the reference's wrap loops decide
-            const bool all_inside = rad > 0.5 * sp.L * sp.L * (1.0 + 1e-9);
-            // A bin whose rectangle (seen from v, unwrapped) stays farther than the
-            // radius from Q (plus 1e-9 L for rounding and bin-edge assignment)
-            // holds no point the reference test can find inside: skip it. Valid
-            // while the window and circle stay well inside half a box, so every
-            // minimum image is the unwrapped one.
-            const double r1 = sqrt(rad) + 1e-9 * sp.L;
-            const bool can_skip = !all_inside && (wcheck + 2) * sp.boxsize + 2.0 * r1 < 0.5 * sp.L;
-            for (int ii = ib - wcheck; ii <= ib + wcheck; ++ii) {
-                int cx = ii;
-                if (cx < 0) cx += xs;
-                if (cx >= xs) cx -= xs;
-                const double x0 = ii * sp.boxsize - v.x;
-                const double ddx = fmax(fmax(x0 - Q.x, Q.x - (x0 + sp.boxsize)), 0.0);
-                for (int jj = jb - wcheck; jj <= jb + wcheck; ++jj) {
-                    int cy = jj;
-                    if (cy < 0) cy += xs;
-                    if (cy >= xs) cy -= xs;
-                    if (can_skip) {
-                        const double y0 = jj * sp.boxsize - v.y;
-                        const double ddy = fmax(fmax(y0 - Q.y, Q.y - (y0 + sp.boxsize)), 0.0);
-                        if (ddx * ddx + ddy * ddy > r1 * r1) continue;
-                    }
-                    int bin = sp.bin_off + cy * xs + cx;
-                    int cnt = min((int)b.bin_count[bin], BCAP);
-                    for (int q = 0; q < cnt; ++q) {
-                        int m = b.bin_idx[bin * BCAP + q];
-                        bool inside = all_inside;
-                        if (!inside) {
-                            dvec2 a = min_dist(box, ldpos(pos, m), v);
-                            dvec2 c = min_dist(box, a, Q);
-                            inside = c.x * c.x + c.y * c.y < rad;
-                        }
-                        if (inside) {
-                            if (m != i && m != j && m != l) {
-                                bad = true;
-                                flag_cell(b, m);
-                            }
-                        }
-                    }
+// Reference window of one triangle (voronoiModelBase.cu:24-109): circumcircle
+// of (0, pt1, pt2) relative to cell i at v, wcheck = ceil(rad / boxsize) capped
+// at xsize / 2 (a non-finite radius takes the cap), bins ib +- wcheck around
+// i's (unclamped) bin with one periodic wrap per index.
+struct TriWindow {
+    dvec2 Q;
+    double rad2;      // squared radius
+    double r1;        // radius + 1e-9 L, for bin rejection
+    int wcheck;
+    bool all_inside;  // wider than any wrapped distance: every candidate is inside
+    bool can_skip;    // bin rejection valid (window and circle well inside half a box)
+};
+__device__ inline TriWindow tri_window(const cgpu_sys_params &sp, dvec2 pt1, dvec2 pt2) {
+    TriWindow t;
+    t.Q = circ0<double>(pt1, pt2);
+    const double dx = pt1.x - t.Q.x, dy = pt1.y - t.Q.y;
+    const double rad = sqrt(dx * dx + dy * dy);
+    const int xs = sp.xsize;
+    const double wc = ceil(rad / sp.boxsize);
+    t.wcheck = (wc < (double)(xs / 2)) ? (int)wc : xs / 2;
+    t.rad2 = rad * rad;
+    // wrapped distances are at most L/sqrt(2): a wider circle contains every
+    // candidate, exactly as the reference's wrap loops decide
+    t.all_inside = t.rad2 > 0.5 * sp.L * sp.L * (1.0 + 1e-9);
+    // A bin whose rectangle (seen from v, unwrapped) stays farther than the
+    // radius from Q (plus 1e-9 L for rounding and bin-edge assignment) holds no
+    // point the reference test can find inside. Valid while the window and the
+    // circle stay well inside half a box, so every minimum image is the
+    // unwrapped one.
+    t.r1 = sqrt(t.rad2) + 1e-9 * sp.L;
+    t.can_skip = !t.all_inside && (t.wcheck + 2) * sp.boxsize + 2.0 * t.r1 < 0.5 * sp.L;
+    return t;
+}
+// true when bin column ii / row jj (unwrapped indices) may hold a point inside
+__device__ inline bool bin_may_hit(const cgpu_sys_params &sp, const TriWindow &t, dvec2 v, int ii, int jj) {
+    if (!t.can_skip) return true;
+    const double x0 = ii * sp.boxsize - v.x, y0 = jj * sp.boxsize - v.y;
+    const double ddx = fmax(fmax(x0 - t.Q.x, t.Q.x - (x0 + sp.boxsize)), 0.0);
+    const double ddy = fmax(fmax(y0 - t.Q.y, t.Q.y - (y0 + sp.boxsize)), 0.0);
+    return !(ddx * ddx + ddy * ddy > t.r1 * t.r1);
+}
+// reference inside test of candidate m: minDist(minDist(p_m, v), Q) < rad^2
+__device__ inline bool cand_inside(const Box &box, const TriWindow &t, dvec2 a) {
+    if (t.all_inside) return true;
+    const dvec2 c = min_dist(box, a, t.Q);
+    return c.x * c.x + c.y * c.y < t.rad2;
+}
+__device__ inline void flag_triangle(const cgpu_voronoi_batch &b, int s, int i, int j, int l) {
+    b.st[s].any_fail = 1;
+    flag_cell(b, i);
+    flag_cell(b, j);
+    flag_cell(b, l);
+}
+
+// serial scan of one triangle's whole window (any wcheck)
+__device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, const cgpu_sys_params &sp,
+                              const Box &box, int s, int i, dvec2 v, int j, int l, const TriWindow &t) {
+    const int xs = sp.xsize;
+    const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+    bool bad = false;
+    for (int ii = ib - t.wcheck; ii <= ib + t.wcheck; ++ii) {
+        int cx = ii;
+        if (cx < 0) cx += xs;
+        if (cx >= xs) cx -= xs;
+        for (int jj = jb - t.wcheck; jj <= jb + t.wcheck; ++jj) {
+            int cy = jj;
+            if (cy < 0) cy += xs;
+            if (cy >= xs) cy -= xs;
+            if (!bin_may_hit(sp, t, v, ii, jj)) continue;
+            const int bin = sp.bin_off + cy * xs + cx;
+            const int cnt = min((int)b.bin_count[bin], BCAP);
+            for (int q = 0; q < cnt; ++q) {
+                const int m = b.bin_idx[bin * BCAP + q];
+                if (m == i || m == j || m == l) continue;
+                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(pos, m), v))) {
+                    bad = true;
+                    flag_cell(b, m);
                 }
-            }
-            if (bad) {
-                b.st[s].any_fail = 1;
-                flag_cell(b, i);
-                flag_cell(b, j);
-                flag_cell(b, l);
             }
         }
     }
-    cell_geometry<R>(b, pos, i, sp);
+    if (bad) flag_triangle(b, s, i, j, l);
+}
+
+// Empty-circumcircle test of the triangles owned by cell i (i the smallest
+// index of the three), reference voronoiModelBase.cu:24-109 with its search
+// window, then the geometry of cell i with its current ring (reference GPU
+// order, voronoiModelBase.cu:115-201). A team of 16 lanes per cell: lane u holds
+// ring members u and u + 16 and the triangles they start; lanes 0-8 each scan
+// one bin of the 3 x 3 window that serves every triangle with wcheck = 1;
+// triangles with a wider window are scanned serially by the lane that holds
+// them. Area and perimeter are summed in ring order, as the serial form does.
+constexpr int TEAM = 16;
+template <typename R>
+__global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+    const int lane = __lane_id();
+    const int u = lane & (TEAM - 1);
+    const int tb = lane & ~(TEAM - 1);
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / TEAM);
+    const int N = b.num_cells;
+    const bool live = i < N;
+    const int s = live ? b.cell_sys[i] : 0;
+    const cgpu_sys_params &sp = b.sys[s];
+    const int z = (live && b.st[s].active) ? min(b.nn[i], CAP) : 0;
+    const Box box = make_box(sp.L);
+    const dvec2 v = live ? ldpos(pos, i) : dvec2{0.0, 0.0};
+
+    // ring members u and u + 16, relative to v
+    int r0 = -1, r1 = -1;
+    dvec2 d0{0.0, 0.0}, d1{0.0, 0.0};
+    if (u < z) {
+        r0 = b.nbr[u * N + i];
+        d0 = min_dist(box, ldpos(pos, r0), v);
+    }
+    if (u + TEAM < z) {
+        r1 = b.nbr[(u + TEAM) * N + i];
+        d1 = min_dist(box, ldpos(pos, r1), v);
+    }
+    int zmax = z;
+    for (int o = 32; o > 0; o >>= 1) zmax = max(zmax, __shfl_xor(zmax, o));
+    const int zz = max(z, 1);
+    // ring member k of this team's cell (k may differ per lane)
+    auto ring_d = [&](int k) {
+        const int src = tb + (k & (TEAM - 1));
+        const double ax = __shfl(d0.x, src), ay = __shfl(d0.y, src);
+        const double bx = __shfl(d1.x, src), by = __shfl(d1.y, src);
+        return k < TEAM ? dvec2{ax, ay} : dvec2{bx, by};
+    };
+    auto ring_id = [&](int k) {
+        const int src = tb + (k & (TEAM - 1));
+        const int a = __shfl(r0, src), c = __shfl(r1, src);
+        return k < TEAM ? a : c;
+    };
+    // successor of slot u and of slot u + 16 (triangle (i, ring k, ring k+1))
+    const dvec2 e0 = ring_d((u + 1) % zz), e1 = ring_d((u + TEAM + 1) % zz);
+    const int n0 = ring_id((u + 1) % zz), n1 = ring_id((u + TEAM + 1) % zz);
+    // predecessor of slot u and of slot u + 16 (Voronoi vertex k = (ring k-1, ring k))
+    const dvec2 p0 = ring_d((u + zz - 1) % zz), p1 = ring_d((u + TEAM - 1) % zz);
+
+    if (test && z >= 3) {
+        const bool own0 = u < z && i < r0 && i < n0;
+        const bool own1 = u + TEAM < z && i < r1 && i < n1;
+        TriWindow t0{}, t1{};
+        if (own0) t0 = tri_window(sp, d0, e0);
+        if (own1) t1 = tri_window(sp, d1, e1);
+        // triangles outside the shared 3 x 3 window: serial scan by their lane
+        const bool sh0 = own0 && t0.wcheck == 1 && !t0.all_inside;
+        const bool sh1 = own1 && t1.wcheck == 1 && !t1.all_inside;
+        if (own0 && !sh0) test_triangle(b, pos, sp, box, s, i, v, r0, n0, t0);
+        if (own1 && !sh1) test_triangle(b, pos, sp, box, s, i, v, r1, n1, t1);
+        // shared window: lane u < 9 scans bin (ib - 1 + u % 3, jb - 1 + u / 3)
+        const int xs = sp.xsize;
+        const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+        const int ii = ib - 1 + u % 3, jj = jb - 1 + u / 3;
+        int cx = ii, cy = jj;
+        if (cx < 0) cx += xs;
+        if (cx >= xs) cx -= xs;
+        if (cy < 0) cy += xs;
+        if (cy >= xs) cy -= xs;
+        const int bin = sp.bin_off + cy * xs + cx;
+        const int cnt = (u < 9) ? min((int)b.bin_count[bin], BCAP) : 0;
+        // the first candidates of the bin stay in registers across triangles
+        constexpr int KEEP = 4;
+        int cid[KEEP];
+        dvec2 cd[KEEP];
+#pragma unroll
+        for (int q = 0; q < KEEP; ++q) {
+            cid[q] = -1;
+            cd[q] = dvec2{0.0, 0.0};
+            if (q < cnt) {
+                cid[q] = b.bin_idx[bin * BCAP + q];
+                cd[q] = min_dist(box, ldpos(pos, cid[q]), v);
+            }
+        }
+        for (int k = 0; k < zmax; ++k) {
+            // triangle k of this team's cell, from the lane that holds it
+            const int src = tb + (k & (TEAM - 1));
+            const bool hi = k >= TEAM;
+            const int shared = __shfl((int)(hi ? sh1 : sh0), src);
+            const double qx = __shfl(hi ? t1.Q.x : t0.Q.x, src), qy = __shfl(hi ? t1.Q.y : t0.Q.y, src);
+            const double rad2 = __shfl(hi ? t1.rad2 : t0.rad2, src), rr1 = __shfl(hi ? t1.r1 : t0.r1, src);
+            const int skip = __shfl((int)(hi ? t1.can_skip : t0.can_skip), src);
+            const int tj = __shfl(hi ? r1 : r0, src), tl = __shfl(hi ? n1 : n0, src);
+            if (!(k < z && shared && cnt > 0)) continue;
+            TriWindow t;
+            t.Q = dvec2{qx, qy};
+            t.rad2 = rad2;
+            t.r1 = rr1;
+            t.wcheck = 1;
+            t.all_inside = false;
+            t.can_skip = skip != 0;
+            if (!bin_may_hit(sp, t, v, ii, jj)) continue;
+            bool bad = false;
+            for (int q = 0; q < cnt; ++q) {
+                int m = -1;
+                dvec2 a{0.0, 0.0};
+                if (q < KEEP) {
+#pragma unroll
+                    for (int r = 0; r < KEEP; ++r)
+                        if (r == q) {
+                            m = cid[r];
+                            a = cd[r];
+                        }
+                } else {
+                    m = b.bin_idx[bin * BCAP + q];
+                    a = min_dist(box, ldpos(pos, m), v);
+                }
+                if (m == i || m == tj || m == tl) continue;
+                if (cand_inside(box, t, a)) {
+                    bad = true;
+                    flag_cell(b, m);
+                }
+            }
+            if (bad) flag_triangle(b, s, i, tj, tl);
+        }
+    }
+
+    // geometry: vertex k = circumcenter(ring k-1, ring k); every lane of the
+    // team forms the sums in ring order
+    vec2<R> w0{R(0), R(0)}, w1{R(0), R(0)};
+    if (u < z) w0 = Prec<R>::c0(p0, d0);
+    if (u + TEAM < z) w1 = Prec<R>::c0(p1, d1);
+    auto vert = [&](int k) {
+        const int src = tb + (k & (TEAM - 1));
+        const R ax = __shfl(w0.x, src), ay = __shfl(w0.y, src), bx = __shfl(w1.x, src), by = __shfl(w1.y, src);
+        return k < TEAM ? vec2<R>{ax, ay} : vec2<R>{bx, by};
+    };
+    const vec2<R> vfirst = vert(0);
+    vec2<R> vlast = vfirst;
+    R A = R(0), P = R(0);
+    for (int k = 1; k < zmax; ++k) {
+        const vec2<R> vnext = vert(k);
+        if (k < z) {
+            A += tri_area<R>(vlast, vnext);
+            const R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
+            P += sqrt(dx * dx + dy * dy);
+            vlast = vnext;
+        }
+    }
+    if (z >= 3 && u == 0) {
+        A += tri_area<R>(vlast, vfirst);
+        const R dx = vlast.x - vfirst.x, dy = vlast.y - vfirst.y;
+        P += sqrt(dx * dx + dy * dy);
+        st2<R>(b.AP, i, vec2<R>{A, P});
+        const vec2<R> pref = ld2<R>(b.APpref, i);
+        st2<R>(b.dAP, i, vec2<R>{(R)sp.KA * (A - pref.x), (R)sp.KP * (P - pref.y)});
+    }
+}
+
+// launch of the test + geometry kernel (16 lanes per cell)
+void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
+    const long long threads = (long long)b.num_cells * TEAM;
+    const dim3 g((unsigned)((threads + 255) / 256)), t(256);
+    if (b.precision)
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
+    else
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, b, pos, test);
 }
 
 // NeedsFixing = flagged cells and their current neighbours (reference
@@ -1348,11 +1519,7 @@ int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stre
 
 int cgpu_test_and_geometry(const cgpu_voronoi_batch *b, int test, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
-    if (b->precision)
-        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, S(stream), *b, b->pos, test);
-    else
-        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, S(stream), *b, b->pos, test);
+    launch_test_geometry(*b, b->pos, test, S(stream));
     return check();
 }
 
@@ -1416,11 +1583,7 @@ static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
     if (r) return r;
     r = cgpu_compute_cell_list(b, 1, stream);
     if (r) return r;
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
-    if (b->precision)
-        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, *b, b->pos_next, 1);
-    else
-        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, *b, b->pos_next, 1);
+    launch_test_geometry(*b, b->pos_next, 1, st);
     r = repair_chain(*b, b->pos_next, st);
     if (r) return r;
     hipLaunchKernelGGL(k_advance_step, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);

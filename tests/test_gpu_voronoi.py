@@ -81,20 +81,18 @@ def test_fp32_forces_within_1e5(oracle):
     assert err <= 1e-5, err
 
 
-@pytest.mark.parametrize("precision", ["fp64"])
-def test_spv_steps_match_oracle_stepwise(oracle, precision):
-    """Every step: oracle restarted from the GPU state, same noise, then compare."""
-    cg = _cg()
-    N = 400
-    b = cg.VoronoiBatch([N], seeds=[3], precision=precision)
-    b.set_prefs(1.0, 3.8)
-    b.set_motility(0.1, 1.0)
+def _stepwise(oracle, b, steps, seed, P0=3.8, v0=0.1):
+    """Every step: oracle restarted from the GPU state, same noise, then compare
+    positions (1e-12 L) and rings (bit-exact). Returns the oracle's repair events."""
+    N = int(b.n.sum())
+    b.set_prefs(1.0, P0)
+    b.set_motility(v0, 1.0)
     b.set_params(dt=0.01)
     b.set_noise_buffer(True)
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(seed)
     repairs = 0
-    for step in range(60):
-        o = _oracle_from_gpu(oracle, b, P0=3.8, v0=0.1, Dr=1.0, dt=0.01)
+    for step in range(steps):
+        o = _oracle_from_gpu(oracle, b, P0=P0, v0=v0, Dr=1.0, dt=0.01)
         eta = rng.standard_normal(N)
         b.upload_noise(eta)
         b.spv_steps(1, use_graph=False)
@@ -107,7 +105,43 @@ def test_spv_steps_match_oracle_stepwise(oracle, precision):
         onn, onbr, onm = o.topology(CAP)
         assert _rings(nn, nbr) == _rings(onn, onbr), "step %d" % step
         repairs += o.counters()["repair_events"]
-    assert repairs > 0
+    return repairs
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_spv_steps_match_oracle_stepwise(oracle, precision):
+    cg = _cg()
+    b = cg.VoronoiBatch([400], seeds=[3], precision=precision)
+    assert _stepwise(oracle, b, 60, 5) > 0
+
+
+@pytest.mark.parametrize("N", [30, 48])
+def test_small_systems_bit_exact(oracle, N):
+    """Systems whose second bin block already covers the whole box."""
+    cg = _cg()
+    b = cg.VoronoiBatch([N], seeds=[5], precision="fp64")
+    x, y, th, L = oracle.random_init(N, 5)
+    nn, nbr, nm = b.topology(CAP)
+    onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert _stepwise(oracle, b, 20, 9) >= 0
+
+
+def test_clustered_points_bit_exact(oracle):
+    """A dense cluster (about 15 points per bin) puts more points in a cell's bin
+    block than one wavefront holds: those stars take the serial path."""
+    cg = _cg()
+    rng = np.random.default_rng(11)
+    N0, Nc = 500, 120
+    L = float(np.sqrt(N0))
+    x = np.concatenate([rng.uniform(0, L, N0), rng.uniform(5.0, 8.5, Nc)])
+    y = np.concatenate([rng.uniform(0, L, N0), rng.uniform(5.0, 8.5, Nc)])
+    th = rng.uniform(0, 2 * np.pi, N0 + Nc)
+    b = cg.VoronoiBatch([N0 + Nc], L=[L], x=x, y=y, theta=th, precision="fp64")
+    nn, nbr, nm = b.topology(CAP)
+    onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert _stepwise(oracle, b, 15, 13) > 0
 
 
 def test_batch_equals_single_systems():
