@@ -113,6 +113,19 @@ def cpu_baseline(n, P0, v0, Dr, seeds, budget_s=12.0, steps=10):
                        % (used, steps))
 
 
+def copy_probe(mib, reps=5):
+    """Plain copy kernel of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +134,12 @@ def main():
     ap.add_argument("--timed-stage-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every system of the ensemble runs on a single MI355X "
               "(multi-GPU sharding of an ensemble is by system; see DESIGN.md)", file=sys.stderr)

@@ -767,173 +767,35 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
 
 // Empty-circumcircle test of the triangles owned by cell i (i the smallest
 // index of the three), reference voronoiModelBase.cu:24-109 with its search
-// window, then the geometry of cell i with its current ring (reference GPU
-// order, voronoiModelBase.cu:115-201). A team of 16 lanes per cell: lane u holds
-// ring members u and u + 16 and the triangles they start; lanes 0-8 each scan
-// one bin of the 3 x 3 window that serves every triangle with wcheck = 1;
-// triangles with a wider window are scanned serially by the lane that holds
-// them. Area and perimeter are summed in ring order, as the serial form does.
-constexpr int TEAM = 16;
+// window, then the geometry of cell i with its current ring.
 template <typename R>
-__global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
-    const int lane = __lane_id();
-    const int u = lane & (TEAM - 1);
-    const int tb = lane & ~(TEAM - 1);
-    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / TEAM);
-    const int N = b.num_cells;
-    const bool live = i < N;
-    const int s = live ? b.cell_sys[i] : 0;
+__global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
-    const int z = (live && b.st[s].active) ? min(b.nn[i], CAP) : 0;
-    const Box box = make_box(sp.L);
-    const dvec2 v = live ? ldpos(pos, i) : dvec2{0.0, 0.0};
-
-    // ring members u and u + 16, relative to v
-    int r0 = -1, r1 = -1;
-    dvec2 d0{0.0, 0.0}, d1{0.0, 0.0};
-    if (u < z) {
-        r0 = b.nbr[u * N + i];
-        d0 = min_dist(box, ldpos(pos, r0), v);
-    }
-    if (u + TEAM < z) {
-        r1 = b.nbr[(u + TEAM) * N + i];
-        d1 = min_dist(box, ldpos(pos, r1), v);
-    }
-    int zmax = z;
-    for (int o = 32; o > 0; o >>= 1) zmax = max(zmax, __shfl_xor(zmax, o));
-    const int zz = max(z, 1);
-    // ring member k of this team's cell (k may differ per lane)
-    auto ring_d = [&](int k) {
-        const int src = tb + (k & (TEAM - 1));
-        const double ax = __shfl(d0.x, src), ay = __shfl(d0.y, src);
-        const double bx = __shfl(d1.x, src), by = __shfl(d1.y, src);
-        return k < TEAM ? dvec2{ax, ay} : dvec2{bx, by};
-    };
-    auto ring_id = [&](int k) {
-        const int src = tb + (k & (TEAM - 1));
-        const int a = __shfl(r0, src), c = __shfl(r1, src);
-        return k < TEAM ? a : c;
-    };
-    // successor of slot u and of slot u + 16 (triangle (i, ring k, ring k+1))
-    const dvec2 e0 = ring_d((u + 1) % zz), e1 = ring_d((u + TEAM + 1) % zz);
-    const int n0 = ring_id((u + 1) % zz), n1 = ring_id((u + TEAM + 1) % zz);
-    // predecessor of slot u and of slot u + 16 (Voronoi vertex k = (ring k-1, ring k))
-    const dvec2 p0 = ring_d((u + zz - 1) % zz), p1 = ring_d((u + TEAM - 1) % zz);
-
-    if (test && z >= 3) {
-        const bool own0 = u < z && i < r0 && i < n0;
-        const bool own1 = u + TEAM < z && i < r1 && i < n1;
-        TriWindow t0{}, t1{};
-        if (own0) t0 = tri_window(sp, d0, e0);
-        if (own1) t1 = tri_window(sp, d1, e1);
-        // triangles outside the shared 3 x 3 window: serial scan by their lane
-        const bool sh0 = own0 && t0.wcheck == 1 && !t0.all_inside;
-        const bool sh1 = own1 && t1.wcheck == 1 && !t1.all_inside;
-        if (own0 && !sh0) test_triangle(b, pos, sp, box, s, i, v, r0, n0, t0);
-        if (own1 && !sh1) test_triangle(b, pos, sp, box, s, i, v, r1, n1, t1);
-        // shared window: lane u < 9 scans bin (ib - 1 + u % 3, jb - 1 + u / 3)
-        const int xs = sp.xsize;
-        const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
-        const int ii = ib - 1 + u % 3, jj = jb - 1 + u / 3;
-        int cx = ii, cy = jj;
-        if (cx < 0) cx += xs;
-        if (cx >= xs) cx -= xs;
-        if (cy < 0) cy += xs;
-        if (cy >= xs) cy -= xs;
-        const int bin = sp.bin_off + cy * xs + cx;
-        const int cnt = (u < 9) ? min((int)b.bin_count[bin], BCAP) : 0;
-        // the first candidates of the bin stay in registers across triangles
-        constexpr int KEEP = 4;
-        int cid[KEEP];
-        dvec2 cd[KEEP];
-#pragma unroll
-        for (int q = 0; q < KEEP; ++q) {
-            cid[q] = -1;
-            cd[q] = dvec2{0.0, 0.0};
-            if (q < cnt) {
-                cid[q] = b.bin_idx[bin * BCAP + q];
-                cd[q] = min_dist(box, ldpos(pos, cid[q]), v);
-            }
-        }
-        for (int k = 0; k < zmax; ++k) {
-            // triangle k of this team's cell, from the lane that holds it
-            const int src = tb + (k & (TEAM - 1));
-            const bool hi = k >= TEAM;
-            const int shared = __shfl((int)(hi ? sh1 : sh0), src);
-            const double qx = __shfl(hi ? t1.Q.x : t0.Q.x, src), qy = __shfl(hi ? t1.Q.y : t0.Q.y, src);
-            const double rad2 = __shfl(hi ? t1.rad2 : t0.rad2, src), rr1 = __shfl(hi ? t1.r1 : t0.r1, src);
-            const int skip = __shfl((int)(hi ? t1.can_skip : t0.can_skip), src);
-            const int tj = __shfl(hi ? r1 : r0, src), tl = __shfl(hi ? n1 : n0, src);
-            if (!(k < z && shared && cnt > 0)) continue;
-            TriWindow t;
-            t.Q = dvec2{qx, qy};
-            t.rad2 = rad2;
-            t.r1 = rr1;
-            t.wcheck = 1;
-            t.all_inside = false;
-            t.can_skip = skip != 0;
-            if (!bin_may_hit(sp, t, v, ii, jj)) continue;
-            bool bad = false;
-            for (int q = 0; q < cnt; ++q) {
-                int m = -1;
-                dvec2 a{0.0, 0.0};
-                if (q < KEEP) {
-#pragma unroll
-                    for (int r = 0; r < KEEP; ++r)
-                        if (r == q) {
-                            m = cid[r];
-                            a = cd[r];
-                        }
-                } else {
-                    m = b.bin_idx[bin * BCAP + q];
-                    a = min_dist(box, ldpos(pos, m), v);
-                }
-                if (m == i || m == tj || m == tl) continue;
-                if (cand_inside(box, t, a)) {
-                    bad = true;
-                    flag_cell(b, m);
-                }
-            }
-            if (bad) flag_triangle(b, s, i, tj, tl);
+    if (!b.st[s].active) return;
+    if (test) {
+        const Box box = make_box(sp.L);
+        const int N = b.num_cells;
+        const int z = b.nn[i];
+        const dvec2 v = ldpos(pos, i);
+        int jn = b.nbr[i];
+        for (int k = 0; k < z; ++k) {
+            const int j = jn;
+            jn = b.nbr[((k + 1) % z) * N + i];
+            const int l = jn;
+            if (!(i < j && i < l)) continue;
+            const dvec2 pt1 = min_dist(box, ldpos(pos, j), v), pt2 = min_dist(box, ldpos(pos, l), v);
+            test_triangle(b, pos, sp, box, s, i, v, j, l, tri_window(sp, pt1, pt2));
         }
     }
-
-    // geometry: vertex k = circumcenter(ring k-1, ring k); every lane of the
-    // team forms the sums in ring order
-    vec2<R> w0{R(0), R(0)}, w1{R(0), R(0)};
-    if (u < z) w0 = Prec<R>::c0(p0, d0);
-    if (u + TEAM < z) w1 = Prec<R>::c0(p1, d1);
-    auto vert = [&](int k) {
-        const int src = tb + (k & (TEAM - 1));
-        const R ax = __shfl(w0.x, src), ay = __shfl(w0.y, src), bx = __shfl(w1.x, src), by = __shfl(w1.y, src);
-        return k < TEAM ? vec2<R>{ax, ay} : vec2<R>{bx, by};
-    };
-    const vec2<R> vfirst = vert(0);
-    vec2<R> vlast = vfirst;
-    R A = R(0), P = R(0);
-    for (int k = 1; k < zmax; ++k) {
-        const vec2<R> vnext = vert(k);
-        if (k < z) {
-            A += tri_area<R>(vlast, vnext);
-            const R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
-            P += sqrt(dx * dx + dy * dy);
-            vlast = vnext;
-        }
-    }
-    if (z >= 3 && u == 0) {
-        A += tri_area<R>(vlast, vfirst);
-        const R dx = vlast.x - vfirst.x, dy = vlast.y - vfirst.y;
-        P += sqrt(dx * dx + dy * dy);
-        st2<R>(b.AP, i, vec2<R>{A, P});
-        const vec2<R> pref = ld2<R>(b.APpref, i);
-        st2<R>(b.dAP, i, vec2<R>{(R)sp.KA * (A - pref.x), (R)sp.KP * (P - pref.y)});
-    }
+    cell_geometry<R>(b, pos, i, sp);
 }
 
-// launch of the test + geometry kernel (16 lanes per cell)
+// launch of the test + geometry kernel (one thread per cell)
 void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
-    const long long threads = (long long)b.num_cells * TEAM;
-    const dim3 g((unsigned)((threads + 255) / 256)), t(256);
+    const dim3 g((unsigned)((b.num_cells + 255) / 256)), t(256);
     if (b.precision)
         hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
     else
