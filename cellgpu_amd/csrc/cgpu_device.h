@@ -22,6 +22,14 @@ using dvec2 = vec2<double>;
 struct Box {
     double L, half, inv;
 };
+// XCD-aware block index. Workgroups are dealt round-robin over the 8 XCDs, each
+// with its own L2; the remap gives every XCD one contiguous range of logical
+// blocks, so cells that are neighbours in memory share one L2.
+__device__ inline int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int per = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
+    return x * per + min(x, rem) + k;
+}
 CGPU_HD Box make_box(double L) { return Box{L, L * 0.5, 1. / L}; }
 // The wrap loops are capped (WRAP_MAX iterations): for coordinates inside the box
 // they run at most twice, and a non-finite or runaway value can never spin.

@@ -44,7 +44,7 @@ template <typename R> __device__ inline vec2<R> cv2(dvec2 d) { return vec2<R>{(R
 // and perimeter relative to the cell's vertex n-2
 template <typename R>
 __global__ void k_vm_geometry(cgpu_vertex_batch b, const double *pos) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= b.num_cells) return;
     const cgpu_vm_params &sp = b.sys[b.cell_sys[c]];
     const Box box = make_box(sp.L);
@@ -71,7 +71,7 @@ __global__ void k_vm_geometry(cgpu_vertex_batch b, const double *pos) {
 // update (brownianParticleDynamics.cu:20-39) and the move
 template <typename R>
 __global__ void k_vm_force_eom_move(cgpu_vertex_batch b, const double *pos, double *pos_next, int move) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= b.num_vertices) return;
     const int s = b.vert_sys[v];
     const cgpu_vm_params &sp = b.sys[s];
@@ -125,7 +125,7 @@ __global__ void k_vm_force_eom_move(cgpu_vertex_batch b, const double *pos, doub
 // length below the threshold, for v1 < v2; flagged edges go to the system's list
 // as (v1, v2) pairs
 __global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= b.num_vertices) return;
     const int s = b.vert_sys[v];
     if (!b.st[s].active) return;

@@ -661,7 +661,7 @@ __global__ void k_zero_bins(cgpu_voronoi_batch b) {
 
 // reference cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)), atomic slot
 __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
     dvec2 p = ldpos(pos, i);
@@ -770,7 +770,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
 // window, then the geometry of cell i with its current ring.
 template <typename R>
 __global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
@@ -807,7 +807,7 @@ void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int te
 __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     const int nflag = b.counters[0];
     const int N = b.num_cells;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
+    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
         int i = b.flagged[q];
         b.repair[i] = 0;
         int z = b.nn[i];
@@ -863,7 +863,7 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
 __global__ void k_star_local(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
     const int nq = *count;
     const int N = b.num_cells;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         int i = queue[q];
         int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+    for (int q = (xcd_block() * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
         const int i = b.nfq[q];
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     const int nq = b.counters[1];
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         int i = b.nfq[q];
         cgpu_sys_state &st = b.st[b.cell_sys[i]];
         if (st.mode != 1 || st.need_global) continue;
@@ -980,7 +980,7 @@ __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, 
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+    for (int q = (xcd_block() * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
         const int i = b.gq[q];
         const int s = b.cell_sys[i];
         int ring = -1;
@@ -1046,7 +1046,7 @@ template <typename R>
 __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
     const int ng = b.counters[2];
     const int nl = b.counters[1];
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
+    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
         if (q < ng) {
             int i = b.gq[q];
             int s = b.cell_sys[i];
@@ -1079,7 +1079,7 @@ __device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, con
 
 template <typename R>
 __global__ void k_forces(cgpu_voronoi_batch b, const double *pos) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
     st2<R>(b.force, i, force_with_exclusions<R>(b, pos, i, sp));
@@ -1109,7 +1109,7 @@ template <> __device__ inline double atan2_r<double>(double y, double x) { retur
 // kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
 template <typename R, int KIND>
 __global__ void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double *pos_next) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
@@ -1168,7 +1168,7 @@ __global__ void k_advance_step(cgpu_voronoi_batch b) {
 }
 
 __global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double *out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
     double z0, z1;
@@ -1198,7 +1198,7 @@ template <typename R> __device__ inline void fire_mix(vec2<R> &v, vec2<R> F, con
 template <typename R>
 __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
                                double *pos_next) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
@@ -1235,7 +1235,7 @@ __global__ void k_fire_clear_pending(cgpu_fire_state *fs, int S) {
 template <typename R>
 __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
                                  double *dots) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
     const cgpu_fire_state &f = fs[s];
@@ -1317,7 +1317,7 @@ __global__ void k_fire_park(cgpu_voronoi_batch b, const cgpu_fire_state *fs) {
 
 template <typename R>
 __global__ void k_fire_apply_mix(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_fire_state &f = fs[b.cell_sys[i]];
     vec2<R> v = ld2<R>(vel, i), F = ld2<R>(b.force, i);
