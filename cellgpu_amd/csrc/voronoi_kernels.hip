@@ -807,7 +807,7 @@ void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int te
 __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     const int nflag = b.counters[0];
     const int N = b.num_cells;
-    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
+    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
         int i = b.flagged[q];
         b.repair[i] = 0;
         int z = b.nn[i];
@@ -863,7 +863,7 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
 __global__ void k_star_local(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
     const int nq = *count;
     const int N = b.num_cells;
-    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         int i = queue[q];
         int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (xcd_block() * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+    for (int q = ((int)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
         const int i = b.nfq[q];
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     const int nq = b.counters[1];
-    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         int i = b.nfq[q];
         cgpu_sys_state &st = b.st[b.cell_sys[i]];
         if (st.mode != 1 || st.need_global) continue;
@@ -980,7 +980,7 @@ __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, 
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (xcd_block() * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+    for (int q = ((int)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
         const int i = b.gq[q];
         const int s = b.cell_sys[i];
         int ring = -1;
@@ -1046,7 +1046,7 @@ template <typename R>
 __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
     const int ng = b.counters[2];
     const int nl = b.counters[1];
-    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
+    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
         if (q < ng) {
             int i = b.gq[q];
             int s = b.cell_sys[i];
