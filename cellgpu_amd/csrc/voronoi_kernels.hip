@@ -454,9 +454,10 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
         if (__shfl(excl, u) <= lane) t = u;
     const int tbin = __shfl(bin, t), toff = __shfl(excl, t);
     if (lane < total) {
-        const int m = b.bin_idx[tbin * BCAP + (lane - toff)];
+        const int slot = tbin * BCAP + (lane - toff);
+        const int m = b.bin_idx[slot];
         if (m != i) {
-            const dvec2 d = min_dist(box, ldpos(pos, m), pi);
+            const dvec2 d = min_dist(box, ldpos(b.bin_pos, slot), pi);
             c = WCand{m, d.x, d.y};
         }
     }
@@ -640,7 +641,7 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
         b.counters[0] = 0;
         b.counters[1] = 0;
         b.counters[2] = 0;
-        b.counters[6] = 0;
+        for (int c = 6; c < 16; ++c) b.counters[c] = 0;
     }
     if (s < b.num_systems) {
         cgpu_sys_state &st = b.st[s];
@@ -669,10 +670,12 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
     int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
     int bin = sp.bin_off + jb * sp.xsize + ib;
     unsigned int slot = atomicAdd(&b.bin_count[bin], 1u);
-    if (slot < (unsigned)BCAP)
+    if (slot < (unsigned)BCAP) {
         b.bin_idx[bin * BCAP + slot] = i;
-    else
+        reinterpret_cast<double2 *>(b.bin_pos)[bin * BCAP + slot] = make_double2(p.x, p.y);
+    } else {
         b.counters[3] = 1;
+    }
 }
 
 __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
@@ -755,7 +758,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
             for (int q = 0; q < cnt; ++q) {
                 const int m = b.bin_idx[bin * BCAP + q];
                 if (m == i || m == j || m == l) continue;
-                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(pos, m), v))) {
+                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(b.bin_pos, bin * BCAP + q), v))) {
                     bad = true;
                     flag_cell(b, m);
                 }
@@ -765,41 +768,109 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
     if (bad) flag_triangle(b, s, i, j, l);
 }
 
-// Empty-circumcircle test of the triangles owned by cell i (i the smallest
-// index of the three), reference voronoiModelBase.cu:24-109 with its search
-// window, then the geometry of cell i with its current ring.
+// Geometry of cell i with its current ring; with enumerate set, also appends
+// the triangles cell i owns (i the smallest index of the three, the triangles
+// the reference tests, voronoiModelBase.cpp:380-423) to the triangle-list
+// segment of the XCD the block runs on.
 template <typename R>
-__global__ void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+__global__ void k_geometry_tris(cgpu_voronoi_batch b, const double *pos, int enumerate) {
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     if (!b.st[s].active) return;
-    if (test) {
-        const Box box = make_box(sp.L);
-        const int N = b.num_cells;
-        const int z = b.nn[i];
-        const dvec2 v = ldpos(pos, i);
-        int jn = b.nbr[i];
-        for (int k = 0; k < z; ++k) {
-            const int j = jn;
-            jn = b.nbr[((k + 1) % z) * N + i];
-            const int l = jn;
-            if (!(i < j && i < l)) continue;
-            const dvec2 pt1 = min_dist(box, ldpos(pos, j), v), pt2 = min_dist(box, ldpos(pos, l), v);
-            test_triangle(b, pos, sp, box, s, i, v, j, l, tri_window(sp, pt1, pt2));
+    cell_geometry<R>(b, pos, i, sp);
+    if (!enumerate) return;
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    const int seg = blockIdx.x & 7;
+    int4 *list = reinterpret_cast<int4 *>(b.tris) + (size_t)seg * b.tri_seg_cap;
+    int jn = b.nbr[i];
+    for (int k = 0; k < z; ++k) {
+        const int j = jn;
+        jn = b.nbr[((k + 1) % z) * N + i];
+        const bool own = i < j && i < jn;
+        const int slot = wave_reserve(own, &b.counters[8 + seg]);
+        if (own) {
+            if (slot < b.tri_seg_cap)
+                list[slot] = make_int4(i, j, jn, 0);
+            else
+                b.counters[7] = 1;
         }
     }
-    cell_geometry<R>(b, pos, i, sp);
 }
 
-// launch of the test + geometry kernel (one thread per cell)
+// Empty-circumcircle test, one thread per owned triangle (reference
+// gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
+// window). Blocks on XCD x read segment x, which the same XCD wrote. Triangles
+// with wcheck = 1 scan the 3 x 3 bins around cell i's bin with independent
+// loads; wider windows take the serial scan.
+__global__ __launch_bounds__(256) void k_test_tris(cgpu_voronoi_batch b, const double *pos) {
+    const int seg = blockIdx.x & 7;
+    const int nseg = b.counters[8 + seg];
+    const int nthreads = (int)(gridDim.x >> 3) * blockDim.x;
+    const int4 *list = reinterpret_cast<const int4 *>(b.tris) + (size_t)seg * b.tri_seg_cap;
+    const int lim = min(nseg, b.tri_seg_cap);
+    for (int q = (int)(blockIdx.x >> 3) * blockDim.x + threadIdx.x; q < lim; q += nthreads) {
+        const int4 t = list[q];
+        const int i = t.x, j = t.y, l = t.z;
+        const int s = b.cell_sys[i];
+        const cgpu_sys_params &sp = b.sys[s];
+        const Box box = make_box(sp.L);
+        const dvec2 v = ldpos(pos, i);
+        const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
+        if (w.wcheck != 1 || w.all_inside) {
+            test_triangle(b, pos, sp, box, s, i, v, j, l, w);
+            continue;
+        }
+        const int xs = sp.xsize;
+        const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+        int bins[9], cnt[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            const int ii = ib - 1 + u / 3, jj = jb - 1 + u % 3;
+            int cx = ii, cy = jj;
+            if (cx < 0) cx += xs;
+            if (cx >= xs) cx -= xs;
+            if (cy < 0) cy += xs;
+            if (cy >= xs) cy -= xs;
+            bins[u] = sp.bin_off + cy * xs + cx;
+            cnt[u] = bin_may_hit(sp, w, v, ii, jj) ? 1 : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 9; ++u) cnt[u] = cnt[u] ? min((int)b.bin_count[bins[u]], BCAP) : 0;
+        bool bad = false;
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            for (int c = 0; c < cnt[u]; ++c) {
+                const int slot = bins[u] * BCAP + c;
+                const int m = b.bin_idx[slot];
+                const dvec2 a = min_dist(box, ldpos(b.bin_pos, slot), v);
+                if (m == i || m == j || m == l) continue;
+                if (cand_inside(box, w, a)) {
+                    bad = true;
+                    flag_cell(b, m);
+                }
+            }
+        }
+        if (bad) flag_triangle(b, s, i, j, l);
+    }
+}
+
+// test (optional) + geometry: geometry and triangle list per cell, then the
+// per-triangle test
 void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
     const dim3 g((unsigned)((b.num_cells + 255) / 256)), t(256);
     if (b.precision)
-        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
+        hipLaunchKernelGGL(k_geometry_tris<float>, g, t, 0, st, b, pos, test);
     else
-        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, b, pos, test);
+        hipLaunchKernelGGL(k_geometry_tris<double>, g, t, 0, st, b, pos, test);
+    if (test) {
+        // 8 segments x enough blocks for a segment of 2N/8 triangles in about
+        // two grid-stride rounds
+        const int per = max(8, min(1024, (b.num_cells / 4 + 255) / 256));
+        hipLaunchKernelGGL(k_test_tris, dim3(8 * per), dim3(256), 0, st, b, pos);
+    }
 }
 
 // NeedsFixing = flagged cells and their current neighbours (reference
