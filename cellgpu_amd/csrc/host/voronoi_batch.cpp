@@ -77,13 +77,10 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     flagged_.alloc(N_);
     nfq_.alloc(N_);
     gq_.alloc(N_);
-    counters_.alloc(16);
+    counters_.alloc(8);
     bin_count_.alloc(total_bins_);
     bin_idx_.alloc((size_t)total_bins_ * CGPU_BIN_CAP);
     bin_pos_.alloc((size_t)total_bins_ * CGPU_BIN_CAP * 2);
-    // a valid triangulation has 2N triangles; each XCD segment can hold them all
-    tri_seg_cap_ = 2 * N_ + 64;
-    tris_.alloc((size_t)8 * tri_seg_cap_ * 4);
     pos0_.alloc(2 * (size_t)N_);
     pos1_.alloc(2 * (size_t)N_);
     AP_.alloc(2 * N_ * rsz_);
@@ -205,9 +202,6 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.counters = counters_.p;
     b.noise = noise_buffer_ ? noise_.p : nullptr;
     b.bin_pos = bin_pos_.p;
-    b.tris = tris_.p;
-    b.tri_seg_cap = tri_seg_cap_;
-    b.pad1 = 0;
     return b;
 }
 
@@ -421,10 +415,9 @@ int VoronoiBatch::sync() {
 
 int VoronoiBatch::status() {
     sync();
-    std::vector<int> c(16);
-    counters_.download(c.data(), 16);
+    std::vector<int> c(8);
+    counters_.download(c.data(), 8);
     if (c[3]) return CGPU_ERR_BIN_CAPACITY;
-    if (c[7]) return CGPU_ERR_TOPOLOGY;
     if (c[5]) return CGPU_ERR_NONFINITE;
     std::vector<cgpu_sys_state> st(S_);
     st_.download(st.data(), S_);

@@ -113,10 +113,9 @@ class VoronoiBatch {
     std::vector<int> cell_sys_h_;
     DevBuf<cgpu_sys_params> sys_;
     DevBuf<cgpu_sys_state> st_;
-    DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_, bin_idx_, tris_;
+    DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_, bin_idx_;
     DevBuf<unsigned int> bin_count_;
     DevBuf<double> pos0_, pos1_, noise_, bin_pos_;
-    int tri_seg_cap_ = 0;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
 };
 

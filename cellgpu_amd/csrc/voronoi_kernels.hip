@@ -641,7 +641,7 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
         b.counters[0] = 0;
         b.counters[1] = 0;
         b.counters[2] = 0;
-        for (int c = 6; c < 16; ++c) b.counters[c] = 0;
+        b.counters[6] = 0;
     }
     if (s < b.num_systems) {
         cgpu_sys_state &st = b.st[s];
@@ -768,109 +768,102 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
     if (bad) flag_triangle(b, s, i, j, l);
 }
 
-// Geometry of cell i with its current ring; with enumerate set, also appends
-// the triangles cell i owns (i the smallest index of the three, the triangles
-// the reference tests, voronoiModelBase.cpp:380-423) to the triangle-list
-// segment of the XCD the block runs on.
-template <typename R>
-__global__ void k_geometry_tris(cgpu_voronoi_batch b, const double *pos, int enumerate) {
-    const int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= b.num_cells) return;
+// Empty-circumcircle test of one owned triangle (i, j, l) (reference
+// gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
+// window): wcheck = 1 windows scan the 3 x 3 bins around cell i's bin with
+// independent loads, wider windows take the serial scan.
+__device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
-    if (!b.st[s].active) return;
-    cell_geometry<R>(b, pos, i, sp);
-    if (!enumerate) return;
-    const int N = b.num_cells;
-    const int z = b.nn[i];
-    const int seg = blockIdx.x & 7;
-    int4 *list = reinterpret_cast<int4 *>(b.tris) + (size_t)seg * b.tri_seg_cap;
-    int jn = b.nbr[i];
-    for (int k = 0; k < z; ++k) {
-        const int j = jn;
-        jn = b.nbr[((k + 1) % z) * N + i];
-        const bool own = i < j && i < jn;
-        const int slot = wave_reserve(own, &b.counters[8 + seg]);
-        if (own) {
-            if (slot < b.tri_seg_cap)
-                list[slot] = make_int4(i, j, jn, 0);
-            else
-                b.counters[7] = 1;
-        }
+    const Box box = make_box(sp.L);
+    const dvec2 v = ldpos(pos, i);
+    const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
+    if (w.wcheck != 1 || w.all_inside) {
+        test_triangle(b, pos, sp, box, s, i, v, j, l, w);
+        return;
     }
-}
-
-// Empty-circumcircle test, one thread per owned triangle (reference
-// gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
-// window). Blocks on XCD x read segment x, which the same XCD wrote. Triangles
-// with wcheck = 1 scan the 3 x 3 bins around cell i's bin with independent
-// loads; wider windows take the serial scan.
-__global__ __launch_bounds__(256) void k_test_tris(cgpu_voronoi_batch b, const double *pos) {
-    const int seg = blockIdx.x & 7;
-    const int nseg = b.counters[8 + seg];
-    const int nthreads = (int)(gridDim.x >> 3) * blockDim.x;
-    const int4 *list = reinterpret_cast<const int4 *>(b.tris) + (size_t)seg * b.tri_seg_cap;
-    const int lim = min(nseg, b.tri_seg_cap);
-    for (int q = (int)(blockIdx.x >> 3) * blockDim.x + threadIdx.x; q < lim; q += nthreads) {
-        const int4 t = list[q];
-        const int i = t.x, j = t.y, l = t.z;
-        const int s = b.cell_sys[i];
-        const cgpu_sys_params &sp = b.sys[s];
-        const Box box = make_box(sp.L);
-        const dvec2 v = ldpos(pos, i);
-        const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
-        if (w.wcheck != 1 || w.all_inside) {
-            test_triangle(b, pos, sp, box, s, i, v, j, l, w);
-            continue;
-        }
-        const int xs = sp.xsize;
-        const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
-        int bins[9], cnt[9];
+    const int xs = sp.xsize;
+    const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+    int bins[9], cnt[9];
 #pragma unroll
-        for (int u = 0; u < 9; ++u) {
-            const int ii = ib - 1 + u / 3, jj = jb - 1 + u % 3;
-            int cx = ii, cy = jj;
-            if (cx < 0) cx += xs;
-            if (cx >= xs) cx -= xs;
-            if (cy < 0) cy += xs;
-            if (cy >= xs) cy -= xs;
-            bins[u] = sp.bin_off + cy * xs + cx;
-            cnt[u] = bin_may_hit(sp, w, v, ii, jj) ? 1 : 0;
-        }
+    for (int u = 0; u < 9; ++u) {
+        const int ii = ib - 1 + u / 3, jj = jb - 1 + u % 3;
+        int cx = ii, cy = jj;
+        if (cx < 0) cx += xs;
+        if (cx >= xs) cx -= xs;
+        if (cy < 0) cy += xs;
+        if (cy >= xs) cy -= xs;
+        bins[u] = sp.bin_off + cy * xs + cx;
+        cnt[u] = bin_may_hit(sp, w, v, ii, jj) ? 1 : 0;
+    }
 #pragma unroll
-        for (int u = 0; u < 9; ++u) cnt[u] = cnt[u] ? min((int)b.bin_count[bins[u]], BCAP) : 0;
-        bool bad = false;
+    for (int u = 0; u < 9; ++u) cnt[u] = cnt[u] ? min((int)b.bin_count[bins[u]], BCAP) : 0;
+    bool bad = false;
 #pragma unroll
-        for (int u = 0; u < 9; ++u) {
-            for (int c = 0; c < cnt[u]; ++c) {
-                const int slot = bins[u] * BCAP + c;
-                const int m = b.bin_idx[slot];
-                const dvec2 a = min_dist(box, ldpos(b.bin_pos, slot), v);
-                if (m == i || m == j || m == l) continue;
-                if (cand_inside(box, w, a)) {
-                    bad = true;
-                    flag_cell(b, m);
-                }
+    for (int u = 0; u < 9; ++u) {
+        for (int c = 0; c < cnt[u]; ++c) {
+            const int slot = bins[u] * BCAP + c;
+            const int m = b.bin_idx[slot];
+            const dvec2 a = min_dist(box, ldpos(b.bin_pos, slot), v);
+            if (m == i || m == j || m == l) continue;
+            if (cand_inside(box, w, a)) {
+                bad = true;
+                flag_cell(b, m);
             }
         }
-        if (bad) flag_triangle(b, s, i, j, l);
+    }
+    if (bad) flag_triangle(b, s, i, j, l);
+}
+
+// Geometry of the block's cells, then the test of the triangles they own (i
+// the smallest index of the three: the triangles the reference tests,
+// voronoiModelBase.cpp:380-423). The owned triangles are gathered in LDS and
+// dealt out over the block's threads, so a cell that owns many triangles does
+// not hold up its wavefront; triangles beyond the LDS list are tested by the
+// thread of their cell.
+constexpr int TRI_LDS = 1280;
+template <typename R>
+__global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+    __shared__ int3 stri[TRI_LDS];
+    __shared__ int scount;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) scount = 0;
+    __syncthreads();
+    const bool live = i < b.num_cells && b.st[b.cell_sys[i]].active;
+    if (live) {
+        const int s = b.cell_sys[i];
+        cell_geometry<R>(b, pos, i, b.sys[s]);
+        if (test) {
+            const int N = b.num_cells;
+            const int z = b.nn[i];
+            int jn = b.nbr[i];
+            for (int k = 0; k < z; ++k) {
+                const int j = jn;
+                jn = b.nbr[((k + 1) % z) * N + i];
+                if (!(i < j && i < jn)) continue;
+                const int slot = atomicAdd(&scount, 1);
+                if (slot < TRI_LDS)
+                    stri[slot] = make_int3(i, j, jn);
+                else
+                    test_owned_triangle(b, pos, i, j, jn);
+            }
+        }
+    }
+    if (!test) return;  // uniform
+    __syncthreads();
+    const int n = min(scount, TRI_LDS);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int3 t = stri[q];
+        test_owned_triangle(b, pos, t.x, t.y, t.z);
     }
 }
 
-// test (optional) + geometry: geometry and triangle list per cell, then the
-// per-triangle test
 void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
     const dim3 g((unsigned)((b.num_cells + 255) / 256)), t(256);
     if (b.precision)
-        hipLaunchKernelGGL(k_geometry_tris<float>, g, t, 0, st, b, pos, test);
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
     else
-        hipLaunchKernelGGL(k_geometry_tris<double>, g, t, 0, st, b, pos, test);
-    if (test) {
-        // 8 segments x enough blocks for a segment of 2N/8 triangles in about
-        // two grid-stride rounds
-        const int per = max(8, min(1024, (b.num_cells / 4 + 255) / 256));
-        hipLaunchKernelGGL(k_test_tris, dim3(8 * per), dim3(256), 0, st, b, pos);
-    }
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, b, pos, test);
 }
 
 // NeedsFixing = flagged cells and their current neighbours (reference

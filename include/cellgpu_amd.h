@@ -108,14 +108,10 @@ typedef struct cgpu_voronoi_batch {
     int *flagged;                /* [N] list of flagged cells       */
     int *nfq;                    /* [N] NeedsFixing queue           */
     int *gq;                     /* [N] global-retriangulation queue */
-    int *counters;               /* [16] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
-                                    5 nonfinite, 6 serial-star queue (in flagged), 7 triangle-list overflow,
-                                    8..15 sizes of the 8 triangle-list segments */
+    int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
+                                    5 nonfinite, 6 serial-star queue (in flagged) */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
     double *bin_pos;             /* [total_bins*BIN_CAP*2] positions of the binned cells, same slots as bin_idx */
-    int *tris;                   /* [8*tri_seg_cap*4] owned triangles (i, j, k, 0), one segment per XCD */
-    int tri_seg_cap;             /* entries per triangle-list segment */
-    int pad1;
 } cgpu_voronoi_batch;
 
 /* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
