@@ -33,7 +33,7 @@ class VoronoiBatchView(C.Structure):
         (name, C.c_void_p) for name in (
             "sys", "st", "cell_sys", "pos", "pos_next", "nn", "nbr", "oth", "AP", "dAP", "APpref", "mot", "vel",
             "theta", "force", "excl", "ext_force", "bin_count", "bin_idx", "repair", "mark", "flagged", "nfq", "gq",
-            "counters", "noise", "bin_pos")]
+            "counters", "noise")]
 
 
 def lib():

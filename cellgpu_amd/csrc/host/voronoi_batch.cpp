@@ -80,7 +80,6 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     counters_.alloc(8);
     bin_count_.alloc(total_bins_);
     bin_idx_.alloc((size_t)total_bins_ * CGPU_BIN_CAP);
-    bin_pos_.alloc((size_t)total_bins_ * CGPU_BIN_CAP * 2);
     pos0_.alloc(2 * (size_t)N_);
     pos1_.alloc(2 * (size_t)N_);
     AP_.alloc(2 * N_ * rsz_);
@@ -201,7 +200,6 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.gq = gq_.p;
     b.counters = counters_.p;
     b.noise = noise_buffer_ ? noise_.p : nullptr;
-    b.bin_pos = bin_pos_.p;
     return b;
 }
 

@@ -115,7 +115,7 @@ class VoronoiBatch {
     DevBuf<cgpu_sys_state> st_;
     DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_, bin_idx_;
     DevBuf<unsigned int> bin_count_;
-    DevBuf<double> pos0_, pos1_, noise_, bin_pos_;
+    DevBuf<double> pos0_, pos1_, noise_;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
 };
 

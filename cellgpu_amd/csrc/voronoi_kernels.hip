@@ -457,7 +457,7 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
         const int slot = tbin * BCAP + (lane - toff);
         const int m = b.bin_idx[slot];
         if (m != i) {
-            const dvec2 d = min_dist(box, ldpos(b.bin_pos, slot), pi);
+            const dvec2 d = min_dist(box, ldpos(pos, m), pi);
             c = WCand{m, d.x, d.y};
         }
     }
@@ -670,12 +670,10 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
     int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
     int bin = sp.bin_off + jb * sp.xsize + ib;
     unsigned int slot = atomicAdd(&b.bin_count[bin], 1u);
-    if (slot < (unsigned)BCAP) {
+    if (slot < (unsigned)BCAP)
         b.bin_idx[bin * BCAP + slot] = i;
-        reinterpret_cast<double2 *>(b.bin_pos)[bin * BCAP + slot] = make_double2(p.x, p.y);
-    } else {
+    else
         b.counters[3] = 1;
-    }
 }
 
 __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
@@ -758,7 +756,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
             for (int q = 0; q < cnt; ++q) {
                 const int m = b.bin_idx[bin * BCAP + q];
                 if (m == i || m == j || m == l) continue;
-                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(b.bin_pos, bin * BCAP + q), v))) {
+                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(pos, m), v))) {
                     bad = true;
                     flag_cell(b, m);
                 }
@@ -804,7 +802,7 @@ __device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *p
         for (int c = 0; c < cnt[u]; ++c) {
             const int slot = bins[u] * BCAP + c;
             const int m = b.bin_idx[slot];
-            const dvec2 a = min_dist(box, ldpos(b.bin_pos, slot), v);
+            const dvec2 a = min_dist(box, ldpos(pos, m), v);
             if (m == i || m == j || m == l) continue;
             if (cand_inside(box, w, a)) {
                 bad = true;

@@ -111,7 +111,6 @@ typedef struct cgpu_voronoi_batch {
     int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
                                     5 nonfinite, 6 serial-star queue (in flagged) */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
-    double *bin_pos;             /* [total_bins*BIN_CAP*2] positions of the binned cells, same slots as bin_idx */
 } cgpu_voronoi_batch;
 
 /* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
