@@ -419,11 +419,13 @@ __device__ inline bool wraps_before(double cx, double cy, const WCand &a, const 
     return ic > 0 || (ic == 0 && a.id < b.id);
 }
 
-// Candidates of the block of half-width w around bin (ib, jb), one per lane
-// (cell i itself excluded: id -1). Returns the number of points in the block
-// (i included), or -1 when they do not fit in one wave. Requires nb^2 <= 64.
+// Candidates of the block of half-width w around bin (ib, jb), K per lane
+// (slot lane + 64 r in c[r]; cell i itself and empty slots have id -1).
+// Returns the number of points in the block (i included), or -1 when they do
+// not fit in K slots per lane. Requires nb^2 <= 64.
+template <int K>
 __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double *pos, const Box &box, dvec2 pi,
-                                      int i, const cgpu_sys_params &sp, int ib, int jb, int w, WCand &c) {
+                                      int i, const cgpu_sys_params &sp, int ib, int jb, int w, WCand (&c)[K]) {
     const int lane = __lane_id();
     const int xs = sp.xsize;
     const int nb = min(2 * w + 1, xs);
@@ -443,32 +445,42 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
         if (lane >= o) incl += t;
     }
     const int total = __shfl(incl, 63);
-    c.id = -1;
-    c.x = c.y = 0.0;
-    if (total > 64) return -1;
+#pragma unroll
+    for (int r = 0; r < K; ++r) c[r] = WCand{-1, 0.0, 0.0};
+    if (total > 64 * K) return -1;
     const int excl = incl - cnt;
-    // bin of slot `lane`: the last bin whose exclusive offset is <= lane (an
-    // empty bin shares its offset with the next one)
-    int t = 0;
-    for (int u = 1; u < B; ++u)
-        if (__shfl(excl, u) <= lane) t = u;
-    const int tbin = __shfl(bin, t), toff = __shfl(excl, t);
-    if (lane < total) {
-        const int slot = tbin * BCAP + (lane - toff);
-        const int m = b.bin_idx[slot];
-        if (m != i) {
-            const dvec2 d = min_dist(box, ldpos(pos, m), pi);
-            c = WCand{m, d.x, d.y};
+    // bin of slot j: the last bin whose exclusive offset is <= j (an empty bin
+    // shares its offset with the next one)
+    int t[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) t[r] = 0;
+    for (int u = 1; u < B; ++u) {
+        const int e = __shfl(excl, u);
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            if (e <= lane + 64 * r) t[r] = u;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const int j = lane + 64 * r;
+        const int tbin = __shfl(bin, t[r]), toff = __shfl(excl, t[r]);
+        if (j < total) {
+            const int m = b.bin_idx[tbin * BCAP + (j - toff)];
+            if (m != i) {
+                const dvec2 d = min_dist(box, ldpos(pos, m), pi);
+                c[r] = WCand{m, d.x, d.y};
+            }
         }
     }
     return total;
 }
 
-// Star of cell i by one wave. Returns the new degree (ring in lanes 0..z-1 of
-// *ring_out, canonical rotation), 0 when the current ring is verified as the
-// Delaunay star (nothing to write), -1 when no certified star exists within
-// the largest block, -2 when the degree exceeds CAP, -3 when the block's
-// candidates do not fit in one wave (the caller uses the serial path).
+// Star of cell i by one wave. Returns the new degree (member k of the ring in
+// lane k's ring_out, canonical rotation), 0 when the current ring is verified
+// as the Delaunay star (nothing to write), -1 when no certified star exists
+// within the largest block, -2 when the degree exceeds CAP, -3 when a block's
+// candidates do not fit in K slots per lane (the caller takes another path).
+template <int K>
 __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp,
                          int &ring_out) {
     const int lane = __lane_id();
@@ -514,8 +526,8 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
     for (int w = 1; w <= wmax; ++w) {
         const int nb = min(2 * w + 1, xs);
         if (nb * nb > 64) return -3;
-        WCand c;
-        const int total = wave_load_block(b, pos, box, pi, i, sp, ib, jb, w, c);
+        WCand c[K];
+        const int total = wave_load_block<K>(b, pos, box, pi, i, sp, ib, jb, w, c);
         if (total < 0) return -3;
         const bool full = (2 * w + 1 >= xs);
         const double xlo = (ib - w) * bs - pi.x + mg, xhi = (ib + w + 1) * bs - pi.x - mg;
@@ -533,10 +545,14 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
                     const double cx = __shfl(ccx, k), cy = __shfl(ccy, k), r2 = __shfl(cr2, k);
                     const double ax = __shfl(px, k), ay = __shfl(py, k), bx = __shfl(rx, k), by = __shfl(ry, k);
                     const int ia = __shfl(pid, k), ibb = __shfl(rid, k);
-                    if (c.id >= 0 && c.id != ia && c.id != ibb) {
-                        const double ex = c.x - cx, ey = c.y - cy;
-                        if (!(ex * ex + ey * ey > r2 * (1.0 + 1e-8)) && incircle0(ax, ay, bx, by, c.x, c.y) > 0)
-                            viol = true;
+#pragma unroll
+                    for (int r = 0; r < K; ++r) {
+                        if (c[r].id >= 0 && c[r].id != ia && c[r].id != ibb) {
+                            const double ex = c[r].x - cx, ey = c[r].y - cy;
+                            if (!(ex * ex + ey * ey > r2 * (1.0 + 1e-8)) &&
+                                incircle0(ax, ay, bx, by, c[r].x, c[r].y) > 0)
+                                viol = true;
+                        }
                     }
                 }
                 verify = (__ballot(viol) == 0ull) ? 2 : 0;
@@ -545,8 +561,14 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         }
 
         // gift wrapping over the block's candidates
-        if (__popcll(__ballot(c.id >= 0)) < 3) continue;
-        WCand first = c;
+        int nvalid = 0;
+#pragma unroll
+        for (int r = 0; r < K; ++r) nvalid += __popcll(__ballot(c[r].id >= 0));
+        if (nvalid < 3) continue;
+        WCand first = c[0];
+#pragma unroll
+        for (int r = 1; r < K; ++r)
+            if (nearer(c[r], first)) first = c[r];
         for (int o = 32; o > 0; o >>= 1) {
             const WCand oth = shfl_xor_cand(first, o);
             if (nearer(oth, first)) first = oth;
@@ -558,8 +580,13 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
             if (z >= CAP) return -2;
             if (lane == z) myring = cur.id;
             ++z;
-            WCand best = c;
-            if (!(c.id >= 0 && c.id != cur.id && orient0(cur.x, cur.y, c.x, c.y) > 0)) best.id = -1;
+            WCand best{-1, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                if (c[r].id >= 0 && c[r].id != cur.id && orient0(cur.x, cur.y, c[r].x, c[r].y) > 0 &&
+                    wraps_before(cur.x, cur.y, c[r], best))
+                    best = c[r];
+            }
             for (int o = 32; o > 0; o >>= 1) {
                 const WCand oth = shfl_xor_cand(best, o);
                 if (wraps_before(cur.x, cur.y, oth, best)) best = oth;
@@ -652,6 +679,7 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
         st.nn_delta = 0;
         st.tri_delta = 0;
         st.max_deg = 0;
+        st.nflag = 0;
     }
 }
 
@@ -676,10 +704,27 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
         b.counters[3] = 1;
 }
 
-__device__ inline void flag_cell(const cgpu_voronoi_batch &b, int m) {
+// flags cell m of system s for repair (once per step) and counts it per system,
+// one atomic per wave when the freshly flagged lanes share their system
+__device__ inline void flag_cell(const cgpu_voronoi_batch &b, int s, int m) {
     const bool fresh = atomicExch(&b.repair[m], 1) == 0;
     const int q = wave_reserve(fresh, &b.counters[0]);
     if (fresh) b.flagged[q] = m;
+    const unsigned long long fm = __ballot(fresh);
+    if (fm == 0ull) return;
+    const int leader = __ffsll((long long)fm) - 1;
+    const int s0 = __shfl(s, leader);
+    if (__ballot(fresh && s != s0) == 0ull) {
+        if (__lane_id() == leader) atomicAdd(&b.st[s0].nflag, __popcll(fm));
+    } else if (fresh) {
+        atomicAdd(&b.st[s].nflag, 1);
+    }
+}
+// More than N/6 flagged cells already send the system to a global
+// retriangulation (|NeedsFixing| >= |flagged|, voronoiModelBase.cpp:650-653),
+// whatever else the test would flag, so its remaining scans can stop.
+__device__ inline bool system_goes_global(const cgpu_voronoi_batch &b, int s, const cgpu_sys_params &sp) {
+    return __atomic_load_n(&b.st[s].nflag, __ATOMIC_RELAXED) > sp.n / 6;
 }
 
 // Reference window of one triangle (voronoiModelBase.cu:24-109): circumcircle
@@ -731,9 +776,9 @@ __device__ inline bool cand_inside(const Box &box, const TriWindow &t, dvec2 a) 
 }
 __device__ inline void flag_triangle(const cgpu_voronoi_batch &b, int s, int i, int j, int l) {
     b.st[s].any_fail = 1;
-    flag_cell(b, i);
-    flag_cell(b, j);
-    flag_cell(b, l);
+    flag_cell(b, s, i);
+    flag_cell(b, s, j);
+    flag_cell(b, s, l);
 }
 
 // serial scan of one triangle's whole window (any wcheck)
@@ -743,6 +788,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
     bool bad = false;
     for (int ii = ib - t.wcheck; ii <= ib + t.wcheck; ++ii) {
+        if (system_goes_global(b, s, sp)) break;
         int cx = ii;
         if (cx < 0) cx += xs;
         if (cx >= xs) cx -= xs;
@@ -758,7 +804,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
                 if (m == i || m == j || m == l) continue;
                 if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(pos, m), v))) {
                     bad = true;
-                    flag_cell(b, m);
+                    flag_cell(b, s, m);
                 }
             }
         }
@@ -775,6 +821,7 @@ __device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *p
     const cgpu_sys_params &sp = b.sys[s];
     const Box box = make_box(sp.L);
     const dvec2 v = ldpos(pos, i);
+    if (system_goes_global(b, s, sp)) return;
     const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
     if (w.wcheck != 1 || w.all_inside) {
         test_triangle(b, pos, sp, box, s, i, v, j, l, w);
@@ -806,7 +853,7 @@ __device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *p
             if (m == i || m == j || m == l) continue;
             if (cand_inside(box, w, a)) {
                 bad = true;
-                flag_cell(b, m);
+                flag_cell(b, s, m);
             }
         }
     }
@@ -917,54 +964,56 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
     if (mode == 2) enqueue_system(b, sp);
 }
 
-// local repair: new Delaunay star of each NeedsFixing cell (reference
+// local repair of cell i: new Delaunay star of a NeedsFixing cell (reference
 // repairTriangulation, voronoiModelBase.cpp:429-505), with the neighMax rule and
-// the changes of the 6N / 2N invariants accumulated per system. Serial form, one
-// thread per cell of queue[0 .. *count): used for the cells whose bin block
-// does not fit the wave-cooperative kernel.
-__global__ void k_star_local(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
-    const int nq = *count;
+// the changes of the 6N / 2N invariants accumulated per system. Serial form.
+__device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos, int i) {
     const int N = b.num_cells;
-    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        int i = queue[q];
-        int s = b.cell_sys[i];
-        cgpu_sys_state &st = b.st[s];
-        if (st.mode != 1) continue;
-        const cgpu_sys_params &sp = b.sys[s];
-        int zold = b.nn[i];
-        if (verify_ring(b, pos, i, sp)) continue;  // unchanged: no invariant change
-        int told = owned_triangles(b, i);
-        int ring[CAP];
-        int z = delaunay_star(b, pos, i, sp, ring);
-        if (z < 0 || z > st.neigh_max) {
-            st.need_global = 1;
-            continue;
-        }
-        b.nn[i] = z;
-        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
-        int tnew = owned_triangles(b, i);
-        if (z != zold) atomicAdd(&st.nn_delta, z - zold);
-        if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
+    const int s = b.cell_sys[i];
+    cgpu_sys_state &st = b.st[s];
+    const cgpu_sys_params &sp = b.sys[s];
+    const int zold = b.nn[i];
+    if (verify_ring(b, pos, i, sp)) return;  // unchanged: no invariant change
+    const int told = owned_triangles(b, i);
+    int ring[CAP];
+    const int z = delaunay_star(b, pos, i, sp, ring);
+    if (z < 0 || z > st.neigh_max) {
+        st.need_global = 1;
+        return;
     }
+    b.nn[i] = z;
+    for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+    const int tnew = owned_triangles(b, i);
+    if (z != zold) atomicAdd(&st.nn_delta, z - zold);
+    if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
 }
 
-// the same, one wavefront per NeedsFixing cell; cells that need the serial
-// form are queued in b.flagged / counters[6]
+// The same, one wavefront per cell. K = 1 runs over the NeedsFixing queue and
+// hands cells whose bin blocks exceed one candidate per lane to the K = 3 pass
+// (queue in b.flagged / counters[6]); the K = 3 pass falls back to the serial
+// form in lane 0.
+template <int K>
 __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
-    const int nq = b.counters[1];
+    const int nq = (K == 1) ? b.counters[1] : b.counters[6];
+    const int *queue = (K == 1) ? b.nfq : b.flagged;
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = ((int)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
-        const int i = b.nfq[q];
+    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+        const int i = queue[q];
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
         if (st.mode != 1) continue;
         int ring = -1;
-        const int z = wave_star(b, pos, i, b.sys[s], ring);
+        const int z = wave_star<K>(b, pos, i, b.sys[s], ring);
         if (z == 0) continue;  // verified: unchanged
         if (z == -3) {
-            if (lane == 0) b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+            if (lane == 0) {
+                if (K == 1)
+                    b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+                else
+                    star_local_serial(b, pos, i);
+            }
             continue;
         }
         if (z < 0 || z > st.neigh_max) {
@@ -1015,41 +1064,43 @@ __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     enqueue_system(b, b.sys[s]);
 }
 
-// global retriangulation of queued systems (globalTriangulationCGAL), serial
-// form over queue[0 .. *count)
-__global__ void k_star_global(cgpu_voronoi_batch b, const double *pos, const int *queue, const int *count) {
-    const int nq = *count;
+// global retriangulation (globalTriangulationCGAL) of cell i, serial form
+__device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *pos, int i) {
     const int N = b.num_cells;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        const int i = queue[q];
-        const int s = b.cell_sys[i];
-        if (verify_ring(b, pos, i, b.sys[s])) continue;
-        int ring[CAP];
-        const int z = delaunay_star(b, pos, i, b.sys[s], ring);
-        if (z < 0) {
-            b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
-            b.counters[4] = 1;
-            continue;
-        }
-        b.nn[i] = z;
-        for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
+    const int s = b.cell_sys[i];
+    if (verify_ring(b, pos, i, b.sys[s])) return;
+    int ring[CAP];
+    const int z = delaunay_star(b, pos, i, b.sys[s], ring);
+    if (z < 0) {
+        b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
+        b.counters[4] = 1;
+        return;
     }
+    b.nn[i] = z;
+    for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
 }
 
-// the same, one wavefront per cell of the global queue
+// the same, one wavefront per cell of the global queue (K as for the local pass)
+template <int K>
 __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, const double *pos) {
-    const int nq = b.counters[2];
+    const int nq = (K == 1) ? b.counters[2] : b.counters[6];
+    const int *queue = (K == 1) ? b.gq : b.flagged;
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = ((int)blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
-        const int i = b.gq[q];
+    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+        const int i = queue[q];
         const int s = b.cell_sys[i];
         int ring = -1;
-        const int z = wave_star(b, pos, i, b.sys[s], ring);
+        const int z = wave_star<K>(b, pos, i, b.sys[s], ring);
         if (z == 0) continue;
         if (z == -3) {
-            if (lane == 0) b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+            if (lane == 0) {
+                if (K == 1)
+                    b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+                else
+                    star_global_serial(b, pos, i);
+            }
             continue;
         }
         if (z < 0) {
@@ -1395,14 +1446,13 @@ int g_block = 256;
 constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
 constexpr int REPAIR_THREADS = 64;
 constexpr int STAR_WAVE_BLOCKS = 4096;  // 16384 wavefronts, one queued cell each at a time
-constexpr int SERIAL_BLOCKS = 64;       // serial-form fallback grid
+constexpr int WIDE_BLOCKS = 256;        // three candidates per lane, for wide bin blocks
 
 // global pass over the queued systems: wave kernel, serial form for the cells
 // it hands back, per-system degree statistics, delOther and geometry
 void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
-    hipLaunchKernelGGL(k_star_global_wave, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_global, dim3(SERIAL_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos, b.flagged,
-                       b.counters + 6);
+    hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_global_wave<3>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_global_stats, dim3(b.num_systems), dim3(256), 0, st, b);
     if (b.precision)
         hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
@@ -1413,9 +1463,8 @@ void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
 int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
     hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
-    hipLaunchKernelGGL(k_star_local_wave, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_local, dim3(SERIAL_BLOCKS), dim3(REPAIR_THREADS), 0, st, b, pos, b.flagged,
-                       b.counters + 6);
+    hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_local_wave<3>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
     global_pass(b, pos, st);

@@ -73,6 +73,8 @@ typedef struct cgpu_sys_state {
     int active;         /* 1 = system advances this step                   */
     int steps_left;     /* serving: remaining steps (-1 = unlimited)       */
     int error;          /* sticky cgpu_status for this system              */
+    int nflag;          /* cells flagged by the circumcircle test this step */
+    int pad_s;
     long long global_fixes, repair_events, local_repairs, t1_events;
 } cgpu_sys_state;
 
