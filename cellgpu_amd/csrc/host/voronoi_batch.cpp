@@ -56,6 +56,8 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
         p.mu = 1.0;
         p.T = 0.0;
         p.seed = seed ? seed[s] : 13377ull;
+        if ((long long)off + n[s] >= (1LL << 26))
+            throw std::invalid_argument("a batch holds fewer than 2^26 cells (queue entries pack the cell index)");
         off += n[s];
         boff += xs * xs;
     }

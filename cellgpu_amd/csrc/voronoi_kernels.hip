@@ -480,9 +480,11 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
 // as the Delaunay star (nothing to write), -1 when no certified star exists
 // within the largest block, -2 when the degree exceeds CAP, -3 when a block's
 // candidates do not fit in K slots per lane (the caller takes another path).
+// The search starts at half-width w0 (blocks below it are known to give no
+// certified star); on -3, w_at is the half-width whose block did not fit.
 template <int K>
 __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp,
-                         int &ring_out) {
+                         int &ring_out, int w0, int &w_at) {
     const int lane = __lane_id();
     const int N = b.num_cells;
     const Box box = make_box(sp.L);
@@ -523,7 +525,8 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         if (__ballot(bad) != 0ull || __popcll(__ballot(wrapped)) != 1) verify = 0;
     }
 
-    for (int w = 1; w <= wmax; ++w) {
+    for (int w = w0; w <= wmax; ++w) {
+        w_at = w;
         const int nb = min(2 * w + 1, xs);
         if (nb * nb > 64) return -3;
         WCand c[K];
@@ -812,21 +815,29 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
     if (bad) flag_triangle(b, s, i, j, l);
 }
 
-// Empty-circumcircle test of one owned triangle (i, j, l) (reference
-// gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
-// window): wcheck = 1 windows scan the 3 x 3 bins around cell i's bin with
-// independent loads, wider windows take the serial scan.
-__device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+// serial scan of the window of owned triangle (i, j, l), any wcheck
+__device__ void serial_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     const Box box = make_box(sp.L);
     const dvec2 v = ldpos(pos, i);
-    if (system_goes_global(b, s, sp)) return;
+    test_triangle(b, pos, sp, box, s, i, v, j, l,
+                  tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v)));
+}
+
+// Empty-circumcircle test of one owned triangle (i, j, l) (reference
+// gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
+// window) when its window is the 3 x 3 bins around cell i's bin (wcheck = 1):
+// independent loads of the nine bins. Returns false, testing nothing, when the
+// window is wider.
+__device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    if (system_goes_global(b, s, sp)) return true;
+    const Box box = make_box(sp.L);
+    const dvec2 v = ldpos(pos, i);
     const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
-    if (w.wcheck != 1 || w.all_inside) {
-        test_triangle(b, pos, sp, box, s, i, v, j, l, w);
-        return;
-    }
+    if (w.wcheck != 1 || w.all_inside) return false;
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
     int bins[9], cnt[9];
@@ -847,11 +858,48 @@ __device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *p
 #pragma unroll
     for (int u = 0; u < 9; ++u) {
         for (int c = 0; c < cnt[u]; ++c) {
-            const int slot = bins[u] * BCAP + c;
-            const int m = b.bin_idx[slot];
+            const int m = b.bin_idx[bins[u] * BCAP + c];
             const dvec2 a = min_dist(box, ldpos(pos, m), v);
             if (m == i || m == j || m == l) continue;
             if (cand_inside(box, w, a)) {
+                bad = true;
+                flag_cell(b, s, m);
+            }
+        }
+    }
+    if (bad) flag_triangle(b, s, i, j, l);
+    return true;
+}
+
+// The same for a wider window, one wavefront per triangle: the lanes deal out
+// the (2 wcheck + 1)^2 bins of the reference window (including its repeated
+// columns when the window is wider than the box).
+__device__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    const Box box = make_box(sp.L);
+    const dvec2 v = ldpos(pos, i);
+    const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
+    const int xs = sp.xsize;
+    const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
+    const int nb = 2 * w.wcheck + 1;
+    const int B = nb * nb;
+    bool bad = false;
+    for (int u = __lane_id(); u < B; u += 64) {
+        if ((u & 511) < 64 && system_goes_global(b, s, sp)) break;
+        const int ii = ib - w.wcheck + u / nb, jj = jb - w.wcheck + u % nb;
+        if (!bin_may_hit(sp, w, v, ii, jj)) continue;
+        int cx = ii, cy = jj;
+        if (cx < 0) cx += xs;
+        if (cx >= xs) cx -= xs;
+        if (cy < 0) cy += xs;
+        if (cy >= xs) cy -= xs;
+        const int bin = sp.bin_off + cy * xs + cx;
+        const int cnt = min((int)b.bin_count[bin], BCAP);
+        for (int c = 0; c < cnt; ++c) {
+            const int m = b.bin_idx[bin * BCAP + c];
+            if (m == i || m == j || m == l) continue;
+            if (w.all_inside || cand_inside(box, w, min_dist(box, ldpos(pos, m), v))) {
                 bad = true;
                 flag_cell(b, s, m);
             }
@@ -864,15 +912,21 @@ __device__ void test_owned_triangle(const cgpu_voronoi_batch &b, const double *p
 // the smallest index of the three: the triangles the reference tests,
 // voronoiModelBase.cpp:380-423). The owned triangles are gathered in LDS and
 // dealt out over the block's threads, so a cell that owns many triangles does
-// not hold up its wavefront; triangles beyond the LDS list are tested by the
-// thread of their cell.
+// not hold up its wavefront; triangles with a window wider than 3 x 3 bins go
+// to a second LDS list that the block's wavefronts scan cooperatively.
+// Overflow of either list is tested serially by the thread at hand.
 constexpr int TRI_LDS = 1280;
+constexpr int WIDE_LDS = 256;
 template <typename R>
 __global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
     __shared__ int3 stri[TRI_LDS];
-    __shared__ int scount;
+    __shared__ int3 swide[WIDE_LDS];
+    __shared__ int scount, swcount;
     const int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (threadIdx.x == 0) scount = 0;
+    if (threadIdx.x == 0) {
+        scount = 0;
+        swcount = 0;
+    }
     __syncthreads();
     const bool live = i < b.num_cells && b.st[b.cell_sys[i]].active;
     if (live) {
@@ -889,8 +943,8 @@ __global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, con
                 const int slot = atomicAdd(&scount, 1);
                 if (slot < TRI_LDS)
                     stri[slot] = make_int3(i, j, jn);
-                else
-                    test_owned_triangle(b, pos, i, j, jn);
+                else if (!test_narrow(b, pos, i, j, jn))
+                    serial_test_wide(b, pos, i, j, jn);
             }
         }
     }
@@ -899,7 +953,18 @@ __global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, con
     const int n = min(scount, TRI_LDS);
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int3 t = stri[q];
-        test_owned_triangle(b, pos, t.x, t.y, t.z);
+        if (test_narrow(b, pos, t.x, t.y, t.z)) continue;
+        const int slot = atomicAdd(&swcount, 1);
+        if (slot < WIDE_LDS)
+            swide[slot] = t;
+        else
+            serial_test_wide(b, pos, t.x, t.y, t.z);
+    }
+    __syncthreads();
+    const int nw = min(swcount, WIDE_LDS);
+    for (int q = threadIdx.x >> 6; q < nw; q += blockDim.x >> 6) {
+        const int3 t = swide[q];
+        wave_test_wide(b, pos, t.x, t.y, t.z);
     }
 }
 
@@ -992,6 +1057,10 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
 // hands cells whose bin blocks exceed one candidate per lane to the K = 3 pass
 // (queue in b.flagged / counters[6]); the K = 3 pass falls back to the serial
 // form in lane 0.
+// wide-pass queue entry: cell index in the low bits, resume half-width above
+constexpr int WIDE_ID_BITS = 26;
+constexpr int WIDE_ID_MASK = (1 << WIDE_ID_BITS) - 1;
+
 template <int K>
 __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
     const int nq = (K == 1) ? b.counters[1] : b.counters[6];
@@ -1000,17 +1069,20 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
     for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
-        const int i = queue[q];
+        // the wide pass resumes at the block width the first pass could not hold
+        const int e = queue[q];
+        const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
+        const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
         if (st.mode != 1) continue;
-        int ring = -1;
-        const int z = wave_star<K>(b, pos, i, b.sys[s], ring);
+        int ring = -1, w_at = 0;
+        const int z = wave_star<K>(b, pos, i, b.sys[s], ring, w0, w_at);
         if (z == 0) continue;  // verified: unchanged
         if (z == -3) {
             if (lane == 0) {
                 if (K == 1)
-                    b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+                    b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
                 else
                     star_local_serial(b, pos, i);
             }
@@ -1089,15 +1161,17 @@ __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, 
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
     for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
-        const int i = queue[q];
+        const int e = queue[q];
+        const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
+        const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
         const int s = b.cell_sys[i];
-        int ring = -1;
-        const int z = wave_star<K>(b, pos, i, b.sys[s], ring);
+        int ring = -1, w_at = 0;
+        const int z = wave_star<K>(b, pos, i, b.sys[s], ring, w0, w_at);
         if (z == 0) continue;
         if (z == -3) {
             if (lane == 0) {
                 if (K == 1)
-                    b.flagged[atomicAdd(&b.counters[6], 1)] = i;
+                    b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
                 else
                     star_global_serial(b, pos, i);
             }
