@@ -840,25 +840,28 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
     if (w.wcheck != 1 || w.all_inside) return false;
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
-    int bins[9], cnt[9];
-#pragma unroll
-    for (int u = 0; u < 9; ++u) {
-        const int ii = ib - 1 + u / 3, jj = jb - 1 + u % 3;
-        int cx = ii, cy = jj;
+    auto bin_of = [&](int u) {
+        int cx = ib - 1 + u / 3, cy = jb - 1 + u % 3;
         if (cx < 0) cx += xs;
         if (cx >= xs) cx -= xs;
         if (cy < 0) cy += xs;
         if (cy >= xs) cy -= xs;
-        bins[u] = sp.bin_off + cy * xs + cx;
-        cnt[u] = bin_may_hit(sp, w, v, ii, jj) ? 1 : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < 9; ++u) cnt[u] = cnt[u] ? min((int)b.bin_count[bins[u]], BCAP) : 0;
-    bool bad = false;
+        return sp.bin_off + cy * xs + cx;
+    };
+    // the nine counts are loaded together and kept packed (5 bits each)
+    unsigned long long packed = 0ull;
 #pragma unroll
     for (int u = 0; u < 9; ++u) {
-        for (int c = 0; c < cnt[u]; ++c) {
-            const int m = b.bin_idx[bins[u] * BCAP + c];
+        const bool hit = bin_may_hit(sp, w, v, ib - 1 + u / 3, jb - 1 + u % 3);
+        const unsigned int c = hit ? min(b.bin_count[bin_of(u)], (unsigned int)BCAP) : 0u;
+        packed |= (unsigned long long)c << (5 * u);
+    }
+    bool bad = false;
+    for (int u = 0; u < 9; ++u) {
+        const int cnt = (int)((packed >> (5 * u)) & 31ull);
+        const int bin = bin_of(u);
+        for (int c = 0; c < cnt; ++c) {
+            const int m = b.bin_idx[bin * BCAP + c];
             const dvec2 a = min_dist(box, ldpos(pos, m), v);
             if (m == i || m == j || m == l) continue;
             if (cand_inside(box, w, a)) {
@@ -918,7 +921,8 @@ __device__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, i
 constexpr int TRI_LDS = 1280;
 constexpr int WIDE_LDS = 256;
 template <typename R>
-__global__ __launch_bounds__(256) void k_test_geometry(cgpu_voronoi_batch b, const double *pos, int test) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
+    cgpu_voronoi_batch b, const double *pos, int test) {
     __shared__ int3 stri[TRI_LDS];
     __shared__ int3 swide[WIDE_LDS];
     __shared__ int scount, swcount;
