@@ -32,7 +32,8 @@ class VoronoiBatchView(C.Structure):
     _fields_ = [("num_systems", C.c_int), ("num_cells", C.c_int), ("total_bins", C.c_int), ("precision", C.c_int)] + [
         (name, C.c_void_p) for name in (
             "sys", "st", "cell_sys", "pos", "pos_next", "nn", "nbr", "oth", "AP", "dAP", "APpref", "mot", "vel",
-            "theta", "force", "excl", "ext_force", "bin_count", "bin_idx", "repair", "mark", "flagged", "nfq", "gq",
+            "theta", "force", "excl", "ext_force", "bin_count", "bin_start", "cell_bin", "cell_rank", "sorted_id",
+            "sorted_pos", "repair", "mark", "flagged", "nfq", "gq",
             "counters", "noise")]
 
 

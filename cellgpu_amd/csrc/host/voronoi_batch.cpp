@@ -81,7 +81,11 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     gq_.alloc(N_);
     counters_.alloc(8);
     bin_count_.alloc(total_bins_);
-    bin_idx_.alloc((size_t)total_bins_ * CGPU_BIN_CAP);
+    bin_start_.alloc((size_t)total_bins_ + 1);
+    cell_bin_.alloc(N_);
+    cell_rank_.alloc(N_);
+    sorted_id_.alloc(N_);
+    sorted_pos_.alloc(2 * (size_t)N_);
     pos0_.alloc(2 * (size_t)N_);
     pos1_.alloc(2 * (size_t)N_);
     AP_.alloc(2 * N_ * rsz_);
@@ -194,7 +198,11 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.excl = excl_.p;
     b.ext_force = ext_.p;
     b.bin_count = bin_count_.p;
-    b.bin_idx = bin_idx_.p;
+    b.bin_start = bin_start_.p;
+    b.cell_bin = cell_bin_.p;
+    b.cell_rank = cell_rank_.p;
+    b.sorted_id = sorted_id_.p;
+    b.sorted_pos = sorted_pos_.p;
     b.repair = repair_.p;
     b.mark = mark_.p;
     b.flagged = flagged_.p;

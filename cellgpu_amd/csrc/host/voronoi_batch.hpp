@@ -113,9 +113,10 @@ class VoronoiBatch {
     std::vector<int> cell_sys_h_;
     DevBuf<cgpu_sys_params> sys_;
     DevBuf<cgpu_sys_state> st_;
-    DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_, bin_idx_;
+    DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_;
+    DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_;
     DevBuf<unsigned int> bin_count_;
-    DevBuf<double> pos0_, pos1_, noise_;
+    DevBuf<double> pos0_, pos1_, noise_, sorted_pos_;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
 };
 

@@ -22,7 +22,6 @@ using namespace cgpu;
 namespace {
 
 constexpr int CAP = CGPU_RING_CAP;
-constexpr int BCAP = CGPU_BIN_CAP;
 
 __device__ inline dvec2 ldpos(const double *p, int i) {
     double2 v = reinterpret_cast<const double2 *>(p)[i];
@@ -246,9 +245,9 @@ __device__ inline void scan_block(const cgpu_voronoi_batch &b, const cgpu_sys_pa
         for (int ii = 0; ii < nb; ++ii) {
             int cx = ib - w + ii;
             cx = ((cx % xs) + xs) % xs;
-            int bin = sp.bin_off + cy * xs + cx;
-            int cnt = min((int)b.bin_count[bin], BCAP);
-            for (int q = 0; q < cnt; ++q) f(b.bin_idx[bin * BCAP + q]);
+            const int bin = sp.bin_off + cy * xs + cx;
+            const int k1 = b.bin_start[bin + 1];
+            for (int k = b.bin_start[bin]; k < k1; ++k) f(b.sorted_id[k], k);
         }
     }
 }
@@ -271,10 +270,10 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
         // nearest candidate
         Cand first{-1, {0, 0}};
         int count = 0;
-        scan_block(b, sp, ib, jb, w, [&](int m) {
+        scan_block(b, sp, ib, jb, w, [&](int m, int k) {
             if (m == i) return;
             ++count;
-            dvec2 d = min_dist(box, ldpos(pos, m), pi);
+            dvec2 d = min_dist(box, ldpos(b.sorted_pos, k), pi);
             if (first.id < 0) { first = Cand{m, d}; return; }
             int c = cmp_norm2(d.x, d.y, first.d.x, first.d.y);
             if (c < 0 || (c == 0 && m < first.id)) first = Cand{m, d};
@@ -287,9 +286,9 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
             if (z >= CAP) return -2;
             ring[z++] = cur.id;
             Cand best{-1, {0, 0}};
-            scan_block(b, sp, ib, jb, w, [&](int m) {
+            scan_block(b, sp, ib, jb, w, [&](int m, int k) {
                 if (m == i || m == cur.id) return;
-                dvec2 d = min_dist(box, ldpos(pos, m), pi);
+                dvec2 d = min_dist(box, ldpos(b.sorted_pos, k), pi);
                 if (orient0(cur.d.x, cur.d.y, d.x, d.y) <= 0) return;
                 if (best.id < 0) { best = Cand{m, d}; return; }
                 int ic = incircle0(cur.d.x, cur.d.y, best.d.x, best.d.y, d.x, d.y);
@@ -371,11 +370,11 @@ __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int 
             for (int gx = x0; gx <= x1; ++gx) {
                 const int cx = ((gx % xs) + xs) % xs;
                 const int bin = sp.bin_off + cy * xs + cx;
-                const int cnt = min((int)b.bin_count[bin], BCAP);
-                for (int q = 0; q < cnt; ++q) {
-                    const int mm = b.bin_idx[bin * BCAP + q];
+                const int k1 = b.bin_start[bin + 1];
+                for (int k = b.bin_start[bin]; k < k1; ++k) {
+                    const int mm = b.sorted_id[k];
                     if (mm == i || mm == ja || mm == jb2) continue;
-                    const dvec2 dm = min_dist(box, ldpos(pos, mm), pi);
+                    const dvec2 dm = min_dist(box, ldpos(b.sorted_pos, k), pi);
                     const double ex = dm.x - c.x, ey = dm.y - c.y;
                     if (ex * ex + ey * ey > r2 * (1.0 + 1e-8)) continue;  // clearly outside
                     if (incircle0(da.x, da.y, db.x, db.y, dm.x, dm.y) > 0) return false;
@@ -437,7 +436,8 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
         cx = ((cx % xs) + xs) % xs;
         cy = ((cy % xs) + xs) % xs;
         bin = sp.bin_off + cy * xs + cx;
-        cnt = min((int)b.bin_count[bin], BCAP);
+        bin = b.bin_start[bin];  // from here on: the bin's first bin-ordered slot
+        cnt = b.bin_start[sp.bin_off + cy * xs + cx + 1] - bin;
     }
     int incl = cnt;
     for (int o = 1; o < 64; o <<= 1) {
@@ -463,11 +463,12 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
 #pragma unroll
     for (int r = 0; r < K; ++r) {
         const int j = lane + 64 * r;
-        const int tbin = __shfl(bin, t[r]), toff = __shfl(excl, t[r]);
+        const int tstart = __shfl(bin, t[r]), toff = __shfl(excl, t[r]);
         if (j < total) {
-            const int m = b.bin_idx[tbin * BCAP + (j - toff)];
+            const int k = tstart + (j - toff);
+            const int m = b.sorted_id[k];
             if (m != i) {
-                const dvec2 d = min_dist(box, ldpos(pos, m), pi);
+                const dvec2 d = min_dist(box, ldpos(b.sorted_pos, k), pi);
                 c[r] = WCand{m, d.x, d.y};
             }
         }
@@ -691,7 +692,10 @@ __global__ void k_zero_bins(cgpu_voronoi_batch b) {
         b.bin_count[q] = 0;
 }
 
-// reference cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)), atomic slot
+// Counting sort of the cells into the cell-list bins (reference
+// cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)); its fixed-capacity bins
+// become contiguous runs in bin order, so no bin can overflow). Pass 1: bin and
+// rank of each cell.
 __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
@@ -700,11 +704,48 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
     int ib = max(0, min(sp.xsize - 1, (int)floor(p.x / sp.boxsize)));
     int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
     int bin = sp.bin_off + jb * sp.xsize + ib;
-    unsigned int slot = atomicAdd(&b.bin_count[bin], 1u);
-    if (slot < (unsigned)BCAP)
-        b.bin_idx[bin * BCAP + slot] = i;
-    else
-        b.counters[3] = 1;
+    b.cell_bin[i] = bin;
+    b.cell_rank[i] = (int)atomicAdd(&b.bin_count[bin], 1u);
+}
+
+// Pass 2, one block per system: exclusive prefix sum of the bin counts. A
+// system's cells fill its bins, so the run of system s starts at its first cell.
+__global__ __launch_bounds__(256) void k_bin_scan(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x;
+    const cgpu_sys_params &sp = b.sys[s];
+    const int nbins = sp.xsize * sp.xsize;
+    __shared__ int wsum[4];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = sp.off;
+    __syncthreads();
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    for (int base = 0; base < nbins; base += 256) {
+        const int u = base + threadIdx.x;
+        const int c = (u < nbins) ? (int)b.bin_count[sp.bin_off + u] : 0;
+        int incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int off = carry;
+        for (int k = 0; k < wv; ++k) off += wsum[k];
+        if (u < nbins) b.bin_start[sp.bin_off + u] = off + incl - c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (s == b.num_systems - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
+}
+
+// Pass 3: every cell to its bin-ordered slot, with its position.
+__global__ void k_bin_scatter(cgpu_voronoi_batch b, const double *pos) {
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= b.num_cells) return;
+    const int k = b.bin_start[b.cell_bin[i]] + b.cell_rank[i];
+    b.sorted_id[k] = i;
+    reinterpret_cast<double2 *>(b.sorted_pos)[k] = reinterpret_cast<const double2 *>(pos)[i];
 }
 
 // flags cell m of system s for repair (once per step) and counts it per system,
@@ -801,11 +842,11 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
             if (cy >= xs) cy -= xs;
             if (!bin_may_hit(sp, t, v, ii, jj)) continue;
             const int bin = sp.bin_off + cy * xs + cx;
-            const int cnt = min((int)b.bin_count[bin], BCAP);
-            for (int q = 0; q < cnt; ++q) {
-                const int m = b.bin_idx[bin * BCAP + q];
+            const int k1 = b.bin_start[bin + 1];
+            for (int k = b.bin_start[bin]; k < k1; ++k) {
+                const int m = b.sorted_id[k];
                 if (m == i || m == j || m == l) continue;
-                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(pos, m), v))) {
+                if (t.all_inside || cand_inside(box, t, min_dist(box, ldpos(b.sorted_pos, k), v))) {
                     bad = true;
                     flag_cell(b, s, m);
                 }
@@ -840,29 +881,41 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
     if (w.wcheck != 1 || w.all_inside) return false;
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
-    auto bin_of = [&](int u) {
-        int cx = ib - 1 + u / 3, cy = jb - 1 + u % 3;
-        if (cx < 0) cx += xs;
-        if (cx >= xs) cx -= xs;
+    // window columns ib-1 .. ib+1 (one periodic wrap each); rows jb-1 .. jb+1.
+    // Within a row the three bins are one run of bin-ordered slots unless the
+    // row wraps, then the run is cut where the columns wrap.
+    int c0 = ib - 1, c1 = ib, c2 = ib + 1;
+    if (c0 < 0) c0 += xs;
+    if (c0 >= xs) c0 -= xs;
+    if (c1 >= xs) c1 -= xs;
+    if (c2 >= xs) c2 -= xs;
+    const bool run = (c1 == c0 + 1) && (c2 == c1 + 1);
+    int lo[9], hi[9];  // up to three runs per row
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        int cy = jb - 1 + r;
         if (cy < 0) cy += xs;
         if (cy >= xs) cy -= xs;
-        return sp.bin_off + cy * xs + cx;
-    };
-    // the nine counts are loaded together and kept packed (5 bits each)
-    unsigned long long packed = 0ull;
-#pragma unroll
-    for (int u = 0; u < 9; ++u) {
-        const bool hit = bin_may_hit(sp, w, v, ib - 1 + u / 3, jb - 1 + u % 3);
-        const unsigned int c = hit ? min(b.bin_count[bin_of(u)], (unsigned int)BCAP) : 0u;
-        packed |= (unsigned long long)c << (5 * u);
+        const int rowb = sp.bin_off + cy * xs;
+        if (run) {
+            lo[3 * r] = b.bin_start[rowb + c0];
+            hi[3 * r] = b.bin_start[rowb + c2 + 1];
+            lo[3 * r + 1] = hi[3 * r + 1] = lo[3 * r + 2] = hi[3 * r + 2] = 0;
+        } else {
+            lo[3 * r] = b.bin_start[rowb + c0];
+            hi[3 * r] = b.bin_start[rowb + c0 + 1];
+            lo[3 * r + 1] = b.bin_start[rowb + c1];
+            hi[3 * r + 1] = b.bin_start[rowb + c1 + 1];
+            lo[3 * r + 2] = b.bin_start[rowb + c2];
+            hi[3 * r + 2] = b.bin_start[rowb + c2 + 1];
+        }
     }
     bool bad = false;
-    for (int u = 0; u < 9; ++u) {
-        const int cnt = (int)((packed >> (5 * u)) & 31ull);
-        const int bin = bin_of(u);
-        for (int c = 0; c < cnt; ++c) {
-            const int m = b.bin_idx[bin * BCAP + c];
-            const dvec2 a = min_dist(box, ldpos(pos, m), v);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        for (int k = lo[q]; k < hi[q]; ++k) {
+            const int m = b.sorted_id[k];
+            const dvec2 a = min_dist(box, ldpos(b.sorted_pos, k), v);
             if (m == i || m == j || m == l) continue;
             if (cand_inside(box, w, a)) {
                 bad = true;
@@ -898,11 +951,11 @@ __device__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, i
         if (cy < 0) cy += xs;
         if (cy >= xs) cy -= xs;
         const int bin = sp.bin_off + cy * xs + cx;
-        const int cnt = min((int)b.bin_count[bin], BCAP);
-        for (int c = 0; c < cnt; ++c) {
-            const int m = b.bin_idx[bin * BCAP + c];
+        const int k1 = b.bin_start[bin + 1];
+        for (int k = b.bin_start[bin]; k < k1; ++k) {
+            const int m = b.sorted_id[k];
             if (m == i || m == j || m == l) continue;
-            if (w.all_inside || cand_inside(box, w, min_dist(box, ldpos(pos, m), v))) {
+            if (w.all_inside || cand_inside(box, w, min_dist(box, ldpos(b.sorted_pos, k), v))) {
                 bad = true;
                 flag_cell(b, s, m);
             }
@@ -1562,9 +1615,11 @@ int cgpu_set_block_size(int bs) {
 
 int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
+    const double *p = use_next ? b->pos_next : b->pos;
     hipLaunchKernelGGL(k_zero_bins, dim3(min(nblocks(b->total_bins, 256), 4096)), dim3(256), 0, S(stream), *b);
-    hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b,
-                       use_next ? b->pos_next : b->pos);
+    hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
+    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(256), 0, S(stream), *b);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
     return check();
 }
 

@@ -32,7 +32,7 @@ enum cgpu_status {
     CGPU_ERR_ARGS = -1,
     CGPU_ERR_HIP = -2,
     CGPU_ERR_RING_CAPACITY = -3,   /* a Delaunay degree exceeded the ring capacity  */
-    CGPU_ERR_BIN_CAPACITY = -4,    /* a cell-list bin overflowed                    */
+    CGPU_ERR_BIN_CAPACITY = -4,    /* reserved (the counting-sort cell list has no bin capacity) */
     CGPU_ERR_TOPOLOGY = -5,        /* global triangulation inconsistent / failed    */
     CGPU_ERR_NONFINITE = -6,       /* a position became NaN / inf                   */
     CGPU_ERR_NO_DEVICE = -7
@@ -40,8 +40,6 @@ enum cgpu_status {
 
 /* Maximum Delaunay degree stored per cell (ring capacity, slot-major storage). */
 #define CGPU_RING_CAP 24
-/* Cell-list bin capacity (the reference grows Nmax on the host; here fixed). */
-#define CGPU_BIN_CAP 24
 
 /* Per-system constants of a batch (host fills, device reads). */
 typedef struct cgpu_sys_params {
@@ -103,8 +101,12 @@ typedef struct cgpu_voronoi_batch {
     void *force;                 /* [2N] real: force                */
     const int *excl;             /* [N] or NULL: exclusions         */
     void *ext_force;             /* [2N] real or NULL               */
-    unsigned int *bin_count;     /* [total_bins]                    */
-    int *bin_idx;                /* [total_bins*BIN_CAP]            */
+    unsigned int *bin_count;     /* [total_bins] cells per bin (counting pass)      */
+    int *bin_start;              /* [total_bins+1] first bin-ordered slot of a bin  */
+    int *cell_bin;               /* [N] bin of each cell                            */
+    int *cell_rank;              /* [N] rank of each cell within its bin            */
+    int *sorted_id;              /* [N] cell indices in bin order                   */
+    double *sorted_pos;          /* [2N] positions in bin order                     */
     int *repair;                 /* [N] flag (cleared by consumer)  */
     int *mark;                   /* [N] NeedsFixing mark            */
     int *flagged;                /* [N] list of flagged cells       */
