@@ -709,31 +709,49 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
 }
 
 // Pass 2, one block per system: exclusive prefix sum of the bin counts. A
-// system's cells fill its bins, so the run of system s starts at its first cell.
-__global__ __launch_bounds__(256) void k_bin_scan(cgpu_voronoi_batch b) {
+// system's cells fill its bins, so the run of system s starts at its first
+// cell. Each thread sums 8 consecutive bins, the block scans the thread sums.
+constexpr int SCAN_THREADS = 1024, SCAN_PER = 8;
+__global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b) {
     const int s = blockIdx.x;
     const cgpu_sys_params &sp = b.sys[s];
     const int nbins = sp.xsize * sp.xsize;
-    __shared__ int wsum[4];
+    const unsigned int *cnt = b.bin_count + sp.bin_off;
+    int *start = b.bin_start + sp.bin_off;
+    __shared__ int wsum[SCAN_THREADS / 64];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = sp.off;
     __syncthreads();
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
-    for (int base = 0; base < nbins; base += 256) {
-        const int u = base + threadIdx.x;
-        const int c = (u < nbins) ? (int)b.bin_count[sp.bin_off + u] : 0;
-        int incl = c;
+    for (int base = 0; base < nbins; base += SCAN_THREADS * SCAN_PER) {
+        const int u0 = base + threadIdx.x * SCAN_PER;
+        int c[SCAN_PER];
+        int tot = 0;
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r) {
+            c[r] = (u0 + r < nbins) ? (int)cnt[u0 + r] : 0;
+            tot += c[r];
+        }
+        int incl = tot;
         for (int o = 1; o < 64; o <<= 1) {
             const int t = __shfl_up(incl, o);
             if (lane >= o) incl += t;
         }
         if (lane == 63) wsum[wv] = incl;
         __syncthreads();
-        int off = carry;
+        int off = carry + incl - tot;
         for (int k = 0; k < wv; ++k) off += wsum[k];
-        if (u < nbins) b.bin_start[sp.bin_off + u] = off + incl - c;
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r) {
+            if (u0 + r < nbins) start[u0 + r] = off;
+            off += c[r];
+        }
         __syncthreads();
-        if (threadIdx.x == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < SCAN_THREADS / 64; ++k) t += wsum[k];
+            carry += t;
+        }
         __syncthreads();
     }
     if (s == b.num_systems - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
@@ -1618,7 +1636,7 @@ int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stre
     const double *p = use_next ? b->pos_next : b->pos;
     hipLaunchKernelGGL(k_zero_bins, dim3(min(nblocks(b->total_bins, 256), 4096)), dim3(256), 0, S(stream), *b);
     hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
-    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(256), 0, S(stream), *b);
+    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(SCAN_THREADS), 0, S(stream), *b);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
     return check();
 }
