@@ -185,6 +185,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_cell": bytes_map[dom], "cells_per_launch": N,
                 "avg_launch_ms": round(stage[dom], 5)}
+    # HBM bytes per launch of the dominant kernel from the committed counter
+    # summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a plain copy probe)
+    tpath = os.path.join(ROOT, "profiles", "voronoi_spv_ensemble512_hbm_traffic.json")
+    if os.path.exists(tpath):
+        tk = json.load(open(tpath)).get("kernels", {}).get(roofline["kernel"])
+        if tk:
+            roofline["traffic"] = round(tk["traffic_bytes"])
+            roofline["traffic_source"] = "profiles/voronoi_spv_ensemble512_hbm_traffic.json"
+            roofline["algorithmic_bytes_per_launch"] = bytes_map[dom] * N
     others = {k: round(bytes_map[k] * N / (stage[k] * 1e-3) / 1e9, 1) for k in bytes_map}
 
     plain_val, plain_desc = (None, None)

@@ -141,9 +141,13 @@ __device__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, in
     dvec2 rik = min_dist(box, ldpos(pos, b.nbr[i]), pi);
     vec2<R> vfirst = Prec<R>::c0(rij, rik), vlast = vfirst;
     R A = R(0), P = R(0);
+    // ring positions are fetched one slot ahead of their use
+    dvec2 pnext = ldpos(pos, b.nbr[(1 % z) * N + i]);
     for (int k = 1; k < z; ++k) {
+        const dvec2 pk = pnext;
+        if (k + 1 < z) pnext = ldpos(pos, b.nbr[(k + 1) * N + i]);
         rij = rik;
-        rik = min_dist(box, ldpos(pos, b.nbr[k * N + i]), pi);
+        rik = min_dist(box, pk, pi);
         vec2<R> vnext = Prec<R>::c0(rij, rik);
         A += tri_area<R>(vlast, vnext);
         R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
@@ -178,11 +182,20 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, in
     vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
     const vec2<R> qI = ld2<R>(b.dAP, i);
     vec2<R> sum{R(0), R(0)};
+    // slot k's neighbour, delOther and their positions are fetched one slot ahead
+    int jDn = b.nbr[i];
+    dvec2 pDn = ldpos(pos, jDn), pOn = ldpos(pos, b.oth[i]);
     for (int k = 0; k < z; ++k) {
-        int jD = b.nbr[k * N + i];
-        dvec2 dD = dd(jD);
+        const int jD = jDn;
+        const dvec2 pD = pDn, pO = pOn;
+        if (k + 1 < z) {
+            jDn = b.nbr[(k + 1) * N + i];
+            pDn = ldpos(pos, jDn);
+            pOn = ldpos(pos, b.oth[(k + 1) * N + i]);
+        }
+        dvec2 dD = min_dist(box, pD, pi);
         vec2<R> vN = Prec<R>::c0(dC, dD);
-        dvec2 dO = dd(b.oth[k * N + i]);
+        dvec2 dO = min_dist(box, pO, pi);
         // --- reference force set for slot k: (nm1, n1) = (ring k-2, ring k-1)
         R m[4];
         dhdr<R>(dB, dC, m);
@@ -687,11 +700,6 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
     }
 }
 
-__global__ void k_zero_bins(cgpu_voronoi_batch b) {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < b.total_bins; q += gridDim.x * blockDim.x)
-        b.bin_count[q] = 0;
-}
-
 // Counting sort of the cells into the cell-list bins (reference
 // cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)); its fixed-capacity bins
 // become contiguous runs in bin order, so no bin can overflow). Pass 1: bin and
@@ -716,7 +724,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b)
     const int s = blockIdx.x;
     const cgpu_sys_params &sp = b.sys[s];
     const int nbins = sp.xsize * sp.xsize;
-    const unsigned int *cnt = b.bin_count + sp.bin_off;
+    unsigned int *cnt = b.bin_count + sp.bin_off;
     int *start = b.bin_start + sp.bin_off;
     __shared__ int wsum[SCAN_THREADS / 64];
     __shared__ int carry;
@@ -732,6 +740,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b)
             c[r] = (u0 + r < nbins) ? (int)cnt[u0 + r] : 0;
             tot += c[r];
         }
+        // the counts are consumed: clear them for the next cell list
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r)
+            if (u0 + r < nbins) cnt[u0 + r] = 0u;
         int incl = tot;
         for (int o = 1; o < 64; o <<= 1) {
             const int t = __shfl_up(incl, o);
@@ -1634,7 +1646,8 @@ int cgpu_set_block_size(int bs) {
 int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     const double *p = use_next ? b->pos_next : b->pos;
-    hipLaunchKernelGGL(k_zero_bins, dim3(min(nblocks(b->total_bins, 256), 4096)), dim3(256), 0, S(stream), *b);
+    // bin counts are zero here: allocation clears them and k_bin_scan clears
+    // them after use
     hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
     hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(SCAN_THREADS), 0, S(stream), *b);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
