@@ -82,8 +82,9 @@ def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
     f, w = _counter(fd, "FETCH_SIZE"), _counter(wd, "WRITE_SIZE")
     cf, cw = _counter(cfd, "FETCH_SIZE"), _counter(cwd, "WRITE_SIZE")
     expect_kib = probe_mib * 1024.0
-    # the probe's copy kernel is the dispatch with the largest write
-    ck = max(cw, key=lambda k: np.median(cw[k]))
+    # the probe's copy kernel reads the most (the fill that creates the source
+    # tensor writes as much but reads nothing)
+    ck = max(cf, key=lambda k: np.median(cf[k]))
     f_scale = expect_kib / np.median(cf[ck])
     w_scale = expect_kib / np.median(cw[ck])
     res = {"copy_probe": {"kernel": ck, "mib": probe_mib, "fetch_kib": float(np.median(cf[ck])),
