@@ -665,6 +665,22 @@ __device__ bool del_sets(const cgpu_voronoi_batch &b, int i) {
     return true;
 }
 
+// slot k of getDelSets for cell i (same rule as del_sets); false when the
+// slot's delOther is one of the triangle's own cells
+__device__ bool del_set_slot(const cgpu_voronoi_batch &b, int i, int z, int k) {
+    const int N = b.num_cells;
+    const int nm1 = b.nbr[((k + z - 2) % z) * N + i], n1 = b.nbr[((k + z - 1) % z) * N + i];
+    const int zn = min(b.nn[n1], CAP);
+    for (int q = 0; q < zn; ++q) {
+        if (b.nbr[q * N + n1] == nm1) {
+            b.oth[k * N + i] = b.nbr[((q + 1) % zn) * N + n1];
+            break;
+        }
+    }
+    const int o = b.oth[k * N + i];
+    return !(nm1 == o || n1 == o || i == o);
+}
+
 __device__ inline int owned_triangles(const cgpu_voronoi_batch &b, int i) {
     const int N = b.num_cells;
     const int z = b.nn[i];
@@ -856,7 +872,7 @@ __device__ inline void flag_triangle(const cgpu_voronoi_batch &b, int s, int i, 
 }
 
 // serial scan of one triangle's whole window (any wcheck)
-__device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, const cgpu_sys_params &sp,
+__device__ __forceinline__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, const cgpu_sys_params &sp,
                               const Box &box, int s, int i, dvec2 v, int j, int l, const TriWindow &t) {
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
@@ -887,7 +903,7 @@ __device__ void test_triangle(const cgpu_voronoi_batch &b, const double *pos, co
 }
 
 // serial scan of the window of owned triangle (i, j, l), any wcheck
-__device__ void serial_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+__device__ __forceinline__ void serial_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     const Box box = make_box(sp.L);
@@ -914,12 +930,24 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
     // window columns ib-1 .. ib+1 (one periodic wrap each); rows jb-1 .. jb+1.
     // Within a row the three bins are one run of bin-ordered slots unless the
     // row wraps, then the run is cut where the columns wrap.
-    int c0 = ib - 1, c1 = ib, c2 = ib + 1;
-    if (c0 < 0) c0 += xs;
-    if (c0 >= xs) c0 -= xs;
-    if (c1 >= xs) c1 -= xs;
-    if (c2 >= xs) c2 -= xs;
-    const bool run = (c1 == c0 + 1) && (c2 == c1 + 1);
+    auto wrapc = [&](int c) { return c < 0 ? c + xs : (c >= xs ? c - xs : c); };
+    const int cc0 = wrapc(ib - 1), cc1 = wrapc(ib), cc2 = wrapc(ib + 1);
+    auto ccol = [&](int c) { return c == 0 ? cc0 : (c == 1 ? cc1 : cc2); };
+    // columns / rows of the window that the circle's extent (plus 1e-9 L) can
+    // reach; all three when bin rejection is not valid (see TriWindow)
+    int ca = 0, cb = 2, ra = 0, rb = 2;
+    if (w.can_skip) {
+        const double bs = sp.boxsize;
+        const double xl = w.Q.x - w.r1, xh = w.Q.x + w.r1, yl = w.Q.y - w.r1, yh = w.Q.y + w.r1;
+        ca = ((ib) * bs - v.x >= xl) ? 0 : (((ib + 1) * bs - v.x >= xl) ? 1 : 2);
+        cb = ((ib + 1) * bs - v.x <= xh) ? 2 : ((ib * bs - v.x <= xh) ? 1 : 0);
+        ra = ((jb) * bs - v.y >= yl) ? 0 : (((jb + 1) * bs - v.y >= yl) ? 1 : 2);
+        rb = ((jb + 1) * bs - v.y <= yh) ? 2 : ((jb * bs - v.y <= yh) ? 1 : 0);
+    }
+    // a row's reachable columns form one run of bin-ordered slots unless the
+    // columns wrap between ca and cb
+    const bool run = !((ca <= 0 && cb >= 1 && cc1 != cc0 + 1) || (ca <= 1 && cb >= 2 && cc2 != cc1 + 1));
+    const int ca_bin = ccol(ca), cb_bin = ccol(cb);
     int lo[9], hi[9];  // up to three runs per row
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -927,23 +955,19 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
         if (cy < 0) cy += xs;
         if (cy >= xs) cy -= xs;
         const int rowb = sp.bin_off + cy * xs;
-        if (run) {
-            lo[3 * r] = b.bin_start[rowb + c0];
-            hi[3 * r] = b.bin_start[rowb + c2 + 1];
-            lo[3 * r + 1] = hi[3 * r + 1] = lo[3 * r + 2] = hi[3 * r + 2] = 0;
-        } else {
-            lo[3 * r] = b.bin_start[rowb + c0];
-            hi[3 * r] = b.bin_start[rowb + c0 + 1];
-            lo[3 * r + 1] = b.bin_start[rowb + c1];
-            hi[3 * r + 1] = b.bin_start[rowb + c1 + 1];
-            lo[3 * r + 2] = b.bin_start[rowb + c2];
-            hi[3 * r + 2] = b.bin_start[rowb + c2 + 1];
+        const bool rowon = r >= ra && r <= rb && ca <= cb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool on = rowon && (run ? (c == 0) : (c >= ca && c <= cb));
+            const int first = run ? ca_bin : ccol(c);
+            const int last = run ? cb_bin : ccol(c);
+            lo[3 * r + c] = on ? b.bin_start[rowb + first] : 0;
+            hi[3 * r + c] = on ? b.bin_start[rowb + last + 1] : 0;
         }
     }
     bool bad = false;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        for (int k = lo[q]; k < hi[q]; ++k) {
+    auto scan_run = [&](int k0, int k1) {
+        for (int k = k0; k < k1; ++k) {
             const int m = b.sorted_id[k];
             const dvec2 a = min_dist(box, ldpos(b.sorted_pos, k), v);
             if (m == i || m == j || m == l) continue;
@@ -952,7 +976,16 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
                 flag_cell(b, s, m);
             }
         }
-    }
+    };
+    scan_run(lo[0], hi[0]);
+    scan_run(lo[1], hi[1]);
+    scan_run(lo[2], hi[2]);
+    scan_run(lo[3], hi[3]);
+    scan_run(lo[4], hi[4]);
+    scan_run(lo[5], hi[5]);
+    scan_run(lo[6], hi[6]);
+    scan_run(lo[7], hi[7]);
+    scan_run(lo[8], hi[8]);
     if (bad) flag_triangle(b, s, i, j, l);
     return true;
 }
@@ -960,7 +993,7 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
 // The same for a wider window, one wavefront per triangle: the lanes deal out
 // the (2 wcheck + 1)^2 bins of the reference window (including its repeated
 // columns when the window is wider than the box).
-__device__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+__device__ __forceinline__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     const Box box = make_box(sp.L);
@@ -999,15 +1032,15 @@ __device__ void wave_test_wide(const cgpu_voronoi_batch &b, const double *pos, i
 // voronoiModelBase.cpp:380-423). The owned triangles are gathered in LDS and
 // dealt out over the block's threads, so a cell that owns many triangles does
 // not hold up its wavefront; triangles with a window wider than 3 x 3 bins go
-// to a second LDS list that the block's wavefronts scan cooperatively.
-// Overflow of either list is tested serially by the thread at hand.
+// to a second LDS list that the block's wavefronts scan cooperatively. A
+// triangle beyond the LDS list (more than five per cell of the block) is tested
+// serially by the thread of its cell.
 constexpr int TRI_LDS = 1280;
-constexpr int WIDE_LDS = 256;
 template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_test_geometry(
     cgpu_voronoi_batch b, const double *pos, int test) {
     __shared__ int3 stri[TRI_LDS];
-    __shared__ int3 swide[WIDE_LDS];
+    __shared__ short swide[TRI_LDS];
     __shared__ int scount, swcount;
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (threadIdx.x == 0) {
@@ -1041,16 +1074,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int3 t = stri[q];
         if (test_narrow(b, pos, t.x, t.y, t.z)) continue;
-        const int slot = atomicAdd(&swcount, 1);
-        if (slot < WIDE_LDS)
-            swide[slot] = t;
-        else
-            serial_test_wide(b, pos, t.x, t.y, t.z);
+        swide[atomicAdd(&swcount, 1)] = (short)q;
     }
     __syncthreads();
-    const int nw = min(swcount, WIDE_LDS);
+    const int nw = swcount;
     for (int q = threadIdx.x >> 6; q < nw; q += blockDim.x >> 6) {
-        const int3 t = swide[q];
+        const int3 t = stri[swide[q]];
         wave_test_wide(b, pos, t.x, t.y, t.z);
     }
 }
@@ -1141,8 +1170,8 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
 }
 
 // The same, one wavefront per cell. K = 1 runs over the NeedsFixing queue and
-// hands cells whose bin blocks exceed one candidate per lane to the K = 3 pass
-// (queue in b.flagged / counters[6]); the K = 3 pass falls back to the serial
+// hands cells whose bin blocks exceed one candidate per lane to the K = 2 pass
+// (queue in b.flagged / counters[6]); the K = 2 pass falls back to the serial
 // form in lane 0.
 // wide-pass queue entry: cell index in the low bits, resume half-width above
 constexpr int WIDE_ID_BITS = 26;
@@ -1194,17 +1223,28 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
 
 // delOther of repaired cells (reference enforceTopology, voronoiModelBase.cpp:
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
+// (8 threads per cell, each taking slots k, k + 8, k + 16)
+constexpr int DELSET_LANES = 8;
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     const int nq = b.counters[1];
-    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        int i = b.nfq[q];
+    const long long total = (long long)nq * DELSET_LANES;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
+        const int i = b.nfq[q];
         cgpu_sys_state &st = b.st[b.cell_sys[i]];
         if (st.mode != 1 || st.need_global) continue;
         if (st.nn_delta != 0 || st.tri_delta != 0) {
             st.need_global = 1;
             continue;
         }
-        if (!del_sets(b, i)) st.need_global = 1;
+        const int z = b.nn[i];
+        if (z < 3) {
+            st.need_global = 1;
+            continue;
+        }
+        for (int k = k0; k < z; k += DELSET_LANES)
+            if (!del_set_slot(b, i, z, k)) st.need_global = 1;
     }
 }
 
@@ -1607,13 +1647,13 @@ int g_block = 256;
 constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
 constexpr int REPAIR_THREADS = 64;
 constexpr int STAR_WAVE_BLOCKS = 4096;  // 16384 wavefronts, one queued cell each at a time
-constexpr int WIDE_BLOCKS = 256;        // three candidates per lane, for wide bin blocks
+constexpr int WIDE_BLOCKS = 256;        // two candidates per lane, for wide bin blocks
 
 // global pass over the queued systems: wave kernel, serial form for the cells
 // it hands back, per-system degree statistics, delOther and geometry
 void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
     hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_global_wave<3>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_global_stats, dim3(b.num_systems), dim3(256), 0, st, b);
     if (b.precision)
         hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
@@ -1625,7 +1665,7 @@ int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
     hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
     hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_local_wave<3>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
     global_pass(b, pos, st);
