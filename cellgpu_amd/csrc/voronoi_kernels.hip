@@ -265,8 +265,10 @@ __device__ inline void scan_block(const cgpu_voronoi_batch &b, const cgpu_sys_pa
     }
 }
 
+// (the search starts at half-width w0: smaller blocks are known to give no
+// certified star)
 __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp,
-                             int *ring) {
+                             int *ring, int w0 = 1) {
     const Box box = make_box(sp.L);
     const dvec2 pi = ldpos(pos, i);
     const double bs = sp.boxsize;
@@ -274,7 +276,7 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
     int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
     int jb = max(0, min(xs - 1, (int)floor(pi.y / bs)));
     const int wmax = max(1, xs / 2);
-    for (int w = 1; w <= wmax; ++w) {
+    for (int w = w0; w <= wmax; ++w) {
         const bool full = (2 * w + 1 >= xs);
         // scanned block relative to pi, shrunk by 1e-9 L for rounding at bin edges
         const double mg = 1e-9 * sp.L;
@@ -1148,7 +1150,7 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
 // local repair of cell i: new Delaunay star of a NeedsFixing cell (reference
 // repairTriangulation, voronoiModelBase.cpp:429-505), with the neighMax rule and
 // the changes of the 6N / 2N invariants accumulated per system. Serial form.
-__device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos, int i) {
+__device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos, int i, int w0) {
     const int N = b.num_cells;
     const int s = b.cell_sys[i];
     cgpu_sys_state &st = b.st[s];
@@ -1157,7 +1159,7 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
     if (verify_ring(b, pos, i, sp)) return;  // unchanged: no invariant change
     const int told = owned_triangles(b, i);
     int ring[CAP];
-    const int z = delaunay_star(b, pos, i, sp, ring);
+    const int z = delaunay_star(b, pos, i, sp, ring, w0);
     if (z < 0 || z > st.neigh_max) {
         st.need_global = 1;
         return;
@@ -1200,7 +1202,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
                 if (K == 1)
                     b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
                 else
-                    star_local_serial(b, pos, i);
+                    star_local_serial(b, pos, i, w_at);
             }
             continue;
         }
@@ -1264,12 +1266,12 @@ __global__ void k_global_fallback(cgpu_voronoi_batch b) {
 }
 
 // global retriangulation (globalTriangulationCGAL) of cell i, serial form
-__device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *pos, int i) {
+__device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *pos, int i, int w0) {
     const int N = b.num_cells;
     const int s = b.cell_sys[i];
     if (verify_ring(b, pos, i, b.sys[s])) return;
     int ring[CAP];
-    const int z = delaunay_star(b, pos, i, b.sys[s], ring);
+    const int z = delaunay_star(b, pos, i, b.sys[s], ring, w0);
     if (z < 0) {
         b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
         b.counters[4] = 1;
@@ -1300,7 +1302,7 @@ __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, 
                 if (K == 1)
                     b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
                 else
-                    star_global_serial(b, pos, i);
+                    star_global_serial(b, pos, i, w_at);
             }
             continue;
         }
