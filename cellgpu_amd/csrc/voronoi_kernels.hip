@@ -552,13 +552,20 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         const double xlo = (ib - w) * bs - pi.x + mg, xhi = (ib + w + 1) * bs - pi.x - mg;
         const double ylo = (jb - w) * bs - pi.y + mg, yhi = (jb + w + 1) * bs - pi.y - mg;
 
+        // a current-ring circle that leaves the block leaves the verification
+        // undecided at this width; the star is then built from this block
+        bool decide = false;
         if (verify == 1) {
             if (full) {
                 verify = 0;
             } else {
                 const bool fits = lane >= zold ||
                                   (ccx - cr >= xlo && ccx + cr <= xhi && ccy - cr >= ylo && ccy + cr <= yhi);
-                if (__ballot(!fits) != 0ull) continue;  // a circle leaves the block: widen
+                decide = __ballot(!fits) == 0ull;
+            }
+        }
+        if (decide) {
+            {
                 bool viol = false;
                 for (int k = 0; k < zold; ++k) {
                     const double cx = __shfl(ccx, k), cy = __shfl(ccy, k), r2 = __shfl(cr2, k);
