@@ -553,7 +553,8 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         const double ylo = (jb - w) * bs - pi.y + mg, yhi = (jb + w + 1) * bs - pi.y - mg;
 
         // a current-ring circle that leaves the block leaves the verification
-        // undecided at this width; the star is then built from this block
+        // undecided at this width; from the second width on, the star is then
+        // built from this block
         bool decide = false;
         if (verify == 1) {
             if (full) {
@@ -584,6 +585,8 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
                 verify = (__ballot(viol) == 0ull) ? 2 : 0;
             }
             if (verify == 2) return 0;
+        } else if (verify == 1 && w == 1) {
+            continue;  // the next block usually decides; building here rarely certifies
         }
 
         // gift wrapping over the block's candidates
