@@ -927,19 +927,17 @@ __device__ __forceinline__ void serial_test_wide(const cgpu_voronoi_batch &b, co
 // Empty-circumcircle test of one owned triangle (i, j, l) (reference
 // gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
 // window) when its window is the 3 x 3 bins around cell i's bin (wcheck = 1):
-// independent loads of the nine bins. Returns NARROW_DONE when tested,
-// NARROW_WIDE (testing nothing) when the window is wider, NARROW_SMALL_BOX when
-// the window is 3 x 3 but the box too small for fixed periodic shifts.
-constexpr int NARROW_DONE = 0, NARROW_WIDE = 1, NARROW_SMALL_BOX = 2;
-__device__ __forceinline__ int test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+// independent loads of the nine bins. Returns false, testing nothing, when the
+// window is wider.
+__device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
-    if (system_goes_global(b, s, sp)) return NARROW_DONE;
+    if (system_goes_global(b, s, sp)) return true;
     const Box box = make_box(sp.L);
     const dvec2 v = ldpos(pos, i);
     const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
-    if (w.wcheck != 1 || w.all_inside) return NARROW_WIDE;
-    if (!w.can_skip) return NARROW_SMALL_BOX;
+    // (small boxes, where bin rejection is not valid, take the wide path)
+    if (w.wcheck != 1 || w.all_inside || !w.can_skip) return false;
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
     // window columns ib-1 .. ib+1 (one periodic wrap each); rows jb-1 .. jb+1.
@@ -1026,7 +1024,7 @@ __device__ __forceinline__ int test_narrow(const cgpu_voronoi_batch &b, const do
         scan_run(lo8, hi8, sx2, sy2);
     }
     if (bad) flag_triangle(b, s, i, j, l);
-    return NARROW_DONE;
+    return true;
 }
 
 // The same for a wider window, one wavefront per triangle: the lanes deal out
@@ -1102,7 +1100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 const int slot = atomicAdd(&scount, 1);
                 if (slot < TRI_LDS)
                     stri[slot] = make_int3(i, j, jn);
-                else if (test_narrow(b, pos, i, j, jn) != NARROW_DONE)
+                else if (!test_narrow(b, pos, i, j, jn))
                     serial_test_wide(b, pos, i, j, jn);
             }
         }
@@ -1112,11 +1110,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const int n = min(scount, TRI_LDS);
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int3 t = stri[q];
-        const int r = test_narrow(b, pos, t.x, t.y, t.z);
-        if (r == NARROW_SMALL_BOX)
-            serial_test_wide(b, pos, t.x, t.y, t.z);
-        else if (r == NARROW_WIDE)
-            swide[atomicAdd(&swcount, 1)] = (short)q;
+        if (test_narrow(b, pos, t.x, t.y, t.z)) continue;
+        swide[atomicAdd(&swcount, 1)] = (short)q;
     }
     __syncthreads();
     const int nw = swcount;
