@@ -929,15 +929,14 @@ __device__ __forceinline__ void serial_test_wide(const cgpu_voronoi_batch &b, co
 // window) when its window is the 3 x 3 bins around cell i's bin (wcheck = 1):
 // independent loads of the nine bins. Returns false, testing nothing, when the
 // window is wider.
-__device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+__device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     if (system_goes_global(b, s, sp)) return true;
     const Box box = make_box(sp.L);
     const dvec2 v = ldpos(pos, i);
     const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
-    // (small boxes, where bin rejection is not valid, take the wide path)
-    if (w.wcheck != 1 || w.all_inside || !w.can_skip) return false;
+    if (w.wcheck != 1 || w.all_inside) return false;
     const int xs = sp.xsize;
     const int ib = (int)floor(v.x / sp.boxsize), jb = (int)floor(v.y / sp.boxsize);
     // window columns ib-1 .. ib+1 (one periodic wrap each); rows jb-1 .. jb+1.
@@ -947,9 +946,9 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
     const int cc0 = wrapc(ib - 1), cc1 = wrapc(ib), cc2 = wrapc(ib + 1);
     auto ccol = [&](int c) { return c == 0 ? cc0 : (c == 1 ? cc1 : cc2); };
     // columns / rows of the window that the circle's extent (plus 1e-9 L) can
-    // reach
+    // reach; all three when bin rejection is not valid (see TriWindow)
     int ca = 0, cb = 2, ra = 0, rb = 2;
-    {
+    if (w.can_skip) {
         const double bs = sp.boxsize;
         const double xl = w.Q.x - w.r1, xh = w.Q.x + w.r1, yl = w.Q.y - w.r1, yh = w.Q.y + w.r1;
         ca = ((ib) * bs - v.x >= xl) ? 0 : (((ib + 1) * bs - v.x >= xl) ? 1 : 2);
@@ -961,68 +960,44 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
     // columns wrap between ca and cb
     const bool run = !((ca <= 0 && cb >= 1 && cc1 != cc0 + 1) || (ca <= 1 && cb >= 2 && cc2 != cc1 + 1));
     const int ca_bin = ccol(ca), cb_bin = ccol(cb);
-    // bounds of up to three runs per row; every bin_start load is issued
-    // before the first candidate is read
-    int lo0, hi0, lo1, hi1, lo2, hi2, lo3, hi3, lo4, hi4, lo5, hi5, lo6, hi6, lo7, hi7, lo8, hi8;
-    auto bounds = [&](int r, int c, int &lo, int &hi) {
+    int lo[9], hi[9];  // up to three runs per row
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
         int cy = jb - 1 + r;
         if (cy < 0) cy += xs;
         if (cy >= xs) cy -= xs;
         const int rowb = sp.bin_off + cy * xs;
-        const bool on = (r >= ra && r <= rb && ca <= cb) && (run ? (c == 0) : (c >= ca && c <= cb));
-        const int first = run ? ca_bin : ccol(c);
-        const int last = run ? cb_bin : ccol(c);
-        lo = on ? b.bin_start[rowb + first] : 0;
-        hi = on ? b.bin_start[rowb + last + 1] : 0;
-    };
-    bounds(0, 0, lo0, hi0);
-    bounds(0, 1, lo1, hi1);
-    bounds(0, 2, lo2, hi2);
-    bounds(1, 0, lo3, hi3);
-    bounds(1, 1, lo4, hi4);
-    bounds(1, 2, lo5, hi5);
-    bounds(2, 0, lo6, hi6);
-    bounds(2, 1, lo7, hi7);
-    bounds(2, 2, lo8, hi8);
+        const bool rowon = r >= ra && r <= rb && ca <= cb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool on = rowon && (run ? (c == 0) : (c >= ca && c <= cb));
+            const int first = run ? ca_bin : ccol(c);
+            const int last = run ? cb_bin : ccol(c);
+            lo[3 * r + c] = on ? b.bin_start[rowb + first] : 0;
+            hi[3 * r + c] = on ? b.bin_start[rowb + last + 1] : 0;
+        }
+    }
     bool bad = false;
-    // With the window and the circle well inside half a box, the minimum images
-    // are fixed per window column / row: one periodic shift of -L, 0 or +L,
-    // added exactly as the reference's single wrap step adds it, and the second
-    // minimum image (relative to Q) needs no wrap. The test is then the
-    // reference arithmetic without the wrap loops.
-    auto shift = [&](int idx) { return idx < 0 ? -sp.L : (idx >= xs ? sp.L : 0.0); };
-    const double sx0 = shift(ib - 1), sx1 = shift(ib), sx2 = shift(ib + 1);
-    auto sxcol = [&](int c) { return c == 0 ? sx0 : (c == 1 ? sx1 : sx2); };
-    auto scan_run = [&](int k0, int k1, double sx, double sy) {
+    auto scan_run = [&](int k0, int k1) {
         for (int k = k0; k < k1; ++k) {
             const int m = b.sorted_id[k];
-            const dvec2 pm = ldpos(b.sorted_pos, k);
+            const dvec2 a = min_dist(box, ldpos(b.sorted_pos, k), v);
             if (m == i || m == j || m == l) continue;
-            const double ax = (pm.x - v.x) + sx, ay = (pm.y - v.y) + sy;
-            const double cx = ax - w.Q.x, cy = ay - w.Q.y;
-            if (cx * cx + cy * cy < w.rad2) {
+            if (cand_inside(box, w, a)) {
                 bad = true;
                 flag_cell(b, s, m);
             }
         }
     };
-    const double sy0 = shift(jb - 1), sy1 = shift(jb), sy2 = shift(jb + 1);
-    if (run) {
-        const double sx = sxcol(ca);
-        scan_run(lo0, hi0, sx, sy0);
-        scan_run(lo3, hi3, sx, sy1);
-        scan_run(lo6, hi6, sx, sy2);
-    } else {
-        scan_run(lo0, hi0, sx0, sy0);
-        scan_run(lo1, hi1, sx1, sy0);
-        scan_run(lo2, hi2, sx2, sy0);
-        scan_run(lo3, hi3, sx0, sy1);
-        scan_run(lo4, hi4, sx1, sy1);
-        scan_run(lo5, hi5, sx2, sy1);
-        scan_run(lo6, hi6, sx0, sy2);
-        scan_run(lo7, hi7, sx1, sy2);
-        scan_run(lo8, hi8, sx2, sy2);
-    }
+    scan_run(lo[0], hi[0]);
+    scan_run(lo[1], hi[1]);
+    scan_run(lo[2], hi[2]);
+    scan_run(lo[3], hi[3]);
+    scan_run(lo[4], hi[4]);
+    scan_run(lo[5], hi[5]);
+    scan_run(lo[6], hi[6]);
+    scan_run(lo[7], hi[7]);
+    scan_run(lo[8], hi[8]);
     if (bad) flag_triangle(b, s, i, j, l);
     return true;
 }
