@@ -127,6 +127,32 @@ template <> __device__ inline void dhdr<float>(dvec2 rij, dvec2 rik, float m[4])
 template <typename R> __device__ inline R clamp_len(R x) { return x; }
 template <> __device__ inline float clamp_len<float>(float x) { return x < 1e-18f ? 1e-18f : x; }
 
+// ------------------------------------------------------- position accessors
+// Gathers of neighbour data go through an accessor: GlobalData reads HBM (via
+// the caches); BlockTile first looks in an LDS copy of the block's own cells,
+// which with Hilbert-ordered cells holds most Delaunay neighbours.
+template <typename R> struct GlobalData {
+    const double *pos;
+    const void *dAP;
+    __device__ dvec2 p(int id) const { return ldpos(pos, id); }
+    __device__ vec2<R> q(int id) const { return ld2<R>(dAP, id); }
+};
+template <typename R> struct BlockTile {
+    const double *pos;
+    const void *dAP;
+    const dvec2 *sp;     // LDS positions of cells c0 .. c0 + n - 1
+    const vec2<R> *sq;   // LDS dAP of the same cells (may be null)
+    int c0, n;
+    __device__ dvec2 p(int id) const {
+        const unsigned t = (unsigned)(id - c0);
+        return t < (unsigned)n ? sp[t] : ldpos(pos, id);
+    }
+    __device__ vec2<R> q(int id) const {
+        const unsigned t = (unsigned)(id - c0);
+        return t < (unsigned)n ? sq[t] : ld2<R>(dAP, id);
+    }
+};
+
 // ------------------------------------------------------------------ geometry
 // Voronoi area and perimeter of cell i in the reference GPU order
 // (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
@@ -167,31 +193,31 @@ __device__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, in
 // (voronoiQuadraticEnergy.cu:99-234): slot k is the Voronoi vertex of
 // (i, ring k-2, ring k-1); its neighbours in the three polygons are vertex k-1,
 // vertex k+1 and the circumcenter of (ring k-2, ring k-1, delOther k).
-template <typename R>
-__device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
+template <typename R, typename Acc>
+__device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
     if (z < 3) return vec2<R>{R(0), R(0)};
-    const dvec2 pi = ldpos(pos, i);
+    const dvec2 pi = acc.p(i);
     auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
-    auto dd = [&](int id) { return min_dist(box, ldpos(pos, id), pi); };
+    auto dd = [&](int id) { return min_dist(box, acc.p(id), pi); };
     int jB = rk(-2), jC = rk(-1);
     dvec2 dA = dd(rk(-3)), dB = dd(jB), dC = dd(jC);
-    vec2<R> qB = ld2<R>(b.dAP, jB), qC = ld2<R>(b.dAP, jC);
+    vec2<R> qB = acc.q(jB), qC = acc.q(jC);
     vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
-    const vec2<R> qI = ld2<R>(b.dAP, i);
+    const vec2<R> qI = acc.q(i);
     vec2<R> sum{R(0), R(0)};
     // slot k's neighbour, delOther and their positions are fetched one slot ahead
     int jDn = b.nbr[i];
-    dvec2 pDn = ldpos(pos, jDn), pOn = ldpos(pos, b.oth[i]);
+    dvec2 pDn = acc.p(jDn), pOn = acc.p(b.oth[i]);
     for (int k = 0; k < z; ++k) {
         const int jD = jDn;
         const dvec2 pD = pDn, pO = pOn;
         if (k + 1 < z) {
             jDn = b.nbr[(k + 1) * N + i];
-            pDn = ldpos(pos, jDn);
-            pOn = ldpos(pos, b.oth[(k + 1) * N + i]);
+            pDn = acc.p(jDn);
+            pOn = acc.p(b.oth[(k + 1) * N + i]);
         }
         dvec2 dD = min_dist(box, pD, pi);
         vec2<R> vN = Prec<R>::c0(dC, dD);
@@ -230,7 +256,7 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const double *pos, in
         dB = dC;
         dC = dD;
         qB = qC;
-        qC = ld2<R>(b.dAP, jD);
+        qC = acc.q(jD);
         vL = vM;
         vM = vN;
     }
@@ -1388,10 +1414,10 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
 
 // force of every active cell, optional exclusions (reference
 // gpu_sum_forces_with_exclusions_kernel, voronoiQuadraticEnergy.cu:59-93)
-template <typename R>
-__device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, const double *pos, int i,
+template <typename R, typename Acc>
+__device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, const Acc &acc, int i,
                                                 const cgpu_sys_params &sp) {
-    vec2<R> F = cell_force<R>(b, pos, i, sp);
+    vec2<R> F = cell_force<R>(b, acc, i, sp);
     if (b.excl) {
         if (b.excl[i] != 0) {
             if (b.ext_force) st2<R>(b.ext_force, i, vec2<R>{-F.x, -F.y});
@@ -1408,7 +1434,7 @@ __global__ void k_forces(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
-    st2<R>(b.force, i, force_with_exclusions<R>(b, pos, i, sp));
+    st2<R>(b.force, i, force_with_exclusions<R>(b, GlobalData<R>{pos, b.dAP}, i, sp));
 }
 
 // reference noise draw for cell i: buffer or in-kernel counter-based deviate
@@ -1433,10 +1459,24 @@ template <> __device__ inline double atan2_r<double>(double y, double x) { retur
 // fused force -> self-propelled EOM (reference GPU semantics,
 // selfPropelledParticleDynamics.cu:35-53) -> move (Simple2DCell.cu:21-35).
 // kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
+constexpr int FORCE_BLOCK = 256;
 template <typename R, int KIND>
-__global__ void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double *pos_next) {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos,
+                                                                double *pos_next) {
+    // LDS copy of the block's cells (positions, dAP): most ring members and
+    // delOther cells of a Hilbert-ordered block are in it
+    __shared__ dvec2 spos[FORCE_BLOCK];
+    __shared__ vec2<R> sdap[FORCE_BLOCK];
+    const int c0 = xcd_block() * FORCE_BLOCK;
+    const int i = c0 + threadIdx.x;
+    const int n = min(FORCE_BLOCK, b.num_cells - c0);
+    if ((int)threadIdx.x < n) {
+        spos[threadIdx.x] = ldpos(pos, i);
+        sdap[threadIdx.x] = ld2<R>(b.dAP, i);
+    }
+    __syncthreads();
     if (i >= b.num_cells) return;
+    const BlockTile<R> tile{pos, b.dAP, spos, sdap, c0, n};
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     cgpu_sys_state &st = b.st[s];
@@ -1446,7 +1486,7 @@ __global__ void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double
         return;
     }
     const int step = st.step + 1;  // Timestep is incremented before integrating
-    vec2<R> F = force_with_exclusions<R>(b, pos, i, sp);
+    vec2<R> F = force_with_exclusions<R>(b, tile, i, sp);
     st2<R>(b.force, i, F);
     const R dt = (R)sp.dt, mu = (R)sp.mu;
     vec2<R> disp;
@@ -1566,7 +1606,7 @@ __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs
     const int s = b.cell_sys[i];
     const cgpu_fire_state &f = fs[s];
     if (f.converged || !b.st[s].active) return;
-    vec2<R> F = force_with_exclusions<R>(b, pos, i, b.sys[s]);
+    vec2<R> F = force_with_exclusions<R>(b, GlobalData<R>{pos, b.dAP}, i, b.sys[s]);
     st2<R>(b.force, i, F);
     vec2<R> v = ld2<R>(vel, i);
     const R dt = (R)f.dt;
@@ -1739,7 +1779,8 @@ int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream) {
 }
 
 static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream) {
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
+    dim3 g(nblocks(b->num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
     hipStream_t st = S(stream);
     if (b->precision) {
         if (kind == 0)
