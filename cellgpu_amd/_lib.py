@@ -70,6 +70,8 @@ def lib():
         "cgpu_vor_num_cells": (i, [vp]),
         "cgpu_vor_energy": (d, [vp, i]),
         "cgpu_vor_set_active": (i, [vp, vp, vp]),
+        "cgpu_vor_get_active": (i, [vp, vp, vp]),
+        "cgpu_vor_device_bytes": (C.c_longlong, [vp]),
         "cgpu_vor_autotune": (i, [vp]),
         "cgpu_vor_view": (i, [vp, P(VoronoiBatchView)]),
         "cgpu_vor_stream": (vp, [vp]),

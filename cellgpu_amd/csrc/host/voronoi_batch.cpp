@@ -95,6 +95,12 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     vel_.alloc(2 * N_ * rsz_);
     theta_.alloc(N_ * rsz_);
     force_.alloc(2 * N_ * rsz_);
+    device_bytes_ = sys_.n * sizeof(cgpu_sys_params) + st_.n * sizeof(cgpu_sys_state) +
+                    sizeof(int) * (cell_sys_.n + nn_.n + nbr_.n + oth_.n + repair_.n + mark_.n + flagged_.n +
+                                   nfq_.n + gq_.n + counters_.n + bin_start_.n + cell_bin_.n + cell_rank_.n +
+                                   sorted_id_.n + bin_count_.n) +
+                    sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
+                    (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
     cell_sys_.upload(cell_sys_h_.data(), N_);
     upload_params();
     std::vector<cgpu_sys_state> st(S_);
@@ -214,6 +220,7 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
 }
 
 void VoronoiBatch::set_params(int s, double KA, double KP, double dt, double mu, double T) {
+    sync();  // host copies below must not race the batch stream
     int lo = s < 0 ? 0 : s, hi = s < 0 ? S_ : s + 1;
     for (int q = lo; q < hi; ++q) {
         hp_[q].KA = KA;
@@ -234,6 +241,7 @@ void VoronoiBatch::set_params(int s, double KA, double KP, double dt, double mu,
 }
 
 void VoronoiBatch::set_prefs(const double *A0, const double *P0) {
+    sync();  // host copies below must not race the batch stream
     std::vector<double> v(2 * (size_t)N_);
     for (int i = 0; i < N_; ++i) {
         v[2 * i] = A0[i];
@@ -254,6 +262,7 @@ void VoronoiBatch::set_prefs(const double *A0, const double *P0) {
 
 // reference Simple2DActiveCell::setv0Dr (.cpp:76-92) resets v = v0 (cos, sin)(theta)
 void VoronoiBatch::set_motility(const double *v0, const double *Dr) {
+    sync();  // host copies below must not race the batch stream
     std::vector<double> th(N_);
     real_download(theta_.p, th.data(), N_);
     std::vector<double> mv(2 * (size_t)N_), vv(2 * (size_t)N_);
@@ -270,6 +279,7 @@ void VoronoiBatch::set_motility(const double *v0, const double *Dr) {
 // reference voronoiModelBase::setExclusions (.cpp:89-107): excluded cells get
 // v0 = Dr = 0 and their force is reported as an external force
 void VoronoiBatch::set_exclusions(const int *ex) {
+    sync();  // host copies below must not race the batch stream
     if (!excl_.p) {
         excl_.alloc(N_);
         ext_.alloc(2 * N_ * rsz_);
@@ -299,6 +309,7 @@ void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
 }
 
 void VoronoiBatch::set_noise_mode(bool use_buffer) {
+    sync();  // host copies below must not race the batch stream
     noise_buffer_ = use_buffer;
     if (use_buffer && !noise_.p) noise_.alloc(2 * (size_t)N_);
     if (graph_exec_) {
@@ -307,11 +318,13 @@ void VoronoiBatch::set_noise_mode(bool use_buffer) {
     }
 }
 void VoronoiBatch::upload_noise(const double *eta, int count) {
+    sync();  // host copies below must not race the batch stream
     if (!noise_.p) noise_.alloc(2 * (size_t)N_);
     noise_.upload(eta, std::min<size_t>(count, 2 * (size_t)N_));
 }
 
 void VoronoiBatch::set_active(const int *active, const int *steps_left) {
+    sync();  // host copies below must not race the batch stream
     std::vector<cgpu_sys_state> st(S_);
     st_.download(st.data(), S_);
     for (int s = 0; s < S_; ++s) {
@@ -604,6 +617,19 @@ int VoronoiBatch::spatial_sort(int *perm_out) {
 
 int VoronoiBatch::autotune() { return CGPU_OK; }
 
+void VoronoiBatch::get_active(int *active, int *steps_left) {
+    std::vector<cgpu_sys_state> st(S_);
+    hip_check(hipMemcpyAsync(st.data(), st_.p, sizeof(cgpu_sys_state) * S_, hipMemcpyDeviceToHost, stream_),
+              "get_active");
+    sync();
+    for (int s = 0; s < S_; ++s) {
+        if (active) active[s] = st[s].active;
+        if (steps_left) steps_left[s] = st[s].steps_left;
+    }
+}
+
+size_t VoronoiBatch::device_bytes() const { return device_bytes_; }
+
 }  // namespace cgpu_host
 
 // ================================================================ object-layer C ABI
@@ -701,6 +727,15 @@ double cgpu_vor_energy(void *h, int s) {
 }
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left) {
     return guarded([&] { ((VoronoiBatch *)h)->set_active(active, steps_left); return 0; });
+}
+
+int cgpu_vor_get_active(void *h, int *active, int *steps_left) {
+    return guarded([&] { ((VoronoiBatch *)h)->get_active(active, steps_left); return 0; });
+}
+
+long long cgpu_vor_device_bytes(void *h) {
+    if (!h) return -1;
+    return (long long)((VoronoiBatch *)h)->device_bytes();
 }
 int cgpu_vor_autotune(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->autotune(); }); }
 int cgpu_vor_spatial_sort(void *h, int *perm) {

@@ -68,6 +68,8 @@ class VoronoiBatch {
     void set_noise_mode(bool use_buffer);
     void upload_noise(const double *eta, int count);
     void set_active(const int *active, const int *steps_left);
+    void get_active(int *active, int *steps_left);
+    size_t device_bytes() const;
 
     int global_triangulation();
     int compute_forces();
@@ -118,6 +120,7 @@ class VoronoiBatch {
     DevBuf<unsigned int> bin_count_;
     DevBuf<double> pos0_, pos1_, noise_, sorted_pos_;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
+    size_t device_bytes_ = 0;
 };
 
 }  // namespace cgpu_host

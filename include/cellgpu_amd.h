@@ -284,6 +284,12 @@ int cgpu_vor_num_cells(void *h);
 double cgpu_vor_energy(void *h, int s);
 int cgpu_vor_status(void *h);
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
+/* Serving layer (no counterpart in the reference, which runs one system per
+ * process): per-system active flags and remaining steps after the work queued
+ * so far (waits for the batch stream), and the device memory a batch holds, for
+ * admission against an HBM budget. */
+int cgpu_vor_get_active(void *h, int *active, int *steps_left);
+long long cgpu_vor_device_bytes(void *h);
 int cgpu_vor_autotune(void *h);
 /* Spatial sorting (reference Simulation::setSortPeriod -> spatialSorting,
  * Simple2DCell.cpp:438-477, voronoiModelBase.cpp:669-680): relabels the cells of

@@ -83,3 +83,12 @@ def test_wrap_matches_reference_loops(so, oracle):
             f(L, C.byref(x), C.byref(y))
             ox, oy = oracle.put_in_box(L, v * L / 10, -0.37 * v * L / 10)
             assert x.value == ox and y.value == oy
+
+
+def test_serving_workload_is_deterministic():
+    from cellgpu_amd.serving import poisson_requests
+    a = poisson_requests(20, 5.0, seed=3)
+    b = poisson_requests(20, 5.0, seed=3)
+    assert [(r.n, r.steps, r.arrival) for r in a] == [(r.n, r.steps, r.arrival) for r in b]
+    assert all(100 <= r.n <= 65536 and 1000 <= r.steps <= 100000 for r in a)
+    assert all(b2.arrival > b1.arrival for b1, b2 in zip(a, a[1:]))

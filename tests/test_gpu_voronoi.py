@@ -221,3 +221,24 @@ def test_spatial_sort_relabels_consistently(oracle):
         onn, onbr, _ = oracle.triangulate(g1["x"][sl], g1["y"][sl], b.L[s], cap=CAP)
         loc = [tuple(int(v) - sl.start for v in nbr[i, : nn[i]]) for i in range(sl.start, sl.stop)]
         assert loc == _rings(onn, onbr)
+
+
+def test_serving_alone_equals_mixed():
+    """A request gives the same final state served alone and inside a mixed
+    batch whose other systems run for fewer and more steps."""
+    _cg()
+    from cellgpu_amd.serving import Request, ServingEngine
+
+    def serve(reqs):
+        eng = ServingEngine(admit_interval=10.0, tick_steps=16, precision="fp64", keep_results=True)
+        done = eng.run(reqs)
+        return {r.rid: r for r in done}
+
+    alone = serve([Request(0, 300, 37, P0=3.8, v0=0.1, Dr=1.0, seed=11)])
+    mixed = serve([Request(1, 150, 80, P0=3.7, v0=0.05, seed=5), Request(0, 300, 37, P0=3.8, v0=0.1, Dr=1.0, seed=11),
+                   Request(2, 500, 20, P0=3.9, v0=0.2, seed=9)])
+    assert set(mixed) == {0, 1, 2}
+    xa, ya = alone[0].result
+    xm, ym = mixed[0].result
+    assert np.array_equal(xa, xm) and np.array_equal(ya, ym)
+    assert all(r.latency is not None and r.latency >= 0 for r in mixed.values())
