@@ -153,6 +153,8 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
 }
 
 VoronoiBatch::~VoronoiBatch() {
+    // queued work may still read the graph and the buffers freed below
+    if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (stream_) (void)hipStreamDestroy(stream_);
