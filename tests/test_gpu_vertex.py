@@ -84,3 +84,31 @@ def test_vertex_brownian_t1_stepwise(oracle):
         assert _canon(cv, cvn) == _canon(ocv, ocvn), step
     assert t1_total > 0
     assert b.counters()["t1_events"] == t1_total
+
+
+def test_vertex_full_size_invariants():
+    """Nc = 32 768 (Nv = 65 536, L = 181.02), Brownian T = 1e-3, dt = 0.005, T1
+    threshold 0.04, P0 = 3.9: after 200 steps with T1 flips the vertex model keeps
+    its bookkeeping (Nv = 2 Nc, sum of cell vertex counts = 3 Nv, every vertex in
+    the lists of its three cells, positive signed cell areas), tiles the box
+    (sum of areas = L^2) and its forces sum to zero."""
+    cg = _cg()
+    Nc = 32768
+    b = cg.VertexBatch([Nc], seeds=[21], precision="fp64")
+    b.set_params(dt=0.005, T=1e-3, t1_threshold=0.04)
+    b.set_prefs(1.0, 3.9)
+    b.brownian_steps(200)
+    b.check_status()
+    assert b.counters(0)["t1_events"] > 0
+    b.compute_forces()
+    g = b.get()
+    vn, vcn, cvn, cv = b.topology()
+    L = float(np.sqrt(Nc))
+    assert b.Nv == 2 * Nc and cvn.sum() == 3 * b.Nv
+    for v in range(0, b.Nv, 97):
+        for c in vcn[3 * v: 3 * v + 3]:
+            assert v in cv[c, : cvn[c]]
+    assert abs(g["A"].sum() - L * L) < 1e-9 * L * L
+    assert (g["A"] > 0).all()
+    F = np.abs(np.concatenate([g["Fx"], g["Fy"]])).max()
+    assert abs(g["Fx"].sum()) < 1e-9 * F * b.Nv and abs(g["Fy"].sum()) < 1e-9 * F * b.Nv

@@ -242,3 +242,27 @@ def test_serving_alone_equals_mixed():
     xm, ym = mixed[0].result
     assert np.array_equal(xa, xm) and np.array_equal(ya, ym)
     assert all(r.latency is not None and r.latency >= 0 for r in mixed.values())
+
+
+def test_large_system_full_size(oracle):
+    """One N = 262 144 system (L = 512): the initial triangulation equals the
+    oracle's exact Delaunay triangulation ring for ring; after 40 FP32-bulk SPV
+    steps (P0 = 3.75, v0 = 0.1) the repaired topology still equals the exact
+    Delaunay triangulation of the current positions, with 6N ring entries."""
+    cg = _cg()
+    N = 262144
+    b = cg.VoronoiBatch([N], seeds=[13377], precision="fp32")
+    x, y, th, L = oracle.random_init(N, 13377)
+    nn, nbr, nm = b.topology(CAP)
+    onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+    assert np.array_equal(nn, onn) and np.array_equal(nbr, onbr)
+    b.set_prefs(1.0, 3.75)
+    b.set_motility(0.1, 1.0)
+    b.spv_steps(40)
+    b.check_status()
+    g = b.get()
+    nn, nbr, nm = b.topology(CAP)
+    assert nn.sum() == 6 * N
+    onn, onbr, onm = oracle.triangulate(g["x"], g["y"], L, cap=CAP)
+    assert np.array_equal(nn, onn) and np.array_equal(nbr, onbr)
+    assert b.counters(0)["repair_events"] > 0
