@@ -1143,9 +1143,18 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
             int x = (k < 0) ? i : b.nbr[k * N + i];
             const bool fresh = atomicExch(&b.mark[x], 1) == 0;
             const int slot = wave_reserve(fresh, &b.counters[1]);
-            if (fresh) {
-                b.nfq[slot] = x;
-                atomicAdd(&b.st[b.cell_sys[x]].nf_count, 1);
+            const int sx = fresh ? b.cell_sys[x] : 0;
+            if (fresh) b.nfq[slot] = x;
+            // per-system count, one atomic per wave when the fresh lanes share a system
+            const unsigned long long fm = __ballot(fresh);
+            if (fm != 0ull) {
+                const int leader = __ffsll((long long)fm) - 1;
+                const int s0 = __shfl(sx, leader);
+                if (__ballot(fresh && sx != s0) == 0ull) {
+                    if (__lane_id() == leader) atomicAdd(&b.st[s0].nf_count, __popcll(fm));
+                } else if (fresh) {
+                    atomicAdd(&b.st[sx].nf_count, 1);
+                }
             }
         }
     }
@@ -1201,6 +1210,7 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
         return;
     }
     b.nn[i] = z;
+    b.mark[i] = 2;
     for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
     const int tnew = owned_triangles(b, i);
     if (z != zold) atomicAdd(&st.nn_delta, z - zold);
@@ -1253,6 +1263,7 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
         if (lane < z) b.nbr[lane * N + i] = ring;
         if (lane == 0) {
             b.nn[i] = z;
+            b.mark[i] = 2;
             if (z != zold) atomicAdd(&st.nn_delta, z - zold);
             if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
         }
@@ -1314,6 +1325,7 @@ __device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *po
         return;
     }
     b.nn[i] = z;
+    b.mark[i] = 2;
     for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
 }
 
@@ -1350,7 +1362,10 @@ __global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, 
             continue;
         }
         if (lane < z) b.nbr[lane * N + i] = ring;
-        if (lane == 0) b.nn[i] = z;
+        if (lane == 0) {
+            b.nn[i] = z;
+            b.mark[i] = 2;
+        }
     }
 }
 
@@ -1392,22 +1407,33 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
     }
 }
 
-// delOther + geometry of retriangulated systems; geometry of locally repaired
-// cells; neighMax rule (voronoiModelBase.cpp:309-312); clears NeedsFixing marks
+// delOther of retriangulated systems (8 threads per cell, each taking slots
+// k, k + 8, k + 16); geometry of the cells whose ring a star kernel rewrote
+// (mark 2; the test kernel already computed it for unchanged rings);
+// clears the marks. NeedsFixing cells of systems that went global are handled
+// with their system's global queue.
 template <typename R>
 __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
     const int ng = b.counters[2];
     const int nl = b.counters[1];
-    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < ng + nl; q += gridDim.x * blockDim.x) {
+    const long long total = (long long)(ng + nl) * DELSET_LANES;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
         if (q < ng) {
-            int i = b.gq[q];
-            int s = b.cell_sys[i];
-            del_sets(b, i);
-            cell_geometry<R>(b, pos, i, b.sys[s]);
-        } else {
-            int i = b.nfq[q - ng];
+            const int i = b.gq[q];
+            const int z = b.nn[i];
+            if (z >= 3)
+                for (int k = k0; k < z; k += DELSET_LANES) del_set_slot(b, i, z, k);
+            if (k0 == 0) {
+                if (b.mark[i] == 2) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
+                b.mark[i] = 0;
+            }
+        } else if (k0 == 0) {
+            const int i = b.nfq[q - ng];
+            if (b.st[b.cell_sys[i]].mode != 1) continue;
+            if (b.mark[i] == 2) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
             b.mark[i] = 0;
-            if (b.st[b.cell_sys[i]].mode == 1) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
         }
     }
 }
@@ -1765,6 +1791,9 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     if (r) return r;
     hipLaunchKernelGGL(k_global_by_count, dim3(b->num_systems), dim3(256), 0, st, *b, 1);
     global_pass(*b, b->pos, st);
+    // the pass recomputes geometry only where a ring changed; positions may have
+    // changed everywhere here
+    launch_test_geometry(*b, b->pos, 0, st);
     return check();
 }
 
