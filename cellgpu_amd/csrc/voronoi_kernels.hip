@@ -754,6 +754,15 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
     }
 }
 
+// pass 1 for one cell at position p (also run by the step's force kernel)
+__device__ inline void bin_cell(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp, int i, dvec2 p) {
+    const int ib = max(0, min(sp.xsize - 1, (int)floor(p.x / sp.boxsize)));
+    const int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
+    const int bin = sp.bin_off + jb * sp.xsize + ib;
+    b.cell_bin[i] = bin;
+    b.cell_rank[i] = (int)atomicAdd(&b.bin_count[bin], 1u);
+}
+
 // Counting sort of the cells into the cell-list bins (reference
 // cellListGPU.cu:20-58: bin = clamp(floor(p / boxsize)); its fixed-capacity bins
 // become contiguous runs in bin order, so no bin can overflow). Pass 1: bin and
@@ -761,13 +770,7 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
 __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
-    const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
-    dvec2 p = ldpos(pos, i);
-    int ib = max(0, min(sp.xsize - 1, (int)floor(p.x / sp.boxsize)));
-    int jb = max(0, min(sp.xsize - 1, (int)floor(p.y / sp.boxsize)));
-    int bin = sp.bin_off + jb * sp.xsize + ib;
-    b.cell_bin[i] = bin;
-    b.cell_rank[i] = (int)atomicAdd(&b.bin_count[bin], 1u);
+    bin_cell(b, b.sys[b.cell_sys[i]], i, ldpos(pos, i));
 }
 
 // Pass 2, one block per system: exclusive prefix sum of the bin counts. A
@@ -1132,15 +1135,21 @@ void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int te
 
 // NeedsFixing = flagged cells and their current neighbours (reference
 // voronoiModelBase.cpp:617-646), built as a deduplicated device queue.
+constexpr int DELSET_LANES = 8;  // threads per cell of the per-slot queue kernels
+
+// (8 threads per flagged cell: entry e = 0 is the cell, e >= 1 its ring slot
+// e - 1; thread k0 takes entries k0, k0 + 8, ...)
 __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     const int nflag = b.counters[0];
     const int N = b.num_cells;
-    for (int q = (int)blockIdx.x * blockDim.x + threadIdx.x; q < nflag; q += gridDim.x * blockDim.x) {
-        int i = b.flagged[q];
-        b.repair[i] = 0;
-        int z = b.nn[i];
-        for (int k = -1; k < z; ++k) {
-            int x = (k < 0) ? i : b.nbr[k * N + i];
+    const long long total = (long long)nflag * DELSET_LANES;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
+        const int i = b.flagged[q];
+        const int z = b.nn[i];
+        for (int e = k0; e <= z; e += DELSET_LANES) {
+            const int x = (e == 0) ? i : b.nbr[(e - 1) * N + i];
             const bool fresh = atomicExch(&b.mark[x], 1) == 0;
             const int slot = wave_reserve(fresh, &b.counters[1]);
             const int sx = fresh ? b.cell_sys[x] : 0;
@@ -1157,6 +1166,7 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
                 }
             }
         }
+        if (k0 == 0) b.repair[i] = 0;
     }
 }
 
@@ -1273,7 +1283,6 @@ __global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, c
 // delOther of repaired cells (reference enforceTopology, voronoiModelBase.cpp:
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
 // (8 threads per cell, each taking slots k, k + 8, k + 16)
-constexpr int DELSET_LANES = 8;
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     const int nq = b.counters[1];
     const long long total = (long long)nq * DELSET_LANES;
@@ -1488,7 +1497,7 @@ template <> __device__ inline double atan2_r<double>(double y, double x) { retur
 constexpr int FORCE_BLOCK = 256;
 template <typename R, int KIND>
 __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos,
-                                                                double *pos_next) {
+                                                                double *pos_next, int bin) {
     // LDS copy of the block's cells (positions, dAP): most ring members and
     // delOther cells of a Hilbert-ordered block are in it
     __shared__ dvec2 spos[FORCE_BLOCK];
@@ -1509,6 +1518,7 @@ __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_bat
     dvec2 p = ldpos(pos, i);
     if (!st.active) {
         stpos(pos_next, i, p);
+        if (bin) bin_cell(b, sp, i, p);
         return;
     }
     const int step = st.step + 1;  // Timestep is incremented before integrating
@@ -1545,6 +1555,7 @@ __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_bat
     }
     put_in_box(make_box(sp.L), p);
     stpos(pos_next, i, p);
+    if (bin) bin_cell(b, sp, i, p);
 }
 
 __global__ void k_advance_step(cgpu_voronoi_batch b) {
@@ -1807,20 +1818,20 @@ int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream) {
     return check();
 }
 
-static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream) {
+static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream, int bin = 0) {
     // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
     dim3 g(nblocks(b->num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
     hipStream_t st = S(stream);
     if (b->precision) {
         if (kind == 0)
-            hipLaunchKernelGGL((k_force_eom_move<float, 0>), g, t, 0, st, *b, b->pos, b->pos_next);
+            hipLaunchKernelGGL((k_force_eom_move<float, 0>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
         else
-            hipLaunchKernelGGL((k_force_eom_move<float, 1>), g, t, 0, st, *b, b->pos, b->pos_next);
+            hipLaunchKernelGGL((k_force_eom_move<float, 1>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
     } else {
         if (kind == 0)
-            hipLaunchKernelGGL((k_force_eom_move<double, 0>), g, t, 0, st, *b, b->pos, b->pos_next);
+            hipLaunchKernelGGL((k_force_eom_move<double, 0>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
         else
-            hipLaunchKernelGGL((k_force_eom_move<double, 1>), g, t, 0, st, *b, b->pos, b->pos_next);
+            hipLaunchKernelGGL((k_force_eom_move<double, 1>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
     }
     return check();
 }
@@ -1838,9 +1849,13 @@ int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream) {
 static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
     hipStream_t st = S(stream);
     hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
-    int r = force_eom_move(b, kind, stream);
+    // the force kernel also runs the binning pass of the cell list on the new
+    // positions (bin counts are zero: the previous scan cleared them)
+    int r = force_eom_move(b, kind, stream, 1);
     if (r) return r;
-    r = cgpu_compute_cell_list(b, 1, stream);
+    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(SCAN_THREADS), 0, st, *b);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
+    r = check();
     if (r) return r;
     launch_test_geometry(*b, b->pos_next, 1, st);
     r = repair_chain(*b, b->pos_next, st);
