@@ -566,8 +566,25 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         }
         if (__ballot(bad) != 0ull || __popcll(__ballot(wrapped)) != 1) verify = 0;
     }
+    // the first block that holds every current-ring circle decides the
+    // verification: start there (no star built from a smaller block is needed,
+    // any certified block gives the same ring)
+    int wfirst = w0;
+    if (verify == 1 && w0 == 1) {
+        int need = 1;
+        if (lane < zold) {
+            while (need < 3) {
+                const double xlo = (ib - need) * bs - pi.x + mg, xhi = (ib + need + 1) * bs - pi.x - mg;
+                const double ylo = (jb - need) * bs - pi.y + mg, yhi = (jb + need + 1) * bs - pi.y - mg;
+                if (ccx - cr >= xlo && ccx + cr <= xhi && ccy - cr >= ylo && ccy + cr <= yhi) break;
+                ++need;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) need = max(need, __shfl_xor(need, o));
+        wfirst = min(need, wmax);
+    }
 
-    for (int w = w0; w <= wmax; ++w) {
+    for (int w = wfirst; w <= wmax; ++w) {
         w_at = w;
         const int nb = min(2 * w + 1, xs);
         if (nb * nb > 64) return -3;
