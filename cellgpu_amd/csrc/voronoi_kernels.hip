@@ -573,7 +573,7 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
     if (verify == 1 && w0 == 1) {
         int need = 1;
         if (lane < zold) {
-            while (need < 3) {
+            while (need < 2) {
                 const double xlo = (ib - need) * bs - pi.x + mg, xhi = (ib + need + 1) * bs - pi.x - mg;
                 const double ylo = (jb - need) * bs - pi.y + mg, yhi = (jb + need + 1) * bs - pi.y - mg;
                 if (ccx - cr >= xlo && ccx + cr <= xhi && ccy - cr >= ylo && ccy + cr <= yhi) break;
