@@ -157,7 +157,8 @@ template <typename R> struct BlockTile {
 // Voronoi area and perimeter of cell i in the reference GPU order
 // (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
 template <typename R>
-__device__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
+__device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, int i,
+                                              const cgpu_sys_params &sp) {
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
@@ -975,7 +976,7 @@ __device__ __forceinline__ void serial_test_wide(const cgpu_voronoi_batch &b, co
 // window) when its window is the 3 x 3 bins around cell i's bin (wcheck = 1):
 // independent loads of the nine bins. Returns false, testing nothing, when the
 // window is wider.
-__device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+__device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     if (system_goes_global(b, s, sp)) return true;
@@ -1025,14 +1026,19 @@ __device__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int 
     }
     bool bad = false;
     auto scan_run = [&](int k0, int k1) {
-        for (int k = k0; k < k1; ++k) {
-            const int m = b.sorted_id[k];
-            const dvec2 a = min_dist(box, ldpos(b.sorted_pos, k), v);
-            if (m == i || m == j || m == l) continue;
-            if (cand_inside(box, w, a)) {
-                bad = true;
-                flag_cell(b, s, m);
-            }
+        // two candidates per iteration: their loads are in flight together
+        for (int k = k0; k < k1; k += 2) {
+            const bool two = k + 1 < k1;
+            const int m0 = b.sorted_id[k];
+            const int m1 = two ? b.sorted_id[k + 1] : -1;
+            const dvec2 q0 = ldpos(b.sorted_pos, k);
+            const dvec2 q1 = two ? ldpos(b.sorted_pos, k + 1) : q0;
+            const dvec2 a0 = min_dist(box, q0, v), a1 = min_dist(box, q1, v);
+            const bool in0 = !(m0 == i || m0 == j || m0 == l) && cand_inside(box, w, a0);
+            const bool in1 = m1 >= 0 && !(m1 == i || m1 == j || m1 == l) && cand_inside(box, w, a1);
+            if (in0) flag_cell(b, s, m0);
+            if (in1) flag_cell(b, s, m1);
+            bad = bad || in0 || in1;
         }
     };
     scan_run(lo[0], hi[0]);
@@ -1095,7 +1101,7 @@ __device__ __forceinline__ void wave_test_wide(const cgpu_voronoi_batch &b, cons
 // serially by the thread of its cell.
 constexpr int TRI_LDS = 1280;
 template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_test_geometry(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
     cgpu_voronoi_batch b, const double *pos, int test) {
     __shared__ int3 stri[TRI_LDS];
     __shared__ short swide[TRI_LDS];
