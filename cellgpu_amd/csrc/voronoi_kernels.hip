@@ -168,11 +168,13 @@ __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const
     dvec2 rik = min_dist(box, ldpos(pos, b.nbr[i]), pi);
     vec2<R> vfirst = Prec<R>::c0(rij, rik), vlast = vfirst;
     R A = R(0), P = R(0);
-    // ring positions are fetched one slot ahead of their use
+    // ring positions are fetched one slot ahead of their use, ring indices two
     dvec2 pnext = ldpos(pos, b.nbr[(1 % z) * N + i]);
+    int jnext = b.nbr[(2 % z) * N + i];
     for (int k = 1; k < z; ++k) {
         const dvec2 pk = pnext;
-        if (k + 1 < z) pnext = ldpos(pos, b.nbr[(k + 1) * N + i]);
+        if (k + 1 < z) pnext = ldpos(pos, jnext);
+        if (k + 2 < z) jnext = b.nbr[(k + 2) * N + i];
         rij = rik;
         rik = min_dist(box, pk, pi);
         vec2<R> vnext = Prec<R>::c0(rij, rik);
@@ -209,16 +211,22 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
     vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
     const vec2<R> qI = acc.q(i);
     vec2<R> sum{R(0), R(0)};
-    // slot k's neighbour, delOther and their positions are fetched one slot ahead
+    // slot k's neighbour, delOther and their positions are fetched one slot
+    // ahead, the indices two slots ahead
     int jDn = b.nbr[i];
     dvec2 pDn = acc.p(jDn), pOn = acc.p(b.oth[i]);
+    int jD2 = b.nbr[(1 % z) * N + i], oD2 = b.oth[(1 % z) * N + i];
     for (int k = 0; k < z; ++k) {
         const int jD = jDn;
         const dvec2 pD = pDn, pO = pOn;
         if (k + 1 < z) {
-            jDn = b.nbr[(k + 1) * N + i];
-            pDn = acc.p(jDn);
-            pOn = acc.p(b.oth[(k + 1) * N + i]);
+            jDn = jD2;
+            pDn = acc.p(jD2);
+            pOn = acc.p(oD2);
+        }
+        if (k + 2 < z) {
+            jD2 = b.nbr[(k + 2) * N + i];
+            oD2 = b.oth[(k + 2) * N + i];
         }
         dvec2 dD = min_dist(box, pD, pi);
         vec2<R> vN = Prec<R>::c0(dC, dD);
