@@ -1267,7 +1267,7 @@ constexpr int WIDE_ID_BITS = 26;
 constexpr int WIDE_ID_MASK = (1 << WIDE_ID_BITS) - 1;
 
 template <int K>
-__global__ __launch_bounds__(256) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
     const int nq = (K == 1) ? b.counters[1] : b.counters[6];
     const int *queue = (K == 1) ? b.nfq : b.flagged;
     const int N = b.num_cells;
@@ -1371,7 +1371,7 @@ __device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *po
 
 // the same, one wavefront per cell of the global queue (K as for the local pass)
 template <int K>
-__global__ __launch_bounds__(256) void k_star_global_wave(cgpu_voronoi_batch b, const double *pos) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_global_wave(cgpu_voronoi_batch b, const double *pos) {
     const int nq = (K == 1) ? b.counters[2] : b.counters[6];
     const int *queue = (K == 1) ? b.gq : b.flagged;
     const int N = b.num_cells;
