@@ -132,6 +132,73 @@ def serving(count=48, rate=8.0, steps_range=(1000, 10000)):
                                     count, rate, steps_range[0], steps_range[1])}
 
 
+def other_workload(cg, name, steps, warmup):
+    """Secondary workloads (survey cfg2 / cfg3), one JSON line each; the
+    headline stays the ensemble."""
+    if name == "single":
+        # one N = 262 144 system (L = 512), P0 = 3.75, v0 = 0.1, Dr = 1, dt = 0.01
+        N = 262144
+        b = cg.VoronoiBatch([N], seeds=[13377], precision="fp32")
+        b.set_params(KA=1.0, KP=1.0, dt=0.01, mu=1.0)
+        b.spatial_sort()
+        b.set_prefs(1.0, 3.75)
+        b.set_motility(0.1, 1.0)
+        b.spv_steps(warmup)
+        b.sync()
+        b.check_status()
+        t0 = time.perf_counter()
+        b.spv_steps(steps)
+        b.sync()
+        el = time.perf_counter() - t0
+        b.check_status()
+        return {"metric": "cell-steps/s, one N=262144 SPV system", "value": round(N * steps / el, 1),
+                "unit": "cell-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * el / steps, 5),
+                "dtype": "f32", "config": {"workload": "single SPV system N=262144 L=512 P0=3.75 v0=0.1 Dr=1 dt=0.01"}}
+    if name == "fire":
+        # FIRE on the same system shape (dtMax 0.1, alpha0 0.1), a bounded number of iterations
+        N = 262144
+        b = cg.VoronoiBatch([N], seeds=[13377], precision="fp64")
+        b.spatial_sort()
+        b.set_prefs(1.0, 3.75)
+        f = cg.EnergyMinimizerFIRE(b)
+        f.setDeltaT(0.01)
+        f.setDeltaTMax(0.1)
+        f.setAlphaStart(0.1)
+        f.setForceCutoff(1e-10)
+        f.setMaximumIterations(warmup)
+        f.minimize()
+        b.sync()
+        it0 = f.state()["iterations"]
+        f.setMaximumIterations(warmup + steps)
+        t0 = time.perf_counter()
+        f.minimize()
+        b.sync()
+        el = time.perf_counter() - t0
+        st = f.state()
+        its = st["iterations"] - it0
+        return {"metric": "cell-iterations/s, FIRE on one N=262144 system (FP64)", "value": round(N * its / el, 1),
+                "unit": "cell-iterations/s", "iterations": its, "ms_per_iteration": round(1e3 * el / max(its, 1), 5),
+                "dtype": "f64", "rms_force": st["force_max"],
+                "config": {"workload": "FIRE N=262144 L=512 P0=3.75 dt0=0.01 dtMax=0.1 alpha0=0.1 cutoff 1e-10"}}
+    if name == "vertex":
+        # vertex model: Nc = 32 768 (Nv = 65 536), Brownian T = 1e-3, dt = 0.005, T1 threshold 0.04, P0 = 3.9
+        Nc = 32768
+        b = cg.VertexBatch([Nc], seeds=[21], precision="fp32")
+        b.set_params(dt=0.005, T=1e-3, t1_threshold=0.04)
+        b.set_prefs(1.0, 3.9)
+        b.brownian_steps(warmup)
+        b.check_status()
+        t0 = time.perf_counter()
+        b.brownian_steps(steps)
+        b.check_status()
+        el = time.perf_counter() - t0
+        return {"metric": "cell-steps/s, vertex model Nc=32768 Brownian with T1", "value": round(Nc * steps / el, 1),
+                "unit": "cell-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * el / steps, 5),
+                "dtype": "f32", "t1_events": b.counters(0)["t1_events"],
+                "config": {"workload": "vertex Nc=32768 Nv=65536 T=1e-3 dt=0.005 thr=0.04 P0=3.9"}}
+    raise SystemExit("unknown workload " + name)
+
+
 def copy_probe(mib, reps=5):
     """Plain copy kernel of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
@@ -154,6 +221,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "single", "fire", "vertex"],
+                    help="ensemble = the headline; the others print their own line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -167,6 +236,9 @@ def main():
 
     import cellgpu_amd as cg
     cg.require_device()
+    if args.workload != "ensemble":
+        print(json.dumps(other_workload(cg, args.workload, args.steps, args.warmup)), flush=True)
+        return
     n, P0, v0, Dr, seeds = ensemble()
     N = int(n.sum())
     b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
