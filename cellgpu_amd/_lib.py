@@ -34,7 +34,8 @@ class VoronoiBatchView(C.Structure):
             "sys", "st", "cell_sys", "pos", "pos_next", "nn", "nbr", "oth", "AP", "dAP", "APpref", "mot", "vel",
             "theta", "force", "excl", "ext_force", "bin_count", "bin_start", "cell_bin", "cell_rank", "sorted_id",
             "sorted_pos", "repair", "mark", "flagged", "nfq", "gq",
-            "counters", "noise")]
+            "counters", "noise", "chunk_sys", "chunk_begin", "chunk_end", "bchunk_sys", "bchunk_begin",
+            "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)]
 
 
 def lib():

@@ -68,6 +68,43 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
         for (int i = 0; i < hp_[s].n; ++i) cell_sys_h_[hp_[s].off + i] = s;
 
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    {
+        // work chunks inside systems (cells and cell-list bins)
+        std::vector<int> cs, cb, ce, bs, bb, be, sb;
+        for (int s = 0; s < S_; ++s) {
+            const cgpu_sys_params &p = hp_[s];
+            for (int c = p.off; c < p.off + p.n; c += CGPU_CHUNK) {
+                cs.push_back(s);
+                cb.push_back(c);
+                ce.push_back(std::min(c + CGPU_CHUNK, p.off + p.n));
+            }
+            sb.push_back((int)bs.size());
+            const int nb = p.xsize * p.xsize;
+            for (int u = 0; u < nb; u += CGPU_BIN_CHUNK) {
+                bs.push_back(s);
+                bb.push_back(p.bin_off + u);
+                be.push_back(p.bin_off + std::min(u + CGPU_BIN_CHUNK, nb));
+            }
+        }
+        sb.push_back((int)bs.size());
+        num_chunks_ = (int)cs.size();
+        num_bin_chunks_ = (int)bs.size();
+        chunk_sys_.alloc(cs.size());
+        chunk_sys_.upload(cs.data(), cs.size());
+        chunk_begin_.alloc(cb.size());
+        chunk_begin_.upload(cb.data(), cb.size());
+        chunk_end_.alloc(ce.size());
+        chunk_end_.upload(ce.data(), ce.size());
+        bchunk_sys_.alloc(bs.size());
+        bchunk_sys_.upload(bs.data(), bs.size());
+        bchunk_begin_.alloc(bb.size());
+        bchunk_begin_.upload(bb.data(), bb.size());
+        bchunk_end_.alloc(be.size());
+        bchunk_end_.upload(be.data(), be.size());
+        sys_bchunk_.alloc(sb.size());
+        sys_bchunk_.upload(sb.data(), sb.size());
+        bchunk_sum_.alloc(bs.size());
+    }
     sys_.alloc(S_);
     st_.alloc(S_);
     cell_sys_.alloc(N_);
@@ -218,6 +255,16 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.gq = gq_.p;
     b.counters = counters_.p;
     b.noise = noise_buffer_ ? noise_.p : nullptr;
+    b.chunk_sys = chunk_sys_.p;
+    b.chunk_begin = chunk_begin_.p;
+    b.chunk_end = chunk_end_.p;
+    b.bchunk_sys = bchunk_sys_.p;
+    b.bchunk_begin = bchunk_begin_.p;
+    b.bchunk_end = bchunk_end_.p;
+    b.sys_bchunk = sys_bchunk_.p;
+    b.bchunk_sum = bchunk_sum_.p;
+    b.num_chunks = num_chunks_;
+    b.num_bin_chunks = num_bin_chunks_;
     return b;
 }
 

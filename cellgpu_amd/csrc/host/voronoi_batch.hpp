@@ -117,6 +117,9 @@ class VoronoiBatch {
     DevBuf<cgpu_sys_state> st_;
     DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_;
     DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_;
+    DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_, bchunk_sys_, bchunk_begin_, bchunk_end_, sys_bchunk_,
+        bchunk_sum_;
+    int num_chunks_ = 0, num_bin_chunks_ = 0;
     DevBuf<unsigned int> bin_count_;
     DevBuf<double> pos0_, pos1_, noise_, sorted_pos_;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;

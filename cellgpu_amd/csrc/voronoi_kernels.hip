@@ -852,6 +852,91 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b)
     if (s == b.num_systems - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
 }
 
+// The same prefix sum for systems with more than one bin chunk, in three
+// passes over bin chunks: chunk sums, per-system offsets of the chunks, then the
+// scan of each chunk from its offset.
+__global__ __launch_bounds__(SCAN_THREADS) void k_bin_chunk_sum(cgpu_voronoi_batch b) {
+    const int c = blockIdx.x;
+    const int u0 = b.bchunk_begin[c], u1 = b.bchunk_end[c];
+    int tot = 0;
+    for (int u = u0 + (int)threadIdx.x; u < u1; u += blockDim.x) tot += (int)b.bin_count[u];
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+    __shared__ int wsum[SCAN_THREADS / 64];
+    if (__lane_id() == 0) wsum[threadIdx.x >> 6] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += wsum[k];
+        b.bchunk_sum[c] = t;
+    }
+}
+__global__ void k_bin_chunk_offsets(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    int off = b.sys[s].off;
+    for (int c = b.sys_bchunk[s]; c < b.sys_bchunk[s + 1]; ++c) {
+        const int t = b.bchunk_sum[c];
+        b.bchunk_sum[c] = off;
+        off += t;
+    }
+}
+__global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan_chunks(cgpu_voronoi_batch b) {
+    const int cidx = blockIdx.x;
+    const int u0 = b.bchunk_begin[cidx], u1 = b.bchunk_end[cidx];
+    unsigned int *cnt = b.bin_count;
+    __shared__ int wsum[SCAN_THREADS / 64];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = b.bchunk_sum[cidx];
+    __syncthreads();
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
+    for (int base = u0; base < u1; base += SCAN_THREADS * SCAN_PER) {
+        const int v0 = base + threadIdx.x * SCAN_PER;
+        int c[SCAN_PER];
+        int tot = 0;
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r) {
+            c[r] = (v0 + r < u1) ? (int)cnt[v0 + r] : 0;
+            tot += c[r];
+        }
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r)
+            if (v0 + r < u1) cnt[v0 + r] = 0u;
+        int incl = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int off = carry + incl - tot;
+        for (int k = 0; k < wv; ++k) off += wsum[k];
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r) {
+            if (v0 + r < u1) b.bin_start[v0 + r] = off;
+            off += c[r];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < SCAN_THREADS / 64; ++k) t += wsum[k];
+            carry += t;
+        }
+        __syncthreads();
+    }
+    if (cidx == b.num_bin_chunks - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
+}
+
+// prefix sum of the bin counts: one pass when every system is one bin chunk
+void launch_bin_scan(const cgpu_voronoi_batch &b, hipStream_t st) {
+    if (b.num_bin_chunks == b.num_systems) {
+        hipLaunchKernelGGL(k_bin_scan, dim3(b.num_systems), dim3(SCAN_THREADS), 0, st, b);
+        return;
+    }
+    hipLaunchKernelGGL(k_bin_chunk_sum, dim3(b.num_bin_chunks), dim3(SCAN_THREADS), 0, st, b);
+    hipLaunchKernelGGL(k_bin_chunk_offsets, dim3((b.num_systems + 63) / 64), dim3(64), 0, st, b);
+    hipLaunchKernelGGL(k_bin_scan_chunks, dim3(b.num_bin_chunks), dim3(SCAN_THREADS), 0, st, b);
+}
+
 // Pass 3: every cell to its bin-ordered slot, with its position.
 __global__ void k_bin_scatter(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -1201,18 +1286,21 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     }
 }
 
-// enqueue every cell of system s into the global queue
-__device__ void enqueue_system(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp) {
+// enqueue the cells of chunk c into the global queue (one reservation per chunk)
+__device__ void enqueue_chunk(const cgpu_voronoi_batch &b, int c) {
     __shared__ int base;
-    if (threadIdx.x == 0) base = atomicAdd(&b.counters[2], sp.n);
+    const int c0 = b.chunk_begin[c], n = b.chunk_end[c] - c0;
+    if (threadIdx.x == 0) base = atomicAdd(&b.counters[2], n);
     __syncthreads();
-    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) b.gq[base + t] = sp.off + t;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) b.gq[base + t] = c0 + t;
 }
 
-// one block per system: |NeedsFixing| > N/6 -> global retriangulation
-// (reference voronoiModelBase.cpp:650-653); all=1 forces a global pass
+// |NeedsFixing| > N/6 -> global retriangulation (reference
+// voronoiModelBase.cpp:650-653); all=1 forces a global pass
 __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
-    const int s = blockIdx.x;
+    // one block per cell chunk; every chunk of a system takes the same decision
+    const int c = blockIdx.x;
+    const int s = b.chunk_sys[c];
     const cgpu_sys_params &sp = b.sys[s];
     cgpu_sys_state &st = b.st[s];
     int mode;
@@ -1222,15 +1310,14 @@ __global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
         if (!st.any_fail) return;
         mode = (st.nf_count > sp.n / 6) ? 2 : 1;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && b.chunk_begin[c] == sp.off) {
         st.mode = mode;
         if (!all) {
             st.repair_events += 1;
             if (mode == 1) st.local_repairs += st.nf_count;
         }
     }
-    if (mode == 2) enqueue_system(b, sp);
+    if (mode == 2) enqueue_chunk(b, c);
 }
 
 // local repair of cell i: new Delaunay star of a NeedsFixing cell (reference
@@ -1337,19 +1424,22 @@ __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     }
 }
 
-// one block per system: a failed local repair becomes a global one; block 0
-// also empties the serial-form queue for the global pass
+// a failed local repair becomes a global one; block 0 also empties the
+// serial-form queue for the global pass
 __global__ void k_global_fallback(cgpu_voronoi_batch b) {
-    const int s = blockIdx.x;
-    if (s == 0 && threadIdx.x == 0) b.counters[6] = 0;
+    // one block per cell chunk; the first chunk of a system switches its mode to
+    // 3, which the others still accept
+    const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x == 0) b.counters[6] = 0;
+    const int s = b.chunk_sys[c];
     cgpu_sys_state &st = b.st[s];
-    if (!(st.mode == 1 && st.need_global)) return;
+    if (!((st.mode == 1 || st.mode == 3) && st.need_global)) return;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && b.chunk_begin[c] == b.sys[s].off) {
         st.mode = 3;
         st.nn_delta = 0;
     }
-    enqueue_system(b, b.sys[s]);
+    enqueue_chunk(b, c);
 }
 
 // global retriangulation (globalTriangulationCGAL) of cell i, serial form
@@ -1409,17 +1499,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
-// one block per retriangulated system: degree sum (6N invariant,
-// voronoiModelBase.cpp:263) and neighMax = max degree + 2 (even) or + 1 (odd)
-// (:309-312)
+// degree sum (6N invariant, voronoiModelBase.cpp:263) and maximum of the
+// retriangulated systems, one block per cell chunk (k_reset and
+// k_global_fallback zero the sums)
 __global__ void k_global_stats(cgpu_voronoi_batch b) {
-    const int s = blockIdx.x;
-    const cgpu_sys_params &sp = b.sys[s];
+    const int c = blockIdx.x;
+    const int s = b.chunk_sys[c];
     cgpu_sys_state &st = b.st[s];
     if (st.mode < 2) return;
     int sum = 0, mx = 0;
-    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) {
-        const int z = b.nn[sp.off + t];
+    for (int i = b.chunk_begin[c] + (int)threadIdx.x; i < b.chunk_end[c]; i += blockDim.x) {
+        const int z = b.nn[i];
         sum += z;
         mx = max(mx, z);
     }
@@ -1439,12 +1529,22 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
             sum += ssum[k];
             mx = max(mx, smax[k]);
         }
-        st.max_deg = mx;
-        st.nn_delta = sum;
-        st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
-        st.global_fixes += 1;
-        if (sum != 6 * sp.n) st.error = CGPU_ERR_TOPOLOGY;
+        atomicAdd(&st.nn_delta, sum);
+        atomicMax(&st.max_deg, mx);
     }
+}
+
+// per retriangulated system: neighMax = max degree + 2 (even) or + 1 (odd)
+// (voronoiModelBase.cpp:309-312), the 6N check, the global-fix counter
+__global__ void k_global_close(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    cgpu_sys_state &st = b.st[s];
+    if (st.mode < 2) return;
+    const int mx = st.max_deg;
+    st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
+    st.global_fixes += 1;
+    if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
 }
 
 // delOther of retriangulated systems (8 threads per cell, each taking slots
@@ -1774,7 +1874,8 @@ constexpr int WIDE_BLOCKS = 256;        // two candidates per lane, for wide bin
 void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
     hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_global_stats, dim3(b.num_systems), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_close, dim3((b.num_systems + 63) / 64), dim3(64), 0, st, b);
     if (b.precision)
         hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
     else
@@ -1783,11 +1884,11 @@ void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
 
 int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
     hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_systems), dim3(256), 0, st, b, 0);
+    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_chunks), dim3(256), 0, st, b, 0);
     hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_systems), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_chunks), dim3(256), 0, st, b);
     global_pass(b, pos, st);
     return check();
 }
@@ -1809,7 +1910,7 @@ int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stre
     // bin counts are zero here: allocation clears them and k_bin_scan clears
     // them after use
     hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
-    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(SCAN_THREADS), 0, S(stream), *b);
+    launch_bin_scan(*b, S(stream));
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
     return check();
 }
@@ -1831,7 +1932,7 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
     int r = cgpu_compute_cell_list(b, 0, stream);
     if (r) return r;
-    hipLaunchKernelGGL(k_global_by_count, dim3(b->num_systems), dim3(256), 0, st, *b, 1);
+    hipLaunchKernelGGL(k_global_by_count, dim3(b->num_chunks), dim3(256), 0, st, *b, 1);
     global_pass(*b, b->pos, st);
     // the pass recomputes geometry only where a ring changed; positions may have
     // changed everywhere here
@@ -1884,7 +1985,7 @@ static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
     // positions (bin counts are zero: the previous scan cleared them)
     int r = force_eom_move(b, kind, stream, 1);
     if (r) return r;
-    hipLaunchKernelGGL(k_bin_scan, dim3(b->num_systems), dim3(SCAN_THREADS), 0, st, *b);
+    launch_bin_scan(*b, st);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
     r = check();
     if (r) return r;

@@ -115,7 +115,21 @@ typedef struct cgpu_voronoi_batch {
     int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
                                     5 nonfinite, 6 serial-star queue (in flagged) */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
+    /* Work chunks: at most CGPU_CHUNK cells (or CGPU_BIN_CHUNK bins) of one
+     * system, so per-system passes spread a large system over many work-groups. */
+    const int *chunk_sys;        /* [num_chunks] system of each cell chunk         */
+    const int *chunk_begin;      /* [num_chunks] first cell                        */
+    const int *chunk_end;        /* [num_chunks] one past the last cell            */
+    const int *bchunk_sys;       /* [num_bin_chunks] system of each bin chunk      */
+    const int *bchunk_begin;     /* [num_bin_chunks] first bin (batch index)       */
+    const int *bchunk_end;       /* [num_bin_chunks] one past the last bin         */
+    const int *sys_bchunk;       /* [S+1] first bin chunk of each system           */
+    int *bchunk_sum;             /* [num_bin_chunks] scratch of the bin scan       */
+    int num_chunks, num_bin_chunks;
 } cgpu_voronoi_batch;
+
+#define CGPU_CHUNK 4096
+#define CGPU_BIN_CHUNK 8192
 
 /* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
  * vertexModelBase.cpp:210, grown on demand there; fixed capacity here). */
