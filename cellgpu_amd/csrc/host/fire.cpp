@@ -18,7 +18,7 @@ class FireMinimizer {
         size_t rsz = vb->precision() ? sizeof(float) : sizeof(double);
         fs_.alloc(S_);
         vel_.alloc(2 * (size_t)N_ * rsz);
-        dots_.alloc(3 * (size_t)N_);
+        dots_.alloc(3 * (size_t)N_ + 4 * (size_t)vb->view().num_chunks);
         hs_.resize(S_);
         for (auto &f : hs_) {
             std::memset(&f, 0, sizeof(f));

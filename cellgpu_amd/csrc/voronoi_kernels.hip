@@ -1286,6 +1286,19 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     }
 }
 
+// index of the chunk that holds cell i (binary search over chunk starts)
+__device__ inline int chunk_of(const cgpu_voronoi_batch &b, int i) {
+    int lo = 0, hi = b.num_chunks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.chunk_begin[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // enqueue the cells of chunk c into the global queue (one reservation per chunk)
 __device__ void enqueue_chunk(const cgpu_voronoi_batch &b, int c) {
     __shared__ int base;
@@ -1789,24 +1802,26 @@ __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs
 }
 
 // one block per system: deterministic FP64 sums, then the FIRE scalar update
-__global__ void k_fire_reduce(cgpu_voronoi_batch b, cgpu_fire_state *fs, const double *dots) {
-    const int s = blockIdx.x;
-    cgpu_fire_state &f = fs[s];
-    if (f.converged || !b.st[s].active) return;
-    const cgpu_sys_params &sp = b.sys[s];
+// per cell chunk: partial sums of |F|^2, F.v, |v|^2 and the largest |F|^2, in
+// a fixed order (thread-strided, then a fixed tree): deterministic
+__global__ void k_fire_partial(cgpu_voronoi_batch b, const cgpu_fire_state *fs, double *dots) {
+    const int c = blockIdx.x;
+    const int s = b.chunk_sys[c];
     const int N = b.num_cells;
+    double *part = dots + 3 * (size_t)N + 4 * (size_t)c;
     __shared__ double sh[4][256];
-    double a = 0, c = 0, d = 0, m = 0;
-    for (int t = threadIdx.x; t < sp.n; t += blockDim.x) {
-        int i = sp.off + t;
-        double ff = dots[i];
-        a += ff;
-        c += dots[N + i];
-        d += dots[2 * N + i];
-        m = fmax(m, ff);
+    double a = 0, cc = 0, d = 0, m = 0;
+    if (!(fs[s].converged || !b.st[s].active)) {
+        for (int i = b.chunk_begin[c] + (int)threadIdx.x; i < b.chunk_end[c]; i += blockDim.x) {
+            const double ff = dots[i];
+            a += ff;
+            cc += dots[N + i];
+            d += dots[2 * N + i];
+            m = fmax(m, ff);
+        }
     }
     sh[0][threadIdx.x] = a;
-    sh[1][threadIdx.x] = c;
+    sh[1][threadIdx.x] = cc;
     sh[2][threadIdx.x] = d;
     sh[3][threadIdx.x] = m;
     __syncthreads();
@@ -1819,12 +1834,30 @@ __global__ void k_fire_reduce(cgpu_voronoi_batch b, cgpu_fire_state *fs, const d
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double forceNorm = sh[0][0], power = sh[1][0], velNorm = sh[2][0];
+    if (threadIdx.x < 4) part[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// per system: the chunk partials summed in chunk order, then the FIRE update
+__global__ void k_fire_reduce(cgpu_voronoi_batch b, cgpu_fire_state *fs, const double *dots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    cgpu_fire_state &f = fs[s];
+    if (f.converged || !b.st[s].active) return;
+    const cgpu_sys_params &sp = b.sys[s];
+    const double *part = dots + 3 * (size_t)b.num_cells;
+    // chunks of a system are consecutive; find the first by its cell range
+    double forceNorm = 0, power = 0, velNorm = 0, mf = 0;
+    for (int c = chunk_of(b, sp.off); c < b.num_chunks && b.chunk_sys[c] == s; ++c) {
+        forceNorm += part[4 * c];
+        power += part[4 * c + 1];
+        velNorm += part[4 * c + 2];
+        mf = fmax(mf, part[4 * c + 3]);
+    }
+    {
         f.iterations += 1;
         f.power = power;
         f.force_max = forceNorm / (double)sp.n;
-        f.max_f2 = sh[3][0];
+        f.max_f2 = mf;
         f.scaling = forceNorm > 0. ? sqrt(velNorm / forceNorm) : 0.0;
         f.mix_alpha = f.alpha;
         f.pending = 1;
@@ -2042,7 +2075,8 @@ int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, voi
         hipLaunchKernelGGL(k_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
     else
         hipLaunchKernelGGL(k_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
-    hipLaunchKernelGGL(k_fire_reduce, dim3(b->num_systems), dim3(256), 0, S(stream), *b, fs, dots);
+    hipLaunchKernelGGL(k_fire_partial, dim3(b->num_chunks), dim3(256), 0, S(stream), *b, fs, dots);
+    hipLaunchKernelGGL(k_fire_reduce, dim3((b->num_systems + 63) / 64), dim3(64), 0, S(stream), *b, fs, dots);
     return check();
 }
 

@@ -251,7 +251,8 @@ typedef struct cgpu_fire_state {
 int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 /* forces on the moved positions, second velocity half step and the three
  * FP64 dot products, then per-system reduction and the FIRE update of alpha,
- * dt and the stopping test (fireStepGPU) */
+ * dt and the stopping test (fireStepGPU). dots: 3 N + 4 num_chunks doubles
+ * (per-cell products, then per-chunk partial sums). */
 int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
 /* mark systems whose FIRE loop has stopped as inactive for the topology kernels */
 int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state *fs, void *stream);
