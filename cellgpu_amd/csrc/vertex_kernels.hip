@@ -145,8 +145,38 @@ __global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos) {
     }
 }
 
+// A cell's vertex list loaded into registers in one batch of independent loads
+// (lists of up to CL vertices; longer lists take the memory loops), so a flip
+// is a few round trips to memory instead of one per list entry.
+constexpr int CL = 12;
+struct CellRegs {
+    int c, n;
+    int v[CL];
+};
+__device__ inline void load_cell(const cgpu_vertex_batch &b, int c, CellRegs &r) {
+    const int Nc = b.num_cells;
+    r.c = c;
+    r.n = b.cvn[c];
+#pragma unroll
+    for (int k = 0; k < CL; ++k) r.v[k] = (k < r.n) ? b.cv[k * Nc + c] : -1;
+}
+__device__ inline int reg_at(const CellRegs &r, int idx) {
+    int out = -1;
+#pragma unroll
+    for (int k = 0; k < CL; ++k)
+        if (k == idx) out = r.v[k];
+    return out;
+}
+__device__ inline int reg_find(const CellRegs &r, int x) {
+    int at = -1;
+#pragma unroll
+    for (int k = 0; k < CL; ++k)
+        if (k < r.n && r.v[k] == x && at < 0) at = k;
+    return at;
+}
+
 // position of x in cell c's list: predecessor and successor (reference loop)
-__device__ inline void locate(const cgpu_vertex_batch &b, int c, int x, int &pred, int &succ) {
+__device__ inline void locate_mem(const cgpu_vertex_batch &b, int c, int x, int &pred, int &succ) {
     const int Nc = b.num_cells;
     const int n = b.cvn[c];
     int vlast = b.cv[(n - 2) * Nc + c], vcur = b.cv[(n - 1) * Nc + c], vnext = b.cv[c];
@@ -159,11 +189,35 @@ __device__ inline void locate(const cgpu_vertex_batch &b, int c, int x, int &pre
     pred = vlast;
     succ = vnext;
 }
-__device__ inline bool cell_has(const cgpu_vertex_batch &b, int c, int x) {
-    const int Nc = b.num_cells;
-    for (int k = 0; k < b.cvn[c]; ++k)
-        if (b.cv[k * Nc + c] == x) return true;
-    return false;
+// the same on a register copy (identical result: the loop above stops at the
+// first k with vcur == x, i.e. the first occurrence scanning from slot n-1)
+__device__ inline void locate(const cgpu_vertex_batch &b, const CellRegs &r, int x, int &pred, int &succ) {
+    if (r.n > CL) {
+        locate_mem(b, r.c, x, pred, succ);
+        return;
+    }
+    const int n = r.n;
+    int at = (reg_at(r, n - 1) == x) ? n - 1 : -1;
+    if (at < 0) {
+        const int f = reg_find(r, x);
+        at = (f >= 0 && f < n - 1) ? f : -1;
+    }
+    if (at < 0) {  // x absent: the reference loop ends with (v[n-2], v[n-1]) -> pred v[n-2], succ v[n-1]
+        pred = reg_at(r, n - 2);
+        succ = reg_at(r, n - 1);
+        return;
+    }
+    pred = reg_at(r, at == 0 ? n - 1 : at - 1);
+    succ = reg_at(r, at == n - 1 ? 0 : at + 1);
+}
+__device__ inline bool cell_has(const cgpu_vertex_batch &b, const CellRegs &r, int x) {
+    if (r.n > CL) {
+        const int Nc = b.num_cells;
+        for (int k = 0; k < r.n; ++k)
+            if (b.cv[k * Nc + r.c] == x) return true;
+        return false;
+    }
+    return reg_find(r, x) >= 0;
 }
 __device__ inline void edge_cells(const cgpu_vertex_batch &b, int v1, int v2, int C[4]) {
     C[0] = b.vcn[3 * v1];
@@ -175,32 +229,84 @@ __device__ inline void edge_cells(const cgpu_vertex_batch &b, int v1, int v2, in
         if (c != C[0] && c != C[1] && c != C[2]) C[3] = c;
     }
 }
+// cell list without x (reference compaction), written back
+__device__ inline void remove_vertex(const cgpu_vertex_batch &b, const CellRegs &r, int x) {
+    const int Nc = b.num_cells;
+    const int c = r.c, n = r.n;
+    if (n > CL) {
+        int w = 0;
+        for (int k = 0; k < n; ++k) {
+            const int val = b.cv[k * Nc + c];
+            if (val != x) b.cv[(w++) * Nc + c] = val;
+        }
+    } else {
+        int w = 0;
+#pragma unroll
+        for (int k = 0; k < CL; ++k)
+            if (k < n && r.v[k] != x) b.cv[(w++) * Nc + c] = r.v[k];
+    }
+    b.cvn[c] = n - 1;
+}
+// cell list with x inserted before the last occurrence of `before` (reference
+// loop: at = last k with v[k] == before, 0 if absent), written back
+__device__ inline void insert_vertex(const cgpu_vertex_batch &b, const CellRegs &r, int x, int before) {
+    const int Nc = b.num_cells;
+    const int c = r.c, n = r.n;
+    if (n > CL) {
+        int at = 0;
+        for (int k = 0; k < n; ++k)
+            if (b.cv[k * Nc + c] == before) at = k;
+        for (int k = n; k > at; --k) b.cv[k * Nc + c] = b.cv[(k - 1) * Nc + c];
+        b.cv[at * Nc + c] = x;
+    } else {
+        int at = 0;
+#pragma unroll
+        for (int k = 0; k < CL; ++k)
+            if (k < n && r.v[k] == before) at = k;
+#pragma unroll
+        for (int k = 0; k <= CL; ++k) {
+            if (k > n) break;
+            const int val = (k < at) ? reg_at(r, k) : (k == at ? x : reg_at(r, k - 1));
+            b.cv[k * Nc + c] = val;
+        }
+    }
+    b.cvn[c] = n + 1;
+}
 
 // the T1 of one edge (reference vm_flip_edges_kernel, vertexModelBase.cu:252-544);
 // returns the four vertices whose slots need re-ordering, or false if forbidden
 __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm_params &sp, int v1, int v2,
                           const int C[4], int touched[4]) {
-    const int Nc = b.num_cells;
-    int cs[4] = {-1, -1, -1, C[3]};  // i, j, k, l
-    for (int q = 0; q < 3; ++q) {
+    if (C[0] < 0 || C[1] < 0 || C[2] < 0 || C[3] < 0) return false;
+    CellRegs R0, R1, R2, R3;
+    load_cell(b, C[0], R0);
+    load_cell(b, C[1], R1);
+    load_cell(b, C[2], R2);
+    load_cell(b, C[3], R3);
+    // i has v1, v2 with v2 before v1; k has v1 before v2; j has v1 only; l = C[3]
+    int si = -1, sj = -1, sk = -1;
+    {
         int pr, su;
-        locate(b, C[q], v1, pr, su);
-        if (pr == v2)
-            cs[0] = C[q];
-        else if (su == v2)
-            cs[2] = C[q];
-        else
-            cs[1] = C[q];
+        locate(b, R0, v1, pr, su);
+        if (pr == v2) si = 0; else if (su == v2) sk = 0; else sj = 0;
+        locate(b, R1, v1, pr, su);
+        if (pr == v2) si = 1; else if (su == v2) sk = 1; else sj = 1;
+        locate(b, R2, v1, pr, su);
+        if (pr == v2) si = 2; else if (su == v2) sk = 2; else sj = 2;
     }
-    if (cs[0] < 0 || cs[1] < 0 || cs[2] < 0 || cs[3] < 0) return false;
-    if (b.cvn[cs[0]] == 3 || b.cvn[cs[2]] == 3) return false;  // would shrink a triangle
-    if (b.cvn[cs[1]] >= VCAP || b.cvn[cs[3]] >= VCAP) {
+    if (si < 0 || sj < 0 || sk < 0) return false;
+    const CellRegs &Ri = si == 0 ? R0 : (si == 1 ? R1 : R2);
+    const CellRegs &Rj = sj == 0 ? R0 : (sj == 1 ? R1 : R2);
+    const CellRegs &Rk = sk == 0 ? R0 : (sk == 1 ? R1 : R2);
+    const CellRegs &Rl = R3;
+    if (Ri.n == 3 || Rk.n == 3) return false;  // would shrink a triangle
+    if (Rj.n >= VCAP || Rl.n >= VCAP) {
         b.st[b.vert_sys[v1]].error = CGPU_ERR_RING_CAPACITY;
         return false;
     }
     int dummy, a, bb;
-    locate(b, cs[1], v1, dummy, a);
-    locate(b, cs[3], v2, dummy, bb);
+    locate(b, Rj, v1, dummy, a);
+    locate(b, Rl, v2, dummy, bb);
     const Box box = make_box(sp.L);
     dvec2 p1 = ldp(pos, v1), p2 = ldp(pos, v2);
     dvec2 e = min_dist(box, p1, p2);
@@ -211,35 +317,20 @@ __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm
     put_in_box(box, p2);
     stp(pos, v1, p1);
     stp(pos, v2, p2);
+    const int ci = Ri.c, cj = Rj.c, ck = Rk.c, cl = Rl.c;
     for (int q = 0; q < 3; ++q) {
-        if (b.vcn[3 * v1 + q] == cs[2]) b.vcn[3 * v1 + q] = cs[3];
-        if (b.vcn[3 * v2 + q] == cs[0]) b.vcn[3 * v2 + q] = cs[1];
+        if (b.vcn[3 * v1 + q] == ck) b.vcn[3 * v1 + q] = cl;
+        if (b.vcn[3 * v2 + q] == ci) b.vcn[3 * v2 + q] = cj;
         if (b.vn[3 * v1 + q] == a) b.vn[3 * v1 + q] = bb;
         if (b.vn[3 * v2 + q] == bb) b.vn[3 * v2 + q] = a;
         if (b.vn[3 * a + q] == v1) b.vn[3 * a + q] = v2;
         if (b.vn[3 * bb + q] == v2) b.vn[3 * bb + q] = v1;
     }
-    // cell i loses v2, cell k loses v1
-    for (int t = 0; t < 2; ++t) {
-        int c = t == 0 ? cs[0] : cs[2], x = t == 0 ? v2 : v1;
-        int n = b.cvn[c], w = 0;
-        for (int k = 0; k < n; ++k) {
-            int val = b.cv[k * Nc + c];
-            if (val != x) b.cv[(w++) * Nc + c] = val;
-        }
-        b.cvn[c] = n - 1;
-    }
-    // cell j gains v2 before a, cell l gains v1 before b
-    for (int t = 0; t < 2; ++t) {
-        int c = t == 0 ? cs[1] : cs[3], x = t == 0 ? v2 : v1, before = t == 0 ? a : bb;
-        int n = b.cvn[c];
-        int at = 0;
-        for (int k = 0; k < n; ++k)
-            if (b.cv[k * Nc + c] == before) at = k;
-        for (int k = n; k > at; --k) b.cv[k * Nc + c] = b.cv[(k - 1) * Nc + c];
-        b.cv[at * Nc + c] = x;
-        b.cvn[c] = n + 1;
-    }
+    // cell i loses v2, cell k loses v1; cell j gains v2 before a, cell l gains v1 before b
+    remove_vertex(b, Ri, v2);
+    remove_vertex(b, Rk, v1);
+    insert_vertex(b, Rj, v2, a);
+    insert_vertex(b, Rl, v1, bb);
     touched[0] = v1;
     touched[1] = v2;
     touched[2] = a;
@@ -251,18 +342,24 @@ __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm
 // counter-clockwise (the next cell shares the edge to w's predecessor in the
 // current one) and each neighbour across from the cell that does not contain it
 __device__ void normalize_vertex(const cgpu_vertex_batch &b, int w) {
-    int c[3] = {b.vcn[3 * w], b.vcn[3 * w + 1], b.vcn[3 * w + 2]};
-    int n[3] = {b.vn[3 * w], b.vn[3 * w + 1], b.vn[3 * w + 2]};
+    const int c[3] = {b.vcn[3 * w], b.vcn[3 * w + 1], b.vcn[3 * w + 2]};
+    const int n[3] = {b.vn[3 * w], b.vn[3 * w + 1], b.vn[3 * w + 2]};
+    CellRegs R0, R1, R2;
+    load_cell(b, c[0], R0);
+    load_cell(b, c[1], R1);
+    load_cell(b, c[2], R2);
     int pr, su;
-    locate(b, c[0], w, pr, su);
-    int c1 = cell_has(b, c[1], pr) ? c[1] : c[2];
-    int c2 = (c1 == c[1]) ? c[2] : c[1];
-    int cc[3] = {c[0], c1, c2};
+    locate(b, R0, w, pr, su);
+    const bool first = cell_has(b, R1, pr);
+    const CellRegs &Rc1 = first ? R1 : R2;
+    const CellRegs &Rc2 = first ? R2 : R1;
+    const int cc[3] = {c[0], Rc1.c, Rc2.c};
     for (int f = 0; f < 3; ++f) {
+        const CellRegs &Rf = f == 0 ? R0 : (f == 1 ? Rc1 : Rc2);
         b.vcn[3 * w + f] = cc[f];
         int opp = n[0];
         for (int q = 0; q < 3; ++q)
-            if (!cell_has(b, cc[f], n[q])) opp = n[q];
+            if (!cell_has(b, Rf, n[q])) opp = n[q];
         b.vn[3 * w + f] = opp;
     }
 }
@@ -319,8 +416,11 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
             for (int t = 0; t < 4; ++t)
                 if (cells[4 * q + t] >= 0) b.cell_min[cells[4 * q + t]] = NONE;
-            if (list[2 * q] == -1 && touched[4 * q] >= 0)
-                for (int t = 0; t < 4; ++t) normalize_vertex(b, touched[4 * q + t]);
+        }
+        // the four touched vertices of every flip, one thread each
+        for (int e = threadIdx.x; e < 4 * nq; e += blockDim.x) {
+            const int q = e >> 2;
+            if (list[2 * q] == -1 && touched[4 * q] >= 0) normalize_vertex(b, touched[4 * q + (e & 3)]);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
