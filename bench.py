@@ -178,7 +178,7 @@ def other_workload(cg, name, steps, warmup):
         its = st["iterations"] - it0
         return {"metric": "cell-iterations/s, FIRE on one N=262144 system (FP64)", "value": round(N * its / el, 1),
                 "unit": "cell-iterations/s", "iterations": its, "ms_per_iteration": round(1e3 * el / max(its, 1), 5),
-                "dtype": "f64", "rms_force": st["force_max"],
+                "dtype": "f64", "rms_force": st["force_max"] ** 0.5,
                 "config": {"workload": "FIRE N=262144 L=512 P0=3.75 dt0=0.01 dtMax=0.1 alpha0=0.1 cutoff 1e-10"}}
     if name == "vertex":
         # vertex model: Nc = 32 768 (Nv = 65 536), Brownian T = 1e-3, dt = 0.005, T1 threshold 0.04, P0 = 3.9
@@ -196,6 +196,49 @@ def other_workload(cg, name, steps, warmup):
                 "unit": "cell-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * el / steps, 5),
                 "dtype": "f32", "t1_events": b.counters(0)["t1_events"],
                 "config": {"workload": "vertex Nc=32768 Nv=65536 T=1e-3 dt=0.005 thr=0.04 P0=3.9"}}
+    if name == "vertex_spp":
+        # self-propelled cell-vertex dynamics (the reference Vertex.cpp active path):
+        # Nc = 32 768, v0 = 0.05, Dr = 1, dt = 0.01, T1 threshold 0.04, P0 = 3.9
+        Nc = 32768
+        b = cg.VertexBatch([Nc], seeds=[21], precision="fp32")
+        b.set_params(dt=0.01, t1_threshold=0.04)
+        b.set_prefs(1.0, 3.9)
+        b.set_motility(0.05, 1.0)
+        b.spp_steps(warmup)
+        b.check_status()
+        t0 = time.perf_counter()
+        b.spp_steps(steps)
+        b.check_status()
+        el = time.perf_counter() - t0
+        return {"metric": "cell-steps/s, vertex model Nc=32768 self-propelled cell-vertex dynamics with T1",
+                "value": round(Nc * steps / el, 1), "unit": "cell-steps/s", "steps": steps, "warmup": warmup,
+                "ms_per_step": round(1e3 * el / steps, 5), "dtype": "f32", "t1_events": b.counters(0)["t1_events"],
+                "config": {"workload": "vertex SPP Nc=32768 Nv=65536 v0=0.05 Dr=1 dt=0.01 thr=0.04 P0=3.9"}}
+    if name == "vertex_fire":
+        # FIRE on the vertices of the same tissue (FP64), a bounded number of iterations
+        Nc = 32768
+        b = cg.VertexBatch([Nc], seeds=[21], precision="fp64")
+        b.set_params(dt=0.01, t1_threshold=0.04)
+        b.set_prefs(1.0, 3.9)
+        f = cg.EnergyMinimizerFIRE(b)
+        f.setDeltaT(0.01)
+        f.setDeltaTMax(0.1)
+        f.setAlphaStart(0.1)
+        f.setForceCutoff(1e-8)
+        f.setMaximumIterations(warmup)
+        f.minimize()
+        it0 = f.state()["iterations"]
+        f.setMaximumIterations(warmup + steps)
+        t0 = time.perf_counter()
+        f.minimize()
+        b.check_status()
+        el = time.perf_counter() - t0
+        st = f.state()
+        its = st["iterations"] - it0
+        return {"metric": "cell-iterations/s, FIRE on a vertex model Nc=32768 (FP64)", "value": round(Nc * its / el, 1),
+                "unit": "cell-iterations/s", "iterations": its, "ms_per_iteration": round(1e3 * el / max(its, 1), 5),
+                "dtype": "f64", "rms_force": st["force_max"] ** 0.5,
+                "config": {"workload": "vertex FIRE Nc=32768 Nv=65536 P0=3.9 dt0=0.01 dtMax=0.1 alpha0=0.1 cutoff 1e-8"}}
     raise SystemExit("unknown workload " + name)
 
 
@@ -221,7 +264,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "single", "fire", "vertex"],
+    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "single", "fire", "vertex", "vertex_spp", "vertex_fire"],
                     help="ensemble = the headline; the others print their own line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")

@@ -14,7 +14,7 @@ from ._lib import CellGPUError, check, lib, require_device, VoronoiBatchView  # 
 
 __all__ = [
     "CellGPUError", "VoronoiBatch", "VertexBatch", "VertexQuadraticEnergy", "VoronoiQuadraticEnergy", "Simulation", "selfPropelledParticleDynamics",
-    "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
+    "selfPropelledCellVertexDynamics", "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
 ]
 
 RING_CAP = 24
@@ -276,14 +276,37 @@ class brownianParticleDynamics(_Updater):
     def setT(self, T):
         self.T = T
 
-    def set2DModel(self, m):
-        self.model = m
-        m.batch.set_params(KA=m.KA, KP=m.KP, dt=m.deltaT, mu=self.mu, T=self.T)
 
     def performUpdate(self):
         m = self.model
-        m.batch.set_params(KA=m.KA, KP=m.KP, dt=m.deltaT, mu=self.mu, T=self.T)
+        if isinstance(m, VertexQuadraticEnergy):
+            m.push_params(mu=self.mu, T=self.T)
+        else:
+            m.batch.set_params(KA=m.KA, KP=m.KP, dt=m.deltaT, mu=self.mu, T=self.T)
         m.batch.brownian_steps(1)
+
+
+class selfPropelledCellVertexDynamics(_Updater):
+    """reference inc/updaters/selfPropelledCellVertexDynamics.h (GPU branch,
+    selfPropelledCellVertexDynamics.cpp:51-76): vertices move by dt (mean of the
+    three cells' v0 e(theta) + mu F), directors diffuse with Dr, then T1 flips."""
+
+    def __init__(self, Ncells=None, Nvertices=None, mu=1.0):
+        super().__init__(Ncells)
+        self.mu = mu
+
+    def set2DModel(self, m):
+        if not isinstance(m, VertexQuadraticEnergy):
+            raise CellGPUError("selfPropelledCellVertexDynamics drives a vertex model")
+        self.model = m
+
+    def performUpdate(self):
+        self.model.push_params(mu=self.mu)
+        self.model.batch.spp_steps(1, use_graph=False)
+
+    def advance(self, steps):
+        self.model.push_params(mu=self.mu)
+        self.model.batch.spp_steps(steps, use_graph=True)
 
 
 class Simulation:
@@ -325,7 +348,8 @@ class Simulation:
 
     def performTimesteps(self, k):
         """k steps at once (graph-captured); same result as k performTimestep calls."""
-        if len(self.updaters) == 1 and isinstance(self.updaters[0], selfPropelledParticleDynamics):
+        if len(self.updaters) == 1 and isinstance(self.updaters[0],
+                                                  (selfPropelledParticleDynamics, selfPropelledCellVertexDynamics)):
             self.updaters[0].advance(k)
             self.integerTimestep += k
             self.Time += k * self.integrationTimestep
@@ -341,7 +365,10 @@ class EnergyMinimizerFIRE:
         self.model = model
         batch = model.batch if hasattr(model, "batch") else model
         self.batch = batch
-        self.h = lib().cgpu_fire_create(batch.h)
+        create = lib().cgpu_vm_fire_create if isinstance(batch, VertexBatch) else lib().cgpu_fire_create
+        if isinstance(model, VertexQuadraticEnergy):
+            model.push_params()
+        self.h = create(batch.h)
         if not self.h:
             raise CellGPUError("cgpu_fire_create failed: " + lib().cgpu_last_error().decode())
         self.params = dict(max_it=1000, force_cutoff=1e-7, alpha_start=0.99, dt=0.01, dt_max=0.1, dt_inc=1.05,
@@ -405,7 +432,7 @@ class VertexBatch:
 
     VCAP = 32
 
-    def __init__(self, ncells, L=None, seeds=None, x=None, y=None, precision="fp32", device=0):
+    def __init__(self, ncells, L=None, seeds=None, x=None, y=None, precision="fp32", device=0, spv_initialize=False):
         require_device()
         nc = np.atleast_1d(np.asarray(ncells, dtype=np.int32))
         self.S = len(nc)
@@ -417,7 +444,7 @@ class VertexBatch:
         ys = None if y is None else _f64(y)
         self._keep = (nc, Ls, sd, xs, ys)
         self.h = lib().cgpu_vm_create(self.S, _p(nc), _p(Ls), _p(sd), _p(xs), _p(ys), 1 if precision == "fp32" else 0,
-                                      device)
+                                      device, 1 if spv_initialize else 0)
         if not self.h:
             raise CellGPUError("cgpu_vm_create failed")
         self.Nv = lib().cgpu_vm_num_vertices(self.h)
@@ -448,11 +475,32 @@ class VertexBatch:
         buf = np.concatenate([_f64(etax, self.Nv), _f64(etay, self.Nv)])
         check(lib().cgpu_vm_upload_noise(self.h, _p(buf)), "vm_upload_noise")
 
+    def upload_cell_noise(self, eta):
+        """N(0,1) draws of the director rotation, one per cell (noise-buffer mode)."""
+        check(lib().cgpu_vm_upload_cell_noise(self.h, _p(_f64(eta, self.Nc))), "vm_upload_cell_noise")
+
+    def set_motility(self, v0, Dr):
+        a, b = _f64(v0, self.Nc), _f64(Dr, self.Nc)
+        check(lib().cgpu_vm_set_motility(self.h, _p(a), _p(b)), "vm_set_motility")
+
+    def set_directors(self, theta):
+        t = _f64(theta, self.Nc)
+        check(lib().cgpu_vm_set_directors(self.h, _p(t)), "vm_set_directors")
+
+    def directors(self):
+        out = np.zeros(self.Nc)
+        check(lib().cgpu_vm_get_directors(self.h, _p(out)), "vm_get_directors")
+        return out
+
     def compute_forces(self):
         check(lib().cgpu_vm_compute_forces(self.h), "vm_compute_forces")
 
     def brownian_steps(self, steps, use_graph=True):
         check(lib().cgpu_vm_brownian_steps(self.h, int(steps), 1 if use_graph else 0), "vm_brownian_steps")
+
+    def spp_steps(self, steps, use_graph=True):
+        """Self-propelled cell-vertex steps (reference selfPropelledCellVertexDynamics)."""
+        check(lib().cgpu_vm_spp_steps(self.h, int(steps), 1 if use_graph else 0), "vm_spp_steps")
 
     def check_status(self):
         check(lib().cgpu_vm_status(self.h), "vertex batch status")
@@ -478,31 +526,67 @@ class VertexBatch:
 
 
 class VertexQuadraticEnergy:
-    """reference inc/models/vertexQuadraticEnergy.h:16-36 (GPU branch), one system."""
+    """reference inc/models/vertexQuadraticEnergy.h:16-36 with vertexModelBase
+    (vertexModelBase.h:26-109) and Simple2DActiveCell's motility (GPU branch),
+    one system."""
 
-    def __init__(self, n, A0=1.0, P0=3.9, reprod=True, precision="fp64", seed=13377, device=0):
+    def __init__(self, n, A0=1.0, P0=3.9, reprod=True, runSPV=False, precision="fp64", seed=13377, device=0):
         self.Ncells = int(n)
-        self.batch = VertexBatch([n], seeds=[seed], precision=precision, device=device)
+        self.batch = VertexBatch([n], seeds=[seed], precision=precision, device=device, spv_initialize=runSPV)
         self.Nvertices = self.batch.Nv
         self.KA = self.KP = 1.0
         self.deltaT = 0.005
         self.T1Threshold = 0.01  # reference initializeVertexModelBase default
         self.setCellPreferencesUniform(A0, P0)
 
+    def getNumberOfDegreesOfFreedom(self):
+        return self.Nvertices
+
     def setCellPreferencesUniform(self, A0, P0):
+        self.A0, self.P0 = float(A0), float(P0)
         self.batch.set_prefs(np.full(self.Ncells, A0), np.full(self.Ncells, P0))
+
+    def setModuliUniform(self, KA, KP):
+        self.KA, self.KP = float(KA), float(KP)
 
     def setT1Threshold(self, t):
         self.T1Threshold = t
 
+    def setDeltaT(self, dt):
+        self.deltaT = dt
+
+    def setv0Dr(self, v0, Dr):
+        """reference Simple2DActiveCell::setv0Dr (Simple2DActiveCell.cpp:76-92)."""
+        self.batch.set_motility(np.full(self.Ncells, v0), np.full(self.Ncells, Dr))
+
+    def setCellDirectors(self, theta):
+        self.batch.set_directors(theta)
+
+    def returnCellDirectors(self):
+        return self.batch.directors()
+
+    def push_params(self, mu=1.0, T=0.0):
+        """Hands the model's constants and an updater's (mu, T) to the device."""
+        self.batch.set_params(KA=self.KA, KP=self.KP, dt=self.deltaT, mu=mu, T=T, t1_threshold=self.T1Threshold)
+
     def computeForces(self):
+        self.push_params()
         self.batch.compute_forces()
 
     def computeEnergy(self):
-        self.batch.compute_forces()
+        """reference VertexQuadraticEnergy::computeEnergy (vertexQuadraticEnergy.cpp:30-44)."""
+        self.computeForces()
         g = self.batch.get()
-        return float(np.sum(self.KA * (g["A"] - 1.0) ** 2 + self.KP * (g["P"] - 3.9) ** 2))
+        return float(np.sum(self.KA * (g["A"] - self.A0) ** 2 + self.KP * (g["P"] - self.P0) ** 2))
 
     def returnForces(self):
         g = self.batch.get()
         return np.stack([g["Fx"], g["Fy"]], 1)
+
+    def returnPositions(self):
+        g = self.batch.get()
+        return np.stack([g["vx"], g["vy"]], 1)
+
+    def returnAreaPeri(self):
+        g = self.batch.get()
+        return np.stack([g["A"], g["P"]], 1)

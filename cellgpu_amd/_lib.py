@@ -94,15 +94,13 @@ def lib():
         "cgpu_normal_pair": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp]),
         "cgpu_put_in_box": (None, [d, P(d), P(d)]),
         "cgpu_vor_spatial_sort": (i, [vp, vp]),
-    }
-    optional = {
         "cgpu_fire_create": (vp, [vp]),
         "cgpu_fire_destroy": (None, [vp]),
         "cgpu_fire_set": (i, [vp, i, d, d, d, d, d, d, d, i]),
         "cgpu_fire_minimize": (i, [vp]),
         "cgpu_fire_get": (i, [vp, i, vp]),
         "cgpu_vor_kernel_times": (i, [vp, i, vp, vp, i]),
-        "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, i, i]),
+        "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, i, i, i]),
         "cgpu_vm_destroy": (None, [vp]),
         "cgpu_vm_num_vertices": (i, [vp]),
         "cgpu_vm_set_params": (i, [vp, i, d, d, d, d, d, d]),
@@ -116,14 +114,15 @@ def lib():
         "cgpu_vm_get_topology": (i, [vp, vp, vp, vp, vp, i]),
         "cgpu_vm_get_counters": (i, [vp, i, vp]),
         "cgpu_vm_status": (i, [vp]),
+        "cgpu_vm_upload_cell_noise": (i, [vp, vp]),
+        "cgpu_vm_set_motility": (i, [vp, vp, vp]),
+        "cgpu_vm_set_directors": (i, [vp, vp]),
+        "cgpu_vm_get_directors": (i, [vp, vp]),
+        "cgpu_vm_spp_steps": (i, [vp, i, i]),
+        "cgpu_vm_fire_create": (vp, [vp]),
     }
-    for name, (res, args) in list(sig.items()) + list(optional.items()):
-        try:
-            f = getattr(L, name)
-        except AttributeError:
-            if name in optional:
-                continue
-            raise
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)  # a missing export is a build error: AttributeError
         f.restype = res
         f.argtypes = args
     _LIB = L

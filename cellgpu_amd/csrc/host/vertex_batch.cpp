@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../cgpu_device.h"
+#include "fire.hpp"
 #include "voronoi_batch.hpp"
 
 namespace cgpu_host {
@@ -16,7 +17,7 @@ namespace cgpu_host {
 class VertexBatch {
    public:
     VertexBatch(int S, const int *ncells, const double *L, const uint64_t *seed, const double *x, const double *y,
-                int precision, int device)
+                int precision, int device, bool spv_initialize)
         : S_(S), prec_(precision ? 1 : 0) {
         hip_check(hipSetDevice(device), "hipSetDevice");
         rsz_ = prec_ ? sizeof(float) : sizeof(double);
@@ -25,8 +26,21 @@ class VertexBatch {
         for (int s = 0; s < S; ++s) Ls[s] = L ? L[s] : std::sqrt((double)ncells[s]);
         VoronoiBatch vor(S, ncells, Ls.data(), seed, x, y, nullptr, 0, device);
         const int Nc = vor.num_cells();
-        std::vector<double> px(Nc), py(Nc);
-        vor.get(px.data(), py.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        // positions, then the directors drawn after them (reference
+        // initializeSimple2DActiveCell: positions, setCellDirectorsRandomly)
+        std::vector<double> px(Nc), py(Nc), pth(Nc);
+        vor.get(px.data(), py.data(), pth.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (spv_initialize) {
+            // reference setCellsVoronoiTesselation(true), vertexModelBase.cpp:118-140:
+            // 100 self-propelled Voronoi steps smooth the random point set
+            std::vector<double> a0(Nc, 1.0), p0(Nc, 3.8), v0(Nc, 0.1), dr(Nc, 1.0);
+            vor.set_params(-1, 1.0, 1.0, 0.1, 1.0, 0.0);
+            vor.set_prefs(a0.data(), p0.data());
+            vor.set_motility(v0.data(), dr.data());
+            if (vor.spv_steps(100, true) != CGPU_OK || vor.status() != CGPU_OK)
+                throw std::runtime_error("spv initialisation failed");
+            vor.get(px.data(), py.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        }
         const int cap = CGPU_RING_CAP;
         std::vector<int> nn(Nc), nbr((size_t)Nc * cap);
         vor.get_topology(nn.data(), nbr.data(), cap, nullptr);
@@ -65,6 +79,18 @@ class VertexBatch {
         }
         Nv_ = voff;
         Nc_ = coff;
+        // vertex chunks for the per-system reductions
+        std::vector<int> ch_sys, ch_begin, ch_end;
+        for (int s = 0; s < S; ++s) {
+            cgpu_vm_params &p = hp_[s];
+            p.chunk0 = (int)ch_sys.size();
+            for (int b = p.voff; b < p.voff + p.nv; b += CGPU_CHUNK) {
+                ch_sys.push_back(s);
+                ch_begin.push_back(b);
+                ch_end.push_back(std::min(b + CGPU_CHUNK, p.voff + p.nv));
+            }
+        }
+        num_chunks_ = (int)ch_sys.size();
         auto find_tri = [&](int a, int b, int c) {  // counter-clockwise (a, b, c)
             int o = std::min(a, std::min(b, c)), p, q;
             if (o == a) { p = b; q = c; } else if (o == b) { p = c; q = a; } else { p = a; q = b; }
@@ -146,6 +172,17 @@ class VertexBatch {
         cell_min_.upload(ones.data(), Nc_);
         std::vector<double> a0(Nc_, 1.0), p0(Nc_, 3.9);
         set_prefs(a0.data(), p0.data());
+        theta_.alloc(Nc_ * rsz_);
+        real_upload(theta_.p, pth.data(), Nc_);
+        mot_.alloc(2 * Nc_ * rsz_);
+        std::vector<double> v0(Nc_, 0.0), Dr(Nc_, 1.0);  // reference setv0Dr(0.0, 1.0)
+        set_motility(v0.data(), Dr.data());
+        chunk_sys_.alloc(num_chunks_);
+        chunk_sys_.upload(ch_sys.data(), num_chunks_);
+        chunk_begin_.alloc(num_chunks_);
+        chunk_begin_.upload(ch_begin.data(), num_chunks_);
+        chunk_end_.alloc(num_chunks_);
+        chunk_end_.upload(ch_end.data(), num_chunks_);
     }
     ~VertexBatch() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -153,6 +190,10 @@ class VertexBatch {
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     int num_vertices() const { return Nv_; }
+    int num_systems() const { return S_; }
+    size_t real_size() const { return rsz_; }
+    hipStream_t stream() const { return stream_; }
+    void swap_parity() { parity_ ^= 1; }
     cgpu_vertex_batch view() const {
         cgpu_vertex_batch b;
         std::memset(&b, 0, sizeof(b));
@@ -180,9 +221,16 @@ class VertexBatch {
         b.flag_touched = flag_touched_.p;
         b.cell_min = cell_min_.p;
         b.noise = noise_buffer_ ? noise_.p : nullptr;
+        b.theta = theta_.p;
+        b.motility = mot_.p;
+        b.chunk_sys = chunk_sys_.p;
+        b.chunk_begin = chunk_begin_.p;
+        b.chunk_end = chunk_end_.p;
+        b.num_chunks = num_chunks_;
         return b;
     }
     void set_params(int s, double KA, double KP, double dt, double mu, double T, double thr) {
+        sync();
         int lo = s < 0 ? 0 : s, hi = s < 0 ? S_ : s + 1;
         for (int q = lo; q < hi; ++q) {
             hp_[q].KA = KA;
@@ -196,6 +244,7 @@ class VertexBatch {
         drop_graph();
     }
     void set_prefs(const double *A0, const double *P0) {
+        sync();
         std::vector<double> v(2 * (size_t)Nc_);
         for (int i = 0; i < Nc_; ++i) {
             v[2 * i] = A0[i];
@@ -204,6 +253,7 @@ class VertexBatch {
         real_upload(APpref_.p, v.data(), v.size());
     }
     void set_positions(const double *vx, const double *vy) {
+        sync();
         std::vector<double> xy(2 * (size_t)Nv_);
         for (int i = 0; i < Nv_; ++i) {
             if (!std::isfinite(vx[i]) || !std::isfinite(vy[i])) throw std::invalid_argument("non-finite position");
@@ -212,14 +262,38 @@ class VertexBatch {
         }
         (parity_ ? pos1_ : pos0_).upload(xy.data(), xy.size());
     }
+    // noise buffer: x then y per vertex, then one per cell (director rotation)
     void set_noise_mode(bool on) {
         noise_buffer_ = on;
-        if (on && !noise_.p) noise_.alloc(2 * (size_t)Nv_);
+        if (on && !noise_.p) noise_.alloc(2 * (size_t)Nv_ + Nc_);
         drop_graph();
     }
     void upload_noise(const double *eta) {
-        if (!noise_.p) noise_.alloc(2 * (size_t)Nv_);
+        sync();
+        if (!noise_.p) noise_.alloc(2 * (size_t)Nv_ + Nc_);
         noise_.upload(eta, 2 * (size_t)Nv_);
+    }
+    void upload_cell_noise(const double *eta) {
+        sync();
+        if (!noise_.p) noise_.alloc(2 * (size_t)Nv_ + Nc_);
+        noise_.upload(eta, Nc_, 2 * (size_t)Nv_);
+    }
+    void set_motility(const double *v0, const double *Dr) {
+        sync();
+        std::vector<double> m(2 * (size_t)Nc_);
+        for (int i = 0; i < Nc_; ++i) {
+            m[2 * i] = v0[i];
+            m[2 * i + 1] = Dr[i];
+        }
+        real_upload(mot_.p, m.data(), m.size());
+    }
+    void set_directors(const double *theta) {
+        sync();
+        real_upload(theta_.p, theta, Nc_);
+    }
+    void get_directors(double *theta) {
+        sync();
+        real_download(theta_.p, theta, Nc_);
     }
     int compute_forces() {
         cgpu_vertex_batch b = view();
@@ -228,27 +302,30 @@ class VertexBatch {
         sync();
         return r;
     }
-    int step_launch() {
+    // kind 0: Brownian step, 1: self-propelled cell-vertex step
+    int step_launch(int kind) {
         cgpu_vertex_batch b = view();
-        int r = cgpu_vm_brownian_step(&b, stream_);
+        int r = kind ? cgpu_vm_spp_step(&b, stream_) : cgpu_vm_brownian_step(&b, stream_);
         parity_ ^= 1;
         return r;
     }
-    int brownian_steps(int steps, bool use_graph) {
+    int run_steps(int kind, int steps, bool use_graph) {
         if (!use_graph || noise_buffer_) {
             for (int k = 0; k < steps; ++k)
-                if (int r = step_launch()) return r;
+                if (int r = step_launch(kind)) return r;
             return CGPU_OK;
         }
+        if (graph_exec_ && graph_kind_ != kind) drop_graph();
         if (graph_exec_ && graph_parity_ != parity_ && steps > 0) {
-            if (int r = step_launch()) return r;
+            if (int r = step_launch(kind)) return r;
             --steps;
         }
         if (!graph_exec_ && steps >= 2) {
             hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
             graph_parity_ = parity_;
-            step_launch();
-            step_launch();
+            graph_kind_ = kind;
+            step_launch(kind);
+            step_launch(kind);
             hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
             hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate");
         }
@@ -256,7 +333,7 @@ class VertexBatch {
             hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
             steps -= 2;
         }
-        if (steps == 1) return step_launch();
+        if (steps == 1) return step_launch(kind);
         return CGPU_OK;
     }
     void sync() { hip_check(hipStreamSynchronize(stream_), "sync"); }
@@ -341,7 +418,7 @@ class VertexBatch {
     }
     int S_, Nv_ = 0, Nc_ = 0, prec_;
     size_t rsz_;
-    int parity_ = 0, graph_parity_ = -1;
+    int parity_ = 0, graph_parity_ = -1, graph_kind_ = -1, num_chunks_ = 0;
     bool noise_buffer_ = false;
     hipStream_t stream_ = nullptr;
     hipGraph_t graph_ = nullptr;
@@ -350,9 +427,47 @@ class VertexBatch {
     DevBuf<cgpu_vm_params> sys_;
     DevBuf<cgpu_sys_state> st_;
     DevBuf<int> vsys_, csys_, vn_, vcn_, cvn_, cv_, flag_count_, flag_list_, flag_cells_, flag_touched_;
+    DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_;
     DevBuf<unsigned long long> cell_min_;
     DevBuf<double> pos0_, pos1_, noise_;
-    DevBuf<unsigned char> AP_, dAP_, APpref_, force_;
+    DevBuf<unsigned char> AP_, dAP_, APpref_, force_, theta_, mot_;
+};
+
+// FIRE on a vertex batch: the vertices are the degrees of freedom and the T1
+// pass is enforceTopology (vertexModelBase.cpp:93-106)
+class VertexFire : public FireBase {
+   public:
+    explicit VertexFire(VertexBatch *vm)
+        : FireBase(vm->num_systems(), vm->num_vertices(), vm->real_size(), vm->view().num_chunks), vm_(vm) {}
+
+   protected:
+    hipStream_t stream() override { return vm_->stream(); }
+    cgpu_sys_state *device_state() override { return vm_->view().st; }
+    int forces() override {
+        cgpu_vertex_batch b = vm_->view();
+        int r = cgpu_vm_geometry(&b, stream());
+        return r ? r : cgpu_vm_forces(&b, stream());
+    }
+    int iterate() override {
+        hipStream_t st = stream();
+        cgpu_vertex_batch b = vm_->view();
+        int r;
+        if ((r = cgpu_vm_fire_vv_move(&b, fs_.p, vel_.p, st))) return r;
+        vm_->swap_parity();
+        cgpu_vertex_batch bn = vm_->view();
+        if ((r = cgpu_vm_t1(&bn, st))) return r;
+        if ((r = cgpu_vm_fire_force_reduce(&bn, fs_.p, vel_.p, dots_.p, st))) return r;
+        return cgpu_vm_fire_park_converged(&bn, fs_.p, st);
+    }
+    int apply_mix() override {
+        cgpu_vertex_batch b = vm_->view();
+        return cgpu_vm_fire_apply_mix(&b, fs_.p, vel_.p, stream());
+    }
+    void sync() override { vm_->sync(); }
+    int status() override { return vm_->status(); }
+
+   private:
+    VertexBatch *vm_;
 };
 
 }  // namespace cgpu_host
@@ -371,9 +486,9 @@ template <typename F> int vguard(F &&f) {
 
 extern "C" {
 void *cgpu_vm_create(int S, const int *ncells, const double *L, const uint64_t *seed, const double *x, const double *y,
-                     int precision, int device) {
+                     int precision, int device, int spv_initialize) {
     try {
-        return new VertexBatch(S, ncells, L, seed, x, y, precision, device);
+        return new VertexBatch(S, ncells, L, seed, x, y, precision, device, spv_initialize != 0);
     } catch (...) {
         return nullptr;
     }
@@ -396,8 +511,42 @@ int cgpu_vm_upload_noise(void *h, const double *eta) {
     return vguard([&] { ((VertexBatch *)h)->upload_noise(eta); return 0; });
 }
 int cgpu_vm_compute_forces(void *h) { return vguard([&] { return ((VertexBatch *)h)->compute_forces(); }); }
+int cgpu_vm_upload_cell_noise(void *h, const double *eta) {
+    return vguard([&] {
+        ((VertexBatch *)h)->upload_cell_noise(eta);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_set_motility(void *h, const double *v0, const double *Dr) {
+    return vguard([&] {
+        ((VertexBatch *)h)->set_motility(v0, Dr);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_set_directors(void *h, const double *theta) {
+    return vguard([&] {
+        ((VertexBatch *)h)->set_directors(theta);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_get_directors(void *h, double *theta) {
+    return vguard([&] {
+        ((VertexBatch *)h)->get_directors(theta);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_spp_steps(void *h, int steps, int use_graph) {
+    return vguard([&] { return ((VertexBatch *)h)->run_steps(1, steps, use_graph != 0); });
+}
+void *cgpu_vm_fire_create(void *vm) {
+    try {
+        return static_cast<cgpu_host::FireBase *>(new cgpu_host::VertexFire((VertexBatch *)vm));
+    } catch (...) {
+        return nullptr;
+    }
+}
 int cgpu_vm_brownian_steps(void *h, int steps, int use_graph) {
-    return vguard([&] { return ((VertexBatch *)h)->brownian_steps(steps, use_graph != 0); });
+    return vguard([&] { return ((VertexBatch *)h)->run_steps(0, steps, use_graph != 0); });
 }
 int cgpu_vm_get(void *h, double *vx, double *vy, double *Fx, double *Fy, double *A, double *P) {
     return vguard([&] { ((VertexBatch *)h)->get(vx, vy, Fx, Fy, A, P); return 0; });

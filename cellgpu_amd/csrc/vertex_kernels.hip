@@ -7,7 +7,8 @@
 // no search of the cell's vertex list. Cell vertex lists are counter-clockwise,
 // slot-major (cv[k*Nc + c]).
 //
-// Step: geometry (per cell) -> force + Brownian update + move (per vertex) ->
+// Step: geometry (per cell) -> force + equation of motion (Brownian, or
+// self-propelled cell-vertex with the director rotation in the next pass) + move (per vertex) ->
 // FP64 T1 edge test (per vertex) -> T1 flips in deterministic rounds (one
 // work-group per system: an edge (v1, v2) flips when its key is the smallest
 // among the flagged edges sharing one of its four cells).
@@ -17,6 +18,7 @@
 
 #include "../../include/cellgpu_amd.h"
 #include "cgpu_device.h"
+#include "fire_device.h"
 
 using namespace cgpu;
 
@@ -66,22 +68,11 @@ __global__ void k_vm_geometry(cgpu_vertex_batch b, const double *pos) {
     str<R>(b.dAP, c, vec2<R>{(R)sp.KA * (A - pref.x), (R)sp.KP * (P - pref.y)});
 }
 
-// force on vertex v: sum over its three cells of the reference force set
-// (functions.h:330-354, vertexQuadraticEnergy.cu:21-70), then the Brownian
-// update (brownianParticleDynamics.cu:20-39) and the move
+// force on vertex v at p: sum over its three cells of the reference force set
+// (functions.h:330-354, vertexQuadraticEnergy.cu:21-70)
 template <typename R>
-__global__ void k_vm_force_eom_move(cgpu_vertex_batch b, const double *pos, double *pos_next, int move) {
-    int v = xcd_block() * blockDim.x + threadIdx.x;
-    if (v >= b.num_vertices) return;
-    const int s = b.vert_sys[v];
-    const cgpu_vm_params &sp = b.sys[s];
-    cgpu_sys_state &st = b.st[s];
-    const Box box = make_box(sp.L);
-    dvec2 p = ldp(pos, v);
-    if (move && !st.active) {
-        stp(pos_next, v, p);
-        return;
-    }
+__device__ __forceinline__ vec2<R> vm_force(const cgpu_vertex_batch &b, const double *pos, int v, const Box &box,
+                                            dvec2 p) {
     vec2<R> nb[3];
     for (int f = 0; f < 3; ++f) nb[f] = cv2<R>(min_dist(box, ldp(pos, b.vn[3 * v + f]), p));
     vec2<R> F{R(0), R(0)};
@@ -97,35 +88,106 @@ __global__ void k_vm_force_eom_move(cgpu_vertex_batch b, const double *pos, doub
         F.x += R(2.0) * (q.x * dAdv.x + q.y * dPdv.x);
         F.y += R(2.0) * (q.x * dAdv.y + q.y * dPdv.y);
     }
+    return F;
+}
+
+// moved position p + d, wrapped; a non-finite or runaway result flags the
+// system and keeps the old position
+__device__ inline dvec2 vm_moved(cgpu_sys_state &st, const cgpu_vm_params &sp, const Box &box, dvec2 p, double dx,
+                                 double dy) {
+    dvec2 q{p.x + dx, p.y + dy};
+    if (!(fabs(q.x) < 1e12 * sp.L && fabs(q.y) < 1e12 * sp.L)) {
+        st.error = CGPU_ERR_NONFINITE;
+        q = p;
+    }
+    put_in_box(box, q);
+    return q;
+}
+
+// Per vertex: force, then (move != 0) the equation of motion and the move.
+// KIND 0: Brownian, disp = sqrt(2 dt T mu) eta + dt mu F (brownianParticleDynamics.cu:20-39).
+// KIND 1: self-propelled cell-vertex, disp = dt (director + mu F) with the
+// director the mean of v0 e(theta) over the vertex's three cells
+// (selfPropelledCellVertexDynamics.cu:21-47).
+template <typename R, int KIND>
+__global__ void k_vm_force_eom_move(cgpu_vertex_batch b, const double *pos, double *pos_next, int move) {
+    int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= b.num_vertices) return;
+    const int s = b.vert_sys[v];
+    const cgpu_vm_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    const Box box = make_box(sp.L);
+    dvec2 p = ldp(pos, v);
+    if (move && !st.active) {
+        stp(pos_next, v, p);
+        return;
+    }
+    vec2<R> F = vm_force<R>(b, pos, v, box, p);
     str<R>(b.force, v, F);
     if (!move) return;
-    const int step = st.step + 1;
-    double z0, z1;
-    if (b.noise) {
-        z0 = b.noise[v];
-        z1 = b.noise[b.num_vertices + v];
-    } else {
-        normal_pair(sp.seed, (uint32_t)(v - sp.voff), (uint32_t)step, 1u, z0, z1);
-    }
     const R dt = (R)sp.dt, mu = (R)sp.mu;
-    R sc = sqrt(dt * (R)sp.T * mu * R(2.0));
-    R dx = (R)z0 * sc + dt * mu * F.x;
-    R dy = (R)z1 * sc + dt * mu * F.y;
-    p.x += (double)dx;
-    p.y += (double)dy;
-    if (!(fabs(p.x) < 1e12 * sp.L && fabs(p.y) < 1e12 * sp.L)) {
-        st.error = CGPU_ERR_NONFINITE;
-        p = ldp(pos, v);
+    R dx, dy;
+    if (KIND == 0) {
+        const int step = st.step + 1;
+        double z0, z1;
+        if (b.noise) {
+            z0 = b.noise[v];
+            z1 = b.noise[b.num_vertices + v];
+        } else {
+            normal_pair(sp.seed, (uint32_t)(v - sp.voff), (uint32_t)step, 1u, z0, z1);
+        }
+        R sc = sqrt(dt * (R)sp.T * mu * R(2.0));
+        dx = (R)z0 * sc + dt * mu * F.x;
+        dy = (R)z1 * sc + dt * mu * F.y;
+    } else {
+        const R *th = reinterpret_cast<const R *>(b.theta);
+        const R *mot = reinterpret_cast<const R *>(b.motility);
+        R dirx = R(0), diry = R(0);
+        for (int f = 0; f < 3; ++f) {
+            const int c = b.vcn[3 * v + f];
+            const R v0 = mot[2 * c], t = th[c];
+            dirx += v0 * cos(t);
+            diry += v0 * sin(t);
+        }
+        dirx /= R(3.0);
+        diry /= R(3.0);
+        dx = dt * (dirx + mu * F.x);
+        dy = dt * (diry + mu * F.y);
     }
-    put_in_box(box, p);
-    stp(pos_next, v, p);
+    stp(pos_next, v, vm_moved(st, sp, box, p, (double)dx, (double)dy));
+}
+
+// director rotation after the displacements (selfPropelledCellVertexDynamics.cu:52-66):
+// theta += eta sqrt(2 dt Dr), eta from the cell noise block or the counter-based
+// generator keyed by (cell, step, stream 2)
+template <typename R> __device__ inline void rotate_director(const cgpu_vertex_batch &b, int c) {
+    const int s = b.cell_sys[c];
+    const cgpu_sys_state &st = b.st[s];
+    if (!st.active) return;
+    const cgpu_vm_params &sp = b.sys[s];
+    double z0, z1;
+    if (b.noise)
+        z0 = b.noise[2 * (size_t)b.num_vertices + c];
+    else
+        normal_pair(sp.seed, (uint32_t)(c - sp.coff), (uint32_t)(st.step + 1), 2u, z0, z1);
+    R *th = reinterpret_cast<R *>(b.theta);
+    const R Dr = reinterpret_cast<const R *>(b.motility)[2 * c + 1];
+    th[c] += (R)z0 * sqrt(R(2.0) * (R)sp.dt * Dr);
 }
 
 // reference vm_simple_T1_test_kernel (vertexModelBase.cu:118-164): FP64 edge
 // length below the threshold, for v1 < v2; flagged edges go to the system's list
 // as (v1, v2) pairs
-__global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos) {
+// rotate != 0: thread c < Nc also rotates cell c's director (the T1 pass of a
+// self-propelled cell-vertex step follows its displacement kernel)
+__global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos, int rotate) {
     int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (rotate && v < b.num_cells) {
+        if (b.precision)
+            rotate_director<float>(b, v);
+        else
+            rotate_director<double>(b, v);
+    }
     if (v >= b.num_vertices) return;
     const int s = b.vert_sys[v];
     if (!b.st[s].active) return;
@@ -451,6 +513,91 @@ __global__ void k_vm_advance(cgpu_vertex_batch b) {
     if (st.steps_left > 0 && --st.steps_left == 0) st.active = 0;
 }
 
+// ------------------------------------------------------------------------ FIRE
+// FIRE with the vertices as degrees of freedom (shared pieces: fire_device.h)
+template <typename R> __global__ void k_vm_fire_vv_move(cgpu_vertex_batch b, const cgpu_fire_state *fs, void *vel) {
+    int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= b.num_vertices) return;
+    const int s = b.vert_sys[v];
+    const cgpu_vm_params &sp = b.sys[s];
+    const cgpu_fire_state &f = fs[s];
+    dvec2 p = ldp(b.pos, v);
+    vec2<R> u = ldr<R>(vel, v), F = ldr<R>(b.force, v);
+    fire_mix<R>(u, F, f);
+    if (f.converged || !b.st[s].active) {
+        str<R>(vel, v, u);
+        stp(b.pos_next, v, p);
+        return;
+    }
+    vec2<R> d = fire_vv_first_half<R>(u, F, (R)f.dt);
+    str<R>(vel, v, u);
+    stp(b.pos_next, v, vm_moved(b.st[s], sp, make_box(sp.L), p, (double)d.x, (double)d.y));
+}
+
+__global__ void k_vm_fire_clear_pending(cgpu_fire_state *fs, int S) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) fs[s].pending = 0;
+}
+
+// forces on the moved (and T1-flipped) vertices, second velocity half step,
+// per-vertex products of the reductions
+template <typename R>
+__global__ void k_vm_fire_force_vv2(cgpu_vertex_batch b, const cgpu_fire_state *fs, void *vel, double *dots) {
+    int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= b.num_vertices) return;
+    const int s = b.vert_sys[v];
+    const cgpu_fire_state &f = fs[s];
+    if (f.converged || !b.st[s].active) return;
+    const cgpu_vm_params &sp = b.sys[s];
+    vec2<R> F = vm_force<R>(b, b.pos, v, make_box(sp.L), ldp(b.pos, v));
+    str<R>(b.force, v, F);
+    vec2<R> u = ldr<R>(vel, v);
+    const R dt = (R)f.dt;
+    u.x += R(0.5) * dt * F.x;
+    u.y += R(0.5) * dt * F.y;
+    str<R>(vel, v, u);
+    fire_dots(dots, b.num_vertices, v, F.x, F.y, u.x, u.y);
+}
+
+__global__ void k_vm_fire_partial(cgpu_vertex_batch b, const cgpu_fire_state *fs, double *dots) {
+    const int c = blockIdx.x;
+    const int s = b.chunk_sys[c];
+    const int N = b.num_vertices;
+    fire_chunk_partial(dots, N, b.chunk_begin[c], b.chunk_end[c], fs[s].converged || !b.st[s].active,
+                       dots + 3 * (size_t)N + 4 * (size_t)c);
+}
+
+__global__ void k_vm_fire_reduce(cgpu_vertex_batch b, cgpu_fire_state *fs, const double *dots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    cgpu_fire_state &f = fs[s];
+    if (f.converged || !b.st[s].active) return;
+    const cgpu_vm_params &sp = b.sys[s];
+    const double *part = dots + 3 * (size_t)b.num_vertices;
+    double forceNorm = 0, power = 0, velNorm = 0, mf = 0;
+    for (int c = sp.chunk0; c < b.num_chunks && b.chunk_sys[c] == s; ++c) {
+        forceNorm += part[4 * c];
+        power += part[4 * c + 1];
+        velNorm += part[4 * c + 2];
+        mf = fmax(mf, part[4 * c + 3]);
+    }
+    fire_update(f, forceNorm, power, velNorm, mf, sp.nv);
+}
+
+__global__ void k_vm_fire_park(cgpu_vertex_batch b, const cgpu_fire_state *fs) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < b.num_systems && fs[s].converged) b.st[s].active = 0;
+}
+
+template <typename R> __global__ void k_vm_fire_apply_mix(cgpu_vertex_batch b, const cgpu_fire_state *fs, void *vel) {
+    int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= b.num_vertices) return;
+    const cgpu_fire_state &f = fs[b.vert_sys[v]];
+    vec2<R> u = ldr<R>(vel, v), F = ldr<R>(b.force, v);
+    fire_mix<R>(u, F, f);
+    str<R>(vel, v, u);
+}
+
 inline int nb(int n, int bs) { return (n + bs - 1) / bs; }
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
@@ -472,32 +619,90 @@ int cgpu_vm_forces(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     dim3 g(nb(b->num_vertices, 256)), t(256);
     if (b->precision)
-        hipLaunchKernelGGL(k_vm_force_eom_move<float>, g, t, 0, S(stream), *b, b->pos, b->pos_next, 0);
+        hipLaunchKernelGGL((k_vm_force_eom_move<float, 0>), g, t, 0, S(stream), *b, b->pos, b->pos_next, 0);
     else
-        hipLaunchKernelGGL(k_vm_force_eom_move<double>, g, t, 0, S(stream), *b, b->pos, b->pos_next, 0);
+        hipLaunchKernelGGL((k_vm_force_eom_move<double, 0>), g, t, 0, S(stream), *b, b->pos, b->pos_next, 0);
     return check();
 }
 
-int cgpu_vm_brownian_step(const cgpu_vertex_batch *b, void *stream) {
+}  // extern "C"
+
+namespace {
+// geometry, force + equation of motion + move, T1 test (+ director rotation),
+// T1 flips, step counter
+template <int KIND> int vm_step(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
+    if (KIND == 1 && (!b->theta || !b->motility)) return CGPU_ERR_ARGS;
     hipStream_t st = S(stream);
     int r = cgpu_vm_geometry(b, stream);
     if (r) return r;
     dim3 g(nb(b->num_vertices, 256)), t(256);
     if (b->precision)
-        hipLaunchKernelGGL(k_vm_force_eom_move<float>, g, t, 0, st, *b, b->pos, b->pos_next, 1);
+        hipLaunchKernelGGL((k_vm_force_eom_move<float, KIND>), g, t, 0, st, *b, b->pos, b->pos_next, 1);
     else
-        hipLaunchKernelGGL(k_vm_force_eom_move<double>, g, t, 0, st, *b, b->pos, b->pos_next, 1);
-    hipLaunchKernelGGL(k_vm_t1_test, g, t, 0, st, *b, b->pos_next);
+        hipLaunchKernelGGL((k_vm_force_eom_move<double, KIND>), g, t, 0, st, *b, b->pos, b->pos_next, 1);
+    hipLaunchKernelGGL(k_vm_t1_test, g, t, 0, st, *b, b->pos_next, KIND == 1 ? 1 : 0);
     hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, st, *b, b->pos_next);
     hipLaunchKernelGGL(k_vm_advance, dim3(nb(b->num_systems, 256)), dim3(256), 0, st, *b);
     return check();
 }
+}  // namespace
+
+extern "C" {
+
+int cgpu_vm_brownian_step(const cgpu_vertex_batch *b, void *stream) { return vm_step<0>(b, stream); }
+
+int cgpu_vm_spp_step(const cgpu_vertex_batch *b, void *stream) { return vm_step<1>(b, stream); }
 
 int cgpu_vm_t1(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    hipLaunchKernelGGL(k_vm_t1_test, dim3(nb(b->num_vertices, 256)), dim3(256), 0, S(stream), *b, b->pos);
+    hipLaunchKernelGGL(k_vm_t1_test, dim3(nb(b->num_vertices, 256)), dim3(256), 0, S(stream), *b, b->pos, 0);
     hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, S(stream), *b, b->pos);
+    return check();
+}
+
+int cgpu_vm_fire_vv_move(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
+    if (!b || !fs || !vel) return CGPU_ERR_ARGS;
+    dim3 g(nb(b->num_vertices, 256)), t(256);
+    if (b->precision)
+        hipLaunchKernelGGL(k_vm_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel);
+    else
+        hipLaunchKernelGGL(k_vm_fire_vv_move<double>, g, t, 0, S(stream), *b, fs, vel);
+    hipLaunchKernelGGL(k_vm_fire_clear_pending, dim3(nb(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
+                       b->num_systems);
+    return check();
+}
+
+int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots,
+                              void *stream) {
+    if (!b || !fs || !vel || !dots || !b->chunk_sys) return CGPU_ERR_ARGS;
+    int r = cgpu_vm_geometry(b, stream);
+    if (r) return r;
+    dim3 g(nb(b->num_vertices, 256)), t(256);
+    if (b->precision)
+        hipLaunchKernelGGL(k_vm_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, dots);
+    else
+        hipLaunchKernelGGL(k_vm_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, dots);
+    hipLaunchKernelGGL(k_vm_fire_partial, dim3(b->num_chunks), dim3(256), 0, S(stream), *b, fs, dots);
+    hipLaunchKernelGGL(k_vm_fire_reduce, dim3(nb(b->num_systems, 64)), dim3(64), 0, S(stream), *b, fs, dots);
+    return check();
+}
+
+int cgpu_vm_fire_park_converged(const cgpu_vertex_batch *b, const cgpu_fire_state *fs, void *stream) {
+    if (!b || !fs) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_vm_fire_park, dim3(nb(b->num_systems, 256)), dim3(256), 0, S(stream), *b, fs);
+    return check();
+}
+
+int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
+    if (!b || !fs || !vel) return CGPU_ERR_ARGS;
+    dim3 g(nb(b->num_vertices, 256)), t(256);
+    if (b->precision)
+        hipLaunchKernelGGL(k_vm_fire_apply_mix<float>, g, t, 0, S(stream), *b, fs, vel);
+    else
+        hipLaunchKernelGGL(k_vm_fire_apply_mix<double>, g, t, 0, S(stream), *b, fs, vel);
+    hipLaunchKernelGGL(k_vm_fire_clear_pending, dim3(nb(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
+                       b->num_systems);
     return check();
 }
 

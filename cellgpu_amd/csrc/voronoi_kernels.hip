@@ -16,6 +16,7 @@
 
 #include "../../include/cellgpu_amd.h"
 #include "cgpu_device.h"
+#include "fire_device.h"
 
 using namespace cgpu;
 
@@ -1726,22 +1727,7 @@ __global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double
 
 
 // ------------------------------------------------------------------------ FIRE
-// reference EnergyMinimizerFIRE GPU branch (EnergyMinimizerFIRE2D.cpp:93-114,
-// 156-207; kernels EnergyMinimizerFIRE2D.cu:16-68). One iteration:
-//   [previous mixing] disp = dt v + dt^2/2 F, v += dt/2 F, move, enforce topology,
-//   F(new), v += dt/2 F, P = F.v, |F|^2, |v|^2 (FP64), v = (1-a) v + a sqrt(vv/ff) F,
-//   dt / alpha update or v = 0.
-template <typename R> __device__ inline void fire_mix(vec2<R> &v, vec2<R> F, const cgpu_fire_state &fs) {
-    if (!fs.pending) return;
-    if (fs.zero_v) {
-        v = vec2<R>{R(0), R(0)};
-        return;
-    }
-    R a = (R)fs.mix_alpha, sc = (R)fs.scaling;
-    v.x = (R(1) - a) * v.x + a * sc * F.x;
-    v.y = (R(1) - a) * v.y + a * sc * F.y;
-}
-
+// (shared pieces and the iteration: fire_device.h)
 template <typename R>
 __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
                                double *pos_next) {
@@ -1758,10 +1744,7 @@ __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, 
         stpos(pos_next, i, p);
         return;
     }
-    const R dt = (R)f.dt;
-    vec2<R> d{dt * v.x + R(0.5) * dt * dt * F.x, dt * v.y + R(0.5) * dt * dt * F.y};
-    v.x += R(0.5) * dt * F.x;
-    v.y += R(0.5) * dt * F.y;
+    vec2<R> d = fire_vv_first_half<R>(v, F, (R)f.dt);
     st2<R>(vel, i, v);
     p.x += (double)d.x;
     p.y += (double)d.y;
@@ -1794,47 +1777,16 @@ __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs
     v.x += R(0.5) * dt * F.x;
     v.y += R(0.5) * dt * F.y;
     st2<R>(vel, i, v);
-    double fx = F.x, fy = F.y, vx = v.x, vy = v.y;
-    const int N = b.num_cells;
-    dots[i] = fx * fx + fy * fy;
-    dots[N + i] = fx * vx + fy * vy;
-    dots[2 * N + i] = vx * vx + vy * vy;
+    fire_dots(dots, b.num_cells, i, F.x, F.y, v.x, v.y);
 }
 
-// one block per system: deterministic FP64 sums, then the FIRE scalar update
-// per cell chunk: partial sums of |F|^2, F.v, |v|^2 and the largest |F|^2, in
-// a fixed order (thread-strided, then a fixed tree): deterministic
+// per cell chunk: partial sums (fixed order, deterministic)
 __global__ void k_fire_partial(cgpu_voronoi_batch b, const cgpu_fire_state *fs, double *dots) {
     const int c = blockIdx.x;
     const int s = b.chunk_sys[c];
     const int N = b.num_cells;
-    double *part = dots + 3 * (size_t)N + 4 * (size_t)c;
-    __shared__ double sh[4][256];
-    double a = 0, cc = 0, d = 0, m = 0;
-    if (!(fs[s].converged || !b.st[s].active)) {
-        for (int i = b.chunk_begin[c] + (int)threadIdx.x; i < b.chunk_end[c]; i += blockDim.x) {
-            const double ff = dots[i];
-            a += ff;
-            cc += dots[N + i];
-            d += dots[2 * N + i];
-            m = fmax(m, ff);
-        }
-    }
-    sh[0][threadIdx.x] = a;
-    sh[1][threadIdx.x] = cc;
-    sh[2][threadIdx.x] = d;
-    sh[3][threadIdx.x] = m;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
-            sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 4) part[threadIdx.x] = sh[threadIdx.x][0];
+    fire_chunk_partial(dots, N, b.chunk_begin[c], b.chunk_end[c], fs[s].converged || !b.st[s].active,
+                       dots + 3 * (size_t)N + 4 * (size_t)c);
 }
 
 // per system: the chunk partials summed in chunk order, then the FIRE update
@@ -1853,28 +1805,7 @@ __global__ void k_fire_reduce(cgpu_voronoi_batch b, cgpu_fire_state *fs, const d
         velNorm += part[4 * c + 2];
         mf = fmax(mf, part[4 * c + 3]);
     }
-    {
-        f.iterations += 1;
-        f.power = power;
-        f.force_max = forceNorm / (double)sp.n;
-        f.max_f2 = mf;
-        f.scaling = forceNorm > 0. ? sqrt(velNorm / forceNorm) : 0.0;
-        f.mix_alpha = f.alpha;
-        f.pending = 1;
-        if (power > 0) {
-            f.zero_v = 0;
-            if (f.nsince > f.nmin) {
-                f.dt = fmin(f.dt * f.dt_inc, f.dt_max);
-                f.alpha = f.alpha * f.alpha_dec;
-            }
-            f.nsince += 1;
-        } else {
-            f.dt = f.dt * f.dt_dec;
-            f.alpha = f.alpha_start;
-            f.zero_v = 1;
-        }
-        if (!(f.iterations < f.max_iterations && sqrt(f.force_max) > f.force_cutoff)) f.converged = 1;
-    }
+    fire_update(f, forceNorm, power, velNorm, mf, sp.n);
 }
 
 __global__ void k_fire_park(cgpu_voronoi_batch b, const cgpu_fire_state *fs) {

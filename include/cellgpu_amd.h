@@ -144,6 +144,8 @@ typedef struct cgpu_vm_params {
     double dt, mu, T;   /* Brownian step                                   */
     double thr;         /* T1 edge-length threshold                        */
     uint64_t seed;      /* counter-based noise key                         */
+    int chunk0;         /* first vertex chunk of the system                */
+    int pad_i;
 } cgpu_vm_params;
 
 /*
@@ -169,7 +171,14 @@ typedef struct cgpu_vertex_batch {
     int *flag_cells;             /* [12Nv] */
     int *flag_touched;           /* [12Nv] */
     unsigned long long *cell_min;/* [Nc], all ones when idle */
-    const double *noise;         /* [2Nv] or NULL */
+    const double *noise;         /* [2Nv + Nc] or NULL: x then y per vertex, then
+                                    one per cell (director rotation) */
+    void *theta;                 /* [Nc] real: cell director angles       */
+    const void *motility;        /* [2Nc] real: (v0, Dr) per cell          */
+    /* vertex chunks (at most CGPU_CHUNK vertices of one system), for the
+     * per-system FIRE reductions */
+    const int *chunk_sys, *chunk_begin, *chunk_end;
+    int num_chunks, pad_c;
 } cgpu_vertex_batch;
 
 /* ---------------------------------------------------------------- kernel layer */
@@ -269,6 +278,22 @@ int cgpu_vm_geometry(const cgpu_vertex_batch *b, void *stream);
 int cgpu_vm_forces(const cgpu_vertex_batch *b, void *stream);
 int cgpu_vm_brownian_step(const cgpu_vertex_batch *b, void *stream);
 int cgpu_vm_t1(const cgpu_vertex_batch *b, void *stream);
+/* One step of self-propelled cell-vertex dynamics, replacing
+ * gpu_spp_cellVertex_eom_integration (reference selfPropelledCellVertexDynamics.cuh,
+ * selfPropelledCellVertexDynamics.cu:21-71, .cpp:51-76): forces, vertex
+ * displacement dt (mean of v0 e(theta) over the three cells + mu F), director
+ * rotation theta += sqrt(2 dt Dr) eta, move, T1 test and flips. */
+int cgpu_vm_spp_step(const cgpu_vertex_batch *b, void *stream);
+/* FIRE on a vertex batch (the degrees of freedom are the vertices; reference
+ * EnergyMinimizerFIRE2D.cpp:93-114, 156-207 with State = vertex model, whose
+ * enforceTopology is the T1 pass, vertexModelBase.cpp:93-106). An iteration is
+ * vv_move, then cgpu_vm_t1 and force_reduce on the moved positions.
+ * dots: 3 Nv + 4 num_chunks doubles. */
+int cgpu_vm_fire_vv_move(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
+int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots,
+                              void *stream);
+int cgpu_vm_fire_park_converged(const cgpu_vertex_batch *b, const cgpu_fire_state *fs, void *stream);
+int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
 /* ---------------------------------------------------------------- object layer */
 
@@ -317,9 +342,11 @@ void *cgpu_vor_stream(void *h);
 /* A batch of vertex-model systems built from the Voronoi tessellation of the
  * given cell positions (reference vertexModelBase::setCellsVoronoiTesselation,
  * vertexModelBase.cpp:115-229). Vertex ids follow (smallest cell, ring slot) of
- * the Delaunay triangles. x, y NULL: reference random points from seed[s]. */
+ * the Delaunay triangles. x, y NULL: reference random points from seed[s].
+ * spv_initialize: first relax the points by 100 self-propelled Voronoi steps
+ * (dt 0.1, P0 3.8, v0 0.1, Dr 1), as the reference's spvInitialize does. */
 void *cgpu_vm_create(int num_systems, const int *ncells, const double *L, const uint64_t *seed, const double *x,
-                     const double *y, int precision, int device);
+                     const double *y, int precision, int device, int spv_initialize);
 void cgpu_vm_destroy(void *h);
 int cgpu_vm_num_vertices(void *h);
 int cgpu_vm_set_params(void *h, int s, double KA, double KP, double dt, double mu, double T, double t1_threshold);
@@ -327,8 +354,16 @@ int cgpu_vm_set_prefs(void *h, const double *A0, const double *P0);       /* per
 int cgpu_vm_set_positions(void *h, const double *vx, const double *vy);  /* per vertex */
 int cgpu_vm_set_noise_mode(void *h, int use_buffer);
 int cgpu_vm_upload_noise(void *h, const double *eta);                     /* 2 Nv: x block, y block */
+int cgpu_vm_upload_cell_noise(void *h, const double *eta);                /* Nc: director rotation */
 int cgpu_vm_compute_forces(void *h);
 int cgpu_vm_brownian_steps(void *h, int steps, int use_graph);
+/* Self-propelled cell-vertex dynamics (reference selfPropelledCellVertexDynamics):
+ * per-cell (v0, Dr) and director angles. The directors start as the reference's
+ * setCellDirectorsRandomly draws after the positions (Simple2DActiveCell.cpp:55-70). */
+int cgpu_vm_set_motility(void *h, const double *v0, const double *Dr);    /* per cell */
+int cgpu_vm_set_directors(void *h, const double *theta);                 /* per cell */
+int cgpu_vm_get_directors(void *h, double *theta);
+int cgpu_vm_spp_steps(void *h, int steps, int use_graph);
 int cgpu_vm_get(void *h, double *vx, double *vy, double *Fx, double *Fy, double *A, double *P);
 int cgpu_vm_get_topology(void *h, int *vn, int *vcn, int *cvn, int *cv, int cap);
 int cgpu_vm_get_counters(void *h, int s, long long *out);
@@ -342,7 +377,9 @@ int cgpu_fire_set(void *h, int max_it, double force_cutoff, double alpha_start, 
                   double dt_inc, double dt_dec, double alpha_dec, int nmin);
 int cgpu_fire_minimize(void *h);
 int cgpu_fire_get(void *h, int s, double *out /* force_max, power, alpha, dt, max_f2, iterations */);
-/* position-only view of the vertex engine is separate; FIRE drives Voronoi batches */
+/* The same minimizer on a vertex batch (handle used with cgpu_fire_set / _minimize
+ * / _get / _destroy); forceMax = sum |F|^2 / Nv per system. */
+void *cgpu_vm_fire_create(void *vm);
 
 int cgpu_device_count(void);
 int cgpu_device_synchronize(void);

@@ -127,16 +127,21 @@ class VoronoiQuadraticEnergy : public Simple2DModel {
 //! reference vertexModelBase + VertexQuadraticEnergy (GPU branch)
 class VertexQuadraticEnergy : public Simple2DModel {
    public:
-    VertexQuadraticEnergy(int n, double A0, double P0, bool reprod, int precision = 0, unsigned seed = 13377)
+    //! runSPV: relax the random points with 100 self-propelled Voronoi steps first
+    //! (reference vertexModelBase::setCellsVoronoiTesselation(spvInitialize))
+    VertexQuadraticEnergy(int n, double A0, double P0, bool reprod, bool runSPV = false, int precision = 0,
+                          unsigned seed = 13377)
         : Ncells(n) {
         uint64_t s = reprod ? seed : (uint64_t)std::random_device{}();
-        h_ = cgpu_vm_create(1, &n, nullptr, &s, nullptr, nullptr, precision, 0);
+        h_ = cgpu_vm_create(1, &n, nullptr, &s, nullptr, nullptr, precision, 0, runSPV ? 1 : 0);
         if (!h_) throw std::runtime_error("cgpu_vm_create failed");
         Nvertices = cgpu_vm_num_vertices(h_);
         setCellPreferencesUniform(A0, P0);
     }
     ~VertexQuadraticEnergy() override { cgpu_vm_destroy(h_); }
     void setCellPreferencesUniform(double A0, double P0) {
+        A0_ = A0;
+        P0_ = P0;
         vector<double> a(Ncells, A0), p(Ncells, P0);
         check(cgpu_vm_set_prefs(h_, a.data(), p.data()), "setCellPreferencesUniform");
     }
@@ -152,6 +157,41 @@ class VertexQuadraticEnergy : public Simple2DModel {
         T = t;
         push();
     }
+    void setMu(double m) {
+        mu = m;
+        push();
+    }
+    //! reference Simple2DActiveCell::setv0Dr (Simple2DActiveCell.cpp:76-92)
+    void setv0Dr(double v0, double Dr) {
+        vector<double> a(Ncells, v0), b(Ncells, Dr);
+        check(cgpu_vm_set_motility(h_, a.data(), b.data()), "setv0Dr");
+    }
+    vector<double> returnCellDirectors() {
+        vector<double> t(Ncells);
+        check(cgpu_vm_get_directors(h_, t.data()), "cellDirectors");
+        return t;
+    }
+    //! reference vertexModelBase::reportMeanVertexForce: mean force on the vertices
+    double2v reportMeanVertexForce() {
+        vector<double> a(Nvertices), b(Nvertices);
+        check(cgpu_vm_get(h_, 0, 0, a.data(), b.data(), 0, 0), "get");
+        double2v m{0, 0};
+        for (int i = 0; i < Nvertices; ++i) {
+            m.x += a[i];
+            m.y += b[i];
+        }
+        m.x /= Nvertices;
+        m.y /= Nvertices;
+        return m;
+    }
+    //! reference Simple2DCell::reportq: mean P / sqrt(A) over the cells
+    double reportq() {
+        vector<double> A(Ncells), P(Ncells);
+        check(cgpu_vm_get(h_, 0, 0, 0, 0, A.data(), P.data()), "get");
+        double q = 0;
+        for (int c = 0; c < Ncells; ++c) q += P[c] / std::sqrt(A[c]);
+        return q / Ncells;
+    }
     int getNumberOfDegreesOfFreedom() override { return Nvertices; }
     void computeForces() override { check(cgpu_vm_compute_forces(h_), "computeForces"); }
     double computeEnergy() override {
@@ -159,7 +199,7 @@ class VertexQuadraticEnergy : public Simple2DModel {
         vector<double> A(Ncells), P(Ncells);
         check(cgpu_vm_get(h_, 0, 0, 0, 0, A.data(), P.data()), "get");
         double E = 0;
-        for (int c = 0; c < Ncells; ++c) E += KA * (A[c] - 1.0) * (A[c] - 1.0) + KP * (P[c] - P0_) * (P[c] - P0_);
+        for (int c = 0; c < Ncells; ++c) E += KA * (A[c] - A0_) * (A[c] - A0_) + KP * (P[c] - P0_) * (P[c] - P0_);
         return E;
     }
     vector<double2v> returnForces() override {
@@ -178,7 +218,7 @@ class VertexQuadraticEnergy : public Simple2DModel {
     }
     void *handle() { return h_; }
     int Ncells, Nvertices;
-    double KA = 1, KP = 1, deltaT = 0.005, mu = 1, T = 0, T1Threshold = 0.01, P0_ = 3.9;
+    double KA = 1, KP = 1, deltaT = 0.005, mu = 1, T = 0, T1Threshold = 0.01, A0_ = 1.0, P0_ = 3.9;
 
    private:
     void push() { check(cgpu_vm_set_params(h_, 0, KA, KP, deltaT, mu, T, T1Threshold), "set params"); }
@@ -242,14 +282,35 @@ class brownianParticleDynamics : public simpleEquationOfMotion {
     double Temperature = 0.0;
 };
 
-//! reference EnergyMinimizerFIRE2D.h:20-132 (GPU branch), Voronoi models
+//! reference selfPropelledCellVertexDynamics.h (GPU branch,
+//! selfPropelledCellVertexDynamics.cpp:51-76)
+class selfPropelledCellVertexDynamics : public simpleEquationOfMotion {
+   public:
+    selfPropelledCellVertexDynamics(int Ncells = 0, int Nvertices = 0) {
+        (void)Ncells;
+        (void)Nvertices;
+    }
+    void integrateEquationsOfMotion() override {
+        auto vm = std::dynamic_pointer_cast<VertexQuadraticEnergy>(model);
+        if (!vm) throw std::runtime_error("selfPropelledCellVertexDynamics drives a vertex model");
+        vm->setMu(mu);
+        check(cgpu_vm_spp_steps(vm->handle(), 1, 0), "self-propelled cell-vertex step");
+    }
+    double mu = 1.0;
+};
+
+//! reference EnergyMinimizerFIRE2D.h:20-132 (GPU branch), Voronoi and vertex models
 class EnergyMinimizerFIRE : public simpleEquationOfMotion {
    public:
     explicit EnergyMinimizerFIRE(shared_ptr<Simple2DModel> system) {
         set2DModel(system);
-        auto v = std::dynamic_pointer_cast<VoronoiQuadraticEnergy>(system);
-        if (!v) throw std::runtime_error("EnergyMinimizerFIRE drives a Voronoi model");
-        h_ = cgpu_fire_create(v->handle());
+        if (auto v = std::dynamic_pointer_cast<VoronoiQuadraticEnergy>(system))
+            h_ = cgpu_fire_create(v->handle());
+        else if (auto vm = std::dynamic_pointer_cast<VertexQuadraticEnergy>(system))
+            h_ = cgpu_vm_fire_create(vm->handle());
+        else
+            throw std::runtime_error("EnergyMinimizerFIRE drives a Voronoi or vertex model");
+        if (!h_) throw std::runtime_error("FIRE create failed");
         push();
     }
     ~EnergyMinimizerFIRE() override { cgpu_fire_destroy(h_); }

@@ -234,6 +234,12 @@ def _vlib():
             "vmo_brownian_step": (C.c_int, [vp, _dp, _dp]),
             "vmo_t1": (C.c_int, [vp]),
             "vmo_last_flagged": (C.c_int, [vp]),
+            "vmo_set_motility": (None, [vp, _dp, _dp]),
+            "vmo_set_directors": (None, [vp, _dp]),
+            "vmo_get_directors": (None, [vp, _dp]),
+            "vmo_spp_step": (C.c_int, [vp, _dp]),
+            "vmo_fire_set": (None, [vp, C.c_int] + [C.c_double] * 7 + [C.c_int]),
+            "vmo_fire_minimize": (C.c_int, [vp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -304,6 +310,34 @@ class VertexSystem:
 
     def last_flagged(self):
         return _vlib().vmo_last_flagged(self.h)
+
+    def set_motility(self, v0, Dr):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v0, np.float64), (self.Nc,)))
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(Dr, np.float64), (self.Nc,)))
+        _vlib().vmo_set_motility(self.h, a, b)
+
+    def set_directors(self, theta):
+        _vlib().vmo_set_directors(self.h, np.ascontiguousarray(theta, np.float64))
+
+    def directors(self):
+        out = np.zeros(self.Nc)
+        _vlib().vmo_get_directors(self.h, out)
+        return out
+
+    def spp_step(self, eta_cell):
+        """One self-propelled cell-vertex step (reference selfPropelledCellVertexDynamics,
+        GPU branch); eta_cell: the N(0,1) draw of every cell's director."""
+        return _vlib().vmo_spp_step(self.h, np.ascontiguousarray(eta_cell, np.float64))
+
+    def fire(self, max_it=1000, force_cutoff=1e-7, alpha_start=0.99, dt=0.01, dt_max=0.1, dt_inc=1.05,
+             dt_dec=0.95, alpha_dec=0.9, n_min=5):
+        """FIRE (GPU branch) on the vertices; returns (iterations, state). The
+        iteration count accumulates over calls, as in the reference."""
+        _vlib().vmo_fire_set(self.h, max_it, force_cutoff, alpha_start, dt, dt_max, dt_inc, dt_dec, alpha_dec,
+                             n_min)
+        out = np.zeros(5)
+        it = _vlib().vmo_fire_minimize(self.h, out)
+        return it, dict(force_max=out[0], power=out[1], alpha=out[2], dt=out[3], max_f2=out[4])
 
 
 def vertex_from_random(Nc, seed=13377, **kw):

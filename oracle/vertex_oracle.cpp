@@ -22,6 +22,8 @@
 #include <cstring>
 #include <vector>
 
+#include "fire_oracle.h"
+
 using std::vector;
 
 namespace vmo {
@@ -69,6 +71,9 @@ struct VM {
     double KA = 1, KP = 1, dt = 0.005, mu = 1, T = 0, thr = 0.04;
     long long t1count = 0;
     int lastFlagged = 0;
+    vector<double> theta;  // cellDirectors
+    vector<d2> mot;        // Motility (v0, Dr); reference default setv0Dr(0.0, 1.0)
+    fireo::State<d2> fire;
 };
 
 // reference vertexModelBase.cpp:248-297 / vertexModelBase.cu:20-77
@@ -330,6 +335,8 @@ void *vmo_create(int Nc, double L, const double *x, const double *y, const int *
     m->AP.assign(Nc, d2{0, 0});
     m->APpref.assign(Nc, d2{1.0, 3.9});
     m->F.assign(m->Nv, d2{0, 0});
+    m->theta.assign(Nc, 0.0);
+    m->mot.assign(Nc, d2{0.0, 1.0});
     m->vcur.assign(3 * m->Nv, d2{0, 0});
     m->vlast = m->vcur;
     m->vnext = m->vcur;
@@ -401,4 +408,77 @@ int vmo_t1(void *h) {
     return (int)(m->t1count - before);
 }
 int vmo_last_flagged(void *h) { return ((VM *)h)->lastFlagged; }
+void vmo_set_motility(void *h, const double *v0, const double *Dr) {
+    VM *m = (VM *)h;
+    for (int c = 0; c < m->Nc; ++c) m->mot[c] = d2{v0[c], Dr[c]};
+}
+void vmo_set_directors(void *h, const double *th) {
+    VM *m = (VM *)h;
+    for (int c = 0; c < m->Nc; ++c) m->theta[c] = th[c];
+}
+void vmo_get_directors(void *h, double *th) {
+    VM *m = (VM *)h;
+    for (int c = 0; c < m->Nc; ++c) th[c] = m->theta[c];
+}
+// one self-propelled cell-vertex step (reference selfPropelledCellVertexDynamics
+// GPU branch, .cpp:51-76, .cu:21-71): forces, vertex displacement
+// dt (mean over the three cells of v0 e(theta) + mu F), then the director
+// rotation theta += eta sqrt(2 dt Dr), the move and the T1 pass; returns the
+// number of T1 flips
+int vmo_spp_step(void *h, const double *eta_cell) {
+    VM *m = (VM *)h;
+    geometry(*m);
+    forces(*m);
+    vector<d2> disp(m->Nv);
+    for (int i = 0; i < m->Nv; ++i) {
+        int c1 = m->vcn[3 * i], c2 = m->vcn[3 * i + 1], c3 = m->vcn[3 * i + 2];
+        double v1 = m->mot[c1].x, v2 = m->mot[c2].x, v3 = m->mot[c3].x;
+        double dirx = (v1 * std::cos(m->theta[c1]) + v2 * std::cos(m->theta[c2]) + v3 * std::cos(m->theta[c3])) / 3.0;
+        double diry = (v1 * std::sin(m->theta[c1]) + v2 * std::sin(m->theta[c2]) + v3 * std::sin(m->theta[c3])) / 3.0;
+        disp[i].x = m->dt * (dirx + m->mu * m->F[i].x);
+        disp[i].y = m->dt * (diry + m->mu * m->F[i].y);
+    }
+    for (int c = 0; c < m->Nc; ++c) m->theta[c] += eta_cell[c] * std::sqrt(2.0 * m->dt * m->mot[c].y);
+    long long before = m->t1count;
+    for (int i = 0; i < m->Nv; ++i) {
+        m->v[i].x += disp[i].x;
+        m->v[i].y += disp[i].y;
+        m->box.putInBoxReal(m->v[i]);
+    }
+    t1(*m);
+    return (int)(m->t1count - before);
+}
+// FIRE (GPU branch, fire_oracle.h) with the vertices as degrees of freedom and
+// the T1 pass as the topology step (vertexModelBase.cpp:93-106)
+struct VertexFireModel {
+    VM *m;
+    int dof() { return m->Nv; }
+    void forces(vector<d2> &F) {
+        geometry(*m);
+        ::forces(*m);
+        F = m->F;
+    }
+    void move(const vector<d2> &d) {
+        for (int i = 0; i < m->Nv; ++i) {
+            m->v[i].x += d[i].x;
+            m->v[i].y += d[i].y;
+            m->box.putInBoxReal(m->v[i]);
+        }
+    }
+    bool topology() {
+        t1(*m);
+        return true;
+    }
+};
+void vmo_fire_set(void *h, int maxIt, double fc, double alphaStart, double dt, double dtMax, double dtInc,
+                  double dtDec, double alphaDec, int NMin) {
+    ((VM *)h)->fire.set(maxIt, fc, alphaStart, dt, dtMax, dtInc, dtDec, alphaDec, NMin);
+}
+// returns the iterations done; out as fireo::minimize
+int vmo_fire_minimize(void *h, double *out) {
+    VM *m = (VM *)h;
+    VertexFireModel model{m};
+    return fireo::minimize(m->fire, model, out);
+}
+
 }

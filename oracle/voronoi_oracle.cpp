@@ -23,6 +23,7 @@
 #include <random>
 #include <vector>
 
+#include "fire_oracle.h"
 #include "predicates.h"
 
 using std::vector;
@@ -950,109 +951,38 @@ int orc_vsys_brownian_step(void *h, const double *etax, const double *etay, doub
 }
 
 // --- FIRE (reference EnergyMinimizerFIRE2D.cpp:24-41, 93-114, 156-207, 269-284) ---
-struct Fire {
-    int iterations = 0, maxIterations = 1000, NMin = 5, NSince = 0;
-    double forceMax = 100., forceCutoff = 1e-7, alphaStart = 0.99, alpha = 0.99, dt = 0.01, dtMin = 1e-4,
-           dtMax = 0.1, dtInc = 1.05, dtDec = 0.95, alphaDec = 0.9, Power = 0;
-    vector<d2> v, f;
+// the loop is fire_oracle.h; the Voronoi model's topology step is the repair
+struct VoronoiFireModel {
+    VSys *s;
+    int dof() { return s->N; }
+    void forces(vector<d2> &F) {
+        geometryGPUStyle(*s);
+        forcesGPUStyle(*s, false);
+        F = s->F;
+    }
+    void move(const vector<d2> &d) {
+        for (int i = 0; i < s->N; ++i) {
+            s->P[i].x += d[i].x;
+            s->P[i].y += d[i].y;
+            s->box.putInBoxReal(s->P[i]);
+        }
+    }
+    bool topology() { return enforceTopology(*s, true); }
 };
 void *orc_fire_create(void *hsys) {
-    VSys *s = (VSys *)hsys;
-    Fire *f = new Fire();
-    f->v.assign(s->N, d2{0, 0});
-    f->f.assign(s->N, d2{0, 0});
-    return f;
+    (void)hsys;
+    return new fireo::State<d2>();
 }
-void orc_fire_destroy(void *h) { delete (Fire *)h; }
+void orc_fire_destroy(void *h) { delete (fireo::State<d2> *)h; }
 void orc_fire_set(void *h, int maxIt, double fc, double alphaStart, double dt, double dtMax, double dtInc,
                   double dtDec, double alphaDec, int NMin) {
-    Fire *f = (Fire *)h;
-    f->maxIterations = maxIt;
-    f->forceCutoff = fc;
-    f->alphaStart = alphaStart;
-    f->alpha = alphaStart;
-    f->dt = dt;
-    f->dtMin = dt * .01;
-    f->dtMax = dtMax;
-    f->dtInc = dtInc;
-    f->dtDec = dtDec;
-    f->alphaDec = alphaDec;
-    f->NMin = NMin;
+    ((fireo::State<d2> *)h)->set(maxIt, fc, alphaStart, dt, dtMax, dtInc, dtDec, alphaDec, NMin);
 }
 // GPU-branch FIRE on a Voronoi system; returns iterations done. out: forceMax
 // (= sum|F|^2/N), Power, alpha, dt, max|F|^2
 int orc_fire_minimize(void *hf, void *hsys, double *out) {
-    Fire *fr = (Fire *)hf;
-    VSys *s = (VSys *)hsys;
-    int N = s->N;
-    geometryGPUStyle(*s);
-    forcesGPUStyle(*s, false);
-    fr->f = s->F;
-    fr->forceMax = 110.0;
-    double fmax2 = 0;
-    while (fr->iterations < fr->maxIterations && std::sqrt(fr->forceMax) > fr->forceCutoff) {
-        fr->iterations += 1;
-        // velocityVerletGPU
-        vector<d2> disp(N);
-        for (int i = 0; i < N; ++i) {
-            disp[i].x = fr->dt * fr->v[i].x + 0.5 * fr->dt * fr->dt * fr->f[i].x;
-            disp[i].y = fr->dt * fr->v[i].y + 0.5 * fr->dt * fr->dt * fr->f[i].y;
-        }
-        for (int i = 0; i < N; ++i) {
-            fr->v[i].x += 0.5 * fr->dt * fr->f[i].x;
-            fr->v[i].y += 0.5 * fr->dt * fr->f[i].y;
-        }
-        for (int i = 0; i < N; ++i) {
-            s->P[i].x += disp[i].x;
-            s->P[i].y += disp[i].y;
-            s->box.putInBoxReal(s->P[i]);
-        }
-        if (!enforceTopology(*s, true)) return -1;
-        geometryGPUStyle(*s);
-        forcesGPUStyle(*s, false);
-        fr->f = s->F;
-        for (int i = 0; i < N; ++i) {
-            fr->v[i].x += 0.5 * fr->dt * fr->f[i].x;
-            fr->v[i].y += 0.5 * fr->dt * fr->f[i].y;
-        }
-        // fireStepGPU
-        double ff = 0, fv = 0, vv = 0;
-        fmax2 = 0;
-        for (int i = 0; i < N; ++i) {
-            double a = fr->f[i].x * fr->f[i].x + fr->f[i].y * fr->f[i].y;
-            ff += a;
-            fmax2 = std::max(fmax2, a);
-            fv += fr->f[i].x * fr->v[i].x + fr->f[i].y * fr->v[i].y;
-            vv += fr->v[i].x * fr->v[i].x + fr->v[i].y * fr->v[i].y;
-        }
-        fr->Power = fv;
-        fr->forceMax = ff / (double)N;
-        double scaling = 0.0;
-        if (ff > 0.) scaling = std::sqrt(vv / ff);
-        for (int i = 0; i < N; ++i) {
-            fr->v[i].x = (1 - fr->alpha) * fr->v[i].x + fr->alpha * scaling * fr->f[i].x;
-            fr->v[i].y = (1 - fr->alpha) * fr->v[i].y + fr->alpha * scaling * fr->f[i].y;
-        }
-        if (fr->Power > 0) {
-            if (fr->NSince > fr->NMin) {
-                fr->dt = std::min(fr->dt * fr->dtInc, fr->dtMax);
-                fr->alpha = fr->alpha * fr->alphaDec;
-            }
-            fr->NSince += 1;
-        } else {
-            fr->dt = fr->dt * fr->dtDec;
-            fr->alpha = fr->alphaStart;
-            for (int i = 0; i < N; ++i) fr->v[i] = d2{0, 0};
-        }
-    }
-    if (out) {
-        out[0] = fr->forceMax;
-        out[1] = fr->Power;
-        out[2] = fr->alpha;
-        out[3] = fr->dt;
-        out[4] = fmax2;
-    }
-    return fr->iterations;
+    VoronoiFireModel m{(VSys *)hsys};
+    return fireo::minimize(*(fireo::State<d2> *)hf, m, out);
 }
 
 }  // extern "C"

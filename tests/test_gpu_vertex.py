@@ -112,3 +112,129 @@ def test_vertex_full_size_invariants():
     assert (g["A"] > 0).all()
     F = np.abs(np.concatenate([g["Fx"], g["Fy"]])).max()
     assert abs(g["Fx"].sum()) < 1e-9 * F * b.Nv and abs(g["Fy"].sum()) < 1e-9 * F * b.Nv
+
+
+def test_vertex_spp_cell_vertex_stepwise(oracle):
+    """Self-propelled cell-vertex dynamics (per-cell v0, Dr = 1, T1 threshold 0.1):
+    each step the oracle restarts from the GPU state (positions, topology,
+    directors) with the same director noise; positions within 1e-11, directors
+    within 1e-13, topology identical."""
+    cg = _cg()
+    Nc = 300
+    b = cg.VertexBatch([Nc], seeds=[9], precision="fp64")
+    b.set_params(dt=0.01, t1_threshold=0.1)
+    b.set_prefs(1.0, 3.9)
+    rng = np.random.default_rng(3)
+    v0 = rng.uniform(0.1, 0.5, Nc)
+    b.set_motility(v0, 1.0)
+    b.set_noise_buffer(True)
+    o = _oracle_like(oracle, Nc, 9, dt=0.01, thr=0.1, P0=3.9)
+    o.set_motility(v0, 1.0)
+    t1_total = 0
+    for step in range(25):
+        g = b.get()
+        vn, vcn, cvn, cv = b.topology()
+        o.set_positions(g["vx"], g["vy"])
+        o.set_topology(vn, vcn, cvn, cv)
+        o.set_directors(b.directors())
+        eta = rng.standard_normal(Nc)
+        b.upload_cell_noise(eta)
+        b.spp_steps(1, use_graph=False)
+        b.check_status()
+        t1_total += o.spp_step(eta)
+        g, r = b.get(), o.compute()
+        np.testing.assert_allclose(g["vx"], r["vx"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(g["vy"], r["vy"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(b.directors(), o.directors(), rtol=0, atol=1e-13)
+        vn, vcn, cvn, cv = b.topology()
+        ovn, ovcn, ocvn, ocv = o.topology()
+        assert _sets(vn) == _sets(ovn), step
+        assert _sets(vcn) == _sets(ovcn), step
+        assert _canon(cv, cvn) == _canon(ocv, ocvn), step
+    assert t1_total > 0
+    assert b.counters()["t1_events"] == t1_total
+
+
+def test_vertex_spp_graph_equals_single_steps():
+    """Graph-captured self-propelled cell-vertex steps (counter-based noise) end
+    bit-identical to the same steps launched one at a time."""
+    cg = _cg()
+    out = []
+    for use_graph in (False, True):
+        b = cg.VertexBatch([400], seeds=[12], precision="fp32")
+        b.set_params(dt=0.01, t1_threshold=0.1)
+        b.set_motility(0.3, 1.0)
+        b.spp_steps(41, use_graph=use_graph)
+        b.check_status()
+        g = b.get()
+        out.append((g["vx"], g["vy"], b.directors(), b.topology()[3], b.counters()["t1_events"]))
+        b.close()
+    a, c = out
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1]) and np.array_equal(a[2], c[2])
+    assert np.array_equal(a[3], c[3]) and a[4] == c[4] and a[4] > 0
+
+
+def test_vertex_fire_matches_oracle_fp64(oracle):
+    """FIRE on the vertices (per-system scalars, FP64 sums) against the oracle's
+    GPU-branch FIRE with the T1 pass as enforceTopology."""
+    cg = _cg()
+    Nc = 200
+    b = cg.VertexBatch([Nc], seeds=[11], precision="fp64")
+    b.set_params(dt=0.01, t1_threshold=0.04)
+    b.set_prefs(1.0, 3.7)
+    o = _oracle_like(oracle, Nc, 11, dt=0.01, thr=0.04, P0=3.7)
+    f = cg.EnergyMinimizerFIRE(b)
+    f.setMaximumIterations(60)
+    f.setForceCutoff(1e-12)
+    f.setAlphaStart(0.1)
+    f.setDeltaT(0.01)
+    f.minimize()
+    st = f.state()
+    it, so = o.fire(max_it=60, force_cutoff=1e-12, alpha_start=0.1, dt=0.01)
+    assert st["iterations"] == it == 60
+    b.compute_forces()
+    g, r = b.get(), o.compute()
+    L = float(np.sqrt(Nc))
+    # reductions run in a different order (tree vs sequential)
+    assert np.abs(g["vx"] - r["vx"]).max() < 1e-10 * L and np.abs(g["vy"] - r["vy"]).max() < 1e-10 * L
+    assert abs(st["force_max"] - so["force_max"]) <= 1e-8 * so["force_max"]
+    assert abs(st["dt"] - so["dt"]) <= 1e-12 and abs(st["alpha"] - so["alpha"]) <= 1e-12
+    vn, vcn, cvn, cv = b.topology()
+    ovn, ovcn, ocvn, ocv = o.topology()
+    assert _canon(cv, cvn) == _canon(ocv, ocvn)
+
+
+def test_vertex_fire_batch_converges_independently():
+    cg = _cg()
+    b = cg.VertexBatch([100, 300], seeds=[1, 2], precision="fp64")
+    b.set_params(dt=0.01, t1_threshold=0.04)
+    b.set_prefs(1.0, np.concatenate([np.full(100, 3.7), np.full(300, 3.75)]))
+    f = cg.EnergyMinimizerFIRE(b)
+    f.setMaximumIterations(20000)
+    f.setForceCutoff(1e-8)
+    f.setAlphaStart(0.1)
+    f.minimize()
+    for s in range(2):
+        st = f.state(s)
+        assert st["force_max"] ** 0.5 < 1e-8 and st["iterations"] < 20000
+    b.compute_forces()
+    g = b.get()
+    assert np.sqrt(np.mean(g["Fx"][:200] ** 2 + g["Fy"][:200] ** 2)) < 1e-8
+    b.check_status()
+
+
+def test_vertex_spv_initialize_smooths_the_tissue():
+    """runSPV start (100 self-propelled Voronoi steps before the tessellation):
+    a valid vertex model whose cells are rounder than the random start's."""
+    cg = _cg()
+    Nc = 400
+    q = []
+    for spv in (False, True):
+        m = cg.VertexQuadraticEnergy(Nc, 1.0, 3.8, True, runSPV=spv, seed=13377)
+        m.computeForces()
+        AP = m.returnAreaPeri()
+        vn, vcn, cvn, cv = m.batch.topology()
+        assert m.Nvertices == 2 * Nc and cvn.sum() == 3 * m.Nvertices
+        assert abs(AP[:, 0].sum() - Nc) < 1e-9 * Nc and (AP[:, 0] > 0).all()
+        q.append(np.mean(AP[:, 1] / np.sqrt(AP[:, 0])))
+    assert q[1] < q[0]

@@ -121,3 +121,63 @@ def test_brownian_zero_temperature_relaxes(oracle):
     assert E1 < E0
     vn, vcn, cvn, cv = vm.topology()
     assert cvn.sum() == 3 * vm.Nv
+
+
+def _wrapped(d, L):
+    return d - L * np.round(d / L)
+
+
+def test_spp_cell_vertex_closed_forms(oracle):
+    """Self-propelled cell-vertex step at the force-free lattice. Uniform
+    directors: every vertex moves by dt v0 e(theta). Per-cell directors: each
+    cell's self-propulsion is shared by its six vertices, a third each, so the
+    summed displacement is 2 dt sum_c v0_c e(theta_c). Directors rotate by
+    eta sqrt(2 dt Dr)."""
+    vm, ax, ay = _lattice_vm(oracle, dt=0.01)
+    k = (ay * ay - ax * ax / 4) / (2 * ay)
+    h = (ay * ay + ax * ax / 4) / (2 * ay)
+    vm.set_prefs(ax * ay, 4 * k + 4 * np.hypot(ax / 2, h - k))
+    L, dt = vm.L, 0.01
+    r0 = vm.compute()
+    vm.set_motility(0.3, 0.5)
+    vm.set_directors(np.full(vm.Nc, 0.7))
+    eta = np.random.default_rng(3).standard_normal(vm.Nc)
+    assert vm.spp_step(eta) == 0
+    r1 = vm.compute()
+    dx, dy = _wrapped(r1["vx"] - r0["vx"], L), _wrapped(r1["vy"] - r0["vy"], L)
+    np.testing.assert_allclose(dx, dt * 0.3 * np.cos(0.7), atol=1e-12)
+    np.testing.assert_allclose(dy, dt * 0.3 * np.sin(0.7), atol=1e-12)
+    np.testing.assert_allclose(vm.directors(), 0.7 + eta * np.sqrt(2 * dt * 0.5), rtol=0, atol=1e-15)
+
+    vm2, _, _ = _lattice_vm(oracle, dt=dt)
+    vm2.set_prefs(ax * ay, 4 * k + 4 * np.hypot(ax / 2, h - k))
+    rng = np.random.default_rng(4)
+    v0, th = rng.uniform(0, 0.5, vm2.Nc), rng.uniform(0, 2 * np.pi, vm2.Nc)
+    vm2.set_motility(v0, 1.0)
+    vm2.set_directors(th)
+    r0 = vm2.compute()
+    vm2.spp_step(np.zeros(vm2.Nc))
+    r1 = vm2.compute()
+    dx, dy = _wrapped(r1["vx"] - r0["vx"], L), _wrapped(r1["vy"] - r0["vy"], L)
+    assert abs(dx.sum() - 2 * dt * np.sum(v0 * np.cos(th))) < 1e-12
+    assert abs(dy.sum() - 2 * dt * np.sum(v0 * np.sin(th))) < 1e-12
+    np.testing.assert_array_equal(vm2.directors(), th)  # zero noise: directors unchanged
+
+
+def test_fire_relaxes_vertex_lattice_to_its_minimum(oracle):
+    """FIRE on the vertices from a perturbed force-free lattice with the lattice's
+    own (A0, P0): the energy minimum is E = 0 and FIRE reaches it."""
+    vm, ax, ay = _lattice_vm(oracle)
+    k = (ay * ay - ax * ax / 4) / (2 * ay)
+    h = (ay * ay + ax * ax / 4) / (2 * ay)
+    vm.set_prefs(ax * ay, 4 * k + 4 * np.hypot(ax / 2, h - k))
+    r0 = vm.compute()
+    rng = np.random.default_rng(5)
+    vm.set_positions(r0["vx"] + 0.02 * rng.standard_normal(vm.Nv), r0["vy"] + 0.02 * rng.standard_normal(vm.Nv))
+    E0 = vm.compute()["E"]
+    assert E0 > 1e-4
+    it, st = vm.fire(max_it=20000, force_cutoff=1e-10, alpha_start=0.1, dt=0.01)
+    assert it < 20000 and st["force_max"] ** 0.5 <= 1e-10
+    r = vm.compute()
+    assert r["E"] < 1e-12 * E0
+    np.testing.assert_allclose(r["A"], ax * ay, rtol=1e-7)
