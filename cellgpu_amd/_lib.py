@@ -99,7 +99,6 @@ def lib():
         "cgpu_fire_set": (i, [vp, i, d, d, d, d, d, d, d, i]),
         "cgpu_fire_minimize": (i, [vp]),
         "cgpu_fire_get": (i, [vp, i, vp]),
-        "cgpu_vor_kernel_times": (i, [vp, i, vp, vp, i]),
         "cgpu_vm_create": (vp, [i, vp, vp, vp, vp, vp, i, i, i]),
         "cgpu_vm_destroy": (None, [vp]),
         "cgpu_vm_num_vertices": (i, [vp]),

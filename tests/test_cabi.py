@@ -35,6 +35,14 @@ def test_every_declared_symbol_is_exported(so):
     assert not missing, missing
 
 
+def test_python_binding_table_resolves(so):
+    """Every signature the Python front end declares names an exported symbol
+    (loading binds them all; nothing is called)."""
+    from cellgpu_amd import _lib
+    L = _lib.lib()
+    assert L is not None
+
+
 def test_philox_known_answers(so):
     # Philox4x32-10 known-answer vectors of the Random123 distribution (Salmon et al. 2011)
     kat = [
