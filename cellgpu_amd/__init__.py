@@ -13,7 +13,7 @@ import numpy as np
 from ._lib import CellGPUError, check, lib, require_device, VoronoiBatchView  # noqa: F401
 
 __all__ = [
-    "CellGPUError", "VoronoiBatch", "VertexBatch", "VertexQuadraticEnergy", "VoronoiQuadraticEnergy", "Simulation", "selfPropelledParticleDynamics",
+    "CellGPUError", "VoronoiBatch", "VertexBatch", "VertexQuadraticEnergy", "VoronoiQuadraticEnergy", "VoronoiQuadraticEnergyWithTension", "Simulation", "selfPropelledParticleDynamics",
     "selfPropelledCellVertexDynamics", "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
 ]
 
@@ -82,6 +82,25 @@ class VoronoiBatch:
     def set_exclusions(self, ex):
         e = np.ascontiguousarray(ex, dtype=np.int32)
         check(lib().cgpu_vor_set_exclusions(self.h, _p(e)), "set_exclusions")
+
+    def set_cell_types(self, types):
+        """Cell types (reference Simple2DCell::setCellType); None clears them."""
+        self._types = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+        if self._types is not None and self._types.shape != (self.N,):
+            raise CellGPUError("one type per cell")
+        check(lib().cgpu_vor_set_cell_types(self.h, _p(self._types)), "set_cell_types")
+
+    def set_tension(self, mode, gamma=0.0, system=-1):
+        """Line tension of system `system` (-1: all): mode 0 off, 1 one value
+        gamma between unlike types, 2 the tension matrix."""
+        check(lib().cgpu_vor_set_tension(self.h, int(system), int(mode), float(gamma)), "set_tension")
+
+    def set_tension_matrix(self, g):
+        """g[a, b]: tension between types a and b (reference setSurfaceTension(vector))."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        if g.ndim != 2 or g.shape[0] != g.shape[1]:
+            raise CellGPUError("the tension matrix is square")
+        check(lib().cgpu_vor_set_tension_matrix(self.h, g.shape[0], _p(g)), "set_tension_matrix")
 
     def set_positions(self, x, y, retriangulate=True):
         a, b = _f64(x, self.N), _f64(y, self.N)
@@ -230,6 +249,50 @@ class VoronoiQuadraticEnergy:
     def reportMeanCellForce(self, verbose=False):
         g = self.batch.get()
         return float(np.mean(g["Fx"])), float(np.mean(g["Fy"]))
+
+    def setCellType(self, types):
+        """reference Simple2DCell::setCellType."""
+        self.batch.set_cell_types(types)
+
+
+class VoronoiQuadraticEnergyWithTension(VoronoiQuadraticEnergy):
+    """reference inc/models/voronoiQuadraticEnergyWithTension.h:16-60 (GPU branch):
+    line tension between cells of different types, one value for every unlike pair
+    or a matrix over types. computeEnergy adds gamma times the length of every
+    edge between unlike cells, counted from both cells (the energy whose gradient
+    the reference's force sets are)."""
+
+    def __init__(self, n, A0=1.0, P0=3.8, reprod=True, **kw):
+        super().__init__(n, A0, P0, reprod, **kw)
+        self.gamma = 0.0
+        self.Tension = False
+        self.simpleTension = True
+
+    def _push(self):
+        mode = 0 if not self.Tension else (1 if self.simpleTension else 2)
+        self.batch.set_tension(mode, self.gamma)
+
+    def setUseSurfaceTension(self, use_tension):
+        self.Tension = bool(use_tension)
+        self._push()
+
+    def setSurfaceTension(self, g):
+        """A number: one tension for every unlike pair. A sequence of n*n values
+        (or an n x n matrix): g[a*n + b] between types a and b."""
+        if np.ndim(g) == 0:
+            self.gamma = float(g)
+            self.simpleTension = True
+        else:
+            m = np.asarray(g, np.float64)
+            n = int(round(np.sqrt(m.size)))
+            if n * n != m.size:
+                raise CellGPUError("the tension matrix has n*n entries")
+            self.batch.set_tension_matrix(m.reshape(n, n))
+            self.simpleTension = False
+        self._push()
+
+    def getSurfaceTension(self):
+        return self.gamma
 
 
 class _Updater:

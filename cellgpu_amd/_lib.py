@@ -25,7 +25,7 @@ class SysParams(C.Structure):
     _fields_ = [("off", C.c_int), ("n", C.c_int), ("bin_off", C.c_int), ("xsize", C.c_int),
                 ("L", C.c_double), ("boxsize", C.c_double), ("KA", C.c_double), ("KP", C.c_double),
                 ("dt", C.c_double), ("mu", C.c_double), ("T", C.c_double), ("seed", C.c_uint64),
-                ("ncells", C.c_int), ("pad0", C.c_int)]
+                ("ncells", C.c_int), ("tension", C.c_int), ("gamma", C.c_double)]
 
 
 class VoronoiBatchView(C.Structure):
@@ -35,7 +35,8 @@ class VoronoiBatchView(C.Structure):
             "theta", "force", "excl", "ext_force", "bin_count", "bin_start", "cell_bin", "cell_rank", "sorted_id",
             "sorted_pos", "repair", "mark", "flagged", "nfq", "gq",
             "counters", "noise", "chunk_sys", "chunk_begin", "chunk_end", "bchunk_sys", "bchunk_begin",
-            "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)]
+            "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)] + [
+        ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("pad_t", C.c_int)]
 
 
 def lib():
@@ -57,6 +58,9 @@ def lib():
         "cgpu_vor_set_prefs": (i, [vp, vp, vp]),
         "cgpu_vor_set_motility": (i, [vp, vp, vp]),
         "cgpu_vor_set_exclusions": (i, [vp, vp]),
+        "cgpu_vor_set_cell_types": (i, [vp, vp]),
+        "cgpu_vor_set_tension": (i, [vp, i, i, d]),
+        "cgpu_vor_set_tension_matrix": (i, [vp, i, vp]),
         "cgpu_vor_set_positions": (i, [vp, vp, vp, i]),
         "cgpu_vor_set_noise_mode": (i, [vp, i]),
         "cgpu_vor_upload_noise": (i, [vp, vp]),

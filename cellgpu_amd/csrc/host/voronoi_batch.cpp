@@ -7,6 +7,8 @@
 #include <cstring>
 #include <random>
 
+#include "../cgpu_device.h"
+
 
 namespace cgpu_host {
 
@@ -265,6 +267,9 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.bchunk_sum = bchunk_sum_.p;
     b.num_chunks = num_chunks_;
     b.num_bin_chunks = num_bin_chunks_;
+    b.cell_type = tension_on_ ? cell_type_.p : nullptr;
+    b.tension = tension_.p;
+    b.num_types = ntypes_;
     return b;
 }
 
@@ -343,6 +348,79 @@ void VoronoiBatch::set_exclusions(const int *ex) {
         (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
     }
+}
+
+// a system in matrix mode may only hold types below the matrix size (the
+// kernels index the matrix with them)
+void VoronoiBatch::update_tension_flag() {
+    bool any = false;
+    for (const auto &p : hp_) {
+        any = any || p.tension != 0;
+        if (p.tension == 2 && !h_type_.empty())
+            for (int i = p.off; i < p.off + p.n; ++i)
+                if (h_type_[i] >= ntypes_) throw std::invalid_argument("cell type outside the tension matrix");
+    }
+    tension_on_ = any && !h_type_.empty();
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void VoronoiBatch::set_cell_types(const int *types) {
+    sync();
+    if (!types) {
+        cell_type_.release();
+        h_type_.clear();
+    } else {
+        for (int i = 0; i < N_; ++i)
+            if (types[i] < 0) throw std::invalid_argument("negative cell type");
+        std::vector<int> keep(h_type_);
+        h_type_.assign(types, types + N_);
+        try {
+            update_tension_flag();
+        } catch (...) {
+            h_type_ = keep;
+            throw;
+        }
+        if (!cell_type_.p) cell_type_.alloc(N_);
+        cell_type_.upload(types, N_);
+    }
+    update_tension_flag();
+}
+
+void VoronoiBatch::set_tension(int s, int mode, double gamma) {
+    if (mode < 0 || mode > 2) throw std::invalid_argument("tension mode must be 0, 1 or 2");
+    if (mode == 2 && !tension_.p) throw std::invalid_argument("tension matrix not set");
+    sync();
+    std::vector<cgpu_sys_params> keep(hp_);
+    int lo = s < 0 ? 0 : s, hi = s < 0 ? S_ : s + 1;
+    for (int q = lo; q < hi; ++q) {
+        hp_[q].tension = mode;
+        hp_[q].gamma = gamma;
+    }
+    try {
+        update_tension_flag();
+    } catch (...) {
+        hp_ = keep;
+        throw;
+    }
+    sys_.upload(hp_.data(), S_);
+}
+
+void VoronoiBatch::set_tension_matrix(int ntypes, const double *g) {
+    if (ntypes <= 0 || !g) throw std::invalid_argument("tension matrix needs ntypes > 0");
+    sync();
+    const int keep = ntypes_;
+    ntypes_ = ntypes;
+    try {
+        update_tension_flag();
+    } catch (...) {
+        ntypes_ = keep;
+        throw;
+    }
+    tension_.alloc((size_t)ntypes * ntypes);
+    tension_.upload(g, (size_t)ntypes * ntypes);
 }
 
 void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
@@ -575,6 +653,38 @@ double VoronoiBatch::energy(int s) {
         E += p.KA * (A[i] - pref[2 * i]) * (A[i] - pref[2 * i]);
         E += p.KP * (P[i] - pref[2 * i + 1]) * (P[i] - pref[2 * i + 1]);
     }
+    if (p.tension != 0 && !h_type_.empty()) {
+        // line tension: every Voronoi edge between cells of different types,
+        // counted from both of its cells (the energy whose gradient the tension
+        // force sets are), gamma times its length
+        std::vector<double> x(N_), y(N_);
+        get(x.data(), y.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        const int cap = CGPU_RING_CAP;
+        std::vector<int> nn(N_), nbr((size_t)N_ * cap);
+        get_topology(nn.data(), nbr.data(), cap, nullptr);
+        std::vector<double> g;
+        if (p.tension == 2) {
+            g.resize((size_t)ntypes_ * ntypes_);
+            tension_.download(g.data(), g.size());
+        }
+        const cgpu::Box box = cgpu::make_box(p.L);
+        for (int i = p.off; i < p.off + p.n; ++i) {
+            const int z = nn[i];
+            const cgpu::dvec2 pi{x[i], y[i]};
+            auto rel = [&](int k) {
+                const int j = nbr[(size_t)i * cap + ((k % z) + z) % z];
+                return cgpu::min_dist(box, cgpu::dvec2{x[j], y[j]}, pi);
+            };
+            for (int k = 0; k < z; ++k) {
+                const int j = nbr[(size_t)i * cap + k];
+                if (h_type_[i] == h_type_[j]) continue;
+                // the edge shared with ring k joins circumcenters (k-1, k) and (k, k+1)
+                cgpu::dvec2 a = cgpu::circ0<double>(rel(k - 1), rel(k)), c = cgpu::circ0<double>(rel(k), rel(k + 1));
+                const double l = std::sqrt((a.x - c.x) * (a.x - c.x) + (a.y - c.y) * (a.y - c.y));
+                E += (p.tension == 1 ? p.gamma : g[(size_t)h_type_[i] * ntypes_ + h_type_[j]]) * l;
+            }
+        }
+    }
     return E;
 }
 
@@ -655,6 +765,12 @@ int VoronoiBatch::spatial_sort(int *perm_out) {
         for (size_t i = 0; i < N; ++i) b[i] = ex[perm[i]];
         excl_.upload(b.data(), N);
     }
+    if (!h_type_.empty()) {
+        std::vector<int> b(N);
+        for (size_t i = 0; i < N; ++i) b[i] = h_type_[perm[i]];
+        h_type_ = b;
+        cell_type_.upload(b.data(), N);
+    }
     if (perm_out)
         for (size_t i = 0; i < N; ++i) perm_out[i] = perm[i];
     if (graph_exec_) {
@@ -689,6 +805,9 @@ thread_local std::string g_last_error;
 template <typename F> int guarded(F &&f) {
     try {
         return f();
+    } catch (const std::invalid_argument &e) {
+        g_last_error = e.what();
+        return CGPU_ERR_ARGS;
     } catch (const std::exception &e) {
         g_last_error = e.what();
         return CGPU_ERR_HIP;
@@ -732,6 +851,15 @@ int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr) {
 }
 int cgpu_vor_set_exclusions(void *h, const int *ex) {
     return guarded([&] { ((VoronoiBatch *)h)->set_exclusions(ex); return 0; });
+}
+int cgpu_vor_set_cell_types(void *h, const int *types) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_cell_types(types); return 0; });
+}
+int cgpu_vor_set_tension(void *h, int s, int mode, double gamma) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_tension(s, mode, gamma); return 0; });
+}
+int cgpu_vor_set_tension_matrix(void *h, int ntypes, const double *g) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_tension_matrix(ntypes, g); return 0; });
 }
 int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retri) {
     return guarded([&] { ((VoronoiBatch *)h)->set_positions(x, y, retri != 0); return ((VoronoiBatch *)h)->status(); });

@@ -64,6 +64,10 @@ class VoronoiBatch {
     void set_prefs(const double *A0, const double *P0);
     void set_motility(const double *v0, const double *Dr);
     void set_exclusions(const int *ex);
+    // line tension between cell types (VoronoiQuadraticEnergyWithTension)
+    void set_cell_types(const int *types);
+    void set_tension(int s, int mode, double gamma);
+    void set_tension_matrix(int ntypes, const double *g);
     void set_positions(const double *x, const double *y, bool retriangulate);
     void set_noise_mode(bool use_buffer);
     void upload_noise(const double *eta, int count);
@@ -123,6 +127,12 @@ class VoronoiBatch {
     DevBuf<unsigned int> bin_count_;
     DevBuf<double> pos0_, pos1_, noise_, sorted_pos_;
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
+    DevBuf<int> cell_type_;
+    std::vector<int> h_type_;  // host copy of the cell types (validation, sorting, energy)
+    DevBuf<double> tension_;
+    int ntypes_ = 0;
+    bool tension_on_ = false;  // some system has a tension mode and cell types are set
+    void update_tension_flag();
     size_t device_bytes_ = 0;
 };
 

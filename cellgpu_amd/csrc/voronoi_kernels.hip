@@ -193,16 +193,52 @@ __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const
 }
 
 // -------------------------------------------------------------------- forces
+// Line-tension terms of one force set (reference voronoiQuadraticEnergyWithTension.cu:
+// 27-199 for the matrix, the one-value kernel after it): which pairs of the
+// vertex's three cells (I = the cell, J = ring k-2, K = ring k-1) differ in type,
+// their tensions, and the reference's way of adding 0.5 (gamma) dT to a term.
+template <typename R> struct TensionSlot {
+    bool ik, ij, jk;
+    R gik, gij, gjk;  // 1 in the one-value mode (gamma multiplies the sum)
+    R half;           // 0.5 (matrix) or 0.5 gamma (one value)
+    __device__ R add(R e, R dT) const { return e + half * dT; }
+};
+template <typename R>
+__device__ inline TensionSlot<R> tension_slot(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp, int tI, int tJ,
+                                              int tK) {
+    TensionSlot<R> t;
+    t.ik = tI != tK;
+    t.ij = tI != tJ;
+    t.jk = tJ != tK;
+    if (sp.tension == 1) {
+        t.gik = t.gij = t.gjk = R(1);
+        t.half = R(0.5) * (R)sp.gamma;
+    } else {
+        const int n = b.num_types;  // reference cellTypeIndexer(b, a) = a*n + b
+        t.gik = t.ik ? (R)b.tension[tI * n + tK] : R(0);
+        t.gij = t.ij ? (R)b.tension[tI * n + tJ] : R(0);
+        t.gjk = t.jk ? (R)b.tension[tJ * n + tK] : R(0);
+        t.half = R(0.5);
+    }
+    return t;
+}
+// the tension kernels clamp edge lengths at THRESHOLD = 1e-18 in both precisions
+template <typename R> __device__ inline R clamp_threshold(R x) { return x < R(1e-18) ? R(1e-18) : x; }
+
 // Force on cell i: sum over ring slots k of the reference force set
 // (voronoiQuadraticEnergy.cu:99-234): slot k is the Voronoi vertex of
 // (i, ring k-2, ring k-1); its neighbours in the three polygons are vertex k-1,
 // vertex k+1 and the circumcenter of (ring k-2, ring k-1, delOther k).
-template <typename R, typename Acc>
+// TENS: the batch has cell types; systems with tension mode != 0 add the
+// line-tension terms.
+template <typename R, bool TENS, typename Acc>
 __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
     if (z < 3) return vec2<R>{R(0), R(0)};
+    const bool tens = TENS && sp.tension != 0;
+    auto clampn = [&](R x) { return tens ? clamp_threshold<R>(x) : clamp_len<R>(x); };
     const dvec2 pi = acc.p(i);
     auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
     auto dd = [&](int id) { return min_dist(box, acc.p(id), pi); };
@@ -211,6 +247,12 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
     vec2<R> qB = acc.q(jB), qC = acc.q(jC);
     vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
     const vec2<R> qI = acc.q(i);
+    int tI = 0, tB = 0, tC = 0;
+    if (tens) {
+        tI = b.cell_type[i];
+        tB = b.cell_type[jB];
+        tC = b.cell_type[jC];
+    }
     vec2<R> sum{R(0), R(0)};
     // slot k's neighbour, delOther and their positions are fetched one slot
     // ahead, the indices two slots ahead
@@ -239,25 +281,73 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
         vec2<R> vother = Prec<R>::c3(dB, dC, dO);
         vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
         vec2<R> dl{vlast.x - vcur.x, vlast.y - vcur.y};
-        R dlnorm = clamp_len<R>(sqrt(dl.x * dl.x + dl.y * dl.y));
+        R dlnorm = clampn(sqrt(dl.x * dl.x + dl.y * dl.y));
         vec2<R> dn{vcur.x - vnext.x, vcur.y - vnext.y};
-        R dnnorm = clamp_len<R>(sqrt(dn.x * dn.x + dn.y * dn.y));
+        R dnnorm = clampn(sqrt(dn.x * dn.x + dn.y * dn.y));
         const vec2<R> dclv = dl, dncv = dn;
         const R dclnorm = dlnorm, dncnorm = dnnorm;
         vec2<R> dPdv{dl.x / dlnorm - dn.x / dnnorm, dl.y / dlnorm - dn.y / dnnorm};
         vec2<R> dEdv{qI.x * dAdv.x + qI.y * dPdv.x, qI.x * dAdv.y + qI.y * dPdv.y};
+        TensionSlot<R> T{};
+        if (tens) {
+            // self terms: edges I-K (vcur, vnext) and I-J (vcur, vlast)
+            T = tension_slot<R>(b, sp, tI, tB, tC);
+            vec2<R> dT{R(0), R(0)};
+            if (T.ik) {
+                dT.x -= T.gik * dn.x / dnnorm;
+                dT.y -= T.gik * dn.y / dnnorm;
+            }
+            if (T.ij) {
+                dT.x += T.gij * dl.x / dlnorm;
+                dT.y += T.gij * dl.y / dlnorm;
+            }
+            dEdv.x = T.add(dEdv.x, dT.x);
+            dEdv.y = T.add(dEdv.y, dT.y);
+        }
         // cell ring k-1
         dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
         dn = vec2<R>{vcur.x - vother.x, vcur.y - vother.y};
-        dnnorm = clamp_len<R>(sqrt(dn.x * dn.x + dn.y * dn.y));
+        dnnorm = clampn(sqrt(dn.x * dn.x + dn.y * dn.y));
         dPdv = vec2<R>{-dncv.x / dncnorm - dn.x / dnnorm, -dncv.y / dncnorm - dn.y / dnnorm};
-        dEdv.x += qC.x * dAdv.x + qC.y * dPdv.x;
-        dEdv.y += qC.x * dAdv.y + qC.y * dPdv.y;
+        {
+            vec2<R> e{qC.x * dAdv.x + qC.y * dPdv.x, qC.x * dAdv.y + qC.y * dPdv.y};
+            if (tens) {  // edges K-I (vcur, vnext) and K-J (vcur, vother)
+                vec2<R> dT{R(0), R(0)};
+                if (T.ik) {
+                    dT.x += T.gik * -dncv.x / dncnorm;
+                    dT.y += T.gik * -dncv.y / dncnorm;
+                }
+                if (T.jk) {
+                    dT.x -= T.gjk * dn.x / dnnorm;
+                    dT.y -= T.gjk * dn.y / dnnorm;
+                }
+                e.x = T.add(e.x, dT.x);
+                e.y = T.add(e.y, dT.y);
+            }
+            dEdv.x += e.x;
+            dEdv.y += e.y;
+        }
         // cell ring k-2
         dAdv = vec2<R>{R(0.5) * (vother.y - vlast.y), R(0.5) * (vlast.x - vother.x)};
         dPdv = vec2<R>{-dn.x / dnnorm + dclv.x / dclnorm, -dn.y / dnnorm + dclv.y / dclnorm};
-        dEdv.x += qB.x * dAdv.x + qB.y * dPdv.x;
-        dEdv.y += qB.x * dAdv.y + qB.y * dPdv.y;
+        {
+            vec2<R> e{qB.x * dAdv.x + qB.y * dPdv.x, qB.x * dAdv.y + qB.y * dPdv.y};
+            if (tens) {  // edges J-I (vcur, vlast) and J-K (vcur, vother)
+                vec2<R> dT{R(0), R(0)};
+                if (T.ij) {
+                    dT.x -= T.gij * -dclv.x / dclnorm;
+                    dT.y -= T.gij * -dclv.y / dclnorm;
+                }
+                if (T.jk) {
+                    dT.x += T.gjk * -dn.x / dnnorm;
+                    dT.y += T.gjk * -dn.y / dnnorm;
+                }
+                e.x = T.add(e.x, dT.x);
+                e.y = T.add(e.y, dT.y);
+            }
+            dEdv.x += e.x;
+            dEdv.y += e.y;
+        }
         dEdv.x *= R(2.0);
         dEdv.y *= R(2.0);
         sum.x += dEdv.x * m[0] + dEdv.y * m[2];
@@ -267,6 +357,10 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
         dC = dD;
         qB = qC;
         qC = acc.q(jD);
+        if (tens) {
+            tB = tC;
+            tC = b.cell_type[jD];
+        }
         vL = vM;
         vM = vN;
     }
@@ -1594,10 +1688,10 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
 
 // force of every active cell, optional exclusions (reference
 // gpu_sum_forces_with_exclusions_kernel, voronoiQuadraticEnergy.cu:59-93)
-template <typename R, typename Acc>
+template <typename R, bool TENS, typename Acc>
 __device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, const Acc &acc, int i,
                                                 const cgpu_sys_params &sp) {
-    vec2<R> F = cell_force<R>(b, acc, i, sp);
+    vec2<R> F = cell_force<R, TENS>(b, acc, i, sp);
     if (b.excl) {
         if (b.excl[i] != 0) {
             if (b.ext_force) st2<R>(b.ext_force, i, vec2<R>{-F.x, -F.y});
@@ -1609,12 +1703,12 @@ __device__ inline vec2<R> force_with_exclusions(const cgpu_voronoi_batch &b, con
     return F;
 }
 
-template <typename R>
+template <typename R, bool TENS>
 __global__ void k_forces(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const cgpu_sys_params &sp = b.sys[b.cell_sys[i]];
-    st2<R>(b.force, i, force_with_exclusions<R>(b, GlobalData<R>{pos, b.dAP}, i, sp));
+    st2<R>(b.force, i, force_with_exclusions<R, TENS>(b, GlobalData<R>{pos, b.dAP}, i, sp));
 }
 
 // reference noise draw for cell i: buffer or in-kernel counter-based deviate
@@ -1640,7 +1734,7 @@ template <> __device__ inline double atan2_r<double>(double y, double x) { retur
 // selfPropelledParticleDynamics.cu:35-53) -> move (Simple2DCell.cu:21-35).
 // kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
 constexpr int FORCE_BLOCK = 256;
-template <typename R, int KIND>
+template <typename R, int KIND, bool TENS>
 __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos,
                                                                 double *pos_next, int bin) {
     // LDS copy of the block's cells (positions, dAP): most ring members and
@@ -1667,7 +1761,7 @@ __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_bat
         return;
     }
     const int step = st.step + 1;  // Timestep is incremented before integrating
-    vec2<R> F = force_with_exclusions<R>(b, tile, i, sp);
+    vec2<R> F = force_with_exclusions<R, TENS>(b, tile, i, sp);
     st2<R>(b.force, i, F);
     const R dt = (R)sp.dt, mu = (R)sp.mu;
     vec2<R> disp;
@@ -1762,7 +1856,7 @@ __global__ void k_fire_clear_pending(cgpu_fire_state *fs, int S) {
     if (s < S) fs[s].pending = 0;
 }
 
-template <typename R>
+template <typename R, bool TENS>
 __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
                                  double *dots) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -1770,7 +1864,7 @@ __global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs
     const int s = b.cell_sys[i];
     const cgpu_fire_state &f = fs[s];
     if (f.converged || !b.st[s].active) return;
-    vec2<R> F = force_with_exclusions<R>(b, GlobalData<R>{pos, b.dAP}, i, b.sys[s]);
+    vec2<R> F = force_with_exclusions<R, TENS>(b, GlobalData<R>{pos, b.dAP}, i, b.sys[s]);
     st2<R>(b.force, i, F);
     vec2<R> v = ld2<R>(vel, i);
     const R dt = (R)f.dt;
@@ -1860,6 +1954,28 @@ int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
 }  // namespace
 
 // ============================================================== C ABI (kernel layer)
+namespace {
+// the tension instantiations run only for batches with cell types
+template <typename R> void launch_forces(const cgpu_voronoi_batch &b, hipStream_t st) {
+    dim3 g(nblocks(b.num_cells, g_block)), t(g_block);
+    if (b.cell_type)
+        hipLaunchKernelGGL((k_forces<R, true>), g, t, 0, st, b, b.pos);
+    else
+        hipLaunchKernelGGL((k_forces<R, false>), g, t, 0, st, b, b.pos);
+}
+
+template <typename R, int KIND>
+void launch_force_eom_move(const cgpu_voronoi_batch &b, hipStream_t st, int bin) {
+    // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
+    dim3 g(nblocks(b.num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
+    if (b.cell_type)
+        hipLaunchKernelGGL((k_force_eom_move<R, KIND, true>), g, t, 0, st, b, b.pos, b.pos_next, bin);
+    else
+        hipLaunchKernelGGL((k_force_eom_move<R, KIND, false>), g, t, 0, st, b, b.pos, b.pos_next, bin);
+}
+
+}  // namespace
+
 extern "C" {
 
 int cgpu_set_block_size(int bs) {
@@ -1906,28 +2022,25 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
 
 int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
     if (b->precision)
-        hipLaunchKernelGGL(k_forces<float>, g, t, 0, S(stream), *b, b->pos);
+        launch_forces<float>(*b, S(stream));
     else
-        hipLaunchKernelGGL(k_forces<double>, g, t, 0, S(stream), *b, b->pos);
+        launch_forces<double>(*b, S(stream));
     return check();
 }
 
 static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream, int bin = 0) {
-    // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
-    dim3 g(nblocks(b->num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
     hipStream_t st = S(stream);
     if (b->precision) {
         if (kind == 0)
-            hipLaunchKernelGGL((k_force_eom_move<float, 0>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
+            launch_force_eom_move<float, 0>(*b, st, bin);
         else
-            hipLaunchKernelGGL((k_force_eom_move<float, 1>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
+            launch_force_eom_move<float, 1>(*b, st, bin);
     } else {
         if (kind == 0)
-            hipLaunchKernelGGL((k_force_eom_move<double, 0>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
+            launch_force_eom_move<double, 0>(*b, st, bin);
         else
-            hipLaunchKernelGGL((k_force_eom_move<double, 1>), g, t, 0, st, *b, b->pos, b->pos_next, bin);
+            launch_force_eom_move<double, 1>(*b, st, bin);
     }
     return check();
 }
@@ -2002,10 +2115,18 @@ int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *ve
 int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
     if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
     dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
-    if (b->precision)
-        hipLaunchKernelGGL(k_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
-    else
-        hipLaunchKernelGGL(k_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+    const bool tens = b->cell_type != nullptr;
+    if (b->precision) {
+        if (tens)
+            hipLaunchKernelGGL((k_fire_force_vv2<float, true>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+        else
+            hipLaunchKernelGGL((k_fire_force_vv2<float, false>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+    } else {
+        if (tens)
+            hipLaunchKernelGGL((k_fire_force_vv2<double, true>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+        else
+            hipLaunchKernelGGL((k_fire_force_vv2<double, false>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+    }
     hipLaunchKernelGGL(k_fire_partial, dim3(b->num_chunks), dim3(256), 0, S(stream), *b, fs, dots);
     hipLaunchKernelGGL(k_fire_reduce, dim3((b->num_systems + 63) / 64), dim3(64), 0, S(stream), *b, fs, dots);
     return check();

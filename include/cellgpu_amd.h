@@ -54,7 +54,9 @@ typedef struct cgpu_sys_params {
     double T;           /* Brownian temperature                            */
     uint64_t seed;      /* counter-based noise key                         */
     int ncells;         /* vertex model: number of cells (else = n)        */
-    int pad0;
+    int tension;        /* line tension between cell types: 0 off, 1 one value
+                           gamma for every unlike pair, 2 the batch matrix     */
+    double gamma;       /* tension of mode 1                               */
 } cgpu_sys_params;
 
 /* Device-mutable per-system state of the topology machinery. */
@@ -126,6 +128,12 @@ typedef struct cgpu_voronoi_batch {
     const int *sys_bchunk;       /* [S+1] first bin chunk of each system           */
     int *bchunk_sum;             /* [num_bin_chunks] scratch of the bin scan       */
     int num_chunks, num_bin_chunks;
+    /* Line tension (reference VoronoiQuadraticEnergyWithTension): cell types, and
+     * the tension matrix g[a*num_types + b] used by systems in mode 2. cell_type
+     * NULL: no system uses tension. */
+    const int *cell_type;        /* [N] or NULL                                     */
+    const double *tension;       /* [num_types^2] or NULL                           */
+    int num_types, pad_t;
 } cgpu_voronoi_batch;
 
 #define CGPU_CHUNK 4096
@@ -309,6 +317,14 @@ int cgpu_vor_set_params(void *h, int s, double KA, double KP, double dt, double 
 int cgpu_vor_set_prefs(void *h, const double *A0, const double *P0);       /* per cell, all systems */
 int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr);    /* per cell; resets v   */
 int cgpu_vor_set_exclusions(void *h, const int *ex);
+/* Line tension between cell types (reference VoronoiQuadraticEnergyWithTension,
+ * voronoiQuadraticEnergyWithTension.h:16-60; Simple2DCell::setCellType):
+ * cell types per cell; mode for system s (-1: all) 0 off, 1 one value gamma
+ * (setSurfaceTension(double)), 2 the matrix (setSurfaceTension(vector), laid out
+ * g[a*ntypes + b] = gamma between types a and b). */
+int cgpu_vor_set_cell_types(void *h, const int *types);
+int cgpu_vor_set_tension(void *h, int s, int mode, double gamma);
+int cgpu_vor_set_tension_matrix(void *h, int ntypes, const double *g);
 int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retriangulate);
 int cgpu_vor_set_noise_mode(void *h, int use_buffer);
 int cgpu_vor_upload_noise(void *h, const double *eta);                      /* N doubles, this step */

@@ -121,7 +121,49 @@ class VoronoiQuadraticEnergy : public Simple2DModel {
         for (int i = 0; i < Ncells; ++i) out[i] = double2v{a[i], b[i]};
         return out;
     }
+
+   protected:
     void *h_;
+
+   public:
+    //! reference Simple2DCell::setCellType
+    void setCellType(const vector<int> &types) { check(cgpu_vor_set_cell_types(h_, types.data()), "setCellType"); }
+};
+
+//! reference voronoiQuadraticEnergyWithTension.h:16-60 (GPU branch): line
+//! tension between cells of different types
+class VoronoiQuadraticEnergyWithTension : public VoronoiQuadraticEnergy {
+   public:
+    VoronoiQuadraticEnergyWithTension(int n, double A0, double P0, bool reprod = false)
+        : VoronoiQuadraticEnergy(n, A0, P0, reprod) {}
+    void setUseSurfaceTension(bool use_tension) {
+        Tension = use_tension;
+        push_tension();
+    }
+    //! one tension value for every pair of unlike cells
+    void setSurfaceTension(double g) {
+        gamma = g;
+        simpleTension = true;
+        push_tension();
+    }
+    //! gammas[a*n + b]: tension between types a and b (n*n values)
+    void setSurfaceTension(const vector<double> &gammas) {
+        int n = (int)std::lround(std::sqrt((double)gammas.size()));
+        if (n * n != (int)gammas.size()) throw std::invalid_argument("the tension matrix has n*n entries");
+        check(cgpu_vor_set_tension_matrix(h_, n, gammas.data()), "setSurfaceTension");
+        simpleTension = false;
+        push_tension();
+    }
+    double getSurfaceTension() { return gamma; }
+
+   protected:
+    double gamma = 0.0;
+    bool Tension = false, simpleTension = true;
+
+   private:
+    void push_tension() {
+        check(cgpu_vor_set_tension(h_, -1, Tension ? (simpleTension ? 1 : 2) : 0, gamma), "set tension");
+    }
 };
 
 //! reference vertexModelBase + VertexQuadraticEnergy (GPU branch)

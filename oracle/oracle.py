@@ -44,6 +44,7 @@ def lib():
             "orc_vsys_get_topology": (C.c_int, [vp, _ip, _ip, C.c_int]),
             "orc_vsys_get_counters": (None, [vp, np.ctypeslib.ndpointer(dtype=np.int64)]),
             "orc_vsys_compute_forces": (C.c_double, [vp, C.c_int]),
+            "orc_vsys_set_tension": (None, [vp, C.c_int, C.c_double, C.c_int, vp, _ip]),
             "orc_vsys_test_flags": (C.c_int, [vp, _ip]),
             "orc_vsys_enforce_topology": (C.c_int, [vp, C.c_int]),
             "orc_vsys_spp_step": (C.c_int, [vp, _dp, C.c_int]),
@@ -171,6 +172,16 @@ class VoronoiSystem:
         out = np.zeros(6, np.int64)
         lib().orc_vsys_get_counters(self.h, out)
         return dict(zip(["global_fixes", "local_repairs", "repair_events", "skipped", "last_mode", "last_nf"], out.tolist()))
+
+    def set_tension(self, types, gamma=0.0, matrix=None):
+        """Line tension between cell types (VoronoiQuadraticEnergyWithTension):
+        one value gamma for every unlike pair, or a symmetric matrix g[a, b]."""
+        types = np.ascontiguousarray(types, np.int32)
+        if matrix is None:
+            lib().orc_vsys_set_tension(self.h, 1, float(gamma), 0, None, types)
+        else:
+            g = np.ascontiguousarray(matrix, np.float64)
+            lib().orc_vsys_set_tension(self.h, 2, 0.0, g.shape[0], g.ctypes.data, types)
 
     def compute_forces(self, gpu_style=True):
         return lib().orc_vsys_compute_forces(self.h, 0 if gpu_style else 1)

@@ -229,6 +229,14 @@ struct VSys {
     int sppTimestep = 0;           // selfPropelledParticleDynamics::Timestep
     vector<int> lastNeedsFixing;
     int lastRepairMode = 0;        // 0 none, 1 local, 2 global
+    // line tension between cell types (VoronoiQuadraticEnergyWithTension):
+    // mode 0 off, 1 one value gamma for every unlike pair, 2 matrix g[a*n + b]
+    int tensionMode = 0, ntypes = 0;
+    double gamma = 0.0;
+    vector<double> tmat;
+    vector<int> type;              // cellType
+    // reference cellTypeIndexer(b, a) = a*n + b (Index2D(n)(i, j) = j*n + i)
+    inline double tension(int a, int b) const { return tmat[(size_t)a * ntypes + b]; }
 
     inline int &ring(int i, int k) { return nb[(size_t)i * cap + k]; }
     inline int ringc(int i, int k) const { return nb[(size_t)i * cap + k]; }
@@ -517,7 +525,19 @@ static inline d2 voroCurGPU(const VSys &s, int i, int k) {
 
 // reference voronoiQuadraticEnergy.cu:99-234 (gpu_force_sets_kernel) then
 // :30-93 (sum with / without exclusions)
+// reference voronoiQuadraticEnergyWithTension.cu:27-199 (matrix) and the
+// one-value variant after it: the line-tension terms of one force set. Lengths
+// are clamped at THRESHOLD = 1e-18 in both precisions there.
+struct TensionTerms {
+    bool on, simple;
+    double g;
+    int tI, tJ, tK;
+    const VSys *s;
+    double gam(int a, int b) const { return simple ? 1.0 : s->tension(a, b); }
+};
+
 static void forcesGPUStyle(VSys &s, bool exclusions) {
+    const bool tens = s.tensionMode != 0;
     for (int i = 0; i < s.N; ++i) {
         int z = s.nn[i];
         d2 sum{0.0, 0.0};
@@ -533,33 +553,95 @@ static void forcesGPUStyle(VSys &s, bool exclusions) {
             d2 vlast = voroCurGPU(s, i, k - 1);
             d2 vnext = voroCurGPU(s, i, k + 1);
             d2 vother = circumcenter3(dnext, dcl, dnc);
+            auto clamp = [&](double x) { return (tens && x < 1e-18) ? 1e-18 : x; };
+            TensionTerms T{tens, s.tensionMode == 1, s.gamma, 0, 0, 0, &s};
+            if (tens) {
+                T.tI = s.type[i];
+                T.tJ = s.type[nx];
+                T.tK = s.type[ny];
+            }
+            const bool Tik = tens && T.tI != T.tK, Tij = tens && T.tI != T.tJ, Tjk = tens && T.tJ != T.tK;
+            // adds 0.5 (gamma dT) to e: the reference's ordering for either variant
+            auto add_tension = [&](double &e, double a, double dT) {
+                e = T.simple ? a + 0.5 * T.g * dT : a + 0.5 * dT;
+            };
             d2 dAdv{0.5 * (vlast.y - vnext.y), 0.5 * (vnext.x - vlast.x)};
             d2 dlast{vlast.x - vcur.x, vlast.y - vcur.y};
             double dlnorm = std::sqrt(dlast.x * dlast.x + dlast.y * dlast.y);
             d2 dn{vcur.x - vnext.x, vcur.y - vnext.y};
             double dnnorm = std::sqrt(dn.x * dn.x + dn.y * dn.y);
+            dnnorm = clamp(dnnorm);
+            dlnorm = clamp(dlnorm);
             d2 dclv = dlast, dncv = dn;
             double dclnorm = dlnorm, dncnorm = dnnorm;
             d2 dPdv{dlast.x / dlnorm - dn.x / dnnorm, dlast.y / dlnorm - dn.y / dnnorm};
             double Adiff = s.KA * (s.AP[i].x - s.APpref[i].x);
             double Pdiff = s.KP * (s.AP[i].y - s.APpref[i].y);
             d2 dEdv{Adiff * dAdv.x + Pdiff * dPdv.x, Adiff * dAdv.y + Pdiff * dPdv.y};
+            if (tens) {
+                d2 dT{0.0, 0.0};
+                if (Tik) {
+                    dT.x -= T.gam(T.tI, T.tK) * dn.x / dnnorm;
+                    dT.y -= T.gam(T.tI, T.tK) * dn.y / dnnorm;
+                }
+                if (Tij) {
+                    dT.x += T.gam(T.tI, T.tJ) * dlast.x / dlnorm;
+                    dT.y += T.gam(T.tI, T.tJ) * dlast.y / dlnorm;
+                }
+                add_tension(dEdv.x, Adiff * dAdv.x + Pdiff * dPdv.x, dT.x);
+                add_tension(dEdv.y, Adiff * dAdv.y + Pdiff * dPdv.y, dT.y);
+            }
             // cell ny ("k")
             dAdv = d2{0.5 * (vnext.y - vother.y), 0.5 * (vother.x - vnext.x)};
             dn = d2{vcur.x - vother.x, vcur.y - vother.y};
             dnnorm = std::sqrt(dn.x * dn.x + dn.y * dn.y);
+            dnnorm = clamp(dnnorm);
             dPdv = d2{-dncv.x / dncnorm - dn.x / dnnorm, -dncv.y / dncnorm - dn.y / dnnorm};
             Adiff = s.KA * (s.AP[ny].x - s.APpref[ny].x);
             Pdiff = s.KP * (s.AP[ny].y - s.APpref[ny].y);
-            dEdv.x += Adiff * dAdv.x + Pdiff * dPdv.x;
-            dEdv.y += Adiff * dAdv.y + Pdiff * dPdv.y;
+            if (tens) {
+                d2 dT{0.0, 0.0};
+                if (Tik) {  // dnc = -(self dnext)
+                    dT.x += T.gam(T.tI, T.tK) * -dncv.x / dncnorm;
+                    dT.y += T.gam(T.tI, T.tK) * -dncv.y / dncnorm;
+                }
+                if (Tjk) {
+                    dT.x -= T.gam(T.tJ, T.tK) * dn.x / dnnorm;
+                    dT.y -= T.gam(T.tJ, T.tK) * dn.y / dnnorm;
+                }
+                double ex, ey;
+                add_tension(ex, Adiff * dAdv.x + Pdiff * dPdv.x, dT.x);
+                add_tension(ey, Adiff * dAdv.y + Pdiff * dPdv.y, dT.y);
+                dEdv.x += ex;
+                dEdv.y += ey;
+            } else {
+                dEdv.x += Adiff * dAdv.x + Pdiff * dPdv.x;
+                dEdv.y += Adiff * dAdv.y + Pdiff * dPdv.y;
+            }
             // cell nx ("j")
             dAdv = d2{0.5 * (vother.y - vlast.y), 0.5 * (vlast.x - vother.x)};
             dPdv = d2{-dn.x / dnnorm + dclv.x / dclnorm, -dn.y / dnnorm + dclv.y / dclnorm};
             Adiff = s.KA * (s.AP[nx].x - s.APpref[nx].x);
             Pdiff = s.KP * (s.AP[nx].y - s.APpref[nx].y);
-            dEdv.x += Adiff * dAdv.x + Pdiff * dPdv.x;
-            dEdv.y += Adiff * dAdv.y + Pdiff * dPdv.y;
+            if (tens) {
+                d2 dT{0.0, 0.0};
+                if (Tij) {  // dcl = -(self dlast)
+                    dT.x -= T.gam(T.tI, T.tJ) * -dclv.x / dclnorm;
+                    dT.y -= T.gam(T.tI, T.tJ) * -dclv.y / dclnorm;
+                }
+                if (Tjk) {  // this cell's dlast = -dn
+                    dT.x += T.gam(T.tJ, T.tK) * -dn.x / dnnorm;
+                    dT.y += T.gam(T.tJ, T.tK) * -dn.y / dnnorm;
+                }
+                double ex, ey;
+                add_tension(ex, Adiff * dAdv.x + Pdiff * dPdv.x, dT.x);
+                add_tension(ey, Adiff * dAdv.y + Pdiff * dPdv.y, dT.y);
+                dEdv.x += ex;
+                dEdv.y += ey;
+            } else {
+                dEdv.x += Adiff * dAdv.x + Pdiff * dPdv.x;
+                dEdv.y += Adiff * dAdv.y + Pdiff * dPdv.y;
+            }
             dEdv.x *= 2.0;
             dEdv.y *= 2.0;
             d2 fs = vecmat(dEdv, dhdr);
@@ -689,13 +771,28 @@ static void geometryAndForcesCPUStyle(VSys &s) {
     }
 }
 
-// reference voronoiQuadraticEnergy.cpp:384-397
+// reference voronoiQuadraticEnergy.cpp:384-397, plus the line-tension energy
+// whose gradient the tension force sets are: every Voronoi edge between cells of
+// different types counted from both of its cells, gamma times its length. (The
+// reference's computeEnergy, voronoiQuadraticEnergyWithTension.cpp:96-155, counts
+// each edge once and, with the GPU vertex convention, pairs the edge with the
+// neighbour one ring slot off; its force sets are what this energy matches.)
 static double energy(const VSys &s) {
     double E = 0.0;
     for (int i = 0; i < s.N; ++i) {
         E += s.KA * (s.AP[i].x - s.APpref[i].x) * (s.AP[i].x - s.APpref[i].x);
         E += s.KP * (s.AP[i].y - s.APpref[i].y) * (s.AP[i].y - s.APpref[i].y);
     }
+    if (s.tensionMode != 0)
+        for (int i = 0; i < s.N; ++i)
+            for (int k = 0; k < s.nn[i]; ++k) {
+                const int j = s.ringc(i, k);
+                if (s.type[i] == s.type[j]) continue;
+                // edge shared with ring k: vertices (k-1, k) and (k, k+1)
+                d2 a = voroCurGPU(s, i, k + 1), b = voroCurGPU(s, i, k + 2);
+                double l = std::sqrt((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y));
+                E += (s.tensionMode == 1 ? s.gamma : s.tension(s.type[i], s.type[j])) * l;
+            }
     return E;
 }
 
@@ -861,6 +958,16 @@ void orc_vsys_get_counters(void *h, long long *out) {
     out[3] = s->skippedFrames;
     out[4] = s->lastRepairMode;
     out[5] = (long long)s->lastNeedsFixing.size();
+}
+// line tension (VoronoiQuadraticEnergyWithTension): mode 0 off, 1 one value
+// gamma between unlike types, 2 matrix g[a*ntypes + b]; types per cell
+void orc_vsys_set_tension(void *h, int mode, double gamma, int ntypes, const double *g, const int *types) {
+    VSys *s = (VSys *)h;
+    s->tensionMode = mode;
+    s->gamma = gamma;
+    s->ntypes = ntypes;
+    s->tmat.assign(g ? g : nullptr, g ? g + (size_t)ntypes * ntypes : nullptr);
+    s->type.assign(types, types + s->N);
 }
 // mode 0: GPU-path geometry + force sets; mode 1: CPU-path geometry + forces
 double orc_vsys_compute_forces(void *h, int mode) {

@@ -266,3 +266,105 @@ def test_large_system_full_size(oracle):
     onn, onbr, onm = oracle.triangulate(g["x"], g["y"], L, cap=CAP)
     assert _rings(nn, nbr) == _rings(onn, onbr)
     assert b.counters(0)["repair_events"] > 0
+
+
+# ------------------------------------------------------------------ line tension
+_GM = np.array([[0.0, 0.2, 0.5], [0.2, 0.0, 0.35], [0.5, 0.35, 0.0]])
+
+
+def _tension_pair(oracle, cg, N, seed, variant, steps=40):
+    """An FP64 system relaxed off its random start, three random cell types, line
+    tension on; returns (batch, oracle) on the same state."""
+    b = cg.VoronoiBatch([N], seeds=[seed], precision="fp64")
+    b.set_prefs(1.0, 3.7)
+    b.set_motility(0.0, 0.0)
+    b.spv_steps(steps, use_graph=False)
+    types = np.random.default_rng(seed).integers(0, 3, N).astype(np.int32)
+    b.set_cell_types(types)
+    if variant == "simple":
+        b.set_tension(1, 0.3)
+    else:
+        b.set_tension_matrix(_GM)
+        b.set_tension(2)
+    o = _oracle_from_gpu(oracle, b, P0=3.7, v0=0.0, Dr=0.0)
+    if variant == "simple":
+        o.set_tension(types, gamma=0.3)
+    else:
+        o.set_tension(types, matrix=_GM)
+    return b, o, types
+
+
+@pytest.mark.parametrize("variant", ["simple", "matrix"])
+def test_tension_forces_match_oracle_fp64(oracle, variant):
+    """VoronoiQuadraticEnergyWithTension force sets (one value / matrix) against
+    the oracle's restatement: forces within 1e-12, energy within 1e-12."""
+    cg = _cg()
+    b, o, _ = _tension_pair(oracle, cg, 400, 31, variant)
+    b.compute_forces()
+    g = b.get()
+    E = o.compute_forces(gpu_style=True)
+    r = o.get()
+    scale = np.abs(np.concatenate([r["Fx"], r["Fy"]])).max()
+    assert np.abs(g["Fx"] - r["Fx"]).max() <= 1e-12 * scale
+    assert np.abs(g["Fy"] - r["Fy"]).max() <= 1e-12 * scale
+    assert abs(b.energy(0) - E) <= 1e-12 * abs(E)
+    # the tension changes the forces
+    b.set_tension(0)
+    b.compute_forces()
+    assert np.abs(b.get()["Fx"] - g["Fx"]).max() > 1e-3 * scale
+
+
+def test_tension_spv_steps_match_oracle_stepwise(oracle):
+    """Self-propelled steps with matrix tension: positions within 1e-12 L and
+    rings bit-exact each step."""
+    cg = _cg()
+    b, _, types = _tension_pair(oracle, cg, 300, 33, "matrix", steps=20)
+    b.set_motility(0.1, 1.0)
+    b.set_params(dt=0.01)
+    b.set_noise_buffer(True)
+    rng = np.random.default_rng(8)
+    for step in range(30):
+        o = _oracle_from_gpu(oracle, b, P0=3.7, v0=0.1, Dr=1.0, dt=0.01)
+        o.set_tension(types, matrix=_GM)
+        eta = rng.standard_normal(300)
+        b.upload_noise(eta)
+        b.spv_steps(1, use_graph=False)
+        b.check_status()
+        o.spp_step(eta, gpu_path=True)
+        g, r = b.get(), o.get()
+        np.testing.assert_allclose(g["x"], r["x"], rtol=0, atol=1e-12 * b.L[0])
+        np.testing.assert_allclose(g["y"], r["y"], rtol=0, atol=1e-12 * b.L[0])
+        nn, nbr, nm = b.topology(CAP)
+        onn, onbr, onm = o.topology(CAP)
+        assert _rings(nn, nbr) == _rings(onn, onbr), "step %d" % step
+
+
+def test_tension_per_system_and_sorting():
+    """In a batch, a system without tension keeps the plain force bit for bit
+    (the tension kernel's other branch); the cell types follow a spatial sort;
+    a type outside the matrix is rejected on the host."""
+    cg = _cg()
+    n = [300, 500]
+    b = cg.VoronoiBatch(n, seeds=[41, 42], precision="fp32")
+    ref = cg.VoronoiBatch(n, seeds=[41, 42], precision="fp32")
+    types = np.random.default_rng(4).integers(0, 3, sum(n)).astype(np.int32)
+    b.set_cell_types(types)
+    b.set_tension(1, 0.4, system=1)
+    b.compute_forces()
+    ref.compute_forces()
+    g, gr = b.get(), ref.get()
+    assert np.array_equal(g["Fx"][:300], gr["Fx"][:300]) and np.array_equal(g["Fy"][:300], gr["Fy"][:300])
+    assert np.abs(g["Fx"][300:] - gr["Fx"][300:]).max() > 1e-3
+    perm = b.spatial_sort()
+    b.compute_forces()
+    g2 = b.get()
+    F1 = np.stack([g["Fx"], g["Fy"]], 1)[perm]
+    F2 = np.stack([g2["Fx"], g2["Fy"]], 1)
+    assert np.abs(F1 - F2).max() <= 1e-5 * np.abs(F1).max()
+    b.set_tension_matrix(_GM)
+    bad = types.copy()
+    bad[310] = 5
+    with pytest.raises(cg.CellGPUError):
+        b.set_cell_types(bad[perm])
+        b.set_tension(2, system=1)
+    b.check_status()

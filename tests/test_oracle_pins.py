@@ -292,3 +292,49 @@ def test_fire_relaxes_perturbed_lattice(oracle):
     f2 = oracle.FireMinimizer(s, max_it=it, force_cutoff=1e-14)
     it2, _ = f2.minimize()
     assert it2 == it
+
+
+@pytest.mark.parametrize("variant", ["simple", "matrix"])
+def test_tension_forces_are_minus_energy_gradient(oracle, variant):
+    """Line tension between cell types (reference VoronoiQuadraticEnergyWithTension
+    force sets): F = -grad(E_AP + sum over cells of gamma times their edges with
+    unlike neighbours), by central differences; with one type everywhere the
+    tension force sets equal the plain ones."""
+    N = 200
+    x, y, th, L = oracle.random_init(N, 22)
+    s = oracle.VoronoiSystem(x, y, th, L, P0=3.7)
+    for _ in range(30):
+        s.spp_step(np.zeros(N))
+    g = s.get()
+    x0, y0 = g["x"].copy(), g["y"].copy()
+    types = np.random.default_rng(6).integers(0, 3, N)
+    s.set_tension(np.zeros(N, np.int32), gamma=0.7)
+    _energy_at(oracle, s, x0, y0)
+    F_one_type = s.get()
+    plain = oracle.VoronoiSystem(x0, y0, th, L, P0=3.7)
+    E_plain = plain.compute_forces(gpu_style=True)
+    Fp = plain.get()
+    assert np.array_equal(F_one_type["Fx"], Fp["Fx"]) and np.array_equal(F_one_type["Fy"], Fp["Fy"])
+    if variant == "simple":
+        s.set_tension(types, gamma=0.3)
+    else:
+        gm = np.array([[0.0, 0.2, 0.5], [0.2, 0.0, 0.35], [0.5, 0.35, 0.0]])
+        s.set_tension(types, matrix=gm)
+    E0 = _energy_at(oracle, s, x0, y0)
+    assert E0 > E_plain  # unlike edges add positive energy
+    F = s.get()
+    assert np.abs(F["Fx"] - Fp["Fx"]).max() > 1e-3  # the tension changes the forces
+    h = 1e-6
+    rng = np.random.default_rng(3)
+    for i in rng.choice(N, 12, replace=False):
+        for comp in ("x", "y"):
+            xp, yp = x0.copy(), y0.copy()
+            (xp if comp == "x" else yp)[i] += h
+            Ep = _energy_at(oracle, s, xp, yp)
+            xm, ym = x0.copy(), y0.copy()
+            (xm if comp == "x" else ym)[i] -= h
+            Em = _energy_at(oracle, s, xm, ym)
+            fd = -(Ep - Em) / (2 * h)
+            an = F["F" + comp][i]
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (variant, i, comp, fd, an)
+    assert abs(F["Fx"].sum()) < 1e-9 and abs(F["Fy"].sum()) < 1e-9
