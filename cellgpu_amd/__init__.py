@@ -83,6 +83,16 @@ class VoronoiBatch:
         e = np.ascontiguousarray(ex, dtype=np.int32)
         check(lib().cgpu_vor_set_exclusions(self.h, _p(e)), "set_exclusions")
 
+    def set_directors(self, theta):
+        t = _f64(theta, self.N)
+        check(lib().cgpu_vor_set_directors(self.h, _p(t)), "set_directors")
+
+    def tags(self):
+        """Label of every cell within its system at creation (spatial sorting permutes cells)."""
+        out = np.zeros(self.N, np.int32)
+        check(lib().cgpu_vor_get_tags(self.h, _p(out)), "get_tags")
+        return out
+
     def set_cell_types(self, types):
         """Cell types (reference Simple2DCell::setCellType); None clears them."""
         self._types = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
@@ -560,6 +570,12 @@ class VertexBatch:
 
     def brownian_steps(self, steps, use_graph=True):
         check(lib().cgpu_vm_brownian_steps(self.h, int(steps), 1 if use_graph else 0), "vm_brownian_steps")
+
+    def set_topology(self, vn, vcn, cvn, cv):
+        """Replace the topology (checked for consistency first); cv is [Nc, cap]."""
+        cv = np.ascontiguousarray(cv, np.int32)
+        a = [np.ascontiguousarray(v, np.int32) for v in (vn, vcn, cvn)]
+        check(lib().cgpu_vm_set_topology(self.h, _p(a[0]), _p(a[1]), _p(a[2]), _p(cv), cv.shape[1]), "vm_set_topology")
 
     def spp_steps(self, steps, use_graph=True):
         """Self-propelled cell-vertex steps (reference selfPropelledCellVertexDynamics)."""

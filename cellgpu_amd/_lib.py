@@ -123,6 +123,21 @@ def lib():
         "cgpu_vm_get_directors": (i, [vp, vp]),
         "cgpu_vm_spp_steps": (i, [vp, i, i]),
         "cgpu_vm_fire_create": (vp, [vp]),
+        "cgpu_vm_set_topology": (i, [vp, vp, vp, vp, vp, i]),
+        "cgpu_vor_set_directors": (i, [vp, vp]),
+        "cgpu_vor_get_tags": (i, [vp, vp]),
+        "cgpu_db_open": (vp, [C.c_char_p, i, i, i, i]),
+        "cgpu_db_close": (None, [vp]),
+        "cgpu_db_last_error": (C.c_char_p, []),
+        "cgpu_db_records": (i, [vp]),
+        "cgpu_db_spv_write": (i, [vp, i, d, d, d] + [vp] * 6),
+        "cgpu_db_spv_read": (i, [vp, i] + [vp] * 8),
+        "cgpu_db_avm_write": (i, [vp, i, d, d, d] + [vp] * 10),
+        "cgpu_db_avm_read": (i, [vp, i] + [vp] * 9),
+        "cgpu_db_spv_write_batch": (i, [vp, vp, i, d, i]),
+        "cgpu_db_spv_read_batch": (i, [vp, vp, i, i]),
+        "cgpu_db_avm_write_batch": (i, [vp, vp, i, d, i]),
+        "cgpu_db_avm_read_batch": (i, [vp, vp, i, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)  # a missing export is a build error: AttributeError

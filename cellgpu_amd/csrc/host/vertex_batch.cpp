@@ -191,6 +191,8 @@ class VertexBatch {
     }
     int num_vertices() const { return Nv_; }
     int num_systems() const { return S_; }
+    int num_cells() const { return Nc_; }
+    const cgpu_vm_params &params(int s) const { return hp_.at(s); }
     size_t real_size() const { return rsz_; }
     hipStream_t stream() const { return stream_; }
     void swap_parity() { parity_ ^= 1; }
@@ -382,6 +384,41 @@ class VertexBatch {
             for (int k = 0; k < n[i] && k < cap; ++k) cv[(size_t)i * cap + k] = c[(size_t)k * Nc_ + i];
         }
     }
+    // replace the topology after checking that it is consistent
+    void set_topology(const int *vn, const int *vcn, const int *cvn, const int *cv, int cap) {
+        if (!vn || !vcn || !cvn || !cv || cap <= 0) throw std::invalid_argument("topology arrays required");
+        std::vector<int> vsys(Nv_), csys(Nc_);
+        vsys_.download(vsys.data(), Nv_);
+        csys_.download(csys.data(), Nc_);
+        long long total = 0;
+        for (int c = 0; c < Nc_; ++c) {
+            if (cvn[c] < 3 || cvn[c] > CGPU_VERTEX_CAP || cvn[c] > cap)
+                throw std::invalid_argument("cell vertex count outside [3, capacity]");
+            total += cvn[c];
+            for (int k = 0; k < cvn[c]; ++k) {
+                const int v = cv[(size_t)c * cap + k];
+                if (v < 0 || v >= Nv_ || vsys[v] != csys[c]) throw std::invalid_argument("cell vertex out of range");
+            }
+        }
+        if (total != 3LL * Nv_) throw std::invalid_argument("cell vertex counts do not sum to 3 Nv");
+        for (int v = 0; v < Nv_; ++v)
+            for (int f = 0; f < 3; ++f) {
+                const int u = vn[3 * v + f], c = vcn[3 * v + f];
+                if (u < 0 || u >= Nv_ || vsys[u] != vsys[v] || c < 0 || c >= Nc_ || csys[c] != vsys[v])
+                    throw std::invalid_argument("vertex neighbour out of range");
+                bool in = false;
+                for (int k = 0; k < cvn[c]; ++k) in = in || cv[(size_t)c * cap + k] == v;
+                if (!in) throw std::invalid_argument("vertex missing from the list of one of its cells");
+            }
+        sync();
+        vn_.upload(vn, 3 * (size_t)Nv_);
+        vcn_.upload(vcn, 3 * (size_t)Nv_);
+        cvn_.upload(cvn, Nc_);
+        std::vector<int> sm((size_t)CGPU_VERTEX_CAP * Nc_, -1);
+        for (int c = 0; c < Nc_; ++c)
+            for (int k = 0; k < cvn[c]; ++k) sm[(size_t)k * Nc_ + c] = cv[(size_t)c * cap + k];
+        cv_.upload(sm.data(), sm.size());
+    }
     void get_counters(int s, long long *out) {
         sync();
         std::vector<cgpu_sys_state> st(S_);
@@ -478,7 +515,11 @@ namespace {
 template <typename F> int vguard(F &&f) {
     try {
         return f();
-    } catch (...) {
+    } catch (const std::invalid_argument &e) {
+        cgpu_host::set_last_error(e.what());
+        return CGPU_ERR_ARGS;
+    } catch (const std::exception &e) {
+        cgpu_host::set_last_error(e.what());
         return CGPU_ERR_HIP;
     }
 }
@@ -535,6 +576,12 @@ int cgpu_vm_get_directors(void *h, double *theta) {
         return CGPU_OK;
     });
 }
+int cgpu_vm_set_topology(void *h, const int *vn, const int *vcn, const int *cvn, const int *cv, int cap) {
+    return vguard([&] {
+        ((VertexBatch *)h)->set_topology(vn, vcn, cvn, cv, cap);
+        return CGPU_OK;
+    });
+}
 int cgpu_vm_spp_steps(void *h, int steps, int use_graph) {
     return vguard([&] { return ((VertexBatch *)h)->run_steps(1, steps, use_graph != 0); });
 }
@@ -558,4 +605,230 @@ int cgpu_vm_get_counters(void *h, int s, long long *out) {
     return vguard([&] { ((VertexBatch *)h)->get_counters(s, out); return 0; });
 }
 int cgpu_vm_status(void *h) { return vguard([&] { return ((VertexBatch *)h)->status(); }); }
+}
+
+// --------------------------------------------------------- database glue
+// AVMDatabaseNetCDF::WriteState / ReadState (reference DatabaseNetCDFAVM.cpp:
+// 192-271, 78-190) for one system of a vertex batch. The file holds vertex
+// positions, forces, Vneighs and VertexCellNeighbors; reading rebuilds every
+// cell's vertex list in counter-clockwise order around the mean of its
+// vertices (as the reference reader does), puts each vertex's three cells in
+// counter-clockwise order and takes Vneighs across from them.
+namespace {
+
+struct SysRange {
+    int voff, nv, coff, nc;
+    double L, dt;
+};
+SysRange sys_range(VertexBatch &vm, int s) {
+    long long c[4];
+    vm.get_counters(s, c);
+    SysRange r{0, (int)c[3], 0, 0, 0.0, 0.0};
+    for (int q = 0; q < s; ++q) {
+        vm.get_counters(q, c);
+        r.voff += (int)c[3];
+    }
+    r.coff = r.voff / 2;
+    r.nc = r.nv / 2;
+    const cgpu_vm_params &p = vm.params(s);
+    r.L = p.L;
+    r.dt = p.dt;
+    return r;
+}
+
+}  // namespace
+
+extern "C" int cgpu_db_avm_write_batch(void *db, void *h, int s, double time, int rec) {
+    return vguard([&] {
+        VertexBatch &vm = *(VertexBatch *)h;
+        if (s < 0 || s >= vm.num_systems()) throw std::invalid_argument("no such system");
+        const SysRange r = sys_range(vm, s);
+        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = CGPU_VERTEX_CAP;
+        std::vector<double> vx(Nv), vy(Nv), fx(Nv), fy(Nv), A(Nc), P(Nc), th(Nc);
+        vm.get(vx.data(), vy.data(), fx.data(), fy.data(), A.data(), P.data());
+        vm.get_directors(th.data());
+        std::vector<int> vn(3 * (size_t)Nv), vcn(3 * (size_t)Nv), cvn(Nc), cv((size_t)Nc * cap);
+        vm.get_topology(vn.data(), vcn.data(), cvn.data(), cv.data(), cap);
+        const cgpu::Box box = cgpu::make_box(r.L);
+        std::vector<int> lvn(3 * (size_t)r.nv), lvcn(3 * (size_t)r.nv);
+        for (int k = 0; k < 3 * r.nv; ++k) {
+            lvn[k] = vn[3 * (size_t)r.voff + k] - r.voff;
+            lvcn[k] = vcn[3 * (size_t)r.voff + k] - r.coff;
+        }
+        // cell position: mean of its vertices (reference getCellPositionsCPU, vertexModelBase.cpp:401-429)
+        std::vector<double> cx(r.nc), cy(r.nc), ct(r.nc, 0.0);
+        std::vector<int> types(r.nc, 0);
+        double q = 0.0;
+        for (int c = 0; c < r.nc; ++c) {
+            const int g = r.coff + c, n = cvn[g];
+            const cgpu::dvec2 base{vx[cv[(size_t)g * cap]], vy[cv[(size_t)g * cap]]};
+            cgpu::dvec2 pos{0.0, 0.0};
+            for (int k = 1; k < n; ++k) {
+                const int v = cv[(size_t)g * cap + k];
+                cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{vx[v], vy[v]}, base);
+                pos.x += d.x;
+                pos.y += d.y;
+            }
+            pos.x = pos.x / n + base.x;
+            pos.y = pos.y / n + base.y;
+            cgpu::put_in_box(box, pos);
+            cx[c] = pos.x;
+            cy[c] = pos.y;
+            q += P[g] / std::sqrt(A[g]);
+        }
+        q /= r.nc;
+        if (time < 0) {
+            long long c[4];
+            vm.get_counters(s, c);
+            time = (double)c[1] * r.dt;
+        }
+        return cgpu_db_avm_write(db, rec, time, r.L, q, vx.data() + r.voff, vy.data() + r.voff, fx.data() + r.voff,
+                                 fy.data() + r.voff, lvn.data(), lvcn.data(), types.data(), th.data() + r.coff,
+                                 cx.data(), cy.data());
+    });
+}
+
+extern "C" int cgpu_db_avm_read_batch(void *db, void *h, int s, int rec) {
+    return vguard([&] {
+        VertexBatch &vm = *(VertexBatch *)h;
+        if (s < 0 || s >= vm.num_systems()) throw std::invalid_argument("no such system");
+        const SysRange r = sys_range(vm, s);
+        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = CGPU_VERTEX_CAP;
+        std::vector<double> lx(r.nv), ly(r.nv), box4(4), ld(r.nc);
+        std::vector<int> lvn(3 * (size_t)r.nv), lvcn(3 * (size_t)r.nv);
+        int rc = cgpu_db_avm_read(db, rec, nullptr, box4.data(), lx.data(), ly.data(), nullptr, nullptr, lvn.data(),
+                                  lvcn.data(), ld.data());
+        if (rc) return rc;
+        if (std::fabs(box4[0] - r.L) > 1e-12 * r.L || box4[1] != 0.0 || box4[2] != 0.0 || box4[3] != box4[0])
+            throw std::invalid_argument("the record's box differs from the system's square box");
+        const cgpu::Box box = cgpu::make_box(r.L);
+        for (int k = 0; k < 3 * r.nv; ++k)
+            if (lvn[k] < 0 || lvn[k] >= r.nv || lvcn[k] < 0 || lvcn[k] >= r.nc)
+                throw std::invalid_argument("record topology out of range");
+        // Topology only (geometry can be locally inverted right after a T1 flip):
+        // 1. one consistent cyclic order of the three cells at every vertex, by
+        //    propagation over edges: where v and u share cells P, Q with P -> Q at
+        //    v, the order at u must be Q -> P (the two ends of an edge see it from
+        //    opposite sides);
+        // 2. each neighbour stored across from the cell that does not contain it;
+        // 3. cell lists by walking polygon successors (the successor of v in its
+        //    cell f is the neighbour stored at f + 1);
+        // 4. the global sense from the total signed area (counter-clockwise > 0).
+        auto nxt = [&](int v, int c) {
+            for (int f = 0; f < 3; ++f)
+                if (lvcn[3 * v + f] == c) return lvcn[3 * v + (f + 1) % 3];
+            throw std::invalid_argument("record topology inconsistent");
+        };
+        auto shared = [&](int v, int u, int &P, int &Q) {
+            int got = 0, sc[2] = {-1, -1};
+            for (int f = 0; f < 3; ++f)
+                for (int g = 0; g < 3; ++g)
+                    if (lvcn[3 * v + f] == lvcn[3 * u + g] && got < 2) sc[got++] = lvcn[3 * v + f];
+            if (got != 2) throw std::invalid_argument("record topology inconsistent");
+            if (nxt(v, sc[0]) == sc[1]) {
+                P = sc[0];
+                Q = sc[1];
+            } else {
+                P = sc[1];
+                Q = sc[0];
+            }
+        };
+        std::vector<char> seen(r.nv, 0);
+        std::vector<int> queue;
+        for (int root = 0; root < r.nv; ++root) {
+            if (seen[root]) continue;
+            seen[root] = 1;
+            queue.assign(1, root);
+            for (size_t h = 0; h < queue.size(); ++h) {
+                const int v = queue[h];
+                for (int k = 0; k < 3; ++k) {
+                    const int u = lvn[3 * v + k];
+                    int P, Q;
+                    shared(v, u, P, Q);
+                    const bool ok = nxt(u, Q) == P;
+                    if (!seen[u]) {
+                        if (!ok) std::swap(lvcn[3 * u + 1], lvcn[3 * u + 2]);
+                        seen[u] = 1;
+                        queue.push_back(u);
+                    } else if (!ok) {
+                        throw std::invalid_argument("record topology is not orientable");
+                    }
+                }
+            }
+        }
+        auto assign_neighbours = [&]() {
+            for (int v = 0; v < r.nv; ++v) {
+                const int n3[3] = {lvn[3 * v], lvn[3 * v + 1], lvn[3 * v + 2]};
+                for (int f = 0; f < 3; ++f) {
+                    int across = -1;
+                    for (int u : n3) {
+                        bool in_u = false;
+                        for (int g = 0; g < 3; ++g) in_u = in_u || lvcn[3 * u + g] == lvcn[3 * v + f];
+                        if (!in_u) across = u;
+                    }
+                    if (across < 0) throw std::invalid_argument("record topology inconsistent");
+                    lvn[3 * v + f] = across;
+                }
+            }
+        };
+        std::vector<int> first(r.nc, INT_MAX), count(r.nc, 0);
+        for (int v = 0; v < r.nv; ++v)
+            for (int f = 0; f < 3; ++f) {
+                const int c = lvcn[3 * v + f];
+                first[c] = std::min(first[c], v);
+                ++count[c];
+            }
+        std::vector<std::vector<int>> cells(r.nc);
+        auto walk = [&]() {
+            double area = 0.0;
+            for (int c = 0; c < r.nc; ++c) {
+                if (count[c] < 3 || count[c] > cap) throw std::invalid_argument("record cell size out of range");
+                cells[c].clear();
+                int v = first[c];
+                const cgpu::dvec2 base{lx[v], ly[v]};
+                cgpu::dvec2 prev{0.0, 0.0};
+                for (int k = 0; k < count[c]; ++k) {
+                    cells[c].push_back(v);
+                    int f = 0;
+                    while (f < 3 && lvcn[3 * v + f] != c) ++f;
+                    if (f == 3) throw std::invalid_argument("record topology inconsistent");
+                    v = lvn[3 * v + (f + 1) % 3];
+                    const cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{lx[v], ly[v]}, base);
+                    area += 0.5 * (prev.x * d.y - prev.y * d.x);
+                    prev = d;
+                }
+                if (v != first[c]) throw std::invalid_argument("record cell polygon does not close");
+            }
+            return area;
+        };
+        assign_neighbours();
+        if (walk() < 0) {  // the propagated sense is clockwise: reverse every vertex
+            for (int v = 0; v < r.nv; ++v) std::swap(lvcn[3 * v + 1], lvcn[3 * v + 2]);
+            assign_neighbours();
+            walk();
+        }
+        // into the batch's full arrays
+        std::vector<double> vx(Nv), vy(Nv), th(Nc);
+        vm.get(vx.data(), vy.data(), nullptr, nullptr, nullptr, nullptr);
+        vm.get_directors(th.data());
+        std::vector<int> vn(3 * (size_t)Nv), vcn(3 * (size_t)Nv), cvn(Nc), cv((size_t)Nc * cap, -1);
+        vm.get_topology(vn.data(), vcn.data(), cvn.data(), cv.data(), cap);
+        for (int v = 0; v < r.nv; ++v) {
+            vx[r.voff + v] = lx[v];
+            vy[r.voff + v] = ly[v];
+            for (int f = 0; f < 3; ++f) {
+                vn[3 * (size_t)(r.voff + v) + f] = lvn[3 * v + f] + r.voff;
+                vcn[3 * (size_t)(r.voff + v) + f] = lvcn[3 * v + f] + r.coff;
+            }
+        }
+        for (int c = 0; c < r.nc; ++c) {
+            th[r.coff + c] = ld[c];
+            cvn[r.coff + c] = (int)cells[c].size();
+            for (size_t k = 0; k < cells[c].size(); ++k) cv[(size_t)(r.coff + c) * cap + k] = cells[c][k] + r.voff;
+        }
+        vm.set_topology(vn.data(), vcn.data(), cvn.data(), cv.data(), cap);
+        vm.set_positions(vx.data(), vy.data());
+        vm.set_directors(th.data());
+        return vm.compute_forces();
+    });
 }

@@ -141,6 +141,9 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
                     sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
                     (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
     cell_sys_.upload(cell_sys_h_.data(), N_);
+    tag_.resize(N_);
+    for (const auto &p : hp_)
+        for (int i = 0; i < p.n; ++i) tag_[p.off + i] = i;
     upload_params();
     std::vector<cgpu_sys_state> st(S_);
     std::memset(st.data(), 0, sizeof(cgpu_sys_state) * S_);
@@ -424,6 +427,7 @@ void VoronoiBatch::set_tension_matrix(int ntypes, const double *g) {
 }
 
 void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
+    sync();  // host copies below must not race the batch stream
     std::vector<double> xy(2 * (size_t)N_);
     for (int i = 0; i < N_; ++i) {
         if (!std::isfinite(x[i]) || !std::isfinite(y[i])) throw std::invalid_argument("non-finite position");
@@ -433,6 +437,11 @@ void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
     double *p = parity_ ? pos1_.p : pos0_.p;
     hip_check(hipMemcpy(p, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice), "set_positions");
     if (retri) global_triangulation();
+}
+
+void VoronoiBatch::set_directors(const double *theta) {
+    sync();
+    real_upload(theta_.p, theta, N_);
 }
 
 void VoronoiBatch::set_noise_mode(bool use_buffer) {
@@ -755,6 +764,11 @@ int VoronoiBatch::spatial_sort(int *perm_out) {
         for (size_t i = 0; i < N; ++i) b[i] = th[perm[i]];
         th.swap(b);
     }
+    {
+        std::vector<int> b(N);
+        for (size_t i = 0; i < N; ++i) b[i] = tag_[perm[i]];
+        tag_.swap(b);
+    }
     (parity_ ? pos1_ : pos0_).upload(xy.data(), xy.size());
     real_upload(theta_.p, th.data(), N);
     real_upload(vel_.p, vel.data(), 2 * N);
@@ -802,6 +816,13 @@ using cgpu_host::VoronoiBatch;
 
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
+
+// message of the last failed object-layer call on this thread (every host
+// module reports through it)
+void cgpu_host::set_last_error(const char *what) { g_last_error = what ? what : ""; }
+
+namespace {
 template <typename F> int guarded(F &&f) {
     try {
         return f();
@@ -851,6 +872,16 @@ int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr) {
 }
 int cgpu_vor_set_exclusions(void *h, const int *ex) {
     return guarded([&] { ((VoronoiBatch *)h)->set_exclusions(ex); return 0; });
+}
+int cgpu_vor_set_directors(void *h, const double *theta) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_directors(theta); return 0; });
+}
+int cgpu_vor_get_tags(void *h, int *tags) {
+    return guarded([&] {
+        const auto &t = ((VoronoiBatch *)h)->tags();
+        std::copy(t.begin(), t.end(), tags);
+        return 0;
+    });
 }
 int cgpu_vor_set_cell_types(void *h, const int *types) {
     return guarded([&] { ((VoronoiBatch *)h)->set_cell_types(types); return 0; });

@@ -18,6 +18,7 @@
 namespace cgpu_host {
 
 void hip_check(hipError_t e, const char *what);
+void set_last_error(const char *what);
 
 template <typename T> struct DevBuf {
     T *p = nullptr;
@@ -69,6 +70,14 @@ class VoronoiBatch {
     void set_tension(int s, int mode, double gamma);
     void set_tension_matrix(int ntypes, const double *g);
     void set_positions(const double *x, const double *y, bool retriangulate);
+    void set_directors(const double *theta);
+    // reference tagToIdx / idxToTag (Simple2DCell.h): tag (label within the
+    // system at creation) of each cell; spatial sorting permutes it
+    const std::vector<int> &tags() const { return tag_; }
+    const std::vector<int> &cell_types() const { return h_type_; }
+    const int *exclusions_device() const { return excl_.p; }
+    const void *ext_force_device() const { return ext_.p; }
+    void real_to_host(const void *src, double *dst, size_t n) { real_download(src, dst, n); }
     void set_noise_mode(bool use_buffer);
     void upload_noise(const double *eta, int count);
     void set_active(const int *active, const int *steps_left);
@@ -129,6 +138,7 @@ class VoronoiBatch {
     DevBuf<unsigned char> AP_, dAP_, APpref_, mot_, vel_, theta_, force_, ext_;
     DevBuf<int> cell_type_;
     std::vector<int> h_type_;  // host copy of the cell types (validation, sorting, energy)
+    std::vector<int> tag_;
     DevBuf<double> tension_;
     int ntypes_ = 0;
     bool tension_on_ = false;  // some system has a tension mode and cell types are set

@@ -323,6 +323,11 @@ int cgpu_vor_set_exclusions(void *h, const int *ex);
  * (setSurfaceTension(double)), 2 the matrix (setSurfaceTension(vector), laid out
  * g[a*ntypes + b] = gamma between types a and b). */
 int cgpu_vor_set_cell_types(void *h, const int *types);
+/* director angles per cell (reference cellDirectors; ReadState restores them) */
+int cgpu_vor_set_directors(void *h, const double *theta);
+/* tag (label within its system at creation) of every cell; spatial sorting
+ * permutes cells, the tags follow them (reference tagToIdx / idxToTag) */
+int cgpu_vor_get_tags(void *h, int *tags);
 int cgpu_vor_set_tension(void *h, int s, int mode, double gamma);
 int cgpu_vor_set_tension_matrix(void *h, int ntypes, const double *g);
 int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retriangulate);
@@ -380,6 +385,47 @@ int cgpu_vm_set_motility(void *h, const double *v0, const double *Dr);    /* per
 int cgpu_vm_set_directors(void *h, const double *theta);                 /* per cell */
 int cgpu_vm_get_directors(void *h, double *theta);
 int cgpu_vm_spp_steps(void *h, int steps, int use_graph);
+/* Replace the topology (vn, vcn [3 Nv]; cvn [Nc]; cv [Nc * cap], counter-clockwise
+ * lists) of the whole batch, e.g. when a trajectory record is read back. The
+ * lists are checked for consistency (every vertex in the lists of its three
+ * cells, cell sizes within the capacity) before anything is uploaded. */
+int cgpu_vm_set_topology(void *h, const int *vn, const int *vcn, const int *cvn, const int *cv, int cap);
+
+/* ------------------------------------------------------ trajectory databases
+ * NetCDF classic (CDF-1) files in the reference's layouts, written by a built-in
+ * encoder (no netCDF library):
+ *   kind 0: SPVDatabaseNetCDF (DatabaseNetCDFSPV.cpp:29-47): dims rec, Nv, dof,
+ *           boxdim, unit; vars time, means0, pos, type, director, BoxMatrix
+ *           [, externalForce]; n = cells;
+ *   kind 1: AVMDatabaseNetCDF (DatabaseNetCDFAVM.cpp:30-52): dims rec, Nv, Nc,
+ *           Nc2, dof, Nvn, boxdim, unit; vars pos, force, VertexCellNeighbors,
+ *           Vneighs, cellType, director, cellPositions, BoxMatrix, meanQ, time;
+ *           n = vertices.
+ * mode (NcFile::FileMode): 0 replace, 1 write (append), 2 read only, 3 new.
+ * Arrays are host arrays in tag order; rec < 0 appends. */
+void *cgpu_db_open(const char *path, int kind, int n, int mode, int exclusions);
+void cgpu_db_close(void *db);
+const char *cgpu_db_last_error(void);
+int cgpu_db_records(void *db);
+int cgpu_db_spv_write(void *db, int rec, double time, double L, double meanq, const double *x, const double *y,
+                      const double *director, const int *type, const double *ext_x, const double *ext_y);
+int cgpu_db_spv_read(void *db, int rec, double *time, double *box4, double *x, double *y, double *director, int *type,
+                     double *ext_x, double *ext_y);
+int cgpu_db_avm_write(void *db, int rec, double time, double L, double meanq, const double *vx, const double *vy,
+                      const double *fx, const double *fy, const int *vn, const int *vcn, const int *cell_type,
+                      const double *director, const double *cell_x, const double *cell_y);
+int cgpu_db_avm_read(void *db, int rec, double *time, double *box4, double *vx, double *vy, double *fx, double *fy,
+                     int *vn, int *vcn, double *director);
+/* One system of a batch to / from a record (reference WriteState / ReadState):
+ * Voronoi cells in tag order with type - 5 marking excluded cells and means0 =
+ * mean P / sqrt(A); reading sets positions, directors and types and
+ * retriangulates. Vertex model: reading rebuilds the cell vertex lists
+ * counter-clockwise around each cell's mean vertex position and the vertex
+ * neighbour order from the positions. time < 0: step counter x dt. */
+int cgpu_db_spv_write_batch(void *db, void *vor, int s, double time, int rec);
+int cgpu_db_spv_read_batch(void *db, void *vor, int s, int rec);
+int cgpu_db_avm_write_batch(void *db, void *vm, int s, double time, int rec);
+int cgpu_db_avm_read_batch(void *db, void *vm, int s, int rec);
 int cgpu_vm_get(void *h, double *vx, double *vy, double *Fx, double *Fy, double *A, double *P);
 int cgpu_vm_get_topology(void *h, int *vn, int *vcn, int *cvn, int *cv, int cap);
 int cgpu_vm_get_counters(void *h, int s, long long *out);
