@@ -3,13 +3,16 @@
 // -e dt, -p p0, -a a0, -v v0, -d Dr; -z 1 selects self-propelled cell-vertex
 // dynamics instead of Brownian vertices at T = v0; -g is accepted and must name
 // a GPU). Cells start from 100 smoothing Voronoi steps (runSPV), the T1
-// threshold is 0.1, as in the reference driver. Written against the C++ drop-in
-// layer.
+// threshold is 0.1, as in the reference driver. -z -1 also saves the state to
+// ./data/test_p<p0>.nc (AVM NetCDF layout) at the start, after the initial
+// steps and every 100/dt steps. Written against the C++ drop-in layer.
 #include <getopt.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <memory>
+#include <string>
 
 #include "../../../include/cellgpu_amd.hpp"
 
@@ -53,13 +56,25 @@ int main(int argc, char *argv[]) {
     sim->setIntegrationTimestep(dt);
     sim->setCPUOperation(false);
     sim->setReproducible(true);
+    std::unique_ptr<AVMDatabaseNetCDF> ncdat;
+    if (program_switch < 0) {
+        char dataname[256];
+        snprintf(dataname, sizeof(dataname), "./data/test_p%.3f.nc", p0);
+        ncdat.reset(new AVMDatabaseNetCDF(avm->Nvertices, dataname, NcFile::Replace));
+        ncdat->WriteState(avm);
+    }
+    const int saveEvery = std::max(1, (int)(100 / dt));
     double2v mf = avm->reportMeanVertexForce();
     printf("mean vertex force = (%g, %g)\n", mf.x, mf.y);
     for (int ii = 0; ii < initSteps + 1; ++ii) sim->performTimestep();
     mf = avm->reportMeanVertexForce();
     printf("mean vertex force = (%g, %g)\n", mf.x, mf.y);
+    if (ncdat) ncdat->WriteState(avm);
     auto t1 = std::chrono::steady_clock::now();
-    for (int ii = 0; ii < tSteps; ++ii) sim->performTimestep();
+    for (int ii = 0; ii < tSteps; ++ii) {
+        sim->performTimestep();
+        if (ncdat && ii % saveEvery == 0) ncdat->WriteState(avm);
+    }
     cgpu_device_synchronize();
     auto t2 = std::chrono::steady_clock::now();
     printf("timestep time per iteration currently at %g\n\n",

@@ -410,6 +410,63 @@ class Simulation {
     vector<shared_ptr<updater>> updaters;
 };
 
+//! reference NcFile::FileMode values used by the databases
+namespace NcFile {
+enum FileMode { Replace = 0, Write = 1, ReadOnly = 2, New = 3 };
+}
+
+//! reference SPVDatabaseNetCDF (DatabaseNetCDFSPV.h): NetCDF classic file,
+//! one record per WriteState
+class SPVDatabaseNetCDF {
+   public:
+    SPVDatabaseNetCDF(int np, const std::string &fn, int mode = NcFile::ReadOnly, bool exclude = false)
+        : filename(fn), Mode(mode) {
+        h_ = cgpu_db_open(fn.c_str(), 0, np, mode, exclude ? 1 : 0);
+        if (!h_) throw std::runtime_error(std::string("SPVDatabaseNetCDF: ") + cgpu_db_last_error());
+    }
+    ~SPVDatabaseNetCDF() { cgpu_db_close(h_); }
+    SPVDatabaseNetCDF(const SPVDatabaseNetCDF &) = delete;
+    SPVDatabaseNetCDF &operator=(const SPVDatabaseNetCDF &) = delete;
+    void WriteState(shared_ptr<VoronoiQuadraticEnergy> s, double time = -1.0, int rec = -1) {
+        check(cgpu_db_spv_write_batch(h_, s->handle(), 0, time, rec), "WriteState");
+    }
+    void ReadState(shared_ptr<VoronoiQuadraticEnergy> s, int rec, bool geometry = true) {
+        (void)geometry;  // positions are always retriangulated
+        check(cgpu_db_spv_read_batch(h_, s->handle(), 0, rec), "ReadState");
+    }
+    int Records() const { return cgpu_db_records(h_); }
+    std::string filename;
+    const int Mode;
+
+   private:
+    void *h_;
+};
+
+//! reference AVMDatabaseNetCDF (DatabaseNetCDFAVM.h)
+class AVMDatabaseNetCDF {
+   public:
+    AVMDatabaseNetCDF(int np, const std::string &fn, int mode = NcFile::ReadOnly) : filename(fn), Mode(mode) {
+        h_ = cgpu_db_open(fn.c_str(), 1, np, mode, 0);
+        if (!h_) throw std::runtime_error(std::string("AVMDatabaseNetCDF: ") + cgpu_db_last_error());
+    }
+    ~AVMDatabaseNetCDF() { cgpu_db_close(h_); }
+    AVMDatabaseNetCDF(const AVMDatabaseNetCDF &) = delete;
+    AVMDatabaseNetCDF &operator=(const AVMDatabaseNetCDF &) = delete;
+    void WriteState(shared_ptr<VertexQuadraticEnergy> s, double time = -1.0, int rec = -1) {
+        check(cgpu_db_avm_write_batch(h_, s->handle(), 0, time, rec), "WriteState");
+    }
+    void ReadState(shared_ptr<VertexQuadraticEnergy> s, int rec, bool geometry = true) {
+        (void)geometry;
+        check(cgpu_db_avm_read_batch(h_, s->handle(), 0, rec), "ReadState");
+    }
+    int Records() const { return cgpu_db_records(h_); }
+    std::string filename;
+    const int Mode;
+
+   private:
+    void *h_;
+};
+
 typedef shared_ptr<Simple2DModel> ForcePtr;
 typedef shared_ptr<simpleEquationOfMotion> EOMPtr;
 typedef shared_ptr<updater> UpdaterPtr;
