@@ -59,15 +59,42 @@ def make_batch(cg, n, P0, v0, Dr, seeds, precision):
     return b
 
 
-def force_error(cg, n, P0, v0, Dr, seeds, b32):
+def system_energies(g, n, P0):
+    """E_s = sum over the system's cells of (A - 1)^2 + (P - P0)^2 (KA = KP = 1)."""
+    e = (g["A"] - 1.0) ** 2 + (g["P"] - per_cell(n, P0)) ** 2
+    return np.add.reduceat(e, np.concatenate([[0], np.cumsum(n)[:-1]]))
+
+
+def fp32_errors(cg, n, P0, v0, Dr, seeds, b32, steps=100):
+    """FP32-bulk vs FP64 at the same state (max-norm force error relative to the
+    largest force; largest relative system energy error), and after `steps`
+    steps from the same start with the same noise: the largest and RMS
+    minimum-image position difference (unit density: in mean cell spacings)."""
     b64 = make_batch(cg, n, P0, v0, Dr, seeds, "fp64")
     b64.compute_forces()
     b32.compute_forces()
     g64, g32 = b64.get(), b32.get()
-    b64.close()
     F64 = np.concatenate([g64["Fx"], g64["Fy"]])
     F32 = np.concatenate([g32["Fx"], g32["Fy"]])
-    return float(np.abs(F32 - F64).max() / np.abs(F64).max())
+    ferr = float(np.abs(F32 - F64).max() / np.abs(F64).max())
+    E64, E32 = system_energies(g64, n, P0), system_energies(g32, n, P0)
+    eerr = float(np.max(np.abs(E32 - E64) / np.abs(E64)))
+    t32 = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
+    t32.spv_steps(steps, use_graph=True)
+    b64.spv_steps(steps, use_graph=True)
+    t32.check_status()
+    b64.check_status()
+    a, c = t32.get(), b64.get()
+    L = np.repeat(np.sqrt(n.astype(np.float64)), n)
+    dx = a["x"] - c["x"]
+    dy = a["y"] - c["y"]
+    dx -= L * np.round(dx / L)
+    dy -= L * np.round(dy / L)
+    d = np.hypot(dx, dy)
+    t32.close()
+    b64.close()
+    return ferr, eerr, {"steps": steps, "max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
+                        "unit": "mean cell spacing"}
 
 
 def plain_fp64(cg, n, P0, v0, Dr, seeds, sample_every=8, steps=20):
@@ -196,6 +223,34 @@ def other_workload(cg, name, steps, warmup):
                 "unit": "cell-steps/s", "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * el / steps, 5),
                 "dtype": "f32", "t1_events": b.counters(0)["t1_events"],
                 "config": {"workload": "vertex Nc=32768 Nv=65536 T=1e-3 dt=0.005 thr=0.04 P0=3.9"}}
+    if name == "ensemble_large":
+        # the same ensemble distribution with 8x the systems (about 6.7 million
+        # cells, several GB of device state): beyond the 256 MB Infinity Cache
+        n, P0, v0, Dr, seeds = ensemble(S=4096, master_seed=20240608)
+        N = int(n.sum())
+        b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
+        b.spv_steps(warmup)
+        b.check_status()
+        t0 = time.perf_counter()
+        b.spv_steps(steps)
+        b.sync()
+        el = time.perf_counter() - t0
+        b.check_status()
+        import ctypes as C
+        ks = 50
+        ms = np.zeros(4)
+        cg.check(cg.lib().cgpu_vor_spv_steps_timed(b.h, ks, ms.ctypes.data_as(C.c_void_p)), "timed steps")
+        avg = ms / ks
+        dev_bytes = int(cg.lib().cgpu_vor_device_bytes(b.h))
+        return {"metric": "cell-steps/s, batched SPV ensemble of 4096 systems (device state beyond the Infinity Cache)",
+                "value": round(N * steps / el, 1), "unit": "cell-steps/s", "steps": steps, "warmup": warmup,
+                "ms_per_step": round(1e3 * el / steps, 5), "dtype": "f32", "device_bytes": dev_bytes,
+                "stage_ms": {"force_eom_move": round(avg[0], 5), "cell_list": round(avg[1], 5),
+                             "test_geometry": round(avg[2], 5), "repair_chain": round(avg[3], 5)},
+                "test_geometry_achieved_gbs": round(BYTES_TEST_GEOMETRY * N / (avg[2] * 1e-3) / 1e9, 1),
+                "force_eom_move_achieved_gbs": round(BYTES_FORCE_EOM_MOVE * N / (avg[0] * 1e-3) / 1e9, 1),
+                "config": {"workload": "4096 systems, N log-uniform [64,8192] (%d cells), unit density, "
+                                       "P0~U[3.5,4], v0~U[0.01,0.2], Dr~U[0.1,10], dt=0.01" % N}}
     if name == "vertex_spp":
         # self-propelled cell-vertex dynamics (the reference Vertex.cpp active path):
         # Nc = 32 768, v0 = 0.05, Dr = 1, dt = 0.01, T1 threshold 0.04, P0 = 3.9
@@ -264,7 +319,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "single", "fire", "vertex", "vertex_spp", "vertex_fire"],
+    ap.add_argument("--workload", default="ensemble", choices=["ensemble", "ensemble_large", "single", "fire", "vertex", "vertex_spp",
+                                                        "vertex_fire"],
                     help="ensemble = the headline; the others print their own line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
@@ -285,7 +341,7 @@ def main():
     n, P0, v0, Dr, seeds = ensemble()
     N = int(n.sum())
     b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
-    ferr = force_error(cg, n, P0, v0, Dr, seeds, b)
+    ferr, eerr, traj = fp32_errors(cg, n, P0, v0, Dr, seeds, b)
     print("[bench] ensemble: %d systems, %d cells; fp32-vs-fp64 force error %.2e" % (len(n), N, ferr), flush=True)
 
     b.spv_steps(args.warmup, use_graph=True)
@@ -368,6 +424,8 @@ def main():
         "plain_fp64_sample": plain_desc,
         "ratio_vs_plain_fp64": None if plain_val is None else round(value / plain_val, 2),
         "force_rel_err_vs_fp64": ferr,
+        "energy_rel_err_vs_fp64": eerr,
+        "trajectory_err_vs_fp64": traj,
         "stage_ms": {k: round(v, 5) for k, v in stage.items()},
         "stage_achieved_gbs": others,
         "latency_ms": lat,
