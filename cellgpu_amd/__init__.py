@@ -192,6 +192,23 @@ class VoronoiBatch:
     def system_slice(self, s):
         return slice(int(self.offsets[s]), int(self.offsets[s + 1]))
 
+    def cell_events(self, kinds, cells, theta=None, frac=None, new_director=None):
+        """Deaths (kind 0) and divisions (kind 1) of batch cells, at most one per
+        system (reference voronoiModelBase::cellDeath / cellDivision); the batch is
+        laid out again and retriangulated."""
+        k = np.ascontiguousarray(np.atleast_1d(kinds), np.int32)
+        c = np.ascontiguousarray(np.atleast_1d(cells), np.int32)
+        m = len(k)
+        t = _f64(np.zeros(m) if theta is None else np.atleast_1d(theta), m)
+        f = _f64(np.zeros(m) if frac is None else np.atleast_1d(frac), m)
+        d = None if new_director is None else _f64(np.atleast_1d(new_director), m)
+        check(lib().cgpu_vor_cell_events(self.h, m, _p(k), _p(c), _p(t), _p(f), _p(d)), "cell_events")
+        n = np.zeros(self.S, np.int32)
+        lib().cgpu_vor_get_sizes(self.h, _p(n))
+        self.n = n
+        self.offsets = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+        self.N = int(self.offsets[-1])
+
 
 # ----------------------------------------------------------------------------
 # Reference-named single-system API (abaso/cellGPU inc/models, inc/updaters,
@@ -263,6 +280,17 @@ class VoronoiQuadraticEnergy:
     def setCellType(self, types):
         """reference Simple2DCell::setCellType."""
         self.batch.set_cell_types(types)
+
+    def cellDeath(self, cellIndex):
+        """reference voronoiModelBase::cellDeath (voronoiModelBase.cpp:1309-1316)."""
+        self.batch.cell_events([0], [cellIndex])
+        self.Ncells = self.batch.N
+
+    def cellDivision(self, parameters, dParams):
+        """reference voronoiModelBase::cellDivision: parameters = [cell], dParams =
+        [theta, separation fraction]; the new cell is the last one."""
+        self.batch.cell_events([1], [parameters[0]], [dParams[0]], [dParams[1]])
+        self.Ncells = self.batch.N
 
 
 class VoronoiQuadraticEnergyWithTension(VoronoiQuadraticEnergy):

@@ -126,6 +126,8 @@ def lib():
         "cgpu_vm_set_topology": (i, [vp, vp, vp, vp, vp, i]),
         "cgpu_vor_set_directors": (i, [vp, vp]),
         "cgpu_vor_get_tags": (i, [vp, vp]),
+        "cgpu_vor_cell_events": (i, [vp, i, vp, vp, vp, vp, vp]),
+        "cgpu_vor_get_sizes": (i, [vp, vp]),
         "cgpu_db_open": (vp, [C.c_char_p, i, i, i, i]),
         "cgpu_db_close": (None, [vp]),
         "cgpu_db_last_error": (C.c_char_p, []),

@@ -36,14 +36,10 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     hip_check(hipSetDevice(device_), "hipSetDevice");
     rsz_ = prec_ ? sizeof(float) : sizeof(double);
     hp_.resize(S_);
-    int off = 0, boff = 0;
     for (int s = 0; s < S_; ++s) {
         if (n[s] < 4) throw std::invalid_argument("each system needs at least 4 cells");
         cgpu_sys_params &p = hp_[s];
         std::memset(&p, 0, sizeof(p));
-        p.off = off;
-        p.n = n[s];
-        p.ncells = n[s];
         p.L = L ? L[s] : std::sqrt((double)n[s]);
         // reference cellListGPU::setGridSize(1.25): even number of bins per side
         int xs = (int)std::floor(p.L / 1.25);
@@ -51,100 +47,22 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
         if (xs < 2) xs = 2;
         p.xsize = xs;
         p.boxsize = p.L / xs;
-        p.bin_off = boff;
         p.KA = 1.0;
         p.KP = 1.0;
         p.dt = 0.01;
         p.mu = 1.0;
         p.T = 0.0;
         p.seed = seed ? seed[s] : 13377ull;
-        if ((long long)off + n[s] >= (1LL << 26))
-            throw std::invalid_argument("a batch holds fewer than 2^26 cells (queue entries pack the cell index)");
-        off += n[s];
-        boff += xs * xs;
     }
-    N_ = off;
-    total_bins_ = boff;
-    cell_sys_h_.resize(N_);
-    for (int s = 0; s < S_; ++s)
-        for (int i = 0; i < hp_[s].n; ++i) cell_sys_h_[hp_[s].off + i] = s;
-
+    std::vector<int> ns(n, n + S_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-    {
-        // work chunks inside systems (cells and cell-list bins)
-        std::vector<int> cs, cb, ce, bs, bb, be, sb;
-        for (int s = 0; s < S_; ++s) {
-            const cgpu_sys_params &p = hp_[s];
-            for (int c = p.off; c < p.off + p.n; c += CGPU_CHUNK) {
-                cs.push_back(s);
-                cb.push_back(c);
-                ce.push_back(std::min(c + CGPU_CHUNK, p.off + p.n));
-            }
-            sb.push_back((int)bs.size());
-            const int nb = p.xsize * p.xsize;
-            for (int u = 0; u < nb; u += CGPU_BIN_CHUNK) {
-                bs.push_back(s);
-                bb.push_back(p.bin_off + u);
-                be.push_back(p.bin_off + std::min(u + CGPU_BIN_CHUNK, nb));
-            }
-        }
-        sb.push_back((int)bs.size());
-        num_chunks_ = (int)cs.size();
-        num_bin_chunks_ = (int)bs.size();
-        chunk_sys_.alloc(cs.size());
-        chunk_sys_.upload(cs.data(), cs.size());
-        chunk_begin_.alloc(cb.size());
-        chunk_begin_.upload(cb.data(), cb.size());
-        chunk_end_.alloc(ce.size());
-        chunk_end_.upload(ce.data(), ce.size());
-        bchunk_sys_.alloc(bs.size());
-        bchunk_sys_.upload(bs.data(), bs.size());
-        bchunk_begin_.alloc(bb.size());
-        bchunk_begin_.upload(bb.data(), bb.size());
-        bchunk_end_.alloc(be.size());
-        bchunk_end_.upload(be.data(), be.size());
-        sys_bchunk_.alloc(sb.size());
-        sys_bchunk_.upload(sb.data(), sb.size());
-        bchunk_sum_.alloc(bs.size());
-    }
     sys_.alloc(S_);
     st_.alloc(S_);
-    cell_sys_.alloc(N_);
-    nn_.alloc(N_);
-    nbr_.alloc((size_t)CGPU_RING_CAP * N_);
-    oth_.alloc((size_t)CGPU_RING_CAP * N_);
-    repair_.alloc(N_);
-    mark_.alloc(N_);
-    flagged_.alloc(N_);
-    nfq_.alloc(N_);
-    gq_.alloc(N_);
     counters_.alloc(8);
-    bin_count_.alloc(total_bins_);
-    bin_start_.alloc((size_t)total_bins_ + 1);
-    cell_bin_.alloc(N_);
-    cell_rank_.alloc(N_);
-    sorted_id_.alloc(N_);
-    sorted_pos_.alloc(2 * (size_t)N_);
-    pos0_.alloc(2 * (size_t)N_);
-    pos1_.alloc(2 * (size_t)N_);
-    AP_.alloc(2 * N_ * rsz_);
-    dAP_.alloc(2 * N_ * rsz_);
-    APpref_.alloc(2 * N_ * rsz_);
-    mot_.alloc(2 * N_ * rsz_);
-    vel_.alloc(2 * N_ * rsz_);
-    theta_.alloc(N_ * rsz_);
-    force_.alloc(2 * N_ * rsz_);
-    device_bytes_ = sys_.n * sizeof(cgpu_sys_params) + st_.n * sizeof(cgpu_sys_state) +
-                    sizeof(int) * (cell_sys_.n + nn_.n + nbr_.n + oth_.n + repair_.n + mark_.n + flagged_.n +
-                                   nfq_.n + gq_.n + counters_.n + bin_start_.n + cell_bin_.n + cell_rank_.n +
-                                   sorted_id_.n + bin_count_.n) +
-                    sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
-                    (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
-    cell_sys_.upload(cell_sys_h_.data(), N_);
+    layout(ns);
     tag_.resize(N_);
     for (const auto &p : hp_)
         for (int i = 0; i < p.n; ++i) tag_[p.off + i] = i;
-    upload_params();
     std::vector<cgpu_sys_state> st(S_);
     std::memset(st.data(), 0, sizeof(cgpu_sys_state) * S_);
     for (auto &q : st) {
@@ -200,6 +118,103 @@ VoronoiBatch::~VoronoiBatch() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Offsets, cell-list bins, work chunks and every per-cell device array for
+// system sizes n (the per-system constants in hp_ are kept). Arrays start zeroed.
+void VoronoiBatch::layout(const std::vector<int> &n) {
+    int off = 0, boff = 0;
+    for (int s = 0; s < S_; ++s) {
+        cgpu_sys_params &p = hp_[s];
+        if (n[s] < 4) throw std::invalid_argument("each system needs at least 4 cells");
+        if ((long long)off + n[s] >= (1LL << 26))
+            throw std::invalid_argument("a batch holds fewer than 2^26 cells (queue entries pack the cell index)");
+        p.off = off;
+        p.n = n[s];
+        p.ncells = n[s];
+        p.bin_off = boff;
+        off += n[s];
+        boff += p.xsize * p.xsize;
+    }
+    N_ = off;
+    total_bins_ = boff;
+    cell_sys_h_.assign(N_, 0);
+    for (int s = 0; s < S_; ++s)
+        for (int i = 0; i < hp_[s].n; ++i) cell_sys_h_[hp_[s].off + i] = s;
+    {
+        // work chunks inside systems (cells and cell-list bins)
+        std::vector<int> cs, cb, ce, bs, bb, be, sb;
+        for (int s = 0; s < S_; ++s) {
+            const cgpu_sys_params &p = hp_[s];
+            for (int c = p.off; c < p.off + p.n; c += CGPU_CHUNK) {
+                cs.push_back(s);
+                cb.push_back(c);
+                ce.push_back(std::min(c + CGPU_CHUNK, p.off + p.n));
+            }
+            sb.push_back((int)bs.size());
+            const int nb = p.xsize * p.xsize;
+            for (int u = 0; u < nb; u += CGPU_BIN_CHUNK) {
+                bs.push_back(s);
+                bb.push_back(p.bin_off + u);
+                be.push_back(p.bin_off + std::min(u + CGPU_BIN_CHUNK, nb));
+            }
+        }
+        sb.push_back((int)bs.size());
+        num_chunks_ = (int)cs.size();
+        num_bin_chunks_ = (int)bs.size();
+        chunk_sys_.alloc(cs.size());
+        chunk_sys_.upload(cs.data(), cs.size());
+        chunk_begin_.alloc(cb.size());
+        chunk_begin_.upload(cb.data(), cb.size());
+        chunk_end_.alloc(ce.size());
+        chunk_end_.upload(ce.data(), ce.size());
+        bchunk_sys_.alloc(bs.size());
+        bchunk_sys_.upload(bs.data(), bs.size());
+        bchunk_begin_.alloc(bb.size());
+        bchunk_begin_.upload(bb.data(), bb.size());
+        bchunk_end_.alloc(be.size());
+        bchunk_end_.upload(be.data(), be.size());
+        sys_bchunk_.alloc(sb.size());
+        sys_bchunk_.upload(sb.data(), sb.size());
+        bchunk_sum_.alloc(bs.size());
+    }
+    cell_sys_.alloc(N_);
+    nn_.alloc(N_);
+    nbr_.alloc((size_t)CGPU_RING_CAP * N_);
+    oth_.alloc((size_t)CGPU_RING_CAP * N_);
+    repair_.alloc(N_);
+    mark_.alloc(N_);
+    flagged_.alloc(N_);
+    nfq_.alloc(N_);
+    gq_.alloc(N_);
+    bin_count_.alloc(total_bins_);
+    bin_start_.alloc((size_t)total_bins_ + 1);
+    cell_bin_.alloc(N_);
+    cell_rank_.alloc(N_);
+    sorted_id_.alloc(N_);
+    sorted_pos_.alloc(2 * (size_t)N_);
+    pos0_.alloc(2 * (size_t)N_);
+    pos1_.alloc(2 * (size_t)N_);
+    AP_.alloc(2 * N_ * rsz_);
+    dAP_.alloc(2 * N_ * rsz_);
+    APpref_.alloc(2 * N_ * rsz_);
+    mot_.alloc(2 * N_ * rsz_);
+    vel_.alloc(2 * N_ * rsz_);
+    theta_.alloc(N_ * rsz_);
+    force_.alloc(2 * N_ * rsz_);
+    device_bytes_ = sys_.n * sizeof(cgpu_sys_params) + st_.n * sizeof(cgpu_sys_state) +
+                    sizeof(int) * (cell_sys_.n + nn_.n + nbr_.n + oth_.n + repair_.n + mark_.n + flagged_.n +
+                                   nfq_.n + gq_.n + counters_.n + bin_start_.n + cell_bin_.n + cell_rank_.n +
+                                   sorted_id_.n + bin_count_.n) +
+                    sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
+                    (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
+    cell_sys_.upload(cell_sys_h_.data(), N_);
+    upload_params();
+    parity_ = 0;
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
 }
 
 void VoronoiBatch::upload_params() { sys_.upload(hp_.data(), S_); }
@@ -876,6 +891,19 @@ int cgpu_vor_set_exclusions(void *h, const int *ex) {
 int cgpu_vor_set_directors(void *h, const double *theta) {
     return guarded([&] { ((VoronoiBatch *)h)->set_directors(theta); return 0; });
 }
+int cgpu_vor_cell_events(void *h, int count, const int *kind, const int *cell, const double *theta,
+                         const double *frac, const double *new_director) {
+    return guarded([&] {
+        int r = ((VoronoiBatch *)h)->cell_events(count, kind, cell, theta, frac, new_director);
+        return r ? r : ((VoronoiBatch *)h)->status();
+    });
+}
+int cgpu_vor_get_sizes(void *h, int *n) {
+    const VoronoiBatch &b = *(VoronoiBatch *)h;
+    if (n)
+        for (int s = 0; s < b.num_systems(); ++s) n[s] = b.params()[s].n;
+    return b.num_cells();
+}
 int cgpu_vor_get_tags(void *h, int *tags) {
     return guarded([&] {
         const auto &t = ((VoronoiBatch *)h)->tags();
@@ -974,3 +1002,173 @@ extern "C" void cgpu_put_in_box(double L, double *x, double *y) {
     *x = p.x;
     *y = p.y;
 }
+
+namespace cgpu_host {
+
+// Cell death and division (reference voronoiModelBase::cellDeath / cellDivision,
+// voronoiModelBase.cpp:1291-1422, with the bookkeeping of Simple2DCell.cpp:775-850
+// and Simple2DActiveCell.cpp:118-147). The state of every cell is gathered on the
+// host, the events edit each system's segment (a dead cell is removed and the
+// cells after it move down, tags above its tag drop by one; a daughter is
+// appended with the next tag), the batch is laid out again and retriangulated.
+int VoronoiBatch::cell_events(int count, const int *kind, const int *cell, const double *theta, const double *frac,
+                              const double *new_director) {
+    if (count <= 0) return CGPU_OK;
+    if (!kind || !cell) throw std::invalid_argument("cell events need kinds and cells");
+    std::vector<int> ev_of_sys(S_, -1);
+    for (int e = 0; e < count; ++e) {
+        if (cell[e] < 0 || cell[e] >= N_) throw std::invalid_argument("cell index out of range");
+        if (kind[e] != 0 && kind[e] != 1) throw std::invalid_argument("event kind is 0 (death) or 1 (division)");
+        const int s = cell_sys_h_[cell[e]];
+        if (ev_of_sys[s] >= 0) throw std::invalid_argument("at most one event per system per call");
+        ev_of_sys[s] = e;
+        if (kind[e] == 0 && hp_[s].n <= 4) throw std::invalid_argument("a system keeps at least 4 cells");
+        if (kind[e] == 1 && (!theta || !frac || !std::isfinite(theta[e]) || !std::isfinite(frac[e])))
+            throw std::invalid_argument("a division needs a finite angle and separation fraction");
+    }
+    sync();
+    const size_t N = N_;
+    std::vector<double> xy(2 * N), th(N), vel(2 * N), pref(2 * N), mot(2 * N);
+    (parity_ ? pos1_ : pos0_).download(xy.data(), 2 * N);
+    real_download(theta_.p, th.data(), N);
+    real_download(vel_.p, vel.data(), 2 * N);
+    real_download(APpref_.p, pref.data(), 2 * N);
+    real_download(mot_.p, mot.data(), 2 * N);
+    std::vector<int> ex;
+    if (excl_.p) {
+        ex.resize(N);
+        excl_.download(ex.data(), N);
+    }
+    const int cap = CGPU_RING_CAP;
+    std::vector<int> nn(N), nbr(N * cap);
+    get_topology(nn.data(), nbr.data(), cap, nullptr);
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+
+    // new order: old index of each new cell (-1: the daughter of parent[.])
+    std::vector<int> src, parent, newn(S_), newtag;
+    std::vector<double> nxy, nth, nvel, npref, nmot;
+    std::vector<int> nex, ntype;
+    auto push = [&](int i, int tag) {
+        src.push_back(i);
+        nxy.push_back(xy[2 * i]);
+        nxy.push_back(xy[2 * i + 1]);
+        nth.push_back(th[i]);
+        nvel.push_back(vel[2 * i]);
+        nvel.push_back(vel[2 * i + 1]);
+        npref.push_back(pref[2 * i]);
+        npref.push_back(pref[2 * i + 1]);
+        nmot.push_back(mot[2 * i]);
+        nmot.push_back(mot[2 * i + 1]);
+        if (!ex.empty()) nex.push_back(ex[i]);
+        if (!h_type_.empty()) ntype.push_back(h_type_[i]);
+        newtag.push_back(tag);
+    };
+    for (int s = 0; s < S_; ++s) {
+        const cgpu_sys_params &p = hp_[s];
+        const int e = ev_of_sys[s];
+        const bool death = e >= 0 && kind[e] == 0, division = e >= 0 && kind[e] == 1;
+        const int target = e >= 0 ? cell[e] : -1;
+        const int dead_tag = death ? tag_[target] : -1;
+        const size_t start = src.size();
+        for (int i = p.off; i < p.off + p.n; ++i) {
+            if (death && i == target) continue;
+            const int t = tag_[i];
+            push(i, (death && t > dead_tag) ? t - 1 : t);
+        }
+        if (division) {
+            // the cell's Voronoi polygon in the reference's CPU vertex order:
+            // vertex k = circumcenter(ring k-1, ring k)
+            const cgpu::Box box = cgpu::make_box(p.L);
+            const int z = nn[target];
+            const cgpu::dvec2 pi{xy[2 * target], xy[2 * target + 1]};
+            auto rel = [&](int k) {
+                const int j = nbr[(size_t)target * cap + k];
+                return cgpu::min_dist(box, cgpu::dvec2{xy[2 * j], xy[2 * j + 1]}, pi);
+            };
+            std::vector<cgpu::dvec2> voro(z);
+            cgpu::dvec2 rij = rel(z - 1);
+            for (int k = 0; k < z; ++k) {
+                const cgpu::dvec2 rik = rel(k);
+                voro[k] = cgpu::circ0<double>(rij, rik);
+                rij = rik;
+            }
+            // where the axis through the cell at angle theta crosses the polygon
+            const double t = theta[e];
+            const cgpu::dvec2 ray{std::cos(t), std::sin(t)}, c{-std::sin(t), std::cos(t)};
+            cgpu::dvec2 v1 = voro[z - 1], int1{0, 0}, int2{0, 0};
+            bool found = false, second = false;
+            for (int k = 0; k < z; ++k) {
+                const cgpu::dvec2 v2 = voro[k];
+                const cgpu::dvec2 a{0.0 - v2.x, 0.0 - v2.y}, b{v1.x - v2.x, v1.y - v2.y};
+                double t2 = -1.0;
+                const double bc = b.x * c.x + b.y * c.y;
+                if (bc != 0) t2 = (a.x * c.x + a.y * c.y) / bc;
+                if (t2 >= 0.0 && t2 <= 1.0) {
+                    const cgpu::dvec2 q{v2.x + t2 * (v1.x - v2.x), v2.y + t2 * (v1.y - v2.y)};
+                    if (found) {
+                        int2 = q;
+                        second = true;
+                    } else {
+                        int1 = q;
+                        found = true;
+                    }
+                }
+                v1 = v2;
+            }
+            if (!found) throw std::runtime_error("division axis misses the cell's polygon");
+            if (!second) int2 = int1;
+            const double sep = std::max(std::sqrt(int1.x * int1.x + int1.y * int1.y),
+                                        std::sqrt(int2.x * int2.x + int2.y * int2.y));
+            const double f = frac[e] * sep;
+            cgpu::dvec2 p1{pi.x + f * ray.x, pi.y + f * ray.y}, p2{pi.x - f * ray.x, pi.y - f * ray.y};
+            cgpu::put_in_box(box, p1);
+            cgpu::put_in_box(box, p2);
+            // the parent moves to p1; the daughter (copy of the parent's
+            // preferences, motility and type) takes p2
+            const size_t pk = start + (size_t)(target - p.off);
+            nxy[2 * pk] = p1.x;
+            nxy[2 * pk + 1] = p1.y;
+            push(target, p.n);
+            const size_t dk = src.size() - 1;
+            nxy[2 * dk] = p2.x;
+            nxy[2 * dk + 1] = p2.y;
+            double dth = new_director ? new_director[e] : NAN;
+            if (!std::isfinite(dth)) {
+                uint32_t ctr[4] = {(uint32_t)p.n, (uint32_t)st[s].step, 3u, 0u};
+                cgpu::philox4x32_10(ctr, (uint32_t)p.seed, (uint32_t)(p.seed >> 32));
+                const uint64_t u = (((uint64_t)ctr[0] << 21) ^ (uint64_t)(ctr[1] >> 11)) & (((uint64_t)1 << 53) - 1);
+                dth = 6.283185307179586476925286766559 * ((double)u / 9007199254740992.0);
+            }
+            nth[dk] = dth;
+            nvel[2 * dk] = nmot[2 * dk] * std::cos(dth);
+            nvel[2 * dk + 1] = nmot[2 * dk] * std::sin(dth);
+            if (!nex.empty()) nex[dk] = 0;
+        }
+        newn[s] = (int)(src.size() - start);
+    }
+    layout(newn);
+    const size_t M = N_;
+    pos0_.upload(nxy.data(), 2 * M);
+    real_upload(theta_.p, nth.data(), M);
+    real_upload(vel_.p, nvel.data(), 2 * M);
+    real_upload(APpref_.p, npref.data(), 2 * M);
+    real_upload(mot_.p, nmot.data(), 2 * M);
+    tag_ = newtag;
+    if (!ex.empty()) {
+        excl_.alloc(M);
+        excl_.upload(nex.data(), M);
+        ext_.alloc(2 * M * rsz_);
+    }
+    if (!h_type_.empty()) {
+        h_type_ = ntype;
+        cell_type_.alloc(M);
+        cell_type_.upload(ntype.data(), M);
+    }
+    if (noise_.p) noise_.alloc(2 * M);
+    st_.upload(st.data(), S_);
+    update_tension_flag();
+    return global_triangulation();
+}
+
+}  // namespace cgpu_host

@@ -109,6 +109,13 @@ class VoronoiBatch {
     // relabel the cells of every system along a Hilbert curve of their
     // positions, then retriangulate; perm[new] = old (global indices)
     int spatial_sort(int *perm_out);
+    // reference voronoiModelBase::cellDeath / cellDivision (voronoiModelBase.cpp:1291-1422):
+    // at most one event per system; kind 0 death, 1 division (theta, separation
+    // fraction; the daughter, appended at the end of its system, gets director
+    // new_director, or a uniform draw when that is NaN). Re-lays the batch out and
+    // retriangulates.
+    int cell_events(int count, const int *kind, const int *cell, const double *theta, const double *frac,
+                    const double *new_director);
 
    private:
     void upload_params();
@@ -143,6 +150,7 @@ class VoronoiBatch {
     int ntypes_ = 0;
     bool tension_on_ = false;  // some system has a tension mode and cell types are set
     void update_tension_flag();
+    void layout(const std::vector<int> &n);
     size_t device_bytes_ = 0;
 };
 

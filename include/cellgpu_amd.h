@@ -328,6 +328,21 @@ int cgpu_vor_set_directors(void *h, const double *theta);
 /* tag (label within its system at creation) of every cell; spatial sorting
  * permutes cells, the tags follow them (reference tagToIdx / idxToTag) */
 int cgpu_vor_get_tags(void *h, int *tags);
+/* Cell death and division (reference voronoiModelBase::cellDeath / cellDivision,
+ * voronoiModelBase.cpp:1291-1422; Simple2DCell.cpp:775-850; Simple2DActiveCell.cpp:
+ * 118-147), at most one event per system per call. kind[e] 0: death of cell[e]
+ * (batch index; the cells after it in its system move down one index). kind[e] 1:
+ * division of cell[e] along the axis at angle theta[e], parent and daughter at
+ * +-frac[e] times the largest distance from the cell to its Voronoi polygon along
+ * that axis; the daughter is appended at the end of its system with the parent's
+ * preferences, motility and type and director new_director[e] (NaN, or
+ * new_director NULL: a uniform draw of the batch's generator). Systems keep their
+ * boxes; the batch is laid out again and retriangulated, so batch views and
+ * graphs taken before are invalid. */
+int cgpu_vor_cell_events(void *h, int count, const int *kind, const int *cell, const double *theta,
+                         const double *frac, const double *new_director);
+/* cells per system (n[S]); returns the total */
+int cgpu_vor_get_sizes(void *h, int *n);
 int cgpu_vor_set_tension(void *h, int s, int mode, double gamma);
 int cgpu_vor_set_tension_matrix(void *h, int ntypes, const double *g);
 int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retriangulate);

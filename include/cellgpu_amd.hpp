@@ -128,6 +128,19 @@ class VoronoiQuadraticEnergy : public Simple2DModel {
    public:
     //! reference Simple2DCell::setCellType
     void setCellType(const vector<int> &types) { check(cgpu_vor_set_cell_types(h_, types.data()), "setCellType"); }
+    //! reference voronoiModelBase::cellDeath
+    void cellDeath(int cellIndex) {
+        int kind = 0;
+        check(cgpu_vor_cell_events(h_, 1, &kind, &cellIndex, nullptr, nullptr, nullptr), "cellDeath");
+        Ncells = cgpu_vor_get_sizes(h_, nullptr);
+    }
+    //! reference voronoiModelBase::cellDivision: parameters = {cell}, dParams = {theta, fraction}
+    void cellDivision(const vector<int> &parameters, const vector<double> &dParams) {
+        int kind = 1;
+        check(cgpu_vor_cell_events(h_, 1, &kind, &parameters.at(0), &dParams.at(0), &dParams.at(1), nullptr),
+              "cellDivision");
+        Ncells = cgpu_vor_get_sizes(h_, nullptr);
+    }
 };
 
 //! reference voronoiQuadraticEnergyWithTension.h:16-60 (GPU branch): line

@@ -45,6 +45,9 @@ def lib():
             "orc_vsys_get_counters": (None, [vp, np.ctypeslib.ndpointer(dtype=np.int64)]),
             "orc_vsys_compute_forces": (C.c_double, [vp, C.c_int]),
             "orc_vsys_set_tension": (None, [vp, C.c_int, C.c_double, C.c_int, vp, _ip]),
+            "orc_vsys_cell_death": (C.c_int, [vp, C.c_int]),
+            "orc_vsys_cell_division": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+            "orc_vsys_num_cells": (C.c_int, [vp]),
             "orc_vsys_test_flags": (C.c_int, [vp, _ip]),
             "orc_vsys_enforce_topology": (C.c_int, [vp, C.c_int]),
             "orc_vsys_spp_step": (C.c_int, [vp, _dp, C.c_int]),
@@ -172,6 +175,19 @@ class VoronoiSystem:
         out = np.zeros(6, np.int64)
         lib().orc_vsys_get_counters(self.h, out)
         return dict(zip(["global_fixes", "local_repairs", "repair_events", "skipped", "last_mode", "last_nf"], out.tolist()))
+
+    def cell_death(self, i):
+        """reference voronoiModelBase::cellDeath: remove cell i, retriangulate."""
+        if lib().orc_vsys_cell_death(self.h, int(i)) != 0:
+            raise RuntimeError("oracle: cell death failed")
+        self.N = lib().orc_vsys_num_cells(self.h)
+
+    def cell_division(self, i, theta, frac, daughter_director):
+        """reference voronoiModelBase::cellDivision (parameters {i}, dParams {theta, frac});
+        the daughter is appended with the given director."""
+        if lib().orc_vsys_cell_division(self.h, int(i), float(theta), float(frac), float(daughter_director)) != 0:
+            raise RuntimeError("oracle: cell division failed")
+        self.N = lib().orc_vsys_num_cells(self.h)
 
     def set_tension(self, types, gamma=0.0, matrix=None):
         """Line tension between cell types (VoronoiQuadraticEnergyWithTension):

@@ -959,6 +959,93 @@ void orc_vsys_get_counters(void *h, long long *out) {
     out[4] = s->lastRepairMode;
     out[5] = (long long)s->lastNeedsFixing.size();
 }
+// reference Simple2DCell::cellDeath + Simple2DActiveCell::cellDeath +
+// voronoiModelBase::cellDeath (Simple2DCell.cpp:775-810, Simple2DActiveCell.cpp:118-123,
+// voronoiModelBase.cpp:1309-1316): the cell's entries are removed, the cells after
+// it move down one index, then a global retriangulation
+int orc_vsys_cell_death(void *h, int i) {
+    VSys *s = (VSys *)h;
+    if (i < 0 || i >= s->N || s->N <= 4) return -1;
+    s->N -= 1;
+    s->P.erase(s->P.begin() + i);
+    s->AP.erase(s->AP.begin() + i);
+    s->APpref.erase(s->APpref.begin() + i);
+    s->F.erase(s->F.begin() + i);
+    s->extF.erase(s->extF.begin() + i);
+    s->excl.erase(s->excl.begin() + i);
+    s->theta.erase(s->theta.begin() + i);
+    s->mot.erase(s->mot.begin() + i);
+    s->vel.erase(s->vel.begin() + i);
+    if (!s->type.empty()) s->type.erase(s->type.begin() + i);
+    if (!globalTriangulation(*s)) return -1;
+    allDelSets(*s);
+    return 0;
+}
+// reference voronoiModelBase::cellDivision (voronoiModelBase.cpp:1318-1422) with
+// Simple2DCell::cellDivision (.cpp:819-850) and Simple2DActiveCell::cellDivision
+// (.cpp:132-147): the cell's Voronoi polygon (CPU vertex order, vertex k =
+// circumcenter(ring k-1, ring k)), the two crossings of the axis at angle theta,
+// the parent moved to +frac * maxSeparation along the axis and a daughter appended
+// at -frac * maxSeparation, copying the parent's preferences, motility and type,
+// with director dtheta and velocity v0 (cos, sin)(dtheta); then a global
+// retriangulation. (One crossing only: the second is taken equal to the first.)
+int orc_vsys_cell_division(void *h, int i, double theta, double frac, double dtheta) {
+    VSys *s = (VSys *)h;
+    if (i < 0 || i >= s->N) return -1;
+    const int z = s->nn[i];
+    d2 pi = s->P[i];
+    vector<d2> voro;
+    d2 rij = s->box.minDist(s->P[s->ring(i, z - 1)], pi);
+    for (int k = 0; k < z; ++k) {
+        d2 rik = s->box.minDist(s->P[s->ring(i, k)], pi);
+        voro.push_back(circumcenter0(rij, rik));
+        rij = rik;
+    }
+    d2 v1 = voro[z - 1], int1{0, 0}, int2{0, 0};
+    bool first = false, second = false;
+    const d2 ray{std::cos(theta), std::sin(theta)}, c{-std::sin(theta), std::cos(theta)};
+    for (int k = 0; k < z; ++k) {
+        d2 v2 = voro[k];
+        d2 a{0.0 - v2.x, 0.0 - v2.y}, b{v1.x - v2.x, v1.y - v2.y};
+        double t2 = -1.;
+        double bc = b.x * c.x + b.y * c.y;
+        if (bc != 0) t2 = (a.x * c.x + a.y * c.y) / bc;
+        if (t2 >= 0. && t2 <= 1.0) {
+            d2 q{v2.x + t2 * (v1.x - v2.x), v2.y + t2 * (v1.y - v2.y)};
+            if (first) {
+                int2 = q;
+                second = true;
+            } else {
+                int1 = q;
+                first = true;
+            }
+        }
+        v1 = v2;
+    }
+    if (!first) return -1;
+    if (!second) int2 = int1;
+    double sep = std::max(std::sqrt(int1.x * int1.x + int1.y * int1.y), std::sqrt(int2.x * int2.x + int2.y * int2.y));
+    double f = frac * sep;
+    d2 p1{pi.x + f * ray.x, pi.y + f * ray.y}, p2{pi.x - f * ray.x, pi.y - f * ray.y};
+    s->box.putInBoxReal(p1);
+    s->box.putInBoxReal(p2);
+    s->N += 1;
+    s->P[i] = p1;
+    s->P.push_back(p2);
+    s->AP.push_back(d2{0, 0});
+    s->APpref.push_back(s->APpref[i]);
+    s->F.push_back(d2{0, 0});
+    s->extF.push_back(d2{0, 0});
+    s->excl.push_back(0);
+    s->mot.push_back(s->mot[i]);
+    s->theta.push_back(dtheta);
+    s->vel.push_back(d2{s->mot[i].x * std::cos(dtheta), s->mot[i].x * std::sin(dtheta)});
+    if (!s->type.empty()) s->type.push_back(s->type[i]);
+    if (!globalTriangulation(*s)) return -1;
+    allDelSets(*s);
+    return 0;
+}
+int orc_vsys_num_cells(void *h) { return ((VSys *)h)->N; }
 // line tension (VoronoiQuadraticEnergyWithTension): mode 0 off, 1 one value
 // gamma between unlike types, 2 matrix g[a*ntypes + b]; types per cell
 void orc_vsys_set_tension(void *h, int mode, double gamma, int ntypes, const double *g, const int *types) {

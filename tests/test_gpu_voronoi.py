@@ -368,3 +368,63 @@ def test_tension_per_system_and_sorting():
         b.set_cell_types(bad[perm])
         b.set_tension(2, system=1)
     b.check_status()
+
+
+# --------------------------------------------------------- division and death
+def test_cell_division_and_death_match_oracle(oracle):
+    """reference cellDivision / cellDeath on a relaxed FP64 system, GPU batch and
+    oracle from the same state: the new positions, copied parameters and
+    velocities are bit-identical, the retriangulated rings equal, and the next
+    self-propelled steps still match step by step."""
+    cg = _cg()
+    b = cg.VoronoiBatch([300], seeds=[21], precision="fp64")
+    b.set_prefs(1.0, 3.8)
+    b.set_motility(0.1, 1.0)
+    b.spv_steps(20, use_graph=False)
+    o = _oracle_from_gpu(oracle, b, P0=3.8, v0=0.1, Dr=1.0)
+    b.cell_events([1], [17], [0.7], [0.4], [2.0])
+    o.cell_division(17, 0.7, 0.4, 2.0)
+    g, r = b.get(), o.get()
+    assert b.N == o.N == 301
+    for k in ("x", "y", "theta", "vx", "vy"):
+        assert np.array_equal(g[k], r[k]), k
+    nn, nbr, _ = b.topology(CAP)
+    onn, onbr, _ = o.topology(CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert b.tags()[300] == 300
+    b.cell_events([0], [5])
+    o.cell_death(5)
+    g, r = b.get(), o.get()
+    assert b.N == o.N == 300
+    assert np.array_equal(g["x"], r["x"]) and np.array_equal(g["vx"], r["vx"])
+    nn, nbr, _ = b.topology(CAP)
+    onn, onbr, _ = o.topology(CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert b.tags()[5] == 5 and b.tags()[299] == 299  # tags above the dead cell's move down
+    b.check_status()
+    assert _stepwise(oracle, b, 10, 9, P0=3.8, v0=0.1) >= 0
+
+
+def test_cell_events_in_a_batch():
+    """Events in two systems of a three-system FP32 batch at once: the untouched
+    system keeps its state bit for bit, sizes and offsets follow, and graph-captured
+    steps run on the new layout."""
+    cg = _cg()
+    b = cg.VoronoiBatch([300, 400, 500], seeds=[1, 2, 3], precision="fp32")
+    b.set_motility(0.1, 1.0)
+    b.spv_steps(10)
+    g0 = b.get()
+    nn0, nbr0, _ = b.topology(CAP)
+    b.cell_events([1, 0], [50, 300 + 400 + 7], [1.1, 0.0], [0.3, 0.0])
+    assert list(b.n) == [301, 400, 499] and b.N == 1200
+    g1 = b.get()
+    s0, s1 = slice(300, 700), slice(301, 701)
+    assert np.array_equal(g0["x"][s0], g1["x"][s1]) and np.array_equal(g0["theta"][s0], g1["theta"][s1])
+    nn1, nbr1, _ = b.topology(CAP)
+    ring0 = [tuple(nbr0[i, :nn0[i]] - 300) for i in range(300, 700)]
+    ring1 = [tuple(nbr1[i, :nn1[i]] - 301) for i in range(301, 701)]
+    assert ring0 == ring1
+    assert nn1.sum() == 6 * 1200
+    b.spv_steps(40)
+    b.check_status()
+    assert np.isfinite(b.get()["x"]).all()

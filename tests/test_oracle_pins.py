@@ -338,3 +338,48 @@ def test_tension_forces_are_minus_energy_gradient(oracle, variant):
             an = F["F" + comp][i]
             assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (variant, i, comp, fd, an)
     assert abs(F["Fx"].sum()) < 1e-9 and abs(F["Fy"].sum()) < 1e-9
+
+
+def test_cell_division_and_death_closed_forms(oracle):
+    """Division (reference voronoiModelBase::cellDivision) on the centred
+    rectangular lattice, whose Voronoi cell is the hexagon with vertices
+    (+-ax/2, +-k), (0, +-h): along theta = 0 the axis leaves the cell at ax/2, so
+    parent and daughter sit at x +- frac ax/2; along pi/2 it leaves through the
+    vertices (0, +-h). The daughter takes the parent's motility and the given
+    director. Death removes the cell. Every result is a valid tiling (sum of
+    areas L^2, mean degree 6)."""
+    nx = ny = 12
+    L = 12.0
+    x, y, ax, ay = _lattice(nx, ny, L)
+    h = (ay * ay + ax * ax / 4) / (2 * ay)
+    N = nx * ny
+    s = oracle.VoronoiSystem(x, y, np.zeros(N), L, P0=3.7)
+    s.set_motility(0.3, 1.0)
+    i, j = 50, 77
+    # (fractions chosen so that no four points become cocircular: the lattice
+    # itself is full of cocircular quadruples, resolved consistently only by symmetry)
+    s.cell_division(i, 0.0, 0.3, 1.2)
+    g = s.get()
+    assert s.N == N + 1
+    assert abs(g["x"][i] - (x[i] + 0.3 * ax / 2)) < 1e-12 and abs(g["y"][i] - y[i]) < 1e-12
+    assert abs(g["x"][N] - (x[i] - 0.3 * ax / 2)) < 1e-12 and abs(g["y"][N] - y[i]) < 1e-12
+    assert g["theta"][N] == 1.2
+    assert abs(g["vx"][N] - 0.3 * np.cos(1.2)) < 1e-15 and abs(g["vy"][N] - 0.3 * np.sin(1.2)) < 1e-15
+    s.cell_division(j, np.pi / 2, 0.4, 0.0)
+    g = s.get()
+    assert abs(g["y"][j] - (y[j] + 0.4 * h)) < 1e-12 and abs(g["y"][N + 1] - (y[j] - 0.4 * h)) < 1e-12
+    assert abs(g["x"][j] - x[j]) < 1e-12
+    s.compute_forces(gpu_style=True)
+    g = s.get()
+    nn, _, _ = s.topology()
+    assert abs(g["A"].sum() - L * L) < 1e-9 and nn.sum() == 6 * s.N
+    before = s.get()
+    s.cell_death(10)
+    g = s.get()
+    assert s.N == N + 1
+    keep = np.r_[0:10, 11:N + 2]
+    assert np.array_equal(g["x"], before["x"][keep]) and np.array_equal(g["theta"], before["theta"][keep])
+    s.compute_forces(gpu_style=True)
+    g = s.get()
+    nn, _, _ = s.topology()
+    assert abs(g["A"].sum() - L * L) < 1e-9 and nn.sum() == 6 * s.N
