@@ -599,6 +599,21 @@ class VertexBatch:
     def brownian_steps(self, steps, use_graph=True):
         check(lib().cgpu_vm_brownian_steps(self.h, int(steps), 1 if use_graph else 0), "vm_brownian_steps")
 
+    def cell_events(self, kinds, cells, p1=None, p2=None, new_director=None):
+        """T2 deaths (kind 0) and divisions (kind 1, through the edges after
+        vertex-list positions p1, p2) of batch cells, at most one per system
+        (reference vertexModelBase::cellDeath / cellDivision)."""
+        k = np.ascontiguousarray(np.atleast_1d(kinds), np.int32)
+        c = np.ascontiguousarray(np.atleast_1d(cells), np.int32)
+        m = len(k)
+        a = np.ascontiguousarray(np.zeros(m) if p1 is None else np.atleast_1d(p1), np.int32)
+        b = np.ascontiguousarray(np.zeros(m) if p2 is None else np.atleast_1d(p2), np.int32)
+        d = None if new_director is None else _f64(np.atleast_1d(new_director), m)
+        check(lib().cgpu_vm_cell_events(self.h, m, _p(k), _p(c), _p(a), _p(b), _p(d)), "vm_cell_events")
+        nc = np.zeros(self.S, np.int32)
+        self.Nv = int(lib().cgpu_vm_get_sizes(self.h, _p(nc), None))
+        self.Nc = int(nc.sum())
+
     def set_topology(self, vn, vcn, cvn, cv):
         """Replace the topology (checked for consistency first); cv is [Nc, cap]."""
         cv = np.ascontiguousarray(cv, np.int32)
@@ -668,6 +683,17 @@ class VertexQuadraticEnergy:
 
     def setCellDirectors(self, theta):
         self.batch.set_directors(theta)
+
+    def cellDeath(self, cellIndex):
+        """reference vertexModelBase::cellDeath (T2 of a triangular cell)."""
+        self.batch.cell_events([0], [cellIndex])
+        self.Ncells, self.Nvertices = self.batch.Nc, self.batch.Nv
+
+    def cellDivision(self, parameters, dParams=()):
+        """reference vertexModelBase::cellDivision: parameters = [cell, list
+        position 1, list position 2]."""
+        self.batch.cell_events([1], [parameters[0]], [parameters[1]], [parameters[2]])
+        self.Ncells, self.Nvertices = self.batch.Nc, self.batch.Nv
 
     def returnCellDirectors(self):
         return self.batch.directors()

@@ -9,7 +9,7 @@
 
 namespace cgpu_host {
 
-FireBase::FireBase(int S, size_t ndof, size_t rsz, size_t nchunks) : S_(S) {
+FireBase::FireBase(int S, size_t ndof, size_t rsz, size_t nchunks) : S_(S), ndof_(ndof), nchunks_(nchunks), rsz_(rsz) {
     fs_.alloc(S_);
     vel_.alloc(2 * ndof * rsz);
     dots_.alloc(3 * ndof + 4 * nchunks);
@@ -48,6 +48,15 @@ void FireBase::set(int max_it, double fc, double alpha_start, double dt, double 
 }
 
 int FireBase::minimize() {
+    // cell events change the degrees of freedom: start from rest with buffers of
+    // the new size (reference minimize -> initializeFromModel)
+    if (dof() != ndof_ || chunks() != nchunks_) {
+        sync();
+        ndof_ = dof();
+        nchunks_ = chunks();
+        vel_.alloc(2 * ndof_ * rsz_);
+        dots_.alloc(3 * ndof_ + 4 * nchunks_);
+    }
     int r = forces();
     if (r) return r;
     sync();
@@ -137,6 +146,8 @@ class VoronoiFire : public FireBase {
     }
     void sync() override { vb_->sync(); }
     int status() override { return vb_->status(); }
+    size_t dof() override { return vb_->num_cells(); }
+    size_t chunks() override { return vb_->view().num_chunks; }
 
    private:
     VoronoiBatch *vb_;

@@ -30,7 +30,10 @@ class FireBase {
     virtual int apply_mix() = 0;  // the pending velocity mixing
     virtual void sync() = 0;
     virtual int status() = 0;     // after a sync: first error of the batch
+    virtual size_t dof() = 0;     // degrees of freedom of the model now
+    virtual size_t chunks() = 0;  // reduction chunks of the model now
     int S_;
+    size_t ndof_ = 0, nchunks_ = 0, rsz_ = 0;
     DevBuf<cgpu_fire_state> fs_;
     DevBuf<unsigned char> vel_;
     DevBuf<double> dots_;

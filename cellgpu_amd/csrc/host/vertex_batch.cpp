@@ -14,6 +14,115 @@
 
 namespace cgpu_host {
 
+// The convention the kernels rely on, rebuilt from the vertex-cell incidences and
+// the vertex-neighbour sets of one system (local indices), using topology only
+// (geometry can be locally inverted right after a T1 flip):
+// 1. one consistent cyclic order of the three cells at every vertex, by
+//    propagation over edges: where v and u share cells P, Q with P -> Q at v,
+//    the order at u must be Q -> P (the two ends of an edge see it from opposite
+//    sides);
+// 2. each neighbour stored across from the cell that does not contain it;
+// 3. cell lists by walking polygon successors (the successor of v in its cell f
+//    is the neighbour stored at f + 1), from the cell's smallest vertex;
+// 4. the global sense from the total signed area (counter-clockwise > 0).
+static void canonical_topology(int nv, int nc, const double *lx, const double *ly, const cgpu::Box &box, int cap,
+                        std::vector<int> &lvn, std::vector<int> &lvcn, std::vector<std::vector<int>> &cells) {
+    auto nxt = [&](int v, int c) {
+        for (int f = 0; f < 3; ++f)
+            if (lvcn[3 * v + f] == c) return lvcn[3 * v + (f + 1) % 3];
+        throw std::invalid_argument("topology inconsistent");
+    };
+    auto shared = [&](int v, int u, int &P, int &Q) {
+        int got = 0, sc[2] = {-1, -1};
+        for (int f = 0; f < 3; ++f)
+            for (int g = 0; g < 3; ++g)
+                if (lvcn[3 * v + f] == lvcn[3 * u + g] && got < 2) sc[got++] = lvcn[3 * v + f];
+        if (got != 2) throw std::invalid_argument("topology inconsistent");
+        if (nxt(v, sc[0]) == sc[1]) {
+            P = sc[0];
+            Q = sc[1];
+        } else {
+            P = sc[1];
+            Q = sc[0];
+        }
+    };
+    std::vector<char> seen(nv, 0);
+    std::vector<int> queue;
+    for (int root = 0; root < nv; ++root) {
+        if (seen[root]) continue;
+        seen[root] = 1;
+        queue.assign(1, root);
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int v = queue[h];
+            for (int k = 0; k < 3; ++k) {
+                const int u = lvn[3 * v + k];
+                int P, Q;
+                shared(v, u, P, Q);
+                const bool ok = nxt(u, Q) == P;
+                if (!seen[u]) {
+                    if (!ok) std::swap(lvcn[3 * u + 1], lvcn[3 * u + 2]);
+                    seen[u] = 1;
+                    queue.push_back(u);
+                } else if (!ok) {
+                    throw std::invalid_argument("topology is not orientable");
+                }
+            }
+        }
+    }
+    auto assign_neighbours = [&]() {
+        for (int v = 0; v < nv; ++v) {
+            const int n3[3] = {lvn[3 * v], lvn[3 * v + 1], lvn[3 * v + 2]};
+            for (int f = 0; f < 3; ++f) {
+                int across = -1;
+                for (int u : n3) {
+                    bool in_u = false;
+                    for (int g = 0; g < 3; ++g) in_u = in_u || lvcn[3 * u + g] == lvcn[3 * v + f];
+                    if (!in_u) across = u;
+                }
+                if (across < 0) throw std::invalid_argument("topology inconsistent");
+                lvn[3 * v + f] = across;
+            }
+        }
+    };
+    std::vector<int> first(nc, INT_MAX), count(nc, 0);
+    for (int v = 0; v < nv; ++v)
+        for (int f = 0; f < 3; ++f) {
+            const int c = lvcn[3 * v + f];
+            first[c] = std::min(first[c], v);
+            ++count[c];
+        }
+    cells.assign(nc, {});
+    auto walk = [&]() {
+        double area = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            if (count[c] < 3 || count[c] > cap) throw std::invalid_argument("cell size out of range");
+            cells[c].clear();
+            int v = first[c];
+            const cgpu::dvec2 base{lx[v], ly[v]};
+            cgpu::dvec2 prev{0.0, 0.0};
+            for (int k = 0; k < count[c]; ++k) {
+                cells[c].push_back(v);
+                int f = 0;
+                while (f < 3 && lvcn[3 * v + f] != c) ++f;
+                if (f == 3) throw std::invalid_argument("topology inconsistent");
+                v = lvn[3 * v + (f + 1) % 3];
+                const cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{lx[v], ly[v]}, base);
+                area += 0.5 * (prev.x * d.y - prev.y * d.x);
+                prev = d;
+            }
+            if (v != first[c]) throw std::invalid_argument("cell polygon does not close");
+        }
+        return area;
+    };
+    assign_neighbours();
+    if (walk() < 0) {  // the propagated sense is clockwise: reverse every vertex
+        for (int v = 0; v < nv; ++v) std::swap(lvcn[3 * v + 1], lvcn[3 * v + 2]);
+        assign_neighbours();
+        walk();
+    }
+}
+
+
 class VertexBatch {
    public:
     VertexBatch(int S, const int *ncells, const double *L, const uint64_t *seed, const double *x, const double *y,
@@ -79,18 +188,7 @@ class VertexBatch {
         }
         Nv_ = voff;
         Nc_ = coff;
-        // vertex chunks for the per-system reductions
-        std::vector<int> ch_sys, ch_begin, ch_end;
-        for (int s = 0; s < S; ++s) {
-            cgpu_vm_params &p = hp_[s];
-            p.chunk0 = (int)ch_sys.size();
-            for (int b = p.voff; b < p.voff + p.nv; b += CGPU_CHUNK) {
-                ch_sys.push_back(s);
-                ch_begin.push_back(b);
-                ch_end.push_back(std::min(b + CGPU_CHUNK, p.voff + p.nv));
-            }
-        }
-        num_chunks_ = (int)ch_sys.size();
+
         auto find_tri = [&](int a, int b, int c) {  // counter-clockwise (a, b, c)
             int o = std::min(a, std::min(b, c)), p, q;
             if (o == a) { p = b; q = c; } else if (o == b) { p = c; q = a; } else { p = a; q = b; }
@@ -135,8 +233,8 @@ class VertexBatch {
         }
         hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
         sys_.alloc(S);
-        sys_.upload(hp_.data(), S);
         st_.alloc(S);
+        flag_count_.alloc(S);
         std::vector<cgpu_sys_state> st(S);
         std::memset(st.data(), 0, sizeof(cgpu_sys_state) * S);
         for (auto &q : st) {
@@ -144,45 +242,73 @@ class VertexBatch {
             q.steps_left = -1;
         }
         st_.upload(st.data(), S);
+        layout();
+        pos0_.upload(vpos.data(), 2 * (size_t)Nv_);
+        vn_.upload(vn.data(), vn.size());
+        vcn_.upload(vcn.data(), vcn.size());
+        cvn_.upload(cvn.data(), Nc_);
+        cv_.upload(cv.data(), cv.size());
+        std::vector<double> a0(Nc_, 1.0), p0(Nc_, 3.9);
+        set_prefs(a0.data(), p0.data());
+        real_upload(theta_.p, pth.data(), Nc_);
+        std::vector<double> v0(Nc_, 0.0), Dr(Nc_, 1.0);  // reference setv0Dr(0.0, 1.0)
+        set_motility(v0.data(), Dr.data());
+    }
+    // Offsets, vertex chunks and every per-vertex and per-cell device array for
+    // the sizes in hp_ (nv, nc of each system); arrays start zeroed.
+    void layout() {
+        int voff = 0, coff = 0;
+        std::vector<int> ch_sys, ch_begin, ch_end, vsys, csys;
+        for (int s = 0; s < S_; ++s) {
+            cgpu_vm_params &p = hp_[s];
+            p.voff = voff;
+            p.coff = coff;
+            p.chunk0 = (int)ch_sys.size();
+            for (int b = voff; b < voff + p.nv; b += CGPU_CHUNK) {
+                ch_sys.push_back(s);
+                ch_begin.push_back(b);
+                ch_end.push_back(std::min(b + CGPU_CHUNK, voff + p.nv));
+            }
+            vsys.insert(vsys.end(), p.nv, s);
+            csys.insert(csys.end(), p.nc, s);
+            voff += p.nv;
+            coff += p.nc;
+        }
+        Nv_ = voff;
+        Nc_ = coff;
+        num_chunks_ = (int)ch_sys.size();
+        sys_.upload(hp_.data(), S_);
         vsys_.alloc(Nv_);
         vsys_.upload(vsys.data(), Nv_);
         csys_.alloc(Nc_);
         csys_.upload(csys.data(), Nc_);
         pos0_.alloc(2 * (size_t)Nv_);
-        pos0_.upload(vpos.data(), 2 * (size_t)Nv_);
         pos1_.alloc(2 * (size_t)Nv_);
         vn_.alloc(3 * (size_t)Nv_);
-        vn_.upload(vn.data(), vn.size());
         vcn_.alloc(3 * (size_t)Nv_);
-        vcn_.upload(vcn.data(), vcn.size());
         cvn_.alloc(Nc_);
-        cvn_.upload(cvn.data(), Nc_);
-        cv_.alloc(cv.size());
-        cv_.upload(cv.data(), cv.size());
+        cv_.alloc((size_t)CGPU_VERTEX_CAP * Nc_);
         AP_.alloc(2 * Nc_ * rsz_);
         dAP_.alloc(2 * Nc_ * rsz_);
         APpref_.alloc(2 * Nc_ * rsz_);
         force_.alloc(2 * Nv_ * rsz_);
-        flag_count_.alloc(S);
         flag_list_.alloc(6 * (size_t)Nv_);
         flag_cells_.alloc(12 * (size_t)Nv_);
         flag_touched_.alloc(12 * (size_t)Nv_);
         cell_min_.alloc(Nc_);
         std::vector<unsigned long long> ones(Nc_, ~0ull);
         cell_min_.upload(ones.data(), Nc_);
-        std::vector<double> a0(Nc_, 1.0), p0(Nc_, 3.9);
-        set_prefs(a0.data(), p0.data());
         theta_.alloc(Nc_ * rsz_);
-        real_upload(theta_.p, pth.data(), Nc_);
         mot_.alloc(2 * Nc_ * rsz_);
-        std::vector<double> v0(Nc_, 0.0), Dr(Nc_, 1.0);  // reference setv0Dr(0.0, 1.0)
-        set_motility(v0.data(), Dr.data());
         chunk_sys_.alloc(num_chunks_);
         chunk_sys_.upload(ch_sys.data(), num_chunks_);
         chunk_begin_.alloc(num_chunks_);
         chunk_begin_.upload(ch_begin.data(), num_chunks_);
         chunk_end_.alloc(num_chunks_);
         chunk_end_.upload(ch_end.data(), num_chunks_);
+        if (noise_.p) noise_.alloc(2 * (size_t)Nv_ + Nc_);
+        parity_ = 0;
+        drop_graph();
     }
     ~VertexBatch() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -419,6 +545,8 @@ class VertexBatch {
             for (int k = 0; k < cvn[c]; ++k) sm[(size_t)k * Nc_ + c] = cv[(size_t)c * cap + k];
         cv_.upload(sm.data(), sm.size());
     }
+    int cell_events(int count, const int *kind, const int *cell, const int *p1, const int *p2,
+                    const double *new_director);
     void get_counters(int s, long long *out) {
         sync();
         std::vector<cgpu_sys_state> st(S_);
@@ -502,10 +630,255 @@ class VertexFire : public FireBase {
     }
     void sync() override { vm_->sync(); }
     int status() override { return vm_->status(); }
+    size_t dof() override { return vm_->num_vertices(); }
+    size_t chunks() override { return vm_->view().num_chunks; }
 
    private:
     VertexBatch *vm_;
 };
+
+
+// Cell death (a T2 transition of a triangular cell) and division through two of
+// its edges (reference vertexModelBase::cellDeath / cellDivision,
+// vertexModelBase.cpp:905-1110, 1124-1385, with Simple2DActiveCell.cpp:118-147).
+// At most one event per system. The state is gathered on the host and each
+// system with an event is edited as the reference does: death merges the
+// triangle's three vertices into its smallest one (which keeps its position),
+// removes the other two and the cell; division inserts a vertex at the midpoint
+// of the edges after list positions p1 and p2 of the cell and moves the vertices
+// between them into a new cell appended to the system, with the parent's
+// preferences and motility and director new_director (NaN: a uniform draw). The
+// edited topology is brought back to the kernels' convention and the batch is
+// laid out again.
+int VertexBatch::cell_events(int count, const int *kind, const int *cell, const int *p1, const int *p2,
+                             const double *new_director) {
+    if (count <= 0) return CGPU_OK;
+    if (!kind || !cell) throw std::invalid_argument("cell events need kinds and cells");
+    std::vector<int> csys(Nc_);
+    csys_.download(csys.data(), Nc_);
+    std::vector<int> ev(S_, -1);
+    for (int e = 0; e < count; ++e) {
+        if (cell[e] < 0 || cell[e] >= Nc_) throw std::invalid_argument("cell index out of range");
+        if (kind[e] != 0 && kind[e] != 1) throw std::invalid_argument("event kind is 0 (death) or 1 (division)");
+        if (ev[csys[cell[e]]] >= 0) throw std::invalid_argument("at most one event per system per call");
+        ev[csys[cell[e]]] = e;
+        if (kind[e] == 1 && (!p1 || !p2)) throw std::invalid_argument("a division needs two list positions");
+    }
+    sync();
+    const int cap = CGPU_VERTEX_CAP;
+    std::vector<double> vx(Nv_), vy(Nv_), pref(2 * (size_t)Nc_), th(Nc_), mot(2 * (size_t)Nc_);
+    get(vx.data(), vy.data(), nullptr, nullptr, nullptr, nullptr);
+    std::vector<int> vn(3 * (size_t)Nv_), vcn(3 * (size_t)Nv_), cvn(Nc_), cv((size_t)Nc_ * cap);
+    get_topology(vn.data(), vcn.data(), cvn.data(), cv.data(), cap);
+    real_download(APpref_.p, pref.data(), pref.size());
+    real_download(theta_.p, th.data(), Nc_);
+    real_download(mot_.p, mot.data(), mot.size());
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    // the new batch, system by system
+    std::vector<double> nx, ny, npref, nth, nmot;
+    std::vector<int> nvn, nvcn, ncvn, ncv;
+    for (int s = 0; s < S_; ++s) {
+        cgpu_vm_params &p = hp_[s];
+        const int vo = p.voff, co = p.coff;
+        int nv = p.nv, nc = p.nc;
+        std::vector<double> lx(vx.begin() + vo, vx.begin() + vo + nv), ly(vy.begin() + vo, vy.begin() + vo + nv);
+        std::vector<int> lvn(3 * (size_t)nv), lvcn(3 * (size_t)nv);
+        for (int k = 0; k < 3 * nv; ++k) {
+            lvn[k] = vn[3 * (size_t)vo + k] - vo;
+            lvcn[k] = vcn[3 * (size_t)vo + k] - co;
+        }
+        std::vector<std::vector<int>> cells(nc);
+        for (int c = 0; c < nc; ++c)
+            for (int k = 0; k < cvn[co + c]; ++k) cells[c].push_back(cv[(size_t)(co + c) * cap + k] - vo);
+        std::vector<double> lpref(pref.begin() + 2 * co, pref.begin() + 2 * (co + nc)),
+            lth(th.begin() + co, th.begin() + co + nc), lmot(mot.begin() + 2 * co, mot.begin() + 2 * (co + nc));
+        const cgpu::Box box = cgpu::make_box(p.L);
+        const int e = ev[s];
+        if (e >= 0 && kind[e] == 0) {
+            const int c = cell[e] - co;
+            if (cells[c].size() != 3) throw std::invalid_argument("cell death (T2) needs a triangular cell");
+            if (nc <= 4) throw std::invalid_argument("a system keeps at least 4 cells");
+            std::vector<int> V = cells[c], around;
+            for (int v : V)
+                for (int f = 0; f < 3; ++f) {
+                    const int q = lvcn[3 * v + f];
+                    if (q != c && std::find(around.begin(), around.end(), q) == around.end()) around.push_back(q);
+                }
+            std::rotate(V.begin(), std::min_element(V.begin(), V.end()), V.end());
+            std::vector<int> outer;
+            for (int v : V)
+                for (int f = 0; f < 3; ++f) {
+                    const int u = lvn[3 * v + f];
+                    if (u != V[0] && u != V[1] && u != V[2] && std::find(outer.begin(), outer.end(), u) == outer.end())
+                        outer.push_back(u);
+                }
+            if (around.size() != 3 || outer.size() != 3) throw std::invalid_argument("cell death: not a simple triangle");
+            for (int q : around) {
+                std::vector<int> l;
+                for (int v : cells[q]) {
+                    const int w = (v == V[1] || v == V[2]) ? V[0] : v;
+                    if (std::find(l.begin(), l.end(), w) == l.end()) l.push_back(w);
+                }
+                cells[q] = l;
+            }
+            for (int k = 0; k < 3; ++k) {
+                lvcn[3 * V[0] + k] = around[k];
+                lvn[3 * V[0] + k] = outer[k];
+                for (int f = 0; f < 3; ++f)
+                    if (lvn[3 * outer[k] + f] == V[1] || lvn[3 * outer[k] + f] == V[2]) lvn[3 * outer[k] + f] = V[0];
+            }
+            // drop vertices V[1], V[2] and cell c, renumbering the rest
+            std::vector<int> vmap(nv), cmap(nc);
+            for (int v = 0, w = 0; v < nv; ++v) vmap[v] = (v == V[1] || v == V[2]) ? -1 : w++;
+            for (int q = 0, w = 0; q < nc; ++q) cmap[q] = q == c ? -1 : w++;
+            std::vector<double> x2, y2;
+            std::vector<int> vn2, vcn2;
+            for (int v = 0; v < nv; ++v) {
+                if (vmap[v] < 0) continue;
+                x2.push_back(lx[v]);
+                y2.push_back(ly[v]);
+                for (int f = 0; f < 3; ++f) {
+                    vn2.push_back(vmap[lvn[3 * v + f]]);
+                    vcn2.push_back(cmap[lvcn[3 * v + f]]);
+                }
+            }
+            std::vector<std::vector<int>> cells2;
+            std::vector<double> pref2, th2, mot2;
+            for (int q = 0; q < nc; ++q) {
+                if (q == c) continue;
+                std::vector<int> l;
+                for (int v : cells[q]) l.push_back(vmap[v]);
+                cells2.push_back(l);
+                pref2.insert(pref2.end(), {lpref[2 * q], lpref[2 * q + 1]});
+                th2.push_back(lth[q]);
+                mot2.insert(mot2.end(), {lmot[2 * q], lmot[2 * q + 1]});
+            }
+            lx = x2;
+            ly = y2;
+            lvn = vn2;
+            lvcn = vcn2;
+            cells = cells2;
+            lpref = pref2;
+            lth = th2;
+            lmot = mot2;
+            nv -= 2;
+            nc -= 1;
+            canonical_topology(nv, nc, lx.data(), ly.data(), box, cap, lvn, lvcn, cells);
+        } else if (e >= 0 && kind[e] == 1) {
+            const int c = cell[e] - co;
+            const std::vector<int> l = cells[c];
+            const int n = (int)l.size();
+            const int i1 = std::min(p1[e], p2[e]), i2 = std::max(p1[e], p2[e]);
+            if (i1 < 0 || i2 >= n || i1 == i2) throw std::invalid_argument("division list positions out of range");
+            const int a = l[i1], an = l[(i1 + 1) % n], b = l[i2], bn = l[(i2 + 1) % n];
+            auto mid = [&](int u, int w) {
+                cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{lx[w], ly[w]}, cgpu::dvec2{lx[u], ly[u]});
+                cgpu::dvec2 q{lx[u] + 0.5 * d.x, ly[u] + 0.5 * d.y};
+                cgpu::put_in_box(box, q);
+                return q;
+            };
+            auto third_cell = [&](int u, int w) {
+                int ans = -1;
+                for (int f = 0; f < 3; ++f)
+                    for (int g = 0; g < 3; ++g)
+                        if (lvcn[3 * u + f] == lvcn[3 * w + g] && lvcn[3 * u + f] != c) ans = lvcn[3 * u + f];
+                if (ans < 0) throw std::invalid_argument("division: edge without a neighbouring cell");
+                return ans;
+            };
+            const int na = third_cell(a, an), nb = third_cell(b, bn);
+            const int A = nv, B = nv + 1, C = nc;  // new vertices and cell
+            const cgpu::dvec2 pa = mid(a, an), pb = mid(b, bn);
+            lx.push_back(pa.x);
+            ly.push_back(pa.y);
+            lx.push_back(pb.x);
+            ly.push_back(pb.y);
+            lvn.insert(lvn.end(), {a, an, B, A, b, bn});
+            lvcn.insert(lvcn.end(), {na, C, c, nb, c, C});
+            auto repl = [&](int v, int from, int to) {
+                for (int f = 0; f < 3; ++f)
+                    if (lvn[3 * v + f] == from) lvn[3 * v + f] = to;
+            };
+            repl(a, an, A);
+            repl(an, a, A);
+            repl(b, bn, B);
+            repl(bn, b, B);
+            // new cell: A, an ... b, B; parent: B, bn ... a, A
+            std::vector<int> child{A}, parent{B};
+            for (int k = i1 + 1; k <= i2; ++k) child.push_back(l[k % n]);
+            child.push_back(B);
+            for (int k = i2 + 1; k <= i1 + n; ++k) parent.push_back(l[k % n]);
+            parent.push_back(A);
+            for (size_t k = 1; k + 1 < child.size(); ++k)
+                for (int f = 0; f < 3; ++f)
+                    if (lvcn[3 * child[k] + f] == c) lvcn[3 * child[k] + f] = C;
+            cells[c] = parent;
+            cells.push_back(child);
+            auto insert_after = [&](int q, int after, int v) {
+                std::vector<int> m;
+                for (int w : cells[q]) {
+                    m.push_back(w);
+                    if (w == after) m.push_back(v);
+                }
+                cells[q] = m;
+            };
+            insert_after(na, an, A);
+            insert_after(nb, bn, B);
+            lpref.insert(lpref.end(), {lpref[2 * c], lpref[2 * c + 1]});
+            lmot.insert(lmot.end(), {lmot[2 * c], lmot[2 * c + 1]});
+            double d = new_director ? new_director[e] : NAN;
+            if (!std::isfinite(d)) {
+                uint32_t ctr[4] = {(uint32_t)nc, (uint32_t)st[s].step, 4u, 0u};
+                cgpu::philox4x32_10(ctr, (uint32_t)p.seed, (uint32_t)(p.seed >> 32));
+                const uint64_t u = (((uint64_t)ctr[0] << 21) ^ (uint64_t)(ctr[1] >> 11)) & (((uint64_t)1 << 53) - 1);
+                d = 6.283185307179586476925286766559 * ((double)u / 9007199254740992.0);
+            }
+            lth.push_back(d);
+            nv += 2;
+            nc += 1;
+            canonical_topology(nv, nc, lx.data(), ly.data(), box, cap, lvn, lvcn, cells);
+        }
+        // append the system to the new batch (global indices)
+        const int nvo = (int)nx.size(), nco = (int)nth.size();
+        p.nv = nv;
+        p.nc = nc;
+        for (int v = 0; v < nv; ++v) {
+            nx.push_back(lx[v]);
+            ny.push_back(ly[v]);
+            for (int f = 0; f < 3; ++f) {
+                nvn.push_back(lvn[3 * v + f] + nvo);
+                nvcn.push_back(lvcn[3 * v + f] + nco);
+            }
+        }
+        for (int q = 0; q < nc; ++q) {
+            if ((int)cells[q].size() > cap) throw std::invalid_argument("a cell would exceed the vertex capacity");
+            ncvn.push_back((int)cells[q].size());
+            for (int k = 0; k < cap; ++k) ncv.push_back(k < (int)cells[q].size() ? cells[q][k] + nvo : -1);
+        }
+        npref.insert(npref.end(), lpref.begin(), lpref.end());
+        nth.insert(nth.end(), lth.begin(), lth.end());
+        nmot.insert(nmot.end(), lmot.begin(), lmot.end());
+    }
+    layout();
+    std::vector<double> xy(2 * (size_t)Nv_);
+    for (int v = 0; v < Nv_; ++v) {
+        xy[2 * v] = nx[v];
+        xy[2 * v + 1] = ny[v];
+    }
+    pos0_.upload(xy.data(), xy.size());
+    vn_.upload(nvn.data(), nvn.size());
+    vcn_.upload(nvcn.data(), nvcn.size());
+    cvn_.upload(ncvn.data(), Nc_);
+    std::vector<int> sm((size_t)cap * Nc_, -1);
+    for (int q = 0; q < Nc_; ++q)
+        for (int k = 0; k < ncvn[q]; ++k) sm[(size_t)k * Nc_ + q] = ncv[(size_t)q * cap + k];
+    cv_.upload(sm.data(), sm.size());
+    real_upload(APpref_.p, npref.data(), npref.size());
+    real_upload(theta_.p, nth.data(), Nc_);
+    real_upload(mot_.p, nmot.data(), nmot.size());
+    st_.upload(st.data(), S_);
+    return compute_forces();
+}
 
 }  // namespace cgpu_host
 
@@ -575,6 +948,18 @@ int cgpu_vm_get_directors(void *h, double *theta) {
         ((VertexBatch *)h)->get_directors(theta);
         return CGPU_OK;
     });
+}
+int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, const int *p1, const int *p2,
+                        const double *new_director) {
+    return vguard([&] { return ((VertexBatch *)h)->cell_events(count, kind, cell, p1, p2, new_director); });
+}
+int cgpu_vm_get_sizes(void *h, int *nc, int *nv) {
+    VertexBatch &vm = *(VertexBatch *)h;
+    for (int s = 0; s < vm.num_systems(); ++s) {
+        if (nc) nc[s] = vm.params(s).nc;
+        if (nv) nv[s] = vm.params(s).nv;
+    }
+    return vm.num_vertices();
 }
 int cgpu_vm_set_topology(void *h, const int *vn, const int *vcn, const int *cvn, const int *cv, int cap) {
     return vguard([&] {
@@ -705,108 +1090,8 @@ extern "C" int cgpu_db_avm_read_batch(void *db, void *h, int s, int rec) {
         for (int k = 0; k < 3 * r.nv; ++k)
             if (lvn[k] < 0 || lvn[k] >= r.nv || lvcn[k] < 0 || lvcn[k] >= r.nc)
                 throw std::invalid_argument("record topology out of range");
-        // Topology only (geometry can be locally inverted right after a T1 flip):
-        // 1. one consistent cyclic order of the three cells at every vertex, by
-        //    propagation over edges: where v and u share cells P, Q with P -> Q at
-        //    v, the order at u must be Q -> P (the two ends of an edge see it from
-        //    opposite sides);
-        // 2. each neighbour stored across from the cell that does not contain it;
-        // 3. cell lists by walking polygon successors (the successor of v in its
-        //    cell f is the neighbour stored at f + 1);
-        // 4. the global sense from the total signed area (counter-clockwise > 0).
-        auto nxt = [&](int v, int c) {
-            for (int f = 0; f < 3; ++f)
-                if (lvcn[3 * v + f] == c) return lvcn[3 * v + (f + 1) % 3];
-            throw std::invalid_argument("record topology inconsistent");
-        };
-        auto shared = [&](int v, int u, int &P, int &Q) {
-            int got = 0, sc[2] = {-1, -1};
-            for (int f = 0; f < 3; ++f)
-                for (int g = 0; g < 3; ++g)
-                    if (lvcn[3 * v + f] == lvcn[3 * u + g] && got < 2) sc[got++] = lvcn[3 * v + f];
-            if (got != 2) throw std::invalid_argument("record topology inconsistent");
-            if (nxt(v, sc[0]) == sc[1]) {
-                P = sc[0];
-                Q = sc[1];
-            } else {
-                P = sc[1];
-                Q = sc[0];
-            }
-        };
-        std::vector<char> seen(r.nv, 0);
-        std::vector<int> queue;
-        for (int root = 0; root < r.nv; ++root) {
-            if (seen[root]) continue;
-            seen[root] = 1;
-            queue.assign(1, root);
-            for (size_t h = 0; h < queue.size(); ++h) {
-                const int v = queue[h];
-                for (int k = 0; k < 3; ++k) {
-                    const int u = lvn[3 * v + k];
-                    int P, Q;
-                    shared(v, u, P, Q);
-                    const bool ok = nxt(u, Q) == P;
-                    if (!seen[u]) {
-                        if (!ok) std::swap(lvcn[3 * u + 1], lvcn[3 * u + 2]);
-                        seen[u] = 1;
-                        queue.push_back(u);
-                    } else if (!ok) {
-                        throw std::invalid_argument("record topology is not orientable");
-                    }
-                }
-            }
-        }
-        auto assign_neighbours = [&]() {
-            for (int v = 0; v < r.nv; ++v) {
-                const int n3[3] = {lvn[3 * v], lvn[3 * v + 1], lvn[3 * v + 2]};
-                for (int f = 0; f < 3; ++f) {
-                    int across = -1;
-                    for (int u : n3) {
-                        bool in_u = false;
-                        for (int g = 0; g < 3; ++g) in_u = in_u || lvcn[3 * u + g] == lvcn[3 * v + f];
-                        if (!in_u) across = u;
-                    }
-                    if (across < 0) throw std::invalid_argument("record topology inconsistent");
-                    lvn[3 * v + f] = across;
-                }
-            }
-        };
-        std::vector<int> first(r.nc, INT_MAX), count(r.nc, 0);
-        for (int v = 0; v < r.nv; ++v)
-            for (int f = 0; f < 3; ++f) {
-                const int c = lvcn[3 * v + f];
-                first[c] = std::min(first[c], v);
-                ++count[c];
-            }
-        std::vector<std::vector<int>> cells(r.nc);
-        auto walk = [&]() {
-            double area = 0.0;
-            for (int c = 0; c < r.nc; ++c) {
-                if (count[c] < 3 || count[c] > cap) throw std::invalid_argument("record cell size out of range");
-                cells[c].clear();
-                int v = first[c];
-                const cgpu::dvec2 base{lx[v], ly[v]};
-                cgpu::dvec2 prev{0.0, 0.0};
-                for (int k = 0; k < count[c]; ++k) {
-                    cells[c].push_back(v);
-                    int f = 0;
-                    while (f < 3 && lvcn[3 * v + f] != c) ++f;
-                    if (f == 3) throw std::invalid_argument("record topology inconsistent");
-                    v = lvn[3 * v + (f + 1) % 3];
-                    const cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{lx[v], ly[v]}, base);
-                    area += 0.5 * (prev.x * d.y - prev.y * d.x);
-                    prev = d;
-                }
-                if (v != first[c]) throw std::invalid_argument("record cell polygon does not close");
-            }
-            return area;
-        };
-        assign_neighbours();
-        if (walk() < 0) {  // the propagated sense is clockwise: reverse every vertex
-            for (int v = 0; v < r.nv; ++v) std::swap(lvcn[3 * v + 1], lvcn[3 * v + 2]);
-            assign_neighbours();
-            walk();
-        }
+        std::vector<std::vector<int>> cells;
+        cgpu_host::canonical_topology(r.nv, r.nc, lx.data(), ly.data(), box, cap, lvn, lvcn, cells);
         // into the batch's full arrays
         std::vector<double> vx(Nv), vy(Nv), th(Nc);
         vm.get(vx.data(), vy.data(), nullptr, nullptr, nullptr, nullptr);

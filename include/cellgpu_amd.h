@@ -405,6 +405,19 @@ int cgpu_vm_spp_steps(void *h, int steps, int use_graph);
  * lists are checked for consistency (every vertex in the lists of its three
  * cells, cell sizes within the capacity) before anything is uploaded. */
 int cgpu_vm_set_topology(void *h, const int *vn, const int *vcn, const int *cvn, const int *cv, int cap);
+/* Cell death and division of the vertex model (reference vertexModelBase::cellDeath
+ * / cellDivision, vertexModelBase.cpp:905-1110, 1124-1385), at most one event per
+ * system per call. kind[e] 0: T2 death of the triangular cell cell[e] (its three
+ * vertices merge into the smallest, which keeps its position). kind[e] 1: division
+ * of cell[e] through the edges that follow list positions p1[e] and p2[e] of its
+ * vertex list (a new vertex at each edge midpoint; the vertices between go to a
+ * new cell appended to the system with the parent's preferences and motility and
+ * director new_director[e], NaN or NULL: a uniform draw). The batch is laid out
+ * again, so views and graphs taken before are invalid. */
+int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, const int *p1, const int *p2,
+                        const double *new_director);
+/* cells and vertices per system (either may be NULL); returns the total vertices */
+int cgpu_vm_get_sizes(void *h, int *nc, int *nv);
 
 /* ------------------------------------------------------ trajectory databases
  * NetCDF classic (CDF-1) files in the reference's layouts, written by a built-in

@@ -226,6 +226,20 @@ class VertexQuadraticEnergy : public Simple2DModel {
         check(cgpu_vm_get_directors(h_, t.data()), "cellDirectors");
         return t;
     }
+    //! reference vertexModelBase::cellDeath: T2 transition of a triangular cell
+    void cellDeath(int cellIndex) {
+        int kind = 0, z = 0;
+        check(cgpu_vm_cell_events(h_, 1, &kind, &cellIndex, &z, &z, nullptr), "cellDeath");
+        refresh_sizes();
+    }
+    //! reference vertexModelBase::cellDivision: parameters = {cell, list position 1, list position 2}
+    void cellDivision(const vector<int> &parameters, const vector<double> &dParams = {}) {
+        (void)dParams;
+        int kind = 1;
+        check(cgpu_vm_cell_events(h_, 1, &kind, &parameters.at(0), &parameters.at(1), &parameters.at(2), nullptr),
+              "cellDivision");
+        refresh_sizes();
+    }
     //! reference vertexModelBase::reportMeanVertexForce: mean force on the vertices
     double2v reportMeanVertexForce() {
         vector<double> a(Nvertices), b(Nvertices);
@@ -277,6 +291,9 @@ class VertexQuadraticEnergy : public Simple2DModel {
 
    private:
     void push() { check(cgpu_vm_set_params(h_, 0, KA, KP, deltaT, mu, T, T1Threshold), "set params"); }
+    void refresh_sizes() {
+        Nvertices = cgpu_vm_get_sizes(h_, &Ncells, nullptr);
+    }
     void *h_;
 };
 

@@ -267,6 +267,9 @@ def _vlib():
             "vmo_spp_step": (C.c_int, [vp, _dp]),
             "vmo_fire_set": (None, [vp, C.c_int] + [C.c_double] * 7 + [C.c_int]),
             "vmo_fire_minimize": (C.c_int, [vp, _dp]),
+            "vmo_cell_death": (C.c_int, [vp, C.c_int]),
+            "vmo_cell_division": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double]),
+            "vmo_nc": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -337,6 +340,18 @@ class VertexSystem:
 
     def last_flagged(self):
         return _vlib().vmo_last_flagged(self.h)
+
+    def cell_death(self, c):
+        """reference vertexModelBase::cellDeath (T2 of a triangular cell)."""
+        if _vlib().vmo_cell_death(self.h, int(c)) != 0:
+            raise RuntimeError("oracle: cell death needs a simple triangular cell")
+        self.Nv, self.Nc = _vlib().vmo_nv(self.h), _vlib().vmo_nc(self.h)
+
+    def cell_division(self, c, p1, p2, daughter_director=0.0):
+        """reference vertexModelBase::cellDivision (parameters c, p1, p2)."""
+        if _vlib().vmo_cell_division(self.h, int(c), int(p1), int(p2), float(daughter_director)) != 0:
+            raise RuntimeError("oracle: cell division failed")
+        self.Nv, self.Nc = _vlib().vmo_nv(self.h), _vlib().vmo_nc(self.h)
 
     def set_motility(self, v0, Dr):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(v0, np.float64), (self.Nc,)))

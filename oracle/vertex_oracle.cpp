@@ -408,6 +408,142 @@ int vmo_t1(void *h) {
     return (int)(m->t1count - before);
 }
 int vmo_last_flagged(void *h) { return ((VM *)h)->lastFlagged; }
+// reference vertexModelBase::cellDeath (vertexModelBase.cpp:905-1110): T2
+// transition of a triangular cell. The cell's vertices (rotated so that the
+// smallest is first) merge into the first, which keeps its position; the three
+// surrounding cells replace the other two by it (first occurrences kept); the
+// three outer neighbours point to it; vertices and cells above the removed ones
+// move down. Returns 0, or -1 when the cell is not a simple triangle.
+int vmo_cell_death(void *h, int c) {
+    VM *m = (VM *)h;
+    if (c < 0 || c >= m->Nc || m->cv[c].size() != 3) return -1;
+    vector<int> V = m->cv[c], cells, outer;
+    for (int v : V)
+        for (int f = 0; f < 3; ++f) {
+            int q = m->vcn[3 * v + f];
+            if (q != c && std::find(cells.begin(), cells.end(), q) == cells.end()) cells.push_back(q);
+        }
+    std::rotate(V.begin(), std::min_element(V.begin(), V.end()), V.end());
+    for (int v : V)
+        for (int f = 0; f < 3; ++f) {
+            int u = m->vn[3 * v + f];
+            if (u != V[0] && u != V[1] && u != V[2] && std::find(outer.begin(), outer.end(), u) == outer.end())
+                outer.push_back(u);
+        }
+    if (cells.size() != 3 || outer.size() != 3) return -1;
+    for (int q : cells) {
+        vector<int> l;
+        for (int v : m->cv[q]) {
+            int w = (v == V[1] || v == V[2]) ? V[0] : v;
+            if (std::find(l.begin(), l.end(), w) == l.end()) l.push_back(w);
+        }
+        m->cv[q] = l;
+    }
+    for (int k = 0; k < 3; ++k) {
+        m->vcn[3 * V[0] + k] = cells[k];
+        m->vn[3 * V[0] + k] = outer[k];
+        for (int f = 0; f < 3; ++f)
+            if (m->vn[3 * outer[k] + f] == V[1] || m->vn[3 * outer[k] + f] == V[2]) m->vn[3 * outer[k] + f] = V[0];
+    }
+    const int a = std::min(V[1], V[2]), b = std::max(V[1], V[2]);
+    auto vshift = [&](int v) { return v - (v > a ? 1 : 0) - (v > b ? 1 : 0); };
+    for (auto &l : m->cv)
+        for (int &v : l) v = vshift(v);
+    for (int &v : m->vn) v = vshift(v);
+    for (int &q : m->vcn)
+        if (q > c) q -= 1;
+    for (int v : {b, a}) {
+        m->v.erase(m->v.begin() + v);
+        m->vn.erase(m->vn.begin() + 3 * v, m->vn.begin() + 3 * v + 3);
+        m->vcn.erase(m->vcn.begin() + 3 * v, m->vcn.begin() + 3 * v + 3);
+    }
+    m->cv.erase(m->cv.begin() + c);
+    m->APpref.erase(m->APpref.begin() + c);
+    m->theta.erase(m->theta.begin() + c);
+    m->mot.erase(m->mot.begin() + c);
+    m->Nv -= 2;
+    m->Nc -= 1;
+    m->AP.assign(m->Nc, d2{0, 0});
+    m->F.assign(m->Nv, d2{0, 0});
+    m->vcur.assign(3 * m->Nv, d2{0, 0});
+    m->vlast = m->vcur;
+    m->vnext = m->vcur;
+    return 0;
+}
+// reference vertexModelBase::cellDivision (vertexModelBase.cpp:1124-1385): new
+// vertices at the midpoints of the edges after list positions p1 < p2 of the cell
+// (indices Nv, Nv + 1), the vertices from the first new vertex to the second go
+// to a new cell (index Nc) with the parent's preferences and motility and
+// director dtheta; the cells across the two edges gain the new vertices.
+int vmo_cell_division(void *h, int c, int p1, int p2, double dtheta) {
+    VM *m = (VM *)h;
+    if (c < 0 || c >= m->Nc) return -1;
+    const vector<int> l = m->cv[c];
+    const int n = (int)l.size();
+    const int i1 = std::min(p1, p2), i2 = std::max(p1, p2);
+    if (i1 < 0 || i2 >= n || i1 == i2) return -1;
+    const int a = l[i1], an = l[(i1 + 1) % n], b = l[i2], bn = l[(i2 + 1) % n];
+    auto mid = [&](int u, int w) {
+        d2 d = m->box.minDist(m->v[w], m->v[u]);
+        d.x = 0.5 * d.x;
+        d.y = 0.5 * d.y;
+        d2 q{m->v[u].x + d.x, m->v[u].y + d.y};
+        m->box.putInBoxReal(q);
+        return q;
+    };
+    auto third = [&](int u, int w) {
+        int ans = -1;
+        for (int f = 0; f < 3; ++f)
+            for (int g = 0; g < 3; ++g)
+                if (m->vcn[3 * u + f] == m->vcn[3 * w + g] && m->vcn[3 * u + f] != c) ans = m->vcn[3 * u + f];
+        return ans;
+    };
+    const int na = third(a, an), nb = third(b, bn);
+    if (na < 0 || nb < 0) return -1;
+    const int A = m->Nv, B = m->Nv + 1, C = m->Nc;
+    m->v.push_back(mid(a, an));
+    m->v.push_back(mid(b, bn));
+    m->vn.insert(m->vn.end(), {a, an, B, A, b, bn});
+    for (int f = 0; f < 3; ++f) {
+        if (m->vn[3 * a + f] == an) m->vn[3 * a + f] = A;
+        if (m->vn[3 * an + f] == a) m->vn[3 * an + f] = A;
+        if (m->vn[3 * b + f] == bn) m->vn[3 * b + f] = B;
+        if (m->vn[3 * bn + f] == b) m->vn[3 * bn + f] = B;
+    }
+    m->vcn.insert(m->vcn.end(), {na, C, c, nb, c, C});
+    vector<int> child{A}, parent{B};
+    for (int k = i1 + 1; k <= i2; ++k) child.push_back(l[k % n]);
+    child.push_back(B);
+    for (int k = i2 + 1; k <= i1 + n; ++k) parent.push_back(l[k % n]);
+    parent.push_back(A);
+    for (size_t k = 1; k + 1 < child.size(); ++k)
+        for (int f = 0; f < 3; ++f)
+            if (m->vcn[3 * child[k] + f] == c) m->vcn[3 * child[k] + f] = C;
+    m->cv[c] = parent;
+    m->cv.push_back(child);
+    auto insert_after = [&](int q, int after, int v) {
+        vector<int> r;
+        for (int w : m->cv[q]) {
+            r.push_back(w);
+            if (w == after) r.push_back(v);
+        }
+        m->cv[q] = r;
+    };
+    insert_after(na, an, A);
+    insert_after(nb, bn, B);
+    m->APpref.push_back(m->APpref[c]);
+    m->mot.push_back(m->mot[c]);
+    m->theta.push_back(dtheta);
+    m->Nv += 2;
+    m->Nc += 1;
+    m->AP.assign(m->Nc, d2{0, 0});
+    m->F.assign(m->Nv, d2{0, 0});
+    m->vcur.assign(3 * m->Nv, d2{0, 0});
+    m->vlast = m->vcur;
+    m->vnext = m->vcur;
+    return 0;
+}
+int vmo_nc(void *h) { return ((VM *)h)->Nc; }
 void vmo_set_motility(void *h, const double *v0, const double *Dr) {
     VM *m = (VM *)h;
     for (int c = 0; c < m->Nc; ++c) m->mot[c] = d2{v0[c], Dr[c]};

@@ -238,3 +238,96 @@ def test_vertex_spv_initialize_smooths_the_tissue():
         assert abs(AP[:, 0].sum() - Nc) < 1e-9 * Nc and (AP[:, 0] > 0).all()
         q.append(np.mean(AP[:, 1] / np.sqrt(AP[:, 0])))
     assert q[1] < q[0]
+
+
+def test_vertex_cell_division_and_death_match_oracle(oracle):
+    """Vertex-model division and T2 death on the device against the oracle's
+    restatement from the same state: identical new vertex positions, the same
+    topology (as sets and canonical cell lists) and forces within 1e-12; then
+    Brownian steps with T1 flips still match step by step."""
+    cg = _cg()
+    Nc = 300
+    b = cg.VertexBatch([Nc], seeds=[9], precision="fp64")
+    b.set_params(dt=0.005, T=1e-3, t1_threshold=0.04)
+    b.set_prefs(1.0, 3.9)
+    b.brownian_steps(30, use_graph=False)
+    o = _oracle_like(oracle, Nc, 9, dt=0.005, T=1e-3, thr=0.04, P0=3.9)
+
+    def load():
+        g = b.get()
+        vn, vcn, cvn, cv = b.topology()
+        o.set_positions(g["vx"], g["vy"])
+        o.set_topology(vn, vcn, cvn, cv)
+
+    def compare():
+        g, r = b.get(), o.compute()
+        assert np.array_equal(g["vx"], r["vx"]) and np.array_equal(g["vy"], r["vy"])
+        vn, vcn, cvn, cv = b.topology()
+        ovn, ovcn, ocvn, ocv = o.topology()
+        assert _sets(vn) == _sets(ovn) and _sets(vcn) == _sets(ovcn)
+        assert _canon(cv, cvn) == _canon(ocv, ocvn)
+        b.compute_forces()
+        g = b.get()
+        scale = np.abs(np.concatenate([r["Fx"], r["Fy"]])).max()
+        assert np.abs(g["Fx"] - r["Fx"]).max() <= 1e-12 * scale
+        np.testing.assert_allclose(g["A"], r["A"], rtol=1e-12)
+
+    load()
+    b.cell_events([1], [17], [0], [3], [0.25])
+    o.cell_division(17, 0, 3, 0.25)
+    assert b.Nc == o.Nc == Nc + 1 and b.Nv == o.Nv == 2 * Nc + 2
+    compare()
+    # a triangle divided off cell 40, then removed by a T2 transition
+    load()
+    b.cell_events([1], [40], [1], [2])
+    o.cell_division(40, 1, 2)
+    compare()
+    tri = Nc + 1
+    assert b.topology()[2][tri] == 3
+    load()
+    b.cell_events([0], [tri])
+    o.cell_death(tri)
+    assert b.Nc == o.Nc == Nc + 1
+    compare()
+    b.set_noise_buffer(True)
+    rng = np.random.default_rng(4)
+    for step in range(10):
+        load()
+        ex, ey = rng.standard_normal(b.Nv), rng.standard_normal(b.Nv)
+        b.upload_noise(ex, ey)
+        b.brownian_steps(1, use_graph=False)
+        b.check_status()
+        o.brownian_step(ex, ey)
+        g, r = b.get(), o.compute()
+        np.testing.assert_allclose(g["vx"], r["vx"], rtol=0, atol=1e-11)
+        vn, vcn, cvn, cv = b.topology()
+        ovn, ovcn, ocvn, ocv = o.topology()
+        assert _canon(cv, cvn) == _canon(ocv, ocvn), step
+
+
+def test_vertex_cell_events_in_a_batch():
+    """A division in one system of a two-system FP32 batch leaves the other
+    system's state bit for bit; graph-captured steps run on the new layout, and a
+    FIRE minimizer created before the event adapts to the new size."""
+    cg = _cg()
+    b = cg.VertexBatch([200, 300], seeds=[1, 2], precision="fp32")
+    b.set_params(dt=0.005, T=1e-3, t1_threshold=0.04)
+    f = cg.EnergyMinimizerFIRE(b)
+    f.setMaximumIterations(5)
+    f.minimize()
+    b.brownian_steps(10)
+    g0 = b.get()
+    vn0, vcn0, cvn0, cv0 = b.topology()
+    b.cell_events([1], [250], [0], [2])
+    assert b.Nc == 501 and b.Nv == 1002
+    g1 = b.get()
+    assert np.array_equal(g0["vx"][:400], g1["vx"][:400])
+    vn1, vcn1, cvn1, cv1 = b.topology()
+    assert np.array_equal(vn0[:1200], vn1[:1200]) and np.array_equal(vcn0[:1200], vcn1[:1200])
+    assert cvn1.sum() == 3 * b.Nv
+    b.brownian_steps(20)
+    b.check_status()
+    f.setMaximumIterations(20)
+    f.minimize()
+    b.check_status()
+    assert np.isfinite(b.get()["vx"]).all()

@@ -181,3 +181,42 @@ def test_fire_relaxes_vertex_lattice_to_its_minimum(oracle):
     r = vm.compute()
     assert r["E"] < 1e-12 * E0
     np.testing.assert_allclose(r["A"], ax * ay, rtol=1e-7)
+
+
+def test_cell_division_and_t2_death(oracle):
+    """Vertex-model division (reference vertexModelBase::cellDivision) of a lattice
+    hexagon through the midpoints of two opposite edges halves it exactly (the
+    chord passes through its centre); the cells across those edges gain a vertex;
+    bookkeeping stays consistent. Dividing off a triangle (two consecutive edges)
+    and killing it by a T2 transition (vertexModelBase::cellDeath) restores the
+    original tissue exactly."""
+    vm, ax, ay = _lattice_vm(oracle)
+    r0 = vm.compute()
+    vn0, vcn0, cvn0, cv0 = vm.topology()
+    Nc, Nv, L = vm.Nc, vm.Nv, vm.L
+    c = 20
+    vm.cell_division(c, 0, 3, 0.5)
+    assert vm.Nc == Nc + 1 and vm.Nv == Nv + 2
+    r = vm.compute()
+    vn, vcn, cvn, cv = vm.topology()
+    assert cvn[c] == 5 and cvn[Nc] == 5 and cvn.sum() == 3 * vm.Nv
+    assert sorted(cvn[:Nc][cvn[:Nc] != 6]) == [5, 7, 7]
+    np.testing.assert_allclose([r["A"][c], r["A"][Nc]], r0["A"][c] / 2, rtol=1e-12)
+    assert abs(r["A"].sum() - L * L) < 1e-9
+    for v in range(vm.Nv):
+        for q in vcn[3 * v: 3 * v + 3]:
+            assert v in cv[q, : cvn[q]]
+    # a triangle divided off and removed by a T2 transition: back to the start
+    vm2, _, _ = _lattice_vm(oracle)
+    vm2.cell_division(c, 0, 1, 0.0)
+    _, _, cvn2, _ = vm2.topology()
+    assert cvn2[Nc] == 3
+    vm2.cell_death(Nc)
+    assert vm2.Nc == Nc and vm2.Nv == Nv
+    r2 = vm2.compute()
+    vn2, vcn2, cvn2, cv2 = vm2.topology()
+    assert np.array_equal(r2["vx"], r0["vx"]) and np.array_equal(r2["vy"], r0["vy"])
+    canon = lambda cv_, n_: [tuple(np.roll(cv_[q, :n_[q]], -int(np.argmin(cv_[q, :n_[q]])))) for q in range(len(n_))]
+    assert canon(cv2, cvn2) == canon(cv0, cvn0)
+    assert [set(vn2[3 * v:3 * v + 3]) for v in range(Nv)] == [set(vn0[3 * v:3 * v + 3]) for v in range(Nv)]
+    np.testing.assert_allclose(r2["A"], r0["A"], rtol=1e-13)
