@@ -140,23 +140,30 @@ def cpu_baseline(n, P0, v0, Dr, seeds, budget_s=12.0, steps=10):
                        % (used, steps))
 
 
-def serving(count=48, rate=8.0, steps_range=(1000, 10000)):
-    """Serving workload (bounded): Poisson arrivals, N log-uniform [100, 65536],
-    steps log-uniform over steps_range, admitted into batches by the continuous
+def serving(count=48, rate=8.0, steps_range=(1000, 10000), fire_fraction=0.25):
+    """Serving workload (bounded): Poisson arrivals; SPV requests with N
+    log-uniform [100, 65536] and steps log-uniform over steps_range, and a
+    fraction of vertex-model FIRE requests (N log-uniform [100, 4096], RMS force
+    below 1e-8 within 2000 iterations), admitted into batches by the continuous
     batching engine; latency = completion - arrival."""
     from cellgpu_amd.serving import ServingEngine, poisson_requests
-    reqs = poisson_requests(count, rate, seed=20240607, steps_range=steps_range)
+    reqs = poisson_requests(count, rate, seed=20240607, steps_range=steps_range, fire_fraction=fire_fraction)
     eng = ServingEngine(admit_interval=0.05, tick_steps=64, precision="fp32")
     t0 = time.perf_counter()
     done = eng.run(reqs, t0=t0)
     wall = time.perf_counter() - t0
     lat = np.array([r.latency for r in done])
+    fire = [r for r in done if r.kind == "vertex_fire"]
     return {"serving_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
             "serving_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
             "serving_cell_steps_per_s": round(eng.cell_steps / wall, 1),
-            "serving_workload": "%d requests, Poisson %.1f/s, N log-uniform [100, 65536], steps log-uniform [%d, %d], "
-                                "P0~U[3.5,4], v0~U[0,0.2], continuous batching, FP32 bulk" % (
-                                    count, rate, steps_range[0], steps_range[1])}
+            "serving_fire_requests": len(fire),
+            "serving_fire_converged": sum(r.iterations < r.steps for r in fire),
+            "serving_workload": "%d requests, Poisson %.1f/s; SPV: N log-uniform [100, 65536], steps log-uniform "
+                                "[%d, %d], P0~U[3.5,4], v0~U[0,0.2], FP32 bulk; %.0f%% vertex-model FIRE: N "
+                                "log-uniform [100, 4096], P0~U[3.5,4], RMS force < 1e-8 within 2000 iterations, "
+                                "FP64; continuous batching (cell-steps include FIRE cell-iterations)" % (
+                                    count, rate, steps_range[0], steps_range[1], 100 * fire_fraction)}
 
 
 def other_workload(cg, name, steps, warmup):

@@ -476,42 +476,53 @@ class EnergyMinimizerFIRE:
                            dt_dec=0.95, alpha_dec=0.9, nmin=5)
         self._push()
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "h", None):
             lib().cgpu_fire_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def _push(self):
         p = self.params
         check(lib().cgpu_fire_set(self.h, p["max_it"], p["force_cutoff"], p["alpha_start"], p["dt"], p["dt_max"],
                                   p["dt_inc"], p["dt_dec"], p["alpha_dec"], p["nmin"]), "fire_set")
 
+    # one parameter at a time, as the reference setters (EnergyMinimizerFIRE2D.h:37-54):
+    # setAlphaStart also sets alpha, setDeltaT the current dt
+    _WHICH = dict(max_it=0, force_cutoff=1, alpha_start=2, dt=3, dt_max=4, dt_inc=5, dt_dec=6, alpha_dec=7, nmin=8)
+
+    def _set(self, name, v):
+        self.params[name] = v
+        check(lib().cgpu_fire_set_param(self.h, self._WHICH[name], float(v)), "fire_set_param")
+
     def setMaximumIterations(self, m):
-        self.params["max_it"] = int(m); self._push()
+        self._set("max_it", int(m))
 
     def setForceCutoff(self, f):
-        self.params["force_cutoff"] = f; self._push()
+        self._set("force_cutoff", f)
 
     def setAlphaStart(self, a):
-        self.params["alpha_start"] = a; self._push()
+        self._set("alpha_start", a)
 
     def setDeltaT(self, dt):
-        self.params["dt"] = dt; self._push()
+        self._set("dt", dt)
 
     def setDeltaTMax(self, t):
-        self.params["dt_max"] = t; self._push()
+        self._set("dt_max", t)
 
     def setDeltaTInc(self, t):
-        self.params["dt_inc"] = t; self._push()
+        self._set("dt_inc", t)
 
     def setDeltaTDec(self, t):
-        self.params["dt_dec"] = t; self._push()
+        self._set("dt_dec", t)
 
     def setAlphaDec(self, a):
-        self.params["alpha_dec"] = a; self._push()
+        self._set("alpha_dec", a)
 
     def setNMin(self, n):
-        self.params["nmin"] = int(n); self._push()
+        self._set("nmin", int(n))
 
     def minimize(self):
         check(lib().cgpu_fire_minimize(self.h), "fire_minimize")

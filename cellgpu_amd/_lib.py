@@ -125,6 +125,9 @@ def lib():
         "cgpu_vm_fire_create": (vp, [vp]),
         "cgpu_vm_set_topology": (i, [vp, vp, vp, vp, vp, i]),
         "cgpu_vm_cell_events": (i, [vp, i, vp, vp, vp, vp, vp]),
+        "cgpu_vm_set_active": (i, [vp, vp]),
+        "cgpu_vm_get_active": (i, [vp, vp]),
+        "cgpu_fire_set_param": (i, [vp, i, d]),
         "cgpu_vm_get_sizes": (i, [vp, vp, vp]),
         "cgpu_vor_set_directors": (i, [vp, vp]),
         "cgpu_vor_get_tags": (i, [vp, vp]),

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 
 namespace cgpu_host {
 
@@ -43,6 +44,26 @@ void FireBase::set(int max_it, double fc, double alpha_start, double dt, double 
         f.dt_dec = dt_dec;
         f.alpha_dec = alpha_dec;
         f.nmin = nmin;
+    }
+    fs_.upload(hs_.data(), S_);
+}
+
+void FireBase::set_param(int which, double v) {
+    sync();
+    fs_.download(hs_.data(), S_);
+    for (auto &f : hs_) {
+        switch (which) {
+            case CGPU_FIRE_MAX_IT: f.max_iterations = (int)v; break;
+            case CGPU_FIRE_FORCE_CUTOFF: f.force_cutoff = v; break;
+            case CGPU_FIRE_ALPHA_START: f.alpha_start = f.alpha = v; break;
+            case CGPU_FIRE_DT: f.dt = v; break;
+            case CGPU_FIRE_DT_MAX: f.dt_max = v; break;
+            case CGPU_FIRE_DT_INC: f.dt_inc = v; break;
+            case CGPU_FIRE_DT_DEC: f.dt_dec = v; break;
+            case CGPU_FIRE_ALPHA_DEC: f.alpha_dec = v; break;
+            case CGPU_FIRE_NMIN: f.nmin = (int)v; break;
+            default: throw std::invalid_argument("unknown FIRE parameter");
+        }
     }
     fs_.upload(hs_.data(), S_);
 }
@@ -173,6 +194,16 @@ int cgpu_fire_set(void *h, int max_it, double fc, double alpha_start, double dt,
     try {
         ((FireBase *)h)->set(max_it, fc, alpha_start, dt, dt_max, dt_inc, dt_dec, alpha_dec, nmin);
         return CGPU_OK;
+    } catch (...) {
+        return CGPU_ERR_HIP;
+    }
+}
+int cgpu_fire_set_param(void *h, int which, double value) {
+    try {
+        ((FireBase *)h)->set_param(which, value);
+        return CGPU_OK;
+    } catch (const std::invalid_argument &) {
+        return CGPU_ERR_ARGS;
     } catch (...) {
         return CGPU_ERR_HIP;
     }

@@ -19,6 +19,7 @@ class FireBase {
     virtual ~FireBase() = default;
     void set(int max_it, double fc, double alpha_start, double dt, double dt_max, double dt_inc, double dt_dec,
              double alpha_dec, int nmin);
+    void set_param(int which, double value);
     int minimize();
     void get(int s, double *out);
 

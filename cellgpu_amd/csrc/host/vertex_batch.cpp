@@ -547,6 +547,19 @@ class VertexBatch {
     }
     int cell_events(int count, const int *kind, const int *cell, const int *p1, const int *p2,
                     const double *new_director);
+    void set_active(const int *active) {
+        sync();
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s) st[s].active = active[s] ? 1 : 0;
+        st_.upload(st.data(), S_);
+    }
+    void get_active(int *active) {
+        sync();
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s) active[s] = st[s].active;
+    }
     void get_counters(int s, long long *out) {
         sync();
         std::vector<cgpu_sys_state> st(S_);
@@ -946,6 +959,18 @@ int cgpu_vm_set_directors(void *h, const double *theta) {
 int cgpu_vm_get_directors(void *h, double *theta) {
     return vguard([&] {
         ((VertexBatch *)h)->get_directors(theta);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_set_active(void *h, const int *active) {
+    return vguard([&] {
+        ((VertexBatch *)h)->set_active(active);
+        return CGPU_OK;
+    });
+}
+int cgpu_vm_get_active(void *h, int *active) {
+    return vguard([&] {
+        ((VertexBatch *)h)->get_active(active);
         return CGPU_OK;
     });
 }

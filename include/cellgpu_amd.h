@@ -400,6 +400,9 @@ int cgpu_vm_set_motility(void *h, const double *v0, const double *Dr);    /* per
 int cgpu_vm_set_directors(void *h, const double *theta);                 /* per cell */
 int cgpu_vm_get_directors(void *h, double *theta);
 int cgpu_vm_spp_steps(void *h, int steps, int use_graph);
+/* per-system activity (inactive systems are skipped by every step and by FIRE) */
+int cgpu_vm_set_active(void *h, const int *active);
+int cgpu_vm_get_active(void *h, int *active);
 /* Replace the topology (vn, vcn [3 Nv]; cvn [Nc]; cv [Nc * cap], counter-clockwise
  * lists) of the whole batch, e.g. when a trajectory record is read back. The
  * lists are checked for consistency (every vertex in the lists of its three
@@ -467,6 +470,21 @@ int cgpu_fire_set(void *h, int max_it, double force_cutoff, double alpha_start, 
                   double dt_inc, double dt_dec, double alpha_dec, int nmin);
 int cgpu_fire_minimize(void *h);
 int cgpu_fire_get(void *h, int s, double *out /* force_max, power, alpha, dt, max_f2, iterations */);
+/* One parameter of every system, as the reference setters do
+ * (EnergyMinimizerFIRE2D.h:37-54): setAlphaStart also sets alpha, setDeltaT sets
+ * the current dt; the others change only their own field. */
+enum {
+    CGPU_FIRE_MAX_IT = 0,
+    CGPU_FIRE_FORCE_CUTOFF = 1,
+    CGPU_FIRE_ALPHA_START = 2,
+    CGPU_FIRE_DT = 3,
+    CGPU_FIRE_DT_MAX = 4,
+    CGPU_FIRE_DT_INC = 5,
+    CGPU_FIRE_DT_DEC = 6,
+    CGPU_FIRE_ALPHA_DEC = 7,
+    CGPU_FIRE_NMIN = 8
+};
+int cgpu_fire_set_param(void *h, int which, double value);
 /* The same minimizer on a vertex batch (handle used with cgpu_fire_set / _minimize
  * / _get / _destroy); forceMax = sum |F|^2 / Nv per system. */
 void *cgpu_vm_fire_create(void *vm);

@@ -386,15 +386,16 @@ class EnergyMinimizerFIRE : public simpleEquationOfMotion {
         push();
     }
     ~EnergyMinimizerFIRE() override { cgpu_fire_destroy(h_); }
-    void setMaximumIterations(int m) { maxIt = m; push(); }
-    void setForceCutoff(double f) { fc = f; push(); }
-    void setAlphaStart(double a) { a0 = a; push(); }
-    void setDeltaT(double d) override { dt = d; if (h_) push(); }
-    void setDeltaTMax(double d) { dtMax = d; push(); }
-    void setDeltaTInc(double d) { dtInc = d; push(); }
-    void setDeltaTDec(double d) { dtDec = d; push(); }
-    void setAlphaDec(double a) { aDec = a; push(); }
-    void setNMin(int n) { nMin = n; push(); }
+    // one parameter at a time, as the reference setters (EnergyMinimizerFIRE2D.h:37-54)
+    void setMaximumIterations(int m) { maxIt = m; one(CGPU_FIRE_MAX_IT, m); }
+    void setForceCutoff(double f) { fc = f; one(CGPU_FIRE_FORCE_CUTOFF, f); }
+    void setAlphaStart(double a) { a0 = a; one(CGPU_FIRE_ALPHA_START, a); }
+    void setDeltaT(double d) override { dt = d; if (h_) one(CGPU_FIRE_DT, d); }
+    void setDeltaTMax(double d) { dtMax = d; one(CGPU_FIRE_DT_MAX, d); }
+    void setDeltaTInc(double d) { dtInc = d; one(CGPU_FIRE_DT_INC, d); }
+    void setDeltaTDec(double d) { dtDec = d; one(CGPU_FIRE_DT_DEC, d); }
+    void setAlphaDec(double a) { aDec = a; one(CGPU_FIRE_ALPHA_DEC, a); }
+    void setNMin(int n) { nMin = n; one(CGPU_FIRE_NMIN, n); }
     void minimize() { check(cgpu_fire_minimize(h_), "minimize"); }
     void integrateEquationsOfMotion() override { minimize(); }
     double getMaxForce() {
@@ -405,6 +406,7 @@ class EnergyMinimizerFIRE : public simpleEquationOfMotion {
 
    private:
     void push() { check(cgpu_fire_set(h_, maxIt, fc, a0, dt, dtMax, dtInc, dtDec, aDec, nMin), "fire set"); }
+    void one(int which, double v) { check(cgpu_fire_set_param(h_, which, v), "fire set"); }
     void *h_ = nullptr;
     int maxIt = 1000, nMin = 5;
     double fc = 1e-7, a0 = 0.99, dt = 0.01, dtMax = 0.1, dtInc = 1.05, dtDec = 0.95, aDec = 0.9;

@@ -244,6 +244,29 @@ def test_serving_alone_equals_mixed():
     assert all(r.latency is not None and r.latency >= 0 for r in mixed.values())
 
 
+def test_serving_fire_alone_equals_mixed():
+    """A vertex-model FIRE request ends bit-identical (positions and iteration
+    count) served alone and inside a FIRE batch with other tissues, next to an
+    SPV batch, and reaches its RMS force cutoff."""
+    _cg()
+    from cellgpu_amd.serving import Request, ServingEngine
+
+    def serve(reqs):
+        eng = ServingEngine(admit_interval=0.0, tick_steps=16, tick_iterations=32, keep_results=True)
+        return {r.rid: r for r in eng.run(reqs)}
+
+    target = dict(n=200, steps=3000, P0=3.75, seed=11, kind="vertex_fire", force_cutoff=1e-8)
+    alone = serve([Request(0, **target)])
+    mixed = serve([Request(0, **target), Request(1, 120, 3000, P0=3.7, seed=5, kind="vertex_fire", force_cutoff=1e-8),
+                   Request(2, 300, 40, P0=3.8, v0=0.1, seed=9),
+                   Request(3, 400, 3000, P0=3.9, seed=7, kind="vertex_fire", force_cutoff=1e-8)])
+    assert set(mixed) == {0, 1, 2, 3}
+    assert alone[0].iterations == mixed[0].iterations < 3000
+    xa, ya = alone[0].result
+    xm, ym = mixed[0].result
+    assert np.array_equal(xa, xm) and np.array_equal(ya, ym)
+
+
 def test_large_system_full_size(oracle):
     """One N = 262 144 system (L = 512): the initial triangulation equals the
     oracle's exact Delaunay triangulation ring for ring; after 40 FP32-bulk SPV
