@@ -211,6 +211,7 @@ __global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos, int rotate)
 // (lists of up to CL vertices; longer lists take the memory loops), so a flip
 // is a few round trips to memory instead of one per list entry.
 constexpr int CL = 12;
+static_assert(CL <= CGPU_VERTEX_CAP, "register copy longer than a cell list");
 struct CellRegs {
     int c, n;
     int v[CL];
@@ -219,8 +220,13 @@ __device__ inline void load_cell(const cgpu_vertex_batch &b, int c, CellRegs &r)
     const int Nc = b.num_cells;
     r.c = c;
     r.n = b.cvn[c];
+    // slots 0 .. CL - 1 exist for every cell (capacity CGPU_VERTEX_CAP), so the
+    // loads need not wait for the count
 #pragma unroll
-    for (int k = 0; k < CL; ++k) r.v[k] = (k < r.n) ? b.cv[k * Nc + c] : -1;
+    for (int k = 0; k < CL; ++k) r.v[k] = b.cv[k * Nc + c];
+#pragma unroll
+    for (int k = 0; k < CL; ++k)
+        if (k >= r.n) r.v[k] = -1;
 }
 __device__ inline int reg_at(const CellRegs &r, int idx) {
     int out = -1;
@@ -340,6 +346,16 @@ __device__ inline void insert_vertex(const cgpu_vertex_batch &b, const CellRegs 
 __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm_params &sp, int v1, int v2,
                           const int C[4], int touched[4]) {
     if (C[0] < 0 || C[1] < 0 || C[2] < 0 || C[3] < 0) return false;
+    // the edge's positions and slot triples are fetched with the four cell lists
+    dvec2 p1 = ldp(pos, v1), p2 = ldp(pos, v2);
+    int c1[3], c2[3], n1[3], n2[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        c1[q] = b.vcn[3 * v1 + q];
+        c2[q] = b.vcn[3 * v2 + q];
+        n1[q] = b.vn[3 * v1 + q];
+        n2[q] = b.vn[3 * v2 + q];
+    }
     CellRegs R0, R1, R2, R3;
     load_cell(b, C[0], R0);
     load_cell(b, C[1], R1);
@@ -369,8 +385,13 @@ __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm
     int dummy, a, bb;
     locate(b, Rj, v1, dummy, a);
     locate(b, Rl, v2, dummy, bb);
+    int na[3], nbb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        na[q] = b.vn[3 * a + q];
+        nbb[q] = b.vn[3 * bb + q];
+    }
     const Box box = make_box(sp.L);
-    dvec2 p1 = ldp(pos, v1), p2 = ldp(pos, v2);
     dvec2 e = min_dist(box, p1, p2);
     dvec2 mid{p2.x + 0.5 * e.x, p2.y + 0.5 * e.y};
     p1 = dvec2{mid.x - e.y, mid.y + e.x};
@@ -380,13 +401,30 @@ __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm
     stp(pos, v1, p1);
     stp(pos, v2, p2);
     const int ci = Ri.c, cj = Rj.c, ck = Rk.c, cl = Rl.c;
+    // the reference's rewiring, slot by slot, on the register copies. v1, v2, a
+    // and bb are distinct except a == bb in a degenerate tiling; then both of
+    // their rules act on one triple, in this order.
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
-        if (b.vcn[3 * v1 + q] == ck) b.vcn[3 * v1 + q] = cl;
-        if (b.vcn[3 * v2 + q] == ci) b.vcn[3 * v2 + q] = cj;
-        if (b.vn[3 * v1 + q] == a) b.vn[3 * v1 + q] = bb;
-        if (b.vn[3 * v2 + q] == bb) b.vn[3 * v2 + q] = a;
-        if (b.vn[3 * a + q] == v1) b.vn[3 * a + q] = v2;
-        if (b.vn[3 * bb + q] == v2) b.vn[3 * bb + q] = v1;
+        if (c1[q] == ck) c1[q] = cl;
+        if (c2[q] == ci) c2[q] = cj;
+        if (n1[q] == a) n1[q] = bb;
+        if (n2[q] == bb) n2[q] = a;
+        if (na[q] == v1) na[q] = v2;
+        if (a == bb) {
+            if (na[q] == v2) na[q] = v1;
+        } else if (nbb[q] == v2) {
+            nbb[q] = v1;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        b.vcn[3 * v1 + q] = c1[q];
+        b.vcn[3 * v2 + q] = c2[q];
+        b.vn[3 * v1 + q] = n1[q];
+        b.vn[3 * v2 + q] = n2[q];
+        b.vn[3 * a + q] = na[q];
+        if (a != bb) b.vn[3 * bb + q] = nbb[q];
     }
     // cell i loses v2, cell k loses v1; cell j gains v2 before a, cell l gains v1 before b
     remove_vertex(b, Ri, v2);
@@ -429,8 +467,9 @@ __device__ void normalize_vertex(const cgpu_vertex_batch &b, int w) {
 // one work-group per system: T1 flips in rounds. An edge flips when its key
 // (v1, v2) is the smallest among the flagged edges sharing one of its four cells;
 // flips of one round touch disjoint cells and vertices. Edges removed by a
-// neighbouring flip are dropped.
-__global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
+// neighbouring flip are dropped. With advance set, the work-group then advances
+// its system's step counter (the last kernel of a time step).
+__global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos, int advance) {
     const int s = blockIdx.x;
     const cgpu_vm_params &sp = b.sys[s];
     cgpu_sys_state &st = b.st[s];
@@ -439,6 +478,7 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
     int *touched = b.flag_touched + 12 * sp.voff;
     const unsigned long long NONE = ~0ull;
     __shared__ int count, rounds, flips;
+    __shared__ int wave_kept[16];
     if (threadIdx.x == 0) {
         count = b.flag_count[s];
         rounds = 0;
@@ -465,19 +505,28 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
             int v1 = list[2 * q], v2 = list[2 * q + 1];
             if (v1 < 0) continue;
             unsigned long long k = ((unsigned long long)(unsigned)v1 << 32) | (unsigned)v2;
+            const int4 c4 = reinterpret_cast<const int4 *>(cells)[q];
+            int C[4] = {c4.x, c4.y, c4.z, c4.w};
+            // the four minima are loaded together (slot 0 stands in for an absent
+            // cell), then combined without branches
+            unsigned long long m[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) m[t] = b.cell_min[max(C[t], 0)];
             bool win = true;
-            for (int t = 0; t < 4; ++t)
-                if (cells[4 * q + t] >= 0 && b.cell_min[cells[4 * q + t]] != k) win = false;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) win = win & ((C[t] < 0) | (m[t] == k));
             if (win) {
-                int C[4] = {cells[4 * q], cells[4 * q + 1], cells[4 * q + 2], cells[4 * q + 3]};
                 if (flip_edge(b, pos, sp, v1, v2, C, &touched[4 * q])) atomicAdd(&flips, 1);
                 list[2 * q] = -1;
             }
         }
         __syncthreads();
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            const int4 c4 = reinterpret_cast<const int4 *>(cells)[q];
+            const int C[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
             for (int t = 0; t < 4; ++t)
-                if (cells[4 * q + t] >= 0) b.cell_min[cells[4 * q + t]] = NONE;
+                if (C[t] >= 0) b.cell_min[C[t]] = NONE;
         }
         // the four touched vertices of every flip, one thread each
         for (int e = threadIdx.x; e < 4 * nq; e += blockDim.x) {
@@ -485,15 +534,35 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
             if (list[2 * q] == -1 && touched[4 * q] >= 0) normalize_vertex(b, touched[4 * q + (e & 3)]);
         }
         __syncthreads();
+        // the edges still waiting, in list order, staged in `cells` (free until
+        // the next round): ballot prefix counts per wavefront, blockDim.x edges
+        // at a time
+        const int nwaves = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = __lane_id();
+        int kept = 0;
+        for (int q0 = 0; q0 < nq; q0 += blockDim.x) {
+            const int q = q0 + threadIdx.x;
+            const int v1 = (q < nq) ? list[2 * q] : -1;
+            const int v2 = (q < nq) ? list[2 * q + 1] : -1;
+            const bool keep = v1 >= 0;
+            const unsigned long long m = __ballot(keep);
+            if (lane == 0) wave_kept[wv] = __popcll(m);
+            __syncthreads();
+            int at = kept + __popcll(m & ((1ull << lane) - 1ull));
+            int total = 0;
+            for (int k = 0; k < nwaves; ++k) {
+                if (k < wv) at += wave_kept[k];
+                total += wave_kept[k];
+            }
+            if (keep) {
+                cells[2 * at] = v1;
+                cells[2 * at + 1] = v2;
+            }
+            kept += total;
+            __syncthreads();
+        }
+        for (int q = threadIdx.x; q < 2 * kept; q += blockDim.x) list[q] = cells[q];
         if (threadIdx.x == 0) {
-            int w = 0;
-            for (int q = 0; q < nq; ++q)
-                if (list[2 * q] >= 0) {
-                    list[2 * w] = list[2 * q];
-                    list[2 * w + 1] = list[2 * q + 1];
-                    ++w;
-                }
-            count = w;
+            count = kept;
             rounds += 1;
         }
         __syncthreads();
@@ -501,16 +570,11 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos) {
     if (threadIdx.x == 0) {
         st.t1_events += flips;
         b.flag_count[s] = 0;
+        if (advance && st.active) {
+            st.step += 1;
+            if (st.steps_left > 0 && --st.steps_left == 0) st.active = 0;
+        }
     }
-}
-
-__global__ void k_vm_advance(cgpu_vertex_batch b) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.num_systems) return;
-    cgpu_sys_state &st = b.st[s];
-    if (!st.active) return;
-    st.step += 1;
-    if (st.steps_left > 0 && --st.steps_left == 0) st.active = 0;
 }
 
 // ------------------------------------------------------------------------ FIRE
@@ -629,7 +693,7 @@ int cgpu_vm_forces(const cgpu_vertex_batch *b, void *stream) {
 
 namespace {
 // geometry, force + equation of motion + move, T1 test (+ director rotation),
-// T1 flips, step counter
+// T1 flips and the step counter
 template <int KIND> int vm_step(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     if (KIND == 1 && (!b->theta || !b->motility)) return CGPU_ERR_ARGS;
@@ -642,8 +706,7 @@ template <int KIND> int vm_step(const cgpu_vertex_batch *b, void *stream) {
     else
         hipLaunchKernelGGL((k_vm_force_eom_move<double, KIND>), g, t, 0, st, *b, b->pos, b->pos_next, 1);
     hipLaunchKernelGGL(k_vm_t1_test, g, t, 0, st, *b, b->pos_next, KIND == 1 ? 1 : 0);
-    hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, st, *b, b->pos_next);
-    hipLaunchKernelGGL(k_vm_advance, dim3(nb(b->num_systems, 256)), dim3(256), 0, st, *b);
+    hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, st, *b, b->pos_next, 1);
     return check();
 }
 }  // namespace
@@ -657,7 +720,7 @@ int cgpu_vm_spp_step(const cgpu_vertex_batch *b, void *stream) { return vm_step<
 int cgpu_vm_t1(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     hipLaunchKernelGGL(k_vm_t1_test, dim3(nb(b->num_vertices, 256)), dim3(256), 0, S(stream), *b, b->pos, 0);
-    hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, S(stream), *b, b->pos);
+    hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, S(stream), *b, b->pos, 0);
     return check();
 }
 
