@@ -464,27 +464,17 @@ __device__ void normalize_vertex(const cgpu_vertex_batch &b, int w) {
     }
 }
 
-// one work-group per system: T1 flips in rounds. An edge flips when its key
+// One work-group per system: T1 flips in rounds. An edge flips when its key
 // (v1, v2) is the smallest among the flagged edges sharing one of its four cells;
 // flips of one round touch disjoint cells and vertices. Edges removed by a
-// neighbouring flip are dropped. With advance set, the work-group then advances
-// its system's step counter (the last kernel of a time step).
-__global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos, int advance) {
-    const int s = blockIdx.x;
-    const cgpu_vm_params &sp = b.sys[s];
-    cgpu_sys_state &st = b.st[s];
-    int *list = b.flag_list + 2 * 3 * sp.voff;
-    int *cells = b.flag_cells + 12 * sp.voff;
-    int *touched = b.flag_touched + 12 * sp.voff;
+// neighbouring flip are dropped. `list` holds the waiting edges, `cells` four
+// and `touched` four ints per edge: LDS while the edges fit, device memory
+// otherwise (the same code on either).
+template <typename Count>
+__device__ __forceinline__ void t1_rounds_loop(const cgpu_vertex_batch &b, double *pos, const cgpu_vm_params &sp,
+                                               int *list, int *cells, int *touched, Count &count, int &rounds,
+                                               int &flips, int *wave_kept) {
     const unsigned long long NONE = ~0ull;
-    __shared__ int count, rounds, flips;
-    __shared__ int wave_kept[16];
-    if (threadIdx.x == 0) {
-        count = b.flag_count[s];
-        rounds = 0;
-        flips = 0;
-    }
-    __syncthreads();
     while (count > 0 && rounds < 100000) {
         const int nq = count;
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
@@ -494,10 +484,9 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos, int advance) {
             if (live) edge_cells(b, v1, v2, C);
             else list[2 * q] = -2;  // dropped
             unsigned long long k = ((unsigned long long)(unsigned)v1 << 32) | (unsigned)v2;
-            for (int t = 0; t < 4; ++t) {
-                cells[4 * q + t] = C[t];
+            reinterpret_cast<int4 *>(cells)[q] = make_int4(C[0], C[1], C[2], C[3]);
+            for (int t = 0; t < 4; ++t)
                 if (C[t] >= 0) atomicMin(&b.cell_min[C[t]], k);
-            }
         }
         __syncthreads();
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
@@ -566,6 +555,38 @@ __global__ void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos, int advance) {
             rounds += 1;
         }
         __syncthreads();
+    }
+}
+
+// The T1 flips of every system, one work-group each (t1_rounds_loop). With
+// advance set, the work-group then advances its system's step counter (the last
+// kernel of a time step).
+constexpr int T1_LDS = 512;  // waiting edges kept in LDS
+__global__ __launch_bounds__(256) void k_vm_t1_rounds(cgpu_vertex_batch b, double *pos, int advance) {
+    const int s = blockIdx.x;
+    const cgpu_vm_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    int *list = b.flag_list + 2 * 3 * sp.voff;
+    __shared__ int count, rounds, flips;
+    __shared__ int wave_kept[16];
+    __shared__ int s_list[2 * T1_LDS];
+    __shared__ int4 s_cells[T1_LDS], s_touched[T1_LDS];
+    if (threadIdx.x == 0) {
+        count = b.flag_count[s];
+        rounds = 0;
+        flips = 0;
+    }
+    __syncthreads();
+    const int n0 = count;
+    if (n0 <= T1_LDS) {
+        // the list only shrinks: it stays in LDS
+        for (int q = threadIdx.x; q < 2 * n0; q += blockDim.x) s_list[q] = list[q];
+        __syncthreads();
+        t1_rounds_loop(b, pos, sp, s_list, reinterpret_cast<int *>(s_cells), reinterpret_cast<int *>(s_touched),
+                       count, rounds, flips, wave_kept);
+    } else {
+        t1_rounds_loop(b, pos, sp, list, b.flag_cells + 12 * sp.voff, b.flag_touched + 12 * sp.voff, count, rounds,
+                       flips, wave_kept);
     }
     if (threadIdx.x == 0) {
         st.t1_events += flips;
