@@ -1403,29 +1403,20 @@ __device__ void enqueue_chunk(const cgpu_voronoi_batch &b, int c) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) b.gq[base + t] = c0 + t;
 }
 
-// |NeedsFixing| > N/6 -> global retriangulation (reference
-// voronoiModelBase.cpp:650-653); all=1 forces a global pass
-__global__ void k_global_by_count(cgpu_voronoi_batch b, int all) {
-    // one block per cell chunk; every chunk of a system takes the same decision
+// The repair of system s once NeedsFixing is complete: 0 none, 1 local, 2 global
+// (|NeedsFixing| > N/6, reference voronoiModelBase.cpp:650-653).
+__device__ inline int repair_mode(const cgpu_voronoi_batch &b, int s) {
+    const cgpu_sys_state &st = b.st[s];
+    return st.any_fail ? (st.nf_count > b.sys[s].n / 6 ? 2 : 1) : 0;
+}
+
+// every system to a global retriangulation (reference globalTriangulationCGAL,
+// voronoiModelBase.cpp:281-351): one block per cell chunk queues its cells
+__global__ void k_global_all(cgpu_voronoi_batch b) {
     const int c = blockIdx.x;
     const int s = b.chunk_sys[c];
-    const cgpu_sys_params &sp = b.sys[s];
-    cgpu_sys_state &st = b.st[s];
-    int mode;
-    if (all) {
-        mode = 2;
-    } else {
-        if (!st.any_fail) return;
-        mode = (st.nf_count > sp.n / 6) ? 2 : 1;
-    }
-    if (threadIdx.x == 0 && b.chunk_begin[c] == sp.off) {
-        st.mode = mode;
-        if (!all) {
-            st.repair_events += 1;
-            if (mode == 1) st.local_repairs += st.nf_count;
-        }
-    }
-    if (mode == 2) enqueue_chunk(b, c);
+    if (threadIdx.x == 0 && b.chunk_begin[c] == b.sys[s].off) b.st[s].mode = 2;
+    enqueue_chunk(b, c);
 }
 
 // local repair of cell i: new Delaunay star of a NeedsFixing cell (reference
@@ -1463,6 +1454,21 @@ constexpr int WIDE_ID_MASK = (1 << WIDE_ID_BITS) - 1;
 
 template <int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
+    if (K == 1) {
+        // the first pass also records each system's repair decision, and queues
+        // the cells of the systems that go global for the global pass
+        const int nthreads = gridDim.x * blockDim.x;
+        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += nthreads) {
+            const int mode = repair_mode(b, s);
+            if (mode == 0) continue;
+            cgpu_sys_state &st = b.st[s];
+            st.mode = mode;
+            st.repair_events += 1;
+            if (mode == 1) st.local_repairs += st.nf_count;
+        }
+        for (int c = blockIdx.x; c < b.num_chunks; c += gridDim.x)  // uniform per block
+            if (repair_mode(b, b.chunk_sys[c]) == 2) enqueue_chunk(b, c);
+    }
     const int nq = (K == 1) ? b.counters[1] : b.counters[6];
     const int *queue = (K == 1) ? b.nfq : b.flagged;
     const int N = b.num_cells;
@@ -1475,7 +1481,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
-        if (st.mode != 1) continue;
+        if ((K == 1 ? repair_mode(b, s) : st.mode) != 1) continue;
         int ring = -1, w_at = 0;
         const int z = wave_star<K>(b, pos, i, b.sys[s], ring, w0, w_at);
         if (z == 0) continue;  // verified: unchanged
@@ -1642,18 +1648,6 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
     }
 }
 
-// per retriangulated system: neighMax = max degree + 2 (even) or + 1 (odd)
-// (voronoiModelBase.cpp:309-312), the 6N check, the global-fix counter
-__global__ void k_global_close(cgpu_voronoi_batch b) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.num_systems) return;
-    cgpu_sys_state &st = b.st[s];
-    if (st.mode < 2) return;
-    const int mx = st.max_deg;
-    st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
-    st.global_fixes += 1;
-    if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
-}
 
 // delOther of retriangulated systems (8 threads per cell, each taking slots
 // k, k + 8, k + 16); geometry of the cells whose ring a star kernel rewrote
@@ -1662,6 +1656,17 @@ __global__ void k_global_close(cgpu_voronoi_batch b) {
 // with their system's global queue.
 template <typename R>
 __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
+    // per retriangulated system (its degree statistics are complete): neighMax =
+    // max degree + 2 (even) or + 1 (odd) (voronoiModelBase.cpp:309-312), the 6N
+    // check, the global-fix counter
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x) {
+        cgpu_sys_state &st = b.st[s];
+        if (st.mode < 2) continue;
+        const int mx = st.max_deg;
+        st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
+        st.global_fixes += 1;
+        if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
+    }
     const int ng = b.counters[2];
     const int nl = b.counters[1];
     const long long total = (long long)(ng + nl) * DELSET_LANES;
@@ -1933,7 +1938,6 @@ void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
     hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_close, dim3((b.num_systems + 63) / 64), dim3(64), 0, st, b);
     if (b.precision)
         hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
     else
@@ -1942,7 +1946,6 @@ void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st)
 
 int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
     hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_by_count, dim3(b.num_chunks), dim3(256), 0, st, b, 0);
     hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
@@ -2012,7 +2015,7 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
     int r = cgpu_compute_cell_list(b, 0, stream);
     if (r) return r;
-    hipLaunchKernelGGL(k_global_by_count, dim3(b->num_chunks), dim3(256), 0, st, *b, 1);
+    hipLaunchKernelGGL(k_global_all, dim3(b->num_chunks), dim3(256), 0, st, *b);
     global_pass(*b, b->pos, st);
     // the pass recomputes geometry only where a ring changed; positions may have
     // changed everywhere here
