@@ -854,8 +854,8 @@ __device__ inline int owned_triangles(const cgpu_voronoi_batch &b, int i) {
 // ===================================================================== kernels
 
 // per-step reset of per-system topology state and queue counters
-__global__ void k_reset(cgpu_voronoi_batch b) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+// the per-step topology state of system s (and, for s = 0, the queue counters)
+__device__ inline void reset_system(const cgpu_voronoi_batch &b, int s) {
     if (s == 0) {
         b.counters[0] = 0;
         b.counters[1] = 0;
@@ -872,6 +872,18 @@ __global__ void k_reset(cgpu_voronoi_batch b) {
         st.tri_delta = 0;
         st.max_deg = 0;
         st.nflag = 0;
+    }
+}
+__global__ void k_reset(cgpu_voronoi_batch b) { reset_system(b, blockIdx.x * blockDim.x + threadIdx.x); }
+
+// end of a time step for one system: the step counter, and the serving
+// countdown that parks the system when it reaches zero
+__device__ inline void advance_system(cgpu_sys_state &st) {
+    if (!st.active) return;
+    st.step += 1;
+    if (st.steps_left > 0) {
+        st.steps_left -= 1;
+        if (st.steps_left == 0) st.active = 0;
     }
 }
 
@@ -1655,17 +1667,20 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
 // clears the marks. NeedsFixing cells of systems that went global are handled
 // with their system's global queue.
 template <typename R>
-__global__ void k_finalize(cgpu_voronoi_batch b, const double *pos) {
+__global__ void k_finalize(cgpu_voronoi_batch b, const double *pos, int advance) {
     // per retriangulated system (its degree statistics are complete): neighMax =
     // max degree + 2 (even) or + 1 (odd) (voronoiModelBase.cpp:309-312), the 6N
-    // check, the global-fix counter
+    // check, the global-fix counter. With advance set (the last kernel of a time
+    // step) also the step counter of every active system (k_advance_step).
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x) {
         cgpu_sys_state &st = b.st[s];
-        if (st.mode < 2) continue;
-        const int mx = st.max_deg;
-        st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
-        st.global_fixes += 1;
-        if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
+        if (st.mode >= 2) {
+            const int mx = st.max_deg;
+            st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
+            st.global_fixes += 1;
+            if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
+        }
+        if (advance) advance_system(st);
     }
     const int ng = b.counters[2];
     const int nl = b.counters[1];
@@ -1739,9 +1754,17 @@ template <> __device__ inline double atan2_r<double>(double y, double x) { retur
 // selfPropelledParticleDynamics.cu:35-53) -> move (Simple2DCell.cu:21-35).
 // kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
 constexpr int FORCE_BLOCK = 256;
+// flags: STEP_BIN also runs the binning pass of the cell list on the new
+// positions; STEP_RESET also clears the step's topology state (k_reset), which
+// this kernel neither reads nor writes
+constexpr int STEP_BIN = 1, STEP_RESET = 2;
 template <typename R, int KIND, bool TENS>
 __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos,
-                                                                double *pos_next, int bin) {
+                                                                double *pos_next, int flags) {
+    if (flags & STEP_RESET)
+        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
+            reset_system(b, s);
+    const bool bin = (flags & STEP_BIN) != 0;
     // LDS copy of the block's cells (positions, dAP): most ring members and
     // delOther cells of a Hilbert-ordered block are in it
     __shared__ dvec2 spos[FORCE_BLOCK];
@@ -1804,14 +1827,7 @@ __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_bat
 
 __global__ void k_advance_step(cgpu_voronoi_batch b) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.num_systems) return;
-    cgpu_sys_state &st = b.st[s];
-    if (!st.active) return;
-    st.step += 1;
-    if (st.steps_left > 0) {
-        st.steps_left -= 1;
-        if (st.steps_left == 0) st.active = 0;
-    }
+    if (s < b.num_systems) advance_system(b.st[s]);
 }
 
 __global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double *out) {
@@ -1934,23 +1950,24 @@ constexpr int WIDE_BLOCKS = 256;        // two candidates per lane, for wide bin
 
 // global pass over the queued systems: wave kernel, serial form for the cells
 // it hands back, per-system degree statistics, delOther and geometry
-void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
+void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0) {
     hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), dim3(256), 0, st, b);
     if (b.precision)
-        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
+        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos, advance);
     else
-        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos);
+        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos, advance);
 }
 
-int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st) {
+// advance: the chain ends a time step (k_finalize advances the step counters)
+int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0) {
     hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
     hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
     hipLaunchKernelGGL(k_global_fallback, dim3(b.num_chunks), dim3(256), 0, st, b);
-    global_pass(b, pos, st);
+    global_pass(b, pos, st, advance);
     return check();
 }
 
@@ -1968,13 +1985,13 @@ template <typename R> void launch_forces(const cgpu_voronoi_batch &b, hipStream_
 }
 
 template <typename R, int KIND>
-void launch_force_eom_move(const cgpu_voronoi_batch &b, hipStream_t st, int bin) {
+void launch_force_eom_move(const cgpu_voronoi_batch &b, hipStream_t st, int flags) {
     // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
     dim3 g(nblocks(b.num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
     if (b.cell_type)
-        hipLaunchKernelGGL((k_force_eom_move<R, KIND, true>), g, t, 0, st, b, b.pos, b.pos_next, bin);
+        hipLaunchKernelGGL((k_force_eom_move<R, KIND, true>), g, t, 0, st, b, b.pos, b.pos_next, flags);
     else
-        hipLaunchKernelGGL((k_force_eom_move<R, KIND, false>), g, t, 0, st, b, b.pos, b.pos_next, bin);
+        hipLaunchKernelGGL((k_force_eom_move<R, KIND, false>), g, t, 0, st, b, b.pos, b.pos_next, flags);
 }
 
 }  // namespace
@@ -2032,18 +2049,18 @@ int cgpu_voronoi_forces(const cgpu_voronoi_batch *b, void *stream) {
     return check();
 }
 
-static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream, int bin = 0) {
+static int force_eom_move(const cgpu_voronoi_batch *b, int kind, void *stream, int flags = 0) {
     hipStream_t st = S(stream);
     if (b->precision) {
         if (kind == 0)
-            launch_force_eom_move<float, 0>(*b, st, bin);
+            launch_force_eom_move<float, 0>(*b, st, flags);
         else
-            launch_force_eom_move<float, 1>(*b, st, bin);
+            launch_force_eom_move<float, 1>(*b, st, flags);
     } else {
         if (kind == 0)
-            launch_force_eom_move<double, 0>(*b, st, bin);
+            launch_force_eom_move<double, 0>(*b, st, flags);
         else
-            launch_force_eom_move<double, 1>(*b, st, bin);
+            launch_force_eom_move<double, 1>(*b, st, flags);
     }
     return check();
 }
@@ -2060,20 +2077,18 @@ int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream) {
 // One step; after it, b->pos_next holds the new positions (the caller swaps).
 static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
     hipStream_t st = S(stream);
-    hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
-    // the force kernel also runs the binning pass of the cell list on the new
-    // positions (bin counts are zero: the previous scan cleared them)
-    int r = force_eom_move(b, kind, stream, 1);
+    // the force kernel also clears the step's topology state and runs the
+    // binning pass of the cell list on the new positions (bin counts are zero:
+    // the previous scan cleared them)
+    int r = force_eom_move(b, kind, stream, STEP_BIN | STEP_RESET);
     if (r) return r;
     launch_bin_scan(*b, st);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
     r = check();
     if (r) return r;
     launch_test_geometry(*b, b->pos_next, 1, st);
-    r = repair_chain(*b, b->pos_next, st);
-    if (r) return r;
-    hipLaunchKernelGGL(k_advance_step, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
-    return check();
+    // its last kernel advances the step counters
+    return repair_chain(*b, b->pos_next, st, 1);
 }
 
 int cgpu_step_begin(const cgpu_voronoi_batch *b, void *stream) {
