@@ -157,7 +157,10 @@ static __host__ __device__ __noinline__ int orient_exact(double ax, double ay, d
 // sign of  |b|^2 (c x d) + |c|^2 (d x b) + |d|^2 (b x c)
 static __host__ __device__ __noinline__ int incircle_exact(double bx, double by, double cx, double cy, double dx,
                                                      double dy) {
-    double acc[200];
+    // every grow() adds at most one component: norm2 and cross give at most 4
+    // each, add_scaled adds 2 per pair, so the three terms reach at most
+    // 3 * 4 * 4 * 2 = 96 (the array is per-lane stack, kept to that bound)
+    double acc[96];
     int n = 0;
     double nrm[8], crs[8];
     int nn, nc;
