@@ -115,6 +115,36 @@ def test_spv_steps_match_oracle_stepwise(oracle, precision):
     assert _stepwise(oracle, b, 60, 5) > 0
 
 
+def test_brownian_steps_match_oracle_stepwise(oracle):
+    """Voronoi model under brownianParticleDynamics (reference
+    brownianParticleDynamics.cu:20-39; displacement sqrt(2 T mu dt) eta + dt mu F,
+    x then y per cell): every step from the GPU state with the same noise,
+    positions within 1e-12 L, rings bit-exact."""
+    cg = _cg()
+    N, T, dt = 300, 0.05, 0.01
+    b = cg.VoronoiBatch([N], seeds=[4], precision="fp64")
+    b.set_prefs(1.0, 3.8)
+    b.set_params(dt=dt, T=T)
+    b.set_noise_buffer(True)
+    rng = np.random.default_rng(17)
+    repairs = 0
+    for step in range(40):
+        o = _oracle_from_gpu(oracle, b, P0=3.8, v0=0.0, dt=dt)
+        ex, ey = rng.standard_normal(N), rng.standard_normal(N)
+        b.upload_noise(ex, ey)
+        b.brownian_steps(1)
+        b.check_status()
+        o.brownian_step(ex, ey, T)
+        g, r = b.get(), o.get()
+        np.testing.assert_allclose(g["x"], r["x"], rtol=0, atol=1e-12 * b.L[0])
+        np.testing.assert_allclose(g["y"], r["y"], rtol=0, atol=1e-12 * b.L[0])
+        nn, nbr, nm = b.topology(CAP)
+        onn, onbr, onm = o.topology(CAP)
+        assert _rings(nn, nbr) == _rings(onn, onbr), "step %d" % step
+        repairs += o.counters()["repair_events"]
+    assert repairs > 0
+
+
 @pytest.mark.parametrize("N", [30, 48])
 def test_small_systems_bit_exact(oracle, N):
     """Systems whose second bin block already covers the whole box."""
