@@ -157,6 +157,32 @@ def test_small_systems_bit_exact(oracle, N):
     assert _stepwise(oracle, b, 20, 9) >= 0
 
 
+@pytest.mark.parametrize("N", [8, 12, 20, 24])
+def test_tiny_systems_triangulate_or_refuse(oracle, N):
+    """Below about 30 cells at unit density a periodic Delaunay triangulation can
+    need a second image of a neighbour, which a ring of cell indices cannot hold.
+    Then the oracle fails and the library refuses the system at construction.
+    Otherwise the rings agree. The bin grid here is 2 x 2 or 4 x 4, so bin
+    blocks repeat bins."""
+    cg = _cg()
+    refused = 0
+    for seed in range(1, 9):
+        print("tiny system N=%d seed=%d" % (N, seed), flush=True)
+        x, y, th, L = oracle.random_init(N, seed)
+        try:
+            onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+        except RuntimeError:
+            with pytest.raises(cg.CellGPUError):
+                cg.VoronoiBatch([N], seeds=[seed], precision="fp64")
+            refused += 1
+            continue
+        b = cg.VoronoiBatch([N], seeds=[seed], precision="fp64")
+        nn, nbr, nm = b.topology(CAP)
+        assert _rings(nn, nbr) == _rings(onn, onbr), "seed %d" % seed
+    # the oracle refuses every seed at N = 8, 12 and two of eight at N = 20, 24
+    assert refused == (8 if N <= 12 else 2)
+
+
 def test_clustered_points_bit_exact(oracle):
     """A dense cluster (about 15 points per bin) puts more points in a cell's bin
     block than one wavefront holds: those stars take the serial path."""
