@@ -1044,13 +1044,16 @@ void launch_bin_scan(const cgpu_voronoi_batch &b, hipStream_t st) {
     hipLaunchKernelGGL(k_bin_scan_chunks, dim3(b.num_bin_chunks), dim3(SCAN_THREADS), 0, st, b);
 }
 
-// Pass 3: every cell to its bin-ordered slot, with its position.
+// Pass 3: every cell to its bin-ordered slot, with its position. The rank is
+// consumed here and replaced by the slot (the circumcircle test skips a
+// triangle's own corners by slot).
 __global__ void k_bin_scatter(cgpu_voronoi_batch b, const double *pos) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int k = b.bin_start[b.cell_bin[i]] + b.cell_rank[i];
     b.sorted_id[k] = i;
     reinterpret_cast<double2 *>(b.sorted_pos)[k] = reinterpret_cast<const double2 *>(pos)[i];
+    b.cell_rank[i] = k;
 }
 
 // flags cell m of system s for repair (once per step) and counts it per system,
@@ -1182,6 +1185,9 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
     if (system_goes_global(b, s, sp)) return true;
     const Box box = make_box(sp.L);
     const dvec2 v = ldpos(pos, i);
+    // the corners' bin-ordered slots (cell_rank after the scatter pass): they are
+    // excluded from the scan without loading them
+    const int ki = b.cell_rank[i], kj = b.cell_rank[j], kl = b.cell_rank[l];
     const TriWindow w = tri_window(sp, min_dist(box, ldpos(pos, j), v), min_dist(box, ldpos(pos, l), v));
     if (w.wcheck != 1 || w.all_inside) return false;
     const int xs = sp.xsize;
@@ -1225,19 +1231,19 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
         }
     }
     bool bad = false;
+    auto corner = [&](int k) { return k == ki || k == kj || k == kl; };
     auto scan_run = [&](int k0, int k1) {
-        // two candidates per iteration: their loads are in flight together
+        // two candidates per iteration: their loads are in flight together; a
+        // candidate's id is read only when it lies inside
         for (int k = k0; k < k1; k += 2) {
-            const bool two = k + 1 < k1;
-            const int m0 = b.sorted_id[k];
-            const int m1 = two ? b.sorted_id[k + 1] : -1;
-            const dvec2 q0 = ldpos(b.sorted_pos, k);
-            const dvec2 q1 = two ? ldpos(b.sorted_pos, k + 1) : q0;
-            const dvec2 a0 = min_dist(box, q0, v), a1 = min_dist(box, q1, v);
-            const bool in0 = !(m0 == i || m0 == j || m0 == l) && cand_inside(box, w, a0);
-            const bool in1 = m1 >= 0 && !(m1 == i || m1 == j || m1 == l) && cand_inside(box, w, a1);
-            if (in0) flag_cell(b, s, m0);
-            if (in1) flag_cell(b, s, m1);
+            const bool c0 = !corner(k);
+            const bool c1 = k + 1 < k1 && !corner(k + 1);
+            const dvec2 q0 = c0 ? ldpos(b.sorted_pos, k) : v;
+            const dvec2 q1 = c1 ? ldpos(b.sorted_pos, k + 1) : v;
+            const bool in0 = c0 && cand_inside(box, w, min_dist(box, q0, v));
+            const bool in1 = c1 && cand_inside(box, w, min_dist(box, q1, v));
+            if (in0) flag_cell(b, s, b.sorted_id[k]);
+            if (in1) flag_cell(b, s, b.sorted_id[k + 1]);
             bad = bad || in0 || in1;
         }
     };

@@ -106,7 +106,8 @@ typedef struct cgpu_voronoi_batch {
     unsigned int *bin_count;     /* [total_bins] cells per bin (counting pass)      */
     int *bin_start;              /* [total_bins+1] first bin-ordered slot of a bin  */
     int *cell_bin;               /* [N] bin of each cell                            */
-    int *cell_rank;              /* [N] rank of each cell within its bin            */
+    int *cell_rank;              /* [N] rank of each cell within its bin; after the
+                                    scatter pass, its bin-ordered slot              */
     int *sorted_id;              /* [N] cell indices in bin order                   */
     double *sorted_pos;          /* [2N] positions in bin order                     */
     int *repair;                 /* [N] flag (cleared by consumer)  */
