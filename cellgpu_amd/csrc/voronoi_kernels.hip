@@ -1075,8 +1075,11 @@ __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int s, int m) {
 // More than N/6 flagged cells already send the system to a global
 // retriangulation (|NeedsFixing| >= |flagged|, voronoiModelBase.cpp:650-653),
 // whatever else the test would flag, so its remaining scans can stop.
+// The count only grows during the test and only lets work be skipped, so a
+// cached (workgroup-scope) read that may lag behind other work-groups' flags is
+// enough: no cache bypass on every triangle.
 __device__ inline bool system_goes_global(const cgpu_voronoi_batch &b, int s, const cgpu_sys_params &sp) {
-    return __atomic_load_n(&b.st[s].nflag, __ATOMIC_RELAXED) > sp.n / 6;
+    return __hip_atomic_load(&b.st[s].nflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > sp.n / 6;
 }
 
 // Reference window of one triangle (voronoiModelBase.cu:24-109): circumcircle
