@@ -911,49 +911,66 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
 // cell. Each thread sums 8 consecutive bins, the block scans the thread sums.
 constexpr int SCAN_THREADS = 1024, SCAN_PER = 8;
 __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b) {
+    // element e = r * SCAN_THREADS + t of a pass belongs to thread t, so every
+    // load and store of the counts and starts is coalesced. The exclusive prefix
+    // of (r, t) is the sum of the (row, wavefront) partial sums before (r, t's
+    // wavefront) in row-major order, plus the lanes before t in its wavefront.
+    constexpr int NW = SCAN_THREADS / 64;
+    constexpr int NP = SCAN_PER * NW;  // partial sums per pass: two wavefronts
+    static_assert(NP == 128, "the partial sums are scanned by two wavefronts");
     const int s = blockIdx.x;
     const cgpu_sys_params &sp = b.sys[s];
     const int nbins = sp.xsize * sp.xsize;
     unsigned int *cnt = b.bin_count + sp.bin_off;
     int *start = b.bin_start + sp.bin_off;
-    __shared__ int wsum[SCAN_THREADS / 64];
+    __shared__ int part[NP], pex[NP], ptot[2];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = sp.off;
-    __syncthreads();
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
     for (int base = 0; base < nbins; base += SCAN_THREADS * SCAN_PER) {
-        const int u0 = base + threadIdx.x * SCAN_PER;
-        int c[SCAN_PER];
-        int tot = 0;
+        int c[SCAN_PER], incl[SCAN_PER];
 #pragma unroll
         for (int r = 0; r < SCAN_PER; ++r) {
-            c[r] = (u0 + r < nbins) ? (int)cnt[u0 + r] : 0;
-            tot += c[r];
+            const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
+            c[r] = (u < nbins) ? (int)cnt[u] : 0;
         }
         // the counts are consumed: clear them for the next cell list
 #pragma unroll
-        for (int r = 0; r < SCAN_PER; ++r)
-            if (u0 + r < nbins) cnt[u0 + r] = 0u;
-        int incl = tot;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
+        for (int r = 0; r < SCAN_PER; ++r) {
+            const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
+            if (u < nbins) cnt[u] = 0u;
         }
-        if (lane == 63) wsum[wv] = incl;
-        __syncthreads();
-        int off = carry + incl - tot;
-        for (int k = 0; k < wv; ++k) off += wsum[k];
 #pragma unroll
         for (int r = 0; r < SCAN_PER; ++r) {
-            if (u0 + r < nbins) start[u0 + r] = off;
-            off += c[r];
+            int x = c[r];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(x, o);
+                if (lane >= o) x += t;
+            }
+            incl[r] = x;
+            if (lane == 63) part[r * NW + wv] = x;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int t = 0;
-            for (int k = 0; k < SCAN_THREADS / 64; ++k) t += wsum[k];
-            carry += t;
+        if ((int)threadIdx.x < NP) {  // wavefronts 0 and 1, whole
+            const int v = part[threadIdx.x];
+            int x = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(x, o);
+                if (lane >= o) x += t;
+            }
+            pex[threadIdx.x] = x - v;
+            if (lane == 63) ptot[wv] = x;
         }
+        __syncthreads();
+        const int c0 = carry;
+#pragma unroll
+        for (int r = 0; r < SCAN_PER; ++r) {
+            const int q = r * NW + wv;
+            const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
+            if (u < nbins) start[u] = c0 + pex[q] + (q >= 64 ? ptot[0] : 0) + incl[r] - c[r];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c0 + ptot[0] + ptot[1];
         __syncthreads();
     }
     if (s == b.num_systems - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
