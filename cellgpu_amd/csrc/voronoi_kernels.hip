@@ -910,35 +910,31 @@ __global__ void k_bin(cgpu_voronoi_batch b, const double *pos) {
 // system's cells fill its bins, so the run of system s starts at its first
 // cell. Each thread sums 8 consecutive bins, the block scans the thread sums.
 constexpr int SCAN_THREADS = 1024, SCAN_PER = 8;
-__global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b) {
-    // element e = r * SCAN_THREADS + t of a pass belongs to thread t, so every
-    // load and store of the counts and starts is coalesced. The exclusive prefix
-    // of (r, t) is the sum of the (row, wavefront) partial sums before (r, t's
-    // wavefront) in row-major order, plus the lanes before t in its wavefront.
+// Exclusive prefix sum of the counts cnt[0 .. n) into start[0 .. n), from
+// carry0, by the whole block; the counts are cleared for the next cell list.
+// Element e = r * SCAN_THREADS + t of a pass belongs to thread t, so every load
+// and store is coalesced. The exclusive prefix of (r, t) is the sum of the
+// (row, wavefront) partial sums before (r, t's wavefront) in row-major order,
+// plus the lanes before t in its wavefront.
+__device__ __forceinline__ void scan_bins(unsigned int *cnt, int *start, int n, int carry0) {
     constexpr int NW = SCAN_THREADS / 64;
     constexpr int NP = SCAN_PER * NW;  // partial sums per pass: two wavefronts
     static_assert(NP == 128, "the partial sums are scanned by two wavefronts");
-    const int s = blockIdx.x;
-    const cgpu_sys_params &sp = b.sys[s];
-    const int nbins = sp.xsize * sp.xsize;
-    unsigned int *cnt = b.bin_count + sp.bin_off;
-    int *start = b.bin_start + sp.bin_off;
     __shared__ int part[NP], pex[NP], ptot[2];
     __shared__ int carry;
-    if (threadIdx.x == 0) carry = sp.off;
+    if (threadIdx.x == 0) carry = carry0;
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
-    for (int base = 0; base < nbins; base += SCAN_THREADS * SCAN_PER) {
+    for (int base = 0; base < n; base += SCAN_THREADS * SCAN_PER) {
         int c[SCAN_PER], incl[SCAN_PER];
 #pragma unroll
         for (int r = 0; r < SCAN_PER; ++r) {
             const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
-            c[r] = (u < nbins) ? (int)cnt[u] : 0;
+            c[r] = (u < n) ? (int)cnt[u] : 0;
         }
-        // the counts are consumed: clear them for the next cell list
 #pragma unroll
         for (int r = 0; r < SCAN_PER; ++r) {
             const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
-            if (u < nbins) cnt[u] = 0u;
+            if (u < n) cnt[u] = 0u;
         }
 #pragma unroll
         for (int r = 0; r < SCAN_PER; ++r) {
@@ -967,12 +963,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b)
         for (int r = 0; r < SCAN_PER; ++r) {
             const int q = r * NW + wv;
             const int u = base + r * SCAN_THREADS + (int)threadIdx.x;
-            if (u < nbins) start[u] = c0 + pex[q] + (q >= 64 ? ptot[0] : 0) + incl[r] - c[r];
+            if (u < n) start[u] = c0 + pex[q] + (q >= 64 ? ptot[0] : 0) + incl[r] - c[r];
         }
         __syncthreads();
         if (threadIdx.x == 0) carry = c0 + ptot[0] + ptot[1];
         __syncthreads();
     }
+}
+
+// one block per system (every system one bin chunk)
+__global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan(cgpu_voronoi_batch b) {
+    const int s = blockIdx.x;
+    const cgpu_sys_params &sp = b.sys[s];
+    scan_bins(b.bin_count + sp.bin_off, b.bin_start + sp.bin_off, sp.xsize * sp.xsize, sp.off);
     if (s == b.num_systems - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
 }
 
@@ -1007,46 +1010,7 @@ __global__ void k_bin_chunk_offsets(cgpu_voronoi_batch b) {
 __global__ __launch_bounds__(SCAN_THREADS) void k_bin_scan_chunks(cgpu_voronoi_batch b) {
     const int cidx = blockIdx.x;
     const int u0 = b.bchunk_begin[cidx], u1 = b.bchunk_end[cidx];
-    unsigned int *cnt = b.bin_count;
-    __shared__ int wsum[SCAN_THREADS / 64];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = b.bchunk_sum[cidx];
-    __syncthreads();
-    const int lane = __lane_id(), wv = threadIdx.x >> 6;
-    for (int base = u0; base < u1; base += SCAN_THREADS * SCAN_PER) {
-        const int v0 = base + threadIdx.x * SCAN_PER;
-        int c[SCAN_PER];
-        int tot = 0;
-#pragma unroll
-        for (int r = 0; r < SCAN_PER; ++r) {
-            c[r] = (v0 + r < u1) ? (int)cnt[v0 + r] : 0;
-            tot += c[r];
-        }
-#pragma unroll
-        for (int r = 0; r < SCAN_PER; ++r)
-            if (v0 + r < u1) cnt[v0 + r] = 0u;
-        int incl = tot;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
-        if (lane == 63) wsum[wv] = incl;
-        __syncthreads();
-        int off = carry + incl - tot;
-        for (int k = 0; k < wv; ++k) off += wsum[k];
-#pragma unroll
-        for (int r = 0; r < SCAN_PER; ++r) {
-            if (v0 + r < u1) b.bin_start[v0 + r] = off;
-            off += c[r];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int t = 0;
-            for (int k = 0; k < SCAN_THREADS / 64; ++k) t += wsum[k];
-            carry += t;
-        }
-        __syncthreads();
-    }
+    scan_bins(b.bin_count + u0, b.bin_start + u0, u1 - u0, b.bchunk_sum[cidx]);
     if (cidx == b.num_bin_chunks - 1 && threadIdx.x == 0) b.bin_start[b.total_bins] = b.num_cells;
 }
 
