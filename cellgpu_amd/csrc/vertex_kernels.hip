@@ -479,10 +479,15 @@ __device__ __forceinline__ void t1_rounds_loop(const cgpu_vertex_batch &b, doubl
         const int nq = count;
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
             int v1 = list[2 * q], v2 = list[2 * q + 1];
+            // the edge test and the cells of its two vertices are loaded together
+            // (every vertex has its three slots)
             bool live = b.vn[3 * v1] == v2 || b.vn[3 * v1 + 1] == v2 || b.vn[3 * v1 + 2] == v2;
-            int C[4] = {-1, -1, -1, -1};
-            if (live) edge_cells(b, v1, v2, C);
-            else list[2 * q] = -2;  // dropped
+            int C[4];
+            edge_cells(b, v1, v2, C);
+            if (!live) {
+                C[0] = C[1] = C[2] = C[3] = -1;
+                list[2 * q] = -2;  // dropped
+            }
             unsigned long long k = ((unsigned long long)(unsigned)v1 << 32) | (unsigned)v2;
             reinterpret_cast<int4 *>(cells)[q] = make_int4(C[0], C[1], C[2], C[3]);
             for (int t = 0; t < 4; ++t)
@@ -684,6 +689,9 @@ template <typename R> __global__ void k_vm_fire_apply_mix(cgpu_vertex_batch b, c
 }
 
 inline int nb(int n, int bs) { return (n + bs - 1) / bs; }
+// per-cell and per-vertex kernels: small work-groups, so a single tissue of tens
+// of thousands of cells spreads over every CU
+constexpr int VM_BLOCK = 64;
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
 
@@ -694,15 +702,15 @@ extern "C" {
 int cgpu_vm_geometry(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
     if (b->precision)
-        hipLaunchKernelGGL(k_vm_geometry<float>, dim3(nb(b->num_cells, 256)), dim3(256), 0, S(stream), *b, b->pos);
+        hipLaunchKernelGGL(k_vm_geometry<float>, dim3(nb(b->num_cells, VM_BLOCK)), dim3(VM_BLOCK), 0, S(stream), *b, b->pos);
     else
-        hipLaunchKernelGGL(k_vm_geometry<double>, dim3(nb(b->num_cells, 256)), dim3(256), 0, S(stream), *b, b->pos);
+        hipLaunchKernelGGL(k_vm_geometry<double>, dim3(nb(b->num_cells, VM_BLOCK)), dim3(VM_BLOCK), 0, S(stream), *b, b->pos);
     return check();
 }
 
 int cgpu_vm_forces(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    dim3 g(nb(b->num_vertices, 256)), t(256);
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
         hipLaunchKernelGGL((k_vm_force_eom_move<float, 0>), g, t, 0, S(stream), *b, b->pos, b->pos_next, 0);
     else
@@ -721,7 +729,7 @@ template <int KIND> int vm_step(const cgpu_vertex_batch *b, void *stream) {
     hipStream_t st = S(stream);
     int r = cgpu_vm_geometry(b, stream);
     if (r) return r;
-    dim3 g(nb(b->num_vertices, 256)), t(256);
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
         hipLaunchKernelGGL((k_vm_force_eom_move<float, KIND>), g, t, 0, st, *b, b->pos, b->pos_next, 1);
     else
@@ -740,14 +748,14 @@ int cgpu_vm_spp_step(const cgpu_vertex_batch *b, void *stream) { return vm_step<
 
 int cgpu_vm_t1(const cgpu_vertex_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    hipLaunchKernelGGL(k_vm_t1_test, dim3(nb(b->num_vertices, 256)), dim3(256), 0, S(stream), *b, b->pos, 0);
+    hipLaunchKernelGGL(k_vm_t1_test, dim3(nb(b->num_vertices, VM_BLOCK)), dim3(VM_BLOCK), 0, S(stream), *b, b->pos, 0);
     hipLaunchKernelGGL(k_vm_t1_rounds, dim3(b->num_systems), dim3(256), 0, S(stream), *b, b->pos, 0);
     return check();
 }
 
 int cgpu_vm_fire_vv_move(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
     if (!b || !fs || !vel) return CGPU_ERR_ARGS;
-    dim3 g(nb(b->num_vertices, 256)), t(256);
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
         hipLaunchKernelGGL(k_vm_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel);
     else
@@ -762,7 +770,7 @@ int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, v
     if (!b || !fs || !vel || !dots || !b->chunk_sys) return CGPU_ERR_ARGS;
     int r = cgpu_vm_geometry(b, stream);
     if (r) return r;
-    dim3 g(nb(b->num_vertices, 256)), t(256);
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
         hipLaunchKernelGGL(k_vm_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, dots);
     else
@@ -780,7 +788,7 @@ int cgpu_vm_fire_park_converged(const cgpu_vertex_batch *b, const cgpu_fire_stat
 
 int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
     if (!b || !fs || !vel) return CGPU_ERR_ARGS;
-    dim3 g(nb(b->num_vertices, 256)), t(256);
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
         hipLaunchKernelGGL(k_vm_fire_apply_mix<float>, g, t, 0, S(stream), *b, fs, vel);
     else
