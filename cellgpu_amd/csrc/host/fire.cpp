@@ -146,20 +146,10 @@ class VoronoiFire : public FireBase {
         return cgpu_voronoi_forces(&b, stream());
     }
     int iterate() override {
-        hipStream_t st = stream();
         cgpu_voronoi_batch v = vb_->view();
-        cgpu_voronoi_batch vn = v;
-        vn.pos = v.pos_next;
-        vn.pos_next = v.pos;
-        int r;
-        if ((r = cgpu_step_begin(&v, st))) return r;
-        if ((r = cgpu_fire_vv_move(&v, fs_.p, vel_.p, st))) return r;
-        if ((r = cgpu_compute_cell_list(&v, 1, st))) return r;
-        if ((r = cgpu_test_and_geometry(&vn, 1, st))) return r;
-        if ((r = cgpu_repair_topology(&vn, st))) return r;
+        const int r = cgpu_fire_iteration(&v, fs_.p, vel_.p, dots_.p, stream());
         vb_->swap_parity();
-        if ((r = cgpu_fire_force_reduce(&vn, fs_.p, vel_.p, dots_.p, st))) return r;
-        return cgpu_fire_park_converged(&vn, fs_.p, st);
+        return r;
     }
     int apply_mix() override {
         cgpu_voronoi_batch v = vb_->view();

@@ -1833,9 +1833,15 @@ __global__ void k_fill_normal(cgpu_voronoi_batch b, int step, int stream, double
 
 // ------------------------------------------------------------------------ FIRE
 // (shared pieces and the iteration: fire_device.h)
+// flags as for k_force_eom_move: STEP_RESET also clears the step's topology
+// state, STEP_BIN also runs the binning pass of the cell list on the new
+// positions
 template <typename R>
 __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
-                               double *pos_next) {
+                               double *pos_next, int flags) {
+    if (flags & STEP_RESET)
+        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
+            reset_system(b, s);
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
@@ -1847,6 +1853,7 @@ __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, 
     if (f.converged || !b.st[s].active) {
         st2<R>(vel, i, v);
         stpos(pos_next, i, p);
+        if (flags & STEP_BIN) bin_cell(b, sp, i, p);
         return;
     }
     vec2<R> d = fire_vv_first_half<R>(v, F, (R)f.dt);
@@ -1860,6 +1867,7 @@ __global__ void k_fire_vv_move(cgpu_voronoi_batch b, const cgpu_fire_state *fs, 
     }
     put_in_box(make_box(sp.L), p);
     stpos(pos_next, i, p);
+    if (flags & STEP_BIN) bin_cell(b, sp, i, p);
 }
 
 __global__ void k_fire_clear_pending(cgpu_fire_state *fs, int S) {
@@ -1867,9 +1875,14 @@ __global__ void k_fire_clear_pending(cgpu_fire_state *fs, int S) {
     if (s < S) fs[s].pending = 0;
 }
 
+// clear: also clears every system's pending mix (k_fire_clear_pending), which
+// only the move kernel reads
 template <typename R, bool TENS>
-__global__ void k_fire_force_vv2(cgpu_voronoi_batch b, const cgpu_fire_state *fs, void *vel, const double *pos,
-                                 double *dots) {
+__global__ void k_fire_force_vv2(cgpu_voronoi_batch b, cgpu_fire_state *fs, void *vel, const double *pos,
+                                 double *dots, int clear) {
+    if (clear)
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < b.num_systems; q += gridDim.x * blockDim.x)
+            fs[q].pending = 0;
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= b.num_cells) return;
     const int s = b.cell_sys[i];
@@ -2112,31 +2125,66 @@ int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *ve
     if (!b || !fs || !vel) return CGPU_ERR_ARGS;
     dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
     if (b->precision)
-        hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next);
+        hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next, 0);
     else
-        hipLaunchKernelGGL(k_fire_vv_move<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next);
+        hipLaunchKernelGGL(k_fire_vv_move<double>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next, 0);
     hipLaunchKernelGGL(k_fire_clear_pending, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
                        b->num_systems);
     return check();
 }
 
-int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
-    if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
+}  // extern "C"
+
+namespace {
+// forces, second velocity half step and the FIRE reductions on b->pos
+void fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, hipStream_t st,
+                       int clear) {
     dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
     const bool tens = b->cell_type != nullptr;
     if (b->precision) {
         if (tens)
-            hipLaunchKernelGGL((k_fire_force_vv2<float, true>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+            hipLaunchKernelGGL((k_fire_force_vv2<float, true>), g, t, 0, st, *b, fs, vel, b->pos, dots, clear);
         else
-            hipLaunchKernelGGL((k_fire_force_vv2<float, false>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+            hipLaunchKernelGGL((k_fire_force_vv2<float, false>), g, t, 0, st, *b, fs, vel, b->pos, dots, clear);
     } else {
         if (tens)
-            hipLaunchKernelGGL((k_fire_force_vv2<double, true>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+            hipLaunchKernelGGL((k_fire_force_vv2<double, true>), g, t, 0, st, *b, fs, vel, b->pos, dots, clear);
         else
-            hipLaunchKernelGGL((k_fire_force_vv2<double, false>), g, t, 0, S(stream), *b, fs, vel, b->pos, dots);
+            hipLaunchKernelGGL((k_fire_force_vv2<double, false>), g, t, 0, st, *b, fs, vel, b->pos, dots, clear);
     }
-    hipLaunchKernelGGL(k_fire_partial, dim3(b->num_chunks), dim3(256), 0, S(stream), *b, fs, dots);
-    hipLaunchKernelGGL(k_fire_reduce, dim3((b->num_systems + 63) / 64), dim3(64), 0, S(stream), *b, fs, dots);
+    hipLaunchKernelGGL(k_fire_partial, dim3(b->num_chunks), dim3(256), 0, st, *b, fs, dots);
+    hipLaunchKernelGGL(k_fire_reduce, dim3((b->num_systems + 63) / 64), dim3(64), 0, st, *b, fs, dots);
+}
+}  // namespace
+
+extern "C" {
+
+int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
+    if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
+    fire_force_reduce(b, fs, vel, dots, S(stream), 0);
+    return check();
+}
+
+int cgpu_fire_iteration(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
+    if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
+    hipStream_t st = S(stream);
+    // the move also clears the topology state and bins the moved cells
+    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    if (b->precision)
+        hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, st, *b, fs, vel, b->pos, b->pos_next, STEP_RESET | STEP_BIN);
+    else
+        hipLaunchKernelGGL(k_fire_vv_move<double>, g, t, 0, st, *b, fs, vel, b->pos, b->pos_next, STEP_RESET | STEP_BIN);
+    launch_bin_scan(*b, st);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
+    cgpu_voronoi_batch bn = *b;  // the rest works on the moved positions
+    bn.pos = b->pos_next;
+    bn.pos_next = b->pos;
+    launch_test_geometry(bn, bn.pos, 1, st);
+    int r = repair_chain(bn, bn.pos, st);
+    if (r) return r;
+    // the force pass also clears the pending mix that the move applied
+    fire_force_reduce(&bn, fs, vel, dots, st, 1);
+    hipLaunchKernelGGL(k_fire_park, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, bn, fs);
     return check();
 }
 

@@ -274,6 +274,15 @@ int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *ve
 int cgpu_fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
 /* mark systems whose FIRE loop has stopped as inactive for the topology kernels */
 int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state *fs, void *stream);
+/* One FIRE iteration of every running system, fused: the sequence
+   cgpu_step_begin, cgpu_fire_vv_move, cgpu_compute_cell_list(use_next = 1),
+   then on the moved positions cgpu_test_and_geometry, cgpu_repair_topology,
+   cgpu_fire_force_reduce and cgpu_fire_park_converged, with the reset and the
+   binning pass inside the move kernel and the pending-mix clear inside the force
+   pass. Afterwards b->pos_next holds the new positions (the caller swaps).
+   Replaces the per-iteration kernel sequence of EnergyMinimizerFIRE2D.cpp:93-114
+   and :156-207. */
+int cgpu_fire_iteration(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
 /* apply a pending FIRE velocity mixing (end of minimize) */
 int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
