@@ -627,15 +627,10 @@ class VertexFire : public FireBase {
         return r ? r : cgpu_vm_forces(&b, stream());
     }
     int iterate() override {
-        hipStream_t st = stream();
         cgpu_vertex_batch b = vm_->view();
-        int r;
-        if ((r = cgpu_vm_fire_vv_move(&b, fs_.p, vel_.p, st))) return r;
+        const int r = cgpu_vm_fire_iteration(&b, fs_.p, vel_.p, dots_.p, stream());
         vm_->swap_parity();
-        cgpu_vertex_batch bn = vm_->view();
-        if ((r = cgpu_vm_t1(&bn, st))) return r;
-        if ((r = cgpu_vm_fire_force_reduce(&bn, fs_.p, vel_.p, dots_.p, st))) return r;
-        return cgpu_vm_fire_park_converged(&bn, fs_.p, st);
+        return r;
     }
     int apply_mix() override {
         cgpu_vertex_batch b = vm_->view();

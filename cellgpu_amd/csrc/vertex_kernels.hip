@@ -630,9 +630,13 @@ __global__ void k_vm_fire_clear_pending(cgpu_fire_state *fs, int S) {
 }
 
 // forces on the moved (and T1-flipped) vertices, second velocity half step,
-// per-vertex products of the reductions
+// per-vertex products of the reductions; clear: also clears every system's
+// pending mix (k_vm_fire_clear_pending), which only the move kernel reads
 template <typename R>
-__global__ void k_vm_fire_force_vv2(cgpu_vertex_batch b, const cgpu_fire_state *fs, void *vel, double *dots) {
+__global__ void k_vm_fire_force_vv2(cgpu_vertex_batch b, cgpu_fire_state *fs, void *vel, double *dots, int clear) {
+    if (clear)
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < b.num_systems; q += gridDim.x * blockDim.x)
+            fs[q].pending = 0;
     int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= b.num_vertices) return;
     const int s = b.vert_sys[v];
@@ -765,18 +769,49 @@ int cgpu_vm_fire_vv_move(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *
     return check();
 }
 
-int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots,
-                              void *stream) {
-    if (!b || !fs || !vel || !dots || !b->chunk_sys) return CGPU_ERR_ARGS;
+}  // extern "C"
+
+namespace {
+// geometry, forces, second velocity half step and the FIRE reductions
+int vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream,
+                         int clear) {
     int r = cgpu_vm_geometry(b, stream);
     if (r) return r;
     dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
     if (b->precision)
-        hipLaunchKernelGGL(k_vm_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, dots);
+        hipLaunchKernelGGL(k_vm_fire_force_vv2<float>, g, t, 0, S(stream), *b, fs, vel, dots, clear);
     else
-        hipLaunchKernelGGL(k_vm_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, dots);
+        hipLaunchKernelGGL(k_vm_fire_force_vv2<double>, g, t, 0, S(stream), *b, fs, vel, dots, clear);
     hipLaunchKernelGGL(k_vm_fire_partial, dim3(b->num_chunks), dim3(256), 0, S(stream), *b, fs, dots);
     hipLaunchKernelGGL(k_vm_fire_reduce, dim3(nb(b->num_systems, 64)), dim3(64), 0, S(stream), *b, fs, dots);
+    return check();
+}
+}  // namespace
+
+extern "C" {
+
+int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots,
+                              void *stream) {
+    if (!b || !fs || !vel || !dots || !b->chunk_sys) return CGPU_ERR_ARGS;
+    return vm_fire_force_reduce(b, fs, vel, dots, stream, 0);
+}
+
+int cgpu_vm_fire_iteration(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream) {
+    if (!b || !fs || !vel || !dots || !b->chunk_sys) return CGPU_ERR_ARGS;
+    dim3 g(nb(b->num_vertices, VM_BLOCK)), t(VM_BLOCK);
+    if (b->precision)
+        hipLaunchKernelGGL(k_vm_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel);
+    else
+        hipLaunchKernelGGL(k_vm_fire_vv_move<double>, g, t, 0, S(stream), *b, fs, vel);
+    cgpu_vertex_batch bn = *b;  // the rest works on the moved positions
+    bn.pos = b->pos_next;
+    bn.pos_next = b->pos;
+    hipLaunchKernelGGL(k_vm_t1_test, g, t, 0, S(stream), bn, bn.pos, 0);
+    hipLaunchKernelGGL(k_vm_t1_rounds, dim3(bn.num_systems), dim3(256), 0, S(stream), bn, bn.pos, 0);
+    // the force pass also clears the pending mix that the move applied
+    const int r = vm_fire_force_reduce(&bn, fs, vel, dots, stream, 1);
+    if (r) return r;
+    hipLaunchKernelGGL(k_vm_fire_park, dim3(nb(bn.num_systems, 256)), dim3(256), 0, S(stream), bn, fs);
     return check();
 }
 

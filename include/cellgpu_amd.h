@@ -311,6 +311,13 @@ int cgpu_vm_fire_vv_move(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *
 int cgpu_vm_fire_force_reduce(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots,
                               void *stream);
 int cgpu_vm_fire_park_converged(const cgpu_vertex_batch *b, const cgpu_fire_state *fs, void *stream);
+/* One vertex-model FIRE iteration of every running system, fused: the sequence
+   cgpu_vm_fire_vv_move, then on the moved positions cgpu_vm_t1,
+   cgpu_vm_fire_force_reduce and cgpu_vm_fire_park_converged, with the
+   pending-mix clear inside the force pass. Afterwards b->pos_next holds the new
+   positions (the caller swaps). Replaces the per-iteration kernel sequence of
+   EnergyMinimizerFIRE2D.cpp:93-114 and :156-207 on a vertex model. */
+int cgpu_vm_fire_iteration(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
 int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
 /* ---------------------------------------------------------------- object layer */
