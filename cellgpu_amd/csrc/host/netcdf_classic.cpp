@@ -313,7 +313,7 @@ void NcClassic::get_record(int v, int64_t rec, double *vals) const {
                 vals[k] = i;
                 break;
             }
-            case 5: {
+            case NC_FLOAT: {
                 float f;
                 get_be(&f, &b[k * ts], 4);
                 vals[k] = f;

@@ -19,7 +19,7 @@ namespace cgpu_host {
 
 class NcClassic {
    public:
-    enum Type { NC_INT = 4, NC_DOUBLE = 6 };
+    enum Type { NC_INT = 4, NC_FLOAT = 5, NC_DOUBLE = 6 };  // classic-format type codes
     struct Dim {
         std::string name;
         int64_t len;  // 0: the record dimension

@@ -549,21 +549,21 @@ int VoronoiBatch::spv_steps_timed(int steps, double *ms_out) {
         bn.pos = b.pos_next;
         bn.pos_next = b.pos;
         r = cgpu_step_begin(&b, stream_);
-        hipEventRecord(ev[0], stream_);
+        hip_check(hipEventRecord(ev[0], stream_), "event record");
         if (!r) r = cgpu_spv_force_eom_move(&b, stream_);
-        hipEventRecord(ev[1], stream_);
+        hip_check(hipEventRecord(ev[1], stream_), "event record");
         if (!r) r = cgpu_compute_cell_list(&b, 1, stream_);
-        hipEventRecord(ev[2], stream_);
+        hip_check(hipEventRecord(ev[2], stream_), "event record");
         if (!r) r = cgpu_test_and_geometry(&bn, 1, stream_);
-        hipEventRecord(ev[3], stream_);
+        hip_check(hipEventRecord(ev[3], stream_), "event record");
         if (!r) r = cgpu_repair_topology(&bn, stream_);
-        hipEventRecord(ev[4], stream_);
+        hip_check(hipEventRecord(ev[4], stream_), "event record");
         if (!r) r = cgpu_step_end(&b, stream_);
         parity_ ^= 1;
         hip_check(hipEventSynchronize(ev[4]), "event sync");
         for (int q = 0; q < 4; ++q) {
             float ms = 0;
-            hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+            hip_check(hipEventElapsedTime(&ms, ev[q], ev[q + 1]), "event time");
             acc[q] += ms;
         }
     }

@@ -56,8 +56,6 @@ template <> __device__ inline void st2<double>(void *a, int i, vec2<double> v) {
 }
 template <typename R> __device__ inline vec2<R> cvt(dvec2 d) { return vec2<R>{(R)d.x, (R)d.y}; }
 
-__device__ inline int ring_at(const cgpu_voronoi_batch &b, int i, int k) { return b.nbr[k * b.num_cells + i]; }
-
 // wave-aggregated reservation: lanes with pred get consecutive slots of *counter
 // (one atomic per wave instead of one per lane); returns -1 for the others
 __device__ inline int wave_reserve(bool pred, int *counter) {
@@ -800,32 +798,12 @@ __device__ inline int wave_owned(int i, int ring_lane, int z) {
     return __popcll(__ballot(lane < z && i < ring_lane && i < nxt));
 }
 
-// reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell i, slot-major.
-// A ring that does not contain nm1 leaves that slot's delOther unchanged.
-__device__ bool del_sets(const cgpu_voronoi_batch &b, int i) {
-    const int N = b.num_cells;
-    const int z = b.nn[i];
-    if (z < 3) return false;
-    int nm1 = b.nbr[(z - 2) * N + i], n1 = b.nbr[(z - 1) * N + i];
-    for (int k = 0; k < z; ++k) {
-        int n2 = b.nbr[k * N + i];
-        int zn = min(b.nn[n1], CAP);
-        for (int q = 0; q < zn; ++q) {
-            if (b.nbr[q * N + n1] == nm1) {
-                b.oth[k * N + i] = b.nbr[((q + 1) % zn) * N + n1];
-                break;
-            }
-        }
-        int o = b.oth[k * N + i];
-        if (nm1 == o || n1 == o || i == o) return false;
-        nm1 = n1;
-        n1 = n2;
-    }
-    return true;
-}
 
-// slot k of getDelSets for cell i (same rule as del_sets); false when the
-// slot's delOther is one of the triangle's own cells
+// slot k of the reference getDelSets (voronoiModelBase.cpp:1236-1275) for cell
+// i, slot-major: delSets[k] = (ring k-2, ring k-1) and delOther[k] the opposite
+// vertex across that edge, found in the ring of ring k-1; when that ring does
+// not contain ring k-2 the slot's delOther stays unchanged. False when the
+// slot's delOther is one of the triangle's own cells.
 __device__ bool del_set_slot(const cgpu_voronoi_batch &b, int i, int z, int k) {
     const int N = b.num_cells;
     const int nm1 = b.nbr[((k + z - 2) % z) * N + i], n1 = b.nbr[((k + z - 1) % z) * N + i];
@@ -853,7 +831,6 @@ __device__ inline int owned_triangles(const cgpu_voronoi_batch &b, int i) {
 
 // ===================================================================== kernels
 
-// per-step reset of per-system topology state and queue counters
 // the per-step topology state of system s (and, for s = 0, the queue counters)
 __device__ inline void reset_system(const cgpu_voronoi_batch &b, int s) {
     if (s == 0) {
@@ -1048,7 +1025,7 @@ __device__ inline void flag_cell(const cgpu_voronoi_batch &b, int s, int m) {
     const int leader = __ffsll((long long)fm) - 1;
     const int s0 = __shfl(s, leader);
     if (__ballot(fresh && s != s0) == 0ull) {
-        if (__lane_id() == leader) atomicAdd(&b.st[s0].nflag, __popcll(fm));
+        if ((int)__lane_id() == leader) atomicAdd(&b.st[s0].nflag, __popcll(fm));
     } else if (fresh) {
         atomicAdd(&b.st[s].nflag, 1);
     }
@@ -1373,7 +1350,7 @@ __global__ void k_mark_nf(cgpu_voronoi_batch b) {
                 const int leader = __ffsll((long long)fm) - 1;
                 const int s0 = __shfl(sx, leader);
                 if (__ballot(fresh && sx != s0) == 0ull) {
-                    if (__lane_id() == leader) atomicAdd(&b.st[s0].nf_count, __popcll(fm));
+                    if ((int)__lane_id() == leader) atomicAdd(&b.st[s0].nf_count, __popcll(fm));
                 } else if (fresh) {
                     atomicAdd(&b.st[sx].nf_count, 1);
                 }
@@ -1947,7 +1924,6 @@ inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR
 
 int g_block = 256;
 constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
-constexpr int REPAIR_THREADS = 64;
 constexpr int STAR_WAVE_BLOCKS = 4096;  // 16384 wavefronts, one queued cell each at a time
 constexpr int WIDE_BLOCKS = 256;        // two candidates per lane, for wide bin blocks
 
