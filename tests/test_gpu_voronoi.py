@@ -4,6 +4,8 @@ Bars: Delaunay rings bit-exact (canonical rotation), FP64 forces / positions
 within 1e-12 relative, FP32-bulk forces within 1e-5 of FP64 (max-norm), and
 step-by-step topology parity (same positions -> same repaired rings).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -507,3 +509,85 @@ def test_cell_events_in_a_batch():
     b.spv_steps(40)
     b.check_status()
     assert np.isfinite(b.get()["x"]).all()
+
+
+class _FireState(ctypes.Structure):
+    """Mirror of cgpu_fire_state (include/cellgpu_amd.h)."""
+
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "alpha", "alpha_start", "alpha_dec", "dt", "dt_max", "dt_inc", "dt_dec", "force_cutoff", "power",
+        "force_max", "scaling", "mix_alpha", "max_f2", "pad_d")] + [(n, ctypes.c_int) for n in (
+            "nmin", "nsince", "iterations", "max_iterations", "converged", "zero_v", "pending", "pad_i")]
+
+
+def test_fire_kernel_layer_fused_equals_piecewise():
+    """The kernel-layer FIRE iteration as one call (cgpu_fire_iteration) and as
+    its documented sequence of calls give bit-identical positions, rings and FIRE
+    scalars, driven through the C ABI with caller-owned device buffers."""
+    cg = _cg()
+    from cellgpu_amd._lib import VoronoiBatchView, lib
+    hip = ctypes.CDLL("libamdhip64.so")
+    L = lib()
+    VP, PV = ctypes.c_void_p, ctypes.POINTER(VoronoiBatchView)
+    for name, args in (("cgpu_fire_iteration", [PV, VP, VP, VP, VP]), ("cgpu_fire_vv_move", [PV, VP, VP, VP]),
+                       ("cgpu_fire_force_reduce", [PV, VP, VP, VP, VP]), ("cgpu_fire_park_converged", [PV, VP, VP]),
+                       ("cgpu_step_begin", [PV, VP]), ("cgpu_compute_cell_list", [PV, ctypes.c_int, VP]),
+                       ("cgpu_test_and_geometry", [PV, ctypes.c_int, VP]), ("cgpu_repair_topology", [PV, VP]),
+                       ("cgpu_voronoi_forces", [PV, VP])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = ctypes.c_int
+
+    def dev(nbytes):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) == 0
+        return p
+
+    runs = []
+    for fused in (True, False):
+        b = cg.VoronoiBatch([300, 500], seeds=[5, 6], precision="fp64")
+        b.set_prefs(1.0, 3.7)
+        b.get()  # construction work done
+        v = b.view()
+        st = ctypes.c_void_p(b.stream())
+        S, N = v.num_systems, v.num_cells
+        fs_h = (_FireState * S)()
+        for f in fs_h:  # EnergyMinimizerFIRE2D defaults, loop armed
+            f.alpha = f.alpha_start = 0.99
+            f.dt, f.dt_max, f.dt_inc, f.dt_dec, f.alpha_dec = 0.01, 0.1, 1.05, 0.95, 0.9
+            f.force_cutoff, f.force_max, f.nmin, f.max_iterations = 1e-12, 110.0, 5, 1000
+        fs, vel = dev(ctypes.sizeof(fs_h)), dev(2 * N * 8)
+        dots = dev((3 * N + 4 * v.num_chunks) * 8)
+        assert hip.hipMemcpy(fs, ctypes.byref(fs_h), ctypes.c_size_t(ctypes.sizeof(fs_h)), 1) == 0
+        assert hip.hipMemset(vel, 0, ctypes.c_size_t(2 * N * 8)) == 0
+        assert hip.hipDeviceSynchronize() == 0  # the batch stream does not wait for the null stream
+        print("FIRE kernel layer, fused=%s, S=%d N=%d chunks=%d" % (fused, S, N, v.num_chunks), flush=True)
+        assert L.cgpu_voronoi_forces(ctypes.byref(v), st) == 0
+        for it in range(40):
+            vn = VoronoiBatchView.from_buffer_copy(v)
+            vn.pos, vn.pos_next = v.pos_next, v.pos
+            if fused:
+                assert L.cgpu_fire_iteration(ctypes.byref(v), fs, vel, dots, st) == 0
+            else:
+                assert L.cgpu_step_begin(ctypes.byref(v), st) == 0
+                assert L.cgpu_fire_vv_move(ctypes.byref(v), fs, vel, st) == 0
+                assert L.cgpu_compute_cell_list(ctypes.byref(v), 1, st) == 0
+                assert L.cgpu_test_and_geometry(ctypes.byref(vn), 1, st) == 0
+                assert L.cgpu_repair_topology(ctypes.byref(vn), st) == 0
+                assert L.cgpu_fire_force_reduce(ctypes.byref(vn), fs, vel, dots, st) == 0
+                assert L.cgpu_fire_park_converged(ctypes.byref(vn), fs, st) == 0
+            v = vn
+        assert hip.hipStreamSynchronize(st) == 0
+        xy = np.zeros(2 * N)
+        assert hip.hipMemcpy(xy.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(v.pos), ctypes.c_size_t(16 * N), 2) == 0
+        out = (_FireState * S)()
+        assert hip.hipMemcpy(ctypes.byref(out), fs, ctypes.c_size_t(ctypes.sizeof(out)), 2) == 0
+        b.check_status()
+        nn, nbr, nm = b.topology(CAP)
+        runs.append((xy, [(f.iterations, f.dt, f.alpha, f.force_max) for f in out], _rings(nn, nbr)))
+        for p in (fs, vel, dots):
+            hip.hipFree(p)
+        b.close()
+    assert runs[0][1][0][0] == 40
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    assert runs[0][2] == runs[1][2]
