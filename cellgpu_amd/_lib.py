@@ -28,6 +28,25 @@ class SysParams(C.Structure):
                 ("ncells", C.c_int), ("tension", C.c_int), ("gamma", C.c_double)]
 
 
+class FireState(C.Structure):
+    """Mirror of cgpu_fire_state (include/cellgpu_amd.h)."""
+
+    _fields_ = [(n, C.c_double) for n in (
+        "alpha", "alpha_start", "alpha_dec", "dt", "dt_max", "dt_inc", "dt_dec", "force_cutoff", "power",
+        "force_max", "scaling", "mix_alpha", "max_f2", "pad_d")] + [(n, C.c_int) for n in (
+            "nmin", "nsince", "iterations", "max_iterations", "converged", "zero_v", "pending", "pad_i")]
+
+
+class VertexBatchView(C.Structure):
+    """Mirror of cgpu_vertex_batch (include/cellgpu_amd.h)."""
+
+    _fields_ = [("num_systems", C.c_int), ("num_vertices", C.c_int), ("num_cells", C.c_int), ("precision", C.c_int)] + [
+        (name, C.c_void_p) for name in (
+            "sys", "st", "vert_sys", "cell_sys", "pos", "pos_next", "vn", "vcn", "cvn", "cv", "AP", "dAP", "APpref",
+            "force", "flag_count", "flag_list", "flag_cells", "flag_touched", "cell_min", "noise", "theta",
+            "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("pad_c", C.c_int)]
+
+
 class VoronoiBatchView(C.Structure):
     _fields_ = [("num_systems", C.c_int), ("num_cells", C.c_int), ("total_bins", C.c_int), ("precision", C.c_int)] + [
         (name, C.c_void_p) for name in (
@@ -79,6 +98,8 @@ def lib():
         "cgpu_vor_device_bytes": (C.c_longlong, [vp]),
         "cgpu_vor_autotune": (i, [vp]),
         "cgpu_vor_view": (i, [vp, P(VoronoiBatchView)]),
+        "cgpu_vm_view": (i, [vp, P(VertexBatchView)]),
+        "cgpu_vm_stream": (vp, [vp]),
         "cgpu_vor_stream": (vp, [vp]),
         "cgpu_compute_cell_list": (i, [P(VoronoiBatchView), i, vp]),
         "cgpu_test_and_geometry": (i, [P(VoronoiBatchView), i, vp]),

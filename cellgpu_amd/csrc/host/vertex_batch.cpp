@@ -973,6 +973,11 @@ int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, co
                         const double *new_director) {
     return vguard([&] { return ((VertexBatch *)h)->cell_events(count, kind, cell, p1, p2, new_director); });
 }
+int cgpu_vm_view(void *h, cgpu_vertex_batch *out) {
+    *out = ((VertexBatch *)h)->view();
+    return CGPU_OK;
+}
+void *cgpu_vm_stream(void *h) { return (void *)((VertexBatch *)h)->stream(); }
 int cgpu_vm_get_sizes(void *h, int *nc, int *nv) {
     VertexBatch &vm = *(VertexBatch *)h;
     for (int s = 0; s < vm.num_systems(); ++s) {

@@ -438,6 +438,11 @@ int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, co
                         const double *new_director);
 /* cells and vertices per system (either may be NULL); returns the total vertices */
 int cgpu_vm_get_sizes(void *h, int *nc, int *nv);
+/* The batch's device arrays as a kernel-layer view, and the stream its work
+   runs on (the vertex counterparts of cgpu_vor_view / cgpu_vor_stream; the view
+   holds the current position buffer in pos). */
+int cgpu_vm_view(void *h, cgpu_vertex_batch *out);
+void *cgpu_vm_stream(void *h);
 
 /* ------------------------------------------------------ trajectory databases
  * NetCDF classic (CDF-1) files in the reference's layouts, written by a built-in

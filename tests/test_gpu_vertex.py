@@ -331,3 +331,78 @@ def test_vertex_cell_events_in_a_batch():
     f.minimize()
     b.check_status()
     assert np.isfinite(b.get()["vx"]).all()
+
+
+def test_vertex_fire_kernel_layer_fused_equals_piecewise():
+    """The vertex-model FIRE iteration as one call (cgpu_vm_fire_iteration) and
+    as its documented sequence of calls (move, T1 pass, forces and reductions,
+    parking) give bit-identical vertex positions, topology and FIRE scalars,
+    driven through the C ABI with caller-owned device buffers."""
+    import ctypes
+
+    cg = _cg()
+    from cellgpu_amd._lib import FireState, VertexBatchView, lib
+    hip = ctypes.CDLL("libamdhip64.so")
+    L = lib()
+    VP, PV = ctypes.c_void_p, ctypes.POINTER(VertexBatchView)
+    for name, args in (("cgpu_vm_fire_iteration", [PV, VP, VP, VP, VP]), ("cgpu_vm_fire_vv_move", [PV, VP, VP, VP]),
+                       ("cgpu_vm_fire_force_reduce", [PV, VP, VP, VP, VP]),
+                       ("cgpu_vm_fire_park_converged", [PV, VP, VP]), ("cgpu_vm_t1", [PV, VP]),
+                       ("cgpu_vm_geometry", [PV, VP]), ("cgpu_vm_forces", [PV, VP])):
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = ctypes.c_int
+
+    def dev(nbytes):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) == 0
+        return p
+
+    runs = []
+    for fused in (True, False):
+        b = cg.VertexBatch([200, 300], seeds=[3, 4], precision="fp64")
+        b.set_prefs(1.0, 3.9)
+        b.get()  # construction work done
+        v = VertexBatchView()
+        assert L.cgpu_vm_view(b.h, ctypes.byref(v)) == 0
+        st = ctypes.c_void_p(L.cgpu_vm_stream(b.h))
+        S, Nv = v.num_systems, v.num_vertices
+        fs_h = (FireState * S)()
+        for f in fs_h:  # EnergyMinimizerFIRE2D defaults, loop armed
+            f.alpha = f.alpha_start = 0.99
+            f.dt, f.dt_max, f.dt_inc, f.dt_dec, f.alpha_dec = 0.01, 0.1, 1.05, 0.95, 0.9
+            f.force_cutoff, f.force_max, f.nmin, f.max_iterations = 1e-12, 110.0, 5, 1000
+        fs, vel = dev(ctypes.sizeof(fs_h)), dev(2 * Nv * 8)
+        dots = dev((3 * Nv + 4 * v.num_chunks) * 8)
+        assert hip.hipMemcpy(fs, ctypes.byref(fs_h), ctypes.c_size_t(ctypes.sizeof(fs_h)), 1) == 0
+        assert hip.hipMemset(vel, 0, ctypes.c_size_t(2 * Nv * 8)) == 0
+        assert hip.hipDeviceSynchronize() == 0  # the batch stream does not wait for the null stream
+        print("vertex FIRE kernel layer, fused=%s, S=%d Nv=%d chunks=%d" % (fused, S, Nv, v.num_chunks), flush=True)
+        assert L.cgpu_vm_geometry(ctypes.byref(v), st) == 0
+        assert L.cgpu_vm_forces(ctypes.byref(v), st) == 0
+        for it in range(40):
+            vn = VertexBatchView.from_buffer_copy(v)
+            vn.pos, vn.pos_next = v.pos_next, v.pos
+            if fused:
+                assert L.cgpu_vm_fire_iteration(ctypes.byref(v), fs, vel, dots, st) == 0
+            else:
+                assert L.cgpu_vm_fire_vv_move(ctypes.byref(v), fs, vel, st) == 0
+                assert L.cgpu_vm_t1(ctypes.byref(vn), st) == 0
+                assert L.cgpu_vm_fire_force_reduce(ctypes.byref(vn), fs, vel, dots, st) == 0
+                assert L.cgpu_vm_fire_park_converged(ctypes.byref(vn), fs, st) == 0
+            v = vn
+        assert hip.hipStreamSynchronize(st) == 0
+        xy = np.zeros(2 * Nv)
+        assert hip.hipMemcpy(xy.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(v.pos), ctypes.c_size_t(16 * Nv),
+                             2) == 0
+        out = (FireState * S)()
+        assert hip.hipMemcpy(ctypes.byref(out), fs, ctypes.c_size_t(ctypes.sizeof(out)), 2) == 0
+        b.check_status()
+        topo = [np.array(a).copy() for a in b.topology()]
+        runs.append((xy, [(f.iterations, f.dt, f.alpha, f.force_max) for f in out], topo))
+        for p in (fs, vel, dots):
+            hip.hipFree(p)
+        b.close()
+    assert runs[0][1][0][0] == 40
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    assert all(np.array_equal(a, c) for a, c in zip(runs[0][2], runs[1][2]))

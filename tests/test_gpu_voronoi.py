@@ -511,20 +511,12 @@ def test_cell_events_in_a_batch():
     assert np.isfinite(b.get()["x"]).all()
 
 
-class _FireState(ctypes.Structure):
-    """Mirror of cgpu_fire_state (include/cellgpu_amd.h)."""
-
-    _fields_ = [(n, ctypes.c_double) for n in (
-        "alpha", "alpha_start", "alpha_dec", "dt", "dt_max", "dt_inc", "dt_dec", "force_cutoff", "power",
-        "force_max", "scaling", "mix_alpha", "max_f2", "pad_d")] + [(n, ctypes.c_int) for n in (
-            "nmin", "nsince", "iterations", "max_iterations", "converged", "zero_v", "pending", "pad_i")]
-
-
 def test_fire_kernel_layer_fused_equals_piecewise():
     """The kernel-layer FIRE iteration as one call (cgpu_fire_iteration) and as
     its documented sequence of calls give bit-identical positions, rings and FIRE
     scalars, driven through the C ABI with caller-owned device buffers."""
     cg = _cg()
+    from cellgpu_amd._lib import FireState as _FireState
     from cellgpu_amd._lib import VoronoiBatchView, lib
     hip = ctypes.CDLL("libamdhip64.so")
     L = lib()
