@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 runs of bench.py for the profiles/ folder.
 
-    python -m cellgpu_amd.profile_summary trace  <dir> <out.md> "<command>"
+    python -m cellgpu_amd.profile_summary trace  <dir> <out.md> "<command>" ["<title>"]
     python -m cellgpu_amd.profile_summary counters <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
         <out.md> <out.json> "<command>"
 
@@ -111,7 +111,10 @@ def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
 if __name__ == "__main__":
     a = sys.argv[1:]
     if a and a[0] == "trace":
-        trace(a[1], a[2], a[3])
+        if len(a) > 4:
+            trace(a[1], a[2], a[3], a[4])
+        else:
+            trace(a[1], a[2], a[3])
     elif a and a[0] == "counters":
         counters(a[1], a[2], a[3], a[4], a[5], a[6], a[7])
     else:
