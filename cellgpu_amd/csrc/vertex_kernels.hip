@@ -475,8 +475,10 @@ __device__ __forceinline__ void t1_rounds_loop(const cgpu_vertex_batch &b, doubl
                                                int *list, int *cells, int *touched, Count &count, int &rounds,
                                                int &flips, int *wave_kept) {
     const unsigned long long NONE = ~0ull;
+    __shared__ int waiting;  // some edge lost this round and waits for the next
     while (count > 0 && rounds < 100000) {
         const int nq = count;
+        if (threadIdx.x == 0) waiting = 0;
         for (int q = threadIdx.x; q < nq; q += blockDim.x) {
             int v1 = list[2 * q], v2 = list[2 * q + 1];
             // the edge test and the cells of its two vertices are loaded together
@@ -512,6 +514,8 @@ __device__ __forceinline__ void t1_rounds_loop(const cgpu_vertex_batch &b, doubl
             if (win) {
                 if (flip_edge(b, pos, sp, v1, v2, C, &touched[4 * q])) atomicAdd(&flips, 1);
                 list[2 * q] = -1;
+            } else {
+                waiting = 1;
             }
         }
         __syncthreads();
@@ -528,6 +532,14 @@ __device__ __forceinline__ void t1_rounds_loop(const cgpu_vertex_batch &b, doubl
             if (list[2 * q] == -1 && touched[4 * q] >= 0) normalize_vertex(b, touched[4 * q + (e & 3)]);
         }
         __syncthreads();
+        if (!waiting) {  // every edge flipped or was dropped: the last round
+            if (threadIdx.x == 0) {
+                count = 0;
+                rounds += 1;
+            }
+            __syncthreads();
+            continue;
+        }
         // the edges still waiting, in list order, staged in `cells` (free until
         // the next round): ballot prefix counts per wavefront, blockDim.x edges
         // at a time
