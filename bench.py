@@ -6,12 +6,19 @@ v0 ~ U[0.01, 0.2], Dr ~ U[0.1, 10], dt = 0.01, A0 = 1, KA = KP = 1; FP32 bulk
 arithmetic with FP64 Delaunay predicates; one hipGraph-captured batched step
 advances every system. A step = one time step of all 512 systems.
 
-Prints ONE JSON line. Besides value (aggregate cell-steps/s) it reports the
-plain one-system-at-a-time FP64 GPU path on a sample of the same systems, the
-FP32-vs-FP64 force error, the roofline of the dominant kernel (HIP events on the
-batch stream) and the oracle's CPU path timed on the host (bounded sample).
+Protocol (the reference driver's, voronoi.cpp:95-126): --init-steps untimed
+initialization steps from the random start (their rate is reported as
+init_phase), the autotuner on the relaxed state (restores it), --warmup steps,
+then exactly --steps timed steps.
 
-usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+Prints ONE JSON line. Besides value (aggregate cell-steps/s) it reports the
+plain one-system-at-a-time FP64 GPU path on a sample of the same systems over
+the same step window, FP32-vs-FP64 force / energy / topology / trajectory errors
+at the relaxed state, per-kernel durations and algorithmic bandwidth (HIP
+events on the batch stream) with the roofline of the dominant kernel, and the
+oracle's CPU path on survey config 1 timed on the host.
+
+usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--init-steps 500]
 """
 import argparse
 import json
@@ -65,79 +72,135 @@ def system_energies(g, n, P0):
     return np.add.reduceat(e, np.concatenate([[0], np.cumsum(n)[:-1]]))
 
 
-def fp32_errors(cg, n, P0, v0, Dr, seeds, b32, steps=100):
-    """FP32-bulk vs FP64 at the same state (max-norm force error relative to the
-    largest force; largest relative system energy error), and after `steps`
-    steps from the same start with the same noise: the largest and RMS
-    minimum-image position difference (unit density: in mean cell spacings)."""
+def _per_system_max(v, n):
+    return np.maximum.reduceat(v, np.concatenate([[0], np.cumsum(n)[:-1]]))
+
+
+def fp32_vs_fp64(cg, n, P0, v0, Dr, seeds, b32, steps=100):
+    """FP32 bulk against FP64 at the FP32 batch's current (relaxed) state: an
+    FP64 batch gets the same positions, directors, velocities, step counters and
+    Delaunay rings, and both compute forces. Reports the force error per system
+    (max over cells of |dF| relative to the system's largest |F|, and relative
+    to its RMS |F|; worst system), the relative energy error, the position
+    difference after `steps` further steps with the same noise, and how many of
+    the maintained rings differ from the exact Delaunay triangulation of the
+    same positions (the reference's test window, voronoiModelBase.cu:47-69, can
+    leave an intruder unflagged: the FP64 path keeps the same rings)."""
+    S = len(n)
+    g = b32.get()
+    st = np.array([b32.counters(s)["step"] for s in range(S)], np.int32)
+    nn32, nbr32, _ = b32.topology()
     b64 = make_batch(cg, n, P0, v0, Dr, seeds, "fp64")
+    b64.set_positions(g["x"], g["y"])  # exact triangulation of the same positions
+    nnx, nbrx, _ = b64.topology()
+    # slots past a cell's degree hold -1 in both copies
+    not_delaunay = int(np.count_nonzero((nn32 != nnx) | np.any(nbr32 != nbrx, axis=1)))
+    b64.set_topology(nn32, nbr32, b32.del_other())
+    b64.set_directors(g["theta"])
+    b64.set_velocities(g["vx"], g["vy"])
+    b64.set_step_counters(st)
     b64.compute_forces()
     b32.compute_forces()
     g64, g32 = b64.get(), b32.get()
-    F64 = np.concatenate([g64["Fx"], g64["Fy"]])
-    F32 = np.concatenate([g32["Fx"], g32["Fy"]])
-    ferr = float(np.abs(F32 - F64).max() / np.abs(F64).max())
+    dF = np.hypot(g32["Fx"] - g64["Fx"], g32["Fy"] - g64["Fy"])
+    F = np.hypot(g64["Fx"], g64["Fy"])
+    off = np.concatenate([[0], np.cumsum(n)[:-1]])
+    err_max = _per_system_max(dF, n) / _per_system_max(F, n)
+    rms = np.sqrt(np.add.reduceat(F * F, off) / n)
+    err_rms = _per_system_max(dF, n) / rms
     E64, E32 = system_energies(g64, n, P0), system_energies(g32, n, P0)
     eerr = float(np.max(np.abs(E32 - E64) / np.abs(E64)))
-    t32 = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
-    t32.spv_steps(steps, use_graph=True)
+    # trajectories from the same state with the same (counter-based) noise
+    b32.spv_steps(steps, use_graph=True)
     b64.spv_steps(steps, use_graph=True)
-    t32.check_status()
+    b32.check_status()
     b64.check_status()
-    a, c = t32.get(), b64.get()
+    a, c = b32.get(), b64.get()
     L = np.repeat(np.sqrt(n.astype(np.float64)), n)
     dx = a["x"] - c["x"]
     dy = a["y"] - c["y"]
     dx -= L * np.round(dx / L)
     dy -= L * np.round(dy / L)
     d = np.hypot(dx, dy)
-    t32.close()
     b64.close()
-    return ferr, eerr, {"steps": steps, "max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
-                        "unit": "mean cell spacing"}
+    return {
+        "force_rel_err_ensemble": float(dF.max() / F.max()),
+        "force_rel_err_max_norm": float(err_max.max()),
+        "force_rel_err_max_norm_worst_system": int(err_max.argmax()),
+        "force_rel_err_vs_rms": float(err_rms.max()),
+        "force_rel_err_vs_rms_worst_system": int(err_rms.argmax()),
+        "force_rel_err_median_system": float(np.median(err_max)),
+        "energy_rel_err": eerr,
+        "cells_with_ring_not_exact_delaunay": not_delaunay,
+        "trajectory": {"steps": steps, "max": float(d.max()), "rms": float(np.sqrt(np.mean(d * d))),
+                       "unit": "mean cell spacing"},
+    }
 
 
-def plain_fp64(cg, n, P0, v0, Dr, seeds, sample_every=8, steps=20):
-    """Reference-structured plain path: one FP64 system at a time, host sync per step."""
+def plain_fp64(cg, n, P0, v0, Dr, seeds, untimed_steps, steps, sample_every=8):
+    """Reference-structured plain path: one FP64 system at a time, host sync per
+    step (the reference reads its "any circumcircle failed" flag back every
+    step). Each sampled system first runs the same untimed steps as the batched
+    run from the same initial condition, so both time the same window."""
     cells = 0
     t = 0.0
     for s in range(0, len(n), sample_every):
         b = make_batch(cg, n[s:s + 1], P0[s:s + 1], v0[s:s + 1], Dr[s:s + 1], seeds[s:s + 1], "fp64")
-        b.spv_steps(2, use_graph=False)
+        b.spv_steps(untimed_steps, use_graph=True)
         b.sync()
+        b.check_status()
         t0 = time.perf_counter()
         for _ in range(steps):
             b.spv_steps(1, use_graph=False)
-            b.status()  # the reference reads the "any circumcircle failed" flag back every step
+            b.status()
         t += time.perf_counter() - t0
         cells += int(n[s]) * steps
         b.close()
-    return cells / t, "every %dth system (%d systems), %d steps each, per-step host sync" % (
-        sample_every, len(range(0, len(n), sample_every)), steps)
+    return cells / t, "every %dth system (%d systems), %d steps each after %d untimed steps, per-step host sync" % (
+        sample_every, len(range(0, len(n), sample_every)), steps, untimed_steps)
 
 
-def cpu_baseline(n, P0, v0, Dr, seeds, budget_s=12.0, steps=10):
-    """Oracle restatement of the reference CPU path (local test-and-repair), 1 thread."""
+def _cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, avail
+
+
+def cpu_baseline(runs=5, steps=1000, N=1000):
+    """Survey config 1 on the oracle's restatement of the reference CPU path
+    (local test-and-repair, CPU forces and SPP update): N = 1000, L = sqrt(N),
+    reference initial condition mt19937(13377), A0 = 1, P0 = 3.8, KA = KP = 1,
+    v0 = 0.05, Dr = 1, dt = 0.01, noise from a pre-generated seeded N(0,1)
+    buffer; 1 warm-up step, then `steps` steps timed by std::chrono inside one C
+    loop; median of `runs` runs; g++ -O3, one thread (the reference has no host
+    threading)."""
     from oracle import oracle as O
-    cells = 0
-    t = 0.0
-    used = 0
-    rng = np.random.default_rng(7)
-    for s in range(len(n)):
-        if t > budget_s:
-            break
-        x, y, th, L = O.random_init(int(n[s]), int(seeds[s]))
-        o = O.VoronoiSystem(x, y, th, L, P0=P0[s], v0=v0[s], Dr=Dr[s], dt=0.01)
-        etas = rng.standard_normal((steps, int(n[s])))
-        t0 = time.perf_counter()
-        for k in range(steps):
-            o.spp_step(etas[k], gpu_path=False)
-        t += time.perf_counter() - t0
-        cells += int(n[s]) * steps
-        used += 1
-    return dict(value=cells / t, unit="cell-steps/s", cores=1, kind="port",
-                sample="first %d systems of the ensemble, %d steps each, oracle CPU path (local repair), 1 thread"
-                       % (used, steps))
+    eta = np.random.default_rng(20240607).standard_normal((steps + 1, N))
+    times = []
+    for _ in range(runs):
+        x, y, th, L = O.random_init(N, 13377)
+        o = O.VoronoiSystem(x, y, th, L, A0=1.0, P0=3.8, v0=0.05, Dr=1.0, dt=0.01)
+        o.spp_step(eta[0], gpu_path=False)
+        times.append(o.spp_run(eta[1:], gpu_path=False))
+        del o
+    med = float(np.median(times))
+    model, avail = _cpu_info()
+    return dict(value=N * steps / med, unit="cell-steps/s", cores=1, kind="port",
+                sample="config 1: N=1000, %d steps after 1 warm-up step from the mt19937(13377) start, P0=3.8, "
+                       "v0=0.05, Dr=1, dt=0.01, oracle restatement of the reference CPU path (local repair), "
+                       "g++ -O3, 1 thread, median of %d runs (%s s)" % (
+                           steps, runs, ", ".join("%.2f" % x for x in times)),
+                cpu_model=model, host_cpus_available=avail)
 
 
 def serving(count=48, rate=8.0, steps_range=(1000, 10000), fire_fraction=0.25):
@@ -317,12 +380,60 @@ def copy_probe(mib, reps=5):
     print(json.dumps({"copy_probe_mib": mib, "reps": reps}), flush=True)
 
 
+# Algorithmic (compulsory) bytes per unit of each kernel of one step (FP32
+# bulk, FP64 positions, mean Delaunay degree 6, neighbour gathers on chip), and
+# the unit: cells of the batch, bins, or the entries of one of the step's
+# queues (flagged cells, NeedsFixing cells, cells of retriangulated systems).
+KERNEL_BYTES = {
+    # pos 16 + nn 4 + ring 24 + delOther 24 + dAP 8 + vel 8 + (v0, Dr) 8 + theta 4 + system 4;
+    # writes pos 16, F 8, vel 8, theta 4; binning pass: bin and rank 8
+    "k_force_eom_move": ("cells", BYTES_FORCE_EOM_MOVE + 8),
+    "k_bin_scan": ("bins", 12),           # count read + clear, start write
+    "k_bin_scatter": ("cells", 52),       # bin 4, rank r/w 8, start 4, pos 16; id 4 + pos 16 written
+    "k_test_geometry": ("cells", BYTES_TEST_GEOMETRY),
+    "k_mark_nf": ("flagged", 88),         # queue 4, nn 4, ring 24, 7 marks r/w + NeedsFixing entries
+    "k_star_local_wave<1>": ("needs_fixing", 72),  # pos 16, nn 4, ring 24 read; ring 24 + nn 4 written
+    "k_star_local_wave<2>": ("needs_fixing", 0),
+    "k_delsets_local": ("needs_fixing", 56),       # queue 4, nn 4, ring 24, delOther 24 written
+    "k_global_fallback": ("global", 4),
+    "k_star_global_wave<1>": ("global", 72),
+    "k_star_global_wave<2>": ("global", 0),
+    "k_global_stats": ("global", 4),
+    "k_finalize": ("repaired", 48),       # queue 4, delOther 24, AP + dAP 16, mark 4
+}
+
+
+def kernel_roofline(cg, b, N, bins, steps):
+    """Per-kernel durations of `steps` profiled steps (a HIP event after every
+    launch on the batch stream) and each kernel's algorithmic bytes / duration."""
+    ms, q = b.profiled_steps(steps)
+    units = {"cells": N, "bins": bins, "flagged": q[0] / steps, "needs_fixing": q[1] / steps,
+             "global": q[2] / steps, "repaired": (q[1] + q[2]) / steps}
+    table = {}
+    for k, tot in ms.items():
+        unit, per = KERNEL_BYTES[k]
+        avg = tot / steps
+        nbytes = per * units[unit]
+        table[k] = {"avg_ms": round(avg, 5), "unit": unit, "units_per_launch": round(units[unit], 1),
+                    "bytes_per_unit": per, "achieved_gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
+    return table, {"flagged": q[0] / steps, "needs_fixing": q[1] / steps, "global": q[2] / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--init-steps", type=int, default=500,
+                    help="untimed initialization steps from the random start before the warm-up (the reference "
+                         "driver's initSteps, voronoi.cpp:95-101); their rate is reported separately")
     ap.add_argument("--timed-stage-steps", type=int, default=200)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch-shape", default="",
+                    help="fixed launch shape instead of autotuning, as field=value,... (fields of cgpu_launch_shape)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step's kernels on the stream instead of replaying a hipGraph (profiler "
+                         "runs: rocprofv3 faults inside hipGraphLaunch on this image); implies --no-autotune")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -349,19 +460,40 @@ def main():
         return
     n, P0, v0, Dr, seeds = ensemble()
     N = int(n.sum())
+    bins = int(sum(max(2, int(np.floor(np.sqrt(k) / 1.25)) + (int(np.floor(np.sqrt(k) / 1.25)) % 2)) ** 2 for k in n))
     b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
-    ferr = eerr = traj = None
-    if not args.no_errors:
-        ferr, eerr, traj = fp32_errors(cg, n, P0, v0, Dr, seeds, b)
-    print("[bench] ensemble: %d systems, %d cells; fp32-vs-fp64 force error %s" % (
-        len(n), N, "skipped" if ferr is None else "%.2e" % ferr), flush=True)
+    print("[bench] ensemble: %d systems, %d cells" % (len(n), N), flush=True)
 
-    b.spv_steps(args.warmup, use_graph=True)
+    # untimed initialization from the random start (its rate is reported)
+    use_graph = not args.no_graph
+    if args.no_graph:
+        args.no_autotune = True
+    b.sync()
+    t0 = time.perf_counter()
+    b.spv_steps(args.init_steps, use_graph=use_graph)
+    b.sync()
+    t_init = time.perf_counter() - t0
+    b.check_status()
+    print("[bench] %d initialization steps in %.3f s" % (args.init_steps, t_init), flush=True)
+
+    if args.launch_shape:
+        b.set_launch_shape(**{k: int(v) for k, v in (kv.split("=") for kv in args.launch_shape.split(","))})
+        args.no_autotune = True
+    shape, cached, t_tune = b.launch_shape(), False, 0.0
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        shape, cached = b.autotune(trial_steps=20)
+        t_tune = time.perf_counter() - t0
+        b.check_status()
+        print("[bench] autotuned launch shape %s (%.2f s%s)" % (shape, t_tune, ", cached" if cached else ""),
+              flush=True)
+
+    b.spv_steps(args.warmup, use_graph=use_graph)
     b.sync()
     b.check_status()
     cg.lib().cgpu_device_synchronize()
     t0 = time.perf_counter()
-    b.spv_steps(args.steps, use_graph=True)
+    b.spv_steps(args.steps, use_graph=use_graph)
     b.sync()
     cg.lib().cgpu_device_synchronize()
     t1 = time.perf_counter()
@@ -370,46 +502,47 @@ def main():
     value = N * args.steps / elapsed
     print("[bench] timed %d steps in %.3f s" % (args.steps, elapsed), flush=True)
 
-    # per-stage durations with HIP events on the batch stream (no graph)
+    # per-kernel durations (HIP events on the batch stream, no graph), continuing
+    # the same trajectory after the timed steps
     ks = max(1, min(args.timed_stage_steps, args.steps))
-    ms = np.zeros(4)
-    import ctypes as C
-    r = cg.lib().cgpu_vor_spv_steps_timed(b.h, ks, ms.ctypes.data_as(C.c_void_p))
-    cg.check(r, "timed steps")
+    kt, queues = kernel_roofline(cg, b, N, bins, ks)
     b.check_status()
-    avg_ms = ms / ks
-    stage = {"force_eom_move": avg_ms[0], "cell_list": avg_ms[1], "test_geometry": avg_ms[2], "repair_chain": avg_ms[3]}
-    bytes_map = {"force_eom_move": BYTES_FORCE_EOM_MOVE, "test_geometry": BYTES_TEST_GEOMETRY}
-    dom = max(bytes_map, key=lambda k: stage[k])
-    achieved = bytes_map[dom] * N / (stage[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": {"force_eom_move": "k_force_eom_move<float, 0, false>",
-                                           "test_geometry": "k_test_geometry<float>"}[dom],
-                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_cell": bytes_map[dom], "cells_per_launch": N,
-                "avg_launch_ms": round(stage[dom], 5)}
-    # HBM bytes per launch of the dominant kernel from the committed counter
-    # summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a plain copy probe)
+    dom = max(kt, key=lambda k: kt[k]["avg_ms"])
+    d = kt[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved_gbs"] / HBM_PEAK_GBS, 4) if d["achieved_gbs"] else None,
+                "traffic": None, "bytes_per_unit": d["bytes_per_unit"], "unit_of_work": d["unit"],
+                "units_per_launch": d["units_per_launch"], "avg_launch_ms": d["avg_ms"],
+                "timing": "HIP events on the batch stream over %d steps after the timed steps" % ks}
+    # HBM bytes per launch of that kernel from the committed counter summary
+    # (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a plain copy probe)
     tpath = os.path.join(ROOT, "profiles", "voronoi_spv_ensemble512_hbm_traffic.json")
     if os.path.exists(tpath):
-        tk = json.load(open(tpath)).get("kernels", {}).get(roofline["kernel"])
-        if tk:
-            roofline["traffic"] = round(tk["traffic_bytes"])
+        tk = json.load(open(tpath)).get("kernels", {})
+        hit = [v for k, v in tk.items() if k.split("<")[0] == dom.split("<")[0] and
+               ("<" not in dom or k.startswith(dom.split(">")[0]))]
+        if hit:
+            roofline["traffic"] = round(hit[0]["traffic_bytes"])
             roofline["traffic_source"] = "profiles/voronoi_spv_ensemble512_hbm_traffic.json"
-            roofline["algorithmic_bytes_per_launch"] = bytes_map[dom] * N
-    others = {k: round(bytes_map[k] * N / (stage[k] * 1e-3) / 1e9, 1) for k in bytes_map}
+            roofline["algorithmic_bytes_per_launch"] = round(d["bytes_per_unit"] * d["units_per_launch"])
+
+    errs = None
+    if not args.no_errors:
+        errs = fp32_vs_fp64(cg, n, P0, v0, Dr, seeds, b)
+        print("[bench] fp32 vs fp64 at the relaxed state: %s" % json.dumps(errs), flush=True)
+    b.close()
 
     plain_val, plain_desc = (None, None)
     if not args.no_plain:
-        plain_val, plain_desc = plain_fp64(cg, n, P0, v0, Dr, seeds)
-    b.close()
+        plain_val, plain_desc = plain_fp64(cg, n, P0, v0, Dr, seeds, args.init_steps + args.warmup,
+                                           min(args.steps, 200))
     lat = None
     if not args.no_serving:
         lat = serving()
         print("[bench] serving: p50 %.1f ms, p99 %.1f ms" % (lat["serving_p50_ms"], lat["serving_p99_ms"]), flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, P0, v0, Dr, seeds)
+        cpu = cpu_baseline()
 
     per_step_ms = 1e3 * elapsed / args.steps
     out = {
@@ -430,16 +563,22 @@ def main():
             "workload": "batched Voronoi SPV ensemble: 512 systems, N log-uniform [64,8192], unit density, "
                         "P0~U[3.5,4], v0~U[0.01,0.2], Dr~U[0.1,10], dt=0.01",
             "systems": int(len(n)), "cells": N, "precision": "fp32-bulk/fp64-predicates",
-            "graph": True,
+            "graph": use_graph,
+            "init_steps": args.init_steps,
+            "timed_window": "steps %d-%d after the random start" % (
+                args.init_steps + args.warmup, args.init_steps + args.warmup + args.steps),
+            "launch_shape": shape, "autotuned": not args.no_autotune,
         },
+        "init_phase": {"steps": args.init_steps, "seconds": round(t_init, 4),
+                       "cell_steps_per_s": round(N * args.init_steps / t_init, 1) if t_init > 0 else None,
+                       "note": "from the random start (repair-heavy transient), untimed in value"},
+        "autotune_s": round(t_tune, 3),
         "plain_fp64_value": None if plain_val is None else round(plain_val, 1),
         "plain_fp64_sample": plain_desc,
         "ratio_vs_plain_fp64": None if plain_val is None else round(value / plain_val, 2),
-        "force_rel_err_vs_fp64": ferr,
-        "energy_rel_err_vs_fp64": eerr,
-        "trajectory_err_vs_fp64": traj,
-        "stage_ms": {k: round(v, 5) for k, v in stage.items()},
-        "stage_achieved_gbs": others,
+        "fp32_vs_fp64": errs,
+        "kernels": kt,
+        "queues_per_step": queues,
         "latency_ms": lat,
         "roofline": roofline,
         "cpu_baseline": cpu,

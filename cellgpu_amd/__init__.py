@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import CellGPUError, check, lib, require_device, VoronoiBatchView  # noqa: F401
+from ._lib import CellGPUError, check, lib, require_device, LaunchShape, VoronoiBatchView  # noqa: F401
 
 __all__ = [
     "CellGPUError", "VoronoiBatch", "VertexBatch", "VertexQuadraticEnergy", "VoronoiQuadraticEnergy", "VoronoiQuadraticEnergyWithTension", "Simulation", "selfPropelledParticleDynamics",
@@ -86,6 +86,32 @@ class VoronoiBatch:
     def set_directors(self, theta):
         t = _f64(theta, self.N)
         check(lib().cgpu_vor_set_directors(self.h, _p(t)), "set_directors")
+
+    def set_velocities(self, vx, vy):
+        check(lib().cgpu_vor_set_velocities(self.h, _p(_f64(vx, self.N)), _p(_f64(vy, self.N))), "set_velocities")
+
+    def set_topology(self, nn, nbr, del_other=None):
+        """Replace every cell's Delaunay ring (nbr[i, :nn[i]], counter-clockwise,
+        batch indices); geometry and neighMax are derived, delOther too unless
+        given (same layout as nbr)."""
+        nn = np.ascontiguousarray(nn, np.int32)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        oth = None
+        if del_other is not None:
+            oth = np.ascontiguousarray(del_other, np.int32)
+            if oth.shape != nbr.shape:
+                raise ValueError("del_other must have the layout of nbr")
+        check(lib().cgpu_vor_set_topology(self.h, _p(nn), _p(nbr), int(nbr.shape[1]), _p(oth)), "set_topology")
+
+    def del_other(self, cap=RING_CAP):
+        """delOther of every slot, [N, cap] (slots past a cell's degree: -1)."""
+        out = np.full(self.N * cap, -1, np.int32)
+        check(lib().cgpu_vor_get_del_other(self.h, _p(out), cap), "get_del_other")
+        return out.reshape(self.N, cap)
+
+    def set_step_counters(self, step):
+        st = np.ascontiguousarray(np.broadcast_to(np.asarray(step, np.int32), (self.S,)))
+        check(lib().cgpu_vor_set_step_counters(self.h, _p(st)), "set_step_counters")
 
     def tags(self):
         """Label of every cell within its system at creation (spatial sorting permutes cells)."""
@@ -181,6 +207,41 @@ class VoronoiBatch:
 
     def stream(self):
         return lib().cgpu_vor_stream(self.h)
+
+    SHAPE_FIELDS = ("force_block", "test_block", "cell_block", "star_blocks", "repair_blocks", "queue_lanes",
+                    "force_waves")
+
+    def launch_shape(self):
+        sh = LaunchShape()
+        check(lib().cgpu_vor_get_launch_shape(self.h, C.byref(sh)), "get_launch_shape")
+        return {k: getattr(sh, k) for k in self.SHAPE_FIELDS}
+
+    def set_launch_shape(self, **kw):
+        """Launch shapes of the kernels (fields of cgpu_launch_shape; 0 = default)."""
+        sh = LaunchShape()
+        for k in self.SHAPE_FIELDS:
+            setattr(sh, k, int(kw.pop(k, 0)))
+        if kw:
+            raise TypeError("unknown launch shape fields: %s" % sorted(kw))
+        check(lib().cgpu_vor_set_launch_shape(self.h, C.byref(sh)), "set_launch_shape")
+
+    def autotune(self, trial_steps=20):
+        """Search the launch shapes on the current state (restored afterwards),
+        apply the best; returns (shape dict, from_cache)."""
+        sh = LaunchShape()
+        cached = C.c_int(0)
+        check(lib().cgpu_vor_autotune(self.h, int(trial_steps), C.byref(sh), C.byref(cached)), "autotune")
+        return {k: getattr(sh, k) for k in self.SHAPE_FIELDS}, bool(cached.value)
+
+    def profiled_steps(self, steps):
+        """Steps with a HIP event after every kernel. Returns ({kernel: summed
+        ms}, summed queue sizes (flagged, NeedsFixing, global))."""
+        from ._lib import PROFILE_SLOTS
+        ms = np.zeros(PROFILE_SLOTS)
+        q = np.zeros(3, np.int64)
+        check(lib().cgpu_vor_spv_steps_profiled(self.h, int(steps), _p(ms), _p(q)), "profiled steps")
+        names = [lib().cgpu_profile_slot_name(k).decode() for k in range(PROFILE_SLOTS)]
+        return dict(zip(names, ms.tolist())), q.tolist()
 
     def spatial_sort(self):
         """Relabel cells along a Hilbert curve (per system) and retriangulate.

@@ -47,6 +47,17 @@ class VertexBatchView(C.Structure):
             "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("pad_c", C.c_int)]
 
 
+PROFILE_SLOTS = 13  # CGPU_PROFILE_SLOTS
+
+
+class LaunchShape(C.Structure):
+    """Mirror of cgpu_launch_shape (include/cellgpu_amd.h)."""
+
+    _fields_ = [(n, C.c_int) for n in (
+        "force_block", "test_block", "cell_block", "star_blocks", "repair_blocks", "queue_lanes", "force_waves",
+        "pad_shape")]
+
+
 class VoronoiBatchView(C.Structure):
     _fields_ = [("num_systems", C.c_int), ("num_cells", C.c_int), ("total_bins", C.c_int), ("precision", C.c_int)] + [
         (name, C.c_void_p) for name in (
@@ -55,7 +66,8 @@ class VoronoiBatchView(C.Structure):
             "sorted_pos", "repair", "mark", "flagged", "nfq", "gq",
             "counters", "noise", "chunk_sys", "chunk_begin", "chunk_end", "bchunk_sys", "bchunk_begin",
             "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)] + [
-        ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("pad_t", C.c_int)]
+        ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("pad_t", C.c_int),
+        ("shape", LaunchShape)]
 
 
 def lib():
@@ -96,7 +108,11 @@ def lib():
         "cgpu_vor_set_active": (i, [vp, vp, vp]),
         "cgpu_vor_get_active": (i, [vp, vp, vp]),
         "cgpu_vor_device_bytes": (C.c_longlong, [vp]),
-        "cgpu_vor_autotune": (i, [vp]),
+        "cgpu_vor_autotune": (i, [vp, i, P(LaunchShape), P(i)]),
+        "cgpu_vor_set_launch_shape": (i, [vp, P(LaunchShape)]),
+        "cgpu_vor_get_launch_shape": (i, [vp, P(LaunchShape)]),
+        "cgpu_vor_spv_steps_profiled": (i, [vp, i, vp, vp]),
+        "cgpu_profile_slot_name": (C.c_char_p, [i]),
         "cgpu_vor_view": (i, [vp, P(VoronoiBatchView)]),
         "cgpu_vm_view": (i, [vp, P(VertexBatchView)]),
         "cgpu_vm_stream": (vp, [vp]),
@@ -151,6 +167,11 @@ def lib():
         "cgpu_fire_set_param": (i, [vp, i, d]),
         "cgpu_vm_get_sizes": (i, [vp, vp, vp]),
         "cgpu_vor_set_directors": (i, [vp, vp]),
+        "cgpu_vor_set_velocities": (i, [vp, vp, vp]),
+        "cgpu_vor_set_topology": (i, [vp, vp, vp, i, vp]),
+        "cgpu_vor_get_del_other": (i, [vp, vp, i]),
+        "cgpu_derive_topology": (i, [P(VoronoiBatchView), vp]),
+        "cgpu_vor_set_step_counters": (i, [vp, vp]),
         "cgpu_vor_get_tags": (i, [vp, vp]),
         "cgpu_vor_cell_events": (i, [vp, i, vp, vp, vp, vp, vp]),
         "cgpu_vor_get_sizes": (i, [vp, vp]),

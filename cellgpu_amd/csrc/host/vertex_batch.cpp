@@ -456,6 +456,9 @@ class VertexBatch {
             step_launch(kind);
             hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
             hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate");
+            // only the executable graph is kept: the template would leak at every re-capture
+            (void)hipGraphDestroy(graph_);
+            graph_ = nullptr;
         }
         while (steps >= 2) {
             hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");

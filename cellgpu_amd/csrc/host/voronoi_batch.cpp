@@ -211,10 +211,7 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
     cell_sys_.upload(cell_sys_h_.data(), N_);
     upload_params();
     parity_ = 0;
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
 }
 
 void VoronoiBatch::upload_params() { sys_.upload(hp_.data(), S_); }
@@ -288,7 +285,21 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.cell_type = tension_on_ ? cell_type_.p : nullptr;
     b.tension = tension_.p;
     b.num_types = ntypes_;
+    b.shape = shape_;
     return b;
+}
+
+void VoronoiBatch::drop_graph() {
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void VoronoiBatch::set_shape(const cgpu_launch_shape &sh) {
+    sync();
+    shape_ = sh;
+    drop_graph();  // the captured launches carry the old shapes
 }
 
 void VoronoiBatch::set_params(int s, double KA, double KP, double dt, double mu, double T) {
@@ -302,10 +313,7 @@ void VoronoiBatch::set_params(int s, double KA, double KP, double dt, double mu,
         hp_[q].T = T;
     }
     upload_params();
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
     // dAP depends on KA, KP
     cgpu_voronoi_batch b = view();
     cgpu_test_and_geometry(&b, 0, stream_);
@@ -362,10 +370,7 @@ void VoronoiBatch::set_exclusions(const int *ex) {
     for (int i = 0; i < N_; ++i)
         if (ex[i]) mv[2 * i] = mv[2 * i + 1] = 0.0;
     real_upload(mot_.p, mv.data(), mv.size());
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
 }
 
 // a system in matrix mode may only hold types below the matrix size (the
@@ -379,10 +384,7 @@ void VoronoiBatch::update_tension_flag() {
                 if (h_type_[i] >= ntypes_) throw std::invalid_argument("cell type outside the tension matrix");
     }
     tension_on_ = any && !h_type_.empty();
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
 }
 
 void VoronoiBatch::set_cell_types(const int *types) {
@@ -454,19 +456,112 @@ void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
     if (retri) global_triangulation();
 }
 
+void VoronoiBatch::get_del_other(int *oth, int cap) {
+    sync();
+    std::vector<int> hn(N_), ho((size_t)CGPU_RING_CAP * N_);
+    nn_.download(hn.data(), N_);
+    oth_.download(ho.data(), ho.size());
+    for (int i = 0; i < N_; ++i)
+        for (int k = 0; k < hn[i] && k < cap; ++k) oth[(size_t)i * cap + k] = ho[(size_t)k * N_ + i];
+}
+
+void VoronoiBatch::set_topology(const int *nn, const int *nbr, int cap, const int *oth) {
+    if (!nn || !nbr || cap <= 0) throw std::invalid_argument("set_topology needs nn, nbr and cap > 0");
+    std::vector<int> ring((size_t)CGPU_RING_CAP * N_, 0);
+    std::vector<int> deg(N_);
+    std::vector<cgpu_sys_state> st(S_);
+    for (int s = 0; s < S_; ++s) {
+        const cgpu_sys_params &p = hp_[s];
+        long long sum = 0;
+        int mx = 0;
+        for (int i = p.off; i < p.off + p.n; ++i) {
+            const int z = nn[i];
+            if (z < 3 || z > CGPU_RING_CAP || z > cap)
+                throw std::invalid_argument("degree of cell " + std::to_string(i) + " outside [3, capacity]");
+            for (int k = 0; k < z; ++k) {
+                const int j = nbr[(size_t)i * cap + k];
+                if (j < p.off || j >= p.off + p.n || j == i)
+                    throw std::invalid_argument("neighbour of cell " + std::to_string(i) + " outside its system");
+                for (int q = 0; q < k; ++q)
+                    if (nbr[(size_t)i * cap + q] == j)
+                        throw std::invalid_argument("repeated neighbour of cell " + std::to_string(i));
+                ring[(size_t)k * N_ + i] = j;
+                if (oth) {
+                    const int o = oth[(size_t)i * cap + k];
+                    if (o < p.off || o >= p.off + p.n)
+                        throw std::invalid_argument("delOther of cell " + std::to_string(i) + " outside its system");
+                }
+            }
+            deg[i] = z;
+            sum += z;
+            mx = std::max(mx, z);
+        }
+        if (sum != 6LL * p.n) throw std::invalid_argument("degree sum of system " + std::to_string(s) + " is not 6N");
+        st[s].neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;  // voronoiModelBase.cpp:309-312
+    }
+    sync();
+    std::vector<cgpu_sys_state> cur(S_);
+    st_.download(cur.data(), S_);
+    for (int s = 0; s < S_; ++s) cur[s].neigh_max = st[s].neigh_max;
+    st_.upload(cur.data(), S_);
+    nn_.upload(deg.data(), N_);
+    nbr_.upload(ring.data(), ring.size());
+    cgpu_voronoi_batch b = view();
+    if (oth) {
+        // the caller's delOther (e.g. another batch's, including values a
+        // repair kept because a neighbour's ring lacked the edge)
+        std::vector<int> o((size_t)CGPU_RING_CAP * N_, 0);
+        for (int i = 0; i < N_; ++i)
+            for (int k = 0; k < deg[i]; ++k) o[(size_t)k * N_ + i] = oth[(size_t)i * cap + k];
+        oth_.upload(o.data(), o.size());
+        launch_geometry_only(b);
+    } else {
+        int r = cgpu_derive_topology(&b, stream_);
+        if (r) throw std::runtime_error("derive topology failed");
+    }
+    sync();
+}
+
+void VoronoiBatch::launch_geometry_only(const cgpu_voronoi_batch &b) {
+    if (cgpu_test_and_geometry(&b, 0, stream_)) throw std::runtime_error("geometry failed");
+}
+
 void VoronoiBatch::set_directors(const double *theta) {
     sync();
     real_upload(theta_.p, theta, N_);
+}
+
+// stored self-propulsion velocities (the reference keeps them between steps:
+// the displacement of a step uses the velocity of the step before)
+void VoronoiBatch::set_velocities(const double *vx, const double *vy) {
+    sync();
+    std::vector<double> v(2 * (size_t)N_);
+    for (int i = 0; i < N_; ++i) {
+        if (!std::isfinite(vx[i]) || !std::isfinite(vy[i])) throw std::invalid_argument("non-finite velocity");
+        v[2 * i] = vx[i];
+        v[2 * i + 1] = vy[i];
+    }
+    real_upload(vel_.p, v.data(), v.size());
+}
+
+// integrator step counters (reference Simulation::setCurrentTimestep /
+// Timestep): the counter also keys the in-kernel noise of the next step
+void VoronoiBatch::set_step_counters(const int *step) {
+    sync();
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    for (int s = 0; s < S_; ++s) {
+        if (step[s] < 0) throw std::invalid_argument("negative step counter");
+        st[s].step = step[s];
+    }
+    st_.upload(st.data(), S_);
 }
 
 void VoronoiBatch::set_noise_mode(bool use_buffer) {
     sync();  // host copies below must not race the batch stream
     noise_buffer_ = use_buffer;
     if (use_buffer && !noise_.p) noise_.alloc(2 * (size_t)N_);
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
 }
 void VoronoiBatch::upload_noise(const double *eta, int count) {
     sync();  // host copies below must not race the batch stream
@@ -529,6 +624,9 @@ int VoronoiBatch::spv_steps(int steps, bool use_graph) {
         step_launch(0);
         hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
         hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate");
+        // only the executable graph is kept: the template would leak at every re-capture
+        (void)hipGraphDestroy(graph_);
+        graph_ = nullptr;
     }
     while (steps >= 2) {
         hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
@@ -802,14 +900,169 @@ int VoronoiBatch::spatial_sort(int *perm_out) {
     }
     if (perm_out)
         for (size_t i = 0; i < N; ++i) perm_out[i] = perm[i];
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
-    }
+    drop_graph();
     return global_triangulation();
 }
 
-int VoronoiBatch::autotune() { return CGPU_OK; }
+// ------------------------------------------------------------------ profiling
+int VoronoiBatch::spv_steps_profiled(int steps, double *ms_slots, long long *queue_sums) {
+    constexpr int NS = CGPU_PROFILE_SLOTS;
+    hipEvent_t ev[NS + 1];
+    for (auto &e : ev) hip_check(hipEventCreate(&e), "event");
+    for (int q = 0; q < NS; ++q) ms_slots[q] = 0.0;
+    for (int q = 0; q < 3; ++q) queue_sums[q] = 0;
+    drop_graph();
+    int r = CGPU_OK;
+    std::vector<int> c(8);
+    for (int k = 0; k < steps && r == CGPU_OK; ++k) {
+        cgpu_voronoi_batch b = view();
+        r = cgpu_profiled_spv_step(&b, stream_, ev);
+        parity_ ^= 1;
+        hip_check(hipEventSynchronize(ev[NS]), "event sync");
+        for (int q = 0; q < NS; ++q) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, ev[q], ev[q + 1]), "event time");
+            ms_slots[q] += ms;
+        }
+        // queue sizes of this step: flagged cells, NeedsFixing, global queue
+        counters_.download(c.data(), 8);
+        queue_sums[0] += c[0];
+        queue_sums[1] += c[1];
+        queue_sums[2] += c[2];
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return r;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Every array a time step writes, copied on the device.
+struct VoronoiBatch::Snapshot {
+    DevBuf<unsigned char> data;
+    std::vector<std::pair<void *, size_t>> parts;  // (device array, bytes)
+    int parity = 0;
+};
+
+void VoronoiBatch::take_snapshot(Snapshot &sn) {
+    sync();
+    sn.parts = {{pos0_.p, pos0_.n * sizeof(double)}, {pos1_.p, pos1_.n * sizeof(double)},
+                {theta_.p, theta_.n}, {vel_.p, vel_.n}, {force_.p, force_.n}, {AP_.p, AP_.n},
+                {dAP_.p, dAP_.n}, {nn_.p, nn_.n * sizeof(int)}, {nbr_.p, nbr_.n * sizeof(int)},
+                {oth_.p, oth_.n * sizeof(int)}, {st_.p, st_.n * sizeof(cgpu_sys_state)},
+                {counters_.p, counters_.n * sizeof(int)}};
+    if (ext_.p) sn.parts.push_back({ext_.p, ext_.n});
+    size_t total = 0;
+    for (auto &p : sn.parts) total += (p.second + 255) & ~(size_t)255;
+    sn.data.alloc(total);
+    size_t off = 0;
+    for (auto &p : sn.parts) {
+        hip_check(hipMemcpyAsync(sn.data.p + off, p.first, p.second, hipMemcpyDeviceToDevice, stream_), "snapshot");
+        off += (p.second + 255) & ~(size_t)255;
+    }
+    sn.parity = parity_;
+    sync();
+}
+
+void VoronoiBatch::restore_snapshot(const Snapshot &sn) {
+    sync();
+    size_t off = 0;
+    for (auto &p : sn.parts) {
+        hip_check(hipMemcpyAsync(p.first, sn.data.p + off, p.second, hipMemcpyDeviceToDevice, stream_), "restore");
+        off += (p.second + 255) & ~(size_t)255;
+    }
+    parity_ = sn.parity;
+    sync();
+}
+
+// wall time (ms, HIP events) of `steps` SPV steps run as the production loop
+// runs them (graph of two steps)
+double VoronoiBatch::time_steps(int steps) {
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, stream_), "event record");
+    int r = spv_steps(steps, true);
+    hip_check(hipEventRecord(e1, stream_), "event record");
+    hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != CGPU_OK) throw std::runtime_error("step failed during autotuning, status " + std::to_string(r));
+    return ms;
+}
+
+namespace {
+// tuned shapes per (device, workload shape), for the life of the process
+std::vector<std::pair<std::string, cgpu_launch_shape>> g_tuned;
+}  // namespace
+
+int VoronoiBatch::autotune(int trial_steps, cgpu_launch_shape *out, int *from_cache) {
+    if (trial_steps < 2) trial_steps = 2;
+    if (trial_steps % 2) ++trial_steps;  // whole graph launches
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device_), "device properties");
+    // workload key: device, systems, cells, precision, tension, noise source
+    const std::string key = std::string(prop.gcnArchName) + "|" + std::to_string(prop.pciBusID) + "|" +
+                            std::to_string(S_) + "|" + std::to_string(N_) + "|" + std::to_string(prec_) + "|" +
+                            std::to_string(tension_on_ ? 1 : 0) + "|" + std::to_string(noise_buffer_ ? 1 : 0);
+    for (const auto &kv : g_tuned)
+        if (kv.first == key) {
+            set_shape(kv.second);
+            if (out) *out = shape_;
+            if (from_cache) *from_cache = 1;
+            return CGPU_OK;
+        }
+    if (from_cache) *from_cache = 0;
+    Snapshot sn;
+    take_snapshot(sn);
+    // best of two timings from the same saved state
+    auto trial = [&](const cgpu_launch_shape &sh) {
+        set_shape(sh);
+        // capture the graph outside the timed region (same buffers and parity
+        // after every restore, so it stays valid)
+        restore_snapshot(sn);
+        time_steps(2);
+        double best = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            restore_snapshot(sn);
+            best = std::min(best, time_steps(trial_steps));
+        }
+        return best;
+    };
+    cgpu_launch_shape cur{256, 256, 256, 4096, 1024, 8, 0, 0};
+    double tcur = trial(cur);
+    struct Knob {
+        int cgpu_launch_shape::*field;
+        std::vector<int> values;
+    };
+    const std::vector<Knob> knobs = {
+        {&cgpu_launch_shape::test_block, {128, 256, 512}},
+        {&cgpu_launch_shape::force_block, {128, 256, 512}},
+        {&cgpu_launch_shape::force_waves, {0, 5}},
+        {&cgpu_launch_shape::star_blocks, {1024, 2048, 4096, 8192, 16384}},
+        {&cgpu_launch_shape::repair_blocks, {256, 512, 1024, 2048}},
+        {&cgpu_launch_shape::queue_lanes, {4, 8, 16}},
+        {&cgpu_launch_shape::cell_block, {128, 256, 512, 1024}},
+    };
+    // coordinate descent, one pass; a change must gain more than 1 %
+    for (const Knob &k : knobs) {
+        for (int v : k.values) {
+            if (cur.*(k.field) == v) continue;
+            cgpu_launch_shape t = cur;
+            t.*(k.field) = v;
+            const double tt = trial(t);
+            if (tt < 0.99 * tcur) {
+                cur = t;
+                tcur = tt;
+            }
+        }
+    }
+    restore_snapshot(sn);
+    set_shape(cur);
+    g_tuned.push_back({key, cur});
+    if (out) *out = shape_;
+    return CGPU_OK;
+}
 
 void VoronoiBatch::get_active(int *active, int *steps_left) {
     std::vector<cgpu_sys_state> st(S_);
@@ -890,6 +1143,24 @@ int cgpu_vor_set_exclusions(void *h, const int *ex) {
 }
 int cgpu_vor_set_directors(void *h, const double *theta) {
     return guarded([&] { ((VoronoiBatch *)h)->set_directors(theta); return 0; });
+}
+int cgpu_vor_set_topology(void *h, const int *nn, const int *nbr, int cap, const int *oth) {
+    return guarded([&] {
+        ((VoronoiBatch *)h)->set_topology(nn, nbr, cap, oth);
+        return ((VoronoiBatch *)h)->status();
+    });
+}
+int cgpu_vor_get_del_other(void *h, int *oth, int cap) {
+    return guarded([&] {
+        ((VoronoiBatch *)h)->get_del_other(oth, cap);
+        return 0;
+    });
+}
+int cgpu_vor_set_velocities(void *h, const double *vx, const double *vy) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_velocities(vx, vy); return 0; });
+}
+int cgpu_vor_set_step_counters(void *h, const int *step) {
+    return guarded([&] { ((VoronoiBatch *)h)->set_step_counters(step); return 0; });
 }
 int cgpu_vor_cell_events(void *h, int count, const int *kind, const int *cell, const double *theta,
                          const double *frac, const double *new_director) {
@@ -973,7 +1244,33 @@ long long cgpu_vor_device_bytes(void *h) {
     if (!h) return -1;
     return (long long)((VoronoiBatch *)h)->device_bytes();
 }
-int cgpu_vor_autotune(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->autotune(); }); }
+int cgpu_vor_autotune(void *h, int trial_steps, cgpu_launch_shape *out, int *from_cache) {
+    return guarded([&] { return ((VoronoiBatch *)h)->autotune(trial_steps, out, from_cache); });
+}
+int cgpu_vor_set_launch_shape(void *h, const cgpu_launch_shape *shape) {
+    return guarded([&] {
+        if (!shape) throw std::invalid_argument("null launch shape");
+        ((VoronoiBatch *)h)->set_shape(*shape);
+        return 0;
+    });
+}
+int cgpu_vor_get_launch_shape(void *h, cgpu_launch_shape *shape) {
+    return guarded([&] {
+        if (!shape) throw std::invalid_argument("null launch shape");
+        *shape = ((VoronoiBatch *)h)->shape();
+        return 0;
+    });
+}
+int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long *queue_sums) {
+    return guarded([&] { return ((VoronoiBatch *)h)->spv_steps_profiled(steps, ms_slots, queue_sums); });
+}
+const char *cgpu_profile_slot_name(int slot) {
+    static const char *names[CGPU_PROFILE_SLOTS] = {
+        "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_mark_nf", "k_star_local_wave<1>",
+        "k_star_local_wave<2>", "k_delsets_local", "k_global_fallback", "k_star_global_wave<1>",
+        "k_star_global_wave<2>", "k_global_stats", "k_finalize"};
+    return (slot >= 0 && slot < CGPU_PROFILE_SLOTS) ? names[slot] : "";
+}
 int cgpu_vor_spatial_sort(void *h, int *perm) {
     return guarded([&] { return ((VoronoiBatch *)h)->spatial_sort(perm); });
 }

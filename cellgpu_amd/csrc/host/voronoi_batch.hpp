@@ -15,6 +15,10 @@
 
 #include "../../../include/cellgpu_amd.h"
 
+// one profiled step of the kernel layer (voronoi_kernels.hip): ev[0] before
+// the first launch, ev[k + 1] after the launches of profile slot k
+int cgpu_profiled_spv_step(const cgpu_voronoi_batch *b, void *stream, hipEvent_t *ev);
+
 namespace cgpu_host {
 
 void hip_check(hipError_t e, const char *what);
@@ -71,6 +75,10 @@ class VoronoiBatch {
     void set_tension_matrix(int ntypes, const double *g);
     void set_positions(const double *x, const double *y, bool retriangulate);
     void set_directors(const double *theta);
+    void set_velocities(const double *vx, const double *vy);
+    void set_topology(const int *nn, const int *nbr, int cap, const int *oth);
+    void get_del_other(int *oth, int cap);
+    void set_step_counters(const int *step);
     // reference tagToIdx / idxToTag (Simple2DCell.h): tag (label within the
     // system at creation) of each cell; spatial sorting permutes it
     const std::vector<int> &tags() const { return tag_; }
@@ -88,6 +96,9 @@ class VoronoiBatch {
     int compute_forces();
     int spv_steps(int steps, bool use_graph);
     int spv_steps_timed(int steps, double *ms_out);
+    // steps with an event after every kernel: summed ms per profile slot
+    // (CGPU_PROFILE_SLOTS) and summed queue sizes (flagged, NeedsFixing, global)
+    int spv_steps_profiled(int steps, double *ms_slots, long long *queue_sums);
     int brownian_steps(int steps);
     int sync();
     int status();
@@ -104,7 +115,13 @@ class VoronoiBatch {
     double *pos_dev() { return parity_ ? pos1_.p : pos0_.p; }
     void *force_dev() { return force_.p; }
     void swap_parity() { parity_ ^= 1; }
-    int autotune();
+    // launch shapes of this batch's kernels (a zero field: default)
+    void set_shape(const cgpu_launch_shape &sh);
+    const cgpu_launch_shape &shape() const { return shape_; }
+    // Search of the launch shapes on this batch's current state (restored
+    // afterwards: tuning does not advance the simulation); the result is cached
+    // per device and workload shape and applied. trial_steps: steps per timing.
+    int autotune(int trial_steps, cgpu_launch_shape *out, int *from_cache);
     // reference spatialSorting (Simple2DCell.cpp:438-477, voronoiModelBase.cpp:669-680):
     // relabel the cells of every system along a Hilbert curve of their
     // positions, then retriangulate; perm[new] = old (global indices)
@@ -122,6 +139,14 @@ class VoronoiBatch {
     void real_upload(void *dst, const double *src2, size_t count);
     void real_download(const void *src, double *dst, size_t count);
     int step_launch(int kind);
+    void drop_graph();
+    void launch_geometry_only(const cgpu_voronoi_batch &b);
+    // device copy of every array a step mutates (for the tuner's trials)
+    struct Snapshot;
+    void take_snapshot(Snapshot &sn);
+    void restore_snapshot(const Snapshot &sn);
+    double time_steps(int steps);
+    cgpu_launch_shape shape_{};
 
     int S_ = 0, N_ = 0, prec_ = 1, device_ = 0, total_bins_ = 0;
     size_t rsz_ = 4;

@@ -13,6 +13,7 @@
 // in a hipGraph with no host round trip.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <type_traits>
 
 #include "../../include/cellgpu_amd.h"
 #include "cgpu_device.h"
@@ -23,6 +24,59 @@ using namespace cgpu;
 namespace {
 
 constexpr int CAP = CGPU_RING_CAP;
+
+// ------------------------------------------------------------- launch shapes
+// A zero (or unsupported) field of the batch's launch shape takes the default.
+int g_block = 256;  // default cell_block (cgpu_set_block_size)
+__host__ __device__ inline int shape_queue_lanes(const cgpu_voronoi_batch &b) {
+    const int q = b.shape.queue_lanes;
+    return (q == 4 || q == 8 || q == 16) ? q : 8;
+}
+__host__ __device__ inline int log2_lanes(int q) { return q == 4 ? 2 : (q == 8 ? 3 : 4); }
+inline int shape_block3(int v) { return (v == 128 || v == 256 || v == 512) ? v : 256; }
+inline int shape_force_block(const cgpu_voronoi_batch &b) { return shape_block3(b.shape.force_block); }
+inline int shape_test_block(const cgpu_voronoi_batch &b) { return shape_block3(b.shape.test_block); }
+inline int shape_cell_block(const cgpu_voronoi_batch &b) {
+    const int v = b.shape.cell_block;
+    return (v >= 64 && v <= 1024 && v % 64 == 0) ? v : g_block;
+}
+inline int shape_star_blocks(const cgpu_voronoi_batch &b) {
+    const int v = b.shape.star_blocks;
+    return (v >= 64 && v <= 65536) ? v : 4096;
+}
+inline int shape_repair_blocks(const cgpu_voronoi_batch &b) {
+    const int v = b.shape.repair_blocks;
+    return (v >= 64 && v <= 65536) ? v : 1024;
+}
+
+// Optional per-launch hook of a profiled step: after each kernel of the step
+// the host records an event (cgpu_vor_spv_steps_profiled). Slot ids follow the
+// launch order of one step; a slot may hold several kernels (the cell-list scan).
+enum ProbeSlot {
+    PROBE_FORCE_EOM_MOVE = 0,  // force, EOM, move, reset, binning pass
+    PROBE_BIN_SCAN,            // cell-list prefix sum (one or three kernels)
+    PROBE_BIN_SCATTER,
+    PROBE_TEST_GEOMETRY,
+    PROBE_MARK_NF,
+    PROBE_STAR_LOCAL,
+    PROBE_STAR_LOCAL_WIDE,
+    PROBE_DELSETS_LOCAL,
+    PROBE_GLOBAL_FALLBACK,
+    PROBE_STAR_GLOBAL,
+    PROBE_STAR_GLOBAL_WIDE,
+    PROBE_GLOBAL_STATS,
+    PROBE_FINALIZE,
+    PROBE_SLOTS
+};
+static_assert(PROBE_SLOTS == CGPU_PROFILE_SLOTS, "profile slots of the C ABI");
+struct Probe {
+    hipEvent_t *ev;  // ev[slot + 1] is recorded after slot's launches
+    hipStream_t st;
+    void mark(int slot) const { (void)hipEventRecord(ev[slot + 1], st); }
+};
+inline void probe(const Probe *p, int slot) {
+    if (p) p->mark(slot);
+}
 
 __device__ inline dvec2 ldpos(const double *p, int i) {
     double2 v = reinterpret_cast<const double2 *>(p)[i];
@@ -155,9 +209,14 @@ template <typename R> struct BlockTile {
 // ------------------------------------------------------------------ geometry
 // Voronoi area and perimeter of cell i in the reference GPU order
 // (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
+// Evaluated in FP64 in both precision modes and rounded once at the store: the
+// force uses KA (A - A0) and KP (P - P0), which in relaxed tissue are a few
+// percent of A and P, so FP32 vertices would leave them with about 1e-5
+// relative error (the dominant FP32 force error).
 template <typename R>
 __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, int i,
                                               const cgpu_sys_params &sp) {
+    using G = double;
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
@@ -165,8 +224,8 @@ __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const
     dvec2 pi = ldpos(pos, i);
     dvec2 rij = min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi);
     dvec2 rik = min_dist(box, ldpos(pos, b.nbr[i]), pi);
-    vec2<R> vfirst = Prec<R>::c0(rij, rik), vlast = vfirst;
-    R A = R(0), P = R(0);
+    vec2<G> vfirst = Prec<G>::c0(rij, rik), vlast = vfirst;
+    G A = G(0), P = G(0);
     // ring positions are fetched one slot ahead of their use, ring indices two
     dvec2 pnext = ldpos(pos, b.nbr[(1 % z) * N + i]);
     int jnext = b.nbr[(2 % z) * N + i];
@@ -176,18 +235,18 @@ __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const
         if (k + 2 < z) jnext = b.nbr[(k + 2) * N + i];
         rij = rik;
         rik = min_dist(box, pk, pi);
-        vec2<R> vnext = Prec<R>::c0(rij, rik);
-        A += tri_area<R>(vlast, vnext);
-        R dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
+        vec2<G> vnext = Prec<G>::c0(rij, rik);
+        A += tri_area<G>(vlast, vnext);
+        G dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
         P += sqrt(dx * dx + dy * dy);
         vlast = vnext;
     }
-    A += tri_area<R>(vlast, vfirst);
-    R dx = vlast.x - vfirst.x, dy = vlast.y - vfirst.y;
+    A += tri_area<G>(vlast, vfirst);
+    G dx = vlast.x - vfirst.x, dy = vlast.y - vfirst.y;
     P += sqrt(dx * dx + dy * dy);
-    st2<R>(b.AP, i, vec2<R>{A, P});
-    vec2<R> pref = ld2<R>(b.APpref, i);
-    st2<R>(b.dAP, i, vec2<R>{(R)sp.KA * (A - pref.x), (R)sp.KP * (P - pref.y)});
+    st2<R>(b.AP, i, vec2<R>{(R)A, (R)P});
+    const vec2<R> pref = ld2<R>(b.APpref, i);
+    st2<R>(b.dAP, i, vec2<R>{(R)(sp.KA * (A - (G)pref.x)), (R)(sp.KP * (P - (G)pref.y))});
 }
 
 // -------------------------------------------------------------------- forces
@@ -223,14 +282,155 @@ __device__ inline TensionSlot<R> tension_slot(const cgpu_voronoi_batch &b, const
 // the tension kernels clamp edge lengths at THRESHOLD = 1e-18 in both precisions
 template <typename R> __device__ inline R clamp_threshold(R x) { return x < R(1e-18) ? R(1e-18) : x; }
 
+// One reference force set (voronoiQuadraticEnergy.cu:99-234) for the vertex
+// vcur of (I = the cell, J = ring k-2, K = ring k-1), from its neighbours vlast,
+// vnext (in I's polygon) and vother (across the J-K edge), the unit vectors ul,
+// un, uo of vlast - vcur, vcur - vnext, vcur - vother, the Jacobian m of the
+// vertex with respect to I, and q = (KA (A - A0), KP (P - P0)) of I, J, K;
+// added to sum.
+template <typename R>
+__device__ __forceinline__ void slot_force_set(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp, bool tens,
+                                               int tI, int tB, int tC, vec2<R> qI, vec2<R> qB, vec2<R> qC,
+                                               vec2<R> vlast, vec2<R> vcur, vec2<R> vnext, vec2<R> vother,
+                                               vec2<R> ul, vec2<R> un, vec2<R> uo, const R m[4], vec2<R> &sum) {
+    vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
+    vec2<R> dPdv{ul.x - un.x, ul.y - un.y};
+    vec2<R> dEdv{qI.x * dAdv.x + qI.y * dPdv.x, qI.x * dAdv.y + qI.y * dPdv.y};
+    TensionSlot<R> T{};
+    if (tens) {
+        // self terms: edges I-K (vcur, vnext) and I-J (vcur, vlast)
+        T = tension_slot<R>(b, sp, tI, tB, tC);
+        vec2<R> dT{R(0), R(0)};
+        if (T.ik) {
+            dT.x -= T.gik * un.x;
+            dT.y -= T.gik * un.y;
+        }
+        if (T.ij) {
+            dT.x += T.gij * ul.x;
+            dT.y += T.gij * ul.y;
+        }
+        dEdv.x = T.add(dEdv.x, dT.x);
+        dEdv.y = T.add(dEdv.y, dT.y);
+    }
+    // cell ring k-1
+    dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
+    dPdv = vec2<R>{-un.x - uo.x, -un.y - uo.y};
+    {
+        vec2<R> e{qC.x * dAdv.x + qC.y * dPdv.x, qC.x * dAdv.y + qC.y * dPdv.y};
+        if (tens) {  // edges K-I (vcur, vnext) and K-J (vcur, vother)
+            vec2<R> dT{R(0), R(0)};
+            if (T.ik) {
+                dT.x += T.gik * -un.x;
+                dT.y += T.gik * -un.y;
+            }
+            if (T.jk) {
+                dT.x -= T.gjk * uo.x;
+                dT.y -= T.gjk * uo.y;
+            }
+            e.x = T.add(e.x, dT.x);
+            e.y = T.add(e.y, dT.y);
+        }
+        dEdv.x += e.x;
+        dEdv.y += e.y;
+    }
+    // cell ring k-2
+    dAdv = vec2<R>{R(0.5) * (vother.y - vlast.y), R(0.5) * (vlast.x - vother.x)};
+    dPdv = vec2<R>{-uo.x + ul.x, -uo.y + ul.y};
+    {
+        vec2<R> e{qB.x * dAdv.x + qB.y * dPdv.x, qB.x * dAdv.y + qB.y * dPdv.y};
+        if (tens) {  // edges J-I (vcur, vlast) and J-K (vcur, vother)
+            vec2<R> dT{R(0), R(0)};
+            if (T.ij) {
+                dT.x -= T.gij * -ul.x;
+                dT.y -= T.gij * -ul.y;
+            }
+            if (T.jk) {
+                dT.x += T.gjk * -uo.x;
+                dT.y += T.gjk * -uo.y;
+            }
+            e.x = T.add(e.x, dT.x);
+            e.y = T.add(e.y, dT.y);
+        }
+        dEdv.x += e.x;
+        dEdv.y += e.y;
+    }
+    dEdv.x *= R(2.0);
+    dEdv.y *= R(2.0);
+    sum.x += dEdv.x * m[0] + dEdv.y * m[2];
+    sum.y += dEdv.x * m[1] + dEdv.y * m[3];
+}
+
+// Unit vectors of the Voronoi edges in the perimeter gradient. Exact identity:
+// the edge between the vertices circ(0, X, Y) and circ(0, Y, Z) of cell i (i
+// at the origin) lies on the perpendicular bisector of i and Y, and the edge
+// between circ(0, X, Y) and circ(X, Y, W) on the bisector of X and Y; each
+// edge's direction is +-perp(s) / |s| for the segment s joining the two cells
+// it separates. The FP64 path evaluates the reference expression d / |d| of the
+// vertex difference d. The FP32 path takes the direction from the identity and
+// only the sign from d: a short edge's d is the difference of two nearly equal
+// vertices, whose direction float cannot resolve, while the force depends on it
+// at O(KP |P - P0|) however short the edge. When the signed length along the
+// direction is within AMBIGUOUS_EDGE x (cell-list bin side, about the cell
+// spacing) of zero, well above the FP32 vertex error, its sign is evaluated in
+// FP64 (edge_sign64).
+constexpr float AMBIGUOUS_EDGE = 2e-5f;
+// Sign (-1, 0, 1) of the signed length of the Voronoi edge from circ(0, X, Y)
+// to circ(0, Y, Z) along perp(Y) / |Y|: both vertices lie on the bisector of 0
+// and Y, at c = Y / 2 + t perp(Y) / |Y| with t(P) = |Y| P.(P - Y) / (2 cross(Y, P)),
+// so the sign is that of t(X) - t(Z), evaluated without division in FP64 from
+// the FP64 displacements.
+__device__ inline int edge_sign64(dvec2 X, dvec2 Y, dvec2 Z) {
+    const double nx = X.x * (X.x - Y.x) + X.y * (X.y - Y.y), dx = Y.x * X.y - Y.y * X.x;
+    const double nz = Z.x * (Z.x - Y.x) + Z.y * (Z.y - Y.y), dz = Y.x * Z.y - Y.y * Z.x;
+    const double v = nx * dz - nz * dx;
+    const int sv = (v > 0.0) - (v < 0.0);
+    return (dx * dz < 0.0) ? -sv : sv;
+}
+// The FP32 force's rare FP64 fallbacks, out of line: FP64 displacements of the
+// cells ix, iy, ... from cell position pi (minimum image in a box of side L),
+// read from pos.
+__device__ inline dvec2 rare_rel(const double *pos, double L, dvec2 pi, int id) {
+    return min_dist(make_box(L), ldpos(pos, id), pi);
+}
+__device__ __attribute__((noinline)) float2 rare_circ0(const double *pos, double L, dvec2 pi, int ix, int iy) {
+    const dvec2 c = circ0<double>(rare_rel(pos, L, pi, ix), rare_rel(pos, L, pi, iy));
+    return make_float2((float)c.x, (float)c.y);
+}
+__device__ __attribute__((noinline)) float2 rare_circ3(const double *pos, double L, dvec2 pi, int ix, int iy,
+                                                         int iw) {
+    const dvec2 c = circ3<double>(rare_rel(pos, L, pi, ix), rare_rel(pos, L, pi, iy), rare_rel(pos, L, pi, iw));
+    return make_float2((float)c.x, (float)c.y);
+}
+__device__ __attribute__((noinline)) float4 rare_dhdr(const double *pos, double L, dvec2 pi, int ix, int iy);
+// sign of the edge from circ(o, X, Y) to circ(o, Y, Z) along perp(Y - o), o the
+// cell io (-1: the cell at pi)
+__device__ __attribute__((noinline)) int rare_edge_sign(const double *pos, double L, dvec2 pi, int io, int ix, int iy,
+                                                        int iz) {
+    const dvec2 o = io < 0 ? dvec2{0.0, 0.0} : rare_rel(pos, L, pi, io);
+    const dvec2 X = rare_rel(pos, L, pi, ix), Y = rare_rel(pos, L, pi, iy), Z = rare_rel(pos, L, pi, iz);
+    return edge_sign64(dvec2{X.x - o.x, X.y - o.y}, dvec2{Y.x - o.x, Y.y - o.y}, dvec2{Z.x - o.x, Z.y - o.y});
+}
+
+__device__ __attribute__((noinline)) float4 rare_dhdr(const double *pos, double L, dvec2 pi, int ix, int iy) {
+    double md[4];
+    dhdr_eval<double>(rare_rel(pos, L, pi, ix), rare_rel(pos, L, pi, iy), md);
+    return make_float4((float)md[0], (float)md[1], (float)md[2], (float)md[3]);
+}
+
+// perp(s) / |s|
+__device__ inline vec2<float> perp_unit(vec2<float> s) {
+    const float inv = 1.0f / sqrtf(s.x * s.x + s.y * s.y);
+    return vec2<float>{-s.y * inv, s.x * inv};
+}
+
 // Force on cell i: sum over ring slots k of the reference force set
 // (voronoiQuadraticEnergy.cu:99-234): slot k is the Voronoi vertex of
 // (i, ring k-2, ring k-1); its neighbours in the three polygons are vertex k-1,
 // vertex k+1 and the circumcenter of (ring k-2, ring k-1, delOther k).
 // TENS: the batch has cell types; systems with tension mode != 0 add the
-// line-tension terms.
+// line-tension terms. FP64 path: the reference expressions.
 template <typename R, bool TENS, typename Acc>
-__device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
+__device__ vec2<R> cell_force_ref(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
@@ -240,10 +440,17 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
     const dvec2 pi = acc.p(i);
     auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
     auto dd = [&](int id) { return min_dist(box, acc.p(id), pi); };
+    auto unit = [&](vec2<R> d) {
+        const R n = clampn(sqrt(d.x * d.x + d.y * d.y));
+        return vec2<R>{d.x / n, d.y / n};
+    };
     int jB = rk(-2), jC = rk(-1);
     dvec2 dA = dd(rk(-3)), dB = dd(jB), dC = dd(jC);
     vec2<R> qB = acc.q(jB), qC = acc.q(jC);
     vec2<R> vL = Prec<R>::c0(dA, dB), vM = Prec<R>::c0(dB, dC);
+    // unit vector of vlast - vcur; slot k + 1's is slot k's vcur - vnext (the
+    // same operands), carried over
+    vec2<R> ul = unit(vec2<R>{vL.x - vM.x, vL.y - vM.y});
     const vec2<R> qI = acc.q(i);
     int tI = 0, tB = 0, tC = 0;
     if (tens) {
@@ -277,80 +484,11 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
         dhdr<R>(dB, dC, m);
         const vec2<R> vcur = vM, vlast = vL, vnext = vN;
         vec2<R> vother = Prec<R>::c3(dB, dC, dO);
-        vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
-        vec2<R> dl{vlast.x - vcur.x, vlast.y - vcur.y};
-        R dlnorm = clampn(sqrt(dl.x * dl.x + dl.y * dl.y));
-        vec2<R> dn{vcur.x - vnext.x, vcur.y - vnext.y};
-        R dnnorm = clampn(sqrt(dn.x * dn.x + dn.y * dn.y));
-        const vec2<R> dclv = dl, dncv = dn;
-        const R dclnorm = dlnorm, dncnorm = dnnorm;
-        vec2<R> dPdv{dl.x / dlnorm - dn.x / dnnorm, dl.y / dlnorm - dn.y / dnnorm};
-        vec2<R> dEdv{qI.x * dAdv.x + qI.y * dPdv.x, qI.x * dAdv.y + qI.y * dPdv.y};
-        TensionSlot<R> T{};
-        if (tens) {
-            // self terms: edges I-K (vcur, vnext) and I-J (vcur, vlast)
-            T = tension_slot<R>(b, sp, tI, tB, tC);
-            vec2<R> dT{R(0), R(0)};
-            if (T.ik) {
-                dT.x -= T.gik * dn.x / dnnorm;
-                dT.y -= T.gik * dn.y / dnnorm;
-            }
-            if (T.ij) {
-                dT.x += T.gij * dl.x / dlnorm;
-                dT.y += T.gij * dl.y / dlnorm;
-            }
-            dEdv.x = T.add(dEdv.x, dT.x);
-            dEdv.y = T.add(dEdv.y, dT.y);
-        }
-        // cell ring k-1
-        dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
-        dn = vec2<R>{vcur.x - vother.x, vcur.y - vother.y};
-        dnnorm = clampn(sqrt(dn.x * dn.x + dn.y * dn.y));
-        dPdv = vec2<R>{-dncv.x / dncnorm - dn.x / dnnorm, -dncv.y / dncnorm - dn.y / dnnorm};
-        {
-            vec2<R> e{qC.x * dAdv.x + qC.y * dPdv.x, qC.x * dAdv.y + qC.y * dPdv.y};
-            if (tens) {  // edges K-I (vcur, vnext) and K-J (vcur, vother)
-                vec2<R> dT{R(0), R(0)};
-                if (T.ik) {
-                    dT.x += T.gik * -dncv.x / dncnorm;
-                    dT.y += T.gik * -dncv.y / dncnorm;
-                }
-                if (T.jk) {
-                    dT.x -= T.gjk * dn.x / dnnorm;
-                    dT.y -= T.gjk * dn.y / dnnorm;
-                }
-                e.x = T.add(e.x, dT.x);
-                e.y = T.add(e.y, dT.y);
-            }
-            dEdv.x += e.x;
-            dEdv.y += e.y;
-        }
-        // cell ring k-2
-        dAdv = vec2<R>{R(0.5) * (vother.y - vlast.y), R(0.5) * (vlast.x - vother.x)};
-        dPdv = vec2<R>{-dn.x / dnnorm + dclv.x / dclnorm, -dn.y / dnnorm + dclv.y / dclnorm};
-        {
-            vec2<R> e{qB.x * dAdv.x + qB.y * dPdv.x, qB.x * dAdv.y + qB.y * dPdv.y};
-            if (tens) {  // edges J-I (vcur, vlast) and J-K (vcur, vother)
-                vec2<R> dT{R(0), R(0)};
-                if (T.ij) {
-                    dT.x -= T.gij * -dclv.x / dclnorm;
-                    dT.y -= T.gij * -dclv.y / dclnorm;
-                }
-                if (T.jk) {
-                    dT.x += T.gjk * -dn.x / dnnorm;
-                    dT.y += T.gjk * -dn.y / dnnorm;
-                }
-                e.x = T.add(e.x, dT.x);
-                e.y = T.add(e.y, dT.y);
-            }
-            dEdv.x += e.x;
-            dEdv.y += e.y;
-        }
-        dEdv.x *= R(2.0);
-        dEdv.y *= R(2.0);
-        sum.x += dEdv.x * m[0] + dEdv.y * m[2];
-        sum.y += dEdv.x * m[1] + dEdv.y * m[3];
+        const vec2<R> un = unit(vec2<R>{vcur.x - vnext.x, vcur.y - vnext.y});
+        const vec2<R> uo = unit(vec2<R>{vcur.x - vother.x, vcur.y - vother.y});
+        slot_force_set<R>(b, sp, tens, tI, tB, tC, qI, qB, qC, vlast, vcur, vnext, vother, ul, un, uo, m, sum);
         // --- advance the window
+        ul = un;
         dB = dC;
         dC = dD;
         qB = qC;
@@ -363,6 +501,131 @@ __device__ vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i
         vM = vN;
     }
     return sum;
+}
+
+// The same in FP32 with the exact edge directions. Relative positions are held
+// in float (the FP64 displacement rounded once, as the FP64-to-float conversion
+// of the FP64 path would); the rare FP64 fallbacks (an ill-conditioned
+// circumcenter or Voronoi-vertex Jacobian, an ambiguous edge sign) recompute the
+// FP64 displacements from the cells' positions, so the loop keeps few FP64
+// registers.
+template <bool TENS, typename Acc>
+__device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
+    using R = float;
+    const Box box = make_box(sp.L);
+    const int N = b.num_cells;
+    const int z = b.nn[i];
+    if (z < 3) return vec2<R>{R(0), R(0)};
+    const bool tens = TENS && sp.tension != 0;
+    const dvec2 pi = acc.p(i);
+    auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
+    auto d64 = [&](int id) { return min_dist(box, acc.p(id), pi); };
+    auto c0 = [&](vec2<R> x, vec2<R> y, int ix, int iy) -> vec2<R> {
+        if (Prec<R>::well(x.x * y.y - x.y * y.x, x, y)) return circ0<R>(x, y);
+        const float2 c = rare_circ0(acc.pos, sp.L, pi, ix, iy);
+        return vec2<R>{c.x, c.y};
+    };
+    auto c3 = [&](vec2<R> x, vec2<R> y, vec2<R> w, int ix, int iy, int iw) -> vec2<R> {
+        const vec2<R> u{x.x - w.x, x.y - w.y}, v{y.x - w.x, y.y - w.y};
+        if (Prec<R>::well(u.x * v.y - u.y * v.x, u, v)) return circ3<R>(x, y, w);
+        const float2 c = rare_circ3(acc.pos, sp.L, pi, ix, iy, iw);
+        return vec2<R>{c.x, c.y};
+    };
+    const R amb = (R)(AMBIGUOUS_EDGE * sp.boxsize);
+    // unit vector of the vertex difference d of the edge from circ(0, X, Y) to
+    // circ(0, Y, Z) (cells ix, iy, iz; origin io, -1 = cell i), along perp(Y)
+    auto unit = [&](vec2<R> d, vec2<R> Y, int io, int ix, int iy, int iz) -> vec2<R> {
+        const vec2<R> e = perp_unit(Y);
+        R lam = d.x * e.x + d.y * e.y;
+        if (fabsf(lam) < amb) {
+            const int sg = rare_edge_sign(acc.pos, sp.L, pi, io, ix, iy, iz);
+            if (sg == 0) return vec2<R>{R(0), R(0)};  // zero-length edge (float build: d / 1e-18 = 0)
+            lam = (R)sg;
+        }
+        return lam >= R(0) ? e : vec2<R>{-e.x, -e.y};
+    };
+    int jA = rk(-3), jB = rk(-2), jC = rk(-1);
+    vec2<R> fB = cvt<R>(d64(jB)), fC = cvt<R>(d64(jC));
+    vec2<R> qB = acc.q(jB), qC = acc.q(jC);
+    vec2<R> vM = c0(fB, fC, jB, jC);
+    vec2<R> vL;
+    {
+        const vec2<R> fA = cvt<R>(d64(jA));
+        vL = c0(fA, fB, jA, jB);
+    }
+    vec2<R> ul = unit(vec2<R>{vL.x - vM.x, vL.y - vM.y}, fB, -1, jA, jB, jC);
+    const vec2<R> qI = acc.q(i);
+    int tI = 0, tB = 0, tC = 0;
+    if (tens) {
+        tI = b.cell_type[i];
+        tB = b.cell_type[jB];
+        tC = b.cell_type[jC];
+    }
+    vec2<R> sum{R(0), R(0)};
+    // slot k's neighbour, delOther and their positions are fetched one slot
+    // ahead, the indices two slots ahead
+    int jDn = b.nbr[i], oDn = b.oth[i];
+    dvec2 pDn = acc.p(jDn), pOn = acc.p(oDn);
+    int jD2 = b.nbr[(1 % z) * N + i], oD2 = b.oth[(1 % z) * N + i];
+    for (int k = 0; k < z; ++k) {
+        const int jD = jDn, jO = oDn;
+        const dvec2 pD = pDn, pO = pOn;
+        if (k + 1 < z) {
+            jDn = jD2;
+            oDn = oD2;
+            pDn = acc.p(jD2);
+            pOn = acc.p(oD2);
+        }
+        if (k + 2 < z) {
+            jD2 = b.nbr[(k + 2) * N + i];
+            oD2 = b.oth[(k + 2) * N + i];
+        }
+        const vec2<R> fD = cvt<R>(min_dist(box, pD, pi)), fO = cvt<R>(min_dist(box, pO, pi));
+        const vec2<R> vN = c0(fC, fD, jC, jD);
+        // --- reference force set for slot k: (nm1, n1) = (ring k-2, ring k-1)
+        R m[4];
+        if (Prec<R>::well(fB.y * fC.x - fB.x * fC.y, fB, fC)) {
+            dhdr_eval<R>(fB, fC, m);
+        } else {
+            const float4 md = rare_dhdr(acc.pos, sp.L, pi, jB, jC);
+            m[0] = md.x;
+            m[1] = md.y;
+            m[2] = md.z;
+            m[3] = md.w;
+        }
+        const vec2<R> vcur = vM, vlast = vL, vnext = vN;
+        const vec2<R> vother = c3(fB, fC, fO, jB, jC, jO);
+        // un: vcur - vnext on the bisector of i and ring k-1; uo: vcur - vother
+        // on the bisector of ring k-2 and ring k-1 (frame of ring k-2:
+        // vcur = circ(B, 0, C), vother = circ(B, O, C))
+        const vec2<R> un = unit(vec2<R>{vcur.x - vnext.x, vcur.y - vnext.y}, fC, -1, jB, jC, jD);
+        const vec2<R> uo =
+            unit(vec2<R>{vcur.x - vother.x, vcur.y - vother.y}, vec2<R>{fC.x - fB.x, fC.y - fB.y}, jB, i, jC, jO);
+        slot_force_set<R>(b, sp, tens, tI, tB, tC, qI, qB, qC, vlast, vcur, vnext, vother, ul, un, uo, m, sum);
+        // --- advance the window
+        ul = un;
+        fB = fC;
+        fC = fD;
+        jB = jC;
+        jC = jD;
+        qB = qC;
+        qC = acc.q(jD);
+        if (tens) {
+            tB = tC;
+            tC = b.cell_type[jD];
+        }
+        vL = vM;
+        vM = vN;
+    }
+    return sum;
+}
+
+template <typename R, bool TENS, typename Acc>
+__device__ inline vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc, int i, const cgpu_sys_params &sp) {
+    if constexpr (std::is_same<R, float>::value)
+        return cell_force_f32<TENS>(b, acc, i, sp);
+    else
+        return cell_force_ref<R, TENS>(b, acc, i, sp);
 }
 
 // ------------------------------------------------------- Delaunay star (repair)
@@ -1265,13 +1528,17 @@ __device__ __forceinline__ void wave_test_wide(const cgpu_voronoi_batch &b, cons
 // not hold up its wavefront; triangles with a window wider than 3 x 3 bins go
 // to a second LDS list that the block's wavefronts scan cooperatively. A
 // triangle beyond the LDS list (more than five per cell of the block) is tested
-// serially by the thread of its cell.
-constexpr int TRI_LDS = 1280;
+// serially by the thread of its cell. The lists live in dynamic LDS sized for
+// TRI_PER_CELL entries per thread (the block size is the launch shape's
+// test_block, at most 512).
+constexpr int TRI_PER_CELL = 5;
 template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
     cgpu_voronoi_batch b, const double *pos, int test) {
-    __shared__ int3 stri[TRI_LDS];
-    __shared__ short swide[TRI_LDS];
+    extern __shared__ int3 test_lds[];
+    const int TRI_LDS = TRI_PER_CELL * (int)blockDim.x;
+    int3 *stri = test_lds;
+    short *swide = reinterpret_cast<short *>(test_lds + TRI_LDS);
     __shared__ int scount, swcount;
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (threadIdx.x == 0) {
@@ -1316,29 +1583,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
 }
 
 void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
-    const dim3 g((unsigned)((b.num_cells + 255) / 256)), t(256);
+    const int bs = shape_test_block(b);
+    const dim3 g((unsigned)((b.num_cells + bs - 1) / bs)), t(bs);
+    const size_t lds = (size_t)TRI_PER_CELL * bs * (sizeof(int3) + sizeof(short));
     if (b.precision)
-        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, lds, st, b, pos, test);
     else
-        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, b, pos, test);
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, lds, st, b, pos, test);
 }
 
 // NeedsFixing = flagged cells and their current neighbours (reference
 // voronoiModelBase.cpp:617-646), built as a deduplicated device queue.
-constexpr int DELSET_LANES = 8;  // threads per cell of the per-slot queue kernels
-
-// (8 threads per flagged cell: entry e = 0 is the cell, e >= 1 its ring slot
-// e - 1; thread k0 takes entries k0, k0 + 8, ...)
+// (QL = the launch shape's queue_lanes threads per flagged cell: entry e = 0 is
+// the cell, e >= 1 its ring slot e - 1; thread k0 takes entries k0, k0 + QL, ...)
 __global__ void k_mark_nf(cgpu_voronoi_batch b) {
     const int nflag = b.counters[0];
     const int N = b.num_cells;
-    const long long total = (long long)nflag * DELSET_LANES;
+    const int QL = shape_queue_lanes(b), lg = log2_lanes(QL);
+    const long long total = (long long)nflag * QL;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
+        const int q = (int)(t >> lg), k0 = (int)(t & (QL - 1));
         const int i = b.flagged[q];
         const int z = b.nn[i];
-        for (int e = k0; e <= z; e += DELSET_LANES) {
+        for (int e = k0; e <= z; e += QL) {
             const int x = (e == 0) ? i : b.nbr[(e - 1) * N + i];
             const bool fresh = atomicExch(&b.mark[x], 1) == 0;
             const int slot = wave_reserve(fresh, &b.counters[1]);
@@ -1373,13 +1641,18 @@ __device__ inline int chunk_of(const cgpu_voronoi_batch &b, int i) {
     return lo;
 }
 
-// enqueue the cells of chunk c into the global queue (one reservation per chunk)
+// enqueue the cells of chunk c into the global queue (one reservation per chunk).
+// Called by every thread of a block (uniform per block). The closing barrier
+// keeps the next call's reservation from overwriting `base` while lanes of this
+// call still read it (callers loop over several chunks per block).
 __device__ void enqueue_chunk(const cgpu_voronoi_batch &b, int c) {
     __shared__ int base;
     const int c0 = b.chunk_begin[c], n = b.chunk_end[c] - c0;
     if (threadIdx.x == 0) base = atomicAdd(&b.counters[2], n);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += blockDim.x) b.gq[base + t] = c0 + t;
+    const int bs = base;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) b.gq[bs + t] = c0 + t;
+    __syncthreads();
 }
 
 // The repair of system s once NeedsFixing is complete: 0 none, 1 local, 2 global
@@ -1493,13 +1766,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
 // delOther of repaired cells (reference enforceTopology, voronoiModelBase.cpp:
 // 700-718) after the 6N/2N invariant check of getCircumcenterIndices (:411-414)
-// (8 threads per cell, each taking slots k, k + 8, k + 16)
+// (QL threads per cell, each taking slots k, k + QL, ...)
 __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     const int nq = b.counters[1];
-    const long long total = (long long)nq * DELSET_LANES;
+    const int QL = shape_queue_lanes(b), lg = log2_lanes(QL);
+    const long long total = (long long)nq * QL;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
+        const int q = (int)(t >> lg), k0 = (int)(t & (QL - 1));
         const int i = b.nfq[q];
         cgpu_sys_state &st = b.st[b.cell_sys[i]];
         if (st.mode != 1 || st.need_global) continue;
@@ -1512,7 +1786,7 @@ __global__ void k_delsets_local(cgpu_voronoi_batch b) {
             st.need_global = 1;
             continue;
         }
-        for (int k = k0; k < z; k += DELSET_LANES)
+        for (int k = k0; k < z; k += QL)
             if (!del_set_slot(b, i, z, k)) st.need_global = 1;
     }
 }
@@ -1628,8 +1902,8 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
 }
 
 
-// delOther of retriangulated systems (8 threads per cell, each taking slots
-// k, k + 8, k + 16); geometry of the cells whose ring a star kernel rewrote
+// delOther of retriangulated systems (QL threads per cell, each taking slots
+// k, k + QL, ...); geometry of the cells whose ring a star kernel rewrote
 // (mark 2; the test kernel already computed it for unchanged rings);
 // clears the marks. NeedsFixing cells of systems that went global are handled
 // with their system's global queue.
@@ -1651,15 +1925,16 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos, int advance)
     }
     const int ng = b.counters[2];
     const int nl = b.counters[1];
-    const long long total = (long long)(ng + nl) * DELSET_LANES;
+    const int QL = shape_queue_lanes(b), lg = log2_lanes(QL);
+    const long long total = (long long)(ng + nl) * QL;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(t / DELSET_LANES), k0 = (int)(t % DELSET_LANES);
+        const int q = (int)(t >> lg), k0 = (int)(t & (QL - 1));
         if (q < ng) {
             const int i = b.gq[q];
             const int z = b.nn[i];
             if (z >= 3)
-                for (int k = k0; k < z; k += DELSET_LANES) del_set_slot(b, i, z, k);
+                for (int k = k0; k < z; k += QL) del_set_slot(b, i, z, k);
             if (k0 == 0) {
                 if (b.mark[i] == 2) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
                 b.mark[i] = 0;
@@ -1670,6 +1945,21 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos, int advance)
             if (b.mark[i] == 2) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
             b.mark[i] = 0;
         }
+    }
+}
+
+// delOther of every slot of every cell from the current rings (reference
+// allDelSets, voronoiModelBase.cpp:1281-1286); QL threads per cell
+__global__ void k_delsets_all(cgpu_voronoi_batch b) {
+    const int QL = shape_queue_lanes(b), lg = log2_lanes(QL);
+    const long long total = (long long)b.num_cells * QL;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t >> lg), k0 = (int)(t & (QL - 1));
+        const int z = b.nn[i];
+        if (z < 3 || z > CAP) continue;
+        for (int k = k0; k < z; k += QL)
+            if (!del_set_slot(b, i, z, k)) b.st[b.cell_sys[i]].error = CGPU_ERR_TOPOLOGY;
     }
 }
 
@@ -1717,47 +2007,16 @@ template <typename R> __device__ inline R atan2_r(R y, R x);
 template <> __device__ inline float atan2_r<float>(float y, float x) { return atan2f(y, x); }
 template <> __device__ inline double atan2_r<double>(double y, double x) { return atan2(y, x); }
 
-// fused force -> self-propelled EOM (reference GPU semantics,
-// selfPropelledParticleDynamics.cu:35-53) -> move (Simple2DCell.cu:21-35).
-// kind 0: self-propelled; kind 1: Brownian (brownianParticleDynamics.cu:20-39)
-constexpr int FORCE_BLOCK = 256;
-// flags: STEP_BIN also runs the binning pass of the cell list on the new
-// positions; STEP_RESET also clears the step's topology state (k_reset), which
-// this kernel neither reads nor writes
-constexpr int STEP_BIN = 1, STEP_RESET = 2;
-template <typename R, int KIND, bool TENS>
-__global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos,
-                                                                double *pos_next, int flags) {
-    if (flags & STEP_RESET)
-        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
-            reset_system(b, s);
-    const bool bin = (flags & STEP_BIN) != 0;
-    // LDS copy of the block's cells (positions, dAP): most ring members and
-    // delOther cells of a Hilbert-ordered block are in it
-    __shared__ dvec2 spos[FORCE_BLOCK];
-    __shared__ vec2<R> sdap[FORCE_BLOCK];
-    const int c0 = xcd_block() * FORCE_BLOCK;
-    const int i = c0 + threadIdx.x;
-    const int n = min(FORCE_BLOCK, b.num_cells - c0);
-    if ((int)threadIdx.x < n) {
-        spos[threadIdx.x] = ldpos(pos, i);
-        sdap[threadIdx.x] = ld2<R>(b.dAP, i);
-    }
-    __syncthreads();
-    if (i >= b.num_cells) return;
-    const BlockTile<R> tile{pos, b.dAP, spos, sdap, c0, n};
-    const int s = b.cell_sys[i];
-    const cgpu_sys_params &sp = b.sys[s];
-    cgpu_sys_state &st = b.st[s];
+// self-propelled (KIND 0, reference GPU semantics, selfPropelledParticleDynamics.cu:
+// 35-53) or Brownian (KIND 1, brownianParticleDynamics.cu:20-39) update of
+// active cell i with force F, then the move (Simple2DCell.cu:21-35) into
+// pos_next and, with bin set, the binning pass of the cell list
+template <typename R, int KIND>
+__device__ inline void eom_move(const cgpu_voronoi_batch &b, int i, const cgpu_sys_params &sp, cgpu_sys_state &st,
+                                vec2<R> F, const double *pos, double *pos_next, bool bin) {
     dvec2 p = ldpos(pos, i);
-    if (!st.active) {
-        stpos(pos_next, i, p);
-        if (bin) bin_cell(b, sp, i, p);
-        return;
-    }
-    const int step = st.step + 1;  // Timestep is incremented before integrating
-    vec2<R> F = force_with_exclusions<R, TENS>(b, tile, i, sp);
     st2<R>(b.force, i, F);
+    const int step = st.step + 1;  // Timestep is incremented before integrating
     const R dt = (R)sp.dt, mu = (R)sp.mu;
     vec2<R> disp;
     if (KIND == 0) {
@@ -1790,6 +2049,60 @@ __global__ __launch_bounds__(FORCE_BLOCK) void k_force_eom_move(cgpu_voronoi_bat
     put_in_box(make_box(sp.L), p);
     stpos(pos_next, i, p);
     if (bin) bin_cell(b, sp, i, p);
+}
+
+// fused force -> EOM -> move of every cell, kind as for eom_move.
+// flags: STEP_BIN also runs the binning pass of the cell list on the new
+// positions; STEP_RESET also clears the step's topology state (k_reset), which
+// this kernel neither reads nor writes.
+constexpr int STEP_BIN = 1, STEP_RESET = 2;
+// the block size is the launch shape's force_block (at most 512); the LDS tile
+// (dynamic shared memory) holds one entry per thread. Two kernels share the
+// body: the register allocator's own choice, and one held to 5 waves per SIMD
+// (a few bytes of spill); the launch shape's force_waves picks one.
+template <typename R, int KIND, bool TENS>
+__device__ __forceinline__ void force_eom_move_body(const cgpu_voronoi_batch &b, const double *pos, double *pos_next,
+                                                    int flags) {
+    if (flags & STEP_RESET)
+        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
+            reset_system(b, s);
+    const bool bin = (flags & STEP_BIN) != 0;
+    // LDS copy of the block's cells (positions, dAP): most ring members and
+    // delOther cells of a Hilbert-ordered block are in it
+    extern __shared__ double2 force_lds[];
+    dvec2 *spos = reinterpret_cast<dvec2 *>(force_lds);
+    vec2<R> *sdap = reinterpret_cast<vec2<R> *>(force_lds + blockDim.x);
+    const int bs = (int)blockDim.x;
+    const int c0 = xcd_block() * bs;
+    const int i = c0 + threadIdx.x;
+    const int n = min(bs, b.num_cells - c0);
+    if ((int)threadIdx.x < n) {
+        spos[threadIdx.x] = ldpos(pos, i);
+        sdap[threadIdx.x] = ld2<R>(b.dAP, i);
+    }
+    __syncthreads();
+    if (i >= b.num_cells) return;
+    const BlockTile<R> tile{pos, b.dAP, spos, sdap, c0, n};
+    const int s = b.cell_sys[i];
+    const cgpu_sys_params &sp = b.sys[s];
+    cgpu_sys_state &st = b.st[s];
+    if (!st.active) {
+        const dvec2 p = ldpos(pos, i);
+        stpos(pos_next, i, p);
+        if (bin) bin_cell(b, sp, i, p);
+        return;
+    }
+    eom_move<R, KIND>(b, i, sp, st, force_with_exclusions<R, TENS>(b, tile, i, sp), pos, pos_next, bin);
+}
+template <typename R, int KIND, bool TENS>
+__global__ __launch_bounds__(512) void k_force_eom_move(cgpu_voronoi_batch b, const double *pos, double *pos_next,
+                                                        int flags) {
+    force_eom_move_body<R, KIND, TENS>(b, pos, pos_next, flags);
+}
+template <typename R, int KIND, bool TENS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_force_eom_move_w5(
+    cgpu_voronoi_batch b, const double *pos, double *pos_next, int flags) {
+    force_eom_move_body<R, KIND, TENS>(b, pos, pos_next, flags);
 }
 
 __global__ void k_advance_step(cgpu_voronoi_batch b) {
@@ -1922,31 +2235,41 @@ inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
 
-int g_block = 256;
-constexpr int REPAIR_BLOCKS = 1024;  // grid-stride grids of the queue kernels
-constexpr int STAR_WAVE_BLOCKS = 4096;  // 16384 wavefronts, one queued cell each at a time
-constexpr int WIDE_BLOCKS = 256;        // two candidates per lane, for wide bin blocks
+constexpr int WIDE_BLOCKS = 256;  // two candidates per lane, for wide bin blocks
 
 // global pass over the queued systems: wave kernel, serial form for the cells
 // it hands back, per-system degree statistics, delOther and geometry
-void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0) {
-    hipLaunchKernelGGL(k_star_global_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), dim3(256), 0, st, b);
+void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0,
+                 const Probe *pr = nullptr) {
+    const dim3 rb(shape_repair_blocks(b)), sb(shape_star_blocks(b)), t(256);
+    hipLaunchKernelGGL(k_star_global_wave<1>, sb, t, 0, st, b, pos);
+    probe(pr, PROBE_STAR_GLOBAL);
+    hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), t, 0, st, b, pos);
+    probe(pr, PROBE_STAR_GLOBAL_WIDE);
+    hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), t, 0, st, b);
+    probe(pr, PROBE_GLOBAL_STATS);
     if (b.precision)
-        hipLaunchKernelGGL(k_finalize<float>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos, advance);
+        hipLaunchKernelGGL(k_finalize<float>, rb, t, 0, st, b, pos, advance);
     else
-        hipLaunchKernelGGL(k_finalize<double>, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b, pos, advance);
+        hipLaunchKernelGGL(k_finalize<double>, rb, t, 0, st, b, pos, advance);
+    probe(pr, PROBE_FINALIZE);
 }
 
 // advance: the chain ends a time step (k_finalize advances the step counters)
-int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0) {
-    hipLaunchKernelGGL(k_mark_nf, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_star_local_wave<1>, dim3(STAR_WAVE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), dim3(256), 0, st, b, pos);
-    hipLaunchKernelGGL(k_delsets_local, dim3(REPAIR_BLOCKS), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_chunks), dim3(256), 0, st, b);
-    global_pass(b, pos, st, advance);
+int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0,
+                 const Probe *pr = nullptr) {
+    const dim3 rb(shape_repair_blocks(b)), sb(shape_star_blocks(b)), t(256);
+    hipLaunchKernelGGL(k_mark_nf, rb, t, 0, st, b);
+    probe(pr, PROBE_MARK_NF);
+    hipLaunchKernelGGL(k_star_local_wave<1>, sb, t, 0, st, b, pos);
+    probe(pr, PROBE_STAR_LOCAL);
+    hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), t, 0, st, b, pos);
+    probe(pr, PROBE_STAR_LOCAL_WIDE);
+    hipLaunchKernelGGL(k_delsets_local, rb, t, 0, st, b);
+    probe(pr, PROBE_DELSETS_LOCAL);
+    hipLaunchKernelGGL(k_global_fallback, dim3(b.num_chunks), t, 0, st, b);
+    probe(pr, PROBE_GLOBAL_FALLBACK);
+    global_pass(b, pos, st, advance, pr);
     return check();
 }
 
@@ -1956,7 +2279,8 @@ int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st,
 namespace {
 // the tension instantiations run only for batches with cell types
 template <typename R> void launch_forces(const cgpu_voronoi_batch &b, hipStream_t st) {
-    dim3 g(nblocks(b.num_cells, g_block)), t(g_block);
+    const int bs = shape_cell_block(b);
+    dim3 g(nblocks(b.num_cells, bs)), t(bs);
     if (b.cell_type)
         hipLaunchKernelGGL((k_forces<R, true>), g, t, 0, st, b, b.pos);
     else
@@ -1965,12 +2289,23 @@ template <typename R> void launch_forces(const cgpu_voronoi_batch &b, hipStream_
 
 template <typename R, int KIND>
 void launch_force_eom_move(const cgpu_voronoi_batch &b, hipStream_t st, int flags) {
-    // fixed 256-thread blocks: the kernel's LDS tile holds one block of cells
-    dim3 g(nblocks(b.num_cells, FORCE_BLOCK)), t(FORCE_BLOCK);
-    if (b.cell_type)
-        hipLaunchKernelGGL((k_force_eom_move<R, KIND, true>), g, t, 0, st, b, b.pos, b.pos_next, flags);
-    else
-        hipLaunchKernelGGL((k_force_eom_move<R, KIND, false>), g, t, 0, st, b, b.pos, b.pos_next, flags);
+    // the kernel's LDS tile holds one block of cells: positions and dAP
+    const int bs = shape_force_block(b);
+    dim3 g(nblocks(b.num_cells, bs)), t(bs);
+    const size_t lds = (size_t)bs * (sizeof(double2) + 2 * sizeof(R));
+    const bool w5 = b.shape.force_waves == 5;
+    if (b.cell_type) {
+        if (w5)
+            hipLaunchKernelGGL((k_force_eom_move_w5<R, KIND, true>), g, t, lds, st, b, b.pos, b.pos_next, flags);
+        else
+            hipLaunchKernelGGL((k_force_eom_move<R, KIND, true>), g, t, lds, st, b, b.pos, b.pos_next, flags);
+    } else {
+        if (w5)
+            hipLaunchKernelGGL((k_force_eom_move_w5<R, KIND, false>), g, t, lds, st, b, b.pos, b.pos_next, flags);
+        else
+            hipLaunchKernelGGL((k_force_eom_move<R, KIND, false>), g, t, lds, st, b, b.pos, b.pos_next, flags);
+    }
+
 }
 
 }  // namespace
@@ -1988,9 +2323,10 @@ int cgpu_compute_cell_list(const cgpu_voronoi_batch *b, int use_next, void *stre
     const double *p = use_next ? b->pos_next : b->pos;
     // bin counts are zero here: allocation clears them and k_bin_scan clears
     // them after use
-    hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
+    const int bs = shape_cell_block(*b);
+    hipLaunchKernelGGL(k_bin, dim3(nblocks(b->num_cells, bs)), dim3(bs), 0, S(stream), *b, p);
     launch_bin_scan(*b, S(stream));
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, S(stream), *b, p);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, bs)), dim3(bs), 0, S(stream), *b, p);
     return check();
 }
 
@@ -2016,6 +2352,13 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     // the pass recomputes geometry only where a ring changed; positions may have
     // changed everywhere here
     launch_test_geometry(*b, b->pos, 0, st);
+    return check();
+}
+
+int cgpu_derive_topology(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_delsets_all, dim3(shape_repair_blocks(*b)), dim3(256), 0, S(stream), *b);
+    launch_test_geometry(*b, b->pos, 0, S(stream));
     return check();
 }
 
@@ -2053,22 +2396,40 @@ int cgpu_brownian_force_eom_move(const cgpu_voronoi_batch *b, void *stream) {
     return force_eom_move(b, 1, stream);
 }
 
+}  // extern "C"
+
 // One step; after it, b->pos_next holds the new positions (the caller swaps).
-static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream) {
+// pr: optional per-launch event hook (profiled steps).
+static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream, const Probe *pr) {
     hipStream_t st = S(stream);
     // the force kernel also clears the step's topology state and runs the
     // binning pass of the cell list on the new positions (bin counts are zero:
     // the previous scan cleared them)
     int r = force_eom_move(b, kind, stream, STEP_BIN | STEP_RESET);
     if (r) return r;
+    probe(pr, PROBE_FORCE_EOM_MOVE);
     launch_bin_scan(*b, st);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
+    probe(pr, PROBE_BIN_SCAN);
+    const int bs = shape_cell_block(*b);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, bs)), dim3(bs), 0, st, *b, b->pos_next);
+    probe(pr, PROBE_BIN_SCATTER);
     r = check();
     if (r) return r;
     launch_test_geometry(*b, b->pos_next, 1, st);
+    probe(pr, PROBE_TEST_GEOMETRY);
     // its last kernel advances the step counters
-    return repair_chain(*b, b->pos_next, st, 1);
+    return repair_chain(*b, b->pos_next, st, 1, pr);
 }
+
+// One profiled step: ev[0] is recorded before the first launch, ev[k + 1]
+// after the launches of slot k (CGPU_PROFILE_SLOTS slots).
+int cgpu_profiled_spv_step(const cgpu_voronoi_batch *b, void *stream, hipEvent_t *ev) {
+    Probe pr{ev, S(stream)};
+    if (hipEventRecord(ev[0], S(stream)) != hipSuccess) return CGPU_ERR_HIP;
+    return step_common(b, 0, stream, &pr);
+}
+
+extern "C" {
 
 int cgpu_step_begin(const cgpu_voronoi_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
@@ -2083,11 +2444,11 @@ int cgpu_step_end(const cgpu_voronoi_batch *b, void *stream) {
 
 int cgpu_spv_step(const cgpu_voronoi_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    return step_common(b, 0, stream);
+    return step_common(b, 0, stream, nullptr);
 }
 int cgpu_brownian_step(const cgpu_voronoi_batch *b, void *stream) {
     if (!b) return CGPU_ERR_ARGS;
-    return step_common(b, 1, stream);
+    return step_common(b, 1, stream, nullptr);
 }
 
 int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, double *out, void *stream) {
@@ -2099,7 +2460,8 @@ int cgpu_fill_normal(const cgpu_voronoi_batch *b, int step, int stream_id, doubl
 
 int cgpu_fire_vv_move(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
     if (!b || !fs || !vel) return CGPU_ERR_ARGS;
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    const int bs = shape_cell_block(*b);
+    dim3 g(nblocks(b->num_cells, bs)), t(bs);
     if (b->precision)
         hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, S(stream), *b, fs, vel, b->pos, b->pos_next, 0);
     else
@@ -2115,7 +2477,8 @@ namespace {
 // forces, second velocity half step and the FIRE reductions on b->pos
 void fire_force_reduce(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, hipStream_t st,
                        int clear) {
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    const int bs = shape_cell_block(*b);
+    dim3 g(nblocks(b->num_cells, bs)), t(bs);
     const bool tens = b->cell_type != nullptr;
     if (b->precision) {
         if (tens)
@@ -2145,13 +2508,14 @@ int cgpu_fire_iteration(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *
     if (!b || !fs || !vel || !dots) return CGPU_ERR_ARGS;
     hipStream_t st = S(stream);
     // the move also clears the topology state and bins the moved cells
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    const int bs = shape_cell_block(*b);
+    dim3 g(nblocks(b->num_cells, bs)), t(bs);
     if (b->precision)
         hipLaunchKernelGGL(k_fire_vv_move<float>, g, t, 0, st, *b, fs, vel, b->pos, b->pos_next, STEP_RESET | STEP_BIN);
     else
         hipLaunchKernelGGL(k_fire_vv_move<double>, g, t, 0, st, *b, fs, vel, b->pos, b->pos_next, STEP_RESET | STEP_BIN);
     launch_bin_scan(*b, st);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, g_block)), dim3(g_block), 0, st, *b, b->pos_next);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(b->num_cells, bs)), dim3(bs), 0, st, *b, b->pos_next);
     cgpu_voronoi_batch bn = *b;  // the rest works on the moved positions
     bn.pos = b->pos_next;
     bn.pos_next = b->pos;
@@ -2172,7 +2536,8 @@ int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state 
 
 int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
     if (!b || !fs || !vel) return CGPU_ERR_ARGS;
-    dim3 g(nblocks(b->num_cells, g_block)), t(g_block);
+    const int bs = shape_cell_block(*b);
+    dim3 g(nblocks(b->num_cells, bs)), t(bs);
     if (b->precision)
         hipLaunchKernelGGL(k_fire_apply_mix<float>, g, t, 0, S(stream), *b, fs, vel);
     else

@@ -79,6 +79,31 @@ typedef struct cgpu_sys_state {
 } cgpu_sys_state;
 
 /*
+ * Launch shapes of the Voronoi kernels (a zero field takes the built-in
+ * default). No field changes a result: every kernel's output is independent of
+ * its launch shape (queue order only permutes independent per-cell work).
+ * cgpu_vor_autotune searches them per device and workload shape.
+ */
+typedef struct cgpu_launch_shape {
+    int force_block;    /* work-group size (= LDS tile of cells) of the fused force/EOM/move
+                           kernel: 128, 256 or 512; default 256                            */
+    int test_block;     /* work-group size of the circumcircle test + geometry kernel (its
+                           LDS triangle list holds 5 per cell): 128, 256 or 512; default 256 */
+    int cell_block;     /* work-group size of the other per-cell kernels: multiple of 64 in
+                           [64, 1024]; default 256                                          */
+    int star_blocks;    /* 256-thread work-groups of the Delaunay-star kernels, i.e. the repair
+                           queue is consumed 4 * star_blocks wavefronts (cells) at a time;
+                           default 4096                                                     */
+    int repair_blocks;  /* 256-thread work-groups of the grid-stride queue kernels; default 1024 */
+    int queue_lanes;    /* threads per queued cell in the queue kernels (64 / queue_lanes cells
+                           per wavefront): 4, 8 or 16; default 8                            */
+    int force_waves;    /* occupancy target of the fused force kernel: 5 (registers held to
+                           5 waves per SIMD, a few bytes of spill) or 0 (the register
+                           allocator's choice); default 0                                    */
+    int pad_shape;
+} cgpu_launch_shape;
+
+/*
  * Batch view of a Voronoi-model batch. All pointers are device pointers.
  * Per-cell arrays have num_cells entries; ring arrays are slot-major:
  * nbr[k * num_cells + i] is the k-th counter-clockwise Delaunay neighbour of cell i.
@@ -135,6 +160,7 @@ typedef struct cgpu_voronoi_batch {
     const int *cell_type;        /* [N] or NULL                                     */
     const double *tension;       /* [num_types^2] or NULL                           */
     int num_types, pad_t;
+    cgpu_launch_shape shape;     /* launch shapes (zero fields: defaults)           */
 } cgpu_voronoi_batch;
 
 #define CGPU_CHUNK 4096
@@ -215,6 +241,12 @@ int cgpu_repair_topology(const cgpu_voronoi_batch *b, void *stream);
  * of the batch: bins current positions, computes all Delaunay stars, delOther and
  * geometry. */
 int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream);
+
+/* Derived topology of rings set by the caller: delOther of every slot (reference
+ * allDelSets / getDelSets, voronoiModelBase.cpp:1236-1286; a slot whose two
+ * cells do not share the edge marks the system CGPU_ERR_TOPOLOGY) and the
+ * Voronoi geometry of every cell. */
+int cgpu_derive_topology(const cgpu_voronoi_batch *b, void *stream);
 
 /* Replaces gpu_force_sets + gpu_sum_force_sets(_with_exclusions) (reference
  * inc/models/voronoiQuadraticEnergy.cuh:20-54; voronoiQuadraticEnergy.cu:30-234):
@@ -342,6 +374,13 @@ int cgpu_vor_set_exclusions(void *h, const int *ex);
 int cgpu_vor_set_cell_types(void *h, const int *types);
 /* director angles per cell (reference cellDirectors; ReadState restores them) */
 int cgpu_vor_set_directors(void *h, const double *theta);
+/* stored velocities per cell (reference cellVelocities, Simple2DActiveCell.h:
+ * the self-propelled displacement of a step uses the velocity stored by the
+ * step before) */
+int cgpu_vor_set_velocities(void *h, const double *vx, const double *vy);
+/* integrator step counter per system (reference Simulation::setCurrentTimestep,
+ * Simulation.h:71; here it also keys the counter-based noise of the next step) */
+int cgpu_vor_set_step_counters(void *h, const int *step);
 /* tag (label within its system at creation) of every cell; spatial sorting
  * permutes cells, the tags follow them (reference tagToIdx / idxToTag) */
 int cgpu_vor_get_tags(void *h, int *tags);
@@ -363,6 +402,19 @@ int cgpu_vor_get_sizes(void *h, int *n);
 int cgpu_vor_set_tension(void *h, int s, int mode, double gamma);
 int cgpu_vor_set_tension_matrix(void *h, int ntypes, const double *g);
 int cgpu_vor_set_positions(void *h, const double *x, const double *y, int retriangulate);
+/* Replace the Delaunay rings of every cell (reference cellNeighborNum /
+ * cellNeighbors, Simple2DCell.h:200-202, public arrays that callers may
+ * overwrite before allDelSets): nn[i] neighbours of cell i in
+ * nbr[i * cap + k], counter-clockwise, batch indices inside the cell's system.
+ * Checked before upload (degree in [3, CGPU_RING_CAP], neighbours distinct and
+ * in the same system, the 6N degree sum per system); geometry and neighMax are
+ * derived on the device, and delOther too when oth is NULL (else oth[i * cap + k]
+ * is the delOther of slot k, reference voronoiModelBase.h:195-199). */
+int cgpu_vor_set_topology(void *h, const int *nn, const int *nbr, int cap, const int *oth);
+/* delOther of every slot (oth[i * cap + k]); a local repair keeps a slot's
+ * previous value when the neighbour's ring lacks the edge, as getDelSets does
+ * (voronoiModelBase.cpp:1236-1275) */
+int cgpu_vor_get_del_other(void *h, int *oth, int cap);
 int cgpu_vor_set_noise_mode(void *h, int use_buffer);
 int cgpu_vor_upload_noise(void *h, const double *eta);                      /* N doubles, this step */
 int cgpu_vor_compute_forces(void *h);
@@ -383,7 +435,19 @@ int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
  * admission against an HBM budget. */
 int cgpu_vor_get_active(void *h, int *active, int *steps_left);
 long long cgpu_vor_device_bytes(void *h);
-int cgpu_vor_autotune(void *h);
+/* Autotuner (no counterpart in the reference, whose launch shapes are fixed:
+ * voronoiModelBase.cu:221-223, 128-thread blocks): coordinate search over the
+ * launch shapes of cgpu_launch_shape (test and force work-group sizes, star
+ * grid = repair-queue depth in wavefronts, queue-kernel grid, threads per queued
+ * cell, per-cell work-group size), each candidate timed with HIP events over
+ * trial_steps steps from the same saved state. The batch's state is restored
+ * afterwards (tuning does not advance the simulation), the best shape is applied
+ * and cached for the process per device and workload shape (systems, cells,
+ * precision, tension, noise source): *from_cache = 1 when it came from there.
+ * Results do not depend on the shape. */
+int cgpu_vor_autotune(void *h, int trial_steps, cgpu_launch_shape *out, int *from_cache);
+int cgpu_vor_set_launch_shape(void *h, const cgpu_launch_shape *shape);
+int cgpu_vor_get_launch_shape(void *h, cgpu_launch_shape *shape);
 /* Spatial sorting (reference Simulation::setSortPeriod -> spatialSorting,
  * Simple2DCell.cpp:438-477, voronoiModelBase.cpp:669-680): relabels the cells of
  * every system along a Hilbert curve of their positions (all per-cell state
@@ -516,6 +580,13 @@ int cgpu_device_synchronize(void);
 /* Steps with HIP events around each stage (no graph); ms_out[0..3] receive the
  * summed durations of force/EOM/move, cell list, test+geometry, repair chain. */
 int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out);
+/* Steps with a HIP event after every kernel launch (no graph): ms_slots[k]
+ * receives the summed duration of profile slot k (the kernels of one step in
+ * launch order, named by cgpu_profile_slot_name), queue_sums[0..2] the summed
+ * sizes of the flagged, NeedsFixing and global-retriangulation queues. */
+#define CGPU_PROFILE_SLOTS 13
+int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long *queue_sums);
+const char *cgpu_profile_slot_name(int slot);
 /* Host-side Philox4x32-10 and the normal pair of (seed, index, step, stream),
  * identical to the device generator; for known-answer tests. */
 void cgpu_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);

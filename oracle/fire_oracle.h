@@ -16,7 +16,15 @@
 
 namespace fireo {
 
+// one iteration's FIRE decision, for pinning the update rules
+struct IterLog {
+    double power, dt_before, dt_after, alpha_before, alpha_after;
+    int nsince_before, nsince_after, zeroed;
+    double vv_after;  // sum |v|^2 after the step's velocity update
+};
+
 template <class D2> struct State {
+    std::vector<IterLog> *log = nullptr;  // when set, one entry per iteration
     int iterations = 0, maxIterations = 1000, NMin = 5, NSince = 0;
     double forceMax = 100., forceCutoff = 1e-7, alphaStart = 0.99, alpha = 0.99, dt = 0.01, dtMin = 1e-4,
            dtMax = 0.1, dtInc = 1.05, dtDec = 0.95, alphaDec = 0.9, Power = 0;
@@ -84,6 +92,7 @@ template <class D2, class M> int minimize(State<D2> &fr, M &model, double *out) 
             fr.v[i].x = (1 - fr.alpha) * fr.v[i].x + fr.alpha * scaling * fr.f[i].x;
             fr.v[i].y = (1 - fr.alpha) * fr.v[i].y + fr.alpha * scaling * fr.f[i].y;
         }
+        IterLog lg{fr.Power, fr.dt, 0, fr.alpha, 0, fr.NSince, 0, 0, 0};
         if (fr.Power > 0) {
             if (fr.NSince > fr.NMin) {
                 fr.dt = std::min(fr.dt * fr.dtInc, fr.dtMax);
@@ -94,6 +103,14 @@ template <class D2, class M> int minimize(State<D2> &fr, M &model, double *out) 
             fr.dt = fr.dt * fr.dtDec;
             fr.alpha = fr.alphaStart;
             for (int i = 0; i < N; ++i) fr.v[i] = D2{0, 0};
+            lg.zeroed = 1;
+        }
+        if (fr.log) {
+            lg.dt_after = fr.dt;
+            lg.alpha_after = fr.alpha;
+            lg.nsince_after = fr.NSince;
+            for (int i = 0; i < N; ++i) lg.vv_after += fr.v[i].x * fr.v[i].x + fr.v[i].y * fr.v[i].y;
+            fr.log->push_back(lg);
         }
     }
     if (out) {

@@ -50,12 +50,17 @@ def lib():
             "orc_vsys_num_cells": (C.c_int, [vp]),
             "orc_vsys_test_flags": (C.c_int, [vp, _ip]),
             "orc_vsys_enforce_topology": (C.c_int, [vp, C.c_int]),
+            "orc_vsys_last_needs_fixing": (C.c_int, [vp, vp]),
+            "orc_vsys_last_repair_mode": (C.c_int, [vp]),
+            "orc_vsys_test_flags_cpu": (C.c_int, [vp, _ip]),
             "orc_vsys_spp_step": (C.c_int, [vp, _dp, C.c_int]),
+            "orc_vsys_spp_run": (C.c_int, [vp, _dp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "orc_vsys_brownian_step": (C.c_int, [vp, _dp, _dp, C.c_double]),
             "orc_fire_create": (vp, [vp]),
             "orc_fire_destroy": (None, [vp]),
             "orc_fire_set": (None, [vp, C.c_int] + [C.c_double] * 7 + [C.c_int]),
             "orc_fire_minimize": (C.c_int, [vp, vp, _dp]),
+            "orc_fire_bowl": (C.c_int, [C.c_int, _dp, _dp, C.c_int] + [C.c_double] * 7 + [C.c_int, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -207,6 +212,20 @@ class VoronoiSystem:
         any_ = lib().orc_vsys_test_flags(self.h, r)
         return bool(any_), r
 
+    def test_flags_cpu(self):
+        """Flags of the reference CPU path's test (window around the circumcenter)."""
+        r = np.zeros(self.N, np.int32)
+        any_ = lib().orc_vsys_test_flags_cpu(self.h, r)
+        return bool(any_), r
+
+    def last_needs_fixing(self):
+        """NeedsFixing set (sorted) and repair mode (0 none, 1 local, 2 global) of
+        the last enforceTopology."""
+        n = lib().orc_vsys_last_needs_fixing(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_vsys_last_needs_fixing(self.h, _ptr(out))
+        return out[:n], lib().orc_vsys_last_repair_mode(self.h)
+
     def enforce_topology(self, gpu_path=True):
         if lib().orc_vsys_enforce_topology(self.h, 1 if gpu_path else 0) != 0:
             raise RuntimeError("oracle: topology repair failed")
@@ -215,10 +234,36 @@ class VoronoiSystem:
         if lib().orc_vsys_spp_step(self.h, np.ascontiguousarray(eta, np.float64), 1 if gpu_path else 0) != 0:
             raise RuntimeError("oracle: topology repair failed")
 
+    def spp_run(self, eta, gpu_path=True):
+        """len(eta) / N self-propelled steps in one C loop (eta[k, i]: noise of
+        cell i at step k); returns the loop's steady_clock seconds."""
+        eta = np.ascontiguousarray(eta, np.float64).reshape(-1)
+        steps = len(eta) // self.N
+        sec = C.c_double(0.0)
+        if lib().orc_vsys_spp_run(self.h, eta, steps, 1 if gpu_path else 0, C.byref(sec)) != 0:
+            raise RuntimeError("oracle: topology repair failed")
+        return sec.value
+
     def brownian_step(self, etax, etay, T):
         r = lib().orc_vsys_brownian_step(self.h, np.ascontiguousarray(etax, np.float64), np.ascontiguousarray(etay, np.float64), T)
         if r != 0:
             raise RuntimeError("oracle: topology repair failed")
+
+
+def fire_bowl(k, x0, max_it=100, force_cutoff=1e-12, alpha_start=0.1, dt=0.1, dt_max=1.0, dt_inc=1.05,
+              dt_dec=0.95, alpha_dec=0.9, n_min=5):
+    """FIRE (GPU branch) on the quadratic bowl E = sum (kx x^2 + ky y^2) / 2.
+    k, x0: (n, 2). Returns (iterations, final x (n, 2), per-iteration log with
+    columns power, dt_before, dt_after, alpha_before, alpha_after,
+    nsince_before, nsince_after, zeroed, vv_after, state [forceMax, Power,
+    alpha, dt, max |F|^2])."""
+    k = np.ascontiguousarray(k, np.float64).reshape(-1)
+    x = np.ascontiguousarray(x0, np.float64).reshape(-1).copy()
+    lg = np.zeros(9 * max_it)
+    out = np.zeros(5)
+    it = lib().orc_fire_bowl(len(x) // 2, k, x, max_it, force_cutoff, alpha_start, dt, dt_max, dt_inc, dt_dec,
+                             alpha_dec, n_min, lg, out)
+    return it, x.reshape(-1, 2), lg[:9 * it].reshape(it, 9), out
 
 
 class FireMinimizer:

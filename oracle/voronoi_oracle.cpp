@@ -17,6 +17,7 @@
 //   * flagging: hand-built intruder cases; initial conditions: std::mt19937(13377);
 //   * SPP / Brownian / FIRE: closed forms at F = 0, quadratic-bowl convergence.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1077,6 +1078,25 @@ int orc_vsys_test_flags(void *h, int *repair) {
     for (int i = 0; i < s->N; ++i) repair[i] = r[i];
     return any ? 1 : 0;
 }
+// the NeedsFixing set of the last enforceTopology (sorted; empty when no
+// triangle failed); returns its size, writes it to out when out is not null
+int orc_vsys_last_needs_fixing(void *h, int *out) {
+    VSys *s = (VSys *)h;
+    if (out)
+        for (size_t q = 0; q < s->lastNeedsFixing.size(); ++q) out[q] = s->lastNeedsFixing[q];
+    return (int)s->lastNeedsFixing.size();
+}
+// last repair mode: 0 none, 1 local, 2 global (N/6 rule or a failed local repair)
+int orc_vsys_last_repair_mode(void *h) { return ((VSys *)h)->lastRepairMode; }
+// CPU-path flags (reference DelaunayLoc::testPointTriangulation window around
+// the circumcenter, DelaunayLoc.cpp:517-574); returns whether any cell failed
+int orc_vsys_test_flags_cpu(void *h, int *repair) {
+    VSys *s = (VSys *)h;
+    vector<int> r;
+    bool any = testTriangulationCPUStyle(*s, r);
+    for (int i = 0; i < s->N; ++i) repair[i] = r[i];
+    return any ? 1 : 0;
+}
 // force-free enforceTopology on the current positions (GPU branch semantics)
 int orc_vsys_enforce_topology(void *h, int gpuPath) {
     VSys *s = (VSys *)h;
@@ -1129,6 +1149,19 @@ int orc_vsys_spp_step(void *h, const double *eta, int gpuPath) {
 }
 // Brownian step on cells (reference brownianParticleDynamics.cu:20-39):
 // disp = eta * sqrt(dt*T*mu*2) + dt*mu*F  (GPU operand order), x draw before y
+// `steps` self-propelled steps in one call, eta[k * N + i] the noise of cell i at
+// step k, timed with std::chrono::steady_clock (the reference driver times its
+// loop with clock(), voronoi.cpp:110-126); returns the first failing status
+int orc_vsys_spp_run(void *h, const double *eta, int steps, int gpuPath, double *seconds) {
+    VSys *s = (VSys *)h;
+    const auto t0 = std::chrono::steady_clock::now();
+    int r = 0;
+    for (int k = 0; k < steps && r == 0; ++k) r = orc_vsys_spp_step(h, eta + (size_t)k * s->N, gpuPath);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return r;
+}
+
 int orc_vsys_brownian_step(void *h, const double *etax, const double *etay, double T) {
     VSys *s = (VSys *)h;
     geometryGPUStyle(*s);
@@ -1174,6 +1207,59 @@ void orc_fire_set(void *h, int maxIt, double fc, double alphaStart, double dt, d
 }
 // GPU-branch FIRE on a Voronoi system; returns iterations done. out: forceMax
 // (= sum|F|^2/N), Power, alpha, dt, max|F|^2
+// FIRE on an anisotropic quadratic bowl, E = sum_i (kx_i x_i^2 + ky_i y_i^2) / 2
+// (F = -k x componentwise, no box, no topology): a model whose trajectory can
+// be followed by hand. k, x: 2n values (x then y per degree of freedom).
+// log_out: 9 doubles per iteration (power, dt before / after, alpha before /
+// after, NSince before / after, velocities zeroed, sum |v|^2 after); x is
+// updated in place. Returns the iterations done.
+struct BowlModel {
+    int n;
+    const double *k;
+    std::vector<d2> x;
+    int dof() const { return n; }
+    void forces(std::vector<d2> &F) {
+        F.resize(n);
+        for (int i = 0; i < n; ++i) F[i] = d2{-k[2 * i] * x[i].x, -k[2 * i + 1] * x[i].y};
+    }
+    void move(const std::vector<d2> &d) {
+        for (int i = 0; i < n; ++i) {
+            x[i].x += d[i].x;
+            x[i].y += d[i].y;
+        }
+    }
+    bool topology() { return true; }
+};
+int orc_fire_bowl(int n, const double *k, double *x, int maxIt, double fc, double alphaStart, double dt,
+                  double dtMax, double dtInc, double dtDec, double alphaDec, int NMin, double *log_out,
+                  double *out) {
+    BowlModel m{n, k, std::vector<d2>(n)};
+    for (int i = 0; i < n; ++i) m.x[i] = d2{x[2 * i], x[2 * i + 1]};
+    fireo::State<d2> st;
+    st.set(maxIt, fc, alphaStart, dt, dtMax, dtInc, dtDec, alphaDec, NMin);
+    std::vector<fireo::IterLog> lg;
+    st.log = &lg;
+    const int it = fireo::minimize(st, m, out);
+    for (size_t q = 0; q < lg.size(); ++q) {
+        const fireo::IterLog &e = lg[q];
+        double *o = log_out + 9 * q;
+        o[0] = e.power;
+        o[1] = e.dt_before;
+        o[2] = e.dt_after;
+        o[3] = e.alpha_before;
+        o[4] = e.alpha_after;
+        o[5] = e.nsince_before;
+        o[6] = e.nsince_after;
+        o[7] = e.zeroed;
+        o[8] = e.vv_after;
+    }
+    for (int i = 0; i < n; ++i) {
+        x[2 * i] = m.x[i].x;
+        x[2 * i + 1] = m.x[i].y;
+    }
+    return it;
+}
+
 int orc_fire_minimize(void *hf, void *hsys, double *out) {
     VoronoiFireModel m{(VSys *)hsys};
     return fireo::minimize(*(fireo::State<d2> *)hf, m, out);
