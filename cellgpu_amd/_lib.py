@@ -18,6 +18,7 @@ class CellGPUError(RuntimeError):
 STATUS = {
     0: "ok", -1: "bad arguments", -2: "HIP error", -3: "Delaunay degree exceeds ring capacity",
     -4: "cell-list bin overflow", -5: "topology inconsistent", -6: "non-finite position", -7: "no device",
+    -8: "input exceeds the static scratch",
 }
 
 

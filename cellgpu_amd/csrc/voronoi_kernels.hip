@@ -13,11 +13,17 @@
 // in a hipGraph with no host round trip.
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "../../include/cellgpu_amd.h"
 #include "cgpu_device.h"
 #include "fire_device.h"
+
+namespace cgpu_host {
+void set_last_error(const char *what);  // message of cgpu_last_error (host layer)
+}
 
 using namespace cgpu;
 
@@ -2526,6 +2532,59 @@ int cgpu_fire_iteration(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *
     fire_force_reduce(&bn, fs, vel, dots, st, 1);
     hipLaunchKernelGGL(k_fire_park, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, bn, fs);
     return check();
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free FIRE iteration
+// Workspace of cgpu_fire_iteration_noalloc: per-cell FIRE products (3 N doubles)
+// and per-chunk partial sums (4 per cell chunk), in a file-scope device array,
+// one copy per GPU, held until the process ends. Its device address is looked
+// up once per device with hipGetSymbolAddress (the host-side name is only a
+// stand-in); calls on one device share it.
+constexpr size_t SCRATCH_DOUBLES = size_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+// device address of this device's scratch (looked up on the first call)
+int scratch_of_current_device(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) return CGPU_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CGPU_ERR_HIP;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t cgpu_fire_noalloc_capacity(void) { return SCRATCH_DOUBLES; }
+
+int cgpu_fire_iteration_noalloc(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream) {
+    if (!b || !fs || !vel) return CGPU_ERR_ARGS;
+    const size_t need = 3 * (size_t)b->num_cells + 4 * (size_t)b->num_chunks;
+    if (need > SCRATCH_DOUBLES) {
+        cgpu_host::set_last_error(
+            ("cgpu_fire_iteration_noalloc: the workspace of 3 N + 4 chunks = " + std::to_string(need) +
+             " doubles exceeds the static scratch of " + std::to_string(SCRATCH_DOUBLES) +
+             " doubles (8 MiB); use cgpu_fire_iteration with a caller workspace")
+                .c_str());
+        return CGPU_ERR_SCRATCH_CAPACITY;
+    }
+    double *dots = nullptr;
+    const int r = scratch_of_current_device(&dots);
+    if (r) {
+        cgpu_host::set_last_error("cgpu_fire_iteration_noalloc: hipGetSymbolAddress of the scratch array failed");
+        return r;
+    }
+    return cgpu_fire_iteration(b, fs, vel, dots, stream);
 }
 
 int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state *fs, void *stream) {

@@ -21,6 +21,7 @@
 #ifndef CELLGPU_AMD_H
 #define CELLGPU_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -35,7 +36,8 @@ enum cgpu_status {
     CGPU_ERR_BIN_CAPACITY = -4,    /* reserved (the counting-sort cell list has no bin capacity) */
     CGPU_ERR_TOPOLOGY = -5,        /* global triangulation inconsistent / failed    */
     CGPU_ERR_NONFINITE = -6,       /* a position became NaN / inf                   */
-    CGPU_ERR_NO_DEVICE = -7
+    CGPU_ERR_NO_DEVICE = -7,
+    CGPU_ERR_SCRATCH_CAPACITY = -8 /* input too large for the static scratch of an allocation-free entry */
 };
 
 /* Maximum Delaunay degree stored per cell (ring capacity, slot-major storage). */
@@ -315,6 +317,18 @@ int cgpu_fire_park_converged(const cgpu_voronoi_batch *b, const cgpu_fire_state 
    Replaces the per-iteration kernel sequence of EnergyMinimizerFIRE2D.cpp:93-114
    and :156-207. */
 int cgpu_fire_iteration(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
+/* The same iteration with no allocation and no workspace argument, for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference. The FIRE products and partial sums live in a
+ * file-scope device array of cgpu_fire_noalloc_capacity() doubles (8 MiB) per
+ * GPU, held until the process ends; a batch needing more than that
+ * (3 num_cells + 4 num_chunks doubles) gets CGPU_ERR_SCRATCH_CAPACITY
+ * (cgpu_last_error says why) and nothing is launched. Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other streams or host
+ * threads) race; nothing in it needs clearing between calls. Replaces the same
+ * kernel sequence as cgpu_fire_iteration. */
+int cgpu_fire_iteration_noalloc(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
+size_t cgpu_fire_noalloc_capacity(void);
 /* apply a pending FIRE velocity mixing (end of minimize) */
 int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
