@@ -229,7 +229,7 @@ def serving(count=48, rate=8.0, steps_range=(1000, 10000), fire_fraction=0.25):
                                     count, rate, steps_range[0], steps_range[1], 100 * fire_fraction)}
 
 
-def other_workload(cg, name, steps, warmup):
+def other_workload(cg, name, steps, warmup, use_graph=True, init_steps=0):
     """Secondary workloads (survey cfg2 / cfg3), one JSON line each; the
     headline stays the ensemble."""
     if name == "single":
@@ -295,30 +295,29 @@ def other_workload(cg, name, steps, warmup):
                 "config": {"workload": "vertex Nc=32768 Nv=65536 T=1e-3 dt=0.005 thr=0.04 P0=3.9"}}
     if name == "ensemble_large":
         # the same ensemble distribution with 8x the systems (about 6.7 million
-        # cells, several GB of device state): beyond the 256 MB Infinity Cache
+        # cells, 2.35 GB of device state): beyond the 256 MB Infinity Cache;
+        # the headline's protocol (untimed initialization steps, warm-up, timed
+        # steps, per-kernel event pass)
         n, P0, v0, Dr, seeds = ensemble(S=4096, master_seed=20240608)
         N = int(n.sum())
+        bins = int(sum(max(2, int(np.floor(np.sqrt(k) / 1.25)) + (int(np.floor(np.sqrt(k) / 1.25)) % 2)) ** 2
+                       for k in n))
         b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
-        b.spv_steps(warmup)
+        b.spv_steps(init_steps, use_graph=use_graph)
+        b.spv_steps(warmup, use_graph=use_graph)
         b.check_status()
+        b.sync()
         t0 = time.perf_counter()
-        b.spv_steps(steps)
+        b.spv_steps(steps, use_graph=use_graph)
         b.sync()
         el = time.perf_counter() - t0
         b.check_status()
-        import ctypes as C
-        ks = 50
-        ms = np.zeros(4)
-        cg.check(cg.lib().cgpu_vor_spv_steps_timed(b.h, ks, ms.ctypes.data_as(C.c_void_p)), "timed steps")
-        avg = ms / ks
+        kt, queues = kernel_roofline(cg, b, N, bins, min(steps, 50))
         dev_bytes = int(cg.lib().cgpu_vor_device_bytes(b.h))
         return {"metric": "cell-steps/s, batched SPV ensemble of 4096 systems (device state beyond the Infinity Cache)",
                 "value": round(N * steps / el, 1), "unit": "cell-steps/s", "steps": steps, "warmup": warmup,
-                "ms_per_step": round(1e3 * el / steps, 5), "dtype": "f32", "device_bytes": dev_bytes,
-                "stage_ms": {"force_eom_move": round(avg[0], 5), "cell_list": round(avg[1], 5),
-                             "test_geometry": round(avg[2], 5), "repair_chain": round(avg[3], 5)},
-                "test_geometry_achieved_gbs": round(BYTES_TEST_GEOMETRY * N / (avg[2] * 1e-3) / 1e9, 1),
-                "force_eom_move_achieved_gbs": round(BYTES_FORCE_EOM_MOVE * N / (avg[0] * 1e-3) / 1e9, 1),
+                "init_steps": init_steps, "ms_per_step": round(1e3 * el / steps, 5), "dtype": "f32",
+                "device_bytes": dev_bytes, "kernels": kt, "queues_per_step": queues,
                 "config": {"workload": "4096 systems, N log-uniform [64,8192] (%d cells), unit density, "
                                        "P0~U[3.5,4], v0~U[0.01,0.2], Dr~U[0.1,10], dt=0.01" % N}}
     if name == "vertex_spp":
@@ -456,7 +455,8 @@ def main():
     import cellgpu_amd as cg
     cg.require_device()
     if args.workload != "ensemble":
-        print(json.dumps(other_workload(cg, args.workload, args.steps, args.warmup)), flush=True)
+        print(json.dumps(other_workload(cg, args.workload, args.steps, args.warmup, not args.no_graph,
+                                        args.init_steps)), flush=True)
         return
     n, P0, v0, Dr, seeds = ensemble()
     N = int(n.sum())

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 runs of bench.py for the profiles/ folder.
 
-    python -m cellgpu_amd.profile_summary trace  <dir> <out.md> "<command>" ["<title>"]
+    python -m cellgpu_amd.profile_summary trace  <dir> <out.md> "<command>" ["<title>"] [phases=a-b,c-d] [steps=K]
     python -m cellgpu_amd.profile_summary counters <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
         <out.md> <out.json> "<command>"
 
@@ -37,7 +37,12 @@ def _one(d, pattern):
     return hits[0]
 
 
-def trace(d, out, cmd, title="Voronoi SPV ensemble (512 systems, 833 607 cells): kernel trace"):
+def trace(d, out, cmd, title="Voronoi SPV ensemble (512 systems, 833 607 cells): kernel trace",
+          phases="0-20,20-60,60-120,120-210", steps=None):
+    """Kernel-trace summary. phases: dispatch-index windows "a-b,..." (a
+    dispatch index = a step for the once-per-step kernels); steps: the number of
+    steps of the run, to align kernels that also ran before the first step
+    (their last `steps` dispatches are the steps)."""
     stats = list(csv.DictReader(open(_one(d, "*kernel_stats.csv"))))
     tr = list(csv.DictReader(open(_one(d, "*kernel_trace.csv"))))
     tr.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -52,15 +57,18 @@ def trace(d, out, cmd, title="Voronoi SPV ensemble (512 systems, 833 607 cells):
         lines.append("| %s | %s | %.2f | %.2f | %.2f | %.2f | %.2f |" % (
             _name(r["Name"]), r["Calls"], int(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
             int(r["MinNs"]) / 1e3, int(r["MaxNs"]) / 1e3, float(r["Percentage"])))
-    phases = [(0, 20), (20, 60), (60, 120), (120, 210)]
+    phases = [tuple(int(x) for x in p.split("-")) for p in phases.split(",")]
+    need = steps if steps else phases[-1][1]
     lines += ["", "## mean us per dispatch by phase of the run (dispatch index = step)", "",
               "| kernel | " + " | ".join("%d-%d" % p for p in phases) + " | last 100 |",
               "|---|" + "---|" * (len(phases) + 1)]
     tot = np.zeros(len(phases) + 1)
-    for n, v in per.items():
+    for n, v in sorted(per.items()):
         v = np.array(v)
-        if len(v) < 300:
+        if len(v) < need:
             continue
+        if steps:
+            v = v[-steps:]
         row = [v[a:b].mean() for a, b in phases] + [v[-100:].mean()]
         tot += row
         lines.append("| %s | " % n + " | ".join("%.1f" % x for x in row) + " |")
@@ -111,8 +119,11 @@ def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
 if __name__ == "__main__":
     a = sys.argv[1:]
     if a and a[0] == "trace":
+        kw = dict(kv.split("=", 1) for kv in a[5:])
+        if "steps" in kw:
+            kw["steps"] = int(kw["steps"])
         if len(a) > 4:
-            trace(a[1], a[2], a[3], a[4])
+            trace(a[1], a[2], a[3], a[4], **kw)
         else:
             trace(a[1], a[2], a[3])
     elif a and a[0] == "counters":
