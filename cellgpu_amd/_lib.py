@@ -97,6 +97,8 @@ def lib():
         "cgpu_vor_set_noise_mode": (i, [vp, i]),
         "cgpu_vor_upload_noise": (i, [vp, vp]),
         "cgpu_vor_compute_forces": (i, [vp]),
+        "cgpu_vor_enforce_topology": (i, [vp]),
+        "cgpu_vm_enforce_topology": (i, [vp]),
         "cgpu_vor_spv_steps": (i, [vp, i, i]),
         "cgpu_vor_brownian_steps": (i, [vp, i]),
         "cgpu_vor_sync": (i, [vp]),

@@ -68,7 +68,7 @@ def build(verbose=False):
             continue
         exe = os.path.join(bdir, f[:-4])
         cmd = [HIPCC] + COMMON + ["-o", exe, os.path.join(ddir, f), "-L" + HERE, "-lcellgpu_amd",
-                                  "-Wl,-rpath," + HERE]
+                                  "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("driver build failed: %s\n%s" % (r.stdout, r.stderr))

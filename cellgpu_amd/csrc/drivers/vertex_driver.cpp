@@ -64,10 +64,10 @@ int main(int argc, char *argv[]) {
         ncdat->WriteState(avm);
     }
     const int saveEvery = std::max(1, (int)(100 / dt));
-    double2v mf = avm->reportMeanVertexForce();
+    double2v mf = avm->meanVertexForce();
     printf("mean vertex force = (%g, %g)\n", mf.x, mf.y);
     for (int ii = 0; ii < initSteps + 1; ++ii) sim->performTimestep();
-    mf = avm->reportMeanVertexForce();
+    mf = avm->meanVertexForce();
     printf("mean vertex force = (%g, %g)\n", mf.x, mf.y);
     if (ncdat) ncdat->WriteState(avm);
     auto t1 = std::chrono::steady_clock::now();
@@ -79,7 +79,7 @@ int main(int argc, char *argv[]) {
     auto t2 = std::chrono::steady_clock::now();
     printf("timestep time per iteration currently at %g\n\n",
            std::chrono::duration<double>(t2 - t1).count() / tSteps);
-    mf = avm->reportMeanVertexForce();
+    mf = avm->meanVertexForce();
     printf("mean vertex force = (%g, %g)\n", mf.x, mf.y);
     printf("Mean q = %f\n", avm->reportq());
     return 0;

@@ -430,6 +430,15 @@ class VertexBatch {
         sync();
         return r;
     }
+    // reference vertexModelBase::enforceTopology (vertexModelBase.cpp:93-106):
+    // T1 test and flips on the current positions, then the geometry
+    int enforce_topology() {
+        cgpu_vertex_batch b = view();
+        int r = cgpu_vm_t1(&b, stream_);
+        if (!r) r = cgpu_vm_geometry(&b, stream_);
+        if (r) return r;
+        return status();
+    }
     // kind 0: Brownian step, 1: self-propelled cell-vertex step
     int step_launch(int kind) {
         cgpu_vertex_batch b = view();
@@ -936,6 +945,7 @@ int cgpu_vm_upload_noise(void *h, const double *eta) {
     return vguard([&] { ((VertexBatch *)h)->upload_noise(eta); return 0; });
 }
 int cgpu_vm_compute_forces(void *h) { return vguard([&] { return ((VertexBatch *)h)->compute_forces(); }); }
+int cgpu_vm_enforce_topology(void *h) { return vguard([&] { return ((VertexBatch *)h)->enforce_topology(); }); }
 int cgpu_vm_upload_cell_noise(void *h, const double *eta) {
     return vguard([&] {
         ((VertexBatch *)h)->upload_cell_noise(eta);

@@ -587,6 +587,19 @@ int VoronoiBatch::global_triangulation() {
     return status();
 }
 
+// reference voronoiModelBase::enforceTopology (voronoiModelBase.cpp:686-739) on
+// the current positions: cell list, circumcircle test, repair; the step
+// counters do not advance
+int VoronoiBatch::enforce_topology() {
+    cgpu_voronoi_batch b = view();
+    int r = cgpu_step_begin(&b, stream_);
+    if (!r) r = cgpu_compute_cell_list(&b, 0, stream_);
+    if (!r) r = cgpu_test_and_geometry(&b, 1, stream_);
+    if (!r) r = cgpu_repair_topology(&b, stream_);
+    if (r) return r;
+    return status();
+}
+
 int VoronoiBatch::compute_forces() {
     cgpu_voronoi_batch b = view();
     int r = cgpu_voronoi_forces(&b, stream_);
@@ -1205,6 +1218,7 @@ int cgpu_vor_upload_noise(void *h, const double *eta) {
     });
 }
 int cgpu_vor_compute_forces(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->compute_forces(); }); }
+int cgpu_vor_enforce_topology(void *h) { return guarded([&] { return ((VoronoiBatch *)h)->enforce_topology(); }); }
 int cgpu_vor_spv_steps(void *h, int steps, int use_graph) {
     return guarded([&] { return ((VoronoiBatch *)h)->spv_steps(steps, use_graph != 0); });
 }

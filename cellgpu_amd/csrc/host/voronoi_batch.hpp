@@ -94,6 +94,7 @@ class VoronoiBatch {
 
     int global_triangulation();
     int compute_forces();
+    int enforce_topology();
     int spv_steps(int steps, bool use_graph);
     int spv_steps_timed(int steps, double *ms_out);
     // steps with an event after every kernel: summed ms per profile slot

@@ -432,6 +432,11 @@ int cgpu_vor_get_del_other(void *h, int *oth, int cap);
 int cgpu_vor_set_noise_mode(void *h, int use_buffer);
 int cgpu_vor_upload_noise(void *h, const double *eta);                      /* N doubles, this step */
 int cgpu_vor_compute_forces(void *h);
+/* reference voronoiModelBase::enforceTopology (voronoiModelBase.cpp:686-739) on
+ * the current positions (after positions were moved by the caller, e.g. with
+ * cgpu_vor_set_positions(..., 0)): cell list, circumcircle test and repair;
+ * step counters unchanged */
+int cgpu_vor_enforce_topology(void *h);
 int cgpu_vor_spv_steps(void *h, int steps, int use_graph);
 int cgpu_vor_brownian_steps(void *h, int steps);
 int cgpu_vor_sync(void *h);
@@ -487,6 +492,9 @@ int cgpu_vm_set_noise_mode(void *h, int use_buffer);
 int cgpu_vm_upload_noise(void *h, const double *eta);                     /* 2 Nv: x block, y block */
 int cgpu_vm_upload_cell_noise(void *h, const double *eta);                /* Nc: director rotation */
 int cgpu_vm_compute_forces(void *h);
+/* reference vertexModelBase::enforceTopology (vertexModelBase.cpp:93-106): T1
+ * test and flips on the current vertex positions, then the geometry */
+int cgpu_vm_enforce_topology(void *h);
 int cgpu_vm_brownian_steps(void *h, int steps, int use_graph);
 /* Self-propelled cell-vertex dynamics (reference selfPropelledCellVertexDynamics):
  * per-cell (v0, Dr) and director angles. The directors start as the reference's
