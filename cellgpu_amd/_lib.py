@@ -29,6 +29,18 @@ class SysParams(C.Structure):
                 ("ncells", C.c_int), ("tension", C.c_int), ("gamma", C.c_double)]
 
 
+class SysState(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "any_fail", "nf_count", "mode", "need_global", "nn_delta", "tri_delta", "max_deg", "neigh_max", "step",
+        "active", "steps_left", "error", "nflag", "pad_s")] + [
+        (n, C.c_longlong) for n in ("global_fixes", "repair_events", "local_repairs", "t1_events")]
+
+
+class VorWorkspace(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("num_systems", "num_cells", "total_bins", "num_chunks", "num_bin_chunks",
+                                       "precision")] + [("bytes", C.c_longlong)]
+
+
 class FireState(C.Structure):
     """Mirror of cgpu_fire_state (include/cellgpu_amd.h)."""
 
@@ -117,6 +129,9 @@ def lib():
         "cgpu_vor_spv_steps_profiled": (i, [vp, i, vp, vp]),
         "cgpu_profile_slot_name": (C.c_char_p, [i]),
         "cgpu_vor_view": (i, [vp, P(VoronoiBatchView)]),
+        "cgpu_vor_default_params": (i, [i, vp, vp, vp, P(SysParams)]),
+        "cgpu_vor_workspace_query": (i, [i, P(SysParams), i, P(VorWorkspace)]),
+        "cgpu_vor_workspace_bind": (i, [P(VorWorkspace), P(SysParams), vp, C.c_size_t, P(VoronoiBatchView), vp]),
         "cgpu_vm_view": (i, [vp, P(VertexBatchView)]),
         "cgpu_vm_stream": (vp, [vp]),
         "cgpu_vor_stream": (vp, [vp]),

@@ -29,6 +29,66 @@ void reference_random_init(int n, uint32_t seed, double L, double *x, double *y,
     }
 }
 
+cgpu_sys_params default_sys_params(int n, double L, uint64_t seed) {
+    if (n < 4) throw std::invalid_argument("each system needs at least 4 cells");
+    cgpu_sys_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.n = n;
+    p.ncells = n;
+    p.L = L > 0 ? L : std::sqrt((double)n);
+    // reference cellListGPU::setGridSize(1.25): even number of bins per side
+    int xs = (int)std::floor(p.L / 1.25);
+    if (xs % 2 == 1) xs += 1;
+    if (xs < 2) xs = 2;
+    p.xsize = xs;
+    p.boxsize = p.L / xs;
+    p.KA = 1.0;
+    p.KP = 1.0;
+    p.dt = 0.01;
+    p.mu = 1.0;
+    p.T = 0.0;
+    p.seed = seed;
+    return p;
+}
+
+BatchTables batch_tables(std::vector<cgpu_sys_params> &hp) {
+    BatchTables t;
+    int off = 0, boff = 0;
+    for (auto &p : hp) {
+        if (p.n < 4) throw std::invalid_argument("each system needs at least 4 cells");
+        if (p.xsize < 2) throw std::invalid_argument("each system needs at least 2 bins per side");
+        if ((long long)off + p.n >= (1LL << 26))
+            throw std::invalid_argument("a batch holds fewer than 2^26 cells (queue entries pack the cell index)");
+        p.off = off;
+        p.ncells = p.n;
+        p.bin_off = boff;
+        off += p.n;
+        boff += p.xsize * p.xsize;
+    }
+    t.N = off;
+    t.total_bins = boff;
+    t.cell_sys.assign(t.N, 0);
+    for (int s = 0; s < (int)hp.size(); ++s) {
+        const cgpu_sys_params &p = hp[s];
+        for (int i = 0; i < p.n; ++i) t.cell_sys[p.off + i] = s;
+        // work chunks inside systems (cells and cell-list bins)
+        for (int c = p.off; c < p.off + p.n; c += CGPU_CHUNK) {
+            t.cs.push_back(s);
+            t.cb.push_back(c);
+            t.ce.push_back(std::min(c + CGPU_CHUNK, p.off + p.n));
+        }
+        t.sb.push_back((int)t.bs.size());
+        const int nb = p.xsize * p.xsize;
+        for (int u = 0; u < nb; u += CGPU_BIN_CHUNK) {
+            t.bs.push_back(s);
+            t.bb.push_back(p.bin_off + u);
+            t.be.push_back(p.bin_off + std::min(u + CGPU_BIN_CHUNK, nb));
+        }
+    }
+    t.sb.push_back((int)t.bs.size());
+    return t;
+}
+
 VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const uint64_t *seed, const double *x,
                            const double *y, const double *theta, int precision, int device)
     : S_(num_systems), prec_(precision ? 1 : 0), device_(device) {
@@ -36,24 +96,7 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     hip_check(hipSetDevice(device_), "hipSetDevice");
     rsz_ = prec_ ? sizeof(float) : sizeof(double);
     hp_.resize(S_);
-    for (int s = 0; s < S_; ++s) {
-        if (n[s] < 4) throw std::invalid_argument("each system needs at least 4 cells");
-        cgpu_sys_params &p = hp_[s];
-        std::memset(&p, 0, sizeof(p));
-        p.L = L ? L[s] : std::sqrt((double)n[s]);
-        // reference cellListGPU::setGridSize(1.25): even number of bins per side
-        int xs = (int)std::floor(p.L / 1.25);
-        if (xs % 2 == 1) xs += 1;
-        if (xs < 2) xs = 2;
-        p.xsize = xs;
-        p.boxsize = p.L / xs;
-        p.KA = 1.0;
-        p.KP = 1.0;
-        p.dt = 0.01;
-        p.mu = 1.0;
-        p.T = 0.0;
-        p.seed = seed ? seed[s] : 13377ull;
-    }
+    for (int s = 0; s < S_; ++s) hp_[s] = default_sys_params(n[s], L ? L[s] : 0.0, seed ? seed[s] : 13377ull);
     std::vector<int> ns(n, n + S_);
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     sys_.alloc(S_);
@@ -123,61 +166,28 @@ VoronoiBatch::~VoronoiBatch() {
 // Offsets, cell-list bins, work chunks and every per-cell device array for
 // system sizes n (the per-system constants in hp_ are kept). Arrays start zeroed.
 void VoronoiBatch::layout(const std::vector<int> &n) {
-    int off = 0, boff = 0;
-    for (int s = 0; s < S_; ++s) {
-        cgpu_sys_params &p = hp_[s];
-        if (n[s] < 4) throw std::invalid_argument("each system needs at least 4 cells");
-        if ((long long)off + n[s] >= (1LL << 26))
-            throw std::invalid_argument("a batch holds fewer than 2^26 cells (queue entries pack the cell index)");
-        p.off = off;
-        p.n = n[s];
-        p.ncells = n[s];
-        p.bin_off = boff;
-        off += n[s];
-        boff += p.xsize * p.xsize;
-    }
-    N_ = off;
-    total_bins_ = boff;
-    cell_sys_h_.assign(N_, 0);
-    for (int s = 0; s < S_; ++s)
-        for (int i = 0; i < hp_[s].n; ++i) cell_sys_h_[hp_[s].off + i] = s;
-    {
-        // work chunks inside systems (cells and cell-list bins)
-        std::vector<int> cs, cb, ce, bs, bb, be, sb;
-        for (int s = 0; s < S_; ++s) {
-            const cgpu_sys_params &p = hp_[s];
-            for (int c = p.off; c < p.off + p.n; c += CGPU_CHUNK) {
-                cs.push_back(s);
-                cb.push_back(c);
-                ce.push_back(std::min(c + CGPU_CHUNK, p.off + p.n));
-            }
-            sb.push_back((int)bs.size());
-            const int nb = p.xsize * p.xsize;
-            for (int u = 0; u < nb; u += CGPU_BIN_CHUNK) {
-                bs.push_back(s);
-                bb.push_back(p.bin_off + u);
-                be.push_back(p.bin_off + std::min(u + CGPU_BIN_CHUNK, nb));
-            }
-        }
-        sb.push_back((int)bs.size());
-        num_chunks_ = (int)cs.size();
-        num_bin_chunks_ = (int)bs.size();
-        chunk_sys_.alloc(cs.size());
-        chunk_sys_.upload(cs.data(), cs.size());
-        chunk_begin_.alloc(cb.size());
-        chunk_begin_.upload(cb.data(), cb.size());
-        chunk_end_.alloc(ce.size());
-        chunk_end_.upload(ce.data(), ce.size());
-        bchunk_sys_.alloc(bs.size());
-        bchunk_sys_.upload(bs.data(), bs.size());
-        bchunk_begin_.alloc(bb.size());
-        bchunk_begin_.upload(bb.data(), bb.size());
-        bchunk_end_.alloc(be.size());
-        bchunk_end_.upload(be.data(), be.size());
-        sys_bchunk_.alloc(sb.size());
-        sys_bchunk_.upload(sb.data(), sb.size());
-        bchunk_sum_.alloc(bs.size());
-    }
+    for (int s = 0; s < S_; ++s) hp_[s].n = n[s];
+    const BatchTables t = batch_tables(hp_);
+    N_ = t.N;
+    total_bins_ = t.total_bins;
+    cell_sys_h_ = t.cell_sys;
+    num_chunks_ = (int)t.cs.size();
+    num_bin_chunks_ = (int)t.bs.size();
+    chunk_sys_.alloc(t.cs.size());
+    chunk_sys_.upload(t.cs.data(), t.cs.size());
+    chunk_begin_.alloc(t.cb.size());
+    chunk_begin_.upload(t.cb.data(), t.cb.size());
+    chunk_end_.alloc(t.ce.size());
+    chunk_end_.upload(t.ce.data(), t.ce.size());
+    bchunk_sys_.alloc(t.bs.size());
+    bchunk_sys_.upload(t.bs.data(), t.bs.size());
+    bchunk_begin_.alloc(t.bb.size());
+    bchunk_begin_.upload(t.bb.data(), t.bb.size());
+    bchunk_end_.alloc(t.be.size());
+    bchunk_end_.upload(t.be.data(), t.be.size());
+    sys_bchunk_.alloc(t.sb.size());
+    sys_bchunk_.upload(t.sb.data(), t.sb.size());
+    bchunk_sum_.alloc(t.bs.size());
     cell_sys_.alloc(N_);
     nn_.alloc(N_);
     nbr_.alloc((size_t)CGPU_RING_CAP * N_);

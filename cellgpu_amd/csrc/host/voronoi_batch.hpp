@@ -54,6 +54,21 @@ template <typename T> struct DevBuf {
 // std::mt19937(seed), x then y per cell, then the directors, uniform in [0,L) / [0,2pi)
 void reference_random_init(int n, uint32_t seed, double L, double *x, double *y, double *theta);
 
+// Per-system constants of a new batch: box side L (L <= 0: sqrt(n), unit
+// density), the cell-list grid of the reference cellListGPU::setGridSize(1.25)
+// (even number of bins per side, at least 2), moduli 1, dt 0.01, mu 1, T 0.
+cgpu_sys_params default_sys_params(int n, double L, uint64_t seed);
+
+// Offsets and work-chunk tables of a batch of systems (the n and xsize of every
+// system are read; off, ncells and bin_off are filled in).
+struct BatchTables {
+    int N = 0, total_bins = 0;
+    std::vector<int> cell_sys;         // [N]
+    std::vector<int> cs, cb, ce;       // cell chunks: system, begin, end
+    std::vector<int> bs, bb, be, sb;   // bin chunks: system, begin, end; first bin chunk per system [S+1]
+};
+BatchTables batch_tables(std::vector<cgpu_sys_params> &hp);
+
 class VoronoiBatch {
    public:
     VoronoiBatch(int num_systems, const int *n, const double *L, const uint64_t *seed, const double *x,

@@ -332,6 +332,31 @@ size_t cgpu_fire_noalloc_capacity(void);
 /* apply a pending FIRE velocity mixing (end of minimize) */
 int cgpu_fire_apply_mix(const cgpu_voronoi_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
+/* Caller-owned kernel-layer batches (the reference's kernels take arrays their
+ * callers own, GPUArray in voronoiModelBase.h:26-107; here one allocation holds
+ * them all). cgpu_vor_default_params fills the per-system constants of a new
+ * batch (L NULL: sqrt(n); seed NULL: 13377) with its offsets; a caller may edit
+ * KA, KP, dt, mu, T, seed, tension, gamma. cgpu_vor_workspace_query gives the
+ * array counts and the bytes of one device allocation holding every array of
+ * the batch; cgpu_vor_workspace_bind carves such an allocation (256-byte
+ * aligned, at least ws->bytes; allocated by the caller) into a batch view:
+ * every array zeroed, the layout tables and parameters uploaded, systems active.
+ * It allocates nothing and returns after its copies have finished (stream).
+ * The caller then writes pos (x, y interleaved), APpref (A0, P0), mot (v0, Dr),
+ * vel and theta in the batch precision, builds the rings with
+ * cgpu_global_triangulation, and runs the step entry points, swapping pos and
+ * pos_next after every step (the step writes pos_next). Optional arrays (excl,
+ * ext_force, noise, cell_type, tension) start NULL. */
+typedef struct cgpu_vor_workspace {
+    int num_systems, num_cells, total_bins, num_chunks, num_bin_chunks, precision;
+    long long bytes;
+} cgpu_vor_workspace;
+int cgpu_vor_default_params(int num_systems, const int *n, const double *L, const uint64_t *seed,
+                            cgpu_sys_params *out);
+int cgpu_vor_workspace_query(int num_systems, const cgpu_sys_params *params, int precision, cgpu_vor_workspace *ws);
+int cgpu_vor_workspace_bind(const cgpu_vor_workspace *ws, const cgpu_sys_params *params, void *mem, size_t mem_bytes,
+                            cgpu_voronoi_batch *out, void *stream);
+
 /* Vertex model, replacing gpu_vm_geometry (reference vertexModelBase.cuh:19-30,
  * vertexModelBase.cu:20-77), gpu_avm_force_sets + gpu_avm_sum_force_sets
  * (vertexQuadraticEnergy.cuh:20-33, .cu:21-70), gpu_brownian_eom_integration
