@@ -203,12 +203,13 @@ def cpu_baseline(runs=5, steps=1000, N=1000):
                 cpu_model=model, host_cpus_available=avail)
 
 
-def serving(count=48, rate=8.0, steps_range=(1000, 10000), fire_fraction=0.25):
-    """Serving workload (bounded): Poisson arrivals; SPV requests with N
-    log-uniform [100, 65536] and steps log-uniform over steps_range, and a
-    fraction of vertex-model FIRE requests (N log-uniform [100, 4096], RMS force
-    below 1e-8 within 2000 iterations), admitted into batches by the continuous
-    batching engine; latency = completion - arrival."""
+def serving(count=48, rate=8.0, steps_range=(1000, 100000), fire_fraction=0.25):
+    """Serving workload: Poisson arrivals; SPV requests with N log-uniform
+    [100, 65536] and steps log-uniform over steps_range, and a fraction of
+    vertex-model FIRE requests (N log-uniform [100, 65536], run until the RMS
+    vertex force is below 1e-8, no iteration limit), served by the continuous
+    batching engine (SPV pool rebuilt at tick boundaries, FIRE batches in worker
+    threads); latency = completion - arrival."""
     from cellgpu_amd.serving import ServingEngine, poisson_requests
     reqs = poisson_requests(count, rate, seed=20240607, steps_range=steps_range, fire_fraction=fire_fraction)
     eng = ServingEngine(admit_interval=0.05, tick_steps=64, precision="fp32")
@@ -217,15 +218,22 @@ def serving(count=48, rate=8.0, steps_range=(1000, 10000), fire_fraction=0.25):
     wall = time.perf_counter() - t0
     lat = np.array([r.latency for r in done])
     fire = [r for r in done if r.kind == "vertex_fire"]
+    spv = [r for r in done if r.kind == "spv"]
     return {"serving_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
             "serving_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 1),
+            "serving_spv_p50_ms": round(1e3 * float(np.percentile([r.latency for r in spv], 50)), 1) if spv else None,
+            "serving_fire_p50_ms": round(1e3 * float(np.percentile([r.latency for r in fire], 50)), 1) if fire else None,
             "serving_cell_steps_per_s": round(eng.cell_steps / wall, 1),
+            "serving_wall_s": round(wall, 2),
+            "serving_pool_rebuilds": eng.rebuilds,
             "serving_fire_requests": len(fire),
-            "serving_fire_converged": sum(r.iterations < r.steps for r in fire),
+            "serving_fire_converged": sum(bool(r.converged) for r in fire),
+            "serving_fire_max_iterations": max((r.iterations for r in fire), default=0),
             "serving_workload": "%d requests, Poisson %.1f/s; SPV: N log-uniform [100, 65536], steps log-uniform "
                                 "[%d, %d], P0~U[3.5,4], v0~U[0,0.2], FP32 bulk; %.0f%% vertex-model FIRE: N "
-                                "log-uniform [100, 4096], P0~U[3.5,4], RMS force < 1e-8 within 2000 iterations, "
-                                "FP64; continuous batching (cell-steps include FIRE cell-iterations)" % (
+                                "log-uniform [100, 65536], P0~U[3.5,4], until RMS force < 1e-8 (no iteration "
+                                "limit), FP64; one SPV pool rebuilt at tick boundaries, FIRE in worker threads "
+                                "(cell-steps include FIRE cell-iterations)" % (
                                     count, rate, steps_range[0], steps_range[1], 100 * fire_fraction)}
 
 

@@ -59,6 +59,30 @@ class VoronoiBatch:
         if not self.h:
             raise CellGPUError("cgpu_vor_create failed: " + lib().cgpu_last_error().decode())
 
+    @classmethod
+    def merge(cls, parts):
+        """A new batch of the systems parts = [(batch, s), ...] (system s of each
+        batch, in that order), each continuing its trajectory exactly; the sources
+        stay valid (cgpu_vor_merge)."""
+        require_device()
+        if not parts:
+            raise ValueError("merge needs at least one system")
+        hs = (C.c_void_p * len(parts))(*[b.h for b, _ in parts])
+        ss = np.ascontiguousarray([int(s) for _, s in parts], np.int32)
+        h = lib().cgpu_vor_merge(len(parts), hs, _p(ss))
+        if not h:
+            raise CellGPUError("cgpu_vor_merge failed: " + lib().cgpu_last_error().decode())
+        self = cls.__new__(cls)
+        self.h = h
+        self.n = np.array([b.n[s] for b, s in parts], np.int32)
+        self.S = len(parts)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+        self.N = int(self.offsets[-1])
+        self.L = np.array([b.L[s] for b, s in parts], np.float64)
+        self.precision = parts[0][0].precision
+        self._keep = ()
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             lib().cgpu_vor_destroy(self.h)

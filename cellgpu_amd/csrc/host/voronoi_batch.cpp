@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <random>
 
 #include "../cgpu_device.h"
@@ -153,6 +154,135 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     }
     int r = global_triangulation();
     if (r != CGPU_OK) throw std::runtime_error("initial triangulation failed, status " + std::to_string(r));
+}
+
+VoronoiBatch::VoronoiBatch(Empty, int num_systems, int precision, int device)
+    : S_(num_systems), prec_(precision ? 1 : 0), device_(device) {
+    if (S_ <= 0) throw std::invalid_argument("num_systems must be positive");
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    rsz_ = prec_ ? sizeof(float) : sizeof(double);
+    hp_.resize(S_);
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    sys_.alloc(S_);
+    st_.alloc(S_);
+    counters_.alloc(8);
+}
+
+VoronoiBatch *VoronoiBatch::merge(int count, VoronoiBatch *const *src, const int *sys) {
+    if (count <= 0 || !src || !sys) throw std::invalid_argument("merge needs at least one system");
+    for (int k = 0; k < count; ++k) {
+        const VoronoiBatch *b = src[k];
+        if (!b || sys[k] < 0 || sys[k] >= b->S_) throw std::invalid_argument("merge: no such system");
+        if (b->prec_ != src[0]->prec_ || b->device_ != src[0]->device_)
+            throw std::invalid_argument("merge: the sources differ in precision or device");
+        if (b->noise_buffer_ || b->excl_.p || b->tension_on_)
+            throw std::invalid_argument("merge: exclusions, noise buffers and line tension are not carried");
+    }
+    // host copies of every source, once each
+    struct Host {
+        std::vector<double> pos;
+        std::vector<int> nn, nbr, oth;
+        std::vector<unsigned char> AP, dAP, pref, mot, vel, theta, force;
+        std::vector<cgpu_sys_state> st;
+    };
+    std::vector<const VoronoiBatch *> uniq;
+    std::vector<Host> host;
+    std::vector<int> which(count);
+    for (int k = 0; k < count; ++k) {
+        auto it = std::find(uniq.begin(), uniq.end(), src[k]);
+        if (it == uniq.end()) {
+            VoronoiBatch *b = src[k];
+            b->sync();
+            Host h;
+            const size_t N = b->N_, rs = b->rsz_;
+            h.pos.resize(2 * N);
+            hip_check(hipMemcpy(h.pos.data(), b->parity_ ? b->pos1_.p : b->pos0_.p, 16 * N, hipMemcpyDeviceToHost),
+                      "merge download");
+            h.nn.resize(N);
+            b->nn_.download(h.nn.data(), N);
+            h.nbr.resize((size_t)CGPU_RING_CAP * N);
+            b->nbr_.download(h.nbr.data(), h.nbr.size());
+            h.oth.resize(h.nbr.size());
+            b->oth_.download(h.oth.data(), h.oth.size());
+            auto raw = [&](std::vector<unsigned char> &v, const DevBuf<unsigned char> &d, size_t per) {
+                v.resize(per * N * rs);
+                d.download(v.data(), v.size());
+            };
+            raw(h.AP, b->AP_, 2);
+            raw(h.dAP, b->dAP_, 2);
+            raw(h.pref, b->APpref_, 2);
+            raw(h.mot, b->mot_, 2);
+            raw(h.vel, b->vel_, 2);
+            raw(h.theta, b->theta_, 1);
+            raw(h.force, b->force_, 2);
+            h.st.resize(b->S_);
+            b->st_.download(h.st.data(), b->S_);
+            uniq.push_back(b);
+            host.push_back(std::move(h));
+            it = uniq.end() - 1;
+        }
+        which[k] = (int)(it - uniq.begin());
+    }
+
+    std::unique_ptr<VoronoiBatch> m(new VoronoiBatch(Empty{}, count, src[0]->prec_, src[0]->device_));
+    std::vector<int> ns(count);
+    for (int k = 0; k < count; ++k) {
+        m->hp_[k] = src[k]->hp_[sys[k]];
+        ns[k] = m->hp_[k].n;
+    }
+    m->layout(ns);  // offsets, tables, zeroed arrays, parameters
+    const size_t N = m->N_, rs = m->rsz_, CAP = CGPU_RING_CAP;
+    std::vector<double> pos(2 * N);
+    std::vector<int> nn(N), nbr(CAP * N, 0), oth(CAP * N, 0), tag(N);
+    std::vector<unsigned char> AP(2 * N * rs), dAP(2 * N * rs), pref(2 * N * rs), mot(2 * N * rs), vel(2 * N * rs),
+        theta(N * rs), force(2 * N * rs);
+    std::vector<cgpu_sys_state> st(count);
+    for (int k = 0; k < count; ++k) {
+        const VoronoiBatch &b = *src[k];
+        const Host &h = host[which[k]];
+        const cgpu_sys_params &op = b.hp_[sys[k]];
+        const int o = op.off, no = m->hp_[k].off, n = op.n;
+        const size_t Nold = b.N_;
+        std::copy(h.pos.begin() + 2 * (size_t)o, h.pos.begin() + 2 * (size_t)(o + n), pos.begin() + 2 * (size_t)no);
+        for (int i = 0; i < n; ++i) {
+            const int z = h.nn[o + i];
+            nn[no + i] = z;
+            tag[no + i] = b.tag_[o + i];
+            for (int q = 0; q < z; ++q) {
+                const int j = h.nbr[q * Nold + o + i], w = h.oth[q * Nold + o + i];
+                if (j < o || j >= o + n || w < o || w >= o + n)
+                    throw std::runtime_error("merge: a ring entry lies outside its system");
+                nbr[q * N + no + i] = j - o + no;
+                oth[q * N + no + i] = w - o + no;
+            }
+        }
+        auto cp = [&](std::vector<unsigned char> &dst, const std::vector<unsigned char> &srcv, size_t per) {
+            std::copy(srcv.begin() + per * rs * o, srcv.begin() + per * rs * (o + n), dst.begin() + per * rs * no);
+        };
+        cp(AP, h.AP, 2);
+        cp(dAP, h.dAP, 2);
+        cp(pref, h.pref, 2);
+        cp(mot, h.mot, 2);
+        cp(vel, h.vel, 2);
+        cp(theta, h.theta, 1);
+        cp(force, h.force, 2);
+        st[k] = h.st[sys[k]];
+    }
+    hip_check(hipMemcpy(m->pos0_.p, pos.data(), 16 * N, hipMemcpyHostToDevice), "merge upload");
+    m->nn_.upload(nn.data(), N);
+    m->nbr_.upload(nbr.data(), nbr.size());
+    m->oth_.upload(oth.data(), oth.size());
+    m->AP_.upload(AP.data(), AP.size());
+    m->dAP_.upload(dAP.data(), dAP.size());
+    m->APpref_.upload(pref.data(), pref.size());
+    m->mot_.upload(mot.data(), mot.size());
+    m->vel_.upload(vel.data(), vel.size());
+    m->theta_.upload(theta.data(), theta.size());
+    m->force_.upload(force.data(), force.size());
+    m->st_.upload(st.data(), count);
+    m->tag_ = tag;
+    m->shape_ = src[0]->shape_;
+    return m.release();
 }
 
 VoronoiBatch::~VoronoiBatch() {
@@ -1146,6 +1276,14 @@ void *cgpu_vor_create(int num_systems, const int *n, const double *L, const uint
                       const double *y, const double *theta, int precision, int device) {
     try {
         return new VoronoiBatch(num_systems, n, L, seed, x, y, theta, precision, device);
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+void *cgpu_vor_merge(int count, void *const *batches, const int *systems) {
+    try {
+        return VoronoiBatch::merge(count, (VoronoiBatch *const *)batches, systems);
     } catch (const std::exception &e) {
         g_last_error = e.what();
         return nullptr;

@@ -35,7 +35,12 @@ template <typename T> struct DevBuf {
         release();
         n = count;
         if (count) hip_check(hipMalloc((void **)&p, count * sizeof(T)), "hipMalloc");
-        if (count) hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+        if (count) {
+            hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+            // the fill runs on the null stream, which the batches' non-blocking
+            // streams do not wait for: finish it before the buffer is used
+            hip_check(hipStreamSynchronize(nullptr), "hipMemset");
+        }
     }
     void release() {
         if (p) (void)hipFree(p);
@@ -149,8 +154,18 @@ class VoronoiBatch {
     // retriangulates.
     int cell_events(int count, const int *kind, const int *cell, const double *theta, const double *frac,
                     const double *new_director);
+    // A new batch holding system sys[k] of batch src[k] as its system k, every
+    // piece of its state copied exactly (positions, velocities, directors,
+    // preferences, motility, rings and delOther with their indices shifted, the
+    // per-system parameters and device state, tags): stepping it gives the same
+    // trajectories as the sources would. The sources must share precision and
+    // device and use no exclusions, noise buffer or line tension.
+    static VoronoiBatch *merge(int count, VoronoiBatch *const *src, const int *sys);
 
    private:
+    struct Empty {};
+    // streams and per-batch buffers only; layout() and uploads follow
+    VoronoiBatch(Empty, int num_systems, int precision, int device);
     void upload_params();
     void real_upload(void *dst, const double *src2, size_t count);
     void real_download(const void *src, double *dst, size_t count);

@@ -473,6 +473,15 @@ int cgpu_vor_num_cells(void *h);
 double cgpu_vor_energy(void *h, int s);
 int cgpu_vor_status(void *h);
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
+/* Serving layer: a new batch whose system k is system systems[k] of batch
+ * batches[k], with every piece of its state copied exactly (positions, stored
+ * velocities, directors, preferences, motility, Delaunay rings and delOther,
+ * per-system parameters, step counter, remaining steps, repair counters, tags),
+ * so each system continues its trajectory bit for bit. Used to drop finished
+ * systems and admit new ones at step boundaries. The sources are unchanged;
+ * they must share precision and device and use no exclusions, noise buffer or
+ * line tension. NULL on error (cgpu_last_error). */
+void *cgpu_vor_merge(int count, void *const *batches, const int *systems);
 /* Serving layer (no counterpart in the reference, which runs one system per
  * process): per-system active flags and remaining steps after the work queued
  * so far (waits for the batch stream), and the device memory a batch holds, for
