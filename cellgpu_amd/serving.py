@@ -123,6 +123,8 @@ class _FireJob(threading.Thread):
             open_ = np.ones(len(group), bool)
             iters = 0
             while open_.any():
+                while eng._capture_pending.is_set():  # let the pool's graph capture in first
+                    time.sleep(0.001)
                 with eng._gpu_lock:
                     self._tick(group, ns, open_, iters + eng.tick_iterations, b, f)
                 iters += eng.tick_iterations
@@ -158,7 +160,8 @@ class _FireJob(threading.Thread):
                     r.result = (g["vx"][lo:lo + 2 * r.n].copy(), g["vy"][lo:lo + 2 * r.n].copy())
                 open_[k] = False
                 active[k] = 0
-                eng._complete(r, t, r.n * r.iterations)
+                # work that never reached its target is not counted as served
+                eng._complete(r, t, r.n * r.iterations if conv else 0)
         if newly:
             check(lib().cgpu_vm_set_active(b.h, active.ctypes.data), "vm_set_active")
 
@@ -171,14 +174,15 @@ class ServingEngine:
     a pool is running; tick_steps: SPV steps between polls; tick_iterations: FIRE
     iterations between checks; reclaim_fraction: rebuild the pool when finished
     systems hold more than this fraction of its cells; keep_results: copy each
-    finished system's positions into request.result.
+    finished system's positions into request.result; on_complete: called with
+    each request as it completes (from the thread that completed it).
     """
 
     BYTES_PER_CELL_ESTIMATE = 600  # FP32 bulk, ring capacity 24, before the exact figure is known
 
     def __init__(self, hbm_budget=64 << 30, admit_interval=0.05, tick_steps=64, precision="fp32", dt=0.01,
                  max_batch_cells=1 << 22, keep_results=False, clock=time.perf_counter, tick_iterations=64,
-                 reclaim_fraction=0.25):
+                 reclaim_fraction=0.25, on_complete=None):
         self.budget, self.admit_interval, self.tick_steps = int(hbm_budget), float(admit_interval), int(tick_steps)
         self.tick_iterations = int(tick_iterations)
         self.precision, self.dt, self.max_batch_cells = precision, float(dt), int(max_batch_cells)
@@ -191,6 +195,8 @@ class ServingEngine:
         self.rebuilds = 0
         self._lock = threading.Lock()  # completions from worker threads
         self._gpu_lock = threading.Lock()  # a graph capture vs the workers' HIP calls
+        self._capture_pending = threading.Event()
+        self.on_complete = on_complete
 
     @property
     def used(self):
@@ -204,6 +210,8 @@ class ServingEngine:
             r.done = t
             self.cell_steps += cell_steps
             self.finished.append(r)
+            if self.on_complete is not None:
+                self.on_complete(r)
 
     # ------------------------------------------------------------------ SPV pool
     def _new_systems(self, group):
@@ -283,8 +291,12 @@ class ServingEngine:
         if p.captured:
             p.batch.spv_steps(self.tick_steps, use_graph=True)
         else:
-            with self._gpu_lock:
-                p.batch.spv_steps(self.tick_steps, use_graph=True)
+            self._capture_pending.set()
+            try:
+                with self._gpu_lock:
+                    p.batch.spv_steps(self.tick_steps, use_graph=True)
+            finally:
+                self._capture_pending.clear()
             p.captured = self.tick_steps >= 2
         active = np.zeros(p.batch.S, np.int32)
         check(lib().cgpu_vor_get_active(p.batch.h, active.ctypes.data, None), "get_active")
