@@ -107,6 +107,17 @@ class VoronoiBatch:
         e = np.ascontiguousarray(ex, dtype=np.int32)
         check(lib().cgpu_vor_set_exclusions(self.h, _p(e)), "set_exclusions")
 
+    def external_forces(self):
+        """(fx, fy): minus the summed force sets of excluded cells, zero elsewhere."""
+        fx, fy = np.zeros(self.N), np.zeros(self.N)
+        check(lib().cgpu_vor_get_external_forces(self.h, _p(fx), _p(fy)), "get_external_forces")
+        return fx, fy
+
+    def enforce_topology(self):
+        """Test and repair of the rings at the current positions (reference
+        voronoiModelBase::enforceTopology); step counters do not advance."""
+        check(lib().cgpu_vor_enforce_topology(self.h), "enforce_topology")
+
     def set_directors(self, theta):
         t = _f64(theta, self.N)
         check(lib().cgpu_vor_set_directors(self.h, _p(t)), "set_directors")

@@ -124,6 +124,7 @@ def lib():
         "cgpu_vor_get_active": (i, [vp, vp, vp]),
         "cgpu_vor_device_bytes": (C.c_longlong, [vp]),
         "cgpu_vor_merge": (vp, [i, vp, vp]),
+        "cgpu_vor_get_external_forces": (i, [vp, vp, vp]),
         "cgpu_vor_autotune": (i, [vp, i, P(LaunchShape), P(i)]),
         "cgpu_vor_set_launch_shape": (i, [vp, P(LaunchShape)]),
         "cgpu_vor_get_launch_shape": (i, [vp, P(LaunchShape)]),

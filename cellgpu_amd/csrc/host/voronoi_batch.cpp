@@ -153,6 +153,9 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
         real_upload(vel_.p, vv.data(), 2 * (size_t)N_);
     }
     int r = global_triangulation();
+    if (r == CGPU_ERR_RING_CAPACITY)
+        throw std::runtime_error("initial triangulation failed: a Delaunay degree exceeds the ring capacity (" +
+                                 std::to_string(CGPU_RING_CAP) + ")");
     if (r != CGPU_OK) throw std::runtime_error("initial triangulation failed, status " + std::to_string(r));
 }
 
@@ -1280,6 +1283,20 @@ void *cgpu_vor_create(int num_systems, const int *n, const double *L, const uint
         g_last_error = e.what();
         return nullptr;
     }
+}
+int cgpu_vor_get_external_forces(void *h, double *fx, double *fy) {
+    return guarded([&] {
+        VoronoiBatch &b = *(VoronoiBatch *)h;
+        const int N = b.num_cells();
+        std::vector<double> f(2 * (size_t)N, 0.0);
+        b.sync();
+        if (b.ext_force_device()) b.real_to_host(b.ext_force_device(), f.data(), f.size());
+        for (int i = 0; i < N; ++i) {
+            fx[i] = f[2 * i];
+            fy[i] = f[2 * i + 1];
+        }
+        return 0;
+    });
 }
 void *cgpu_vor_merge(int count, void *const *batches, const int *systems) {
     try {

@@ -1925,7 +1925,8 @@ __global__ void k_finalize(cgpu_voronoi_batch b, const double *pos, int advance)
             const int mx = st.max_deg;
             st.neigh_max = (mx % 2 == 0) ? mx + 2 : mx + 1;
             st.global_fixes += 1;
-            if (st.nn_delta != 6 * b.sys[s].n) st.error = CGPU_ERR_TOPOLOGY;
+            // the first error stands (a capacity overflow also breaks the sum)
+            if (st.nn_delta != 6 * b.sys[s].n && st.error == 0) st.error = CGPU_ERR_TOPOLOGY;
         }
         if (advance) advance_system(st);
     }
@@ -1965,7 +1966,10 @@ __global__ void k_delsets_all(cgpu_voronoi_batch b) {
         const int z = b.nn[i];
         if (z < 3 || z > CAP) continue;
         for (int k = k0; k < z; k += QL)
-            if (!del_set_slot(b, i, z, k)) b.st[b.cell_sys[i]].error = CGPU_ERR_TOPOLOGY;
+            if (!del_set_slot(b, i, z, k)) {
+                int &err = b.st[b.cell_sys[i]].error;
+                if (err == 0) err = CGPU_ERR_TOPOLOGY;
+            }
     }
 }
 

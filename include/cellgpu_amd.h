@@ -405,6 +405,10 @@ int cgpu_vor_set_params(void *h, int s, double KA, double KP, double dt, double 
 int cgpu_vor_set_prefs(void *h, const double *A0, const double *P0);       /* per cell, all systems */
 int cgpu_vor_set_motility(void *h, const double *v0, const double *Dr);    /* per cell; resets v   */
 int cgpu_vor_set_exclusions(void *h, const int *ex);
+/* External force per cell (reference VoronoiQuadraticEnergy::external_forces,
+ * voronoiQuadraticEnergy.cu:59-93): minus the summed force sets of an excluded
+ * cell, zero elsewhere (all zero before any exclusions are set). */
+int cgpu_vor_get_external_forces(void *h, double *fx, double *fy);
 /* Line tension between cell types (reference VoronoiQuadraticEnergyWithTension,
  * voronoiQuadraticEnergyWithTension.h:16-60; Simple2DCell::setCellType):
  * cell types per cell; mode for system s (-1: all) 0 off, 1 one value gamma
