@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic (compulsory) bytes per cell for one launch, FP32 bulk, FP64 positions,
 # mean Delaunay degree 6 (Euler): neighbour gathers are served on chip.
 BYTES_FORCE_EOM_MOVE = 16 + 4 + 24 + 24 + 8 + 8 + 8 + 4 + 4 + 16 + 8 + 8 + 4  # = 136
-BYTES_TEST_GEOMETRY = 16 + 4 + 24 + 4 + 8 + 8 + 8  # = 72
+BYTES_TEST_GEOMETRY = 16 + 4 + 24 + 4 + 8 + 8 + 8 + 4  # = 76 (the last 4: cell_rmax written)
 BYTES_CELL_LIST = 16 + 4 + 4 + 8  # = 32
 
 
@@ -409,6 +409,15 @@ KERNEL_BYTES = {
     "k_bin_scan": ("bins", 12),           # count read + clear, start write
     "k_bin_scatter": ("cells", 52),       # bin 4, rank r/w 8, start 4, pos 16; id 4 + pos 16 written
     "k_test_geometry": ("cells", BYTES_TEST_GEOMETRY),
+    # one record (3 x 8) and cell_sys (4) per 64 cells; the stamps of the regions
+    "k_ld_reduce": ("cells", 28 / 64),
+    "k_ld_rule": ("systems", 48),         # params 16, sums 24 + skip flag 4 read, rule 4 written
+    # cell_sys 4 + cell_bin 4 + stamp 4 + cell_rmax 4 (system state and rule stay in
+    # cache); cells near a stamp read their rings and queue their triangles
+    "k_select_triangles": ("cells", 16),
+    # the triangle 12 and its corners' slots 12 and positions 48; the window's bin runs
+    # and candidate positions are served on chip
+    "k_test_queue": ("queued", 72),
     "k_mark_nf": ("flagged", 88),         # queue 4, nn 4, ring 24, 7 marks r/w + NeedsFixing entries
     "k_star_local_wave<1>": ("needs_fixing", 72),  # pos 16, nn 4, ring 24 read; ring 24 + nn 4 written
     "k_star_local_wave<2>": ("needs_fixing", 0),
@@ -425,8 +434,8 @@ def kernel_roofline(cg, b, N, bins, steps):
     """Per-kernel durations of `steps` profiled steps (a HIP event after every
     launch on the batch stream) and each kernel's algorithmic bytes / duration."""
     ms, q = b.profiled_steps(steps)
-    units = {"cells": N, "bins": bins, "flagged": q[0] / steps, "needs_fixing": q[1] / steps,
-             "global": q[2] / steps, "repaired": (q[1] + q[2]) / steps}
+    units = {"cells": N, "bins": bins, "systems": len(b.L), "flagged": q[0] / steps, "needs_fixing": q[1] / steps,
+             "global": q[2] / steps, "repaired": (q[1] + q[2]) / steps, "queued": q[3] / steps}
     table = {}
     for k, tot in ms.items():
         unit, per = KERNEL_BYTES[k]
@@ -434,7 +443,8 @@ def kernel_roofline(cg, b, N, bins, steps):
         nbytes = per * units[unit]
         table[k] = {"avg_ms": round(avg, 5), "unit": unit, "units_per_launch": round(units[unit], 1),
                     "bytes_per_unit": per, "achieved_gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
-    return table, {"flagged": q[0] / steps, "needs_fixing": q[1] / steps, "global": q[2] / steps}
+    return table, {"flagged": q[0] / steps, "needs_fixing": q[1] / steps, "global": q[2] / steps,
+                   "test_queue": q[3] / steps, "stamp_regions": q[4] / steps}
 
 
 def main():

@@ -270,10 +270,11 @@ class VoronoiBatch:
 
     def profiled_steps(self, steps):
         """Steps with a HIP event after every kernel. Returns ({kernel: summed
-        ms}, summed queue sizes (flagged, NeedsFixing, global))."""
+        ms}, summed queue sizes (flagged, NeedsFixing, global, triangles left to
+        the circumcircle test, regions stamped))."""
         from ._lib import PROFILE_SLOTS
         ms = np.zeros(PROFILE_SLOTS)
-        q = np.zeros(3, np.int64)
+        q = np.zeros(5, np.int64)
         check(lib().cgpu_vor_spv_steps_profiled(self.h, int(steps), _p(ms), _p(q)), "profiled steps")
         names = [lib().cgpu_profile_slot_name(k).decode() for k in range(PROFILE_SLOTS)]
         return dict(zip(names, ms.tolist())), q.tolist()

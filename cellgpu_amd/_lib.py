@@ -33,7 +33,9 @@ class SysState(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "any_fail", "nf_count", "mode", "need_global", "nn_delta", "tri_delta", "max_deg", "neigh_max", "step",
         "active", "steps_left", "error", "nflag", "pad_s")] + [
-        (n, C.c_longlong) for n in ("global_fixes", "repair_events", "local_repairs", "t1_events")]
+        (n, C.c_longlong) for n in ("global_fixes", "repair_events", "local_repairs", "t1_events")] + [
+        ("tri_hash_all", C.c_ulonglong), ("tri_hash_own", C.c_ulonglong), ("tri_area", C.c_longlong),
+        ("skip_off", C.c_int), ("pad_k", C.c_int)]
 
 
 class VorWorkspace(C.Structure):
@@ -60,7 +62,7 @@ class VertexBatchView(C.Structure):
             "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("pad_c", C.c_int)]
 
 
-PROFILE_SLOTS = 13  # CGPU_PROFILE_SLOTS
+PROFILE_SLOTS = 17  # CGPU_PROFILE_SLOTS
 
 
 class LaunchShape(C.Structure):
@@ -80,6 +82,9 @@ class VoronoiBatchView(C.Structure):
             "counters", "noise", "chunk_sys", "chunk_begin", "chunk_end", "bchunk_sys", "bchunk_begin",
             "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)] + [
         ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("pad_t", C.c_int),
+        ("ld_tri", C.c_void_p), ("bin_mark_wide", C.c_void_p), ("cell_rmax", C.c_void_p),
+        ("ld_regions", C.c_void_p), ("ld_region_cap", C.c_int), ("ld_tri_cap", C.c_int),
+        ("ld_part", C.c_void_p), ("ld_rule", C.c_void_p),
         ("shape", LaunchShape)]
 
 

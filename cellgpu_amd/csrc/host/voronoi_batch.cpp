@@ -102,7 +102,7 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     sys_.alloc(S_);
     st_.alloc(S_);
-    counters_.alloc(8);
+    counters_.alloc(CGPU_COUNTERS);
     layout(ns);
     tag_.resize(N_);
     for (const auto &p : hp_)
@@ -168,7 +168,7 @@ VoronoiBatch::VoronoiBatch(Empty, int num_systems, int precision, int device)
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     sys_.alloc(S_);
     st_.alloc(S_);
-    counters_.alloc(8);
+    counters_.alloc(CGPU_COUNTERS);
 }
 
 VoronoiBatch *VoronoiBatch::merge(int count, VoronoiBatch *const *src, const int *sys) {
@@ -332,6 +332,14 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
     gq_.alloc(N_);
     bin_count_.alloc(total_bins_);
     bin_start_.alloc((size_t)total_bins_ + 1);
+    ld_tri_cap_ = ld_tri_capacity(N_);
+    ld_tri_.alloc(3 * (size_t)ld_tri_cap_);
+    bin_mark_wide_.alloc(total_bins_);
+    cell_rmax_.alloc(N_);
+    ld_region_cap_ = ld_region_capacity(N_);
+    ld_regions_.alloc((size_t)ld_region_cap_ * CGPU_LD_REGION_BYTES);
+    ld_part_.alloc(3 * ((size_t)N_ / 64 + 1));
+    ld_rule_.alloc(S_);
     cell_bin_.alloc(N_);
     cell_rank_.alloc(N_);
     sorted_id_.alloc(N_);
@@ -348,7 +356,9 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
     device_bytes_ = sys_.n * sizeof(cgpu_sys_params) + st_.n * sizeof(cgpu_sys_state) +
                     sizeof(int) * (cell_sys_.n + nn_.n + nbr_.n + oth_.n + repair_.n + mark_.n + flagged_.n +
                                    nfq_.n + gq_.n + counters_.n + bin_start_.n + cell_bin_.n + cell_rank_.n +
-                                   sorted_id_.n + bin_count_.n) +
+                                   sorted_id_.n + bin_count_.n + ld_tri_.n + bin_mark_wide_.n) +
+                    sizeof(float) * cell_rmax_.n + ld_regions_.n + sizeof(unsigned long long) * ld_part_.n +
+                    sizeof(int) * ld_rule_.n +
                     sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
                     (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
     cell_sys_.upload(cell_sys_h_.data(), N_);
@@ -428,6 +438,14 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.cell_type = tension_on_ ? cell_type_.p : nullptr;
     b.tension = tension_.p;
     b.num_types = ntypes_;
+    b.ld_tri = ld_tri_.p;
+    b.ld_tri_cap = ld_tri_cap_;
+    b.bin_mark_wide = bin_mark_wide_.p;
+    b.cell_rmax = cell_rmax_.p;
+    b.ld_regions = ld_regions_.p;
+    b.ld_region_cap = ld_region_cap_;
+    b.ld_part = ld_part_.p;
+    b.ld_rule = ld_rule_.p;
     b.shape = shape_;
     return b;
 }
@@ -1066,10 +1084,10 @@ int VoronoiBatch::spv_steps_profiled(int steps, double *ms_slots, long long *que
     hipEvent_t ev[NS + 1];
     for (auto &e : ev) hip_check(hipEventCreate(&e), "event");
     for (int q = 0; q < NS; ++q) ms_slots[q] = 0.0;
-    for (int q = 0; q < 3; ++q) queue_sums[q] = 0;
+    for (int q = 0; q < 5; ++q) queue_sums[q] = 0;
     drop_graph();
     int r = CGPU_OK;
-    std::vector<int> c(8);
+    std::vector<int> c(CGPU_COUNTERS);
     for (int k = 0; k < steps && r == CGPU_OK; ++k) {
         cgpu_voronoi_batch b = view();
         r = cgpu_profiled_spv_step(&b, stream_, ev);
@@ -1080,11 +1098,14 @@ int VoronoiBatch::spv_steps_profiled(int steps, double *ms_slots, long long *que
             hip_check(hipEventElapsedTime(&ms, ev[q], ev[q + 1]), "event time");
             ms_slots[q] += ms;
         }
-        // queue sizes of this step: flagged cells, NeedsFixing, global queue
-        counters_.download(c.data(), 8);
+        // queue sizes of this step: flagged cells, NeedsFixing, global queue,
+        // triangles left to the circumcircle test, regions stamped
+        counters_.download(c.data(), CGPU_COUNTERS);
         queue_sums[0] += c[0];
         queue_sums[1] += c[1];
         queue_sums[2] += c[2];
+        queue_sums[3] += c[9];
+        queue_sums[4] += c[8];
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return r;
@@ -1445,7 +1466,8 @@ int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long 
 }
 const char *cgpu_profile_slot_name(int slot) {
     static const char *names[CGPU_PROFILE_SLOTS] = {
-        "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_mark_nf", "k_star_local_wave<1>",
+        "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_ld_reduce", "k_ld_rule",
+        "k_select_triangles", "k_test_queue", "k_mark_nf", "k_star_local_wave<1>",
         "k_star_local_wave<2>", "k_delsets_local", "k_global_fallback", "k_star_global_wave<1>",
         "k_star_global_wave<2>", "k_global_stats", "k_finalize"};
     return (slot >= 0 && slot < CGPU_PROFILE_SLOTS) ? names[slot] : "";

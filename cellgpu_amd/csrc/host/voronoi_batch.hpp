@@ -73,6 +73,10 @@ struct BatchTables {
     std::vector<int> bs, bb, be, sb;   // bin chunks: system, begin, end; first bin chunk per system [S+1]
 };
 BatchTables batch_tables(std::vector<cgpu_sys_params> &hp);
+// capacity (records) of the circumcircle test's region list for N cells
+inline int ld_region_capacity(int N) { return N / 8 > 1024 ? N / 8 : 1024; }
+// capacity (triangles) of the circumcircle test's triangle queue for N cells
+inline int ld_tri_capacity(int N) { return 2 * N + 1024; }
 
 class VoronoiBatch {
    public:
@@ -192,7 +196,12 @@ class VoronoiBatch {
     DevBuf<cgpu_sys_params> sys_;
     DevBuf<cgpu_sys_state> st_;
     DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_;
-    DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_;
+    DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_, ld_tri_, bin_mark_wide_;
+    DevBuf<float> cell_rmax_;
+    DevBuf<unsigned char> ld_regions_;
+    DevBuf<unsigned long long> ld_part_;
+    DevBuf<int> ld_rule_;
+    int ld_region_cap_ = 0, ld_tri_cap_ = 0;
     DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_, bchunk_sys_, bchunk_begin_, bchunk_end_, sys_bchunk_,
         bchunk_sum_;
     int num_chunks_ = 0, num_bin_chunks_ = 0;

@@ -55,7 +55,7 @@ void carve(Carver &c, const cgpu_vor_workspace &ws, cgpu_voronoi_batch &b) {
     b.flagged = c.take<int>(N);
     b.nfq = c.take<int>(N);
     b.gq = c.take<int>(N);
-    b.counters = c.take<int>(8);
+    b.counters = c.take<int>(CGPU_COUNTERS);
     b.chunk_sys = c.take<int>(ws.num_chunks);
     b.chunk_begin = c.take<int>(ws.num_chunks);
     b.chunk_end = c.take<int>(ws.num_chunks);
@@ -64,6 +64,14 @@ void carve(Carver &c, const cgpu_vor_workspace &ws, cgpu_voronoi_batch &b) {
     b.bchunk_end = c.take<int>(ws.num_bin_chunks);
     b.sys_bchunk = c.take<int>(S + 1);
     b.bchunk_sum = c.take<int>(ws.num_bin_chunks);
+    b.ld_tri = c.take<int>(3 * (size_t)ld_tri_capacity((int)N));
+    b.ld_tri_cap = ld_tri_capacity((int)N);
+    b.bin_mark_wide = c.take<int>(ws.total_bins);
+    b.cell_rmax = c.take<float>(N);
+    b.ld_regions = c.take<char>((size_t)ld_region_capacity((int)N) * CGPU_LD_REGION_BYTES);
+    b.ld_region_cap = ld_region_capacity((int)N);
+    b.ld_part = c.take<unsigned long long>(3 * (N / 64 + 1));
+    b.ld_rule = c.take<int>(S);
 }
 
 template <typename F> int ws_guard(F &&f) {

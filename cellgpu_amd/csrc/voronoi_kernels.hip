@@ -62,7 +62,11 @@ enum ProbeSlot {
     PROBE_FORCE_EOM_MOVE = 0,  // force, EOM, move, reset, binning pass
     PROBE_BIN_SCAN,            // cell-list prefix sum (one or three kernels)
     PROBE_BIN_SCATTER,
-    PROBE_TEST_GEOMETRY,
+    PROBE_TEST_GEOMETRY,       // geometry and the skip rule's checks
+    PROBE_LD_REDUCE,           // the rule's per-system sums and the stamps
+    PROBE_LD_RULE,             // the rule of each system
+    PROBE_SELECT_TRIANGLES,    // the triangles the rule leaves, queued
+    PROBE_TEST_QUEUE,          // their circumcircle test
     PROBE_MARK_NF,
     PROBE_STAR_LOCAL,
     PROBE_STAR_LOCAL_WIDE,
@@ -212,36 +216,158 @@ template <typename R> struct BlockTile {
     }
 };
 
+// ------------------------------------------------- skip rule of the circumcircle test
+// If every edge of the triangulation that meets a triangle's circumdisk is
+// locally Delaunay (the far vertex of its other triangle lies outside the
+// circle), no point lies inside the circle: the segment from the triangle to
+// such a point stays in the disk and crosses an edge that is not locally
+// Delaunay (the classical walk argument). The reference test then flags nothing
+// for that triangle. The geometry pass (cell_geometry<R, true>) checks every
+// spoke of its cell and stamps the bins around each spoke that fails (with a
+// margin far above rounding) and around each ring that is not a proper
+// counter-clockwise fan; the reference scan then runs only for the triangles
+// of cells near a stamp (k_select_triangles, k_test_queue). Per system it also sums a hash of the
+// ring triangles over all slots and over the owned slots, and their area: the
+// rule holds only where the rings are consistent (every triangle in the rings
+// of its three cells, all = 3 x owned) and cover the box once (area L^2).
+constexpr double LD_MARGIN = 1e-9;  // power of the far vertex, squared cell spacings
+constexpr int MARK_DILATE = 3;      // bins of bin_mark_wide around a stamped region
+__device__ inline int quadrant(dvec2 d);
+
+struct LdSums {
+    unsigned long long all, own;
+    long long area;  // units of 2^-30 (integer sums: exact, native atomics)
+};
+constexpr double AREA_UNIT = 1073741824.0;  // 2^30
+__device__ inline unsigned long long mix64(unsigned long long x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+// hash of the counter-clockwise triangle (a, b, c) rotated to start at its
+// smallest index (the same triangle from each of its cells)
+__device__ inline unsigned long long tri_hash(int a, int b, int c) {
+    if (b < a && b < c) {
+        const int t = a;
+        a = b;
+        b = c;
+        c = t;
+    } else if (c < a && c < b) {
+        const int t = c;
+        c = b;
+        b = a;
+        a = t;
+    }
+    return mix64(mix64(mix64(0x9e3779b97f4a7c15ull + (unsigned)a) ^ (unsigned)b) ^ (unsigned)c);
+}
+// A region for k_ld_reduce to stamp: the rectangle [x0, x1] x [y0, y1] of system
+// sys, or (sys = -1 - i) the bounding box of cell i's star. The geometry pass
+// only appends records (no loops, no calls); a record that does not fit turns
+// the rule off for its system.
+struct LdRegion {
+    int sys, pad;
+    double x0, x1, y0, y1;
+};
+static_assert(sizeof(LdRegion) == CGPU_LD_REGION_BYTES, "region record size of the C ABI");
+__device__ inline void push_region(const cgpu_voronoi_batch &b, int s, int sys, double x0, double x1, double y0,
+                                   double y1) {
+    const int slot = atomicAdd(&b.counters[8], 1);
+    if (slot >= b.ld_region_cap) {
+        b.st[s].skip_off = 1;
+        return;
+    }
+    reinterpret_cast<LdRegion *>(b.ld_regions)[slot] = LdRegion{sys, 0, x0, x1, y0, y1};
+}
+// spoke (i, q) of the triangle whose circle is centred at c (relative to i):
+// a region unless the far vertex f lies outside the circle by LD_MARGIN (a
+// circle wider than 4 bins counts too: its power is not resolved there)
+__device__ inline void check_spoke(const cgpu_voronoi_batch &b, const cgpu_sys_params &sp, int s, dvec2 pi,
+                                   vec2<double> c, dvec2 q, dvec2 f) {
+    const double pw = f.x * (f.x - 2.0 * c.x) + f.y * (f.y - 2.0 * c.y);
+    const double r2 = c.x * c.x + c.y * c.y;
+    if (pw > LD_MARGIN && r2 < 16.0 * sp.boxsize * sp.boxsize) return;
+    push_region(b, s, s, pi.x + fmin(0.0, q.x), pi.x + fmax(0.0, q.x), pi.y + fmin(0.0, q.y), pi.y + fmax(0.0, q.y));
+}
+
+// the skip rule's per-cell state: fan order and winding, largest squared
+// circumradius
+struct LdFan {
+    bool bad;
+    int wraps;
+    float rmax2;
+};
+// one ring pair (a, c) = (ring k-1, ring k) with indices ia, ic and circumcenter v
+__device__ __forceinline__ void ld_pair(LdFan &f, LdSums &sums, int i, dvec2 a, dvec2 c, int ia, int ic,
+                                        vec2<double> v) {
+    // a pair not clearly counter-clockwise (FP64, with a margin) counts as a defect
+    if (!(a.x * c.y - a.y * c.x > 1e-12 * (a.x * a.x + a.y * a.y + c.x * c.x + c.y * c.y))) f.bad = true;
+    f.wraps += quadrant(c) < quadrant(a);
+    f.rmax2 = fmaxf(f.rmax2, (float)(v.x * v.x + v.y * v.y));
+    const unsigned long long h = tri_hash(i, ia, ic);
+    sums.all += h;
+    if (i < ia && i < ic) {
+        sums.own += h;
+        sums.area += llrint(0.5 * AREA_UNIT * (a.x * c.y - a.y * c.x));
+    }
+}
+
 // ------------------------------------------------------------------ geometry
-// Voronoi area and perimeter of cell i in the reference GPU order
-// (voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k).
-// Evaluated in FP64 in both precision modes and rounded once at the store: the
+// Voronoi geometry of cell i (reference gpu_compute_voronoi_geometry_kernel,
+// voronoiModelBase.cu:115-201): vertex k = circumcenter(ring k-1, ring k), the
+// circumcenters of the cell and consecutive ring pairs. Area and perimeter are
+// evaluated in FP64 in both precision modes and rounded once at the store: the
 // force uses KA (A - A0) and KP (P - P0), which in relaxed tissue are a few
 // percent of A and P, so FP32 vertices would leave them with about 1e-5
-// relative error (the dominant FP32 force error).
-template <typename R>
+// relative error (the dominant FP32 force error). With LD also the skip rule's
+// checks of the cell's spokes and fan, its largest circumradius (cell_rmax)
+// and its triangles' hash and area sums.
+template <typename R, bool LD = false>
 __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const double *pos, int i,
-                                              const cgpu_sys_params &sp) {
+                                              const cgpu_sys_params &sp, LdSums *sums = nullptr) {
     using G = double;
     const Box box = make_box(sp.L);
     const int N = b.num_cells;
     const int z = b.nn[i];
-    if (z < 3) return;
+    if (z < 3) {
+        if (LD) b.st[b.cell_sys[i]].skip_off = 1;
+        return;
+    }
+    const int s = LD ? b.cell_sys[i] : 0;
     dvec2 pi = ldpos(pos, i);
-    dvec2 rij = min_dist(box, ldpos(pos, b.nbr[(z - 1) * N + i]), pi);
-    dvec2 rik = min_dist(box, ldpos(pos, b.nbr[i]), pi);
+    const int jlast = b.nbr[(z - 1) * N + i], jfirst = b.nbr[i];
+    dvec2 rij = min_dist(box, ldpos(pos, jlast), pi);
+    dvec2 rik = min_dist(box, ldpos(pos, jfirst), pi);
+    const dvec2 r0 = rik;
     vec2<G> vfirst = Prec<G>::c0(rij, rik), vlast = vfirst;
     G A = G(0), P = G(0);
+    // skip-rule state: fan order, winding, circumradius, star bounding box
+    LdFan fan{};
+    int idprev = jfirst;
+    if (LD) ld_pair(fan, *sums, i, rij, rik, jlast, jfirst, vfirst);
     // ring positions are fetched one slot ahead of their use, ring indices two
-    dvec2 pnext = ldpos(pos, b.nbr[(1 % z) * N + i]);
+    int idnext = b.nbr[(1 % z) * N + i];
+    dvec2 pnext = ldpos(pos, idnext);
     int jnext = b.nbr[(2 % z) * N + i];
     for (int k = 1; k < z; ++k) {
         const dvec2 pk = pnext;
-        if (k + 1 < z) pnext = ldpos(pos, jnext);
+        const int idk = idnext;
+        if (k + 1 < z) {
+            idnext = jnext;
+            pnext = ldpos(pos, jnext);
+        }
         if (k + 2 < z) jnext = b.nbr[(k + 2) * N + i];
         rij = rik;
         rik = min_dist(box, pk, pi);
+        // spoke (i, ring k-1): far vertex ring k, circle of (i, ring k-2, ring k-1)
+        if (LD) check_spoke(b, sp, s, pi, vlast, rij, rik);
         vec2<G> vnext = Prec<G>::c0(rij, rik);
+        if (LD) {
+            ld_pair(fan, *sums, i, rij, rik, idprev, idk, vnext);
+            idprev = idk;
+        }
         A += tri_area<G>(vlast, vnext);
         G dx = vlast.x - vnext.x, dy = vlast.y - vnext.y;
         P += sqrt(dx * dx + dy * dy);
@@ -253,6 +379,12 @@ __device__ __forceinline__ void cell_geometry(const cgpu_voronoi_batch &b, const
     st2<R>(b.AP, i, vec2<R>{(R)A, (R)P});
     const vec2<R> pref = ld2<R>(b.APpref, i);
     st2<R>(b.dAP, i, vec2<R>{(R)(sp.KA * (A - (G)pref.x)), (R)(sp.KP * (P - (G)pref.y))});
+    if (LD) {
+        // spoke (i, ring z-1): far vertex ring 0
+        check_spoke(b, sp, s, pi, vlast, rik, r0);
+        if (fan.bad || fan.wraps != 1) push_region(b, s, -1 - i, 0.0, 0.0, 0.0, 0.0);
+        b.cell_rmax[i] = sqrtf(fan.rmax2) * 1.0001f;
+    }
 }
 
 // -------------------------------------------------------------------- forces
@@ -1107,6 +1239,9 @@ __device__ inline void reset_system(const cgpu_voronoi_batch &b, int s) {
         b.counters[1] = 0;
         b.counters[2] = 0;
         b.counters[6] = 0;
+        b.counters[7] += 1;  // the circumcircle test's epoch (bin stamps)
+        b.counters[8] = 0;   // its region records
+        b.counters[9] = 0;   // its triangle queue
     }
     if (s < b.num_systems) {
         cgpu_sys_state &st = b.st[s];
@@ -1118,6 +1253,10 @@ __device__ inline void reset_system(const cgpu_voronoi_batch &b, int s) {
         st.tri_delta = 0;
         st.max_deg = 0;
         st.nflag = 0;
+        st.tri_hash_all = 0ull;
+        st.tri_hash_own = 0ull;
+        st.tri_area = 0ll;
+        st.skip_off = 0;
     }
 }
 __global__ void k_reset(cgpu_voronoi_batch b) { reset_system(b, blockIdx.x * blockDim.x + threadIdx.x); }
@@ -1527,36 +1666,195 @@ __device__ __forceinline__ void wave_test_wide(const cgpu_voronoi_batch &b, cons
     if (bad) flag_triangle(b, s, i, j, l);
 }
 
-// Geometry of the block's cells, then the test of the triangles they own (i
-// the smallest index of the three: the triangles the reference tests,
-// voronoiModelBase.cpp:380-423). The owned triangles are gathered in LDS and
-// dealt out over the block's threads, so a cell that owns many triangles does
-// not hold up its wavefront; triangles with a window wider than 3 x 3 bins go
-// to a second LDS list that the block's wavefronts scan cooperatively. A
-// triangle beyond the LDS list (more than five per cell of the block) is tested
-// serially by the thread of its cell. The lists live in dynamic LDS sized for
-// TRI_PER_CELL entries per thread (the block size is the launch shape's
-// test_block, at most 512).
-constexpr int TRI_PER_CELL = 5;
+// sum of three 64-bit values over the wavefront (all lanes call)
+__device__ inline void wave_sum3(unsigned long long &a, unsigned long long &o, unsigned long long &r) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += ((unsigned long long)__shfl_xor((unsigned int)(a >> 32), off) << 32) | __shfl_xor((unsigned int)a, off);
+        o += ((unsigned long long)__shfl_xor((unsigned int)(o >> 32), off) << 32) | __shfl_xor((unsigned int)o, off);
+        r += ((unsigned long long)__shfl_xor((unsigned int)(r >> 32), off) << 32) | __shfl_xor((unsigned int)r, off);
+    }
+}
+// add each system's part of the wavefront's (a, o, r) to its sums with one set
+// of atomics per system; lanes hold cells of consecutive systems (all lanes call)
+__device__ inline void wave_add_by_system(const cgpu_voronoi_batch &b, bool live, int s, unsigned long long a,
+                                          unsigned long long o, unsigned long long r) {
+    unsigned long long todo = __ballot(live);
+    while (todo != 0ull) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int sx = __shfl(s, leader);
+        const bool mine = live && s == sx;
+        unsigned long long x = mine ? a : 0ull, y = mine ? o : 0ull, z = mine ? r : 0ull;
+        wave_sum3(x, y, z);
+        if ((int)__lane_id() == leader) {
+            atomicAdd(&b.st[sx].tri_hash_all, x);
+            atomicAdd(&b.st[sx].tri_hash_own, y);
+            atomicAdd((unsigned long long *)&b.st[sx].tri_area, z);
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+// the skip rule's per-cell sums of a wavefront. The wavefront covers the 64
+// cells from a multiple of 64 (i of lane 0). When they are 64 live cells of
+// one system its sums go to that group's record in ld_part, which k_ld_reduce
+// adds up: no contention on the system's sums. A wavefront at a system
+// boundary adds each system's part with atomics. All lanes call.
+__device__ inline void add_ld_sums(const cgpu_voronoi_batch &b, int i, bool live, int s, const LdSums &v) {
+    const unsigned long long m = __ballot(live);
+    if (m == 0ull) return;
+    const int s0 = __shfl(s, __ffsll((long long)m) - 1);
+    unsigned long long a = live ? v.all : 0ull, o = live ? v.own : 0ull;
+    unsigned long long r = live ? (unsigned long long)v.area : 0ull;
+    if (m == ~0ull && __ballot(s != s0) == 0ull) {
+        wave_sum3(a, o, r);
+        if (__lane_id() == 0) {
+            unsigned long long *g = b.ld_part + 3 * (size_t)(i >> 6);
+            g[0] = a;
+            g[1] = o;
+            g[2] = r;
+        }
+        return;
+    }
+    wave_add_by_system(b, live, s, a, o, r);
+}
+
+// Geometry of every cell. With test set also the skip rule's checks (regions
+// to stamp, circumradii, per-system sums) for k_select_triangles.
 template <typename R>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_test_geometry(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_test_geometry(
     cgpu_voronoi_batch b, const double *pos, int test) {
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    const bool live = i < b.num_cells && b.st[b.cell_sys[i]].active;
+    if (!test) {  // uniform
+        if (live) cell_geometry<R>(b, pos, i, b.sys[b.cell_sys[i]]);
+        return;
+    }
+    LdSums sums{0ull, 0ull, 0ll};
+    int s = 0;
+    if (live) {
+        s = b.cell_sys[i];
+        cell_geometry<R, true>(b, pos, i, b.sys[s], &sums);
+    }
+    add_ld_sums(b, i, live, s, sums);
+}
+
+// whether the skip rule holds for system s in this test: no region too wide to
+// stamp, a grid wide enough for the dilation, consistent rings, one covering
+__device__ inline bool skip_rule_on(const cgpu_voronoi_batch &b, int s, const cgpu_sys_params &sp) {
+    const cgpu_sys_state &st = b.st[s];
+    const double L2 = sp.L * sp.L;
+    return !st.skip_off && sp.xsize >= 2 * MARK_DILATE + 2 && st.tri_hash_all == 3ull * st.tri_hash_own &&
+           fabs((double)st.tri_area / AREA_UNIT - L2) <= 1e-6 * L2;
+}
+
+// After the geometry pass: (1) the records of the 64-cell groups that lie
+// inside one active system are added to that system's sums (one thread per
+// group, one set of atomics per system and wavefront); (2) the recorded regions
+// are stamped, one wavefront per region with a lane per bin (a star's bounding
+// box from its ring first).
+__global__ void k_ld_reduce(cgpu_voronoi_batch b, const double *pos) {
+    const int N = b.num_cells, G = N >> 6;
+    for (int base = blockIdx.x * blockDim.x; base < G; base += gridDim.x * blockDim.x) {  // uniform per block
+        const int g = base + threadIdx.x;
+        bool full = false;
+        int s = 0;
+        if (g < G) {
+            const int c0 = g << 6;
+            s = b.cell_sys[c0];
+            const cgpu_sys_params &sp = b.sys[s];
+            full = c0 >= sp.off && c0 + 64 <= sp.off + sp.n && b.st[s].active;
+        }
+        const unsigned long long *r = b.ld_part + 3 * (size_t)g;
+        wave_add_by_system(b, full, s, full ? r[0] : 0ull, full ? r[1] : 0ull, full ? r[2] : 0ull);
+    }
+    const int n = min(b.counters[8], b.ld_region_cap);
+    const int epoch = b.counters[7];
+    const LdRegion *reg = reinterpret_cast<const LdRegion *>(b.ld_regions);
+    const int lane = __lane_id();
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < n; q += nw) {  // uniform per wavefront
+        const LdRegion rg = reg[q];
+        int sys = rg.sys;
+        double x0 = rg.x0, x1 = rg.x1, y0 = rg.y0, y1 = rg.y1;
+        if (sys < 0) {  // star of cell i: bounding box of its ring (relative), then absolute
+            const int i = -1 - sys;
+            sys = b.cell_sys[i];
+            const cgpu_sys_params &sp = b.sys[sys];
+            const Box box = make_box(sp.L);
+            const int z = min(b.nn[i], CAP);
+            const dvec2 pi = ldpos(pos, i);
+            dvec2 d{0.0, 0.0};
+            if (lane < z) d = min_dist(box, ldpos(pos, b.nbr[lane * N + i]), pi);
+            double a0 = fmin(d.x, 0.0), a1 = fmax(d.x, 0.0), c0 = fmin(d.y, 0.0), c1 = fmax(d.y, 0.0);
+            for (int off = 32; off > 0; off >>= 1) {
+                a0 = fmin(a0, __shfl_xor(a0, off));
+                a1 = fmax(a1, __shfl_xor(a1, off));
+                c0 = fmin(c0, __shfl_xor(c0, off));
+                c1 = fmax(c1, __shfl_xor(c1, off));
+            }
+            x0 = pi.x + a0;
+            x1 = pi.x + a1;
+            y0 = pi.y + c0;
+            y1 = pi.y + c1;
+        }
+        const cgpu_sys_params &sp = b.sys[sys];
+        const int xs = sp.xsize;
+        if (!(x1 - x0 < 0.25 * sp.L && y1 - y0 < 0.25 * sp.L)) {
+            if (lane == 0) b.st[sys].skip_off = 1;
+            continue;
+        }
+        const double inv = 1.0 / sp.boxsize;
+        const int ia = (int)floor(x0 * inv), ib = (int)floor(x1 * inv);
+        const int ja = (int)floor(y0 * inv), jb = (int)floor(y1 * inv);
+        const int nu = ib - ia + 2 * MARK_DILATE + 1, nv = jb - ja + 2 * MARK_DILATE + 1;
+        if (!(nu < xs && nv < xs)) {
+            if (lane == 0) b.st[sys].skip_off = 1;
+            continue;
+        }
+        for (int t = lane; t < nu * nv; t += 64) {
+            const int u = ia - MARK_DILATE + t % nu, w = ja - MARK_DILATE + t / nu;
+            const int cu = ((u % xs) + xs) % xs, cw = ((w % xs) + xs) % xs;
+            const int bin = sp.bin_off + cw * xs + cu;
+            b.bin_mark_wide[bin] = epoch;
+        }
+    }
+}
+
+// the rule of every system for this test, once its sums are complete
+__global__ void k_ld_rule(cgpu_voronoi_batch b) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
+        b.ld_rule[s] = skip_rule_on(b, s, b.sys[s]) ? 1 : 0;
+}
+
+
+// The triangles each cell owns (i the smallest index of the three: the
+// triangles the reference tests, voronoiModelBase.cpp:380-423) that the skip
+// rule leaves to the empty-circumcircle test: all of them where the rule is
+// off for the system, else those of cells whose bin is stamped or whose largest
+// circumradius exceeds the dilation. They are gathered in LDS and appended to
+// the triangle queue with one reservation per block; a triangle beyond the LDS
+// list (more than TRI_PER_CELL per cell of the block) or beyond the queue's
+// capacity is tested where it is found.
+constexpr int TRI_PER_CELL = 5;
+// grid of k_test_queue: 8 blocks per CU; the blocks loop over the queue
+constexpr int TEST_QUEUE_BLOCKS = 2048;
+__global__ __launch_bounds__(512) void k_select_triangles(cgpu_voronoi_batch b, const double *pos) {
     extern __shared__ int3 test_lds[];
     const int TRI_LDS = TRI_PER_CELL * (int)blockDim.x;
     int3 *stri = test_lds;
-    short *swide = reinterpret_cast<short *>(test_lds + TRI_LDS);
-    __shared__ int scount, swcount;
+    __shared__ int scount, sepoch, sbase;
     const int i = xcd_block() * blockDim.x + threadIdx.x;
     if (threadIdx.x == 0) {
         scount = 0;
-        swcount = 0;
+        sepoch = b.counters[7];
     }
     __syncthreads();
     const bool live = i < b.num_cells && b.st[b.cell_sys[i]].active;
     if (live) {
         const int s = b.cell_sys[i];
-        cell_geometry<R>(b, pos, i, b.sys[s]);
-        if (test) {
+        const cgpu_sys_params &sp = b.sys[s];
+        const bool near = b.ld_rule[s] == 0 || b.bin_mark_wide[b.cell_bin[i]] == sepoch ||
+                          !(2.0 * (double)b.cell_rmax[i] <= (MARK_DILATE - 1) * sp.boxsize);
+        if (near) {
             const int N = b.num_cells;
             const int z = b.nn[i];
             int jn = b.nbr[i];
@@ -1572,30 +1870,75 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8))) voi
             }
         }
     }
-    if (!test) return;  // uniform
     __syncthreads();
     const int n = min(scount, TRI_LDS);
+    if (threadIdx.x == 0) sbase = n > 0 ? atomicAdd(&b.counters[9], n) : 0;
+    __syncthreads();
+    const int base = sbase;
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int3 t = stri[q];
-        if (test_narrow(b, pos, t.x, t.y, t.z)) continue;
-        swide[atomicAdd(&swcount, 1)] = (short)q;
-    }
-    __syncthreads();
-    const int nw = swcount;
-    for (int q = threadIdx.x >> 6; q < nw; q += blockDim.x >> 6) {
-        const int3 t = stri[swide[q]];
-        wave_test_wide(b, pos, t.x, t.y, t.z);
+        const long long at = (long long)base + q;
+        if (at < b.ld_tri_cap) {
+            int *d = b.ld_tri + 3 * at;
+            d[0] = t.x;
+            d[1] = t.y;
+            d[2] = t.z;
+        } else if (!test_narrow(b, pos, t.x, t.y, t.z)) {
+            serial_test_wide(b, pos, t.x, t.y, t.z);
+        }
     }
 }
 
-void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st) {
+// The empty-circumcircle test of the queued triangles, spread evenly over the
+// grid: a thread per triangle for the 3 x 3-bin windows; triangles with a wider
+// window go to an LDS list that the block's wavefronts scan cooperatively, one
+// wavefront per triangle.
+__global__ __launch_bounds__(256) void k_test_queue(cgpu_voronoi_batch b, const double *pos) {
+    __shared__ int sn, swcount;
+    __shared__ int swide[256];
+    if (threadIdx.x == 0) sn = min(b.counters[9], b.ld_tri_cap);
+    __syncthreads();
+    const int n = sn;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {  // uniform per block
+        if (threadIdx.x == 0) swcount = 0;
+        __syncthreads();
+        const int q = base + threadIdx.x;
+        if (q < n) {
+            const int *t = b.ld_tri + 3 * (size_t)q;
+            if (!test_narrow(b, pos, t[0], t[1], t[2])) swide[atomicAdd(&swcount, 1)] = q;
+        }
+        __syncthreads();
+        const int nw = swcount;
+        for (int w = threadIdx.x >> 6; w < nw; w += blockDim.x >> 6) {
+            const int *t = b.ld_tri + 3 * (size_t)swide[w];
+            wave_test_wide(b, pos, t[0], t[1], t[2]);
+        }
+        __syncthreads();
+    }
+}
+
+void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int test, hipStream_t st,
+                          const Probe *pr = nullptr) {
     const int bs = shape_test_block(b);
     const dim3 g((unsigned)((b.num_cells + bs - 1) / bs)), t(bs);
-    const size_t lds = (size_t)TRI_PER_CELL * bs * (sizeof(int3) + sizeof(short));
     if (b.precision)
-        hipLaunchKernelGGL(k_test_geometry<float>, g, t, lds, st, b, pos, test);
+        hipLaunchKernelGGL(k_test_geometry<float>, g, t, 0, st, b, pos, test);
     else
-        hipLaunchKernelGGL(k_test_geometry<double>, g, t, lds, st, b, pos, test);
+        hipLaunchKernelGGL(k_test_geometry<double>, g, t, 0, st, b, pos, test);
+    if (!test) {
+        probe(pr, PROBE_TEST_GEOMETRY);
+        return;
+    }
+    probe(pr, PROBE_TEST_GEOMETRY);
+    hipLaunchKernelGGL(k_ld_reduce, dim3(256), dim3(256), 0, st, b, pos);
+    probe(pr, PROBE_LD_REDUCE);
+    hipLaunchKernelGGL(k_ld_rule, dim3((b.num_systems + 255) / 256), dim3(256), 0, st, b);
+    probe(pr, PROBE_LD_RULE);
+    const size_t lds = (size_t)TRI_PER_CELL * bs * sizeof(int3);
+    hipLaunchKernelGGL(k_select_triangles, g, t, lds, st, b, pos);
+    probe(pr, PROBE_SELECT_TRIANGLES);
+    hipLaunchKernelGGL(k_test_queue, dim3(TEST_QUEUE_BLOCKS), dim3(256), 0, st, b, pos);
+    probe(pr, PROBE_TEST_QUEUE);
 }
 
 // NeedsFixing = flagged cells and their current neighbours (reference
@@ -2425,8 +2768,7 @@ static int step_common(const cgpu_voronoi_batch *b, int kind, void *stream, cons
     probe(pr, PROBE_BIN_SCATTER);
     r = check();
     if (r) return r;
-    launch_test_geometry(*b, b->pos_next, 1, st);
-    probe(pr, PROBE_TEST_GEOMETRY);
+    launch_test_geometry(*b, b->pos_next, 1, st, pr);
     // its last kernel advances the step counters
     return repair_chain(*b, b->pos_next, st, 1, pr);
 }

@@ -78,6 +78,15 @@ typedef struct cgpu_sys_state {
     int nflag;          /* cells flagged by the circumcircle test this step */
     int pad_s;
     long long global_fixes, repair_events, local_repairs, t1_events;
+    /* Triangulation checks of the circumcircle test's skip rule, per step:
+     * wrapping sums of a hash of every ring triangle over all ring slots and
+     * over the owned slots (consistent rings: all = 3 x owned), the owned
+     * triangles' signed area in units of 2^-30 (one covering of the box: L^2),
+     * and a flag that
+     * turns the rule off for the system. */
+    unsigned long long tri_hash_all, tri_hash_own;
+    long long tri_area;          /* fixed point, units of 2^-30 */
+    int skip_off, pad_k;
 } cgpu_sys_state;
 
 /*
@@ -142,8 +151,9 @@ typedef struct cgpu_voronoi_batch {
     int *flagged;                /* [N] list of flagged cells       */
     int *nfq;                    /* [N] NeedsFixing queue           */
     int *gq;                     /* [N] global-retriangulation queue */
-    int *counters;               /* [8] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
-                                    5 nonfinite, 6 serial-star queue (in flagged) */
+    int *counters;               /* [16] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
+                                    5 nonfinite, 6 serial-star queue (in flagged), 7 test epoch,
+                                    8 regions to stamp, 9 triangles queued for the circumcircle test */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
     /* Work chunks: at most CGPU_CHUNK cells (or CGPU_BIN_CHUNK bins) of one
      * system, so per-system passes spread a large system over many work-groups. */
@@ -162,10 +172,26 @@ typedef struct cgpu_voronoi_batch {
     const int *cell_type;        /* [N] or NULL                                     */
     const double *tension;       /* [num_types^2] or NULL                           */
     int num_types, pad_t;
+    /* Circumcircle-test skip rule: bins within 3 bins of an edge that is not
+     * locally Delaunay (or of a ring that is not a proper fan), stamped with the
+     * test's epoch (counters[7]); each cell's largest circumradius; the queue of
+     * triangles the rule leaves to test. */
+    int *ld_tri;                 /* [3 ld_tri_cap] (i, j, l) of each queued triangle */
+    int *bin_mark_wide;          /* [total_bins] */
+    float *cell_rmax;            /* [N] */
+    void *ld_regions;            /* [ld_region_cap] regions to stamp (system or -1-cell, bounds) */
+    int ld_region_cap, ld_tri_cap;
+    unsigned long long *ld_part; /* [3 (N/64 + 1)] sums of each 64-cell group inside one system */
+    int *ld_rule;                /* [S] 1: the skip rule holds for the system in this test */
     cgpu_launch_shape shape;     /* launch shapes (zero fields: defaults)           */
 } cgpu_voronoi_batch;
 
 #define CGPU_CHUNK 4096
+#define CGPU_COUNTERS 16
+/* capacity of the circumcircle test's region list: max(1024, N / 8) records of
+ * 40 bytes; of its triangle queue: 2 N + 1024 triangles (a triangulation of the
+ * torus has 2 N; more, from rings that disagree, are tested where they arise) */
+#define CGPU_LD_REGION_BYTES 40
 #define CGPU_BIN_CHUNK 8192
 
 /* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
@@ -642,9 +668,10 @@ int cgpu_device_synchronize(void);
 int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out);
 /* Steps with a HIP event after every kernel launch (no graph): ms_slots[k]
  * receives the summed duration of profile slot k (the kernels of one step in
- * launch order, named by cgpu_profile_slot_name), queue_sums[0..2] the summed
- * sizes of the flagged, NeedsFixing and global-retriangulation queues. */
-#define CGPU_PROFILE_SLOTS 13
+ * launch order, named by cgpu_profile_slot_name), queue_sums[0..4] the summed
+ * sizes of the flagged, NeedsFixing and global-retriangulation queues, of the
+ * circumcircle test's triangle queue and of its list of regions to stamp. */
+#define CGPU_PROFILE_SLOTS 17
 int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long *queue_sums);
 const char *cgpu_profile_slot_name(int slot);
 /* Host-side Philox4x32-10 and the normal pair of (seed, index, step, stream),

@@ -156,7 +156,8 @@ def _kernel_layer():
 def test_flags_and_needs_fixing_match_oracle(oracle):
     """Kernel layer, step by step from the random start (FP64, two systems):
     the circumcircle-test flags of every cell and each system's NeedsFixing set
-    equal the oracle's on the same positions and previous rings."""
+    equal the oracle's on the same positions and previous rings. The test's
+    skip rule is in force: most owned triangles never reach the scan."""
     cg = _cg()
     hip = _hip()
     L, View = _kernel_layer()
@@ -166,6 +167,7 @@ def test_flags_and_needs_fixing_match_oracle(oracle):
     st = ctypes.c_void_p(b.stream())
     N = v.num_cells
     nflags_total = 0
+    queued = []
     for step in range(25):
         nn0, nbr0, nm0 = b.topology(CAP)
         vn = View.from_buffer_copy(v)
@@ -183,8 +185,9 @@ def test_flags_and_needs_fixing_match_oracle(oracle):
         assert L.cgpu_repair_topology(ctypes.byref(vn), st) == 0
         assert L.cgpu_step_end(ctypes.byref(vn), st) == 0
         assert hip.hipStreamSynchronize(st) == 0
-        cnt = np.zeros(8, np.int32)
-        _d2h(hip, cnt, vn.counters, 32)
+        cnt = np.zeros(16, np.int32)
+        _d2h(hip, cnt, vn.counters, 64)
+        queued.append(int(cnt[9]))  # triangles left to the scan in this step
         nfq = np.zeros(max(int(cnt[1]), 1), np.int32)
         if cnt[1] > 0:
             _d2h(hip, nfq, vn.nfq, 4 * int(cnt[1]))
@@ -213,6 +216,8 @@ def test_flags_and_needs_fixing_match_oracle(oracle):
             nflags_total += int(of.sum())
         v = vn
     assert nflags_total > 0
+    # a triangulation of the torus has 2 N triangles, each owned by one cell
+    assert 0 < sum(queued) < 0.5 * 25 * 2 * N, queued
     b.close()
 
 
