@@ -161,6 +161,13 @@ template <> struct Prec<float> {
     }
 };
 
+// a * b + c: fused (one rounding) in the FP32 path; in FP64 the product and the
+// sum round separately, as in the reference expression written out (c + a b and
+// a b + c are the same sum)
+template <typename R> __device__ inline R madd(R a, R b, R c);
+template <> __device__ inline float madd<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ inline double madd<double>(double a, double b, double c) { return a * b + c; }
+
 // reference getdhdr (functions.h:234-242): d(Voronoi vertex)/d(r_i) for the
 // vertex of (i, j, k) with rij, rik relative to i; m = (x11, x12, x21, x22)
 template <typename R> __device__ inline void dhdr_eval(vec2<R> rij, vec2<R> rik, R m[4]) {
@@ -433,7 +440,7 @@ __device__ __forceinline__ void slot_force_set(const cgpu_voronoi_batch &b, cons
                                                vec2<R> ul, vec2<R> un, vec2<R> uo, const R m[4], vec2<R> &sum) {
     vec2<R> dAdv{R(0.5) * (vlast.y - vnext.y), R(0.5) * (vnext.x - vlast.x)};
     vec2<R> dPdv{ul.x - un.x, ul.y - un.y};
-    vec2<R> dEdv{qI.x * dAdv.x + qI.y * dPdv.x, qI.x * dAdv.y + qI.y * dPdv.y};
+    vec2<R> dEdv{madd(qI.y, dPdv.x, qI.x * dAdv.x), madd(qI.y, dPdv.y, qI.x * dAdv.y)};
     TensionSlot<R> T{};
     if (tens) {
         // self terms: edges I-K (vcur, vnext) and I-J (vcur, vlast)
@@ -454,7 +461,7 @@ __device__ __forceinline__ void slot_force_set(const cgpu_voronoi_batch &b, cons
     dAdv = vec2<R>{R(0.5) * (vnext.y - vother.y), R(0.5) * (vother.x - vnext.x)};
     dPdv = vec2<R>{-un.x - uo.x, -un.y - uo.y};
     {
-        vec2<R> e{qC.x * dAdv.x + qC.y * dPdv.x, qC.x * dAdv.y + qC.y * dPdv.y};
+        vec2<R> e{madd(qC.y, dPdv.x, qC.x * dAdv.x), madd(qC.y, dPdv.y, qC.x * dAdv.y)};
         if (tens) {  // edges K-I (vcur, vnext) and K-J (vcur, vother)
             vec2<R> dT{R(0), R(0)};
             if (T.ik) {
@@ -475,7 +482,7 @@ __device__ __forceinline__ void slot_force_set(const cgpu_voronoi_batch &b, cons
     dAdv = vec2<R>{R(0.5) * (vother.y - vlast.y), R(0.5) * (vlast.x - vother.x)};
     dPdv = vec2<R>{-uo.x + ul.x, -uo.y + ul.y};
     {
-        vec2<R> e{qB.x * dAdv.x + qB.y * dPdv.x, qB.x * dAdv.y + qB.y * dPdv.y};
+        vec2<R> e{madd(qB.y, dPdv.x, qB.x * dAdv.x), madd(qB.y, dPdv.y, qB.x * dAdv.y)};
         if (tens) {  // edges J-I (vcur, vlast) and J-K (vcur, vother)
             vec2<R> dT{R(0), R(0)};
             if (T.ij) {
@@ -494,8 +501,8 @@ __device__ __forceinline__ void slot_force_set(const cgpu_voronoi_batch &b, cons
     }
     dEdv.x *= R(2.0);
     dEdv.y *= R(2.0);
-    sum.x += dEdv.x * m[0] + dEdv.y * m[2];
-    sum.y += dEdv.x * m[1] + dEdv.y * m[3];
+    sum.x += madd(dEdv.y, m[2], dEdv.x * m[0]);
+    sum.y += madd(dEdv.y, m[3], dEdv.x * m[1]);
 }
 
 // Unit vectors of the Voronoi edges in the perimeter gradient. Exact identity:
@@ -555,11 +562,30 @@ __device__ __attribute__((noinline)) float4 rare_dhdr(const double *pos, double 
     return make_float4((float)md[0], (float)md[1], (float)md[2], (float)md[3]);
 }
 
-// perp(s) / |s|
-__device__ inline vec2<float> perp_unit(vec2<float> s) {
-    const float inv = 1.0f / sqrtf(s.x * s.x + s.y * s.y);
-    return vec2<float>{-s.y * inv, s.x * inv};
+// FP32 pieces of the batched force: cross product and squared norm with one
+// rounding of the product pair, the circumcenter of (0, x, y) from them and
+// h = 0.5 / cross(x, y), the Jacobian of that vertex with respect to the origin
+// cell (reference getdhdr, functions.h:234-242, with rij = x, rik = y:
+// 0.5 / cross^2 = 2 h^2), and the hardware reciprocal and reciprocal square
+// root (1 ulp). The force's FP32 error is set by the vertex positions, far
+// above these roundings.
+__device__ inline float cross_f(vec2<float> x, vec2<float> y) { return __builtin_fmaf(x.x, y.y, -(x.y * y.x)); }
+__device__ inline float norm2_f(vec2<float> x) { return __builtin_fmaf(x.x, x.x, x.y * x.y); }
+__device__ inline float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline vec2<float> circ0_f(vec2<float> x, vec2<float> y, float nx, float ny, float h) {
+    return vec2<float>{h * __builtin_fmaf(nx, y.y, -(x.y * ny)), h * __builtin_fmaf(x.x, ny, -(nx * y.x))};
 }
+__device__ inline void dhdr_f(vec2<float> rij, vec2<float> rik, float nij, float nik, float h, float m[4]) {
+    const float g = 2.0f * h * h;
+    const float P = __builtin_fmaf(rik.y, nij, -(rij.y * nik)), Q = __builtin_fmaf(rik.x, nij, -(rij.x * nik));
+    const float gy = g * (rik.y - rij.y), gx = g * (rik.x - rij.x);
+    m[0] = gy * P;
+    m[1] = -gy * Q;
+    m[2] = -gx * P;
+    m[3] = gx * Q;
+}
+
 
 // Force on cell i: sum over ring slots k of the reference force set
 // (voronoiQuadraticEnergy.cu:99-234): slot k is the Voronoi vertex of
@@ -658,23 +684,14 @@ __device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &ac
     const dvec2 pi = acc.p(i);
     auto rk = [&](int k) { return b.nbr[((k % z + z) % z) * N + i]; };
     auto d64 = [&](int id) { return min_dist(box, acc.p(id), pi); };
-    auto c0 = [&](vec2<R> x, vec2<R> y, int ix, int iy) -> vec2<R> {
-        if (Prec<R>::well(x.x * y.y - x.y * y.x, x, y)) return circ0<R>(x, y);
-        const float2 c = rare_circ0(acc.pos, sp.L, pi, ix, iy);
-        return vec2<R>{c.x, c.y};
-    };
-    auto c3 = [&](vec2<R> x, vec2<R> y, vec2<R> w, int ix, int iy, int iw) -> vec2<R> {
-        const vec2<R> u{x.x - w.x, x.y - w.y}, v{y.x - w.x, y.y - w.y};
-        if (Prec<R>::well(u.x * v.y - u.y * v.x, u, v)) return circ3<R>(x, y, w);
-        const float2 c = rare_circ3(acc.pos, sp.L, pi, ix, iy, iw);
-        return vec2<R>{c.x, c.y};
-    };
     const R amb = (R)(AMBIGUOUS_EDGE * sp.boxsize);
     // unit vector of the vertex difference d of the edge from circ(0, X, Y) to
     // circ(0, Y, Z) (cells ix, iy, iz; origin io, -1 = cell i), along perp(Y)
-    auto unit = [&](vec2<R> d, vec2<R> Y, int io, int ix, int iy, int iz) -> vec2<R> {
-        const vec2<R> e = perp_unit(Y);
-        R lam = d.x * e.x + d.y * e.y;
+    // (|Y|^2 = nY)
+    auto unit = [&](vec2<R> d, vec2<R> Y, R nY, int io, int ix, int iy, int iz) -> vec2<R> {
+        const R inv = rsq_f(nY);
+        const vec2<R> e{-Y.y * inv, Y.x * inv};
+        R lam = __builtin_fmaf(d.x, e.x, d.y * e.y);
         if (fabsf(lam) < amb) {
             const int sg = rare_edge_sign(acc.pos, sp.L, pi, io, ix, iy, iz);
             if (sg == 0) return vec2<R>{R(0), R(0)};  // zero-length edge (float build: d / 1e-18 = 0)
@@ -682,16 +699,35 @@ __device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &ac
         }
         return lam >= R(0) ? e : vec2<R>{-e.x, -e.y};
     };
+    // The Voronoi vertex circ(0, X, Y) of the ring pair (X, Y) = (ring k-1, ring
+    // k) with its cross product's h = 0.5 / cross(X, Y) and well-conditioning
+    // flag; the next slot's Jacobian of the same vertex reuses all three.
     int jA = rk(-3), jB = rk(-2), jC = rk(-1);
     vec2<R> fB = cvt<R>(d64(jB)), fC = cvt<R>(d64(jC));
+    R nB = norm2_f(fB), nC = norm2_f(fC);
     vec2<R> qB = acc.q(jB), qC = acc.q(jC);
-    vec2<R> vM = c0(fB, fC, jB, jC);
+    auto vertex = [&](vec2<R> X, vec2<R> Y, R nX, R nY, int iX, int iY, R &h, bool &ok) -> vec2<R> {
+        const R cr = cross_f(X, Y);
+        ok = Prec<R>::well(cr, X, Y);
+        if (ok) {
+            h = R(0.5) * rcp_f(cr);
+            return circ0_f(X, Y, nX, nY, h);
+        }
+        h = R(0);
+        const float2 c = rare_circ0(acc.pos, sp.L, pi, iX, iY);
+        return vec2<R>{c.x, c.y};
+    };
+    R hM;
+    bool okM;
+    vec2<R> vM = vertex(fB, fC, nB, nC, jB, jC, hM, okM);
     vec2<R> vL;
     {
         const vec2<R> fA = cvt<R>(d64(jA));
-        vL = c0(fA, fB, jA, jB);
+        R hL;
+        bool okL;
+        vL = vertex(fA, fB, norm2_f(fA), nB, jA, jB, hL, okL);
     }
-    vec2<R> ul = unit(vec2<R>{vL.x - vM.x, vL.y - vM.y}, fB, -1, jA, jB, jC);
+    vec2<R> ul = unit(vec2<R>{vL.x - vM.x, vL.y - vM.y}, fB, nB, -1, jA, jB, jC);
     const vec2<R> qI = acc.q(i);
     int tI = 0, tB = 0, tC = 0;
     if (tens) {
@@ -719,11 +755,15 @@ __device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &ac
             oD2 = b.oth[(k + 2) * N + i];
         }
         const vec2<R> fD = cvt<R>(min_dist(box, pD, pi)), fO = cvt<R>(min_dist(box, pO, pi));
-        const vec2<R> vN = c0(fC, fD, jC, jD);
-        // --- reference force set for slot k: (nm1, n1) = (ring k-2, ring k-1)
+        const R nD = norm2_f(fD);
+        R hN;
+        bool okN;
+        const vec2<R> vN = vertex(fC, fD, nC, nD, jC, jD, hN, okN);
+        // --- reference force set for slot k: (nm1, n1) = (ring k-2, ring k-1);
+        // the Jacobian of vertex M = circ(0, B, C)
         R m[4];
-        if (Prec<R>::well(fB.y * fC.x - fB.x * fC.y, fB, fC)) {
-            dhdr_eval<R>(fB, fC, m);
+        if (okM) {
+            dhdr_f(fB, fC, nB, nC, hM, m);
         } else {
             const float4 md = rare_dhdr(acc.pos, sp.L, pi, jB, jC);
             m[0] = md.x;
@@ -732,18 +772,33 @@ __device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &ac
             m[3] = md.w;
         }
         const vec2<R> vcur = vM, vlast = vL, vnext = vN;
-        const vec2<R> vother = c3(fB, fC, fO, jB, jC, jO);
+        // vother = circ(B, C, O)
+        vec2<R> vother;
+        {
+            const vec2<R> u{fB.x - fO.x, fB.y - fO.y}, v{fC.x - fO.x, fC.y - fO.y};
+            const R cr = cross_f(u, v);
+            if (Prec<R>::well(cr, u, v)) {
+                const R h = R(0.5) * rcp_f(cr), nu = norm2_f(u), nv = norm2_f(v);
+                vother = vec2<R>{fO.x + h * __builtin_fmaf(nu, v.y, -(u.y * nv)),
+                                 fO.y + h * __builtin_fmaf(u.x, nv, -(nu * v.x))};
+            } else {
+                const float2 c = rare_circ3(acc.pos, sp.L, pi, jB, jC, jO);
+                vother = vec2<R>{c.x, c.y};
+            }
+        }
         // un: vcur - vnext on the bisector of i and ring k-1; uo: vcur - vother
         // on the bisector of ring k-2 and ring k-1 (frame of ring k-2:
         // vcur = circ(B, 0, C), vother = circ(B, O, C))
-        const vec2<R> un = unit(vec2<R>{vcur.x - vnext.x, vcur.y - vnext.y}, fC, -1, jB, jC, jD);
-        const vec2<R> uo =
-            unit(vec2<R>{vcur.x - vother.x, vcur.y - vother.y}, vec2<R>{fC.x - fB.x, fC.y - fB.y}, jB, i, jC, jO);
+        const vec2<R> un = unit(vec2<R>{vcur.x - vnext.x, vcur.y - vnext.y}, fC, nC, -1, jB, jC, jD);
+        const vec2<R> sBC{fC.x - fB.x, fC.y - fB.y};
+        const vec2<R> uo = unit(vec2<R>{vcur.x - vother.x, vcur.y - vother.y}, sBC, norm2_f(sBC), jB, i, jC, jO);
         slot_force_set<R>(b, sp, tens, tI, tB, tC, qI, qB, qC, vlast, vcur, vnext, vother, ul, un, uo, m, sum);
         // --- advance the window
         ul = un;
         fB = fC;
         fC = fD;
+        nB = nC;
+        nC = nD;
         jB = jC;
         jC = jD;
         qB = qC;
@@ -754,6 +809,8 @@ __device__ vec2<float> cell_force_f32(const cgpu_voronoi_batch &b, const Acc &ac
         }
         vL = vM;
         vM = vN;
+        hM = hN;
+        okM = okN;
     }
     return sum;
 }

@@ -153,16 +153,22 @@ def _kernel_layer():
     return L, VoronoiBatchView
 
 
-def test_flags_and_needs_fixing_match_oracle(oracle):
-    """Kernel layer, step by step from the random start (FP64, two systems):
-    the circumcircle-test flags of every cell and each system's NeedsFixing set
-    equal the oracle's on the same positions and previous rings. The test's
-    skip rule is in force: most owned triangles never reach the scan."""
+@pytest.mark.parametrize("relax", [0, 400])
+def test_flags_and_needs_fixing_match_oracle(oracle, relax):
+    """Kernel layer, step by step (FP64, two systems), from the random start and
+    after `relax` steps: the circumcircle-test flags of every cell and each
+    system's NeedsFixing set equal the oracle's on the same positions and
+    previous rings. From the random start the local repair leaves rings that
+    disagree, which turns the test's skip rule off; after relaxation the rule
+    is in force and most owned triangles never reach the scan."""
     cg = _cg()
     hip = _hip()
     L, View = _kernel_layer()
     b = cg.VoronoiBatch([500, 1500], seeds=[21, 22], precision="fp64")
     b.set_motility(0.1, 1.0)
+    if relax:
+        b.spv_steps(relax, use_graph=False)
+        b.check_status()
     v = b.view()
     st = ctypes.c_void_p(b.stream())
     N = v.num_cells
@@ -217,7 +223,8 @@ def test_flags_and_needs_fixing_match_oracle(oracle):
         v = vn
     assert nflags_total > 0
     # a triangulation of the torus has 2 N triangles, each owned by one cell
-    assert 0 < sum(queued) < 0.5 * 25 * 2 * N, queued
+    if relax:
+        assert 0 < sum(queued) < 0.5 * 25 * 2 * N, queued
     b.close()
 
 
