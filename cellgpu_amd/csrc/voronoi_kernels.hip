@@ -268,7 +268,8 @@ __device__ inline unsigned long long tri_hash(int a, int b, int c) {
         b = a;
         a = t;
     }
-    return mix64(mix64(mix64(0x9e3779b97f4a7c15ull + (unsigned)a) ^ (unsigned)b) ^ (unsigned)c);
+    // (a, b) packed, c spread by an odd multiplier (a bijection), then mixed
+    return mix64(((unsigned long long)(unsigned)a << 32 | (unsigned)b) ^ ((unsigned long long)(unsigned)c * 0x9e3779b97f4a7c15ull));
 }
 // A region for k_ld_reduce to stamp: the rectangle [x0, x1] x [y0, y1] of system
 // sys, or (sys = -1 - i) the bounding box of cell i's star. The geometry pass
@@ -1604,8 +1605,12 @@ __device__ __forceinline__ void serial_test_wide(const cgpu_voronoi_batch &b, co
 // gpu_test_circumcenters_kernel, voronoiModelBase.cu:24-109, with its search
 // window) when its window is the 3 x 3 bins around cell i's bin (wcheck = 1):
 // independent loads of the nine bins. Returns false, testing nothing, when the
-// window is wider.
-__device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l) {
+// window is wider. TEAM > 1: the TEAM lanes of an aligned group (t = 0 .. TEAM-1)
+// test the same triangle, lane t taking the candidates p = t mod TEAM of the
+// concatenated runs; the team's ballot decides the triangle's flag.
+template <int TEAM = 1>
+__device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const double *pos, int i, int j, int l,
+                                            int t = 0) {
     const int s = b.cell_sys[i];
     const cgpu_sys_params &sp = b.sys[s];
     if (system_goes_global(b, s, sp)) return true;
@@ -1673,16 +1678,34 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
             bad = bad || in0 || in1;
         }
     };
-    scan_run(lo[0], hi[0]);
-    scan_run(lo[1], hi[1]);
-    scan_run(lo[2], hi[2]);
-    scan_run(lo[3], hi[3]);
-    scan_run(lo[4], hi[4]);
-    scan_run(lo[5], hi[5]);
-    scan_run(lo[6], hi[6]);
-    scan_run(lo[7], hi[7]);
-    scan_run(lo[8], hi[8]);
-    if (bad) flag_triangle(b, s, i, j, l);
+    if constexpr (TEAM == 1) {
+        scan_run(lo[0], hi[0]);
+        scan_run(lo[1], hi[1]);
+        scan_run(lo[2], hi[2]);
+        scan_run(lo[3], hi[3]);
+        scan_run(lo[4], hi[4]);
+        scan_run(lo[5], hi[5]);
+        scan_run(lo[6], hi[6]);
+        scan_run(lo[7], hi[7]);
+        scan_run(lo[8], hi[8]);
+        if (bad) flag_triangle(b, s, i, j, l);
+    } else {
+        int acc = 0;  // candidates of the runs before run r
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            for (int k = lo[r] + ((t - acc) & (TEAM - 1)); k < hi[r]; k += TEAM) {
+                if (corner(k)) continue;
+                if (cand_inside(box, w, min_dist(box, ldpos(b.sorted_pos, k), v))) {
+                    flag_cell(b, s, b.sorted_id[k]);
+                    bad = true;
+                }
+            }
+            acc += hi[r] - lo[r];
+        }
+        const unsigned long long m = __ballot(bad);
+        const int first = (int)__lane_id() & ~(TEAM - 1);
+        if (t == 0 && ((m >> first) & ((1ull << TEAM) - 1ull)) != 0ull) flag_triangle(b, s, i, j, l);
+    }
     return true;
 }
 
@@ -1947,28 +1970,31 @@ __global__ __launch_bounds__(512) void k_select_triangles(cgpu_voronoi_batch b, 
 }
 
 // The empty-circumcircle test of the queued triangles, spread evenly over the
-// grid: a thread per triangle for the 3 x 3-bin windows; triangles with a wider
-// window go to an LDS list that the block's wavefronts scan cooperatively, one
-// wavefront per triangle.
+// grid: a team of TEST_TEAM lanes per triangle for the 3 x 3-bin windows (the
+// queue is short, so the lanes shorten each triangle's chain of loads instead
+// of idling); triangles with a wider window go to an LDS list that the block's
+// wavefronts scan cooperatively, one wavefront per triangle.
+constexpr int TEST_TEAM = 8;
 __global__ __launch_bounds__(256) void k_test_queue(cgpu_voronoi_batch b, const double *pos) {
     __shared__ int sn, swcount;
-    __shared__ int swide[256];
+    __shared__ int swide[256 / TEST_TEAM];
     if (threadIdx.x == 0) sn = min(b.counters[9], b.ld_tri_cap);
     __syncthreads();
     const int n = sn;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {  // uniform per block
+    const int per_block = (int)blockDim.x / TEST_TEAM, t = (int)threadIdx.x & (TEST_TEAM - 1);
+    for (int base = blockIdx.x * per_block; base < n; base += gridDim.x * per_block) {  // uniform per block
         if (threadIdx.x == 0) swcount = 0;
         __syncthreads();
-        const int q = base + threadIdx.x;
+        const int q = base + (int)threadIdx.x / TEST_TEAM;
         if (q < n) {
-            const int *t = b.ld_tri + 3 * (size_t)q;
-            if (!test_narrow(b, pos, t[0], t[1], t[2])) swide[atomicAdd(&swcount, 1)] = q;
+            const int *tr = b.ld_tri + 3 * (size_t)q;
+            if (!test_narrow<TEST_TEAM>(b, pos, tr[0], tr[1], tr[2], t) && t == 0) swide[atomicAdd(&swcount, 1)] = q;
         }
         __syncthreads();
         const int nw = swcount;
         for (int w = threadIdx.x >> 6; w < nw; w += blockDim.x >> 6) {
-            const int *t = b.ld_tri + 3 * (size_t)swide[w];
-            wave_test_wide(b, pos, t[0], t[1], t[2]);
+            const int *tw = b.ld_tri + 3 * (size_t)swide[w];
+            wave_test_wide(b, pos, tw[0], tw[1], tw[2]);
         }
         __syncthreads();
     }

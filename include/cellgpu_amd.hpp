@@ -52,7 +52,7 @@ inline void check(int status, const char *what) {
 }
 
 // ------------------------------------------------------------ host mirrors
-//! reference access_location / access_mode (gpuarray.h:91-108)
+//! reference access_location / access_mode (gpuarray.h:41-75)
 namespace access_location {
 enum Enum { host, device };
 }
@@ -94,7 +94,7 @@ template <class T> class GPUArray {
     std::function<void(const vector<T> &)> push_;
 };
 
-//! reference ArrayHandle (gpuarray.h:289-328), host access only
+//! reference ArrayHandle (gpuarray.h:91-101, 180-190), host access only
 template <class T> class ArrayHandle {
    public:
     ArrayHandle(GPUArray<T> &a, access_location::Enum loc = access_location::host,
