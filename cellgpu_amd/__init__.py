@@ -378,6 +378,20 @@ class VoronoiQuadraticEnergy:
         """reference Simple2DCell::setCellType."""
         self.batch.set_cell_types(types)
 
+    def spatialSorting(self):
+        """Hilbert relabelling of the cells on the device, then retriangulation
+        (reference voronoiModelBase::spatialSorting, voronoiModelBase.cpp:669-680).
+        itt[new] = old index of the last sort; idxToTag / tagToIdx relate current
+        indices to the labels at creation (Simple2DCell.h:241-253)."""
+        self.itt = self.batch.spatial_sort()
+        self.tti = np.empty_like(self.itt)
+        self.tti[self.itt] = np.arange(len(self.itt), dtype=self.itt.dtype)
+        t = np.zeros(self.Ncells, np.int32)
+        check(lib().cgpu_vor_get_tags(self.batch.h, _p(t)), "tags")
+        self.idxToTag = t
+        self.tagToIdx = np.empty_like(t)
+        self.tagToIdx[t] = np.arange(self.Ncells, dtype=np.int32)
+
     def cellDeath(self, cellIndex):
         """reference voronoiModelBase::cellDeath (voronoiModelBase.cpp:1309-1316)."""
         self.batch.cell_events([0], [cellIndex])
@@ -538,19 +552,41 @@ class Simulation:
         for u in self.updaters:
             u.setReproducible(r)
 
+    def setSortPeriod(self, sp):
+        """Spatial sorting every sp steps (reference Simulation::setSortPeriod;
+        sp <= 0: never)."""
+        self.sortPeriod = int(sp)
+
+    def _sort_if_due(self):
+        # reference Simulation.cpp:155-164: after the updaters of a step whose
+        # number is a multiple of the period
+        sp = getattr(self, "sortPeriod", -1)
+        if sp > 0 and self.integerTimestep % sp == 0:
+            self.config.spatialSorting()
+            for u in self.updaters:
+                if hasattr(u, "spatialSorting"):
+                    u.spatialSorting()
+
     def performTimestep(self):
         self.integerTimestep += 1
         self.Time += self.integrationTimestep
         for u in self.updaters:
             u.performUpdate()
+        self._sort_if_due()
 
     def performTimesteps(self, k):
-        """k steps at once (graph-captured); same result as k performTimestep calls."""
+        """k steps at once (graph-captured between sorts); same result as k
+        performTimestep calls."""
         if len(self.updaters) == 1 and isinstance(self.updaters[0],
                                                   (selfPropelledParticleDynamics, selfPropelledCellVertexDynamics)):
-            self.updaters[0].advance(k)
-            self.integerTimestep += k
-            self.Time += k * self.integrationTimestep
+            sp = getattr(self, "sortPeriod", -1)
+            while k > 0:
+                run = k if sp <= 0 else min(k, sp - self.integerTimestep % sp)
+                self.updaters[0].advance(run)
+                self.integerTimestep += run
+                self.Time += run * self.integrationTimestep
+                k -= run
+                self._sort_if_due()
         else:
             for _ in range(k):
                 self.performTimestep()
@@ -722,6 +758,15 @@ class VertexBatch:
         self.Nv = int(lib().cgpu_vm_get_sizes(self.h, _p(nc), None))
         self.Nc = int(nc.sum())
 
+    def spatial_sort(self):
+        """Relabel vertices along a Hilbert curve (per system) and cells in order of
+        first appearance around them (reference vertexModelBase::spatialSorting).
+        Returns (vperm, cperm), new index -> old index."""
+        vp = np.zeros(self.Nv, np.int32)
+        cp = np.zeros(self.Nc, np.int32)
+        check(lib().cgpu_vm_spatial_sort(self.h, _p(vp), _p(cp)), "vm_spatial_sort")
+        return vp, cp
+
     def set_topology(self, vn, vcn, cvn, cv):
         """Replace the topology (checked for consistency first); cv is [Nc, cap]."""
         cv = np.ascontiguousarray(cv, np.int32)
@@ -781,6 +826,20 @@ class VertexQuadraticEnergy:
 
     def setT1Threshold(self, t):
         self.T1Threshold = t
+
+    def spatialSorting(self):
+        """Vertices along a Hilbert curve, cells by first appearance around them,
+        on the device (reference vertexModelBase::spatialSorting,
+        vertexModelBase.cpp:236-242). ittVertex / itt: new -> old of the last
+        sort; idxToTag(Vertex): current index -> label at creation."""
+        vp, cp = self.batch.spatial_sort()
+        self.ittVertex, self.itt = vp, cp
+        if not hasattr(self, "idxToTagVertex") or len(self.idxToTagVertex) != len(vp):
+            self.idxToTagVertex = np.arange(len(vp), dtype=np.int32)
+        if not hasattr(self, "idxToTag") or len(self.idxToTag) != len(cp):
+            self.idxToTag = np.arange(len(cp), dtype=np.int32)
+        self.idxToTagVertex = self.idxToTagVertex[vp]
+        self.idxToTag = self.idxToTag[cp]
 
     def setDeltaT(self, dt):
         self.deltaT = dt

@@ -187,6 +187,10 @@ def lib():
         "cgpu_vm_fire_create": (vp, [vp]),
         "cgpu_vm_set_topology": (i, [vp, vp, vp, vp, vp, i]),
         "cgpu_vm_cell_events": (i, [vp, i, vp, vp, vp, vp, vp]),
+        "cgpu_vm_spatial_sort": (i, [vp, vp, vp]),
+        "cgpu_vor_sort_workspace": (C.c_size_t, [i]),
+        "cgpu_vor_hilbert_order": (i, [P(VoronoiBatchView), vp, vp, C.c_size_t, vp]),
+        "cgpu_vor_permute_cells": (i, [P(VoronoiBatchView), vp, vp, vp]),
         "cgpu_vm_set_active": (i, [vp, vp]),
         "cgpu_vm_get_active": (i, [vp, vp]),
         "cgpu_fire_set_param": (i, [vp, i, d]),

@@ -559,6 +559,30 @@ class VertexBatch {
     }
     int cell_events(int count, const int *kind, const int *cell, const int *p1, const int *p2,
                     const double *new_director);
+    // reference vertexModelBase::spatialSorting (vertexModelBase.cpp:236-242,
+    // Simple2DCell.cpp:484-599): vertices along a Hilbert curve, cells by first
+    // appearance around them, every per-vertex and per-cell array re-indexed on
+    // the device; vperm[new] = old vertex, cperm[new] = old cell (either may be
+    // NULL). The device arrays keep their addresses, so captured graphs stay valid.
+    int spatial_sort(int *vperm_out, int *cperm_out) {
+        cgpu_vertex_batch b = view();
+        const size_t need = std::max(cgpu_vm_sort_workspace(Nv_, Nc_), cgpu_vm_permute_workspace(Nv_, Nc_));
+        if (sort_work_.n < need) sort_work_.alloc(need);
+        if (sort_perm_.n < (size_t)Nv_ + Nc_) sort_perm_.alloc((size_t)Nv_ + Nc_);
+        int *vp = sort_perm_.p, *cp = sort_perm_.p + Nv_;
+        int r = cgpu_vm_hilbert_order(&b, vp, cp, sort_work_.p, sort_work_.n, stream_);
+        if (!r) r = cgpu_vm_permute(&b, vp, cp, sort_work_.p, sort_work_.n, stream_);
+        if (!r) r = cgpu_vm_geometry(&b, stream_);
+        if (r) return r;
+        if (vperm_out)
+            hip_check(hipMemcpyAsync(vperm_out, vp, sizeof(int) * (size_t)Nv_, hipMemcpyDeviceToHost, stream_),
+                      "download");
+        if (cperm_out)
+            hip_check(hipMemcpyAsync(cperm_out, cp, sizeof(int) * (size_t)Nc_, hipMemcpyDeviceToHost, stream_),
+                      "download");
+        sync();
+        return CGPU_OK;
+    }
     void set_active(const int *active) {
         sync();
         std::vector<cgpu_sys_state> st(S_);
@@ -620,7 +644,8 @@ class VertexBatch {
     DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_;
     DevBuf<unsigned long long> cell_min_;
     DevBuf<double> pos0_, pos1_, noise_;
-    DevBuf<unsigned char> AP_, dAP_, APpref_, force_, theta_, mot_;
+    DevBuf<unsigned char> AP_, dAP_, APpref_, force_, theta_, mot_, sort_work_;
+    DevBuf<int> sort_perm_;
 };
 
 // FIRE on a vertex batch: the vertices are the degrees of freedom and the T1
@@ -985,6 +1010,9 @@ int cgpu_vm_get_active(void *h, int *active) {
 int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, const int *p1, const int *p2,
                         const double *new_director) {
     return vguard([&] { return ((VertexBatch *)h)->cell_events(count, kind, cell, p1, p2, new_director); });
+}
+int cgpu_vm_spatial_sort(void *h, int *vperm, int *cperm) {
+    return vguard([&] { return ((VertexBatch *)h)->spatial_sort(vperm, cperm); });
 }
 int cgpu_vm_view(void *h, cgpu_vertex_batch *out) {
     *out = ((VertexBatch *)h)->view();

@@ -984,97 +984,33 @@ double VoronoiBatch::energy(int s) {
     return E;
 }
 
-// Hilbert index of cell (x, y) on a 2^k x 2^k grid (the classical rotate-and-
-// reflect construction of the curve, D. Hilbert 1891; iterative form)
-static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
-    uint64_t d = 0;
-    for (uint32_t s = n / 2; s > 0; s /= 2) {
-        uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
-        d += (uint64_t)s * s * ((3 * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) {
-                x = s - 1 - x;
-                y = s - 1 - y;
-            }
-            uint32_t t = x;
-            x = y;
-            y = t;
-        }
-    }
-    return d;
-}
-
+// Keys, sort and re-indexing run on the device (cgpu_vor_hilbert_order,
+// cgpu_vor_permute_cells); the cell list's scratch arrays, rebuilt by the
+// retriangulation, hold the permutation (sorted_id) and the gather staging
+// (sorted_pos, 16 bytes per cell). Only the permutation comes to the host, for
+// the tags and the host copy of the cell types.
 int VoronoiBatch::spatial_sort(int *perm_out) {
-    sync();
-    const size_t N = N_;
-    std::vector<double> xy(2 * N), th(N), vel(2 * N), pref(2 * N), mot(2 * N);
-    (parity_ ? pos1_ : pos0_).download(xy.data(), xy.size());
-    real_download(theta_.p, th.data(), N);
-    real_download(vel_.p, vel.data(), 2 * N);
-    real_download(APpref_.p, pref.data(), 2 * N);
-    real_download(mot_.p, mot.data(), 2 * N);
-    std::vector<int> ex;
-    if (excl_.p) {
-        ex.resize(N);
-        excl_.download(ex.data(), N);
-    }
-    std::vector<int> perm(N);
-    for (int s = 0; s < S_; ++s) {
-        const cgpu_sys_params &p = hp_[s];
-        uint32_t g = 1;
-        while (g * g < (uint32_t)p.n) g *= 2;  // about one cell per Hilbert pixel
-        std::vector<std::pair<uint64_t, int>> key(p.n);
-        for (int i = 0; i < p.n; ++i) {
-            int c = p.off + i;
-            uint32_t gx = std::min<uint32_t>(g - 1, (uint32_t)(xy[2 * c] / p.L * g));
-            uint32_t gy = std::min<uint32_t>(g - 1, (uint32_t)(xy[2 * c + 1] / p.L * g));
-            key[i] = {hilbert_index(g, gx, gy), c};
-        }
-        std::stable_sort(key.begin(), key.end(),
-                         [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) { return a.first < b.first; });
-        for (int i = 0; i < p.n; ++i) perm[p.off + i] = key[i].second;
-    }
-    auto apply2 = [&](std::vector<double> &a) {
-        std::vector<double> b(a.size());
-        for (size_t i = 0; i < N; ++i) {
-            b[2 * i] = a[2 * perm[i]];
-            b[2 * i + 1] = a[2 * perm[i] + 1];
-        }
-        a.swap(b);
-    };
-    apply2(xy);
-    apply2(vel);
-    apply2(pref);
-    apply2(mot);
+    cgpu_voronoi_batch b = view();
+    const size_t wb = cgpu_vor_sort_workspace(N_);
+    if (sort_work_.n < wb) sort_work_.alloc(wb);
+    int r = cgpu_vor_hilbert_order(&b, sorted_id_.p, sort_work_.p, sort_work_.n, stream_);
+    if (!r) r = cgpu_vor_permute_cells(&b, sorted_id_.p, sorted_pos_.p, stream_);
+    if (r) return r;
+    std::vector<int> perm(N_);
+    hip_check(hipMemcpyAsync(perm.data(), sorted_id_.p, sizeof(int) * (size_t)N_, hipMemcpyDeviceToHost, stream_),
+              "download");
+    hip_check(hipStreamSynchronize(stream_), "sync");
     {
-        std::vector<double> b(N);
-        for (size_t i = 0; i < N; ++i) b[i] = th[perm[i]];
-        th.swap(b);
-    }
-    {
-        std::vector<int> b(N);
-        for (size_t i = 0; i < N; ++i) b[i] = tag_[perm[i]];
-        tag_.swap(b);
-    }
-    (parity_ ? pos1_ : pos0_).upload(xy.data(), xy.size());
-    real_upload(theta_.p, th.data(), N);
-    real_upload(vel_.p, vel.data(), 2 * N);
-    real_upload(APpref_.p, pref.data(), 2 * N);
-    real_upload(mot_.p, mot.data(), 2 * N);
-    if (excl_.p) {
-        std::vector<int> b(N);
-        for (size_t i = 0; i < N; ++i) b[i] = ex[perm[i]];
-        excl_.upload(b.data(), N);
+        std::vector<int> t(N_);
+        for (int i = 0; i < N_; ++i) t[i] = tag_[perm[i]];
+        tag_.swap(t);
     }
     if (!h_type_.empty()) {
-        std::vector<int> b(N);
-        for (size_t i = 0; i < N; ++i) b[i] = h_type_[perm[i]];
-        h_type_ = b;
-        cell_type_.upload(b.data(), N);
+        std::vector<int> t(N_);
+        for (int i = 0; i < N_; ++i) t[i] = h_type_[perm[i]];
+        h_type_.swap(t);
     }
-    if (perm_out)
-        for (size_t i = 0; i < N; ++i) perm_out[i] = perm[i];
-    drop_graph();
+    if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
     return global_triangulation();
 }
 

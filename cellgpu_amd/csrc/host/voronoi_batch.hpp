@@ -198,7 +198,7 @@ class VoronoiBatch {
     DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_;
     DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_, ld_tri_, bin_mark_wide_;
     DevBuf<float> cell_rmax_;
-    DevBuf<unsigned char> ld_regions_;
+    DevBuf<unsigned char> ld_regions_, sort_work_;
     DevBuf<unsigned long long> ld_part_;
     DevBuf<int> ld_rule_;
     int ld_region_cap_ = 0, ld_tri_cap_ = 0;

@@ -417,6 +417,35 @@ int cgpu_vm_fire_park_converged(const cgpu_vertex_batch *b, const cgpu_fire_stat
 int cgpu_vm_fire_iteration(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, double *dots, void *stream);
 int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void *vel, void *stream);
 
+/* Spatial sorting on the device, replacing Simple2DCell::spatiallySortCells and
+ * spatiallySortVertices with the HilbertSorter (reference Simple2DCell.cpp:438-599,
+ * inc/utility/HilbertSort.h:27-104). A system of box side L uses the reference's
+ * curve order M = min(30, mm + 4), 2^mm >= L (capped at 20), and its pixel
+ * (floor(2^M x / L), floor(2^M y / L)); ties keep the current order. Nothing is
+ * allocated: work is caller-owned device memory of the queried size.
+ *
+ * cgpu_vor_hilbert_order: perm[new] = old cell index (global indices; cells
+ * stay inside their system's range). cgpu_vor_permute_cells: re-indexes the
+ * per-cell arrays of the view in place (pos, theta, vel, APpref, mot, force and,
+ * when present, ext_force, excl, cell_type) through tmp (16 num_cells bytes).
+ * The rings then refer to old indices: rebuild them with
+ * cgpu_global_triangulation, as the reference's spatialSorting does
+ * (voronoiModelBase.cpp:669-680). */
+size_t cgpu_vor_sort_workspace(int num_cells);
+int cgpu_vor_hilbert_order(const cgpu_voronoi_batch *b, int *perm, void *work, size_t work_bytes, void *stream);
+int cgpu_vor_permute_cells(const cgpu_voronoi_batch *b, const int *perm, void *tmp, void *stream);
+/* Vertex model (vertexModelBase::spatialSorting, vertexModelBase.cpp:236-242):
+ * vertices along the curve (vperm[new] = old), then cells in order of first
+ * appearance around the sorted vertices (cperm[new] = old). cgpu_vm_permute
+ * re-indexes positions, forces, vn / vcn, cvn / cv (ids mapped, list order
+ * kept), APpref, AP, dAP, theta and motility in place. */
+size_t cgpu_vm_sort_workspace(int num_vertices, int num_cells);
+int cgpu_vm_hilbert_order(const cgpu_vertex_batch *b, int *vperm, int *cperm, void *work, size_t work_bytes,
+                          void *stream);
+size_t cgpu_vm_permute_workspace(int num_vertices, int num_cells);
+int cgpu_vm_permute(const cgpu_vertex_batch *b, const int *vperm, const int *cperm, void *work, size_t work_bytes,
+                    void *stream);
+
 /* ---------------------------------------------------------------- object layer */
 
 /* A batch of independent Voronoi systems (one system = the reference's
@@ -586,6 +615,12 @@ int cgpu_vm_set_topology(void *h, const int *vn, const int *vcn, const int *cvn,
  * again, so views and graphs taken before are invalid. */
 int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, const int *p1, const int *p2,
                         const double *new_director);
+/* reference vertexModelBase::spatialSorting (vertexModelBase.cpp:236-242,
+ * Simple2DCell.cpp:484-599): vertices relabelled along a Hilbert curve, cells in
+ * order of first appearance around the sorted vertices, all state re-indexed on
+ * the device (cgpu_vm_hilbert_order, cgpu_vm_permute) and the geometry
+ * recomputed. vperm / cperm (either may be NULL) receive new -> old. */
+int cgpu_vm_spatial_sort(void *h, int *vperm, int *cperm);
 /* cells and vertices per system (either may be NULL); returns the total vertices */
 int cgpu_vm_get_sizes(void *h, int *nc, int *nv);
 /* The batch's device arrays as a kernel-layer view, and the stream its work

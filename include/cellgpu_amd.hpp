@@ -361,6 +361,15 @@ class voronoiModelBase : public Simple2DActiveCell {
         tti.assign(Ncells, 0);
         for (int i = 0; i < Ncells; ++i) tti[itt[i]] = i;
         refreshTags();
+        reIndex(prefs_);
+        reIndex(motility_);
+    }
+    //! reference reIndexCellArray (Simple2DCell.h): a host copy follows the sort
+    template <typename T> void reIndex(vector<T> &a) const {
+        if (a.size() != itt.size()) return;
+        vector<T> b(a.size());
+        for (size_t i = 0; i < a.size(); ++i) b[i] = a[itt[i]];
+        a.swap(b);
     }
     void setCellPositions(vector<Dscalar2> newCellPositions) {
         vector<double> x(Ncells), y(Ncells);
@@ -681,9 +690,24 @@ class vertexModelBase : public Simple2DActiveCell {
         return m;
     }
     void reportMeanCellForce(bool) override { reportMeanVertexForce(false); }
-    //! vertex model: the vertex ids follow the Delaunay construction; no
-    //! relabelling is done
-    void spatialSorting() override {}
+    //! reference vertexModelBase::spatialSorting (vertexModelBase.cpp:236-242):
+    //! vertices along a Hilbert curve, cells by first appearance around them,
+    //! re-indexed on the device; ittVertex / itt hold new -> old of this sort
+    void spatialSorting() override {
+        vector<int> vp(Nvertices), cp(Ncells);
+        check(cgpu_vm_spatial_sort(h_, vp.data(), cp.data()), "spatialSorting");
+        compose(vp, ittVertex, ttiVertex, idxToTagVertex, tagToIdxVertex);
+        compose(cp, itt, tti, idxToTag, tagToIdx);
+        for (vector<Dscalar2> *a : {&prefs_, &motility_}) {
+            if (a->size() != cp.size()) continue;
+            vector<Dscalar2> b(a->size());
+            for (size_t i = 0; i < b.size(); ++i) b[i] = (*a)[cp[i]];
+            a->swap(b);
+        }
+    }
+    //! index maps of the last vertex sort and vertex labels at creation
+    //! (reference Simple2DCell.h:253-262)
+    vector<int> ittVertex, ttiVertex, tagToIdxVertex, idxToTagVertex;
     //! reference cellDeath (T2, vertexModelBase.cpp:905-1110) / cellDivision
     //! (:1124-1385): parameters = {cell, vertex-list position 1, position 2}
     virtual void cellDeath(int cellIndex) {
@@ -700,6 +724,26 @@ class vertexModelBase : public Simple2DActiveCell {
     }
     void *handle() { return h_; }
 
+   private:
+    static void compose(const vector<int> &perm, vector<int> &it, vector<int> &ti, vector<int> &i2t,
+                        vector<int> &t2i) {
+        const int n = (int)perm.size();
+        if ((int)i2t.size() != n) {
+            i2t.resize(n);
+            for (int i = 0; i < n; ++i) i2t[i] = i;
+        }
+        it = perm;
+        ti.assign(n, 0);
+        vector<int> old = i2t;
+        t2i.assign(n, 0);
+        for (int i = 0; i < n; ++i) {
+            ti[perm[i]] = i;
+            i2t[i] = old[perm[i]];
+            t2i[i2t[i]] = i;
+        }
+    }
+
+   public:
     //! vertexNeighbors[3 v + k], vertexCellNeighbors[3 v + k] (reference
     //! Simple2DCell.h:168-189); cellVertexNum[c], cellVertices[c * vertexMax + k]
     GPUArray<int> vertexNeighbors, vertexCellNeighbors, cellVertexNum, cellVertices;

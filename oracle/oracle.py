@@ -431,3 +431,71 @@ def vertex_from_random(Nc, seed=13377, **kw):
     x, y, th, L = random_init(Nc, seed)
     nn, nbr, _ = triangulate(x, y, L, cap=32)
     return VertexSystem(x, y, L, nn, nbr, **kw), (x, y, L, nn, nbr)
+
+
+# ------------------------------------------------------------ spatial sorting
+def hilbert_curve_order(L):
+    """Curve order of the reference HilbertSorter for a square box of side L
+    (inc/utility/HilbertSort.h:33-44): the smallest mm >= 1 with 2^mm >= L,
+    order min(30, mm + 4)."""
+    mm, t = 1, 2
+    while float(t) < L:
+        t *= 2
+        mm += 1
+    return min(30, mm + 4)
+
+
+def hilbert_xy2d(M, x, y):
+    """Index along the order-M Hilbert curve of pixel (x, y), the iterative
+    rotate-and-reflect construction the reference calls (xy2d in
+    HilbertSort.h:95-101); vectorised over integer arrays."""
+    x = np.asarray(x, np.int64).copy()
+    y = np.asarray(y, np.int64).copy()
+    d = np.zeros(np.broadcast(x, y).shape, np.int64)
+    s = 1 << (M - 1)
+    while s > 0:
+        rx = ((x & s) > 0).astype(np.int64)
+        ry = ((y & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        flip = (ry == 0) & (rx == 1)
+        x = np.where(flip, s - 1 - x, x)
+        y = np.where(flip, s - 1 - y, y)
+        swap = ry == 0
+        x, y = np.where(swap, y, x), np.where(swap, x, y)
+        s >>= 1
+    return d
+
+
+def hilbert_keys(x, y, L):
+    """HilbertSorter::getIdx (HilbertSort.h:88-101) of points in a square box:
+    the general-box inverse transform (1 / (L L)) L times the coordinate
+    (gpubox.h:86-101, 117-129), then floor(2^M v)."""
+    M = hilbert_curve_order(L)
+    inv = (1.0 / (L * L)) * L
+    n = float(1 << M)
+    px = np.floor(n * (inv * np.asarray(x, np.float64))).astype(np.int64)
+    py = np.floor(n * (inv * np.asarray(y, np.float64))).astype(np.int64)
+    return hilbert_xy2d(M, px, py)
+
+
+def hilbert_sort_order(x, y, L):
+    """new -> old order of Simple2DCell::spatiallySortCells
+    (Simple2DCell.cpp:438-477): std::sort of (index, i) pairs."""
+    k = hilbert_keys(x, y, L)
+    return np.lexsort((np.arange(len(k)), k)).astype(np.int32)
+
+
+def vertex_sort_orders(vx, vy, vcn, L):
+    """Simple2DCell::spatiallySortVertices (Simple2DCell.cpp:484-599): vertices
+    by curve index, then cells numbered by first appearance walking the sorted
+    vertices and each vertex's three cells (vcn, 3 per vertex, old indices).
+    Returns (vertex new -> old, cell new -> old)."""
+    vperm = hilbert_sort_order(vx, vy, L)
+    vcn = np.asarray(vcn).reshape(-1, 3)
+    order, seen = [], set()
+    for v in vperm:
+        for c in vcn[v]:
+            if int(c) not in seen:
+                seen.add(int(c))
+                order.append(int(c))
+    return vperm, np.asarray(order, np.int32)
