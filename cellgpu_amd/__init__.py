@@ -17,7 +17,7 @@ __all__ = [
     "selfPropelledCellVertexDynamics", "brownianParticleDynamics", "EnergyMinimizerFIRE", "RING_CAP",
 ]
 
-RING_CAP = 24
+RING_CAP = 64  # largest ring capacity (CGPU_RING_CAP_MAX): getters copy up to this many slots
 
 
 def _p(a):
@@ -134,8 +134,13 @@ class VoronoiBatch:
         oth = None
         if del_other is not None:
             oth = np.ascontiguousarray(del_other, np.int32)
-            if oth.shape != nbr.shape:
-                raise ValueError("del_other must have the layout of nbr")
+            if oth.ndim != 2 or oth.shape[0] != nbr.shape[0]:
+                raise ValueError("del_other must have one row per cell, as nbr")
+            if oth.shape[1] != nbr.shape[1]:  # the slots past a degree are unused
+                w = min(oth.shape[1], nbr.shape[1])
+                o2 = np.full(nbr.shape, -1, np.int32)
+                o2[:, :w] = oth[:, :w]
+                oth = o2
         check(lib().cgpu_vor_set_topology(self.h, _p(nn), _p(nbr), int(nbr.shape[1]), _p(oth)), "set_topology")
 
     def del_other(self, cap=RING_CAP):
@@ -278,6 +283,10 @@ class VoronoiBatch:
         check(lib().cgpu_vor_spv_steps_profiled(self.h, int(steps), _p(ms), _p(q)), "profiled steps")
         names = [lib().cgpu_profile_slot_name(k).decode() for k in range(PROFILE_SLOTS)]
         return dict(zip(names, ms.tolist())), q.tolist()
+
+    def ring_capacity(self):
+        """ring slots per cell now allocated (grows when a star needs more)."""
+        return int(lib().cgpu_vor_ring_capacity(self.h))
 
     def spatial_sort(self):
         """Relabel cells along a Hilbert curve (per system) and retriangulate.
@@ -790,9 +799,14 @@ class VertexBatch:
         vn = np.zeros(3 * self.Nv, np.int32)
         vcn = np.zeros(3 * self.Nv, np.int32)
         cvn = np.zeros(self.Nc, np.int32)
-        cv = np.full(self.Nc * self.VCAP, -1, np.int32)
-        check(lib().cgpu_vm_get_topology(self.h, _p(vn), _p(vcn), _p(cvn), _p(cv), self.VCAP), "vm_get_topology")
-        return vn, vcn, cvn, cv.reshape(self.Nc, self.VCAP)
+        cap = max(self.VCAP, self.vertex_capacity())
+        cv = np.full(self.Nc * cap, -1, np.int32)
+        check(lib().cgpu_vm_get_topology(self.h, _p(vn), _p(vcn), _p(cvn), _p(cv), cap), "vm_get_topology")
+        return vn, vcn, cvn, cv.reshape(self.Nc, cap)
+
+    def vertex_capacity(self):
+        """cell-list slots per cell now allocated (grows when a flip needs more)."""
+        return int(lib().cgpu_vm_vertex_capacity(self.h))
 
     def counters(self, s=0):
         out = np.zeros(4, np.int64)

@@ -59,7 +59,7 @@ class VertexBatchView(C.Structure):
         (name, C.c_void_p) for name in (
             "sys", "st", "vert_sys", "cell_sys", "pos", "pos_next", "vn", "vcn", "cvn", "cv", "AP", "dAP", "APpref",
             "force", "flag_count", "flag_list", "flag_cells", "flag_touched", "cell_min", "noise", "theta",
-            "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("pad_c", C.c_int)]
+            "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("vertex_cap", C.c_int)]
 
 
 PROFILE_SLOTS = 17  # CGPU_PROFILE_SLOTS
@@ -81,7 +81,7 @@ class VoronoiBatchView(C.Structure):
             "sorted_pos", "repair", "mark", "flagged", "nfq", "gq",
             "counters", "noise", "chunk_sys", "chunk_begin", "chunk_end", "bchunk_sys", "bchunk_begin",
             "bchunk_end", "sys_bchunk", "bchunk_sum")] + [("num_chunks", C.c_int), ("num_bin_chunks", C.c_int)] + [
-        ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("pad_t", C.c_int),
+        ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("ring_cap", C.c_int),
         ("ld_tri", C.c_void_p), ("bin_mark_wide", C.c_void_p), ("cell_rmax", C.c_void_p),
         ("ld_regions", C.c_void_p), ("ld_region_cap", C.c_int), ("ld_tri_cap", C.c_int),
         ("ld_part", C.c_void_p), ("ld_rule", C.c_void_p),
@@ -188,6 +188,9 @@ def lib():
         "cgpu_vm_set_topology": (i, [vp, vp, vp, vp, vp, i]),
         "cgpu_vm_cell_events": (i, [vp, i, vp, vp, vp, vp, vp]),
         "cgpu_vm_spatial_sort": (i, [vp, vp, vp]),
+        "cgpu_vor_ring_capacity": (i, [vp]),
+        "cgpu_vm_vertex_capacity": (i, [vp]),
+        "cgpu_global_triangulation_active": (i, [P(VoronoiBatchView), vp]),
         "cgpu_vor_sort_workspace": (C.c_size_t, [i]),
         "cgpu_vor_hilbert_order": (i, [P(VoronoiBatchView), vp, vp, C.c_size_t, vp]),
         "cgpu_vor_permute_cells": (i, [P(VoronoiBatchView), vp, vp, vp]),

@@ -101,6 +101,8 @@ int FireBase::minimize() {
             if ((r = iterate())) return r;
         sync();
         fs_.download(hs_.data(), S_);
+        if ((r = recover_stalled())) return r;
+        fs_.download(hs_.data(), S_);
         done = std::all_of(hs_.begin(), hs_.end(), [](const cgpu_fire_state &f) { return f.converged != 0; });
         if ((r = status())) return r;
     }
@@ -157,6 +159,7 @@ class VoronoiFire : public FireBase {
     }
     void sync() override { vb_->sync(); }
     int status() override { return vb_->status(); }
+    int recover_stalled() override { return vb_->finish_stalled_fire(fs_.p, vel_.p, dots_.p); }
     size_t dof() override { return vb_->num_cells(); }
     size_t chunks() override { return vb_->view().num_chunks; }
 

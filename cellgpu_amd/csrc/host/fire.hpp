@@ -31,6 +31,9 @@ class FireBase {
     virtual int apply_mix() = 0;  // the pending velocity mixing
     virtual void sync() = 0;
     virtual int status() = 0;     // after a sync: first error of the batch
+    // systems that stalled on a storage capacity inside an iteration: grown and
+    // finished (default: none can stall)
+    virtual int recover_stalled() { return 0; }
     virtual size_t dof() = 0;     // degrees of freedom of the model now
     virtual size_t chunks() = 0;  // reduction chunks of the model now
     int S_;

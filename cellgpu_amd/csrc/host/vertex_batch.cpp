@@ -150,7 +150,7 @@ class VertexBatch {
                 throw std::runtime_error("spv initialisation failed");
             vor.get(px.data(), py.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         }
-        const int cap = CGPU_RING_CAP;
+        const int cap = vor.ring_capacity();
         std::vector<int> nn(Nc), nbr((size_t)Nc * cap);
         vor.get_topology(nn.data(), nbr.data(), cap, nullptr);
 
@@ -223,9 +223,9 @@ class VertexBatch {
             }
             for (int c = p.coff; c < p.coff + p.nc; ++c) csys[c] = s;
         }
-        std::vector<int> cvn(Nc_), cv((size_t)CGPU_VERTEX_CAP * Nc_, -1);
+        for (int i = 0; i < Nc_; ++i) vertex_cap_ = std::max(vertex_cap_, (nn[i] + 1) & ~1);
+        std::vector<int> cvn(Nc_), cv((size_t)vertex_cap_ * Nc_, -1);
         for (int i = 0; i < Nc_; ++i) {
-            if (nn[i] > CGPU_VERTEX_CAP) throw std::runtime_error("cell with more vertices than the capacity");
             cvn[i] = nn[i];
             for (int k = 0; k < nn[i]; ++k)
                 cv[(size_t)k * Nc_ + i] =
@@ -287,7 +287,7 @@ class VertexBatch {
         vn_.alloc(3 * (size_t)Nv_);
         vcn_.alloc(3 * (size_t)Nv_);
         cvn_.alloc(Nc_);
-        cv_.alloc((size_t)CGPU_VERTEX_CAP * Nc_);
+        cv_.alloc((size_t)vertex_cap_ * Nc_);
         AP_.alloc(2 * Nc_ * rsz_);
         dAP_.alloc(2 * Nc_ * rsz_);
         APpref_.alloc(2 * Nc_ * rsz_);
@@ -355,6 +355,7 @@ class VertexBatch {
         b.chunk_begin = chunk_begin_.p;
         b.chunk_end = chunk_end_.p;
         b.num_chunks = num_chunks_;
+        b.vertex_cap = vertex_cap_;
         return b;
     }
     void set_params(int s, double KA, double KP, double dt, double mu, double T, double thr) {
@@ -446,7 +447,107 @@ class VertexBatch {
         parity_ ^= 1;
         return r;
     }
+    int vertex_capacity() const { return vertex_cap_; }
+    // Cell-list storage of cap slots per cell: slot-major rows keep their place
+    // (new rows -1); a captured step graph is dropped (it holds the old address).
+    void grow_vertex_capacity(int cap) {
+        if (cap <= vertex_cap_) return;
+        sync();
+        DevBuf<int> g;
+        g.alloc((size_t)cap * Nc_);
+        hip_check(hipMemset(g.p, 0xff, sizeof(int) * (size_t)cap * Nc_), "grow cell lists");
+        hip_check(hipMemcpy(g.p, cv_.p, sizeof(int) * (size_t)vertex_cap_ * Nc_, hipMemcpyDeviceToDevice),
+                  "grow cell lists");
+        cv_.swap(g);
+        vertex_cap_ = cap;
+        drop_graph();
+    }
+    // Systems stalled by a T1 flip beyond the cell-list capacity (error
+    // CGPU_ERR_RING_CAPACITY, inactive). False when none.
+    bool take_stalled(std::vector<int> &stalled) {
+        sync();
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        stalled.clear();
+        for (int s = 0; s < S_; ++s)
+            if (st[s].error == CGPU_ERR_RING_CAPACITY && !st[s].active) stalled.push_back(s);
+        return !stalled.empty();
+    }
+    // Finish stalled systems as the reference does when a flip needs a longer
+    // cell list (growCellVerticesList, then the flips, vertexModelBase.cpp:
+    // 815-824): grown storage, the T1 pass of the stalled step on its moved
+    // positions, with the other systems parked. complete: the rest of the
+    // stalled step or iteration, run for the stalled systems alone.
+    template <typename F> int finish_stalled(F &&complete) {
+        std::vector<int> stalled;
+        while (take_stalled(stalled)) {
+            grow_vertex_capacity(vertex_cap_ + 8);
+            std::vector<cgpu_sys_state> st(S_);
+            st_.download(st.data(), S_);
+            const std::vector<cgpu_sys_state> saved = st;
+            for (int s = 0; s < S_; ++s) st[s].active = 0;
+            for (int s : stalled) {
+                st[s].error = 0;
+                st[s].active = 1;
+            }
+            st_.upload(st.data(), S_);
+            cgpu_vertex_batch b = view();
+            if (int r = cgpu_vm_t1(&b, stream_)) return r;
+            sync();
+            if (int r = complete(stalled, saved)) return r;
+            sync();
+            st_.download(st.data(), S_);
+            for (int s = 0; s < S_; ++s)
+                if (std::find(stalled.begin(), stalled.end(), s) == stalled.end()) {
+                    st[s].active = saved[s].active;
+                    st[s].steps_left = saved[s].steps_left;
+                }
+            st_.upload(st.data(), S_);
+        }
+        return CGPU_OK;
+    }
+    // steps: the stalled step is counted, then the steps still owed run for the
+    // stalled systems alone (a countdown system keeps its countdown instead)
     int run_steps(int kind, int steps, bool use_graph) {
+        if (steps <= 0) return CGPU_OK;
+        std::vector<cgpu_sys_state> before(S_);
+        if (st_call_.n != st_.n) st_call_.alloc(st_.n);
+        hip_check(hipMemcpyAsync(st_call_.p, st_.p, sizeof(cgpu_sys_state) * S_, hipMemcpyDeviceToDevice, stream_),
+                  "state copy");
+        if (int r = launch_steps(kind, steps, use_graph)) return r;
+        return finish_stalled([&](const std::vector<int> &stalled, const std::vector<cgpu_sys_state> &saved) {
+            st_call_.download(before.data(), S_);
+            std::vector<cgpu_sys_state> st(S_);
+            st_.download(st.data(), S_);
+            int owe = 0;
+            for (int s : stalled) {
+                st[s].step += 1;
+                if (saved[s].steps_left >= 0) {
+                    st[s].steps_left = saved[s].steps_left > 0 ? saved[s].steps_left - 1 : 0;
+                    if (st[s].steps_left == 0 && saved[s].steps_left > 0) st[s].active = 0;
+                    continue;
+                }
+                const int left = steps - (st[s].step - before[s].step);
+                if (left > 0) {
+                    st[s].steps_left = left;
+                    owe = std::max(owe, left);
+                }
+            }
+            st_.upload(st.data(), S_);
+            for (int k = 0; k < owe; ++k)
+                if (int r = step_launch(kind)) return r;
+            sync();
+            st_.download(st.data(), S_);
+            for (int s : stalled)
+                if (saved[s].steps_left < 0 && st[s].error == 0) {
+                    st[s].active = 1;
+                    st[s].steps_left = -1;
+                }
+            st_.upload(st.data(), S_);
+            return (int)CGPU_OK;
+        });
+    }
+    int launch_steps(int kind, int steps, bool use_graph) {
         if (!use_graph || noise_buffer_) {
             for (int k = 0; k < steps; ++k)
                 if (int r = step_launch(kind)) return r;
@@ -514,7 +615,7 @@ class VertexBatch {
         sync();
         vn_.download(vn, 3 * (size_t)Nv_);
         vcn_.download(vcn, 3 * (size_t)Nv_);
-        std::vector<int> n(Nc_), c((size_t)CGPU_VERTEX_CAP * Nc_);
+        std::vector<int> n(Nc_), c((size_t)vertex_cap_ * Nc_);
         cvn_.download(n.data(), Nc_);
         cv_.download(c.data(), c.size());
         for (int i = 0; i < Nc_; ++i) {
@@ -530,8 +631,7 @@ class VertexBatch {
         csys_.download(csys.data(), Nc_);
         long long total = 0;
         for (int c = 0; c < Nc_; ++c) {
-            if (cvn[c] < 3 || cvn[c] > CGPU_VERTEX_CAP || cvn[c] > cap)
-                throw std::invalid_argument("cell vertex count outside [3, capacity]");
+            if (cvn[c] < 3 || cvn[c] > cap) throw std::invalid_argument("cell vertex count outside [3, capacity]");
             total += cvn[c];
             for (int k = 0; k < cvn[c]; ++k) {
                 const int v = cv[(size_t)c * cap + k];
@@ -549,10 +649,13 @@ class VertexBatch {
                 if (!in) throw std::invalid_argument("vertex missing from the list of one of its cells");
             }
         sync();
+        int longest = 0;
+        for (int c = 0; c < Nc_; ++c) longest = std::max(longest, cvn[c]);
+        if (longest > vertex_cap_) grow_vertex_capacity((longest + 1) & ~1);
         vn_.upload(vn, 3 * (size_t)Nv_);
         vcn_.upload(vcn, 3 * (size_t)Nv_);
         cvn_.upload(cvn, Nc_);
-        std::vector<int> sm((size_t)CGPU_VERTEX_CAP * Nc_, -1);
+        std::vector<int> sm((size_t)vertex_cap_ * Nc_, -1);
         for (int c = 0; c < Nc_; ++c)
             for (int k = 0; k < cvn[c]; ++k) sm[(size_t)k * Nc_ + c] = cv[(size_t)c * cap + k];
         cv_.upload(sm.data(), sm.size());
@@ -639,7 +742,8 @@ class VertexBatch {
     hipGraphExec_t graph_exec_ = nullptr;
     std::vector<cgpu_vm_params> hp_;
     DevBuf<cgpu_vm_params> sys_;
-    DevBuf<cgpu_sys_state> st_;
+    DevBuf<cgpu_sys_state> st_, st_call_;
+    int vertex_cap_ = CGPU_VERTEX_CAP;
     DevBuf<int> vsys_, csys_, vn_, vcn_, cvn_, cv_, flag_count_, flag_list_, flag_cells_, flag_touched_;
     DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_;
     DevBuf<unsigned long long> cell_min_;
@@ -675,6 +779,15 @@ class VertexFire : public FireBase {
     }
     void sync() override { vm_->sync(); }
     int status() override { return vm_->status(); }
+    // a system stalled inside an iteration: its T1 pass on grown cell lists,
+    // then the iteration's force half for it alone
+    int recover_stalled() override {
+        return vm_->finish_stalled([&](const std::vector<int> &, const std::vector<cgpu_sys_state> &) {
+            cgpu_vertex_batch b = vm_->view();
+            int r = cgpu_vm_fire_force_reduce(&b, fs_.p, vel_.p, dots_.p, vm_->stream());
+            return r ? r : cgpu_vm_fire_park_converged(&b, fs_.p, vm_->stream());
+        });
+    }
     size_t dof() override { return vm_->num_vertices(); }
     size_t chunks() override { return vm_->view().num_chunks; }
 
@@ -710,7 +823,8 @@ int VertexBatch::cell_events(int count, const int *kind, const int *cell, const 
         if (kind[e] == 1 && (!p1 || !p2)) throw std::invalid_argument("a division needs two list positions");
     }
     sync();
-    const int cap = CGPU_VERTEX_CAP;
+    // host lists may outgrow the device capacity, which grows to fit them
+    const int cap = std::max(vertex_cap_, 256);
     std::vector<double> vx(Nv_), vy(Nv_), pref(2 * (size_t)Nc_), th(Nc_), mot(2 * (size_t)Nc_);
     get(vx.data(), vy.data(), nullptr, nullptr, nullptr, nullptr);
     std::vector<int> vn(3 * (size_t)Nv_), vcn(3 * (size_t)Nv_), cvn(Nc_), cv((size_t)Nc_ * cap);
@@ -904,6 +1018,7 @@ int VertexBatch::cell_events(int count, const int *kind, const int *cell, const 
         nth.insert(nth.end(), lth.begin(), lth.end());
         nmot.insert(nmot.end(), lmot.begin(), lmot.end());
     }
+    for (int n : ncvn) vertex_cap_ = std::max(vertex_cap_, (n + 1) & ~1);
     layout();
     std::vector<double> xy(2 * (size_t)Nv_);
     for (int v = 0; v < Nv_; ++v) {
@@ -914,7 +1029,7 @@ int VertexBatch::cell_events(int count, const int *kind, const int *cell, const 
     vn_.upload(nvn.data(), nvn.size());
     vcn_.upload(nvcn.data(), nvcn.size());
     cvn_.upload(ncvn.data(), Nc_);
-    std::vector<int> sm((size_t)cap * Nc_, -1);
+    std::vector<int> sm((size_t)vertex_cap_ * Nc_, -1);
     for (int q = 0; q < Nc_; ++q)
         for (int k = 0; k < ncvn[q]; ++k) sm[(size_t)k * Nc_ + q] = ncv[(size_t)q * cap + k];
     cv_.upload(sm.data(), sm.size());
@@ -1011,6 +1126,7 @@ int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, co
                         const double *new_director) {
     return vguard([&] { return ((VertexBatch *)h)->cell_events(count, kind, cell, p1, p2, new_director); });
 }
+int cgpu_vm_vertex_capacity(void *h) { return ((VertexBatch *)h)->vertex_capacity(); }
 int cgpu_vm_spatial_sort(void *h, int *vperm, int *cperm) {
     return vguard([&] { return ((VertexBatch *)h)->spatial_sort(vperm, cperm); });
 }
@@ -1094,7 +1210,7 @@ extern "C" int cgpu_db_avm_write_batch(void *db, void *h, int s, double time, in
         VertexBatch &vm = *(VertexBatch *)h;
         if (s < 0 || s >= vm.num_systems()) throw std::invalid_argument("no such system");
         const SysRange r = sys_range(vm, s);
-        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = CGPU_VERTEX_CAP;
+        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = vm.vertex_capacity();
         std::vector<double> vx(Nv), vy(Nv), fx(Nv), fy(Nv), A(Nc), P(Nc), th(Nc);
         vm.get(vx.data(), vy.data(), fx.data(), fy.data(), A.data(), P.data());
         vm.get_directors(th.data());
@@ -1144,7 +1260,7 @@ extern "C" int cgpu_db_avm_read_batch(void *db, void *h, int s, int rec) {
         VertexBatch &vm = *(VertexBatch *)h;
         if (s < 0 || s >= vm.num_systems()) throw std::invalid_argument("no such system");
         const SysRange r = sys_range(vm, s);
-        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = CGPU_VERTEX_CAP;
+        const int Nv = vm.num_vertices(), Nc = vm.num_cells(), cap = std::max(vm.vertex_capacity(), 64);
         std::vector<double> lx(r.nv), ly(r.nv), box4(4), ld(r.nc);
         std::vector<int> lvn(3 * (size_t)r.nv), lvcn(3 * (size_t)r.nv);
         int rc = cgpu_db_avm_read(db, rec, nullptr, box4.data(), lx.data(), ly.data(), nullptr, nullptr, lvn.data(),

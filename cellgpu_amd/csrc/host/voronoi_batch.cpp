@@ -154,8 +154,8 @@ VoronoiBatch::VoronoiBatch(int num_systems, const int *n, const double *L, const
     }
     int r = global_triangulation();
     if (r == CGPU_ERR_RING_CAPACITY)
-        throw std::runtime_error("initial triangulation failed: a Delaunay degree exceeds the ring capacity (" +
-                                 std::to_string(CGPU_RING_CAP) + ")");
+        throw std::runtime_error("initial triangulation failed: a Delaunay degree exceeds the largest ring capacity (" +
+                                 std::to_string(CGPU_RING_CAP_MAX) + ")");
     if (r != CGPU_OK) throw std::runtime_error("initial triangulation failed, status " + std::to_string(r));
 }
 
@@ -203,7 +203,7 @@ VoronoiBatch *VoronoiBatch::merge(int count, VoronoiBatch *const *src, const int
                       "merge download");
             h.nn.resize(N);
             b->nn_.download(h.nn.data(), N);
-            h.nbr.resize((size_t)CGPU_RING_CAP * N);
+            h.nbr.resize((size_t)b->ring_cap_ * N);
             b->nbr_.download(h.nbr.data(), h.nbr.size());
             h.oth.resize(h.nbr.size());
             b->oth_.download(h.oth.data(), h.oth.size());
@@ -228,13 +228,14 @@ VoronoiBatch *VoronoiBatch::merge(int count, VoronoiBatch *const *src, const int
     }
 
     std::unique_ptr<VoronoiBatch> m(new VoronoiBatch(Empty{}, count, src[0]->prec_, src[0]->device_));
+    for (int k = 0; k < count; ++k) m->ring_cap_ = std::max(m->ring_cap_, src[k]->ring_cap_);
     std::vector<int> ns(count);
     for (int k = 0; k < count; ++k) {
         m->hp_[k] = src[k]->hp_[sys[k]];
         ns[k] = m->hp_[k].n;
     }
     m->layout(ns);  // offsets, tables, zeroed arrays, parameters
-    const size_t N = m->N_, rs = m->rsz_, CAP = CGPU_RING_CAP;
+    const size_t N = m->N_, rs = m->rsz_, CAP = m->ring_cap_;
     std::vector<double> pos(2 * N);
     std::vector<int> nn(N), nbr(CAP * N, 0), oth(CAP * N, 0), tag(N);
     std::vector<unsigned char> AP(2 * N * rs), dAP(2 * N * rs), pref(2 * N * rs), mot(2 * N * rs), vel(2 * N * rs),
@@ -323,8 +324,8 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
     bchunk_sum_.alloc(t.bs.size());
     cell_sys_.alloc(N_);
     nn_.alloc(N_);
-    nbr_.alloc((size_t)CGPU_RING_CAP * N_);
-    oth_.alloc((size_t)CGPU_RING_CAP * N_);
+    nbr_.alloc((size_t)ring_cap_ * N_);
+    oth_.alloc((size_t)ring_cap_ * N_);
     repair_.alloc(N_);
     mark_.alloc(N_);
     flagged_.alloc(N_);
@@ -438,6 +439,7 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.cell_type = tension_on_ ? cell_type_.p : nullptr;
     b.tension = tension_.p;
     b.num_types = ntypes_;
+    b.ring_cap = ring_cap_;
     b.ld_tri = ld_tri_.p;
     b.ld_tri_cap = ld_tri_cap_;
     b.bin_mark_wide = bin_mark_wide_.p;
@@ -619,7 +621,7 @@ void VoronoiBatch::set_positions(const double *x, const double *y, bool retri) {
 
 void VoronoiBatch::get_del_other(int *oth, int cap) {
     sync();
-    std::vector<int> hn(N_), ho((size_t)CGPU_RING_CAP * N_);
+    std::vector<int> hn(N_), ho((size_t)ring_cap_ * N_);
     nn_.download(hn.data(), N_);
     oth_.download(ho.data(), ho.size());
     for (int i = 0; i < N_; ++i)
@@ -628,7 +630,10 @@ void VoronoiBatch::get_del_other(int *oth, int cap) {
 
 void VoronoiBatch::set_topology(const int *nn, const int *nbr, int cap, const int *oth) {
     if (!nn || !nbr || cap <= 0) throw std::invalid_argument("set_topology needs nn, nbr and cap > 0");
-    std::vector<int> ring((size_t)CGPU_RING_CAP * N_, 0);
+    int zmax = 0;
+    for (int i = 0; i < N_; ++i) zmax = std::max(zmax, nn[i]);
+    if (zmax > ring_cap_ && zmax <= CGPU_RING_CAP_MAX && zmax <= cap) grow_ring_capacity(zmax);
+    std::vector<int> ring((size_t)ring_cap_ * N_, 0);
     std::vector<int> deg(N_);
     std::vector<cgpu_sys_state> st(S_);
     for (int s = 0; s < S_; ++s) {
@@ -637,7 +642,7 @@ void VoronoiBatch::set_topology(const int *nn, const int *nbr, int cap, const in
         int mx = 0;
         for (int i = p.off; i < p.off + p.n; ++i) {
             const int z = nn[i];
-            if (z < 3 || z > CGPU_RING_CAP || z > cap)
+            if (z < 3 || z > ring_cap_ || z > cap)
                 throw std::invalid_argument("degree of cell " + std::to_string(i) + " outside [3, capacity]");
             for (int k = 0; k < z; ++k) {
                 const int j = nbr[(size_t)i * cap + k];
@@ -671,7 +676,7 @@ void VoronoiBatch::set_topology(const int *nn, const int *nbr, int cap, const in
     if (oth) {
         // the caller's delOther (e.g. another batch's, including values a
         // repair kept because a neighbour's ring lacked the edge)
-        std::vector<int> o((size_t)CGPU_RING_CAP * N_, 0);
+        std::vector<int> o((size_t)ring_cap_ * N_, 0);
         for (int i = 0; i < N_; ++i)
             for (int k = 0; k < deg[i]; ++k) o[(size_t)k * N_ + i] = oth[(size_t)i * cap + k];
         oth_.upload(o.data(), o.size());
@@ -741,11 +746,143 @@ void VoronoiBatch::set_active(const int *active, const int *steps_left) {
     st_.upload(st.data(), S_);
 }
 
+// With growth: a star beyond the ring capacity stalls its system; the capacity
+// then doubles (up to CGPU_RING_CAP_MAX) and the whole triangulation is redone,
+// as the reference's globalTriangulationCGAL grows neighMax and its arrays
+// (voronoiModelBase.cpp:309-312).
 int VoronoiBatch::global_triangulation() {
+    for (;;) {
+        cgpu_voronoi_batch b = view();
+        int r = cgpu_global_triangulation(&b, stream_);
+        if (r) return r;
+        std::vector<int> stalled;
+        if (!take_stalled(stalled)) return status();
+        if (ring_cap_ >= CGPU_RING_CAP_MAX) return CGPU_ERR_RING_CAPACITY;
+        grow_ring_capacity(std::min(CGPU_RING_CAP_MAX, 2 * ring_cap_));
+        resume(stalled);
+    }
+}
+
+// Ring storage of cap slots per cell. Slot-major rows keep their place, so the
+// old rows are copied as one block; a captured step graph is dropped (it holds
+// the old addresses).
+void VoronoiBatch::grow_ring_capacity(int cap) {
+    if (cap <= ring_cap_) return;
+    if (cap > CGPU_RING_CAP_MAX) throw std::invalid_argument("ring capacity above CGPU_RING_CAP_MAX");
+    sync();
+    for (DevBuf<int> *a : {&nbr_, &oth_}) {
+        DevBuf<int> g;
+        g.alloc((size_t)cap * N_);
+        hip_check(hipMemcpy(g.p, a->p, sizeof(int) * (size_t)ring_cap_ * N_, hipMemcpyDeviceToDevice), "grow ring");
+        a->swap(g);
+    }
+    device_bytes_ += 2 * sizeof(int) * (size_t)(cap - ring_cap_) * N_;
+    ring_cap_ = cap;
+    drop_graph();
+}
+
+// Systems stalled by a star beyond the ring capacity (device flag counters[4]
+// = 2; their error CGPU_ERR_RING_CAPACITY and active 0). False when none.
+bool VoronoiBatch::take_stalled(std::vector<int> &stalled) {
+    sync();
+    int c4 = 0;
+    hip_check(hipMemcpy(&c4, counters_.p + 4, sizeof(int), hipMemcpyDeviceToHost), "stall flag");
+    stalled.clear();
+    if (c4 != 2) return false;
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    for (int s = 0; s < S_; ++s)
+        if (st[s].error == CGPU_ERR_RING_CAPACITY && !st[s].active) stalled.push_back(s);
+    return !stalled.empty();
+}
+
+// clears the stall of the given systems (error, active) and the device flag
+void VoronoiBatch::resume(const std::vector<int> &stalled) {
+    std::vector<cgpu_sys_state> st(S_);
+    st_.download(st.data(), S_);
+    for (int s : stalled) {
+        st[s].error = 0;
+        st[s].active = 1;
+    }
+    st_.upload(st.data(), S_);
+    int zero = 0;
+    hip_check(hipMemcpy(counters_.p + 4, &zero, sizeof(int), hipMemcpyHostToDevice), "stall flag");
+}
+
+// After a call of 'steps' steps of kind (0 SPV, 1 Brownian): systems that
+// stalled on the ring capacity are finished as the reference finishes a repair
+// that exceeds neighMax, by a global retriangulation with grown storage
+// (voronoiModelBase.cpp:456-460, 309-312), at their current positions (the
+// stalled step's move is done); their step counter then counts that step and
+// they run the steps they still owe alone, with the other systems parked.
+int VoronoiBatch::finish_stalled(int steps, int kind, const std::vector<int> &step_before) {
+    std::vector<int> stalled;
+    while (take_stalled(stalled)) {
+        if (ring_cap_ >= CGPU_RING_CAP_MAX) return CGPU_ERR_RING_CAPACITY;
+        grow_ring_capacity(std::min(CGPU_RING_CAP_MAX, 2 * ring_cap_));
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        const std::vector<cgpu_sys_state> saved = st;
+        // only the stalled systems take part in the retriangulation and the rest
+        for (int s = 0; s < S_; ++s) st[s].active = 0;
+        for (int s : stalled) {
+            st[s].error = 0;
+            st[s].active = 1;
+            st[s].step += 1;  // the stalled step, completed below
+        }
+        st_.upload(st.data(), S_);
+        int zero = 0;
+        hip_check(hipMemcpy(counters_.p + 4, &zero, sizeof(int), hipMemcpyHostToDevice), "stall flag");
+        int r = retriangulate_active();
+        if (r) return r;
+        // steps still owed (a countdown system keeps its countdown instead)
+        st_.download(st.data(), S_);
+        int owe = 0;
+        for (int s : stalled) {
+            if (saved[s].steps_left >= 0) {
+                st[s].steps_left = saved[s].steps_left > 0 ? saved[s].steps_left - 1 : 0;
+                if (st[s].steps_left == 0 && saved[s].steps_left > 0) st[s].active = 0;
+                continue;
+            }
+            const int left = steps - (st[s].step - step_before[s]);
+            if (left > 0) {
+                st[s].steps_left = left;
+                owe = std::max(owe, left);
+            } else {
+                st[s].active = 1;
+            }
+        }
+        st_.upload(st.data(), S_);
+        for (int k = 0; k < owe; ++k) {
+            r = step_launch(kind);
+            if (r) return r;
+        }
+        // restore: the other systems as they were, the stalled ones running on
+        sync();
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s) {
+            const bool was_stalled = std::find(stalled.begin(), stalled.end(), s) != stalled.end();
+            if (!was_stalled) {
+                st[s].active = saved[s].active;
+                st[s].steps_left = saved[s].steps_left;
+            } else if (saved[s].steps_left < 0 && st[s].error == 0) {
+                st[s].active = 1;
+                st[s].steps_left = -1;
+            }
+        }
+        st_.upload(st.data(), S_);
+    }
+    return CGPU_OK;
+}
+
+// global retriangulation of the active systems only (the others keep their
+// rings): every cell of an active system is queued for the global star pass
+int VoronoiBatch::retriangulate_active() {
     cgpu_voronoi_batch b = view();
-    int r = cgpu_global_triangulation(&b, stream_);
+    int r = cgpu_global_triangulation_active(&b, stream_);
     if (r) return r;
-    return status();
+    sync();
+    return CGPU_OK;
 }
 
 // reference voronoiModelBase::enforceTopology (voronoiModelBase.cpp:686-739) on
@@ -758,6 +895,27 @@ int VoronoiBatch::enforce_topology() {
     if (!r) r = cgpu_test_and_geometry(&b, 1, stream_);
     if (!r) r = cgpu_repair_topology(&b, stream_);
     if (r) return r;
+    // a stalled system is finished by a global retriangulation with grown storage
+    std::vector<int> stalled;
+    while (take_stalled(stalled)) {
+        if (ring_cap_ >= CGPU_RING_CAP_MAX) return CGPU_ERR_RING_CAPACITY;
+        grow_ring_capacity(std::min(CGPU_RING_CAP_MAX, 2 * ring_cap_));
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        const std::vector<cgpu_sys_state> saved = st;
+        for (int s = 0; s < S_; ++s) st[s].active = 0;
+        for (int s : stalled) {
+            st[s].error = 0;
+            st[s].active = 1;
+        }
+        st_.upload(st.data(), S_);
+        int zero = 0;
+        hip_check(hipMemcpy(counters_.p + 4, &zero, sizeof(int), hipMemcpyHostToDevice), "stall flag");
+        if ((r = retriangulate_active())) return r;
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s) st[s].active = saved[s].active || (st[s].error == 0 && std::find(stalled.begin(), stalled.end(), s) != stalled.end());
+        st_.upload(st.data(), S_);
+    }
     return status();
 }
 
@@ -777,6 +935,30 @@ int VoronoiBatch::step_launch(int kind) {
 
 int VoronoiBatch::spv_steps(int steps, bool use_graph) {
     if (steps <= 0) return CGPU_OK;
+    mark_call_start();
+    int r = launch_spv_steps(steps, use_graph);
+    return r ? r : after_steps(steps, 0);
+}
+
+// the step counters at the start of a stepping call, copied on the device
+// (no host wait), for the steps a stalled system still owes
+void VoronoiBatch::mark_call_start() {
+    if (st_call_.n != st_.n) st_call_.alloc(st_.n);
+    hip_check(hipMemcpyAsync(st_call_.p, st_.p, sizeof(cgpu_sys_state) * st_.n, hipMemcpyDeviceToDevice, stream_),
+              "state copy");
+}
+
+int VoronoiBatch::after_steps(int steps, int kind) {
+    std::vector<int> stalled;
+    if (!take_stalled(stalled)) return CGPU_OK;
+    std::vector<cgpu_sys_state> before(S_);
+    st_call_.download(before.data(), S_);
+    std::vector<int> step_before(S_);
+    for (int s = 0; s < S_; ++s) step_before[s] = before[s].step;
+    return finish_stalled(steps, kind, step_before);
+}
+
+int VoronoiBatch::launch_spv_steps(int steps, bool use_graph) {
     if (!use_graph || noise_buffer_) {
         for (int k = 0; k < steps; ++k) {
             int r = step_launch(0);
@@ -845,9 +1027,44 @@ int VoronoiBatch::spv_steps_timed(int steps, double *ms_out) {
 }
 
 int VoronoiBatch::brownian_steps(int steps) {
+    if (steps <= 0) return CGPU_OK;
+    mark_call_start();
     for (int k = 0; k < steps; ++k) {
         int r = step_launch(1);
         if (r) return r;
+    }
+    return after_steps(steps, 1);
+}
+
+// FIRE: systems that stalled inside an iteration (after its move) are
+// retriangulated with grown storage and the iteration's force half is run for
+// them alone (complete), then they rejoin the loop
+int VoronoiBatch::finish_stalled_fire(cgpu_fire_state *fs, void *vel, double *dots) {
+    std::vector<int> stalled;
+    while (take_stalled(stalled)) {
+        if (ring_cap_ >= CGPU_RING_CAP_MAX) return CGPU_ERR_RING_CAPACITY;
+        grow_ring_capacity(std::min(CGPU_RING_CAP_MAX, 2 * ring_cap_));
+        std::vector<cgpu_sys_state> st(S_);
+        st_.download(st.data(), S_);
+        const std::vector<cgpu_sys_state> saved = st;
+        for (int s = 0; s < S_; ++s) st[s].active = 0;
+        for (int s : stalled) {
+            st[s].error = 0;
+            st[s].active = 1;
+        }
+        st_.upload(st.data(), S_);
+        int zero = 0;
+        hip_check(hipMemcpy(counters_.p + 4, &zero, sizeof(int), hipMemcpyHostToDevice), "stall flag");
+        int r = retriangulate_active();
+        cgpu_voronoi_batch b = view();
+        if (!r) r = cgpu_fire_force_reduce(&b, fs, vel, dots, stream_);
+        if (!r) r = cgpu_fire_park_converged(&b, fs, stream_);
+        if (r) return r;
+        sync();
+        st_.download(st.data(), S_);
+        for (int s = 0; s < S_; ++s)
+            if (std::find(stalled.begin(), stalled.end(), s) == stalled.end()) st[s].active = saved[s].active;
+        st_.upload(st.data(), S_);
     }
     return CGPU_OK;
 }
@@ -908,7 +1125,7 @@ void VoronoiBatch::get(double *x, double *y, double *theta, double *vx, double *
 
 void VoronoiBatch::get_topology(int *nn, int *nbr, int cap, int *neigh_max) {
     sync();
-    std::vector<int> hn(N_), hr((size_t)CGPU_RING_CAP * N_);
+    std::vector<int> hn(N_), hr((size_t)ring_cap_ * N_);
     nn_.download(hn.data(), N_);
     nbr_.download(hr.data(), hr.size());
     for (int i = 0; i < N_; ++i) {
@@ -955,7 +1172,7 @@ double VoronoiBatch::energy(int s) {
         // force sets are), gamma times its length
         std::vector<double> x(N_), y(N_);
         get(x.data(), y.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-        const int cap = CGPU_RING_CAP;
+        const int cap = ring_cap_;
         std::vector<int> nn(N_), nbr((size_t)N_ * cap);
         get_topology(nn.data(), nbr.data(), cap, nullptr);
         std::vector<double> g;
@@ -1360,6 +1577,7 @@ int cgpu_vor_get_counters(void *h, int s, long long *out) {
     return guarded([&] { ((VoronoiBatch *)h)->get_counters(s, out); return 0; });
 }
 int cgpu_vor_num_cells(void *h) { return ((VoronoiBatch *)h)->num_cells(); }
+int cgpu_vor_ring_capacity(void *h) { return ((VoronoiBatch *)h)->ring_capacity(); }
 double cgpu_vor_energy(void *h, int s) {
     try {
         return ((VoronoiBatch *)h)->energy(s);
@@ -1450,6 +1668,7 @@ int VoronoiBatch::cell_events(int count, const int *kind, const int *cell, const
     if (count <= 0) return CGPU_OK;
     if (!kind || !cell) throw std::invalid_argument("cell events need kinds and cells");
     std::vector<int> ev_of_sys(S_, -1);
+    bool draw = false;
     for (int e = 0; e < count; ++e) {
         if (cell[e] < 0 || cell[e] >= N_) throw std::invalid_argument("cell index out of range");
         if (kind[e] != 0 && kind[e] != 1) throw std::invalid_argument("event kind is 0 (death) or 1 (division)");
@@ -1459,114 +1678,68 @@ int VoronoiBatch::cell_events(int count, const int *kind, const int *cell, const
         if (kind[e] == 0 && hp_[s].n <= 4) throw std::invalid_argument("a system keeps at least 4 cells");
         if (kind[e] == 1 && (!theta || !frac || !std::isfinite(theta[e]) || !std::isfinite(frac[e])))
             throw std::invalid_argument("a division needs a finite angle and separation fraction");
+        if (kind[e] == 1 && !(new_director && std::isfinite(new_director[e]))) draw = true;
     }
     sync();
-    const size_t N = N_;
-    std::vector<double> xy(2 * N), th(N), vel(2 * N), pref(2 * N), mot(2 * N);
-    (parity_ ? pos1_ : pos0_).download(xy.data(), 2 * N);
-    real_download(theta_.p, th.data(), N);
-    real_download(vel_.p, vel.data(), 2 * N);
-    real_download(APpref_.p, pref.data(), 2 * N);
-    real_download(mot_.p, mot.data(), 2 * N);
-    std::vector<int> ex;
-    if (excl_.p) {
-        ex.resize(N);
-        excl_.download(ex.data(), N);
-    }
-    const int cap = CGPU_RING_CAP;
-    std::vector<int> nn(N), nbr(N * cap);
-    get_topology(nn.data(), nbr.data(), cap, nullptr);
+    // step counters, for directors drawn from the counter-based stream
     std::vector<cgpu_sys_state> st(S_);
-    st_.download(st.data(), S_);
-
-    // new order: old index of each new cell (-1: the daughter of parent[.])
-    std::vector<int> src, parent, newn(S_), newtag;
-    std::vector<double> nxy, nth, nvel, npref, nmot;
-    std::vector<int> nex, ntype;
-    auto push = [&](int i, int tag) {
-        src.push_back(i);
-        nxy.push_back(xy[2 * i]);
-        nxy.push_back(xy[2 * i + 1]);
-        nth.push_back(th[i]);
-        nvel.push_back(vel[2 * i]);
-        nvel.push_back(vel[2 * i + 1]);
-        npref.push_back(pref[2 * i]);
-        npref.push_back(pref[2 * i + 1]);
-        nmot.push_back(mot[2 * i]);
-        nmot.push_back(mot[2 * i + 1]);
-        if (!ex.empty()) nex.push_back(ex[i]);
-        if (!h_type_.empty()) ntype.push_back(h_type_[i]);
-        newtag.push_back(tag);
-    };
+    if (draw) st_.download(st.data(), S_);
+    // the division axes, from the current rings on the device: one record of
+    // six doubles per dividing cell
+    std::vector<int> div_of_event(count, -1), parent;
+    std::vector<double> ray;
+    for (int e = 0; e < count; ++e)
+        if (kind[e] == 1) {
+            div_of_event[e] = (int)parent.size();
+            parent.push_back(cell[e]);
+            ray.push_back(std::cos(theta[e]));
+            ray.push_back(std::sin(theta[e]));
+        }
+    const int nd = (int)parent.size();
+    std::vector<double> axis(6 * (size_t)nd);
+    if (nd) {
+        DevBuf<int> dpar;
+        DevBuf<double> dray, dout;
+        dpar.alloc(nd);
+        dpar.upload(parent.data(), nd);
+        dray.alloc(2 * (size_t)nd);
+        dray.upload(ray.data(), 2 * (size_t)nd);
+        dout.alloc(6 * (size_t)nd);
+        cgpu_voronoi_batch b = view();
+        int r = cgpu_vor_division_axis(&b, nd, dpar.p, dray.p, dout.p, stream_);
+        if (r) return r;
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        dout.download(axis.data(), axis.size());
+    }
+    // one record per system; the new positions follow the reference
+    // (voronoiModelBase.cpp:1399-1406) from the crossings
+    std::vector<cgpu_cell_event> rec(S_);
+    std::vector<int> newn(S_);
+    int noff = 0;
     for (int s = 0; s < S_; ++s) {
         const cgpu_sys_params &p = hp_[s];
+        cgpu_cell_event &r = rec[s];
+        std::memset(&r, 0, sizeof(r));
+        r.kind = -1;
+        r.old_off = p.off;
+        r.old_n = p.n;
+        r.new_off = noff;
         const int e = ev_of_sys[s];
-        const bool death = e >= 0 && kind[e] == 0, division = e >= 0 && kind[e] == 1;
-        const int target = e >= 0 ? cell[e] : -1;
-        const int dead_tag = death ? tag_[target] : -1;
-        const size_t start = src.size();
-        for (int i = p.off; i < p.off + p.n; ++i) {
-            if (death && i == target) continue;
-            const int t = tag_[i];
-            push(i, (death && t > dead_tag) ? t - 1 : t);
+        if (e >= 0) {
+            r.kind = kind[e];
+            r.target = cell[e] - p.off;
         }
-        if (division) {
-            // the cell's Voronoi polygon in the reference's CPU vertex order:
-            // vertex k = circumcenter(ring k-1, ring k)
+        if (e >= 0 && kind[e] == 1) {
+            const double *ax = &axis[6 * (size_t)div_of_event[e]];
+            if (!std::isfinite(ax[2])) throw std::runtime_error("division axis misses the cell's polygon");
             const cgpu::Box box = cgpu::make_box(p.L);
-            const int z = nn[target];
-            const cgpu::dvec2 pi{xy[2 * target], xy[2 * target + 1]};
-            auto rel = [&](int k) {
-                const int j = nbr[(size_t)target * cap + k];
-                return cgpu::min_dist(box, cgpu::dvec2{xy[2 * j], xy[2 * j + 1]}, pi);
-            };
-            std::vector<cgpu::dvec2> voro(z);
-            cgpu::dvec2 rij = rel(z - 1);
-            for (int k = 0; k < z; ++k) {
-                const cgpu::dvec2 rik = rel(k);
-                voro[k] = cgpu::circ0<double>(rij, rik);
-                rij = rik;
-            }
-            // where the axis through the cell at angle theta crosses the polygon
-            const double t = theta[e];
-            const cgpu::dvec2 ray{std::cos(t), std::sin(t)}, c{-std::sin(t), std::cos(t)};
-            cgpu::dvec2 v1 = voro[z - 1], int1{0, 0}, int2{0, 0};
-            bool found = false, second = false;
-            for (int k = 0; k < z; ++k) {
-                const cgpu::dvec2 v2 = voro[k];
-                const cgpu::dvec2 a{0.0 - v2.x, 0.0 - v2.y}, b{v1.x - v2.x, v1.y - v2.y};
-                double t2 = -1.0;
-                const double bc = b.x * c.x + b.y * c.y;
-                if (bc != 0) t2 = (a.x * c.x + a.y * c.y) / bc;
-                if (t2 >= 0.0 && t2 <= 1.0) {
-                    const cgpu::dvec2 q{v2.x + t2 * (v1.x - v2.x), v2.y + t2 * (v1.y - v2.y)};
-                    if (found) {
-                        int2 = q;
-                        second = true;
-                    } else {
-                        int1 = q;
-                        found = true;
-                    }
-                }
-                v1 = v2;
-            }
-            if (!found) throw std::runtime_error("division axis misses the cell's polygon");
-            if (!second) int2 = int1;
-            const double sep = std::max(std::sqrt(int1.x * int1.x + int1.y * int1.y),
-                                        std::sqrt(int2.x * int2.x + int2.y * int2.y));
+            const cgpu::dvec2 pi{ax[0], ax[1]};
+            const double c = ray[2 * div_of_event[e]], sn = ray[2 * div_of_event[e] + 1];
+            const double sep = std::max(std::sqrt(ax[2] * ax[2] + ax[3] * ax[3]), std::sqrt(ax[4] * ax[4] + ax[5] * ax[5]));
             const double f = frac[e] * sep;
-            cgpu::dvec2 p1{pi.x + f * ray.x, pi.y + f * ray.y}, p2{pi.x - f * ray.x, pi.y - f * ray.y};
+            cgpu::dvec2 p1{pi.x + f * c, pi.y + f * sn}, p2{pi.x - f * c, pi.y - f * sn};
             cgpu::put_in_box(box, p1);
             cgpu::put_in_box(box, p2);
-            // the parent moves to p1; the daughter (copy of the parent's
-            // preferences, motility and type) takes p2
-            const size_t pk = start + (size_t)(target - p.off);
-            nxy[2 * pk] = p1.x;
-            nxy[2 * pk + 1] = p1.y;
-            push(target, p.n);
-            const size_t dk = src.size() - 1;
-            nxy[2 * dk] = p2.x;
-            nxy[2 * dk + 1] = p2.y;
             double dth = new_director ? new_director[e] : NAN;
             if (!std::isfinite(dth)) {
                 uint32_t ctr[4] = {(uint32_t)p.n, (uint32_t)st[s].step, 3u, 0u};
@@ -1574,33 +1747,74 @@ int VoronoiBatch::cell_events(int count, const int *kind, const int *cell, const
                 const uint64_t u = (((uint64_t)ctr[0] << 21) ^ (uint64_t)(ctr[1] >> 11)) & (((uint64_t)1 << 53) - 1);
                 dth = 6.283185307179586476925286766559 * ((double)u / 9007199254740992.0);
             }
-            nth[dk] = dth;
-            nvel[2 * dk] = nmot[2 * dk] * std::cos(dth);
-            nvel[2 * dk + 1] = nmot[2 * dk] * std::sin(dth);
-            if (!nex.empty()) nex[dk] = 0;
+            r.p1[0] = p1.x;
+            r.p1[1] = p1.y;
+            r.p2[0] = p2.x;
+            r.p2[1] = p2.y;
+            r.director = dth;
+            r.dcos = std::cos(dth);
+            r.dsin = std::sin(dth);
         }
-        newn[s] = (int)(src.size() - start);
+        newn[s] = p.n + (r.kind == 1 ? 1 : 0) - (r.kind == 0 ? 1 : 0);
+        noff += newn[s];
+    }
+    const size_t M = (size_t)noff;
+    // the new per-cell arrays, gathered on the device from the old ones
+    DevBuf<cgpu_cell_event> drec;
+    drec.alloc(S_);
+    drec.upload(rec.data(), S_);
+    DevBuf<double> npos;
+    DevBuf<unsigned char> nth, nvel, npref, nmot;
+    DevBuf<int> nex, ntype;
+    npos.alloc(2 * M);
+    nth.alloc(M * rsz_);
+    nvel.alloc(2 * M * rsz_);
+    npref.alloc(2 * M * rsz_);
+    nmot.alloc(2 * M * rsz_);
+    if (excl_.p) nex.alloc(M);
+    if (cell_type_.p) ntype.alloc(M);
+    {
+        cgpu_voronoi_batch b = view();
+        int r = cgpu_vor_events_gather(&b, drec.p, (int)M, npos.p, nth.p, nvel.p, npref.p, nmot.p, nex.p, ntype.p,
+                                       stream_);
+        if (r) return r;
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
+    // labels (reference tagToIdx / idxToTag): a death shifts the labels above
+    // the dead cell's down by one; a daughter takes the old system size
+    std::vector<int> newtag, newtype;
+    newtag.reserve(M);
+    for (int s = 0; s < S_; ++s) {
+        const cgpu_sys_params &p = hp_[s];
+        const cgpu_cell_event &r = rec[s];
+        const int dead_tag = r.kind == 0 ? tag_[p.off + r.target] : -1;
+        for (int i = p.off; i < p.off + p.n; ++i) {
+            if (r.kind == 0 && i == p.off + r.target) continue;
+            const int t = tag_[i];
+            newtag.push_back((r.kind == 0 && t > dead_tag) ? t - 1 : t);
+            if (!h_type_.empty()) newtype.push_back(h_type_[i]);
+        }
+        if (r.kind == 1) {
+            newtag.push_back(p.n);
+            if (!h_type_.empty()) newtype.push_back(h_type_[p.off + r.target]);
+        }
     }
     layout(newn);
-    const size_t M = N_;
-    pos0_.upload(nxy.data(), 2 * M);
-    real_upload(theta_.p, nth.data(), M);
-    real_upload(vel_.p, nvel.data(), 2 * M);
-    real_upload(APpref_.p, npref.data(), 2 * M);
-    real_upload(mot_.p, nmot.data(), 2 * M);
+    pos0_.swap(npos);
+    theta_.swap(nth);
+    vel_.swap(nvel);
+    APpref_.swap(npref);
+    mot_.swap(nmot);
     tag_ = newtag;
-    if (!ex.empty()) {
-        excl_.alloc(M);
-        excl_.upload(nex.data(), M);
+    if (nex.p) {
+        excl_.swap(nex);
         ext_.alloc(2 * M * rsz_);
     }
-    if (!h_type_.empty()) {
-        h_type_ = ntype;
-        cell_type_.alloc(M);
-        cell_type_.upload(ntype.data(), M);
+    if (ntype.p) {
+        h_type_ = newtype;
+        cell_type_.swap(ntype);
     }
     if (noise_.p) noise_.alloc(2 * M);
-    st_.upload(st.data(), S_);
     update_tension_flag();
     return global_triangulation();
 }

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/cellgpu_amd.h"
@@ -46,6 +47,10 @@ template <typename T> struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+    }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
     }
     void upload(const T *h, size_t count, size_t offset = 0) {
         hip_check(hipMemcpy(p + offset, h, count * sizeof(T), hipMemcpyHostToDevice), "upload");
@@ -125,6 +130,12 @@ class VoronoiBatch {
     // (CGPU_PROFILE_SLOTS) and summed queue sizes (flagged, NeedsFixing, global)
     int spv_steps_profiled(int steps, double *ms_slots, long long *queue_sums);
     int brownian_steps(int steps);
+    // ring storage (slots per cell) and its growth, outside any graph
+    int ring_capacity() const { return ring_cap_; }
+    void grow_ring_capacity(int cap);
+    // FIRE systems that stalled on the ring capacity: grown, retriangulated, the
+    // iteration's force half completed
+    int finish_stalled_fire(cgpu_fire_state *fs, void *vel, double *dots);
     int sync();
     int status();
 
@@ -174,6 +185,14 @@ class VoronoiBatch {
     void real_upload(void *dst, const double *src2, size_t count);
     void real_download(const void *src, double *dst, size_t count);
     int step_launch(int kind);
+    int launch_spv_steps(int steps, bool use_graph);
+    void mark_call_start();
+    int after_steps(int steps, int kind);
+    bool take_stalled(std::vector<int> &stalled);
+    void resume(const std::vector<int> &stalled);
+    int finish_stalled(int steps, int kind, const std::vector<int> &step_before);
+    int retriangulate_active();
+    int ring_cap_ = CGPU_RING_CAP;
     void drop_graph();
     void launch_geometry_only(const cgpu_voronoi_batch &b);
     // device copy of every array a step mutates (for the tuner's trials)
@@ -194,7 +213,7 @@ class VoronoiBatch {
     std::vector<cgpu_sys_params> hp_;
     std::vector<int> cell_sys_h_;
     DevBuf<cgpu_sys_params> sys_;
-    DevBuf<cgpu_sys_state> st_;
+    DevBuf<cgpu_sys_state> st_, st_call_;
     DevBuf<int> cell_sys_, nn_, nbr_, oth_, excl_, repair_, mark_, flagged_, nfq_, gq_, counters_;
     DevBuf<int> bin_start_, cell_bin_, cell_rank_, sorted_id_, ld_tri_, bin_mark_wide_;
     DevBuf<float> cell_rmax_;

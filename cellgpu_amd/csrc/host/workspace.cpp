@@ -151,6 +151,7 @@ int cgpu_vor_workspace_bind(const cgpu_vor_workspace *ws, const cgpu_sys_params 
         b.precision = ws->precision;
         b.num_chunks = ws->num_chunks;
         b.num_bin_chunks = ws->num_bin_chunks;
+        b.ring_cap = CGPU_RING_CAP;
         Carver c{(char *)mem};
         carve(c, *ws, b);
         // every array starts at zero; systems start active with unlimited steps

@@ -24,7 +24,19 @@ using namespace cgpu;
 
 namespace {
 
-constexpr int VCAP = CGPU_VERTEX_CAP;
+// cell-list slots per cell (slot-major storage), at least the register copy
+__host__ __device__ inline int vertex_cap(const cgpu_vertex_batch &b) {
+    return b.vertex_cap <= 0 ? CGPU_VERTEX_CAP : b.vertex_cap;
+}
+// A system that needs more cell-list slots stalls: it stops advancing (this
+// step's flips are not made and its counter not advanced) until the host grows
+// the storage and finishes the step (reference growCellVerticesList before the
+// flips, vertexModelBase.cpp:475-500, 815-824). The T1 test only records the
+// need (the director rotation of the same kernel reads the activity flag); the
+// flip kernel parks the system.
+__device__ inline void vm_need_room(const cgpu_vertex_batch &b, int s) {
+    if (b.st[s].error == 0) atomicCAS(&b.st[s].error, 0, (int)CGPU_ERR_RING_CAPACITY);
+}
 
 __device__ inline dvec2 ldp(const double *p, int i) {
     double2 v = reinterpret_cast<const double2 *>(p)[i];
@@ -199,6 +211,12 @@ __global__ void k_vm_t1_test(cgpu_vertex_batch b, const double *pos, int rotate)
         if (v < v2) {
             dvec2 e = min_dist(box, p, ldp(pos, v2));
             if (sqrt(e.x * e.x + e.y * e.y) < sp.thr) {
+                // a flip adds a vertex to a cell touching the edge: every one of
+                // them must have a free slot (reference vm_simple_T1_test_kernel
+                // grow flag, vertexModelBase.cu:150-160)
+                const int cap = vertex_cap(b);
+                for (int g = 0; g < 3; ++g)
+                    if (b.cvn[b.vcn[3 * v + g]] >= cap || b.cvn[b.vcn[3 * v2 + g]] >= cap) vm_need_room(b, s);
                 int q = atomicAdd(&b.flag_count[s], 1);
                 b.flag_list[2 * (3 * sp.voff + q)] = v;
                 b.flag_list[2 * (3 * sp.voff + q) + 1] = v2;
@@ -378,8 +396,9 @@ __device__ bool flip_edge(const cgpu_vertex_batch &b, double *pos, const cgpu_vm
     const CellRegs &Rk = sk == 0 ? R0 : (sk == 1 ? R1 : R2);
     const CellRegs &Rl = R3;
     if (Ri.n == 3 || Rk.n == 3) return false;  // would shrink a triangle
-    if (Rj.n >= VCAP || Rl.n >= VCAP) {
-        b.st[b.vert_sys[v1]].error = CGPU_ERR_RING_CAPACITY;
+    if (Rj.n >= vertex_cap(b) || Rl.n >= vertex_cap(b)) {
+        vm_need_room(b, b.vert_sys[v1]);
+        b.st[b.vert_sys[v1]].active = 0;
         return false;
     }
     int dummy, a, bb;
@@ -589,7 +608,10 @@ __global__ __launch_bounds__(256) void k_vm_t1_rounds(cgpu_vertex_batch b, doubl
     __shared__ int s_list[2 * T1_LDS];
     __shared__ int4 s_cells[T1_LDS], s_touched[T1_LDS];
     if (threadIdx.x == 0) {
-        count = b.flag_count[s];
+        // a system that needs longer cell lists stalls here; a stalled (or
+        // parked) system makes no flips
+        if (st.active && st.error == CGPU_ERR_RING_CAPACITY) st.active = 0;
+        count = st.active ? b.flag_count[s] : 0;
         rounds = 0;
         flips = 0;
     }

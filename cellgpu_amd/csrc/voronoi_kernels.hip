@@ -29,7 +29,13 @@ using namespace cgpu;
 
 namespace {
 
-constexpr int CAP = CGPU_RING_CAP;
+// Ring storage: ring_cap(b) slots per cell, slot-major. Register copies of a
+// ring (serial star forms) hold CAP_MAX entries; the wavefront forms hold one
+// neighbour per lane.
+constexpr int CAP_MAX = CGPU_RING_CAP_MAX;
+__host__ __device__ inline int ring_cap(const cgpu_voronoi_batch &b) {
+    return b.ring_cap <= 0 ? CGPU_RING_CAP : (b.ring_cap < CAP_MAX ? b.ring_cap : CAP_MAX);
+}
 
 // ------------------------------------------------------------- launch shapes
 // A zero (or unsupported) field of the batch's launch shape takes the default.
@@ -829,7 +835,8 @@ __device__ inline vec2<R> cell_force(const cgpu_voronoi_batch &b, const Acc &acc
 // candidates with exact predicates. Each triangle is certified empty by
 // requiring its circumcircle to lie inside the scanned bin block; otherwise the
 // block grows. Rotation is canonical (smallest neighbour first). Returns the
-// degree, -1 when no certified star exists, -2 when the degree exceeds CAP.
+// degree, -1 when no certified star exists, -2 when the degree exceeds the ring
+// capacity.
 struct Cand {
     int id;
     dvec2 d;
@@ -884,8 +891,9 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
         Cand cur = first;
         int z = 0;
         bool ok = true;
-        for (int guard = 0; guard <= CAP; ++guard) {
-            if (z >= CAP) return -2;
+        const int cap = ring_cap(b);
+        for (int guard = 0; guard <= cap; ++guard) {
+            if (z >= cap) return -2;
             ring[z++] = cur.id;
             Cand best{-1, {0, 0}};
             scan_block(b, sp, ib, jb, w, [&](int m, int k) {
@@ -916,7 +924,7 @@ __device__ int delaunay_star(const cgpu_voronoi_batch &b, const double *pos, int
         for (int k = 1; k < z; ++k)
             if (ring[k] < ring[k0]) k0 = k;
         if (k0 != 0) {
-            int tmp[CAP];
+            int tmp[CAP_MAX];
             for (int k = 0; k < z; ++k) tmp[k] = ring[(k + k0) % z];
             for (int k = 0; k < z; ++k) ring[k] = tmp[k];
         }
@@ -945,7 +953,7 @@ __device__ inline int quadrant(dvec2 d) {
 __device__ bool verify_ring(const cgpu_voronoi_batch &b, const double *pos, int i, const cgpu_sys_params &sp) {
     const int N = b.num_cells;
     const int z = b.nn[i];
-    if (z < 3 || z > CAP) return false;
+    if (z < 3 || z > ring_cap(b)) return false;
     const Box box = make_box(sp.L);
     const dvec2 pi = ldpos(pos, i);
     const double bs = sp.boxsize;
@@ -1081,7 +1089,7 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
 // Star of cell i by one wave. Returns the new degree (member k of the ring in
 // lane k's ring_out, canonical rotation), 0 when the current ring is verified
 // as the Delaunay star (nothing to write), -1 when no certified star exists
-// within the largest block, -2 when the degree exceeds CAP, -3 when a block's
+// within the largest block, -2 when the degree exceeds the ring capacity, -3 when a block's
 // candidates do not fit in K slots per lane (the caller takes another path).
 // The search starts at half-width w0 (blocks below it are known to give no
 // certified star); on -3, w_at is the half-width whose block did not fit.
@@ -1101,7 +1109,7 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
 
     // current ring: member k in lane k, with the triangle (k-1, k) and its circle
     const int zold = b.nn[i];
-    int verify = (zold >= 3 && zold <= CAP) ? 1 : 0;  // 1 undecided, 0 rebuild, 2 verified
+    int verify = (zold >= 3 && zold <= ring_cap(b)) ? 1 : 0;  // 1 undecided, 0 rebuild, 2 verified
     int rid = -1, pid = -1;
     double rx = 0, ry = 0, px = 0, py = 0, ccx = 0, ccy = 0, cr2 = 0, cr = 0;
     if (verify) {
@@ -1209,8 +1217,9 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         WCand cur = first;
         int z = 0, myring = -1;
         bool ok = true;
-        for (int guard = 0; guard <= CAP; ++guard) {
-            if (z >= CAP) return -2;
+        const int cap = ring_cap(b);
+        for (int guard = 0; guard <= cap; ++guard) {
+            if (z >= cap) return -2;
             if (lane == z) myring = cur.id;
             ++z;
             WCand best{-1, 0.0, 0.0};
@@ -1266,7 +1275,7 @@ __device__ inline int wave_owned(int i, int ring_lane, int z) {
 __device__ bool del_set_slot(const cgpu_voronoi_batch &b, int i, int z, int k) {
     const int N = b.num_cells;
     const int nm1 = b.nbr[((k + z - 2) % z) * N + i], n1 = b.nbr[((k + z - 1) % z) * N + i];
-    const int zn = min(b.nn[n1], CAP);
+    const int zn = min(b.nn[n1], ring_cap(b));
     for (int q = 0; q < zn; ++q) {
         if (b.nbr[q * N + n1] == nm1) {
             b.oth[k * N + i] = b.nbr[((q + 1) % zn) * N + n1];
@@ -1860,7 +1869,7 @@ __global__ void k_ld_reduce(cgpu_voronoi_batch b, const double *pos) {
             sys = b.cell_sys[i];
             const cgpu_sys_params &sp = b.sys[sys];
             const Box box = make_box(sp.L);
-            const int z = min(b.nn[i], CAP);
+            const int z = min(b.nn[i], ring_cap(b));
             const dvec2 pi = ldpos(pos, i);
             dvec2 d{0.0, 0.0};
             if (lane < z) d = min_dist(box, ldpos(pos, b.nbr[lane * N + i]), pi);
@@ -2103,6 +2112,15 @@ __global__ void k_global_all(cgpu_voronoi_batch b) {
     enqueue_chunk(b, c);
 }
 
+// the same for the active systems only
+__global__ void k_global_active(cgpu_voronoi_batch b) {
+    const int c = blockIdx.x;
+    const int s = b.chunk_sys[c];
+    if (!b.st[s].active) return;  // uniform per block
+    if (threadIdx.x == 0 && b.chunk_begin[c] == b.sys[s].off) b.st[s].mode = 2;
+    enqueue_chunk(b, c);
+}
+
 // local repair of cell i: new Delaunay star of a NeedsFixing cell (reference
 // repairTriangulation, voronoiModelBase.cpp:429-505), with the neighMax rule and
 // the changes of the 6N / 2N invariants accumulated per system. Serial form.
@@ -2114,7 +2132,7 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
     const int zold = b.nn[i];
     if (verify_ring(b, pos, i, sp)) return;  // unchanged: no invariant change
     const int told = owned_triangles(b, i);
-    int ring[CAP];
+    int ring[CAP_MAX];
     const int z = delaunay_star(b, pos, i, sp, ring, w0);
     if (z < 0 || z > st.neigh_max) {
         st.need_global = 1;
@@ -2183,8 +2201,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             continue;
         }
         const int zold = b.nn[i];
-        const int rold = (lane < zold && lane < CAP) ? b.nbr[lane * N + i] : -1;
-        const int told = wave_owned(i, rold, min(zold, CAP));
+        const int cap = ring_cap(b);
+        const int rold = (lane < zold && lane < cap) ? b.nbr[lane * N + i] : -1;
+        const int told = wave_owned(i, rold, min(zold, cap));
         const int tnew = wave_owned(i, ring, z);
         if (lane < z) b.nbr[lane * N + i] = ring;
         if (lane == 0) {
@@ -2241,16 +2260,30 @@ __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     enqueue_chunk(b, c);
 }
 
+// A global star that could not be built. Beyond the ring capacity (z = -2) the
+// system stalls: it stops advancing from here on (its step counter is not
+// advanced for this step) until the host grows the capacity, retriangulates it
+// and resumes it. Any other failure is a topology error.
+__device__ inline void star_failed(const cgpu_voronoi_batch &b, int s, int z) {
+    if (z == -2) {
+        b.st[s].error = CGPU_ERR_RING_CAPACITY;
+        b.st[s].active = 0;
+        b.counters[4] = 2;
+    } else {
+        b.st[s].error = CGPU_ERR_TOPOLOGY;
+        if (b.counters[4] == 0) b.counters[4] = 1;
+    }
+}
+
 // global retriangulation (globalTriangulationCGAL) of cell i, serial form
 __device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *pos, int i, int w0) {
     const int N = b.num_cells;
     const int s = b.cell_sys[i];
     if (verify_ring(b, pos, i, b.sys[s])) return;
-    int ring[CAP];
+    int ring[CAP_MAX];
     const int z = delaunay_star(b, pos, i, b.sys[s], ring, w0);
     if (z < 0) {
-        b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
-        b.counters[4] = 1;
+        star_failed(b, s, z);
         return;
     }
     b.nn[i] = z;
@@ -2284,10 +2317,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             continue;
         }
         if (z < 0) {
-            if (lane == 0) {
-                b.st[s].error = (z == -2) ? CGPU_ERR_RING_CAPACITY : CGPU_ERR_TOPOLOGY;
-                b.counters[4] = 1;
-            }
+            if (lane == 0) star_failed(b, s, z);
             continue;
         }
         if (lane < z) b.nbr[lane * N + i] = ring;
@@ -2390,7 +2420,7 @@ __global__ void k_delsets_all(cgpu_voronoi_batch b) {
          t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(t >> lg), k0 = (int)(t & (QL - 1));
         const int z = b.nn[i];
-        if (z < 3 || z > CAP) continue;
+        if (z < 3 || z > ring_cap(b)) continue;
         for (int k = k0; k < z; k += QL)
             if (!del_set_slot(b, i, z, k)) {
                 int &err = b.st[b.cell_sys[i]].error;
@@ -2787,6 +2817,18 @@ int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream) {
     global_pass(*b, b->pos, st);
     // the pass recomputes geometry only where a ring changed; positions may have
     // changed everywhere here
+    launch_test_geometry(*b, b->pos, 0, st);
+    return check();
+}
+
+int cgpu_global_triangulation_active(const cgpu_voronoi_batch *b, void *stream) {
+    if (!b) return CGPU_ERR_ARGS;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_reset, dim3(nblocks(b->num_systems, 256)), dim3(256), 0, st, *b);
+    int r = cgpu_compute_cell_list(b, 0, stream);
+    if (r) return r;
+    hipLaunchKernelGGL(k_global_active, dim3(b->num_chunks), dim3(256), 0, st, *b);
+    global_pass(*b, b->pos, st);
     launch_test_geometry(*b, b->pos, 0, st);
     return check();
 }

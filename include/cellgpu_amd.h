@@ -40,8 +40,13 @@ enum cgpu_status {
     CGPU_ERR_SCRATCH_CAPACITY = -8 /* input too large for the static scratch of an allocation-free entry */
 };
 
-/* Maximum Delaunay degree stored per cell (ring capacity, slot-major storage). */
+/* Ring capacity: Delaunay neighbours stored per cell (slot-major storage). A
+ * batch starts at CGPU_RING_CAP slots and the object layer grows it, up to
+ * CGPU_RING_CAP_MAX (one neighbour per lane of a wavefront in the star
+ * kernels), when a star needs more (the reference's neighMax growth,
+ * voronoiModelBase.cpp:309-312). */
 #define CGPU_RING_CAP 24
+#define CGPU_RING_CAP_MAX 64
 
 /* Per-system constants of a batch (host fills, device reads). */
 typedef struct cgpu_sys_params {
@@ -128,8 +133,8 @@ typedef struct cgpu_voronoi_batch {
     double *pos;                 /* [2N] x,y interleaved (current)  */
     double *pos_next;            /* [2N] x,y interleaved (next)     */
     int *nn;                     /* [N] Delaunay degree             */
-    int *nbr;                    /* [CAP*N] rings                   */
-    int *oth;                    /* [CAP*N] reference delOther      */
+    int *nbr;                    /* [ring_cap*N] rings              */
+    int *oth;                    /* [ring_cap*N] reference delOther */
     void *AP;                    /* [2N] real: area, perimeter      */
     void *dAP;                   /* [2N] real: KA(A-A0), KP(P-P0)   */
     const void *APpref;          /* [2N] real: A0, P0               */
@@ -171,7 +176,10 @@ typedef struct cgpu_voronoi_batch {
      * NULL: no system uses tension. */
     const int *cell_type;        /* [N] or NULL                                     */
     const double *tension;       /* [num_types^2] or NULL                           */
-    int num_types, pad_t;
+    int num_types;
+    int ring_cap;                /* ring slots allocated per cell (nbr, oth hold ring_cap * N);
+                                    0: CGPU_RING_CAP. A star above it stalls its system:
+                                    error CGPU_ERR_RING_CAPACITY, active 0, step not advanced */
     /* Circumcircle-test skip rule: bins within 3 bins of an edge that is not
      * locally Delaunay (or of a ring that is not a proper fan), stamped with the
      * test's epoch (counters[7]); each cell's largest circumradius; the queue of
@@ -194,8 +202,10 @@ typedef struct cgpu_voronoi_batch {
 #define CGPU_LD_REGION_BYTES 40
 #define CGPU_BIN_CHUNK 8192
 
-/* Maximum number of vertices of one vertex-model cell (reference vertexMax = 30,
- * vertexModelBase.cpp:210, grown on demand there; fixed capacity here). */
+/* Vertices stored per vertex-model cell (reference vertexMax = 30,
+ * vertexModelBase.cpp:210): a batch starts at CGPU_VERTEX_CAP slots and the
+ * object layer grows it when a T1 flip needs more (reference
+ * growCellVerticesList, vertexModelBase.cpp:475-500, 815-824). */
 #define CGPU_VERTEX_CAP 32
 
 /* Per-system constants of a vertex-model batch. */
@@ -225,7 +235,7 @@ typedef struct cgpu_vertex_batch {
     const int *cell_sys;         /* [Nc] */
     double *pos, *pos_next;      /* [2Nv] vertex positions */
     int *vn, *vcn;               /* [3Nv] */
-    int *cvn, *cv;               /* [Nc], [VERTEX_CAP*Nc] */
+    int *cvn, *cv;               /* [Nc], [vertex_cap*Nc] */
     void *AP, *dAP;              /* [2Nc] real */
     const void *APpref;          /* [2Nc] real */
     void *force;                 /* [2Nv] real */
@@ -241,7 +251,10 @@ typedef struct cgpu_vertex_batch {
     /* vertex chunks (at most CGPU_CHUNK vertices of one system), for the
      * per-system FIRE reductions */
     const int *chunk_sys, *chunk_begin, *chunk_end;
-    int num_chunks, pad_c;
+    int num_chunks;
+    int vertex_cap;              /* cell-list slots allocated per cell (cv holds vertex_cap * Nc);
+                                    0: CGPU_VERTEX_CAP. A T1 flip that would exceed it stalls its
+                                    system: error CGPU_ERR_RING_CAPACITY, active 0 */
 } cgpu_vertex_batch;
 
 /* ---------------------------------------------------------------- kernel layer */
@@ -269,6 +282,10 @@ int cgpu_repair_topology(const cgpu_voronoi_batch *b, void *stream);
  * of the batch: bins current positions, computes all Delaunay stars, delOther and
  * geometry. */
 int cgpu_global_triangulation(const cgpu_voronoi_batch *b, void *stream);
+/* The same for the active systems only (the others keep their rings): used to
+ * finish a system that stalled on the ring capacity once the caller has grown
+ * the ring storage (b->ring_cap) and resumed it. */
+int cgpu_global_triangulation_active(const cgpu_voronoi_batch *b, void *stream);
 
 /* Derived topology of rings set by the caller: delOther of every slot (reference
  * allDelSets / getDelSets, voronoiModelBase.cpp:1236-1286; a slot whose two
@@ -446,6 +463,41 @@ size_t cgpu_vm_permute_workspace(int num_vertices, int num_cells);
 int cgpu_vm_permute(const cgpu_vertex_batch *b, const int *vperm, const int *cperm, void *work, size_t work_bytes,
                     void *stream);
 
+/* Cell division and death on the device, replacing the array growth and removal
+ * of voronoiModelBase::cellDivision / cellDeath (reference
+ * voronoiModelBase.cpp:1291-1422; Simple2DActiveCell::cellDivision / cellDeath
+ * through growGPUArray / removeGPUArrayElement, functions.h:81-154).
+ *
+ * cgpu_vor_division_axis: for each dividing cell parent[e] (device array) and
+ * axis direction ray[2e], ray[2e+1] = (cos theta, sin theta), writes out[6e..6e+5]
+ * = the cell's position, then the first and second crossing of the axis with the
+ * cell's Voronoi polygon relative to it (the first twice when only one exists),
+ * NaN when the axis misses; scanned in the reference's vertex order. The caller
+ * forms the new positions from these (cell +- fraction * max |crossing| * ray,
+ * wrapped into the box).
+ *
+ * cgpu_vor_events_gather: one record per system (device array ev[num_systems],
+ * new_off ascending) lays the per-cell arrays out for the new sizes into
+ * caller-owned arrays of new_num_cells entries: a death drops its cell (the
+ * cells above it move down), a division puts p1 at the parent and appends a
+ * daughter at the end of its system, at p2, with the parent's preferences,
+ * motility, type, exclusion 0, the director 'director' and the velocity v0 (dcos,
+ * dsin). excl / cell_type must be given exactly when the view has them. The
+ * rings are then rebuilt by cgpu_global_triangulation on a view of the new
+ * layout (the reference's resizeAndReset, voronoiModelBase.cpp:1291-1307). */
+typedef struct cgpu_cell_event {
+    int kind;             /* -1 none, 0 death, 1 division                  */
+    int target;           /* the cell, as an index within its old system   */
+    int old_off, old_n;   /* the system in the old layout                  */
+    int new_off, pad_e;   /* its first cell in the new layout              */
+    double p1[2], p2[2];  /* division: new positions of parent, daughter   */
+    double director, dcos, dsin;  /* division: the daughter's director, its cosine and sine */
+} cgpu_cell_event;
+int cgpu_vor_division_axis(const cgpu_voronoi_batch *b, int count, const int *parent, const double *ray, double *out,
+                           void *stream);
+int cgpu_vor_events_gather(const cgpu_voronoi_batch *b, const cgpu_cell_event *ev, int new_num_cells, double *pos,
+                           void *theta, void *vel, void *APpref, void *mot, int *excl, int *cell_type, void *stream);
+
 /* ---------------------------------------------------------------- object layer */
 
 /* A batch of independent Voronoi systems (one system = the reference's
@@ -529,6 +581,9 @@ int cgpu_vor_get(void *h, double *x, double *y, double *theta, double *vx, doubl
 int cgpu_vor_get_topology(void *h, int *nn, int *nbr, int cap, int *neigh_max);
 int cgpu_vor_get_counters(void *h, int s, long long *out);
 int cgpu_vor_num_cells(void *h);
+/* ring slots per cell now allocated (CGPU_RING_CAP, grown up to CGPU_RING_CAP_MAX
+ * when a star needs more: the reference's neighMax storage growth) */
+int cgpu_vor_ring_capacity(void *h);
 double cgpu_vor_energy(void *h, int s);
 int cgpu_vor_status(void *h);
 int cgpu_vor_set_active(void *h, const int *active, const int *steps_left);
@@ -621,6 +676,9 @@ int cgpu_vm_cell_events(void *h, int count, const int *kind, const int *cell, co
  * the device (cgpu_vm_hilbert_order, cgpu_vm_permute) and the geometry
  * recomputed. vperm / cperm (either may be NULL) receive new -> old. */
 int cgpu_vm_spatial_sort(void *h, int *vperm, int *cperm);
+/* cell-list slots per cell now allocated (CGPU_VERTEX_CAP, grown when a T1 flip
+ * or a cell event needs more: the reference's vertexMax, growCellVerticesList) */
+int cgpu_vm_vertex_capacity(void *h);
 /* cells and vertices per system (either may be NULL); returns the total vertices */
 int cgpu_vm_get_sizes(void *h, int *nc, int *nv);
 /* The batch's device arrays as a kernel-layer view, and the stream its work

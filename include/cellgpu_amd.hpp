@@ -460,7 +460,7 @@ class voronoiModelBase : public Simple2DActiveCell {
         for (int i = 0; i < Ncells; ++i) out[i] = Dscalar2{a[i], b[i]};
     }
     void topo(vector<int> *nn, vector<int> *nbr, vector<int> *oth) {
-        const int cap = CGPU_RING_CAP;
+        const int cap = CGPU_RING_CAP_MAX;
         vector<int> n(Ncells), r((size_t)Ncells * cap, -1);
         check(cgpu_vor_get_topology(h_, n.data(), r.data(), cap, nullptr), "topology");
         if (nn) *nn = n;
@@ -768,6 +768,7 @@ class vertexModelBase : public Simple2DActiveCell {
         for (int i = 0; i < Nvertices; ++i) out[i] = Dscalar2{a[i], b[i]};
     }
     void topo(vector<int> *vn, vector<int> *vcn, vector<int> *cvn, vector<int> *cv) {
+        vertexMax = cgpu_vm_vertex_capacity(h_);  // the reference's vertexMax grows with the lists
         vector<int> a(3 * (size_t)Nvertices), b(3 * (size_t)Nvertices), c(Ncells), d((size_t)Ncells * vertexMax, -1);
         check(cgpu_vm_get_topology(h_, a.data(), b.data(), c.data(), d.data(), vertexMax), "topology");
         if (vn) *vn = a;
@@ -826,7 +827,8 @@ class vertexModelBase : public Simple2DActiveCell {
         cellVertexNum.bind(
             [this] { return (size_t)Ncells; }, [this](vector<int> &h) { topo(nullptr, nullptr, &h, nullptr); });
         cellVertices.bind(
-            [this] { return (size_t)Ncells * vertexMax; }, [this](vector<int> &h) { topo(nullptr, nullptr, nullptr, &h); });
+            [this] { return (size_t)Ncells * (vertexMax = cgpu_vm_vertex_capacity(h_)); },
+            [this](vector<int> &h) { topo(nullptr, nullptr, nullptr, &h); });
         // cell positions: mean of each cell's vertices (minimum image about the
         // first vertex), as the reference's getCellPositions
         cellPositions.bind(

@@ -12,8 +12,8 @@
 * a displacement that leaves the representable range: the system reports
   CGPU_ERR_NONFINITE and keeps its last position, the other systems of the batch
   carry on unaffected;
-* a Delaunay degree above the ring capacity (24) at creation: refused with
-  CGPU_ERR_RING_CAPACITY.
+* ring and cell-list capacities: test_gpu_capacity.py (growth, and refusal
+  beyond the largest ring capacity).
 """
 import numpy as np
 import pytest
@@ -146,24 +146,3 @@ def test_runaway_system_reports_nonfinite_and_others_carry_on(precision):
     solo.check_status()
     solo.close()
     pair.close()
-
-
-def test_degree_above_ring_capacity_is_refused(oracle):
-    cg = _cg()
-    N = 200
-    L = float(np.sqrt(N))
-    c = np.array([L / 2, L / 2])
-    k = np.arange(30)
-    r = 1.5 + 0.01 * np.sin(1.3 * k)
-    ring = c + np.stack([r * np.cos(2 * np.pi * k / 30 + 0.01), r * np.sin(2 * np.pi * k / 30 + 0.01)], 1)
-    rng = np.random.default_rng(9)
-    rest = []
-    while len(rest) < N - 31:
-        p = rng.uniform(0, L, 2)
-        if np.hypot(*(p - c)) > 2.6:
-            rest.append(p)
-    pts = np.vstack([c[None], ring, np.array(rest)])
-    onn, _, _ = oracle.triangulate(pts[:, 0], pts[:, 1], L, cap=40)
-    assert onn[0] == 30 and onn.sum() == 6 * N
-    with pytest.raises(cg.CellGPUError, match="ring capacity"):
-        cg.VoronoiBatch([N], x=pts[:, 0], y=pts[:, 1], theta=np.zeros(N), precision="fp64")
