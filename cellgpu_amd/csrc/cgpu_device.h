@@ -36,6 +36,14 @@ CGPU_HD Box make_box(double L) { return Box{L, L * 0.5, 1. / L}; }
 constexpr int WRAP_MAX = 64;
 CGPU_HD dvec2 min_dist(const Box &b, dvec2 p1, dvec2 p2) {
     dvec2 a{p1.x - p2.x, p1.y - p2.y};
+    // Two points of the box [0, L) differ by less than L per axis: each loop
+    // below then runs at most once and only one of the two per axis, so a
+    // select gives the same bits without the loop control flow.
+    if (fabs(a.x) < b.L && fabs(a.y) < b.L) {
+        a.x = a.x < -b.half ? a.x + b.L : (a.x > b.half ? a.x - b.L : a.x);
+        a.y = a.y < -b.half ? a.y + b.L : (a.y > b.half ? a.y - b.L : a.y);
+        return a;
+    }
     for (int t = 0; t < WRAP_MAX && a.x < -b.half; ++t) a.x += b.L;
     for (int t = 0; t < WRAP_MAX && a.y < -b.half; ++t) a.y += b.L;
     for (int t = 0; t < WRAP_MAX && a.x > b.half; ++t) a.x -= b.L;
