@@ -14,6 +14,10 @@
 
 namespace cgpu_host {
 
+// longest cell list a division or death may make (one thread's stack in the
+// event surgery kernel)
+constexpr int EV_LIST_MAX = 256;
+
 // The convention the kernels rely on, rebuilt from the vertex-cell incidences and
 // the vertex-neighbour sets of one system (local indices), using topology only
 // (geometry can be locally inverted right after a T1 flip):
@@ -808,235 +812,121 @@ class VertexFire : public FireBase {
 // preferences and motility and director new_director (NaN: a uniform draw). The
 // edited topology is brought back to the kernels' convention and the batch is
 // laid out again.
+// On the device (cgpu_vm_event_prepare, cgpu_vm_events_gather,
+// cgpu_vm_events_surgery): the host sees one record per system.
 int VertexBatch::cell_events(int count, const int *kind, const int *cell, const int *p1, const int *p2,
                              const double *new_director) {
     if (count <= 0) return CGPU_OK;
     if (!kind || !cell) throw std::invalid_argument("cell events need kinds and cells");
-    std::vector<int> csys(Nc_);
-    csys_.download(csys.data(), Nc_);
-    std::vector<int> ev(S_, -1);
+    std::vector<cgpu_vm_event> rec(S_);
+    for (int s = 0; s < S_; ++s) {
+        cgpu_vm_event &r = rec[s];
+        std::memset(&r, 0, sizeof(r));
+        r.kind = -1;
+        r.old_voff = hp_[s].voff;
+        r.old_nv = hp_[s].nv;
+        r.old_coff = hp_[s].coff;
+        r.old_nc = hp_[s].nc;
+    }
+    std::vector<int> ev_of(S_, -1);
+    bool draw = false;
     for (int e = 0; e < count; ++e) {
         if (cell[e] < 0 || cell[e] >= Nc_) throw std::invalid_argument("cell index out of range");
         if (kind[e] != 0 && kind[e] != 1) throw std::invalid_argument("event kind is 0 (death) or 1 (division)");
-        if (ev[csys[cell[e]]] >= 0) throw std::invalid_argument("at most one event per system per call");
-        ev[csys[cell[e]]] = e;
+        int s = 0;
+        while (s + 1 < S_ && hp_[s + 1].coff <= cell[e]) ++s;
+        if (ev_of[s] >= 0) throw std::invalid_argument("at most one event per system per call");
+        ev_of[s] = e;
         if (kind[e] == 1 && (!p1 || !p2)) throw std::invalid_argument("a division needs two list positions");
+        rec[s].kind = kind[e];
+        rec[s].cell = cell[e] - hp_[s].coff;
+        rec[s].p1 = kind[e] == 1 ? p1[e] : 0;
+        rec[s].p2 = kind[e] == 1 ? p2[e] : 0;
+        if (kind[e] == 1 && !(new_director && std::isfinite(new_director[e]))) draw = true;
     }
     sync();
-    // host lists may outgrow the device capacity, which grows to fit them
-    const int cap = std::max(vertex_cap_, 256);
-    std::vector<double> vx(Nv_), vy(Nv_), pref(2 * (size_t)Nc_), th(Nc_), mot(2 * (size_t)Nc_);
-    get(vx.data(), vy.data(), nullptr, nullptr, nullptr, nullptr);
-    std::vector<int> vn(3 * (size_t)Nv_), vcn(3 * (size_t)Nv_), cvn(Nc_), cv((size_t)Nc_ * cap);
-    get_topology(vn.data(), vcn.data(), cvn.data(), cv.data(), cap);
-    real_download(APpref_.p, pref.data(), pref.size());
-    real_download(theta_.p, th.data(), Nc_);
-    real_download(mot_.p, mot.data(), mot.size());
+    DevBuf<cgpu_vm_event> dev;
+    dev.alloc(S_);
+    dev.upload(rec.data(), S_);
+    {
+        cgpu_vertex_batch b = view();
+        if (int r = cgpu_vm_event_prepare(&b, dev.p, stream_)) return r;
+        sync();
+    }
+    dev.download(rec.data(), S_);
+    static const char *why[] = {"", "cell death (T2) needs a triangular cell", "a system keeps at least 4 cells",
+                                "cell death: not a simple triangle", "division list positions out of range",
+                                "division: edge without a neighbouring cell"};
+    for (const auto &r : rec)
+        if (r.kind >= 0 && r.status != 0) throw std::invalid_argument(why[r.status >= 1 && r.status <= 5 ? r.status : 0]);
     std::vector<cgpu_sys_state> st(S_);
-    st_.download(st.data(), S_);
-    // the new batch, system by system
-    std::vector<double> nx, ny, npref, nth, nmot;
-    std::vector<int> nvn, nvcn, ncvn, ncv;
+    if (draw) st_.download(st.data(), S_);
+    // the new layout and the daughters' directors
+    int longest = 0, voff = 0, coff = 0;
+    std::vector<int> nv(S_), nc(S_);
     for (int s = 0; s < S_; ++s) {
-        cgpu_vm_params &p = hp_[s];
-        const int vo = p.voff, co = p.coff;
-        int nv = p.nv, nc = p.nc;
-        std::vector<double> lx(vx.begin() + vo, vx.begin() + vo + nv), ly(vy.begin() + vo, vy.begin() + vo + nv);
-        std::vector<int> lvn(3 * (size_t)nv), lvcn(3 * (size_t)nv);
-        for (int k = 0; k < 3 * nv; ++k) {
-            lvn[k] = vn[3 * (size_t)vo + k] - vo;
-            lvcn[k] = vcn[3 * (size_t)vo + k] - co;
-        }
-        std::vector<std::vector<int>> cells(nc);
-        for (int c = 0; c < nc; ++c)
-            for (int k = 0; k < cvn[co + c]; ++k) cells[c].push_back(cv[(size_t)(co + c) * cap + k] - vo);
-        std::vector<double> lpref(pref.begin() + 2 * co, pref.begin() + 2 * (co + nc)),
-            lth(th.begin() + co, th.begin() + co + nc), lmot(mot.begin() + 2 * co, mot.begin() + 2 * (co + nc));
-        const cgpu::Box box = cgpu::make_box(p.L);
-        const int e = ev[s];
-        if (e >= 0 && kind[e] == 0) {
-            const int c = cell[e] - co;
-            if (cells[c].size() != 3) throw std::invalid_argument("cell death (T2) needs a triangular cell");
-            if (nc <= 4) throw std::invalid_argument("a system keeps at least 4 cells");
-            std::vector<int> V = cells[c], around;
-            for (int v : V)
-                for (int f = 0; f < 3; ++f) {
-                    const int q = lvcn[3 * v + f];
-                    if (q != c && std::find(around.begin(), around.end(), q) == around.end()) around.push_back(q);
-                }
-            std::rotate(V.begin(), std::min_element(V.begin(), V.end()), V.end());
-            std::vector<int> outer;
-            for (int v : V)
-                for (int f = 0; f < 3; ++f) {
-                    const int u = lvn[3 * v + f];
-                    if (u != V[0] && u != V[1] && u != V[2] && std::find(outer.begin(), outer.end(), u) == outer.end())
-                        outer.push_back(u);
-                }
-            if (around.size() != 3 || outer.size() != 3) throw std::invalid_argument("cell death: not a simple triangle");
-            for (int q : around) {
-                std::vector<int> l;
-                for (int v : cells[q]) {
-                    const int w = (v == V[1] || v == V[2]) ? V[0] : v;
-                    if (std::find(l.begin(), l.end(), w) == l.end()) l.push_back(w);
-                }
-                cells[q] = l;
-            }
-            for (int k = 0; k < 3; ++k) {
-                lvcn[3 * V[0] + k] = around[k];
-                lvn[3 * V[0] + k] = outer[k];
-                for (int f = 0; f < 3; ++f)
-                    if (lvn[3 * outer[k] + f] == V[1] || lvn[3 * outer[k] + f] == V[2]) lvn[3 * outer[k] + f] = V[0];
-            }
-            // drop vertices V[1], V[2] and cell c, renumbering the rest
-            std::vector<int> vmap(nv), cmap(nc);
-            for (int v = 0, w = 0; v < nv; ++v) vmap[v] = (v == V[1] || v == V[2]) ? -1 : w++;
-            for (int q = 0, w = 0; q < nc; ++q) cmap[q] = q == c ? -1 : w++;
-            std::vector<double> x2, y2;
-            std::vector<int> vn2, vcn2;
-            for (int v = 0; v < nv; ++v) {
-                if (vmap[v] < 0) continue;
-                x2.push_back(lx[v]);
-                y2.push_back(ly[v]);
-                for (int f = 0; f < 3; ++f) {
-                    vn2.push_back(vmap[lvn[3 * v + f]]);
-                    vcn2.push_back(cmap[lvcn[3 * v + f]]);
-                }
-            }
-            std::vector<std::vector<int>> cells2;
-            std::vector<double> pref2, th2, mot2;
-            for (int q = 0; q < nc; ++q) {
-                if (q == c) continue;
-                std::vector<int> l;
-                for (int v : cells[q]) l.push_back(vmap[v]);
-                cells2.push_back(l);
-                pref2.insert(pref2.end(), {lpref[2 * q], lpref[2 * q + 1]});
-                th2.push_back(lth[q]);
-                mot2.insert(mot2.end(), {lmot[2 * q], lmot[2 * q + 1]});
-            }
-            lx = x2;
-            ly = y2;
-            lvn = vn2;
-            lvcn = vcn2;
-            cells = cells2;
-            lpref = pref2;
-            lth = th2;
-            lmot = mot2;
-            nv -= 2;
-            nc -= 1;
-            canonical_topology(nv, nc, lx.data(), ly.data(), box, cap, lvn, lvcn, cells);
-        } else if (e >= 0 && kind[e] == 1) {
-            const int c = cell[e] - co;
-            const std::vector<int> l = cells[c];
-            const int n = (int)l.size();
-            const int i1 = std::min(p1[e], p2[e]), i2 = std::max(p1[e], p2[e]);
-            if (i1 < 0 || i2 >= n || i1 == i2) throw std::invalid_argument("division list positions out of range");
-            const int a = l[i1], an = l[(i1 + 1) % n], b = l[i2], bn = l[(i2 + 1) % n];
-            auto mid = [&](int u, int w) {
-                cgpu::dvec2 d = cgpu::min_dist(box, cgpu::dvec2{lx[w], ly[w]}, cgpu::dvec2{lx[u], ly[u]});
-                cgpu::dvec2 q{lx[u] + 0.5 * d.x, ly[u] + 0.5 * d.y};
-                cgpu::put_in_box(box, q);
-                return q;
-            };
-            auto third_cell = [&](int u, int w) {
-                int ans = -1;
-                for (int f = 0; f < 3; ++f)
-                    for (int g = 0; g < 3; ++g)
-                        if (lvcn[3 * u + f] == lvcn[3 * w + g] && lvcn[3 * u + f] != c) ans = lvcn[3 * u + f];
-                if (ans < 0) throw std::invalid_argument("division: edge without a neighbouring cell");
-                return ans;
-            };
-            const int na = third_cell(a, an), nb = third_cell(b, bn);
-            const int A = nv, B = nv + 1, C = nc;  // new vertices and cell
-            const cgpu::dvec2 pa = mid(a, an), pb = mid(b, bn);
-            lx.push_back(pa.x);
-            ly.push_back(pa.y);
-            lx.push_back(pb.x);
-            ly.push_back(pb.y);
-            lvn.insert(lvn.end(), {a, an, B, A, b, bn});
-            lvcn.insert(lvcn.end(), {na, C, c, nb, c, C});
-            auto repl = [&](int v, int from, int to) {
-                for (int f = 0; f < 3; ++f)
-                    if (lvn[3 * v + f] == from) lvn[3 * v + f] = to;
-            };
-            repl(a, an, A);
-            repl(an, a, A);
-            repl(b, bn, B);
-            repl(bn, b, B);
-            // new cell: A, an ... b, B; parent: B, bn ... a, A
-            std::vector<int> child{A}, parent{B};
-            for (int k = i1 + 1; k <= i2; ++k) child.push_back(l[k % n]);
-            child.push_back(B);
-            for (int k = i2 + 1; k <= i1 + n; ++k) parent.push_back(l[k % n]);
-            parent.push_back(A);
-            for (size_t k = 1; k + 1 < child.size(); ++k)
-                for (int f = 0; f < 3; ++f)
-                    if (lvcn[3 * child[k] + f] == c) lvcn[3 * child[k] + f] = C;
-            cells[c] = parent;
-            cells.push_back(child);
-            auto insert_after = [&](int q, int after, int v) {
-                std::vector<int> m;
-                for (int w : cells[q]) {
-                    m.push_back(w);
-                    if (w == after) m.push_back(v);
-                }
-                cells[q] = m;
-            };
-            insert_after(na, an, A);
-            insert_after(nb, bn, B);
-            lpref.insert(lpref.end(), {lpref[2 * c], lpref[2 * c + 1]});
-            lmot.insert(lmot.end(), {lmot[2 * c], lmot[2 * c + 1]});
+        cgpu_vm_event &r = rec[s];
+        const int dv = r.kind == 1 ? 2 : (r.kind == 0 ? -2 : 0), dc = r.kind == 1 ? 1 : (r.kind == 0 ? -1 : 0);
+        nv[s] = r.old_nv + dv;
+        nc[s] = r.old_nc + dc;
+        r.new_voff = voff;
+        r.new_coff = coff;
+        voff += nv[s];
+        coff += nc[s];
+        longest = std::max(longest, r.grow);
+        if (r.kind == 1) {
+            const int e = ev_of[s];
             double d = new_director ? new_director[e] : NAN;
             if (!std::isfinite(d)) {
-                uint32_t ctr[4] = {(uint32_t)nc, (uint32_t)st[s].step, 4u, 0u};
+                const cgpu_vm_params &p = hp_[s];
+                uint32_t ctr[4] = {(uint32_t)r.old_nc, (uint32_t)st[s].step, 4u, 0u};
                 cgpu::philox4x32_10(ctr, (uint32_t)p.seed, (uint32_t)(p.seed >> 32));
                 const uint64_t u = (((uint64_t)ctr[0] << 21) ^ (uint64_t)(ctr[1] >> 11)) & (((uint64_t)1 << 53) - 1);
                 d = 6.283185307179586476925286766559 * ((double)u / 9007199254740992.0);
             }
-            lth.push_back(d);
-            nv += 2;
-            nc += 1;
-            canonical_topology(nv, nc, lx.data(), ly.data(), box, cap, lvn, lvcn, cells);
+            r.director = d;
         }
-        // append the system to the new batch (global indices)
-        const int nvo = (int)nx.size(), nco = (int)nth.size();
-        p.nv = nv;
-        p.nc = nc;
-        for (int v = 0; v < nv; ++v) {
-            nx.push_back(lx[v]);
-            ny.push_back(ly[v]);
-            for (int f = 0; f < 3; ++f) {
-                nvn.push_back(lvn[3 * v + f] + nvo);
-                nvcn.push_back(lvcn[3 * v + f] + nco);
-            }
-        }
-        for (int q = 0; q < nc; ++q) {
-            if ((int)cells[q].size() > cap) throw std::invalid_argument("a cell would exceed the vertex capacity");
-            ncvn.push_back((int)cells[q].size());
-            for (int k = 0; k < cap; ++k) ncv.push_back(k < (int)cells[q].size() ? cells[q][k] + nvo : -1);
-        }
-        npref.insert(npref.end(), lpref.begin(), lpref.end());
-        nth.insert(nth.end(), lth.begin(), lth.end());
-        nmot.insert(nmot.end(), lmot.begin(), lmot.end());
     }
-    for (int n : ncvn) vertex_cap_ = std::max(vertex_cap_, (n + 1) & ~1);
+    if (longest > EV_LIST_MAX) throw std::invalid_argument("a cell would exceed the longest list an event handles");
+    if (longest > vertex_cap_) vertex_cap_ = (longest + 1) & ~1;
+    dev.upload(rec.data(), S_);
+    const size_t NV = (size_t)voff, NC = (size_t)coff, cap = (size_t)vertex_cap_;
+    DevBuf<double> npos;
+    DevBuf<int> nvn, nvcn, ncvn, ncv;
+    DevBuf<unsigned char> npref, nth, nmot;
+    npos.alloc(2 * NV);
+    nvn.alloc(3 * NV);
+    nvcn.alloc(3 * NV);
+    ncvn.alloc(NC);
+    ncv.alloc(cap * NC);
+    npref.alloc(2 * NC * rsz_);
+    nth.alloc(NC * rsz_);
+    nmot.alloc(2 * NC * rsz_);
+    {
+        cgpu_vertex_batch b = view();
+        if (int r = cgpu_vm_events_gather(&b, dev.p, (int)NV, (int)NC, (int)cap, npos.p, nvn.p, nvcn.p, ncvn.p, ncv.p,
+                                          npref.p, nth.p, nmot.p, stream_))
+            return r;
+        sync();
+    }
+    for (int s = 0; s < S_; ++s) {
+        hp_[s].nv = nv[s];
+        hp_[s].nc = nc[s];
+    }
     layout();
-    std::vector<double> xy(2 * (size_t)Nv_);
-    for (int v = 0; v < Nv_; ++v) {
-        xy[2 * v] = nx[v];
-        xy[2 * v + 1] = ny[v];
+    pos0_.swap(npos);
+    vn_.swap(nvn);
+    vcn_.swap(nvcn);
+    cvn_.swap(ncvn);
+    cv_.swap(ncv);
+    APpref_.swap(npref);
+    theta_.swap(nth);
+    mot_.swap(nmot);
+    {
+        cgpu_vertex_batch b = view();
+        if (int r = cgpu_vm_events_surgery(&b, dev.p, stream_)) return r;
+        sync();
     }
-    pos0_.upload(xy.data(), xy.size());
-    vn_.upload(nvn.data(), nvn.size());
-    vcn_.upload(nvcn.data(), nvcn.size());
-    cvn_.upload(ncvn.data(), Nc_);
-    std::vector<int> sm((size_t)vertex_cap_ * Nc_, -1);
-    for (int q = 0; q < Nc_; ++q)
-        for (int k = 0; k < ncvn[q]; ++k) sm[(size_t)k * Nc_ + q] = ncv[(size_t)q * cap + k];
-    cv_.upload(sm.data(), sm.size());
-    real_upload(APpref_.p, npref.data(), npref.size());
-    real_upload(theta_.p, nth.data(), Nc_);
-    real_upload(mot_.p, nmot.data(), nmot.size());
-    st_.upload(st.data(), S_);
     return compute_forces();
 }
 

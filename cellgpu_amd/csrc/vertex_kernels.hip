@@ -726,6 +726,285 @@ template <typename R> __global__ void k_vm_fire_apply_mix(cgpu_vertex_batch b, c
     str<R>(vel, v, u);
 }
 
+// ---------------------------------------------------------------- cell events
+// Death (T2) and division of vertex-model cells (reference vertexModelBase::
+// cellDeath / cellDivision, vertexModelBase.cpp:905-1385), one record per system.
+
+// Longest cell list the surgery handles in one thread's stack.
+constexpr int EV_LIST = 256;
+
+__device__ inline int list_at(const cgpu_vertex_batch &b, int c, int k) { return b.cv[(size_t)k * b.num_cells + c]; }
+
+// the cell other than c that holds both u and w (-1: none)
+__device__ inline int third_cell(const cgpu_vertex_batch &b, int u, int w, int c) {
+    int ans = -1;
+    for (int f = 0; f < 3; ++f)
+        for (int g = 0; g < 3; ++g)
+            if (b.vcn[3 * u + f] == b.vcn[3 * w + g] && b.vcn[3 * u + f] != c) ans = b.vcn[3 * u + f];
+    return ans;
+}
+
+__global__ void k_vm_event_prepare(const cgpu_vertex_batch b, cgpu_vm_event *ev) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    cgpu_vm_event &e = ev[s];
+    if (e.kind < 0) return;
+    const cgpu_vm_params &sp = b.sys[s];
+    const int c = sp.coff + e.cell;
+    const int n = b.cvn[c];
+    e.status = 0;
+    e.grow = 0;
+    if (e.kind == 0) {
+        if (n != 3) {
+            e.status = 1;
+            return;
+        }
+        if (sp.nc <= 4) {
+            e.status = 2;
+            return;
+        }
+        const int V[3] = {list_at(b, c, 0), list_at(b, c, 1), list_at(b, c, 2)};
+        // the cells around the triangle, in order of appearance over its list
+        int around[3], na = 0;
+        for (int k = 0; k < 3; ++k)
+            for (int f = 0; f < 3; ++f) {
+                const int q = b.vcn[3 * V[k] + f];
+                bool seen = q == c;
+                for (int t = 0; t < na; ++t) seen = seen || around[t] == q;
+                if (!seen) {
+                    if (na == 3) {
+                        e.status = 3;
+                        return;
+                    }
+                    around[na++] = q;
+                }
+            }
+        // the list rotated to its smallest vertex, then the vertices beyond it
+        int m = 0;
+        for (int k = 1; k < 3; ++k)
+            if (V[k] < V[m]) m = k;
+        const int W[3] = {V[m], V[(m + 1) % 3], V[(m + 2) % 3]};
+        int outer[3], no = 0;
+        for (int k = 0; k < 3; ++k)
+            for (int f = 0; f < 3; ++f) {
+                const int u = b.vn[3 * W[k] + f];
+                bool seen = u == W[0] || u == W[1] || u == W[2];
+                for (int t = 0; t < no; ++t) seen = seen || outer[t] == u;
+                if (!seen) {
+                    if (no == 3) {
+                        e.status = 3;
+                        return;
+                    }
+                    outer[no++] = u;
+                }
+            }
+        if (na != 3 || no != 3) {
+            e.status = 3;
+            return;
+        }
+        for (int k = 0; k < 3; ++k) {
+            e.v[k] = W[k] - sp.voff;
+            e.around[k] = around[k] - sp.coff;
+            e.outer[k] = outer[k] - sp.voff;
+        }
+    } else {
+        const int i1 = min(e.p1, e.p2), i2 = max(e.p1, e.p2);
+        if (i1 < 0 || i2 >= n || i1 == i2) {
+            e.status = 4;
+            return;
+        }
+        const int a = list_at(b, c, i1), an = list_at(b, c, (i1 + 1) % n);
+        const int bb = list_at(b, c, i2), bn = list_at(b, c, (i2 + 1) % n);
+        const int na = third_cell(b, a, an, c), nbc = third_cell(b, bb, bn, c);
+        if (na < 0 || nbc < 0) {
+            e.status = 5;
+            return;
+        }
+        e.a = a - sp.voff;
+        e.an = an - sp.voff;
+        e.b = bb - sp.voff;
+        e.bn = bn - sp.voff;
+        e.na = na - sp.coff;
+        e.nb = nbc - sp.coff;
+        const int child = (i2 - i1) + 2, parent = n - (i2 - i1) + 2;
+        const int grow_n = (na == nbc) ? b.cvn[na] + 2 : max(b.cvn[na], b.cvn[nbc]) + 1;
+        e.grow = max(max(child, parent), grow_n);
+    }
+}
+
+// new local vertex id of old local vertex v (the dying triangle's vertices merge
+// into v[0], its smallest; the two others leave)
+__device__ inline int ev_vertex(const cgpu_vm_event &e, int v) {
+    if (e.kind != 0) return v;
+    const int r1 = min(e.v[1], e.v[2]), r2 = max(e.v[1], e.v[2]);
+    if (v == e.v[1] || v == e.v[2]) v = e.v[0];
+    return v - (v > r1 ? 1 : 0) - (v > r2 ? 1 : 0);
+}
+__device__ inline int ev_cell(const cgpu_vm_event &e, int q) {
+    if (e.kind != 0) return q;
+    return q == e.cell ? -1 : q - (q > e.cell ? 1 : 0);
+}
+// system of new index j: the last s with off(s) <= j
+template <typename F> __device__ inline int ev_system(const cgpu_vm_event *ev, int S, int j, F &&off) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off(ev[mid]) <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_vm_events_gather_vertices(const cgpu_vertex_batch b, const cgpu_vm_event *ev, int new_nv,
+                                            double *pos, int *vn, int *vcn) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= new_nv) return;
+    const int s = ev_system(ev, b.num_systems, j, [](const cgpu_vm_event &e) { return e.new_voff; });
+    const cgpu_vm_event &e = ev[s];
+    const int local = j - e.new_voff;
+    if (e.kind == 1 && local >= e.old_nv) return;  // new vertices: the surgery
+    int lo = local;
+    if (e.kind == 0) {
+        const int r1 = min(e.v[1], e.v[2]), r2 = max(e.v[1], e.v[2]);
+        if (lo >= r1) ++lo;
+        if (lo >= r2) ++lo;
+    }
+    const int o = e.old_voff + lo;
+    stp(pos, j, ldp(b.pos, o));
+    for (int f = 0; f < 3; ++f) {
+        vn[3 * j + f] = e.new_voff + ev_vertex(e, b.vn[3 * o + f] - e.old_voff);
+        const int q = ev_cell(e, b.vcn[3 * o + f] - e.old_coff);
+        vcn[3 * j + f] = q < 0 ? -1 : e.new_coff + q;
+    }
+}
+
+template <typename R>
+__global__ void k_vm_events_gather_cells(const cgpu_vertex_batch b, const cgpu_vm_event *ev, int new_nc, int cap,
+                                         int *cvn, int *cv, R *pref, R *theta, R *mot) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= new_nc) return;
+    const int s = ev_system(ev, b.num_systems, j, [](const cgpu_vm_event &e) { return e.new_coff; });
+    const cgpu_vm_event &e = ev[s];
+    const int local = j - e.new_coff;
+    const bool daughter = e.kind == 1 && local >= e.old_nc;
+    const int lo = daughter ? e.cell : (e.kind == 0 && local >= e.cell ? local + 1 : local);
+    const int o = e.old_coff + lo;
+    const int n = daughter ? 0 : b.cvn[o];
+    cvn[j] = n;
+    for (int k = 0; k < cap; ++k)
+        cv[(size_t)k * new_nc + j] = k < n ? e.new_voff + ev_vertex(e, list_at(b, o, k) - e.old_voff) : -1;
+    const R *p0 = (const R *)b.APpref, *t0 = (const R *)b.theta, *m0 = (const R *)b.motility;
+    pref[2 * j] = p0[2 * o];
+    pref[2 * j + 1] = p0[2 * o + 1];
+    mot[2 * j] = m0[2 * o];
+    mot[2 * j + 1] = m0[2 * o + 1];
+    theta[j] = daughter ? (R)e.director : t0[o];
+}
+
+__device__ inline void set_list(const cgpu_vertex_batch &b, int c, const int *l, int n) {
+    const int Nc = b.num_cells;
+    const int old = b.cvn[c];
+    for (int k = 0; k < n; ++k) b.cv[(size_t)k * Nc + c] = l[k];
+    for (int k = n; k < old; ++k) b.cv[(size_t)k * Nc + c] = -1;
+    b.cvn[c] = n;
+}
+__device__ inline void replace_neighbour(const cgpu_vertex_batch &b, int v, int from, int to) {
+    for (int f = 0; f < 3; ++f)
+        if (b.vn[3 * v + f] == from) b.vn[3 * v + f] = to;
+}
+// x inserted right after `after` in cell c's list
+__device__ inline void insert_after(const cgpu_vertex_batch &b, int c, int after, int x) {
+    const int Nc = b.num_cells, n = b.cvn[c];
+    int at = n - 1;
+    for (int k = 0; k < n; ++k)
+        if (b.cv[(size_t)k * Nc + c] == after) at = k;
+    for (int k = n; k > at + 1; --k) b.cv[(size_t)k * Nc + c] = b.cv[(size_t)(k - 1) * Nc + c];
+    b.cv[(size_t)(at + 1) * Nc + c] = x;
+    b.cvn[c] = n + 1;
+}
+
+// the local topology change, one thread per system with an event, on the new layout
+__global__ void k_vm_events_surgery(const cgpu_vertex_batch b, const cgpu_vm_event *ev) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.num_systems) return;
+    const cgpu_vm_event &e = ev[s];
+    if (e.kind < 0) return;
+    const cgpu_vm_params &sp = b.sys[s];
+    int l[EV_LIST];
+    if (e.kind == 0) {
+        // the cells around the triangle lose their now repeated merged vertex;
+        // the merged vertex takes them and the vertices beyond
+        const int v0 = sp.voff + ev_vertex(e, e.v[0]);
+        for (int k = 0; k < 3; ++k) {
+            const int q = sp.coff + ev_cell(e, e.around[k]);
+            const int n = b.cvn[q];
+            int w = 0;
+            bool seen = false;
+            for (int t = 0; t < n; ++t) {
+                const int x = list_at(b, q, t);
+                if (x == v0) {
+                    if (seen) continue;
+                    seen = true;
+                }
+                l[w++] = x;
+            }
+            set_list(b, q, l, w);
+            b.vcn[3 * v0 + k] = q;
+            b.vn[3 * v0 + k] = sp.voff + ev_vertex(e, e.outer[k]);
+        }
+        normalize_vertex(b, v0);
+        return;
+    }
+    // division: A, B on the two edges' midpoints, the daughter C takes the part
+    // of the list from an to b
+    const int c = sp.coff + e.cell, C = sp.coff + e.old_nc;
+    const int A = sp.voff + e.old_nv, B = A + 1;
+    const int a = sp.voff + e.a, an = sp.voff + e.an, bb = sp.voff + e.b, bn = sp.voff + e.bn;
+    const int na = sp.coff + e.na, nbc = sp.coff + e.nb;
+    const int n = b.cvn[c];
+    const int i1 = min(e.p1, e.p2), i2 = max(e.p1, e.p2);
+    for (int k = 0; k < n; ++k) l[k] = list_at(b, c, k);
+    const Box box = make_box(sp.L);
+    auto mid = [&](int u, int w) {
+        const dvec2 pu = ldp(b.pos, u);
+        const dvec2 d = min_dist(box, ldp(b.pos, w), pu);
+        dvec2 q{pu.x + 0.5 * d.x, pu.y + 0.5 * d.y};
+        put_in_box(box, q);
+        return q;
+    };
+    stp(b.pos, A, mid(a, an));
+    stp(b.pos, B, mid(bb, bn));
+    replace_neighbour(b, a, an, A);
+    replace_neighbour(b, an, a, A);
+    replace_neighbour(b, bb, bn, B);
+    replace_neighbour(b, bn, bb, B);
+    // the daughter's vertices, an .. b, change cell
+    for (int k = i1 + 1; k <= i2; ++k)
+        for (int f = 0; f < 3; ++f)
+            if (b.vcn[3 * l[k] + f] == c) b.vcn[3 * l[k] + f] = C;
+    int child[EV_LIST], m = 0;
+    child[m++] = A;
+    for (int k = i1 + 1; k <= i2; ++k) child[m++] = l[k];
+    child[m++] = B;
+    b.cvn[C] = 0;
+    set_list(b, C, child, m);
+    int par[EV_LIST], q = 0;
+    par[q++] = B;
+    for (int k = i2 + 1; k <= i1 + n; ++k) par[q++] = l[k % n];
+    par[q++] = A;
+    set_list(b, c, par, q);
+    insert_after(b, na, an, A);
+    insert_after(b, nbc, bn, B);
+    const int vA[3] = {a, an, B}, cA[3] = {na, C, c}, vB[3] = {A, bb, bn}, cB[3] = {nbc, c, C};
+    for (int f = 0; f < 3; ++f) {
+        b.vn[3 * A + f] = vA[f];
+        b.vcn[3 * A + f] = cA[f];
+        b.vn[3 * B + f] = vB[f];
+        b.vcn[3 * B + f] = cB[f];
+    }
+    normalize_vertex(b, A);
+    normalize_vertex(b, B);
+}
+
 inline int nb(int n, int bs) { return (n + bs - 1) / bs; }
 // per-cell and per-vertex kernels: small work-groups, so a single tissue of tens
 // of thousands of cells spreads over every CU
@@ -864,6 +1143,40 @@ int cgpu_vm_fire_apply_mix(const cgpu_vertex_batch *b, cgpu_fire_state *fs, void
         hipLaunchKernelGGL(k_vm_fire_apply_mix<double>, g, t, 0, S(stream), *b, fs, vel);
     hipLaunchKernelGGL(k_vm_fire_clear_pending, dim3(nb(b->num_systems, 256)), dim3(256), 0, S(stream), fs,
                        b->num_systems);
+    return check();
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cgpu_vm_event_prepare(const cgpu_vertex_batch *b, cgpu_vm_event *ev, void *stream) {
+    if (!b || !ev) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_vm_event_prepare, dim3(nb(b->num_systems, 64)), dim3(64), 0, S(stream), *b, ev);
+    return check();
+}
+
+int cgpu_vm_events_gather(const cgpu_vertex_batch *b, const cgpu_vm_event *ev, int new_nv, int new_nc, int cap,
+                          double *pos, int *vn, int *vcn, int *cvn, int *cv, void *APpref, void *theta, void *mot,
+                          void *stream) {
+    if (!b || !ev || new_nv <= 0 || new_nc <= 0 || cap < CGPU_VERTEX_CAP || !pos || !vn || !vcn || !cvn || !cv ||
+        !APpref || !theta || !mot)
+        return CGPU_ERR_ARGS;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_vm_events_gather_vertices, dim3(nb(new_nv, 256)), dim3(256), 0, st, *b, ev, new_nv, pos, vn,
+                       vcn);
+    if (b->precision)
+        hipLaunchKernelGGL(k_vm_events_gather_cells<float>, dim3(nb(new_nc, 256)), dim3(256), 0, st, *b, ev, new_nc,
+                           cap, cvn, cv, (float *)APpref, (float *)theta, (float *)mot);
+    else
+        hipLaunchKernelGGL(k_vm_events_gather_cells<double>, dim3(nb(new_nc, 256)), dim3(256), 0, st, *b, ev, new_nc,
+                           cap, cvn, cv, (double *)APpref, (double *)theta, (double *)mot);
+    return check();
+}
+
+int cgpu_vm_events_surgery(const cgpu_vertex_batch *nb_, const cgpu_vm_event *ev, void *stream) {
+    if (!nb_ || !ev) return CGPU_ERR_ARGS;
+    hipLaunchKernelGGL(k_vm_events_surgery, dim3(nb(nb_->num_systems, 64)), dim3(64), 0, S(stream), *nb_, ev);
     return check();
 }
 

@@ -498,6 +498,47 @@ int cgpu_vor_division_axis(const cgpu_voronoi_batch *b, int count, const int *pa
 int cgpu_vor_events_gather(const cgpu_voronoi_batch *b, const cgpu_cell_event *ev, int new_num_cells, double *pos,
                            void *theta, void *vel, void *APpref, void *mot, int *excl, int *cell_type, void *stream);
 
+/* Vertex-model cell death (T2 of a triangular cell) and division through two
+ * edges on the device, replacing vertexModelBase::cellDeath / cellDivision
+ * (reference vertexModelBase.cpp:905-1385). One record per system (device array
+ * ev[num_systems]); kind -1 leaves the system as it is.
+ *
+ * cgpu_vm_event_prepare, on the current layout: checks each event and fills the
+ * record's local ids (death: the triangle's vertices v[0] < v[1], v[2], the
+ * three cells around it and the three vertices beyond it; division: the edges'
+ * end vertices a, an, b, bn and the cells na, nb across them) and `grow`, the
+ * longest cell list the event makes; status 0 or the reason it is refused
+ * (1 a death of a cell that is not a triangle, 2 a system would keep fewer than
+ * 4 cells, 3 the triangle is not simple, 4 division positions out of range,
+ * 5 an edge without a neighbouring cell).
+ *
+ * cgpu_vm_events_gather: the caller fills new_voff / new_coff (the new layout:
+ * a death drops two vertices and one cell, a division appends two vertices and
+ * a daughter cell at the end of its system) and the daughter's director, then
+ * this gathers every per-vertex and per-cell array into caller-owned arrays of
+ * the new sizes (cell lists with cap slots), ids mapped (the dying triangle's
+ * vertices to v[0]). cgpu_vm_events_surgery then makes the local topology
+ * change on a view of the new layout: the merged vertex and the lists around it,
+ * or the two new vertices (edge midpoints), the split lists and the neighbours'
+ * lists, with the slot convention of every vertex it touches restored. The
+ * geometry is then stale: cgpu_vm_geometry. */
+typedef struct cgpu_vm_event {
+    int kind;                    /* -1 none, 0 death, 1 division                       */
+    int cell;                    /* the cell, index within its old system             */
+    int p1, p2;                  /* division: positions in the cell's vertex list     */
+    int old_voff, old_nv, old_coff, old_nc;
+    int new_voff, new_coff;      /* set by the caller before the gather                */
+    int v[3], around[3], outer[3];            /* death (prepare), old local ids         */
+    int a, an, b, bn, na, nb;    /* division (prepare), old local ids                  */
+    int status, grow;            /* prepare                                           */
+    double director;             /* division: the daughter's director                  */
+} cgpu_vm_event;
+int cgpu_vm_event_prepare(const cgpu_vertex_batch *b, cgpu_vm_event *ev, void *stream);
+int cgpu_vm_events_gather(const cgpu_vertex_batch *b, const cgpu_vm_event *ev, int new_nv, int new_nc, int cap,
+                          double *pos, int *vn, int *vcn, int *cvn, int *cv, void *APpref, void *theta, void *mot,
+                          void *stream);
+int cgpu_vm_events_surgery(const cgpu_vertex_batch *nb, const cgpu_vm_event *ev, void *stream);
+
 /* ---------------------------------------------------------------- object layer */
 
 /* A batch of independent Voronoi systems (one system = the reference's
