@@ -554,6 +554,18 @@ def main():
             roofline["traffic"] = round(hit[0]["traffic_bytes"])
             roofline["traffic_source"] = "profiles/voronoi_spv_ensemble512_hbm_traffic.json"
             roofline["algorithmic_bytes_per_launch"] = round(d["bytes_per_unit"] * d["units_per_launch"])
+    # the bound that limits the kernel: its share of the VALU issue (rocprofv3
+    # SQ_INSTS_VALU x 4 cycles per wavefront instruction / SIMD-cycles of the
+    # dispatch), from the committed counter summary
+    ipath = os.path.join(ROOT, "profiles", "voronoi_spv_ensemble512_sq_issue.json")
+    if os.path.exists(ipath):
+        ik = json.load(open(ipath))
+        hit = [v for k, v in ik.items() if k.split("<")[0] == dom.split("<")[0] and
+               ("<" not in dom or k.startswith(dom.split(">")[0]))]
+        if hit:
+            roofline["valu_issue_frac"] = round(hit[0]["issue"], 3)
+            roofline["valu_per_wavefront"] = round(hit[0]["valu_per_wave"])
+            roofline["valu_issue_source"] = "profiles/voronoi_spv_ensemble512_sq_issue.json"
 
     errs = None
     if not args.no_errors:

@@ -3,6 +3,7 @@
     python -m cellgpu_amd.profile_summary trace  <dir> <out.md> "<command>" ["<title>"] [phases=a-b,c-d] [steps=K]
     python -m cellgpu_amd.profile_summary counters <fetch_dir> <write_dir> <copy_fetch_dir> <copy_write_dir> \
         <out.md> <out.json> "<command>"
+    python -m cellgpu_amd.profile_summary issue <sq_dir> <out.md> "<command>" ["<title>"]
 
 `trace` reads <dir>/*_kernel_stats.csv and *_kernel_trace.csv (rocprofv3
 --kernel-trace --stats --output-format csv) and tabulates per-kernel times, over
@@ -15,6 +16,15 @@ of 256 MiB). The probe fixes the factor that turns each counter into bytes:
 expected bytes / counted bytes. The JSON holds, per kernel, the median scaled
 HBM traffic per launch over the last dispatches; bench.py reports it as
 roofline.traffic for the dominant kernel.
+
+`issue` reads one rocprofv3 --pmc pass of SQ counters (SQ_WAVES, SQ_INSTS_VALU,
+SQ_INSTS_SALU, SQ_INSTS_VMEM_RD, SQ_BUSY_CYCLES, ...) and tabulates per kernel
+the instructions per wavefront and the VALU issue fraction: a wavefront's VALU
+instruction occupies its SIMD for 4 cycles (64 lanes on 16-lane ALUs), so
+4 SQ_INSTS_VALU / (SIMDs x cycles of the dispatch) is the share of the
+machine's VALU issue the kernel used. SQ_BUSY_CYCLES is summed over the shader
+arrays (agent Array_Count), so SQ_BUSY_CYCLES / arrays is the dispatch's
+length in cycles.
 """
 import csv
 import glob
@@ -116,6 +126,52 @@ def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
     return res
 
 
+def _short(k):
+    if "rocprim" in k:  # library sort kernels: their configuration name
+        for tag in ("radix_sort_block_sort", "merge_sort_block_merge", "merge_mergepath_partition",
+                    "merge_mergepath", "radix_sort"):
+            if tag in k:
+                return "rocprim " + tag
+        return "rocprim"
+    return k.split("(Words")[0]
+
+
+def issue(d, out, cmd, title="SQ instruction counters per kernel", last=10):
+    agent = [r for r in csv.DictReader(open(_one(d, "*agent_info.csv"))) if r.get("Name", "").startswith("gfx")][0]
+    simds, arrays = int(agent["Simd_Count"]), int(agent["Array_Count"])
+    clock_mhz = float(agent["Max_Engine_Clk_Fcompute"])
+    rows = list(csv.DictReader(open(_one(d, "*counter_collection.csv"))))
+    per = {}
+    for r in rows:
+        k = _short(_name(r["Kernel_Name"]))
+        per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(
+            (int(r["Dispatch_Id"]), float(r["Counter_Value"]),
+             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    lines = ["# " + title, "", "Command (repository root, MI355X):", "", "```", cmd, "```", "",
+             "Median over the last %d dispatches of each kernel; %d SIMDs in %d shader arrays, %.0f MHz." % (
+                 last, simds, arrays, clock_mhz), "",
+             "| kernel | waves | VALU / wave | SALU / wave | VMEM reads / wave | busy us | VALU issue |",
+             "|---|---|---|---|---|---|---|"]
+    res = {}
+    for k, cs in sorted(per.items()):
+        if "SQ_INSTS_VALU" not in cs or "SQ_WAVES" not in cs:
+            continue
+
+        def med(c):
+            v = [x[1] for x in sorted(cs[c], key=lambda t: t[0])[-last:]] if c in cs else []
+            return float(np.median(v)) if v else float("nan")
+        waves, valu = med("SQ_WAVES"), med("SQ_INSTS_VALU")
+        busy = med("SQ_BUSY_CYCLES") / arrays if "SQ_BUSY_CYCLES" in cs else float("nan")
+        frac = 4.0 * valu / (simds * busy) if busy == busy and busy > 0 else float("nan")
+        res[k] = dict(waves=waves, valu_per_wave=valu / waves, issue=frac)
+        lines.append("| %s | %.0f | %.0f | %.0f | %.0f | %.1f | %.2f |" % (
+            k, waves, valu / waves, med("SQ_INSTS_SALU") / waves, med("SQ_INSTS_VMEM_RD") / waves,
+            busy / clock_mhz, frac))
+    open(out, "w").write("\n".join(lines) + "\n")
+    json.dump(res, open(os.path.splitext(out)[0] + ".json", "w"), indent=1)
+    return res
+
+
 if __name__ == "__main__":
     a = sys.argv[1:]
     if a and a[0] == "trace":
@@ -128,6 +184,8 @@ if __name__ == "__main__":
             trace(a[1], a[2], a[3])
     elif a and a[0] == "counters":
         counters(a[1], a[2], a[3], a[4], a[5], a[6], a[7])
+    elif a and a[0] == "issue":
+        issue(*a[1:5])
     else:
         print(__doc__)
         sys.exit(2)
