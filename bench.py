@@ -411,7 +411,6 @@ KERNEL_BYTES = {
     "k_test_geometry": ("cells", BYTES_TEST_GEOMETRY),
     # one record (3 x 8) and cell_sys (4) per 64 cells; the stamps of the regions
     "k_ld_reduce": ("cells", 28 / 64),
-    "k_ld_rule": ("systems", 48),         # params 16, sums 24 + skip flag 4 read, rule 4 written
     # cell_sys 4 + cell_bin 4 + stamp 4 + cell_rmax 4 (system state and rule stay in
     # cache); cells near a stamp read their rings and queue their triangles
     "k_select_triangles": ("cells", 16),

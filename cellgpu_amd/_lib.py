@@ -62,7 +62,7 @@ class VertexBatchView(C.Structure):
             "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("vertex_cap", C.c_int)]
 
 
-PROFILE_SLOTS = 17  # CGPU_PROFILE_SLOTS
+PROFILE_SLOTS = 16  # CGPU_PROFILE_SLOTS
 
 
 class LaunchShape(C.Structure):
@@ -84,7 +84,7 @@ class VoronoiBatchView(C.Structure):
         ("cell_type", C.c_void_p), ("tension", C.c_void_p), ("num_types", C.c_int), ("ring_cap", C.c_int),
         ("ld_tri", C.c_void_p), ("bin_mark_wide", C.c_void_p), ("cell_rmax", C.c_void_p),
         ("ld_regions", C.c_void_p), ("ld_region_cap", C.c_int), ("ld_tri_cap", C.c_int),
-        ("ld_part", C.c_void_p), ("ld_rule", C.c_void_p),
+        ("ld_part", C.c_void_p),
         ("shape", LaunchShape)]
 
 

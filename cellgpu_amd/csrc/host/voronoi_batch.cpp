@@ -340,7 +340,6 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
     ld_region_cap_ = ld_region_capacity(N_);
     ld_regions_.alloc((size_t)ld_region_cap_ * CGPU_LD_REGION_BYTES);
     ld_part_.alloc(3 * ((size_t)N_ / 64 + 1));
-    ld_rule_.alloc(S_);
     cell_bin_.alloc(N_);
     cell_rank_.alloc(N_);
     sorted_id_.alloc(N_);
@@ -359,7 +358,6 @@ void VoronoiBatch::layout(const std::vector<int> &n) {
                                    nfq_.n + gq_.n + counters_.n + bin_start_.n + cell_bin_.n + cell_rank_.n +
                                    sorted_id_.n + bin_count_.n + ld_tri_.n + bin_mark_wide_.n) +
                     sizeof(float) * cell_rmax_.n + ld_regions_.n + sizeof(unsigned long long) * ld_part_.n +
-                    sizeof(int) * ld_rule_.n +
                     sizeof(double) * (pos0_.n + pos1_.n + sorted_pos_.n) +
                     (AP_.n + dAP_.n + APpref_.n + mot_.n + vel_.n + theta_.n + force_.n);
     cell_sys_.upload(cell_sys_h_.data(), N_);
@@ -447,7 +445,6 @@ cgpu_voronoi_batch VoronoiBatch::view() const {
     b.ld_regions = ld_regions_.p;
     b.ld_region_cap = ld_region_cap_;
     b.ld_part = ld_part_.p;
-    b.ld_rule = ld_rule_.p;
     b.shape = shape_;
     return b;
 }
@@ -1620,7 +1617,7 @@ int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long 
 }
 const char *cgpu_profile_slot_name(int slot) {
     static const char *names[CGPU_PROFILE_SLOTS] = {
-        "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_ld_reduce", "k_ld_rule",
+        "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_ld_reduce",
         "k_select_triangles", "k_test_queue", "k_mark_nf", "k_star_local_wave<1>",
         "k_star_local_wave<2>", "k_delsets_local", "k_global_fallback", "k_star_global_wave<1>",
         "k_star_global_wave<2>", "k_global_stats", "k_finalize"};

@@ -219,7 +219,6 @@ class VoronoiBatch {
     DevBuf<float> cell_rmax_;
     DevBuf<unsigned char> ld_regions_, sort_work_;
     DevBuf<unsigned long long> ld_part_;
-    DevBuf<int> ld_rule_;
     int ld_region_cap_ = 0, ld_tri_cap_ = 0;
     DevBuf<int> chunk_sys_, chunk_begin_, chunk_end_, bchunk_sys_, bchunk_begin_, bchunk_end_, sys_bchunk_,
         bchunk_sum_;

@@ -71,7 +71,6 @@ void carve(Carver &c, const cgpu_vor_workspace &ws, cgpu_voronoi_batch &b) {
     b.ld_regions = c.take<char>((size_t)ld_region_capacity((int)N) * CGPU_LD_REGION_BYTES);
     b.ld_region_cap = ld_region_capacity((int)N);
     b.ld_part = c.take<unsigned long long>(3 * (N / 64 + 1));
-    b.ld_rule = c.take<int>(S);
 }
 
 template <typename F> int ws_guard(F &&f) {

@@ -70,8 +70,7 @@ enum ProbeSlot {
     PROBE_BIN_SCATTER,
     PROBE_TEST_GEOMETRY,       // geometry and the skip rule's checks
     PROBE_LD_REDUCE,           // the rule's per-system sums and the stamps
-    PROBE_LD_RULE,             // the rule of each system
-    PROBE_SELECT_TRIANGLES,    // the triangles the rule leaves, queued
+    PROBE_SELECT_TRIANGLES,    // the rule of each system, the triangles it leaves, queued
     PROBE_TEST_QUEUE,          // their circumcircle test
     PROBE_MARK_NF,
     PROBE_STAR_LOCAL,
@@ -1908,11 +1907,6 @@ __global__ void k_ld_reduce(cgpu_voronoi_batch b, const double *pos) {
     }
 }
 
-// the rule of every system for this test, once its sums are complete
-__global__ void k_ld_rule(cgpu_voronoi_batch b) {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += gridDim.x * blockDim.x)
-        b.ld_rule[s] = skip_rule_on(b, s, b.sys[s]) ? 1 : 0;
-}
 
 
 // The triangles each cell owns (i the smallest index of the three: the
@@ -1941,7 +1935,8 @@ __global__ __launch_bounds__(512) void k_select_triangles(cgpu_voronoi_batch b, 
     if (live) {
         const int s = b.cell_sys[i];
         const cgpu_sys_params &sp = b.sys[s];
-        const bool near = b.ld_rule[s] == 0 || b.bin_mark_wide[b.cell_bin[i]] == sepoch ||
+        // the rule of the cell's system, from the sums the reduce pass completed
+        const bool near = !skip_rule_on(b, s, sp) || b.bin_mark_wide[b.cell_bin[i]] == sepoch ||
                           !(2.0 * (double)b.cell_rmax[i] <= (MARK_DILATE - 1) * sp.boxsize);
         if (near) {
             const int N = b.num_cells;
@@ -2024,8 +2019,6 @@ void launch_test_geometry(const cgpu_voronoi_batch &b, const double *pos, int te
     probe(pr, PROBE_TEST_GEOMETRY);
     hipLaunchKernelGGL(k_ld_reduce, dim3(256), dim3(256), 0, st, b, pos);
     probe(pr, PROBE_LD_REDUCE);
-    hipLaunchKernelGGL(k_ld_rule, dim3((b.num_systems + 255) / 256), dim3(256), 0, st, b);
-    probe(pr, PROBE_LD_RULE);
     const size_t lds = (size_t)TRI_PER_CELL * bs * sizeof(int3);
     hipLaunchKernelGGL(k_select_triangles, g, t, lds, st, b, pos);
     probe(pr, PROBE_SELECT_TRIANGLES);

@@ -190,7 +190,6 @@ typedef struct cgpu_voronoi_batch {
     void *ld_regions;            /* [ld_region_cap] regions to stamp (system or -1-cell, bounds) */
     int ld_region_cap, ld_tri_cap;
     unsigned long long *ld_part; /* [3 (N/64 + 1)] sums of each 64-cell group inside one system */
-    int *ld_rule;                /* [S] 1: the skip rule holds for the system in this test */
     cgpu_launch_shape shape;     /* launch shapes (zero fields: defaults)           */
 } cgpu_voronoi_batch;
 
@@ -805,7 +804,7 @@ int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out);
  * launch order, named by cgpu_profile_slot_name), queue_sums[0..4] the summed
  * sizes of the flagged, NeedsFixing and global-retriangulation queues, of the
  * circumcircle test's triangle queue and of its list of regions to stamp. */
-#define CGPU_PROFILE_SLOTS 17
+#define CGPU_PROFILE_SLOTS 16
 int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long *queue_sums);
 const char *cgpu_profile_slot_name(int slot);
 /* Host-side Philox4x32-10 and the normal pair of (seed, index, step, stream),
