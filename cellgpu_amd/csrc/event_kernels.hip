@@ -38,7 +38,8 @@ __global__ void k_division_axis(const cgpu_voronoi_batch b, int count, const int
     double *o = out + 6 * e;
     o[0] = pi.x;
     o[1] = pi.y;
-    if (z < 3 || z > CGPU_RING_CAP) {
+    const int cap = b.ring_cap > 0 ? b.ring_cap : CGPU_RING_CAP;
+    if (z < 3 || z > cap) {
         for (int k = 2; k < 6; ++k) o[k] = NAN;
         return;
     }

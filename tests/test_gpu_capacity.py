@@ -66,6 +66,29 @@ def test_ring_storage_grows_at_creation(oracle):
     assert np.abs(g["Fy"] - r["Fy"]).max() <= 1e-12 * scale
 
 
+def test_division_of_a_cell_beyond_the_starting_capacity(oracle):
+    """The division axis of a degree-30 cell (above the starting 24 slots) is
+    found from its 30-slot ring: daughters and rings equal the oracle's."""
+    cg = _cg()
+    N = 200
+    pts, L = ring_config(N, 30)
+    th = np.random.default_rng(2).uniform(0, 2 * np.pi, N)
+    b = cg.VoronoiBatch([N], x=pts[:, 0], y=pts[:, 1], theta=th, precision="fp64")
+    assert b.ring_capacity() == 48
+    o = oracle.VoronoiSystem(pts[:, 0], pts[:, 1], th, L, P0=3.8)
+    print("division of a degree-30 cell", flush=True)
+    b.cell_events([1], [0], [0.7], [0.4], [2.0])
+    o.cell_division(0, 0.7, 0.4, 2.0)
+    b.check_status()
+    g, r = b.get(), o.get()
+    assert b.N == o.N == N + 1
+    for k in ("x", "y", "theta", "vx", "vy"):
+        assert np.array_equal(g[k], r[k]), k
+    nn, nbr, _ = b.topology()
+    onn, onbr, _ = o.topology(cap=b.ring_capacity())
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+
+
 def test_degree_above_the_largest_ring_capacity_is_refused():
     cg = _cg()
     N = 400
