@@ -1027,6 +1027,97 @@ __device__ inline bool wraps_before(double cx, double cy, const WCand &a, const 
     return ic > 0 || (ic == 0 && a.id < b.id);
 }
 
+__device__ inline double readlane_d(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// minimum over the 64 lanes (all active) by DPP row moves, in every lane
+__device__ inline float wave_min_f32(float v) {
+    int x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0xb1, 0xf, 0xf, false)));  // quad_perm 1,0,3,2
+    x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x4e, 0xf, 0xf, false)));  // quad_perm 2,3,0,1
+    x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false)));  // row_half_mirror
+    x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false)));  // row_mirror
+    x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false)));  // row_bcast15
+    x = __float_as_int(v);
+    v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false)));  // row_bcast31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// One gift-wrapping step seen from cur, decided in floating point: the next
+// ring member is the candidate strictly left of 0 -> cur whose circle through
+// 0, cur and itself has the smallest centre parameter t along the bisector of
+// 0 and cur (t = c.(c - cur) / (2 cross(cur, c)); a lies inside the circle of b
+// exactly when t_a < t_b, so this is the wraps_before order). Each t carries a
+// rigorous bound e on its rounding error; the step is decided only when one
+// candidate's interval [t - e, t + e] lies wholly below every other's. Returns
+// false, selecting nothing, otherwise (a tie, a near-tie or an ill-conditioned
+// circle): the exact comparison then decides. best.id = -1 when no candidate
+// lies left of 0 -> cur.
+template <int K>
+__device__ inline bool wrap_select_fast(const WCand (&c)[K], const WCand &cur, WCand &best) {
+    constexpr double U4 = 4.0 * 0x1p-53;  // four units of roundoff
+    double lo[K];
+    float key = INFINITY;
+    bool amb = false, any = false;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        lo[r] = INFINITY;
+        if (c[r].id >= 0 && c[r].id != cur.id && orient0(cur.x, cur.y, c[r].x, c[r].y) > 0) {
+            any = true;
+            const double px = c[r].x * (c[r].x - cur.x), py = c[r].y * (c[r].y - cur.y);
+            const double qx = cur.x * c[r].y, qy = cur.y * c[r].x;
+            const double num = px + py, den = 2.0 * (qx - qy);
+            // |num - num*| <= 3.02 u (|px| + |py|), |den - den*| <= 6.02 u (|qx| + |qy|)
+            const double en = U4 * (fabs(px) + fabs(py)), ed = 2.0 * U4 * (fabs(qx) + fabs(qy));
+            if (!(2.0 * ed <= den)) {
+                amb = true;  // den* may be far below den (or the sign rests on the exact test alone)
+                continue;
+            }
+            const double inv = 1.0 / den, t = num * inv;
+            // |t - t*| <= 2 (|t| ed + en) / den + 2.01 u |t| when den* >= den / 2
+            const double e = 4.0 * (fabs(t) * ed + en) * inv + U4 * fabs(t);
+            const double hi = t + e;
+            if (!(fabs(t) < 1e300 && e < 1e300)) {
+                amb = true;
+                continue;
+            }
+            lo[r] = t - e;
+            // float threshold at or above hi
+            float f = (float)hi;
+            f = f + fabsf(f) * 2.4e-7f + 1e-37f;
+            key = fminf(key, f);
+        }
+    }
+    if (__ballot(amb) != 0ull) return false;
+    const float kmin = wave_min_f32(key);
+    if (kmin == INFINITY) {
+        if (__ballot(any) != 0ull) return false;
+        best = WCand{-1, 0.0, 0.0};
+        return true;
+    }
+    const double th = (double)kmin;
+    int cnt = 0;
+    WCand mine = c[0];
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+        if (lo[r] <= th) {
+            ++cnt;
+            mine = c[r];
+        }
+    const unsigned long long m = __ballot(cnt > 0);
+    if (__popcll(m) != 1 || __ballot(cnt > 1) != 0ull) return false;
+    const int wl = __ffsll((long long)m) - 1;
+    best = WCand{__builtin_amdgcn_readlane(mine.id, wl), readlane_d(mine.x, wl), readlane_d(mine.y, wl)};
+    return true;
+}
+
 // Candidates of the block of half-width w around bin (ib, jb), K per lane
 // (slot lane + 64 r in c[r]; cell i itself and empty slots have id -1).
 // Returns the number of points in the block (i included), or -1 when they do
@@ -1222,15 +1313,18 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
             if (lane == z) myring = cur.id;
             ++z;
             WCand best{-1, 0.0, 0.0};
+            if (!wrap_select_fast<K>(c, cur, best)) {
+                best = WCand{-1, 0.0, 0.0};
 #pragma unroll
-            for (int r = 0; r < K; ++r) {
-                if (c[r].id >= 0 && c[r].id != cur.id && orient0(cur.x, cur.y, c[r].x, c[r].y) > 0 &&
-                    wraps_before(cur.x, cur.y, c[r], best))
-                    best = c[r];
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const WCand oth = shfl_xor_cand(best, o);
-                if (wraps_before(cur.x, cur.y, oth, best)) best = oth;
+                for (int r = 0; r < K; ++r) {
+                    if (c[r].id >= 0 && c[r].id != cur.id && orient0(cur.x, cur.y, c[r].x, c[r].y) > 0 &&
+                        wraps_before(cur.x, cur.y, c[r], best))
+                        best = c[r];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const WCand oth = shfl_xor_cand(best, o);
+                    if (wraps_before(cur.x, cur.y, oth, best)) best = oth;
+                }
             }
             if (best.id < 0) {
                 ok = false;
