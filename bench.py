@@ -498,7 +498,15 @@ def main():
         args.no_autotune = True
     b.sync()
     t0 = time.perf_counter()
-    b.spv_steps(args.init_steps, use_graph=use_graph)
+    if use_graph:
+        b.spv_steps(args.init_steps, use_graph=True)
+    else:
+        # eager launches (profiler runs, where each dispatch is slow): progress
+        # per 100 steps
+        for k in range(0, args.init_steps, 100):
+            b.spv_steps(min(100, args.init_steps - k), use_graph=False)
+            b.sync()
+            print("[bench] initialization step %d" % min(k + 100, args.init_steps), flush=True)
     b.sync()
     t_init = time.perf_counter() - t0
     b.check_status()

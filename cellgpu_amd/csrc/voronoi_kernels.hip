@@ -1027,6 +1027,12 @@ __device__ inline bool wraps_before(double cx, double cy, const WCand &a, const 
     return ic > 0 || (ic == 0 && a.id < b.id);
 }
 
+__device__ inline double uniform_d(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ inline double readlane_d(double v, int l) {
     const long long u = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
@@ -1118,6 +1124,43 @@ __device__ inline bool wrap_select_fast(const WCand (&c)[K], const WCand &cur, W
     return true;
 }
 
+// The candidate nearest to the origin, decided in floating point as above
+// (|c|^2 carries at most 2.01 u |c|^2 of rounding error); false when two
+// candidates are too close to call (nearer then decides exactly).
+template <int K>
+__device__ inline bool nearest_fast(const WCand (&c)[K], WCand &first) {
+    constexpr double U4 = 4.0 * 0x1p-53;
+    double lo[K];
+    float key = INFINITY;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        lo[r] = INFINITY;
+        if (c[r].id >= 0) {
+            const double n2 = c[r].x * c[r].x + c[r].y * c[r].y, e = U4 * n2;
+            lo[r] = n2 - e;
+            float f = (float)(n2 + e);
+            f = f + f * 2.4e-7f + 1e-37f;
+            key = fminf(key, f);
+        }
+    }
+    const float kmin = wave_min_f32(key);
+    if (kmin == INFINITY) return false;
+    const double th = (double)kmin;
+    int cnt = 0;
+    WCand mine = c[0];
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+        if (lo[r] <= th) {
+            ++cnt;
+            mine = c[r];
+        }
+    const unsigned long long m = __ballot(cnt > 0);
+    if (__popcll(m) != 1 || __ballot(cnt > 1) != 0ull) return false;
+    const int wl = __ffsll((long long)m) - 1;
+    first = WCand{__builtin_amdgcn_readlane(mine.id, wl), readlane_d(mine.x, wl), readlane_d(mine.y, wl)};
+    return true;
+}
+
 // Candidates of the block of half-width w around bin (ib, jb), K per lane
 // (slot lane + 64 r in c[r]; cell i itself and empty slots have id -1).
 // Returns the number of points in the block (i included), or -1 when they do
@@ -1144,7 +1187,7 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
         const int t = __shfl_up(incl, o);
         if (lane >= o) incl += t;
     }
-    const int total = __shfl(incl, 63);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
 #pragma unroll
     for (int r = 0; r < K; ++r) c[r] = WCand{-1, 0.0, 0.0};
     if (total > 64 * K) return -1;
@@ -1155,7 +1198,7 @@ __device__ inline int wave_load_block(const cgpu_voronoi_batch &b, const double 
 #pragma unroll
     for (int r = 0; r < K; ++r) t[r] = 0;
     for (int u = 1; u < B; ++u) {
-        const int e = __shfl(excl, u);
+        const int e = __builtin_amdgcn_readlane(excl, u);
 #pragma unroll
         for (int r = 0; r < K; ++r)
             if (e <= lane + 64 * r) t[r] = u;
@@ -1189,7 +1232,7 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
     const int lane = __lane_id();
     const int N = b.num_cells;
     const Box box = make_box(sp.L);
-    const dvec2 pi = ldpos(pos, i);
+    const dvec2 pi{uniform_d(pos[2 * i]), uniform_d(pos[2 * i + 1])};
     const double bs = sp.boxsize;
     const int xs = sp.xsize;
     const int ib = max(0, min(xs - 1, (int)floor(pi.x / bs)));
@@ -1198,7 +1241,7 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
     const double mg = 1e-9 * sp.L;
 
     // current ring: member k in lane k, with the triangle (k-1, k) and its circle
-    const int zold = b.nn[i];
+    const int zold = __builtin_amdgcn_readfirstlane(b.nn[i]);
     int verify = (zold >= 3 && zold <= ring_cap(b)) ? 1 : 0;  // 1 undecided, 0 rebuild, 2 verified
     int rid = -1, pid = -1;
     double rx = 0, ry = 0, px = 0, py = 0, ccx = 0, ccy = 0, cr2 = 0, cr = 0;
@@ -1271,9 +1314,10 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
             {
                 bool viol = false;
                 for (int k = 0; k < zold; ++k) {
-                    const double cx = __shfl(ccx, k), cy = __shfl(ccy, k), r2 = __shfl(cr2, k);
-                    const double ax = __shfl(px, k), ay = __shfl(py, k), bx = __shfl(rx, k), by = __shfl(ry, k);
-                    const int ia = __shfl(pid, k), ibb = __shfl(rid, k);
+                    const double cx = readlane_d(ccx, k), cy = readlane_d(ccy, k), r2 = readlane_d(cr2, k);
+                    const double ax = readlane_d(px, k), ay = readlane_d(py, k);
+                    const double bx = readlane_d(rx, k), by = readlane_d(ry, k);
+                    const int ia = __builtin_amdgcn_readlane(pid, k), ibb = __builtin_amdgcn_readlane(rid, k);
 #pragma unroll
                     for (int r = 0; r < K; ++r) {
                         if (c[r].id >= 0 && c[r].id != ia && c[r].id != ibb) {
@@ -1297,12 +1341,15 @@ __device__ int wave_star(const cgpu_voronoi_batch &b, const double *pos, int i, 
         for (int r = 0; r < K; ++r) nvalid += __popcll(__ballot(c[r].id >= 0));
         if (nvalid < 3) continue;
         WCand first = c[0];
+        if (!nearest_fast<K>(c, first)) {
+            first = c[0];
 #pragma unroll
-        for (int r = 1; r < K; ++r)
-            if (nearer(c[r], first)) first = c[r];
-        for (int o = 32; o > 0; o >>= 1) {
-            const WCand oth = shfl_xor_cand(first, o);
-            if (nearer(oth, first)) first = oth;
+            for (int r = 1; r < K; ++r)
+                if (nearer(c[r], first)) first = c[r];
+            for (int o = 32; o > 0; o >>= 1) {
+                const WCand oth = shfl_xor_cand(first, o);
+                if (nearer(oth, first)) first = oth;
+            }
         }
         WCand cur = first;
         int z = 0, myring = -1;
@@ -2263,9 +2310,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
+    // one wavefront per queue entry: the entry and its cell are wave-uniform
+    for (int q = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6); q < nq;
+         q += nw) {
         // the wide pass resumes at the block width the first pass could not hold
-        const int e = queue[q];
+        const int e = __builtin_amdgcn_readfirstlane(queue[q]);
         const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
         const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
         const int s = b.cell_sys[i];
@@ -2386,8 +2435,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
-    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6; q < nq; q += nw) {
-        const int e = queue[q];
+    // one wavefront per queue entry: the entry and its cell are wave-uniform
+    for (int q = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6); q < nq;
+         q += nw) {
+        const int e = __builtin_amdgcn_readfirstlane(queue[q]);
         const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
         const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
         const int s = b.cell_sys[i];
