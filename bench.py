@@ -418,12 +418,10 @@ KERNEL_BYTES = {
     # and candidate positions are served on chip
     "k_test_queue": ("queued", 72),
     "k_mark_nf": ("flagged", 88),         # queue 4, nn 4, ring 24, 7 marks r/w + NeedsFixing entries
-    "k_star_local_wave<1>": ("needs_fixing", 72),  # pos 16, nn 4, ring 24 read; ring 24 + nn 4 written
-    "k_star_local_wave<2>": ("needs_fixing", 0),
+    "k_star_local_wave": ("needs_fixing", 72),  # pos 16, nn 4, ring 24 read; ring 24 + nn 4 written
     "k_delsets_local": ("needs_fixing", 56),       # queue 4, nn 4, ring 24, delOther 24 written
     "k_global_fallback": ("global", 4),
-    "k_star_global_wave<1>": ("global", 72),
-    "k_star_global_wave<2>": ("global", 0),
+    "k_star_global_wave": ("global", 72),
     "k_global_stats": ("global", 4),
     "k_finalize": ("repaired", 48),       # queue 4, delOther 24, AP + dAP 16, mark 4
 }

@@ -62,7 +62,7 @@ class VertexBatchView(C.Structure):
             "motility", "chunk_sys", "chunk_begin", "chunk_end")] + [("num_chunks", C.c_int), ("vertex_cap", C.c_int)]
 
 
-PROFILE_SLOTS = 16  # CGPU_PROFILE_SLOTS
+PROFILE_SLOTS = 14  # CGPU_PROFILE_SLOTS
 
 
 class LaunchShape(C.Structure):

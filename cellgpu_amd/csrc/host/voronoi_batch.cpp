@@ -1618,9 +1618,9 @@ int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long 
 const char *cgpu_profile_slot_name(int slot) {
     static const char *names[CGPU_PROFILE_SLOTS] = {
         "k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_ld_reduce",
-        "k_select_triangles", "k_test_queue", "k_mark_nf", "k_star_local_wave<1>",
-        "k_star_local_wave<2>", "k_delsets_local", "k_global_fallback", "k_star_global_wave<1>",
-        "k_star_global_wave<2>", "k_global_stats", "k_finalize"};
+        "k_select_triangles", "k_test_queue", "k_mark_nf", "k_star_local_wave",
+        "k_delsets_local", "k_global_fallback", "k_star_global_wave", "k_global_stats",
+        "k_finalize"};
     return (slot >= 0 && slot < CGPU_PROFILE_SLOTS) ? names[slot] : "";
 }
 int cgpu_vor_spatial_sort(void *h, int *perm) {

@@ -74,11 +74,9 @@ enum ProbeSlot {
     PROBE_TEST_QUEUE,          // their circumcircle test
     PROBE_MARK_NF,
     PROBE_STAR_LOCAL,
-    PROBE_STAR_LOCAL_WIDE,
     PROBE_DELSETS_LOCAL,
     PROBE_GLOBAL_FALLBACK,
     PROBE_STAR_GLOBAL,
-    PROBE_STAR_GLOBAL_WIDE,
     PROBE_GLOBAL_STATS,
     PROBE_FINALIZE,
     PROBE_SLOTS
@@ -1445,7 +1443,6 @@ __device__ inline void reset_system(const cgpu_voronoi_batch &b, int s) {
         b.counters[0] = 0;
         b.counters[1] = 0;
         b.counters[2] = 0;
-        b.counters[6] = 0;
         b.counters[7] += 1;  // the circumcircle test's epoch (bin stamps)
         b.counters[8] = 0;   // its region records
         b.counters[9] = 0;   // its triangle queue
@@ -2280,19 +2277,21 @@ __device__ void star_local_serial(const cgpu_voronoi_batch &b, const double *pos
     if (tnew != told) atomicAdd(&st.tri_delta, tnew - told);
 }
 
-// The same, one wavefront per cell. K = 1 runs over the NeedsFixing queue and
-// hands cells whose bin blocks exceed one candidate per lane to the K = 2 pass
-// (queue in b.flagged / counters[6]); the K = 2 pass falls back to the serial
-// form in lane 0.
-// wide-pass queue entry: cell index in the low bits, resume half-width above
-constexpr int WIDE_ID_BITS = 26;
-constexpr int WIDE_ID_MASK = (1 << WIDE_ID_BITS) - 1;
+// Star of queued cell i by one wavefront: one candidate per lane, two where a
+// bin block holds more than 64 points. Returns as wave_star; on -3 the serial
+// form is left, starting at half-width w_at.
+__device__ inline int wave_star_any(const cgpu_voronoi_batch &b, const double *pos, int i,
+                                    const cgpu_sys_params &sp, int &ring, int w0, int &w_at) {
+    int z = wave_star<1>(b, pos, i, sp, ring, w0, w_at);
+    if (z == -3) z = wave_star<2>(b, pos, i, sp, ring, w_at, w_at);
+    return z;
+}
 
-template <int K>
+// The same, one wavefront per cell of the NeedsFixing queue.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
-    if (K == 1) {
-        // the first pass also records each system's repair decision, and queues
-        // the cells of the systems that go global for the global pass
+    {
+        // also records each system's repair decision, and queues the cells of
+        // the systems that go global for the global pass
         const int nthreads = gridDim.x * blockDim.x;
         for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += nthreads) {
             const int mode = repair_mode(b, s);
@@ -2305,31 +2304,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         for (int c = blockIdx.x; c < b.num_chunks; c += gridDim.x)  // uniform per block
             if (repair_mode(b, b.chunk_sys[c]) == 2) enqueue_chunk(b, c);
     }
-    const int nq = (K == 1) ? b.counters[1] : b.counters[6];
-    const int *queue = (K == 1) ? b.nfq : b.flagged;
+    const int nq = b.counters[1];
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
     // one wavefront per queue entry: the entry and its cell are wave-uniform
     for (int q = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6); q < nq;
          q += nw) {
-        // the wide pass resumes at the block width the first pass could not hold
-        const int e = __builtin_amdgcn_readfirstlane(queue[q]);
-        const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
-        const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
+        const int i = __builtin_amdgcn_readfirstlane(b.nfq[q]);
         const int s = b.cell_sys[i];
         cgpu_sys_state &st = b.st[s];
-        if ((K == 1 ? repair_mode(b, s) : st.mode) != 1) continue;
+        if (repair_mode(b, s) != 1) continue;
         int ring = -1, w_at = 0;
-        const int z = wave_star<K>(b, pos, i, b.sys[s], ring, w0, w_at);
+        const int z = wave_star_any(b, pos, i, b.sys[s], ring, 1, w_at);
         if (z == 0) continue;  // verified: unchanged
         if (z == -3) {
-            if (lane == 0) {
-                if (K == 1)
-                    b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
-                else
-                    star_local_serial(b, pos, i, w_at);
-            }
+            if (lane == 0) star_local_serial(b, pos, i, w_at);
             continue;
         }
         if (z < 0 || z > st.neigh_max) {
@@ -2378,13 +2368,11 @@ __global__ void k_delsets_local(cgpu_voronoi_batch b) {
     }
 }
 
-// a failed local repair becomes a global one; block 0 also empties the
-// serial-form queue for the global pass
+// a failed local repair becomes a global one
 __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     // one block per cell chunk; the first chunk of a system switches its mode to
     // 3, which the others still accept
     const int c = blockIdx.x;
-    if (c == 0 && threadIdx.x == 0) b.counters[6] = 0;
     const int s = b.chunk_sys[c];
     cgpu_sys_state &st = b.st[s];
     if (!((st.mode == 1 || st.mode == 3) && st.need_global)) return;
@@ -2427,31 +2415,22 @@ __device__ void star_global_serial(const cgpu_voronoi_batch &b, const double *po
     for (int k = 0; k < z; ++k) b.nbr[k * N + i] = ring[k];
 }
 
-// the same, one wavefront per cell of the global queue (K as for the local pass)
-template <int K>
+// the same, one wavefront per cell of the global queue
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_global_wave(cgpu_voronoi_batch b, const double *pos) {
-    const int nq = (K == 1) ? b.counters[2] : b.counters[6];
-    const int *queue = (K == 1) ? b.gq : b.flagged;
+    const int nq = b.counters[2];
     const int N = b.num_cells;
     const int lane = __lane_id();
     const int nw = (gridDim.x * blockDim.x) >> 6;
     // one wavefront per queue entry: the entry and its cell are wave-uniform
     for (int q = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + threadIdx.x) >> 6); q < nq;
          q += nw) {
-        const int e = __builtin_amdgcn_readfirstlane(queue[q]);
-        const int i = (K == 1) ? e : (e & WIDE_ID_MASK);
-        const int w0 = (K == 1) ? 1 : (e >> WIDE_ID_BITS);
+        const int i = __builtin_amdgcn_readfirstlane(b.gq[q]);
         const int s = b.cell_sys[i];
         int ring = -1, w_at = 0;
-        const int z = wave_star<K>(b, pos, i, b.sys[s], ring, w0, w_at);
-        if (z == 0) continue;
+        const int z = wave_star_any(b, pos, i, b.sys[s], ring, 1, w_at);
+        if (z == 0) continue;  // verified: unchanged
         if (z == -3) {
-            if (lane == 0) {
-                if (K == 1)
-                    b.flagged[atomicAdd(&b.counters[6], 1)] = i | (w_at << WIDE_ID_BITS);
-                else
-                    star_global_serial(b, pos, i, w_at);
-            }
+            if (lane == 0) star_global_serial(b, pos, i, w_at);
             continue;
         }
         if (z < 0) {
@@ -2500,7 +2479,6 @@ __global__ void k_global_stats(cgpu_voronoi_batch b) {
         atomicMax(&st.max_deg, mx);
     }
 }
-
 
 // delOther of retriangulated systems (QL threads per cell, each taking slots
 // k, k + QL, ...); geometry of the cells whose ring a star kernel rewrote
@@ -2839,17 +2817,13 @@ inline int nblocks(int n, int bs) { return (n + bs - 1) / bs; }
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int check() { return hipGetLastError() == hipSuccess ? CGPU_OK : CGPU_ERR_HIP; }
 
-constexpr int WIDE_BLOCKS = 256;  // two candidates per lane, for wide bin blocks
-
-// global pass over the queued systems: wave kernel, serial form for the cells
-// it hands back, per-system degree statistics, delOther and geometry
+// global pass over the queued systems: star kernel, per-system degree
+// statistics, delOther and geometry
 void global_pass(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st, int advance = 0,
                  const Probe *pr = nullptr) {
     const dim3 rb(shape_repair_blocks(b)), sb(shape_star_blocks(b)), t(256);
-    hipLaunchKernelGGL(k_star_global_wave<1>, sb, t, 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_global_wave, sb, t, 0, st, b, pos);
     probe(pr, PROBE_STAR_GLOBAL);
-    hipLaunchKernelGGL(k_star_global_wave<2>, dim3(WIDE_BLOCKS), t, 0, st, b, pos);
-    probe(pr, PROBE_STAR_GLOBAL_WIDE);
     hipLaunchKernelGGL(k_global_stats, dim3(b.num_chunks), t, 0, st, b);
     probe(pr, PROBE_GLOBAL_STATS);
     if (b.precision)
@@ -2865,10 +2839,8 @@ int repair_chain(const cgpu_voronoi_batch &b, const double *pos, hipStream_t st,
     const dim3 rb(shape_repair_blocks(b)), sb(shape_star_blocks(b)), t(256);
     hipLaunchKernelGGL(k_mark_nf, rb, t, 0, st, b);
     probe(pr, PROBE_MARK_NF);
-    hipLaunchKernelGGL(k_star_local_wave<1>, sb, t, 0, st, b, pos);
+    hipLaunchKernelGGL(k_star_local_wave, sb, t, 0, st, b, pos);
     probe(pr, PROBE_STAR_LOCAL);
-    hipLaunchKernelGGL(k_star_local_wave<2>, dim3(WIDE_BLOCKS), t, 0, st, b, pos);
-    probe(pr, PROBE_STAR_LOCAL_WIDE);
     hipLaunchKernelGGL(k_delsets_local, rb, t, 0, st, b);
     probe(pr, PROBE_DELSETS_LOCAL);
     hipLaunchKernelGGL(k_global_fallback, dim3(b.num_chunks), t, 0, st, b);

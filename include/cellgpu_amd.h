@@ -157,7 +157,7 @@ typedef struct cgpu_voronoi_batch {
     int *nfq;                    /* [N] NeedsFixing queue           */
     int *gq;                     /* [N] global-retriangulation queue */
     int *counters;               /* [16] device counters: 0 flagged, 1 nfq, 2 gq, 3 bin overflow, 4 ring overflow,
-                                    5 nonfinite, 6 serial-star queue (in flagged), 7 test epoch,
+                                    5 nonfinite, 6 unused, 7 test epoch,
                                     8 regions to stamp, 9 triangles queued for the circumcircle test */
     const double *noise;         /* [N] per-step N(0,1) buffer or NULL (in-kernel Philox) */
     /* Work chunks: at most CGPU_CHUNK cells (or CGPU_BIN_CHUNK bins) of one
@@ -804,7 +804,7 @@ int cgpu_vor_spv_steps_timed(void *h, int steps, double *ms_out);
  * launch order, named by cgpu_profile_slot_name), queue_sums[0..4] the summed
  * sizes of the flagged, NeedsFixing and global-retriangulation queues, of the
  * circumcircle test's triangle queue and of its list of regions to stamp. */
-#define CGPU_PROFILE_SLOTS 16
+#define CGPU_PROFILE_SLOTS 14
 int cgpu_vor_spv_steps_profiled(void *h, int steps, double *ms_slots, long long *queue_sums);
 const char *cgpu_profile_slot_name(int slot);
 /* Host-side Philox4x32-10 and the normal pair of (seed, index, step, stream),
