@@ -96,6 +96,16 @@ def _counter(d, cname):
     return vals
 
 
+def _short(k):
+    if "rocprim" in k:  # library sort kernels: their configuration name
+        for tag in ("radix_sort_block_sort", "merge_sort_block_merge", "merge_mergepath_partition",
+                    "merge_mergepath", "radix_sort"):
+            if tag in k:
+                return "rocprim " + tag
+        return "rocprim"
+    return k.split("(Words")[0]
+
+
 def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
     f, w = _counter(fd, "FETCH_SIZE"), _counter(wd, "WRITE_SIZE")
     cf, cw = _counter(cfd, "FETCH_SIZE"), _counter(cwd, "WRITE_SIZE")
@@ -120,20 +130,10 @@ def counters(fd, wd, cfd, cwd, out_md, out_json, cmd, probe_mib=256, last=10):
         wv = np.median(w.get(k, [0.0])[-last:])
         mb = (fv * f_scale + wv * w_scale) * 1024 / 1e6
         res["kernels"][k] = {"fetch_kib": float(fv), "write_kib": float(wv), "traffic_bytes": mb * 1e6}
-        lines.append("| %s | %.0f | %.0f | %.1f |" % (k, fv, wv, mb))
+        lines.append("| %s | %.0f | %.0f | %.1f |" % (_short(k), fv, wv, mb))
     open(out_md, "w").write("\n".join(lines) + "\n")
     json.dump(res, open(out_json, "w"), indent=1)
     return res
-
-
-def _short(k):
-    if "rocprim" in k:  # library sort kernels: their configuration name
-        for tag in ("radix_sort_block_sort", "merge_sort_block_merge", "merge_mergepath_partition",
-                    "merge_mergepath", "radix_sort"):
-            if tag in k:
-                return "rocprim " + tag
-        return "rocprim"
-    return k.split("(Words")[0]
 
 
 def issue(d, out, cmd, title="SQ instruction counters per kernel", last=10):
