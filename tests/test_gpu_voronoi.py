@@ -202,6 +202,59 @@ def test_clustered_points_bit_exact(oracle):
     assert _stepwise(oracle, b, 15, 13) > 0
 
 
+@pytest.mark.parametrize("jitter", [0.0, 1e-13, 1e-9])
+def test_lattice_ties_bit_exact(oracle, jitter):
+    """A square lattice (every unit square's corners cocircular: exact ties,
+    broken by index) and the same lattice displaced by 1e-13 and 1e-9 (near-ties
+    inside and just outside the rounding bounds of the floating-point
+    gift-wrapping steps): the initial rings equal the oracle's exact ones. From
+    the lattice displaced by 1e-9, self-propelled steps with zero moduli (positions move
+    by v0 alone, so they are reproducible bit for bit whatever the conditioning
+    of the forces) keep the repaired rings equal to the oracle's. The exact
+    lattice is not stepped: its degree-4 Voronoi vertices give zero-length
+    edges, whose unit vectors are not finite in the reference either, and the
+    lattice displaced by 1e-13 has edges too short for finite ones in places."""
+    cg = _cg()
+    n = 24
+    L = float(n)
+    g = np.arange(n) + 0.5
+    x, y = np.meshgrid(g, g)
+    rng = np.random.default_rng(17)
+    x = x.ravel() + jitter * rng.uniform(-1, 1, n * n)
+    y = y.ravel() + jitter * rng.uniform(-1, 1, n * n)
+    th = rng.uniform(0, 2 * np.pi, n * n)
+    print("lattice, jitter %g" % jitter, flush=True)
+    b = cg.VoronoiBatch([n * n], L=[L], x=x, y=y, theta=th, precision="fp64")
+    nn, nbr, nm = b.topology(CAP)
+    onn, onbr, onm = oracle.triangulate(x, y, L, cap=CAP)
+    assert _rings(nn, nbr) == _rings(onn, onbr)
+    assert int(nn.sum()) == 6 * n * n
+    if jitter < 1e-10:
+        return
+    b.set_params(KA=0.0, KP=0.0, dt=0.01)
+    b.set_motility(0.05, 1.0)
+    b.set_noise_buffer(True)
+    repairs = 0
+    for step in range(8):
+        gs = b.get()
+        nn, nbr, nm = b.topology(CAP)
+        o = oracle.VoronoiSystem(gs["x"], gs["y"], gs["theta"], L, KA=0.0, KP=0.0, dt=0.01, v0=0.05, Dr=1.0)
+        o.set_state(gs["x"], gs["y"], gs["theta"], gs["vx"], gs["vy"])
+        o.set_topology(nn, nbr, int(nm[0]))
+        eta = rng.standard_normal(n * n)
+        b.upload_noise(eta)
+        b.spv_steps(1, use_graph=False)
+        b.check_status()
+        o.spp_step(eta, gpu_path=True)
+        gs, r = b.get(), o.get()
+        assert np.array_equal(gs["x"], r["x"]) and np.array_equal(gs["y"], r["y"]), step
+        nn, nbr, _ = b.topology(CAP)
+        onn, onbr, _ = o.topology(CAP)
+        assert _rings(nn, nbr) == _rings(onn, onbr), "step %d" % step
+        repairs += o.counters()["repair_events"]
+    assert repairs > 0
+
+
 def test_batch_equals_single_systems():
     """A system gives the same result alone and inside a batch (FP64)."""
     cg = _cg()
