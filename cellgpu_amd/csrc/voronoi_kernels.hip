@@ -1836,17 +1836,40 @@ __device__ __forceinline__ bool test_narrow(const cgpu_voronoi_batch &b, const d
         scan_run(lo[8], hi[8]);
         if (bad) flag_triangle(b, s, i, j, l);
     } else {
-        int acc = 0;  // candidates of the runs before run r
+        // lane t's candidates are p = t, t + TEAM, ... of the concatenated runs:
+        // their slots are found from the run lengths first, then up to CAND of
+        // them are loaded together (one round trip instead of one per run)
+        constexpr int CAND = 4;
+        int len[9], tot = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            for (int k = lo[r] + ((t - acc) & (TEAM - 1)); k < hi[r]; k += TEAM) {
-                if (corner(k)) continue;
-                if (cand_inside(box, w, min_dist(box, ldpos(b.sorted_pos, k), v))) {
-                    flag_cell(b, s, b.sorted_id[k]);
+            len[r] = hi[r] - lo[r];
+            tot += len[r];
+        }
+        for (int base = t; base < tot; base += CAND * TEAM) {
+            int kk[CAND];
+            bool ok[CAND];
+#pragma unroll
+            for (int m = 0; m < CAND; ++m) {
+                const int p = base + m * TEAM;
+                int acc = 0, k = 0;
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (p >= acc && p < acc + len[r]) k = lo[r] + (p - acc);
+                    acc += len[r];
+                }
+                kk[m] = k;
+                ok[m] = p < tot && !corner(k);
+            }
+            dvec2 q[CAND];
+#pragma unroll
+            for (int m = 0; m < CAND; ++m) q[m] = ok[m] ? ldpos(b.sorted_pos, kk[m]) : v;
+#pragma unroll
+            for (int m = 0; m < CAND; ++m)
+                if (ok[m] && cand_inside(box, w, min_dist(box, q[m], v))) {
+                    flag_cell(b, s, b.sorted_id[kk[m]]);
                     bad = true;
                 }
-            }
-            acc += hi[r] - lo[r];
         }
         const unsigned long long m = __ballot(bad);
         const int first = (int)__lane_id() & ~(TEAM - 1);
