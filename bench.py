@@ -322,7 +322,16 @@ def other_workload(cg, name, steps, warmup, use_graph=True, init_steps=0):
         bins = int(sum(max(2, int(np.floor(np.sqrt(k) / 1.25)) + (int(np.floor(np.sqrt(k) / 1.25)) % 2)) ** 2
                        for k in n))
         b = make_batch(cg, n, P0, v0, Dr, seeds, "fp32")
-        b.spv_steps(init_steps, use_graph=use_graph)
+        print("[bench] ensemble_large: %d systems, %d cells" % (len(n), N), flush=True)
+        if use_graph:
+            b.spv_steps(init_steps, use_graph=True)
+        else:
+            # eager launches (profiler runs, where each dispatch is slow): progress
+            # per 100 steps
+            for k in range(0, init_steps, 100):
+                b.spv_steps(min(100, init_steps - k), use_graph=False)
+                b.sync()
+                print("[bench] initialization step %d" % min(k + 100, init_steps), flush=True)
         b.spv_steps(warmup, use_graph=use_graph)
         b.check_status()
         b.sync()
