@@ -484,7 +484,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every system of the ensemble runs on a single MI355X "
-              "(multi-GPU sharding of an ensemble is by system; see DESIGN.md)", file=sys.stderr)
+              "(an ensemble would shard across GPUs by whole systems, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import cellgpu_amd as cg
