@@ -16,7 +16,8 @@ plain one-system-at-a-time FP64 GPU path on a sample of the same systems over
 the same step window, FP32-vs-FP64 force / energy / topology / trajectory errors
 at the relaxed state, per-kernel durations and algorithmic bandwidth (HIP
 events on the batch stream) with the roofline of the dominant kernel, and the
-oracle's CPU path on survey config 1 timed on the host.
+oracle's restatement of the reference CPU path on one N = 1000 system, timed
+on the host.
 
 usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--init-steps 500]
 """
@@ -177,7 +178,7 @@ def _cpu_info():
 
 
 def cpu_baseline(runs=5, steps=1000, N=1000):
-    """Survey config 1 on the oracle's restatement of the reference CPU path
+    """The reference's CPU-runnable case on the oracle's restatement of its CPU path
     (local test-and-repair, CPU forces and SPP update): N = 1000, L = sqrt(N),
     reference initial condition mt19937(13377), A0 = 1, P0 = 3.8, KA = KP = 1,
     v0 = 0.05, Dr = 1, dt = 0.01, noise from a pre-generated seeded N(0,1)
@@ -196,7 +197,7 @@ def cpu_baseline(runs=5, steps=1000, N=1000):
     med = float(np.median(times))
     model, avail = _cpu_info()
     return dict(value=N * steps / med, unit="cell-steps/s", cores=1, kind="port",
-                sample="config 1: N=1000, %d steps after 1 warm-up step from the mt19937(13377) start, P0=3.8, "
+                sample="N=1000, %d steps after 1 warm-up step from the mt19937(13377) start, P0=3.8, "
                        "v0=0.05, Dr=1, dt=0.01, oracle restatement of the reference CPU path (local repair), "
                        "g++ -O3, 1 thread, median of %d runs (%s s)" % (
                            steps, runs, ", ".join("%.2f" % x for x in times)),
@@ -249,7 +250,8 @@ def serving(count=48, rate=8.0, steps_range=(1000, 100000), fire_fraction=0.25, 
 
 
 def other_workload(cg, name, steps, warmup, use_graph=True, init_steps=0):
-    """Secondary workloads (survey cfg2 / cfg3), one JSON line each; the
+    """Secondary workloads (a single large system and its FIRE minimization, the
+    vertex model, a larger ensemble), one JSON line each; the
     headline stays the ensemble."""
     if name == "single":
         # one N = 262 144 system (L = 512), P0 = 3.75, v0 = 0.1, Dr = 1, dt = 0.01

@@ -267,7 +267,7 @@ def fire_bowl(k, x0, max_it=100, force_cutoff=1e-12, alpha_start=0.1, dt=0.1, dt
 
 
 class FireMinimizer:
-    """Oracle of EnergyMinimizerFIRE (GPU branch, survey note N2)."""
+    """Oracle of EnergyMinimizerFIRE (GPU branch)."""
 
     def __init__(self, sys, max_it=1000, force_cutoff=1e-7, alpha_start=0.99, dt=0.01, dt_max=0.1,
                  dt_inc=1.05, dt_dec=0.95, alpha_dec=0.9, n_min=5):

@@ -5,13 +5,13 @@
 // updaters/brownianParticleDynamics.cu), with one deliberate change: T1 flips
 // are applied in deterministic rounds (a flagged edge slot is flipped when its
 // slot index is the smallest among the flagged edges sharing one of its four
-// cells) instead of the reference's atomicExch race (survey note N5). For edges
+// cells) instead of the reference's atomicExch race. For edges
 // whose four cells are not shared with another flagged edge both rules flip
 // exactly the same edges.
 //
 // Initial topology: Voronoi tessellation of the periodic Delaunay triangulation;
 // vertex ids follow (owner cell, ring slot) of the triangles owned by their
-// smallest cell (the reference's CGAL face order is unpinned, survey N6).
+// smallest cell (the reference's CGAL face order is internal to CGAL).
 //
 // Parity pins: tests/test_oracle_vertex.py (honeycomb closed forms, -grad E by
 // finite differences, single-T1 known answer, sum of cellVertexNum invariant).

@@ -139,7 +139,7 @@ struct CellList {
 // orient/incircle predicates on minimum-image displacements, each triangle
 // certified empty by checking that its circumcircle lies inside the scanned bins.
 // Ring rotation is canonical: the smallest neighbour index first (the reference's
-// rotation is CGAL-internal and carries no meaning, survey note N6).
+// rotation is CGAL-internal and carries no meaning).
 // Returns the degree, or -1 on failure.
 int delaunayStar(int i, const vector<d2> &P, const Box &box, const CellList &cl, vector<int> &ring) {
     const int xs = cl.xsize;

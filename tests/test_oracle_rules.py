@@ -8,7 +8,7 @@ every iteration checked against the stated rules, covering positive power
 before and after NMin, negative power with the velocities zeroed, NSince kept
 (the reference never resets it), and no dt floor on the GPU branch.
 
-Search window (voronoiModelBase.cu:47-69, survey note N3): hand-built
+Search window (voronoiModelBase.cu:47-69): hand-built
 configurations in which an intruder lies inside a triangle's circumcircle but
 outside the bins within wcheck of the triangle's first vertex, so the
 reference GPU test does not flag it while the CPU path's window around the
