@@ -355,7 +355,7 @@ class VoronoiQuadraticEnergy:
         self.batch.set_exclusions(ex)
 
     def setCPU(self, globalOnly=True):
-        raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+        raise CellGPUError("cellgpu_amd has no CPU path: the time step runs on the GPU only")
 
     def computeForces(self):
         self.batch.compute_forces()
@@ -470,7 +470,7 @@ class _Updater:
         pass
 
     def setCPU(self):
-        raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+        raise CellGPUError("cellgpu_amd has no CPU path: the time step runs on the GPU only")
 
     def setReproducible(self, r):
         pass
@@ -555,7 +555,7 @@ class Simulation:
 
     def setCPUOperation(self, setcpu):
         if setcpu:
-            raise CellGPUError("this build has no CPU path: the time step runs on the GPU only")
+            raise CellGPUError("cellgpu_amd has no CPU path: the time step runs on the GPU only")
 
     def setReproducible(self, r):
         for u in self.updaters:

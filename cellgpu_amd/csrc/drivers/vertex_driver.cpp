@@ -35,7 +35,7 @@ int main(int argc, char *argv[]) {
             default: break;
         }
     if (USE_GPU < 0) {
-        fprintf(stderr, "this build runs on the GPU only\n");
+        fprintf(stderr, "cellgpu_amd runs on the GPU only\n");
         return 1;
     }
     auto avm = std::make_shared<VertexQuadraticEnergy>(numpts, 1.0, 4.0, true, true);

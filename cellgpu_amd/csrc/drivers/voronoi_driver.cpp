@@ -27,7 +27,7 @@ int main(int argc, char *argv[]) {
             default: break;
         }
     if (USE_GPU < 0) {
-        fprintf(stderr, "this build runs on the GPU only\n");
+        fprintf(stderr, "cellgpu_amd runs on the GPU only\n");
         return 1;
     }
     EOMPtr spp = std::make_shared<selfPropelledParticleDynamics>(numpts);

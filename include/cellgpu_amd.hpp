@@ -137,7 +137,7 @@ class Simple2DModel {
     virtual void setCPU() = 0;
     virtual int getNumberOfDegreesOfFreedom() = 0;
     virtual void computeForces() = 0;
-    //! dynamical-matrix entries are not part of this build
+    //! dynamical-matrix entries are not provided by this library
     virtual void getDynMatEntries(vector<int2v> &, vector<Dscalar> &, Dscalar = 1.0, Dscalar = 1.0) {
         throw std::runtime_error("cellgpu: dynamical matrix entries are not provided");
     }
