@@ -2313,8 +2313,8 @@ __device__ inline int wave_star_any(const cgpu_voronoi_batch &b, const double *p
 // The same, one wavefront per cell of the NeedsFixing queue.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_star_local_wave(cgpu_voronoi_batch b, const double *pos) {
     {
-        // also records each system's repair decision, and queues the cells of
-        // the systems that go global for the global pass
+        // also records each system's repair decision (k_global_fallback queues
+        // the cells of the systems that go global)
         const int nthreads = gridDim.x * blockDim.x;
         for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < b.num_systems; s += nthreads) {
             const int mode = repair_mode(b, s);
@@ -2324,8 +2324,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             st.repair_events += 1;
             if (mode == 1) st.local_repairs += st.nf_count;
         }
-        for (int c = blockIdx.x; c < b.num_chunks; c += gridDim.x)  // uniform per block
-            if (repair_mode(b, b.chunk_sys[c]) == 2) enqueue_chunk(b, c);
     }
     const int nq = b.counters[1];
     const int N = b.num_cells;
@@ -2394,13 +2392,16 @@ __global__ void k_delsets_local(cgpu_voronoi_batch b) {
 // a failed local repair becomes a global one
 __global__ void k_global_fallback(cgpu_voronoi_batch b) {
     // one block per cell chunk; the first chunk of a system switches its mode to
-    // 3, which the others still accept
+    // 3, which the others still accept. The chunks of systems that went global
+    // at once (mode 2, the N/6 rule) are queued here too.
     const int c = blockIdx.x;
     const int s = b.chunk_sys[c];
     cgpu_sys_state &st = b.st[s];
-    if (!((st.mode == 1 || st.mode == 3) && st.need_global)) return;
+    const int mode = st.mode;
+    const bool fallback = (mode == 1 || mode == 3) && st.need_global;
+    if (!(fallback || mode == 2)) return;  // uniform per block
     __syncthreads();
-    if (threadIdx.x == 0 && b.chunk_begin[c] == b.sys[s].off) {
+    if (fallback && threadIdx.x == 0 && b.chunk_begin[c] == b.sys[s].off) {
         st.mode = 3;
         st.nn_delta = 0;
     }
