@@ -317,3 +317,31 @@ def test_more_chunks_than_star_work_groups(oracle):
     g = [b.counters(s)["global_fixes"] for s in (0, 1, 777, 2048, 4199)]
     assert sum(g) > 0
     b.close()
+
+
+def test_profiled_steps_equal_graph_steps():
+    """The per-kernel event path the benchmark times (one event after every
+    kernel, no graph) runs the same kernels: the state equals graph-captured
+    steps bit for bit, and every stage of the step is named and timed."""
+    cg = _cg()
+    n = [300, 1000, 64]
+    a = cg.VoronoiBatch(n, seeds=[5, 6, 7], precision="fp32")
+    c = cg.VoronoiBatch(n, seeds=[5, 6, 7], precision="fp32")
+    for b in (a, c):
+        b.set_motility(0.1, 1.0)
+        b.spv_steps(10)
+    ms, q = a.profiled_steps(6)
+    c.spv_steps(6)
+    for b in (a, c):
+        b.check_status()
+    ga, gc = a.get(), c.get()
+    for k in ("x", "y", "theta", "vx", "vy", "Fx", "Fy", "A", "P"):
+        assert np.array_equal(ga[k], gc[k]), k
+    assert list(ms) == ["k_force_eom_move", "k_bin_scan", "k_bin_scatter", "k_test_geometry", "k_ld_reduce",
+                        "k_select_triangles", "k_test_queue", "k_mark_nf", "k_star_local_wave",
+                        "k_delsets_local", "k_global_fallback", "k_star_global_wave", "k_global_stats",
+                        "k_finalize"]
+    assert all(v > 0.0 for v in ms.values())
+    assert len(q) == 5 and min(q) >= 0 and q[3] > 0  # triangles queued for the scan
+    a.close()
+    c.close()
